@@ -1,0 +1,832 @@
+// C ABI of libdnhip.so: context, device memory, the per-shot time loop.
+// The loop enqueues every time step of a shot on the context's stream and
+// synchronises once at the end (the reference crosses its process boundary
+// twice per step for the halo exchange; here nothing leaves the device).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <set>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "dn_internal.h"
+
+using namespace dn;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(DN_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+enum { F_VX = 0, F_VY, F_SXX, F_SYY, F_SXY, F_VXP1, F_VYP1, F_UX, F_UY, F_UXY, F_N };
+
+}  // namespace
+
+struct dn_ctx {
+    dn_params p{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Layout lay;
+    HostSetup setup;
+    int fdoh = 0, nd = 0, ntdtinv = 0;
+    bool model_set = false, fwd_complete = false;
+    std::vector<std::pair<void *, size_t>> allocs;
+    size_t bytes = 0;
+    float *fld[F_N] = {};  // plane bases
+    float *rip = nullptr, *rjp = nullptr, *muipjp = nullptr, *f = nullptr, *g = nullptr;
+    float *vp = nullptr, *vs = nullptr, *rho = nullptr;
+    float *rmem = nullptr, *pmem = nullptr, *qmem = nullptr, *dvis = nullptr, *evis = nullptr, *dipvis = nullptr;
+    float *fs_fjm = nullptr, *fs_g = nullptr, *fs_dme = nullptr;
+    float *psi[9] = {};
+    float *pml[P_NVEC] = {};
+    float *fwd[5] = {};
+    float *wc = nullptr, *wcu = nullptr, *wcr = nullptr, *gvp = nullptr, *gvs = nullptr, *grho = nullptr;
+    HostMaterial mat;
+    std::vector<float> h_vp, h_vs, h_rho;  // x-major model kept for receiver factors
+    DevAcq acq;
+    std::vector<int> rec_ix, rec_iy;
+    int qtype = 0, has_force = 0;
+    KArgs a{};
+    // timing
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double run_ms = 0.0, v_ms = 0.0, s_ms = 0.0;
+    int sample_every = 0, samples = 0;
+    std::vector<hipEvent_t> pool;
+
+    int dalloc(void **ptr, size_t nbytes)
+    {
+        if (nbytes == 0) nbytes = 4;
+        hipError_t e = hipMalloc(ptr, nbytes);
+        if (e != hipSuccess) {
+            *ptr = nullptr;
+            return fail(DN_ERR_OOM, "hipMalloc of " + std::to_string(nbytes) + " bytes failed: " +
+                                        hipGetErrorString(e));
+        }
+        allocs.emplace_back(*ptr, nbytes);
+        bytes += nbytes;
+        (void)hipMemset(*ptr, 0, nbytes);
+        return DN_OK;
+    }
+    void dfree(void *ptr)
+    {
+        for (auto it = allocs.begin(); it != allocs.end(); ++it)
+            if (it->first == ptr) {
+                (void)hipFree(ptr);
+                bytes -= it->second;
+                allocs.erase(it);
+                return;
+            }
+    }
+    ~dn_ctx()
+    {
+        if (device >= 0) (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &pr : allocs) (void)hipFree(pr.first);
+        for (auto ev : pool) (void)hipEventDestroy(ev);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+int upload(dn_ctx *c, float *dst, const float *src, size_t n)
+{
+    HIP_OK(hipMemcpyAsync(dst, src, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    return DN_OK;
+}
+
+void fill_args(dn_ctx *c)
+{
+    KArgs &a = c->a;
+    const dn_params &p = c->p;
+    const size_t org = c->lay.origin();
+    a.nx = p.nx;
+    a.ny = p.ny;
+    a.pitch = c->lay.pitch;
+    a.fw = p.fw;
+    a.fw2 = 2 * p.fw;
+    a.free_surf = p.free_surf;
+    a.boundary = p.boundary;
+    a.grad_form = p.grad_form;
+    a.L = p.L;
+    a.nt_total = p.nt;
+    a.dt = p.dt;
+    a.dh = p.dh;
+    a.dhi_el = p.dt / p.dh;
+    a.dhi_vis = (float)(1.0 / p.dh);
+    a.dthalbe = (float)(p.dt / 2.0);
+    for (int k = 0; k < 7; ++k) a.hc[k] = c->setup.hc[k];
+    a.hc[7] = 0.0f;
+    a.plane = (long)c->lay.plane();
+    auto at = [&](float *base) { return base ? base + org : nullptr; };
+    a.vx = at(c->fld[F_VX]);
+    a.vy = at(c->fld[F_VY]);
+    a.sxx = at(c->fld[F_SXX]);
+    a.syy = at(c->fld[F_SYY]);
+    a.sxy = at(c->fld[F_SXY]);
+    a.vxp1 = at(c->fld[F_VXP1]);
+    a.vyp1 = at(c->fld[F_VYP1]);
+    a.ux = at(c->fld[F_UX]);
+    a.uy = at(c->fld[F_UY]);
+    a.uxy = at(c->fld[F_UXY]);
+    a.rip = at(c->rip);
+    a.rjp = at(c->rjp);
+    a.muipjp = at(c->muipjp);
+    a.f = at(c->f);
+    a.g = at(c->g);
+    a.vp = at(c->vp);
+    a.vs = at(c->vs);
+    a.rho = at(c->rho);
+    a.rmem = at(c->rmem);
+    a.pmem = at(c->pmem);
+    a.qmem = at(c->qmem);
+    a.dvis = at(c->dvis);
+    a.evis = at(c->evis);
+    a.dipvis = at(c->dipvis);
+    for (int l = 0; l <= MAXL; ++l) {
+        a.bip[l] = c->mat.bip[l];
+        a.cip[l] = c->mat.cip[l];
+        a.bjm[l] = c->mat.bjm[l];
+        a.cjm[l] = c->mat.cjm[l];
+        a.fs_bjm[l] = c->mat.fs_bjm[l];
+    }
+    a.fs_fjm = c->fs_fjm;
+    a.fs_g = c->fs_g;
+    a.fs_dme = c->fs_dme;
+    a.psi_sxx_x = c->psi[0];
+    a.psi_sxy_x = c->psi[1];
+    a.psi_vxx = c->psi[2];
+    a.psi_vyx = c->psi[3];
+    a.psi_vxxs = c->psi[4];
+    a.psi_syy_y = c->psi[5];
+    a.psi_sxy_y = c->psi[6];
+    a.psi_vyy = c->psi[7];
+    a.psi_vxy = c->psi[8];
+    a.K_x = c->pml[P_KX];
+    a.a_x = c->pml[P_AX];
+    a.b_x = c->pml[P_BX];
+    a.K_xh = c->pml[P_KXH];
+    a.a_xh = c->pml[P_AXH];
+    a.b_xh = c->pml[P_BXH];
+    a.K_y = c->pml[P_KY];
+    a.a_y = c->pml[P_AY];
+    a.b_y = c->pml[P_BY];
+    a.K_yh = c->pml[P_KYH];
+    a.a_yh = c->pml[P_AYH];
+    a.b_yh = c->pml[P_BYH];
+}
+
+int validate(const dn_params *p)
+{
+    if (!p) return fail(DN_ERR_ARG, "null params");
+    if (p->nx < 1 || p->ny < 1) return fail(DN_ERR_ARG, "NX and NY must be positive");
+    if (p->nt < 2) return fail(DN_ERR_ARG, "NT must be at least 2");
+    if (p->fdorder < 2 || p->fdorder > 12 || p->fdorder % 2)
+        return fail(DN_ERR_ARG, "Error in selection of FD coefficients: wrong FDORDER!");
+    if (p->L < 0 || p->L > MAXL) return fail(DN_ERR_ARG, "L must be within 0..8");
+    if (p->boundary != 0) return fail(DN_ERR_ARG, "BOUNDARY=1 (periodic x) is not supported");
+    if (p->fw < 0 || p->fw > p->nx || p->fw > p->ny)
+        return fail(DN_ERR_ARG, " The width of the absorbing boundary is larger than one computational domain.");
+    if (p->seismo < 0 || p->seismo > 4) return fail(DN_ERR_ARG, "SEISMO must be within 0..4");
+    if (p->invmat1 != 1 && p->invmat1 != 3) return fail(DN_ERR_ARG, "INVMAT1 must be 1 or 3");
+    if (p->mode > 0 && p->dtinv < 1) return fail(DN_ERR_ARG, "DTINV must be >= 1");
+    if (p->mode > 0 && p->grad_form != 1 && p->grad_form != 2) return fail(DN_ERR_ARG, "GRAD_FORM must be 1 or 2");
+    if (!(p->dh > 0.0f) || !(p->dt > 0.0f)) return fail(DN_ERR_ARG, "DH and DT must be positive");
+    if (p->L > 0) {
+        for (int l = 0; l < p->L; ++l)
+            if (!(p->fl[l] > 0.0f)) return fail(DN_ERR_ARG, "relaxation frequencies FL must be positive");
+        if (p->free_surf && !(p->ts > 0.0f)) return fail(DN_ERR_ARG, "TS must be positive for the viscoelastic free surface");
+    }
+    return DN_OK;
+}
+
+int zero_state(dn_ctx *c)
+{
+    const size_t pb = c->lay.plane() * sizeof(float);
+    for (int k = 0; k < F_N; ++k)
+        if (c->fld[k]) HIP_OK(hipMemsetAsync(c->fld[k], 0, pb, c->stream));
+    if (c->p.L > 0) {
+        HIP_OK(hipMemsetAsync(c->rmem, 0, pb * c->p.L, c->stream));
+        HIP_OK(hipMemsetAsync(c->pmem, 0, pb * c->p.L, c->stream));
+        HIP_OK(hipMemsetAsync(c->qmem, 0, pb * c->p.L, c->stream));
+    }
+    const size_t nxb = (size_t)c->p.ny * 2 * c->p.fw, nyb = (size_t)2 * c->p.fw * c->p.nx;
+    for (int k = 0; k < 9; ++k) HIP_OK(hipMemsetAsync(c->psi[k], 0, (k < 5 ? nxb : nyb) * sizeof(float) + 4, c->stream));
+    HIP_OK(hipMemsetAsync(c->acq.nan_step, 0, sizeof(int), c->stream));
+    return DN_OK;
+}
+
+hipEvent_t pool_event(dn_ctx *c, size_t k)
+{
+    while (c->pool.size() <= k) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->pool.push_back(e);
+    }
+    return c->pool[k];
+}
+
+int finish_run(dn_ctx *c, size_t nsample_events)
+{
+    HIP_OK(hipEventRecord(c->ev1, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->run_ms = ms;
+    c->v_ms = c->s_ms = 0.0;
+    c->samples = 0;
+    for (size_t k = 0; k + 3 < nsample_events; k += 4) {
+        float tv = 0.0f, ts = 0.0f;
+        HIP_OK(hipEventElapsedTime(&tv, c->pool[k], c->pool[k + 1]));
+        HIP_OK(hipEventElapsedTime(&ts, c->pool[k + 2], c->pool[k + 3]));
+        c->v_ms += tv;
+        c->s_ms += ts;
+        c->samples++;
+    }
+    if (c->samples) {
+        c->v_ms /= c->samples;
+        c->s_ms /= c->samples;
+    }
+    int nan_step = 0;
+    HIP_OK(hipMemcpy(&nan_step, c->acq.nan_step, sizeof(int), hipMemcpyDeviceToHost));
+    if (nan_step)
+        return fail(DN_ERR_UNSTABLE, " Simulation is unstable ! (NaN at time step " + std::to_string(nan_step) + ")");
+    return DN_OK;
+}
+
+int receiver_factors(dn_ctx *c)
+{
+    const int ntr = c->acq.ntr;
+    if (ntr == 0 || c->h_vp.empty()) return DN_OK;
+    std::vector<double> sl(ntr), sm(ntr);
+    for (int t = 0; t < ntr; ++t) {
+        const size_t k = (size_t)(c->rec_ix[t] - 1) * c->p.ny + (c->rec_iy[t] - 1);
+        const float pi = c->h_vp[k], u = c->h_vs[k], r = c->h_rho[k];
+        if (c->p.invmat1 == 3) {
+            sl[t] = std::sqrt((double)pi);
+            sm[t] = std::sqrt((double)u);
+        } else {
+            const float muss = u * u * r;
+            const float lamss = pi * pi * r;
+            sl[t] = std::sqrt((double)lamss);
+            sm[t] = std::sqrt((double)muss);
+        }
+    }
+    HIP_OK(hipMemcpy(c->acq.rec_sqlam, sl.data(), sizeof(double) * ntr, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(c->acq.rec_sqmu, sm.data(), sizeof(double) * ntr, hipMemcpyHostToDevice));
+    return DN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dn_abi_version(void) { return DN_ABI_VERSION; }
+
+const char *dn_last_error(void) { return g_err.c_str(); }
+
+int dn_device_count(int *count)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    if (count) *count = n;
+    return e == hipSuccess ? DN_OK : fail(DN_ERR_HIP, hipGetErrorString(e));
+}
+
+int dn_ctx_create(const dn_params *p, int device, dn_ctx **out)
+{
+    if (!out) return fail(DN_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int rc = validate(p);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(DN_ERR_HIP, "no HIP device available: libdnhip needs an AMD GPU");
+    if (device < 0 || device >= ndev) return fail(DN_ERR_ARG, "device ordinal out of range");
+    dn_ctx *c = new (std::nothrow) dn_ctx();
+    if (!c) return fail(DN_ERR_OOM, "host allocation failed");
+    c->p = *p;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return fail(DN_ERR_HIP, "hipSetDevice failed");
+    }
+    std::string err;
+    rc = holberg(p->fdorder, p->maxrelerror, c->setup.hc, err);
+    if (rc) {
+        delete c;
+        return fail(rc, err);
+    }
+    c->fdoh = p->fdorder / 2;
+    c->nd = c->fdoh + 1;
+    c->ntdtinv = p->mode > 0 ? (int)std::ceil((float)p->nt / (float)p->dtinv) : 0;
+    c->lay.init(p->nx, p->ny);
+    pml_profiles(*p, c->setup);
+    auto bail = [&](int code) {
+        std::string m = g_err;
+        delete c;
+        g_err = m;
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return bail(fail(DN_ERR_HIP, "stream creation failed"));
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return bail(fail(DN_ERR_HIP, "event creation failed"));
+    const size_t pb = c->lay.plane() * sizeof(float);
+    const bool gf2 = p->mode > 0 && p->grad_form == 2;
+    for (int k = 0; k < F_N; ++k) {
+        if ((k == F_UX || k == F_UY || k == F_UXY) && !gf2) continue;
+        if ((rc = c->dalloc((void **)&c->fld[k], pb))) return bail(rc);
+    }
+    float **mats[] = {&c->rip, &c->rjp, &c->muipjp, &c->f, &c->g};
+    for (auto m : mats)
+        if ((rc = c->dalloc((void **)m, pb))) return bail(rc);
+    if (p->mode > 0) {
+        if ((rc = c->dalloc((void **)&c->vp, pb)) || (rc = c->dalloc((void **)&c->vs, pb)) ||
+            (rc = c->dalloc((void **)&c->rho, pb)))
+            return bail(rc);
+    }
+    if (p->L > 0) {
+        float **vis[] = {&c->rmem, &c->pmem, &c->qmem, &c->dvis, &c->evis, &c->dipvis};
+        for (auto m : vis)
+            if ((rc = c->dalloc((void **)m, pb * p->L))) return bail(rc);
+        if ((rc = c->dalloc((void **)&c->fs_fjm, sizeof(float) * (p->nx + 1))) ||
+            (rc = c->dalloc((void **)&c->fs_g, sizeof(float) * (p->nx + 1))) ||
+            (rc = c->dalloc((void **)&c->fs_dme, sizeof(float) * (p->nx + 1))))
+            return bail(rc);
+    }
+    const size_t nxb = (size_t)p->ny * 2 * p->fw, nyb = (size_t)2 * p->fw * p->nx;
+    for (int k = 0; k < 9; ++k)
+        if ((rc = c->dalloc((void **)&c->psi[k], (k < 5 ? nxb : nyb) * sizeof(float) + 4))) return bail(rc);
+    for (int k = 0; k < P_NVEC; ++k) {
+        if ((rc = c->dalloc((void **)&c->pml[k], sizeof(float) * (2 * p->fw + 1)))) return bail(rc);
+        if (hipMemcpy(c->pml[k], c->setup.pml[k].data(), sizeof(float) * (2 * p->fw + 1), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(DN_ERR_HIP, "upload of CPML profiles failed"));
+    }
+    if (p->mode > 0) {
+        const size_t snap = (size_t)p->nx * p->ny * c->ntdtinv * sizeof(float);
+        for (int k = 0; k < 5; ++k)
+            if ((rc = c->dalloc((void **)&c->fwd[k], snap))) return bail(rc);
+        const size_t g = (size_t)p->nx * p->ny * sizeof(float);
+        float **grads[] = {&c->wc, &c->wcu, &c->wcr, &c->gvp, &c->gvs, &c->grho};
+        for (auto m : grads)
+            if ((rc = c->dalloc((void **)m, g))) return bail(rc);
+    }
+    if ((rc = c->dalloc((void **)&c->acq.nan_step, sizeof(int))) || (rc = c->dalloc((void **)&c->acq.l2, sizeof(double))))
+        return bail(rc);
+    c->acq.nt = p->nt;
+    fill_args(c);
+    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(DN_ERR_HIP, "device synchronisation failed"));
+    *out = c;
+    return DN_OK;
+}
+
+int dn_ctx_destroy(dn_ctx *ctx)
+{
+    delete ctx;
+    return DN_OK;
+}
+
+int dn_set_model(dn_ctx *c, const float *vp, const float *vs, const float *rho, const float *taus,
+                 const float *taup)
+{
+    if (!c || !vp || !vs || !rho) return fail(DN_ERR_ARG, "null argument");
+    if (c->p.L > 0 && (!taus || !taup)) return fail(DN_ERR_ARG, "viscoelastic model needs taus and taup");
+    HIP_OK(hipSetDevice(c->device));
+    std::string err;
+    int rc = check_stability(c->p, c->setup.hc, vp, vs, rho, taus, taup, err);
+    if (rc) return fail(rc, err);
+    prepare_material(c->p, c->lay, vp, vs, rho, taus, taup, c->mat);
+    const size_t pl = c->lay.plane();
+    if ((rc = upload(c, c->rip, c->mat.rip.data(), pl)) || (rc = upload(c, c->rjp, c->mat.rjp.data(), pl)) ||
+        (rc = upload(c, c->muipjp, c->mat.muipjp.data(), pl)) || (rc = upload(c, c->f, c->mat.f.data(), pl)) ||
+        (rc = upload(c, c->g, c->mat.g.data(), pl)))
+        return rc;
+    if (c->vp) {
+        if ((rc = upload(c, c->vp, c->mat.vp.data(), pl)) || (rc = upload(c, c->vs, c->mat.vs.data(), pl)) ||
+            (rc = upload(c, c->rho, c->mat.rho.data(), pl)))
+            return rc;
+    }
+    if (c->p.L > 0) {
+        if ((rc = upload(c, c->dvis, c->mat.dvis.data(), pl * c->p.L)) ||
+            (rc = upload(c, c->evis, c->mat.evis.data(), pl * c->p.L)) ||
+            (rc = upload(c, c->dipvis, c->mat.dipvis.data(), pl * c->p.L)) ||
+            (rc = upload(c, c->fs_fjm, c->mat.fs_fjm.data(), c->p.nx + 1)) ||
+            (rc = upload(c, c->fs_g, c->mat.fs_g.data(), c->p.nx + 1)) ||
+            (rc = upload(c, c->fs_dme, c->mat.fs_dme.data(), c->p.nx + 1)))
+            return rc;
+    }
+    HIP_OK(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->p.nx * c->p.ny;
+    c->h_vp.assign(vp, vp + n);
+    c->h_vs.assign(vs, vs + n);
+    c->h_rho.assign(rho, rho + n);
+    // host staging of the device-layout material is no longer needed
+    c->mat.rip.clear(); c->mat.rip.shrink_to_fit();
+    c->mat.rjp.clear(); c->mat.rjp.shrink_to_fit();
+    c->mat.muipjp.clear(); c->mat.muipjp.shrink_to_fit();
+    c->mat.f.clear(); c->mat.f.shrink_to_fit();
+    c->mat.g.clear(); c->mat.g.shrink_to_fit();
+    c->mat.vp.clear(); c->mat.vp.shrink_to_fit();
+    c->mat.vs.clear(); c->mat.vs.shrink_to_fit();
+    c->mat.rho.clear(); c->mat.rho.shrink_to_fit();
+    c->mat.dvis.clear(); c->mat.dvis.shrink_to_fit();
+    c->mat.evis.clear(); c->mat.evis.shrink_to_fit();
+    c->mat.dipvis.clear(); c->mat.dipvis.shrink_to_fit();
+    fill_args(c);
+    c->model_set = true;
+    c->fwd_complete = false;
+    return receiver_factors(c);
+}
+
+int dn_set_sources(dn_ctx *c, int nsrc, const float *srcpos, const float *signals)
+{
+    if (!c || nsrc < 0 || (nsrc > 0 && (!srcpos || !signals))) return fail(DN_ERR_ARG, "bad source arguments");
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    DevAcq &q = c->acq;
+    for (void *ptr : {(void *)q.src_ix, (void *)q.src_iy, (void *)q.src_type, (void *)q.src_sin, (void *)q.src_cos, (void *)q.signals})
+        if (ptr) c->dfree(ptr);
+    q.src_ix = q.src_iy = q.src_type = nullptr;
+    q.src_sin = q.src_cos = nullptr;
+    q.signals = nullptr;
+    q.nsrc = 0;
+    c->qtype = 0;
+    c->has_force = 0;
+    if (nsrc == 0) return DN_OK;
+    std::vector<int> ix(nsrc), iy(nsrc), type(nsrc);
+    std::vector<double> sn(nsrc), cs(nsrc);
+    for (int l = 0; l < nsrc; ++l) {
+        ix[l] = (int)srcpos[0 * nsrc + l];
+        iy[l] = (int)srcpos[1 * nsrc + l];
+        type[l] = (int)srcpos[7 * nsrc + l];
+        // azimuth in radians as the reference's float: src/PSV/update_v_PML_PSV.c:743
+        const float azi = (float)(srcpos[6 * nsrc + l] * 3.141592653589793 / 180);
+        sn[l] = std::sin((double)azi);
+        cs[l] = std::cos((double)azi);
+        if (ix[l] < 1 || ix[l] > c->p.nx || iy[l] < 1 || iy[l] > c->p.ny)
+            return fail(DN_ERR_ARG, "source " + std::to_string(l + 1) + " lies outside the grid");
+        if (type[l] == 5 && (ix[l] < 2 || iy[l] < 1)) return fail(DN_ERR_ARG, "moment-tensor source on the left edge");
+        if (type[l] >= 2 && type[l] <= 4) c->has_force = 1;
+    }
+    c->qtype = type[nsrc - 1];
+    int rc;
+    if ((rc = c->dalloc((void **)&q.src_ix, sizeof(int) * nsrc)) || (rc = c->dalloc((void **)&q.src_iy, sizeof(int) * nsrc)) ||
+        (rc = c->dalloc((void **)&q.src_type, sizeof(int) * nsrc)) ||
+        (rc = c->dalloc((void **)&q.src_sin, sizeof(double) * nsrc)) ||
+        (rc = c->dalloc((void **)&q.src_cos, sizeof(double) * nsrc)) ||
+        (rc = c->dalloc((void **)&q.signals, sizeof(float) * (size_t)nsrc * c->p.nt)))
+        return rc;
+    HIP_OK(hipMemcpy(q.src_ix, ix.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(q.src_iy, iy.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(q.src_type, type.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(q.src_sin, sn.data(), sizeof(double) * nsrc, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(q.src_cos, cs.data(), sizeof(double) * nsrc, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(q.signals, signals, sizeof(float) * (size_t)nsrc * c->p.nt, hipMemcpyHostToDevice));
+    q.nsrc = nsrc;
+    return DN_OK;
+}
+
+int dn_set_receivers(dn_ctx *c, int ntr, const int *recpos)
+{
+    if (!c || ntr < 0 || (ntr > 0 && !recpos)) return fail(DN_ERR_ARG, "bad receiver arguments");
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    DevAcq &q = c->acq;
+    std::set<std::pair<int, int>> seen;
+    for (int t = 0; t < ntr; ++t) {
+        const int ix = recpos[t], iy = recpos[ntr + t];
+        if (ix < 1 || ix > c->p.nx || iy < 1 || iy > c->p.ny)
+            return fail(DN_ERR_ARG, "receiver " + std::to_string(t + 1) + " lies outside the grid");
+        if (!seen.insert({ix, iy}).second)
+            return fail(DN_ERR_ARG, "receiver " + std::to_string(t + 1) + " repeats a grid position");
+    }
+    std::vector<void *> old = {q.rec_ix, q.rec_iy, q.rec_sqlam, q.rec_sqmu, q.obs};
+    for (int k = 0; k < 5; ++k) old.push_back(q.sec[k]);
+    for (int k = 0; k < 3; ++k) old.push_back(q.adj[k]);
+    for (void *ptr : old)
+        if (ptr) c->dfree(ptr);
+    q.rec_ix = q.rec_iy = nullptr;
+    q.rec_sqlam = q.rec_sqmu = nullptr;
+    q.obs = nullptr;
+    for (auto &s : q.sec) s = nullptr;
+    for (auto &s : q.adj) s = nullptr;
+    q.ntr = 0;
+    c->rec_ix.assign(recpos, recpos + ntr);
+    c->rec_iy.assign(recpos + ntr, recpos + 2 * ntr);
+    if (ntr == 0) return DN_OK;
+    const size_t tr = (size_t)ntr * c->p.nt * sizeof(float);
+    int rc;
+    if ((rc = c->dalloc((void **)&q.rec_ix, sizeof(int) * ntr)) || (rc = c->dalloc((void **)&q.rec_iy, sizeof(int) * ntr)) ||
+        (rc = c->dalloc((void **)&q.rec_sqlam, sizeof(double) * ntr)) ||
+        (rc = c->dalloc((void **)&q.rec_sqmu, sizeof(double) * ntr)) || (rc = c->dalloc((void **)&q.obs, tr)))
+        return rc;
+    for (int k = 0; k < 5; ++k)
+        if ((rc = c->dalloc((void **)&q.sec[k], tr))) return rc;
+    for (int k = 0; k < 3; ++k)
+        if ((rc = c->dalloc((void **)&q.adj[k], tr))) return rc;
+    HIP_OK(hipMemcpy(q.rec_ix, c->rec_ix.data(), sizeof(int) * ntr, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(q.rec_iy, c->rec_iy.data(), sizeof(int) * ntr, hipMemcpyHostToDevice));
+    q.ntr = ntr;
+    return receiver_factors(c);
+}
+
+int dn_set_kernel_sampling(dn_ctx *c, int every)
+{
+    if (!c || every < 0) return fail(DN_ERR_ARG, "bad sampling interval");
+    c->sample_every = every;
+    return DN_OK;
+}
+
+int dn_run_forward(dn_ctx *c, int nsteps)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    if (!c->model_set) return fail(DN_ERR_STATE, "dn_set_model must precede dn_run_forward");
+    const int NT = c->p.nt;
+    if (nsteps <= 0 || nsteps > NT) nsteps = NT;
+    HIP_OK(hipSetDevice(c->device));
+    int rc = zero_state(c);
+    if (rc) return rc;
+    c->fwd_complete = false;
+    const KArgs &a = c->a;
+    const int fwdstore = c->p.mode > 0;
+    const int mode0_gf2 = fwdstore && c->p.grad_form == 2;
+    const int vp1_form = c->p.grad_form == 1 || c->p.grad_form == 2;
+    size_t nev = 0;
+    HIP_OK(hipEventRecord(c->ev0, c->stream));
+    int hin = 1, hin1 = 1;
+    for (int nt = 1; nt <= nsteps; ++nt) {
+        const bool store = fwdstore && nt == hin1;
+        const bool sample = c->sample_every > 0 && (nt % c->sample_every) == 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+        if (sample) {
+            e0 = pool_event(c, nev); e1 = pool_event(c, nev + 1); e2 = pool_event(c, nev + 2); e3 = pool_event(c, nev + 3);
+            if (!e0 || !e1 || !e2 || !e3) return fail(DN_ERR_HIP, "event creation failed");
+            HIP_OK(hipEventRecord(e0, c->stream));
+        }
+        launch_update_v(a, c->fdoh, 0, vp1_form && (store || nt == nsteps), c->stream);
+        if (sample) HIP_OK(hipEventRecord(e1, c->stream));
+        if (c->has_force && c->acq.nsrc > 0) launch_force_sources(a, c->acq, nt, c->stream);
+        if (sample) HIP_OK(hipEventRecord(e2, c->stream));
+        launch_update_s(a, c->fdoh, mode0_gf2, c->stream);
+        if (sample) {
+            HIP_OK(hipEventRecord(e3, c->stream));
+            nev += 4;
+        }
+        if (c->acq.nsrc > 0) launch_sources_fwd(a, c->acq, nt, c->qtype, c->has_force, c->stream);
+        if (c->p.free_surf) launch_surface(a, c->fdoh, c->stream);
+        launch_end_step(a, c->acq, nt, c->p.seismo, c->fdoh, c->p.invmat1, nt < NT, c->stream);
+        if (store) {
+            launch_store(a, c->fwd, hin - 1, c->stream);
+            ++hin;
+            hin1 += c->p.dtinv;
+        }
+    }
+    HIP_OK(hipGetLastError());
+    rc = finish_run(c, nev);
+    if (rc) return rc;
+    c->fwd_complete = fwdstore && nsteps == NT;
+    return DN_OK;
+}
+
+int dn_get_seismogram(dn_ctx *c, int comp, float *out)
+{
+    if (!c || !out || comp < 0 || comp > 4) return fail(DN_ERR_ARG, "bad seismogram request");
+    if (c->acq.ntr == 0) return DN_OK;
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipMemcpy(out, c->acq.sec[comp], (size_t)c->acq.ntr * c->p.nt * sizeof(float), hipMemcpyDeviceToHost));
+    return DN_OK;
+}
+
+int dn_calc_residuals(dn_ctx *c, const float *obs_vx, const float *obs_vy, const float *obs_p, int testshot,
+                      double *l2)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    HIP_OK(hipSetDevice(c->device));
+    DevAcq &q = c->acq;
+    const int QB = c->p.quelltypb;
+    const size_t n = (size_t)q.ntr * c->p.nt;
+    HIP_OK(hipMemsetAsync(q.l2, 0, sizeof(double), c->stream));
+    for (int k = 0; k < 3; ++k)
+        if (q.adj[k]) HIP_OK(hipMemsetAsync(q.adj[k], 0, n * sizeof(float), c->stream));
+    const float *obs[3] = {obs_vx, obs_vy, obs_p};
+    const bool use[3] = {QB == 1 || QB == 3 || QB == 5 || QB == 7, QB == 1 || QB == 2 || QB == 6 || QB == 7, QB >= 4};
+    for (int comp = 0; comp < 3; ++comp) {
+        if (!use[comp] || !obs[comp] || q.ntr == 0) continue;
+        HIP_OK(hipMemcpyAsync(q.obs, obs[comp], n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        launch_residual(q, comp, q.obs, c->p.dt, c->stream);
+        if (testshot) launch_l2(q, comp, c->stream);
+        HIP_OK(hipStreamSynchronize(c->stream));
+    }
+    HIP_OK(hipGetLastError());
+    double v = 0.0;
+    HIP_OK(hipMemcpy(&v, q.l2, sizeof(double), hipMemcpyDeviceToHost));
+    if (l2) *l2 = v;
+    return DN_OK;
+}
+
+int dn_set_adjoint_sources(dn_ctx *c, const float *dvx, const float *dvy, const float *dp)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    HIP_OK(hipSetDevice(c->device));
+    const size_t n = (size_t)c->acq.ntr * c->p.nt;
+    const float *src[3] = {dvx, dvy, dp};
+    for (int k = 0; k < 3; ++k) {
+        if (!c->acq.adj[k]) continue;
+        if (src[k])
+            HIP_OK(hipMemcpy(c->acq.adj[k], src[k], n * sizeof(float), hipMemcpyHostToDevice));
+        else
+            HIP_OK(hipMemset(c->acq.adj[k], 0, n * sizeof(float)));
+    }
+    return DN_OK;
+}
+
+int dn_get_adjoint_sources(dn_ctx *c, int comp, float *out)
+{
+    if (!c || !out || comp < 0 || comp > 2) return fail(DN_ERR_ARG, "bad adjoint-source request");
+    if (c->acq.ntr == 0) return DN_OK;
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipMemcpy(out, c->acq.adj[comp], (size_t)c->acq.ntr * c->p.nt * sizeof(float), hipMemcpyDeviceToHost));
+    return DN_OK;
+}
+
+int dn_run_adjoint(dn_ctx *c, int nsteps)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    if (c->p.mode <= 0) return fail(DN_ERR_STATE, "adjoint run needs params.mode > 0");
+    if (!c->fwd_complete) return fail(DN_ERR_STATE, "adjoint run needs a complete forward run with snapshots");
+    const int NT = c->p.nt;
+    if (nsteps <= 0 || nsteps > NT) nsteps = NT;
+    HIP_OK(hipSetDevice(c->device));
+    int rc = zero_state(c);
+    if (rc) return rc;
+    const size_t g = (size_t)c->p.nx * c->p.ny * sizeof(float);
+    HIP_OK(hipMemsetAsync(c->wc, 0, g, c->stream));
+    HIP_OK(hipMemsetAsync(c->wcu, 0, g, c->stream));
+    HIP_OK(hipMemsetAsync(c->wcr, 0, g, c->stream));
+    const KArgs &a = c->a;
+    size_t nev = 0;
+    HIP_OK(hipEventRecord(c->ev0, c->stream));
+    int hin = 1;
+    for (int nt = 1; nt <= nsteps; ++nt) {
+        const bool sample = c->sample_every > 0 && (nt % c->sample_every) == 0;
+        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+        if (sample) {
+            e0 = pool_event(c, nev); e1 = pool_event(c, nev + 1); e2 = pool_event(c, nev + 2); e3 = pool_event(c, nev + 3);
+            if (!e0 || !e1 || !e2 || !e3) return fail(DN_ERR_HIP, "event creation failed");
+            HIP_OK(hipEventRecord(e0, c->stream));
+        }
+        launch_update_v(a, c->fdoh, 1, 0, c->stream);
+        if (sample) HIP_OK(hipEventRecord(e1, c->stream));
+        launch_sources_adj(a, c->acq, nt, c->p.quelltypb, c->stream);
+        if (sample) HIP_OK(hipEventRecord(e2, c->stream));
+        launch_update_s(a, c->fdoh, 0, c->stream);
+        if (sample) {
+            HIP_OK(hipEventRecord(e3, c->stream));
+            nev += 4;
+        }
+        if (c->p.quelltypb >= 4) launch_adj_psource(a, c->acq, nt, c->stream);
+        if (c->p.free_surf) launch_surface(a, c->fdoh, c->stream);
+        launch_end_step(a, c->acq, nt, 0, c->fdoh, c->p.invmat1, nt < NT, c->stream);
+        const int nfwd = NT - nt + 1;  // forward step this adjoint step meets
+        if ((nfwd - 1) % c->p.dtinv == 0) {
+            launch_correlate(a, c->fwd, (long)(c->ntdtinv - hin), c->wc, c->wcu, c->wcr, c->p.invmat1, c->stream);
+            ++hin;
+        }
+    }
+    HIP_OK(hipGetLastError());
+    return finish_run(c, nev);
+}
+
+int dn_assemble_gradient(dn_ctx *c)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    if (c->p.mode <= 0) return fail(DN_ERR_STATE, "gradient assembly needs params.mode > 0");
+    HIP_OK(hipSetDevice(c->device));
+    launch_assemble(c->a, c->wc, c->wcu, c->wcr, c->gvp, c->gvs, c->grho, c->p.invmat1, c->stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    return DN_OK;
+}
+
+int dn_get_gradient(dn_ctx *c, int which, float *out)
+{
+    if (!c || !out || which < 0 || which > 5) return fail(DN_ERR_ARG, "bad gradient request");
+    if (c->p.mode <= 0) return fail(DN_ERR_STATE, "no gradient without params.mode > 0");
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    const int NX = c->p.nx, NY = c->p.ny;
+    const size_t n = (size_t)NX * NY;
+    if (which >= 3) {
+        float *src = which == 3 ? c->gvp : which == 4 ? c->gvs : c->grho;
+        HIP_OK(hipMemcpy(out, src, n * sizeof(float), hipMemcpyDeviceToHost));
+        return DN_OK;
+    }
+    std::vector<float> tmp(n);
+    float *src = which == 0 ? c->wc : which == 1 ? c->wcu : c->wcr;
+    HIP_OK(hipMemcpy(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (int i = 0; i < NX; ++i)
+        for (int j = 0; j < NY; ++j) out[(size_t)i * NY + j] = tmp[(size_t)j * NX + i];
+    return DN_OK;
+}
+
+int dn_get_dims(dn_ctx *c, int *nd, int *ntdtinv)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    if (nd) *nd = c->nd;
+    if (ntdtinv) *ntdtinv = c->ntdtinv;
+    return DN_OK;
+}
+
+static int get_plane_nr(dn_ctx *c, const float *plane, float *out)
+{
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    std::vector<float> h(c->lay.plane());
+    HIP_OK(hipMemcpy(h.data(), plane, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    const int nd = c->nd, NX = c->p.nx, NY = c->p.ny;
+    const int ncol = NX + 2 * nd;
+    size_t k = 0;
+    for (int j = -nd + 1; j <= NY + nd; ++j)
+        for (int i = -nd + 1; i <= NX + nd; ++i, ++k) out[k] = h[c->lay.origin() + c->lay.off(j, i)];
+    (void)ncol;
+    return DN_OK;
+}
+
+int dn_get_field(dn_ctx *c, int id, float *out)
+{
+    if (!c || !out || id < 0 || id >= F_N) return fail(DN_ERR_ARG, "bad field request");
+    if (!c->fld[id]) return fail(DN_ERR_STATE, "field not allocated in this configuration");
+    return get_plane_nr(c, c->fld[id], out);
+}
+
+int dn_get_psi(dn_ctx *c, int id, float *out)
+{
+    if (!c || !out || id < 0 || id > 8) return fail(DN_ERR_ARG, "bad CPML memory request");
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    const size_t n = id < 5 ? (size_t)c->p.ny * 2 * c->p.fw : (size_t)2 * c->p.fw * c->p.nx;
+    HIP_OK(hipMemcpy(out, c->psi[id], n * sizeof(float), hipMemcpyDeviceToHost));
+    return DN_OK;
+}
+
+int dn_get_memvar(dn_ctx *c, int which, int l, float *out)
+{
+    if (!c || !out || which < 0 || which > 2 || l < 1 || l > c->p.L) return fail(DN_ERR_ARG, "bad memory-variable request");
+    const float *base = which == 0 ? c->rmem : which == 1 ? c->pmem : c->qmem;
+    return get_plane_nr(c, base + (size_t)(l - 1) * c->lay.plane(), out);
+}
+
+int dn_get_pml(dn_ctx *c, int which, float *out)
+{
+    if (!c || !out || which < 0 || which >= P_NVEC) return fail(DN_ERR_ARG, "bad CPML profile request");
+    std::memcpy(out, c->setup.pml[which].data() + 1, sizeof(float) * 2 * c->p.fw);
+    return DN_OK;
+}
+
+int dn_get_hc(dn_ctx *c, float *out7)
+{
+    if (!c || !out7) return fail(DN_ERR_ARG, "null argument");
+    std::memcpy(out7, c->setup.hc, sizeof(float) * 7);
+    return DN_OK;
+}
+
+int dn_get_timing(dn_ctx *c, double *run_ms, double *v_kernel_ms, double *s_kernel_ms, int *samples)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    if (run_ms) *run_ms = c->run_ms;
+    if (v_kernel_ms) *v_kernel_ms = c->v_ms;
+    if (s_kernel_ms) *s_kernel_ms = c->s_ms;
+    if (samples) *samples = c->samples;
+    return DN_OK;
+}
+
+int dn_get_footprint(dn_ctx *c, size_t *bytes)
+{
+    if (!c || !bytes) return fail(DN_ERR_ARG, "null argument");
+    *bytes = c->bytes;
+    return DN_OK;
+}
+
+}  // extern "C"

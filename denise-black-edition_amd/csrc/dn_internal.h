@@ -1,0 +1,134 @@
+// Internal declarations shared by the host set-up, the HIP kernels and the
+// C-ABI layer of libdnhip.so.  Not part of the public interface.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "denise_hip.h"
+
+namespace dn {
+
+// Device layout of every 2D field: row-major, x fastest, with HX halo columns
+// on the left/right and HY halo rows on top/bottom.  Element (j,i) in the
+// reference's 1-based indices lives at (j-1+HY)*pitch + (i-1+HX).  HX = 32
+// puts column i=1 on a 128-byte boundary; the pitch is a multiple of 64
+// floats so every row starts 256-byte aligned.  HY = 8 covers the widest
+// stencil reach (FDORDER 12: +-6) and the free-surface ghost rows.
+constexpr int HX = 32;
+constexpr int HY = 8;
+constexpr int MAXL = 8;
+
+struct Layout {
+    int nx = 0, ny = 0, pitch = 0, rows = 0;
+    void init(int nx_, int ny_) {
+        nx = nx_;
+        ny = ny_;
+        pitch = ((nx + 2 * HX) + 63) / 64 * 64;
+        rows = ny + 2 * HY;
+    }
+    size_t plane() const { return (size_t)pitch * rows; }
+    size_t origin() const { return (size_t)HY * pitch + HX; }
+    // offset of (j,i) relative to the origin (j=1,i=1)
+    long off(int j, int i) const { return (long)(j - 1) * pitch + (i - 1); }
+};
+
+// CPML profile vectors in the argument order of src/PML_pro.c:65-68.
+enum PmlVec {
+    P_DX = 0, P_KX, P_ALX, P_AX, P_BX, P_DXH, P_KXH, P_ALXH, P_AXH, P_BXH,
+    P_DY, P_KY, P_ALY, P_AY, P_BY, P_DYH, P_KYH, P_ALYH, P_AYH, P_BYH, P_NVEC
+};
+
+// Host-side results of the set-up math (all float, reference operation order).
+struct HostSetup {
+    float hc[7] = {0, 0, 0, 0, 0, 0, 0};
+    std::vector<float> pml[P_NVEC];    // index 0..2FW (0 unused)
+};
+
+// Material fields prepared on the host in the device layout.
+struct HostMaterial {
+    std::vector<float> rip, rjp;           // 1/rho at i+1/2 and j+1/2
+    std::vector<float> muipjp, f, g;       // elastic: uipjp, rho vs^2, rho(vp^2-2vs^2)
+                                           // visco:   fipjp, f, g (contain DT)
+    std::vector<float> dvis, evis, dipvis; // visco: L planes each
+    float bip[MAXL + 1] = {0}, cip[MAXL + 1] = {0}, bjm[MAXL + 1] = {0}, cjm[MAXL + 1] = {0};
+    std::vector<float> fs_fjm, fs_g, fs_dme; // visco free-surface per column
+    float fs_bjm[MAXL + 1] = {0};
+    std::vector<float> vp, vs, rho;        // raw model in device layout
+};
+
+// set-up math (dn_setup.cpp)
+int holberg(int fdorder, int maxrelerror, float hc[7], std::string &err);
+void pml_profiles(const dn_params &p, HostSetup &s);
+int check_stability(const dn_params &p, const float hc[7], const float *vp, const float *vs,
+                    const float *rho, const float *taus, const float *taup, std::string &err);
+void prepare_material(const dn_params &p, const Layout &lay, const float *vp, const float *vs,
+                      const float *rho, const float *taus, const float *taup, HostMaterial &m);
+
+// Kernel argument block (passed by value).
+struct KArgs {
+    int nx, ny, pitch, fw, fw2;
+    int free_surf, boundary, grad_form, L, nt_total;
+    float dt, dh;
+    float dhi_el;   // DT/DH  (update_s_elastic, seismo_ssg)
+    float dhi_vis;  // 1/DH   (update_s_visc, surfaces)
+    float dthalbe;  // DT/2
+    float hc[8];
+    long plane;
+    // wavefields at (1,1)
+    float *vx, *vy, *sxx, *syy, *sxy, *vxp1, *vyp1, *ux, *uy, *uxy;
+    // velocity material
+    const float *rip, *rjp;
+    // stress material
+    const float *muipjp, *f, *g;
+    // visco
+    float *rmem, *pmem, *qmem;
+    const float *dvis, *evis, *dipvis;
+    float bip[MAXL + 1], cip[MAXL + 1], bjm[MAXL + 1], cjm[MAXL + 1];
+    const float *fs_fjm, *fs_g, *fs_dme;
+    float fs_bjm[MAXL + 1];
+    // raw model (FWI)
+    const float *vp, *vs, *rho;
+    // CPML memory and profiles (profiles indexed 1..2FW)
+    float *psi_sxx_x, *psi_sxy_x, *psi_vxx, *psi_vyx, *psi_vxxs;
+    float *psi_syy_y, *psi_sxy_y, *psi_vyy, *psi_vxy;
+    const float *K_x, *a_x, *b_x, *K_xh, *a_xh, *b_xh;
+    const float *K_y, *a_y, *b_y, *K_yh, *a_yh, *b_yh;
+};
+
+// Per-shot acquisition on the device.
+struct DevAcq {
+    int nsrc = 0, ntr = 0, nt = 0;
+    int *src_ix = nullptr, *src_iy = nullptr, *src_type = nullptr;
+    double *src_sin = nullptr, *src_cos = nullptr;
+    float *signals = nullptr;             // nsrc x NT
+    int *rec_ix = nullptr, *rec_iy = nullptr;
+    double *rec_sqlam = nullptr, *rec_sqmu = nullptr;   // seismo_ssg SEISMO 3/4
+    float *sec[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // ntr x NT
+    float *adj[3] = {nullptr, nullptr, nullptr};                     // ntr x NT
+    float *obs = nullptr;                 // scratch for observed traces
+    double *l2 = nullptr;                 // device scalar
+    int *nan_step = nullptr;              // device scalar
+};
+
+// kernel launchers (dn_kernels.hip); all enqueue on `stream`
+void launch_update_v(const KArgs &a, int fdoh, int sw, int write_vp1, void *stream);
+void launch_update_s(const KArgs &a, int fdoh, int mode0_gf2, void *stream);
+void launch_surface(const KArgs &a, int fdoh, void *stream);
+void launch_force_sources(const KArgs &a, const DevAcq &q, int nt, void *stream);
+void launch_sources_fwd(const KArgs &a, const DevAcq &q, int nt, int qtype, int has_force, void *stream);
+void launch_sources_adj(const KArgs &a, const DevAcq &q, int nt, int quelltypb, void *stream);
+void launch_adj_psource(const KArgs &a, const DevAcq &q, int nt, void *stream);
+void launch_end_step(const KArgs &a, const DevAcq &q, int nt, int seismo, int fdoh, int invmat1,
+                     int probe, void *stream);
+void launch_store(const KArgs &a, float *const fwd[5], long k, void *stream);
+void launch_correlate(const KArgs &a, const float *const fwd[5], long k, float *wc, float *wcu,
+                      float *wcr, int invmat1, void *stream);
+void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const float *wcr, float *gvp,
+                     float *gvs, float *grho, int invmat1, void *stream);
+void launch_residual(const DevAcq &q, int comp, const float *obs, float dt, void *stream);
+void launch_l2(const DevAcq &q, int comp, void *stream);
+
+}  // namespace dn

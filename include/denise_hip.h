@@ -1,0 +1,181 @@
+/*
+ * denise_hip.h -- C ABI of the MI355X (gfx950) P/SV time-stepping library
+ * (libdnhip.so).
+ *
+ * This is the drop-in seam for DENISE Black Edition's per-shot P/SV path.
+ * The reference crosses it with
+ *     psv(struct wavePSV*, struct wavePSV_PML*, struct matPSV*, struct fwiPSV*,
+ *         struct mpiPSV*, struct seisPSV*, struct seisPSVfwi*, struct acq*,
+ *         float *hc, int ishot, int nshots, int nsrc_loc, int ns, int ntr,
+ *         float **Ws, float **Wr, int hin, int *DTINV_help, int mode, ...)
+ *     (src/PSV/psv.c:17-19, prototype include/fd.h:274-276)
+ * called from src/PSV/FD_PSV.c:351 (forward), src/PSV/grad_obj_psv.c:173
+ * (forward with snapshot store) and :211 (adjoint with correlation), plus the
+ * host set-up it relies on (PML_pro, holbergcoeff, av_mue/av_rho/av_tau,
+ * prepare_update_s_visc_PSV, calc_res, ass_gradPSV).
+ *
+ * Conventions (all entry points):
+ *   - plain pointers and sizes; caller owns host buffers; the context owns
+ *     device memory;
+ *   - models and gradients are NX*NY float32 in the reference's file order
+ *     (x-major, y fastest: src/PSV/readmod_elastic_PSV.c:72-76,
+ *     src/PSV/extract_LBFGS_PSV.c:103-110);
+ *   - seismograms / adjoint sources are trace-major ntr x NT float32
+ *     (the reference's section[itr][nt], src/seismo_ssg.c:32-35);
+ *   - grid indices are the reference's 1-based (i = 1..NX, j = 1..NY);
+ *   - every call returns DN_OK (0) or a negative dn_status; nothing exits the
+ *     process; dn_last_error() gives the message (thread-local).
+ * Contexts are independent; one context must not be used from two threads at
+ * once.  Each context owns one HIP stream.
+ */
+#ifndef DENISE_HIP_H
+#define DENISE_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DN_ABI_VERSION 1
+
+typedef enum dn_status {
+    DN_OK = 0,
+    DN_ERR_ARG = -1,        /* invalid argument / configuration            */
+    DN_ERR_HIP = -2,        /* HIP runtime error                           */
+    DN_ERR_OOM = -3,        /* device allocation failed                    */
+    DN_ERR_UNSTABLE = -4,   /* NaN probe tripped (src/PSV/psv.c:99-103)     */
+    DN_ERR_STATE = -5,      /* call out of order (e.g. adjoint before store)*/
+    DN_ERR_CFL = -6         /* DT above the stability limit
+                               (src/checkfd_ssg_elastic.c:127-128)          */
+} dn_status;
+
+/* Run parameters: the reference globals the P/SV path reads
+ * (include/globvar.h; values as parsed by src/read_par.c:92-449). */
+typedef struct dn_params {
+    int nx, ny, nt;          /* NX, NY, NT = iround(TIME/DT)                  */
+    int fdorder;             /* FDORDER 2..12                                  */
+    int maxrelerror;         /* MAXRELERROR 0..4                               */
+    int fw;                  /* FW (CPML width in cells)                       */
+    int free_surf;           /* FREE_SURF                                      */
+    int boundary;            /* BOUNDARY (must be 0)                           */
+    int L;                   /* relaxation mechanisms, 0 = elastic (<= 8)      */
+    int seismo;              /* SEISMO 0..4                                    */
+    int ndt;                 /* NDT                                            */
+    int invmat1;             /* INVMAT1 1 (vp,vs,rho) or 3 (lambda,mu,rho)     */
+    int grad_form;           /* GRAD_FORM 1 or 2                               */
+    int dtinv;               /* DTINV                                          */
+    int quelltypb;           /* QUELLTYPB 1..7                                 */
+    int mode;                /* MODE: 0 forward only, 1 FWI (store snapshots)  */
+    float dh, dt;            /* DH, DT                                         */
+    float damping, fpml, npower, k_max_pml;   /* CPML (src/PML_pro.c)          */
+    float ts;                /* TS = 1/max(fc) (src/sources.c:92)              */
+    float fl[8];             /* FL[1..L]                                       */
+} dn_params;
+
+typedef struct dn_ctx dn_ctx;
+
+/* ids for dn_get_field: padded NR layout (rows -nd+1..NY+nd, columns
+ * -nd+1..NX+nd, nd = FDORDER/2+1, x fastest), i.e. the reference's
+ * float** arrays (src/util.c:177-203). */
+enum {
+    DN_F_VX = 0, DN_F_VY, DN_F_SXX, DN_F_SYY, DN_F_SXY, DN_F_VXP1, DN_F_VYP1,
+    DN_F_UX, DN_F_UY, DN_F_UXY
+};
+
+/* Library / device information. */
+int dn_abi_version(void);
+const char *dn_last_error(void);
+int dn_device_count(int *count);
+
+/* Context.  Replaces FD_PSV.c:170-249: allocation (alloc_PSV), CPML profiles
+ * (src/PML_pro.c:65-407), FD coefficients (src/holbergcoeff.c:21-90) and the
+ * stability check (src/checkfd_ssg_elastic.c:9-145 / checkfd_ssg_visc.c).
+ * device: HIP device ordinal. */
+int dn_ctx_create(const dn_params *p, int device, dn_ctx **out);
+int dn_ctx_destroy(dn_ctx *ctx);
+
+/* Model upload + material preparation.  Replaces readmod_*_PSV (in-memory
+ * arrays instead of files), matcopy_elastic_PSV / matcopy_PSV (one-rank
+ * periodic ghosts), av_mue, av_rho, av_tau, prepare_update_s_visc_PSV
+ * (src/PSV/FD_PSV.c:203-249).  vp/vs/rho (INVMAT1=3: lambda/mu/rho) and, for
+ * L>0, taus/taup: NX*NY x-major.  Runs checkfd_ssg_*: DN_ERR_CFL if DT is
+ * above the stability limit. */
+int dn_set_model(dn_ctx *ctx, const float *vp, const float *vs, const float *rho,
+                 const float *taus, const float *taup);
+
+/* Sources of the shot.  srcpos: 8 x nsrc rows as srcpos_loc
+ * (src/splitsrc.c:28-36): [0] x grid index, [1] y grid index, [2] source
+ * number, [3] tshift, [4] fc, [5] amp, [6] azimuth (deg), [7] type
+ * (QUELLTYP 1 explosive, 2/3 force x/y, 4 rotated force, 5 moment tensor).
+ * signals: nsrc x NT (src/wavelet.c:10-121 output). */
+int dn_set_sources(dn_ctx *ctx, int nsrc, const float *srcpos, const float *signals);
+
+/* Receivers: recpos 2 x ntr, [0..ntr) x grid indices, [ntr..2ntr) y grid
+ * indices (src/splitrec.c:29-31).  Positions must be distinct (the reference
+ * removes duplicates in src/receiver.c:73-86). */
+int dn_set_receivers(dn_ctx *ctx, int ntr, const int *recpos);
+
+/* psv() mode 0: nsteps (<= NT, 0 = NT) forward time steps on the device.
+ * With params.mode > 0 the forward fields are stored every DTINV steps
+ * (src/PSV/psv.c:240-309). Blocks until done. */
+int dn_run_forward(dn_ctx *ctx, int nsteps);
+
+/* Seismograms of the last forward run, comp 0 vx, 1 vy, 2 p, 3 div, 4 curl,
+ * ntr x NT (src/seismo_ssg.c:25-106). */
+int dn_get_seismogram(dn_ctx *ctx, int comp, float *out);
+
+/* L2 data residuals as adjoint sources (LNORM=2, GRAD_FORM=1): time-integrated
+ * synthetic minus observed, time reversed (src/calc_res.c:296-452,
+ * src/PSV/calc_res_PSV.c:8-184, src/inseis.c:96-103).  obs_*: ntr x NT SU
+ * samples (NULL for a component QUELLTYPB does not use).  l2 (may be NULL)
+ * gets the misfit 0.5*sum(residual^2) when testshot != 0, else 0. */
+int dn_calc_residuals(dn_ctx *ctx, const float *obs_vx, const float *obs_vy,
+                      const float *obs_p, int testshot, double *l2);
+
+/* Adjoint sources supplied directly (ntr x NT each, NULL = zero):
+ * sectionvxdiff / sectionvydiff / sectionpdiff of the reference. */
+int dn_set_adjoint_sources(dn_ctx *ctx, const float *dvx, const float *dvy, const float *dp);
+int dn_get_adjoint_sources(dn_ctx *ctx, int comp, float *out);
+
+/* psv() mode 1: adjoint propagation from the receivers with the
+ * cross-correlation against the stored forward fields
+ * (src/PSV/psv.c:313-387). Requires a preceding dn_run_forward with
+ * params.mode > 0 over the full NT. */
+int dn_run_adjoint(dn_ctx *ctx, int nsteps);
+
+/* ass_gradPSV (src/PSV/ass_gradPSV.c:10-155), iteration 1 with INV_*_ITER=0. */
+int dn_assemble_gradient(dn_ctx *ctx);
+
+/* which: 0..2 raw correlation sums (waveconv_shot, waveconv_u_shot,
+ * waveconv_rho_shot), 3..5 per-shot vp / vs / rho gradients.  NX*NY
+ * x-major. */
+int dn_get_gradient(dn_ctx *ctx, int which, float *out);
+
+/* Diagnostics used by the tests. */
+int dn_get_dims(dn_ctx *ctx, int *nd, int *ntdtinv);
+int dn_get_field(dn_ctx *ctx, int id, float *out);
+/* CPML memory: 0 psi_sxx_x 1 psi_sxy_x 2 psi_vxx 3 psi_vyx 4 psi_vxxs
+ * (NY x 2FW), 5 psi_syy_y 6 psi_sxy_y 7 psi_vyy 8 psi_vxy (2FW x NX). */
+int dn_get_psi(dn_ctx *ctx, int id, float *out);
+/* memory variables which 0 r, 1 p, 2 q; l = 1..L; padded NR layout */
+int dn_get_memvar(dn_ctx *ctx, int which, int l, float *out);
+/* CPML profiles (20 vectors of 2FW, PML_pro argument order) and hc[0..6] */
+int dn_get_pml(dn_ctx *ctx, int which, float *out);
+int dn_get_hc(dn_ctx *ctx, float *out7);
+
+/* Timing of the last run: total device milliseconds (hipEvents around the
+ * time loop on the context stream) and, when kernel sampling is on, the
+ * average duration of the velocity and stress kernels. every = 0 turns
+ * sampling off; every = k times one launch of each in k steps. */
+int dn_set_kernel_sampling(dn_ctx *ctx, int every);
+int dn_get_timing(dn_ctx *ctx, double *run_ms, double *v_kernel_ms, double *s_kernel_ms,
+                  int *samples);
+
+/* Device bytes held by the context. */
+int dn_get_footprint(dn_ctx *ctx, size_t *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DENISE_HIP_H */
