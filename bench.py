@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the P/SV hot path on one MI355X.
+
+A "step" here is one whole shot: NT time steps of the viscoelastic P/SV
+update over the 4000 x 1200 grid (the single-shot workload the metric is
+quoted on: DH=5 m, FD8 Holberg, DT=0.4 ms, NT=6000, L=2 relaxation
+mechanisms with Q=50, CPML 20 on three sides plus free surface, Ricker
+15 Hz, 800 receivers), inputs resident in HBM before timing starts.
+
+value  = cell-updates per second over the K timed shots (Gcell-updates/s).
+roofline = algorithmic HBM bytes of the dominant kernel (the viscoelastic
+           stress/memory-variable update, 4*(14+6L) = 104 B per cell at L=2)
+           divided by its average launch time from HIP events on the
+           context stream, against the 8 TB/s MI355X HBM peak.
+cpu_baseline = the FP32 CPU oracle (the reference algorithm restated in C,
+           OpenMP over the host cores) on the first steps of the same shot.
+The elastic FD8 4000x1200 shot (the north-star roofline target) is reported
+alongside in "elastic".
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "denise-black-edition_amd"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-updates/s and %HBM roofline per shot; batched shots/s + p50/p99"
+
+
+def bytes_per_cell(L):
+    """Algorithmic HBM bytes per cell-update: velocity sweep 36 B plus the
+    stress sweep (elastic 44 B; viscoelastic 4*(14+6L) B)."""
+    vel = 36
+    stress = 44 if L == 0 else 4 * (14 + 6 * L)
+    return vel, stress
+
+
+def workload(L, nt=6000, nrec=800, seed=11):
+    import dnhip
+    import synth
+    nx, ny, dh, dt, fc = 4000, 1200, 5.0, 4.0e-4, 15.0
+    vp, vs, rho = synth.layered_model(nx, ny, dh, seed=seed)
+    taus = taup = None
+    if L:
+        taus = np.full((nx, ny), 2.0 / 50.0, np.float32)
+        taup = taus.copy()
+    p = dnhip.Params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=8, maxrelerror=1, fw=20,
+                     free_surf=1, L=L, seismo=1, ts=float(np.float32(1.0 / fc)),
+                     damping=1500.0, fpml=fc, fl=[3.2, 32.0][:L] if L else [])
+    src = synth.source_table(nx * dh / 2, 2 * dh, dh, fc=fc)
+    sig = synth.ricker(nt, dt, fc)[None, :]
+    rec = synth.receiver_line(2 * dh, 25.0, nrec, 2 * dh, dh)
+    return p, dict(vp=vp, vs=vs, rho=rho, taus=taus, taup=taup), src, sig, rec
+
+
+def run_gpu(L, steps, warmup, sample_every=20):
+    import dnhip
+    p, m, src, sig, rec = workload(L)
+    ctx = dnhip.Context(p)
+    ctx.set_model(m["vp"], m["vs"], m["rho"], m["taus"], m["taup"])
+    ctx.set_sources(src, sig)
+    ctx.set_receivers(rec)
+    for _ in range(warmup):
+        ctx.forward()
+    ctx.set_kernel_sampling(sample_every)
+    walls, v_ms, s_ms, dev_ms = [], [], [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        ctx.forward()
+        walls.append(time.perf_counter() - t0)
+        tm = ctx.timing()
+        v_ms.append(tm["v_kernel_ms"])
+        s_ms.append(tm["s_kernel_ms"])
+        dev_ms.append(tm["run_ms"])
+    seis = ctx.seismogram(0)
+    assert np.isfinite(seis).all()
+    cells = p.nx * p.ny
+    total = float(sum(walls))
+    vel_b, str_b = bytes_per_cell(L)
+    s_avg = float(np.mean(s_ms))
+    v_avg = float(np.mean(v_ms))
+    out = {
+        "cells_per_step": cells, "nt": p.nt,
+        "gcells_per_s": cells * p.nt * steps / total / 1e9,
+        "ms_per_shot": total / steps * 1e3,
+        "device_ms_per_shot": float(np.mean(dev_ms)),
+        "v_kernel_ms": v_avg, "s_kernel_ms": s_avg,
+        "v_kernel_GBs": vel_b * cells / (v_avg * 1e-3) / 1e9 if v_avg > 0 else None,
+        "s_kernel_GBs": str_b * cells / (s_avg * 1e-3) / 1e9 if s_avg > 0 else None,
+        "step_GBs": (vel_b + str_b) * cells * p.nt * steps / total / 1e9,
+        "stress_bytes_per_cell": str_b, "vel_bytes_per_cell": vel_b,
+        "footprint_GB": ctx.footprint() / 1e9,
+    }
+    ctx.close()
+    return out
+
+
+def cpu_baseline(L, nsteps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import orc
+    p, m, src, sig, rec = workload(L)
+    orc.set_threads(threads)
+    o = orc.Oracle(p, m["vp"], m["vs"], m["rho"], m["taus"], m["taup"], prec="f32")
+    o.set_sources(src, sig)
+    o.set_receivers(rec)
+    t0 = time.perf_counter()
+    o.forward(nsteps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return p.nx * p.ny * nsteps / dt / 1e9, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=150)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-elastic", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU shot sharding is a "
+                                   "design note (one shot per GPU), run with --gpus 1"}))
+        sys.exit(2)
+    L = 2
+    res = run_gpu(L, args.steps, args.warmup)
+    print(f"# visco: {res}", file=sys.stderr, flush=True)
+    elastic = None
+    if not args.no_elastic:
+        elastic = run_gpu(0, 1, 1)
+        print(f"# elastic: {elastic}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        gcps, secs = cpu_baseline(L, args.cpu_steps, threads)
+        cpu = {"value": gcps, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+               "sample": f"first {args.cpu_steps} of 6000 time steps of the same 4000x1200 "
+                         f"viscoelastic shot, FP32 oracle, {secs:.1f} s wall"}
+    achieved = res["s_kernel_GBs"]
+    line = {
+        "metric": METRIC,
+        "value": res["gcells_per_s"],
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["ms_per_shot"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded smooth-random layered model, Ricker source)",
+        "config": {"workload": "single-shot viscoelastic P/SV 4000x1200, DH=5 m, FD8 Holberg, "
+                               "DT=0.4 ms, NT=6000, L=2 (Q=50), CPML 20 + free surface, "
+                               "Ricker 15 Hz, 800 receivers",
+                   "nx": 4000, "ny": 1200, "nt": 6000, "L": 2, "fdorder": 8,
+                   "parallelism": "single GPU, whole-grid domain"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "kernel": "viscoelastic stress + memory-variable update (FD8, L=2)",
+                     "bytes_per_cell": res["stress_bytes_per_cell"],
+                     "avg_launch_ms": res["s_kernel_ms"]},
+        "cpu_baseline": cpu,
+        "detail": {"visco": res, "elastic": elastic,
+                   "elastic_step_frac_of_hbm": (elastic["step_GBs"] / HBM_PEAK_GBS) if elastic else None},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
