@@ -52,7 +52,8 @@ struct dn_ctx {
     float *fld[F_N] = {};  // plane bases
     float *rip = nullptr, *rjp = nullptr, *muipjp = nullptr, *f = nullptr, *g = nullptr;
     float *vp = nullptr, *vs = nullptr, *rho = nullptr;
-    float *rmem = nullptr, *pmem = nullptr, *qmem = nullptr, *dvis = nullptr, *evis = nullptr, *dipvis = nullptr;
+    float *rmem = nullptr, *pmem = nullptr, *qmem = nullptr;
+    float *vmu = nullptr, *vts = nullptr, *vpi = nullptr, *vtp = nullptr, *vmuij = nullptr, *vtsij = nullptr;
     float *fs_fjm = nullptr, *fs_g = nullptr, *fs_dme = nullptr;
     float *psi[9] = {};
     float *pml[P_NVEC] = {};
@@ -61,9 +62,15 @@ struct dn_ctx {
     HostMaterial mat;
     std::vector<float> h_vp, h_vs, h_rho;  // x-major model kept for receiver factors
     DevAcq acq;
-    std::vector<int> rec_ix, rec_iy;
+    std::vector<int> rec_ix, rec_iy, src_ix, src_iy, src_type;
     int qtype = 0, has_force = 0;
     KArgs a{};
+    // tile shape of the sweeps and the per-tile cell work lists
+    // (0 forward velocity, 1 forward stress, 2 adjoint velocity, 3 adjoint stress)
+    Tile tile;
+    int ntx = 0, ntiles = 0;
+    int *lst_off[4] = {};
+    Ent *lst_ent[4] = {};
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double run_ms = 0.0, v_ms = 0.0, s_ms = 0.0;
@@ -159,10 +166,14 @@ void fill_args(dn_ctx *c)
     a.rmem = at(c->rmem);
     a.pmem = at(c->pmem);
     a.qmem = at(c->qmem);
-    a.dvis = at(c->dvis);
-    a.evis = at(c->evis);
-    a.dipvis = at(c->dipvis);
+    a.vmu = at(c->vmu);
+    a.vts = at(c->vts);
+    a.vpi = at(c->vpi);
+    a.vtp = at(c->vtp);
+    a.vmuij = at(c->vmuij);
+    a.vtsij = at(c->vtsij);
     for (int l = 0; l <= MAXL; ++l) {
+        a.peta[l] = c->mat.peta[l];
         a.bip[l] = c->mat.bip[l];
         a.cip[l] = c->mat.cip[l];
         a.bjm[l] = c->mat.bjm[l];
@@ -193,6 +204,96 @@ void fill_args(dn_ctx *c)
     a.K_yh = c->pml[P_KYH];
     a.a_yh = c->pml[P_AYH];
     a.b_yh = c->pml[P_BYH];
+}
+
+// Tile shape used when the caller does not pick one: the widest tile that
+// still gives every CU several workgroups.
+Tile default_tile(int nx, int ny)
+{
+    (void)nx;
+    (void)ny;
+    Tile t;
+    t.tbx = 64;
+    t.by = 4;
+    t.ty = 1;
+    return t;
+}
+
+// Per-tile cell work lists of the two sweeps (see Ent in dn_internal.h).
+int build_lists(dn_ctx *c)
+{
+    const int NX = c->p.nx, NY = c->p.ny;
+    const Tile &t = c->tile;
+    c->ntx = t.ntx(NX);
+    c->ntiles = c->ntx * t.nty(NY);
+    struct Item {
+        int tile;
+        Ent e;
+    };
+    std::vector<Item> lists[4];
+    auto add = [&](int which, int i, int j, int kind, int idx) {
+        const int tx = (i - 1) / t.tbx, ty = (j - 1) / t.rows();
+        Item it;
+        it.tile = ty * c->ntx + tx;
+        it.e.lane = (i - 1) - tx * t.tbx;
+        it.e.row = j;
+        it.e.kind = kind;
+        it.e.idx = idx;
+        lists[which].push_back(it);
+    };
+    const int ns = (int)c->src_ix.size(), nr = (int)c->rec_ix.size();
+    const int seismo = c->p.seismo;
+    for (int l = 0; l < ns; ++l)
+        if (c->src_type[l] >= 2 && c->src_type[l] <= 4) add(0, c->src_ix[l], c->src_iy[l], E_FORCE, l);
+    if (seismo == 1 || seismo == 4)
+        for (int r = 0; r < nr; ++r) add(0, c->rec_ix[r], c->rec_iy[r], E_REC_V, r);
+    if (c->qtype == 1)
+        for (int l = 0; l < ns; ++l) add(1, c->src_ix[l], c->src_iy[l], E_EXPLOSIVE, l);
+    if (seismo >= 2)
+        for (int r = 0; r < nr; ++r) add(1, c->rec_ix[r], c->rec_iy[r], E_REC_S, r);
+    for (int r = 0; r < nr; ++r) add(2, c->rec_ix[r], c->rec_iy[r], E_ADJ_FORCE, r);
+    if (c->p.quelltypb >= 4)
+        for (int r = 0; r < nr; ++r) add(3, c->rec_ix[r], c->rec_iy[r], E_ADJ_P, r);
+    for (int w = 0; w < 4; ++w) {
+        std::stable_sort(lists[w].begin(), lists[w].end(),
+                         [](const Item &x, const Item &y) { return x.tile < y.tile; });
+        std::vector<int> off((size_t)c->ntiles + 1, 0);
+        std::vector<Ent> ent(lists[w].size() + 1);
+        for (auto &it : lists[w]) off[(size_t)it.tile + 1]++;
+        for (int k = 0; k < c->ntiles; ++k) off[(size_t)k + 1] += off[k];
+        for (size_t k = 0; k < lists[w].size(); ++k) ent[k] = lists[w][k].e;
+        if (c->lst_off[w]) c->dfree(c->lst_off[w]);
+        if (c->lst_ent[w]) c->dfree(c->lst_ent[w]);
+        c->lst_off[w] = nullptr;
+        c->lst_ent[w] = nullptr;
+        int rc;
+        if ((rc = c->dalloc((void **)&c->lst_off[w], off.size() * sizeof(int))) ||
+            (rc = c->dalloc((void **)&c->lst_ent[w], ent.size() * sizeof(Ent))))
+            return rc;
+        HIP_OK(hipMemcpy(c->lst_off[w], off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(c->lst_ent[w], ent.data(), ent.size() * sizeof(Ent), hipMemcpyHostToDevice));
+    }
+    return DN_OK;
+}
+
+StepCtl base_ctl(dn_ctx *c, int lv, int ls)
+{
+    StepCtl s{};
+    s.ntx = c->ntx;
+    s.ntiles = c->ntiles;
+    s.vlist.off = c->lst_off[lv];
+    s.vlist.ent = c->lst_ent[lv];
+    s.slist.off = c->lst_off[ls];
+    s.slist.ent = c->lst_ent[ls];
+    for (int k = 0; k < 5; ++k) s.fwd[k] = c->fwd[k];
+    s.wc = c->wc;
+    s.wcu = c->wcu;
+    s.wcr = c->wcr;
+    s.quelltypb = c->p.quelltypb;
+    s.invmat1 = c->p.invmat1;
+    s.store_k = -1;
+    s.corr_k = -1;
+    return s;
 }
 
 int validate(const dn_params *p)
@@ -357,18 +458,24 @@ int dn_ctx_create(const dn_params *p, int device, dn_ctx **out)
         if ((k == F_UX || k == F_UY || k == F_UXY) && !gf2) continue;
         if ((rc = c->dalloc((void **)&c->fld[k], pb))) return bail(rc);
     }
-    float **mats[] = {&c->rip, &c->rjp, &c->muipjp, &c->f, &c->g};
-    for (auto m : mats)
-        if ((rc = c->dalloc((void **)m, pb))) return bail(rc);
+    if ((rc = c->dalloc((void **)&c->rip, pb)) || (rc = c->dalloc((void **)&c->rjp, pb))) return bail(rc);
+    if (p->L == 0) {
+        float **mats[] = {&c->muipjp, &c->f, &c->g};
+        for (auto m : mats)
+            if ((rc = c->dalloc((void **)m, pb))) return bail(rc);
+    }
     if (p->mode > 0) {
         if ((rc = c->dalloc((void **)&c->vp, pb)) || (rc = c->dalloc((void **)&c->vs, pb)) ||
             (rc = c->dalloc((void **)&c->rho, pb)))
             return bail(rc);
     }
     if (p->L > 0) {
-        float **vis[] = {&c->rmem, &c->pmem, &c->qmem, &c->dvis, &c->evis, &c->dipvis};
-        for (auto m : vis)
+        float **mem[] = {&c->rmem, &c->pmem, &c->qmem};
+        for (auto m : mem)
             if ((rc = c->dalloc((void **)m, pb * p->L))) return bail(rc);
+        float **vis[] = {&c->vmu, &c->vts, &c->vpi, &c->vtp, &c->vmuij, &c->vtsij};
+        for (auto m : vis)
+            if ((rc = c->dalloc((void **)m, pb))) return bail(rc);
         if ((rc = c->dalloc((void **)&c->fs_fjm, sizeof(float) * (p->nx + 1))) ||
             (rc = c->dalloc((void **)&c->fs_g, sizeof(float) * (p->nx + 1))) ||
             (rc = c->dalloc((void **)&c->fs_dme, sizeof(float) * (p->nx + 1))))
@@ -394,6 +501,8 @@ int dn_ctx_create(const dn_params *p, int device, dn_ctx **out)
     if ((rc = c->dalloc((void **)&c->acq.nan_step, sizeof(int))) || (rc = c->dalloc((void **)&c->acq.l2, sizeof(double))))
         return bail(rc);
     c->acq.nt = p->nt;
+    c->tile = default_tile(p->nx, p->ny);
+    if ((rc = build_lists(c))) return bail(rc);
     fill_args(c);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(DN_ERR_HIP, "device synchronisation failed"));
     *out = c;
@@ -417,19 +526,22 @@ int dn_set_model(dn_ctx *c, const float *vp, const float *vs, const float *rho, 
     if (rc) return fail(rc, err);
     prepare_material(c->p, c->lay, vp, vs, rho, taus, taup, c->mat);
     const size_t pl = c->lay.plane();
-    if ((rc = upload(c, c->rip, c->mat.rip.data(), pl)) || (rc = upload(c, c->rjp, c->mat.rjp.data(), pl)) ||
-        (rc = upload(c, c->muipjp, c->mat.muipjp.data(), pl)) || (rc = upload(c, c->f, c->mat.f.data(), pl)) ||
-        (rc = upload(c, c->g, c->mat.g.data(), pl)))
+    if ((rc = upload(c, c->rip, c->mat.rip.data(), pl)) || (rc = upload(c, c->rjp, c->mat.rjp.data(), pl)))
         return rc;
+    if (c->p.L == 0) {
+        if ((rc = upload(c, c->muipjp, c->mat.muipjp.data(), pl)) || (rc = upload(c, c->f, c->mat.f.data(), pl)) ||
+            (rc = upload(c, c->g, c->mat.g.data(), pl)))
+            return rc;
+    }
     if (c->vp) {
         if ((rc = upload(c, c->vp, c->mat.vp.data(), pl)) || (rc = upload(c, c->vs, c->mat.vs.data(), pl)) ||
             (rc = upload(c, c->rho, c->mat.rho.data(), pl)))
             return rc;
     }
     if (c->p.L > 0) {
-        if ((rc = upload(c, c->dvis, c->mat.dvis.data(), pl * c->p.L)) ||
-            (rc = upload(c, c->evis, c->mat.evis.data(), pl * c->p.L)) ||
-            (rc = upload(c, c->dipvis, c->mat.dipvis.data(), pl * c->p.L)) ||
+        if ((rc = upload(c, c->vmu, c->mat.vmu.data(), pl)) || (rc = upload(c, c->vts, c->mat.vts.data(), pl)) ||
+            (rc = upload(c, c->vpi, c->mat.vpi.data(), pl)) || (rc = upload(c, c->vtp, c->mat.vtp.data(), pl)) ||
+            (rc = upload(c, c->vmuij, c->mat.vmuij.data(), pl)) || (rc = upload(c, c->vtsij, c->mat.vtsij.data(), pl)) ||
             (rc = upload(c, c->fs_fjm, c->mat.fs_fjm.data(), c->p.nx + 1)) ||
             (rc = upload(c, c->fs_g, c->mat.fs_g.data(), c->p.nx + 1)) ||
             (rc = upload(c, c->fs_dme, c->mat.fs_dme.data(), c->p.nx + 1)))
@@ -449,9 +561,10 @@ int dn_set_model(dn_ctx *c, const float *vp, const float *vs, const float *rho, 
     c->mat.vp.clear(); c->mat.vp.shrink_to_fit();
     c->mat.vs.clear(); c->mat.vs.shrink_to_fit();
     c->mat.rho.clear(); c->mat.rho.shrink_to_fit();
-    c->mat.dvis.clear(); c->mat.dvis.shrink_to_fit();
-    c->mat.evis.clear(); c->mat.evis.shrink_to_fit();
-    c->mat.dipvis.clear(); c->mat.dipvis.shrink_to_fit();
+    for (auto *v : {&c->mat.vmu, &c->mat.vts, &c->mat.vpi, &c->mat.vtp, &c->mat.vmuij, &c->mat.vtsij}) {
+        v->clear();
+        v->shrink_to_fit();
+    }
     fill_args(c);
     c->model_set = true;
     c->fwd_complete = false;
@@ -472,7 +585,10 @@ int dn_set_sources(dn_ctx *c, int nsrc, const float *srcpos, const float *signal
     q.nsrc = 0;
     c->qtype = 0;
     c->has_force = 0;
-    if (nsrc == 0) return DN_OK;
+    c->src_ix.clear();
+    c->src_iy.clear();
+    c->src_type.clear();
+    if (nsrc == 0) return build_lists(c);
     std::vector<int> ix(nsrc), iy(nsrc), type(nsrc);
     std::vector<double> sn(nsrc), cs(nsrc);
     for (int l = 0; l < nsrc; ++l) {
@@ -503,7 +619,10 @@ int dn_set_sources(dn_ctx *c, int nsrc, const float *srcpos, const float *signal
     HIP_OK(hipMemcpy(q.src_cos, cs.data(), sizeof(double) * nsrc, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(q.signals, signals, sizeof(float) * (size_t)nsrc * c->p.nt, hipMemcpyHostToDevice));
     q.nsrc = nsrc;
-    return DN_OK;
+    c->src_ix = ix;
+    c->src_iy = iy;
+    c->src_type = type;
+    return build_lists(c);
 }
 
 int dn_set_receivers(dn_ctx *c, int ntr, const int *recpos)
@@ -533,7 +652,7 @@ int dn_set_receivers(dn_ctx *c, int ntr, const int *recpos)
     q.ntr = 0;
     c->rec_ix.assign(recpos, recpos + ntr);
     c->rec_iy.assign(recpos + ntr, recpos + 2 * ntr);
-    if (ntr == 0) return DN_OK;
+    if (ntr == 0) return build_lists(c);
     const size_t tr = (size_t)ntr * c->p.nt * sizeof(float);
     int rc;
     if ((rc = c->dalloc((void **)&q.rec_ix, sizeof(int) * ntr)) || (rc = c->dalloc((void **)&q.rec_iy, sizeof(int) * ntr)) ||
@@ -547,7 +666,32 @@ int dn_set_receivers(dn_ctx *c, int ntr, const int *recpos)
     HIP_OK(hipMemcpy(q.rec_ix, c->rec_ix.data(), sizeof(int) * ntr, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(q.rec_iy, c->rec_iy.data(), sizeof(int) * ntr, hipMemcpyHostToDevice));
     q.ntr = ntr;
+    int rc2 = build_lists(c);
+    if (rc2) return rc2;
     return receiver_factors(c);
+}
+
+int dn_set_tile(dn_ctx *c, int tbx, int by, int ty)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    Tile t;
+    t.tbx = tbx;
+    t.by = by;
+    t.ty = ty;
+    if (!tile_supported(t)) return fail(DN_ERR_ARG, "tile shape not compiled into the library");
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->tile = t;
+    return build_lists(c);
+}
+
+int dn_get_tile(dn_ctx *c, int *tbx, int *by, int *ty)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    if (tbx) *tbx = c->tile.tbx;
+    if (by) *by = c->tile.by;
+    if (ty) *ty = c->tile.ty;
+    return DN_OK;
 }
 
 int dn_set_kernel_sampling(dn_ctx *c, int every)
@@ -569,34 +713,39 @@ int dn_run_forward(dn_ctx *c, int nsteps)
     c->fwd_complete = false;
     const KArgs &a = c->a;
     const int fwdstore = c->p.mode > 0;
-    const int mode0_gf2 = fwdstore && c->p.grad_form == 2;
     const int vp1_form = c->p.grad_form == 1 || c->p.grad_form == 2;
+    StepCtl ctl = base_ctl(c, 0, 1);
+    ctl.mode0_gf2 = fwdstore && c->p.grad_form == 2;
+    ctl.seismo = c->p.seismo;
+    ctl.probe = 1;
     size_t nev = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
     int hin = 1, hin1 = 1;
     for (int nt = 1; nt <= nsteps; ++nt) {
         const bool store = fwdstore && nt == hin1;
         const bool sample = c->sample_every > 0 && (nt % c->sample_every) == 0;
-        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+        ctl.nt = nt;
+        ctl.store_k = (store && c->qtype != 5) ? hin - 1 : -1;
+        ctl.write_vp1 = vp1_form && (nt == nsteps || (store && c->qtype == 5));
+        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
         if (sample) {
-            e0 = pool_event(c, nev); e1 = pool_event(c, nev + 1); e2 = pool_event(c, nev + 2); e3 = pool_event(c, nev + 3);
-            if (!e0 || !e1 || !e2 || !e3) return fail(DN_ERR_HIP, "event creation failed");
+            e0 = pool_event(c, nev); e1 = pool_event(c, nev + 1); e2 = pool_event(c, nev + 2);
+            if (!e0 || !e1 || !e2 || !pool_event(c, nev + 3)) return fail(DN_ERR_HIP, "event creation failed");
             HIP_OK(hipEventRecord(e0, c->stream));
         }
-        launch_update_v(a, c->fdoh, 0, vp1_form && (store || nt == nsteps), c->stream);
-        if (sample) HIP_OK(hipEventRecord(e1, c->stream));
-        if (c->has_force && c->acq.nsrc > 0) launch_force_sources(a, c->acq, nt, c->stream);
-        if (sample) HIP_OK(hipEventRecord(e2, c->stream));
-        launch_update_s(a, c->fdoh, mode0_gf2, c->stream);
+        launch_vel(a, ctl, c->acq, c->tile, c->fdoh, 0, c->stream);
         if (sample) {
-            HIP_OK(hipEventRecord(e3, c->stream));
+            HIP_OK(hipEventRecord(e1, c->stream));
+            HIP_OK(hipEventRecord(c->pool[nev + 2], c->stream));
+        }
+        launch_str(a, ctl, c->acq, c->tile, c->fdoh, c->stream);
+        if (sample) {
+            HIP_OK(hipEventRecord(c->pool[nev + 3], c->stream));
             nev += 4;
         }
-        if (c->acq.nsrc > 0) launch_sources_fwd(a, c->acq, nt, c->qtype, c->has_force, c->stream);
-        if (c->p.free_surf) launch_surface(a, c->fdoh, c->stream);
-        launch_end_step(a, c->acq, nt, c->p.seismo, c->fdoh, c->p.invmat1, nt < NT, c->stream);
+        if (c->qtype == 5 && c->acq.nsrc > 0) launch_msource(a, c->acq, nt, c->stream);
+        if (store && c->qtype == 5) launch_store(a, c->fwd, hin - 1, c->stream);
         if (store) {
-            launch_store(a, c->fwd, hin - 1, c->stream);
             ++hin;
             hin1 += c->p.dtinv;
         }
@@ -686,34 +835,33 @@ int dn_run_adjoint(dn_ctx *c, int nsteps)
     HIP_OK(hipMemsetAsync(c->wcu, 0, g, c->stream));
     HIP_OK(hipMemsetAsync(c->wcr, 0, g, c->stream));
     const KArgs &a = c->a;
+    StepCtl ctl = base_ctl(c, 2, 3);
+    ctl.probe = 1;
     size_t nev = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
     int hin = 1;
     for (int nt = 1; nt <= nsteps; ++nt) {
         const bool sample = c->sample_every > 0 && (nt % c->sample_every) == 0;
-        hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+        const int nfwd = NT - nt + 1;  // forward step this adjoint step meets
+        const bool corr = (nfwd - 1) % c->p.dtinv == 0;
+        ctl.nt = nt;
+        ctl.corr_k = corr ? c->ntdtinv - hin : -1;
         if (sample) {
-            e0 = pool_event(c, nev); e1 = pool_event(c, nev + 1); e2 = pool_event(c, nev + 2); e3 = pool_event(c, nev + 3);
-            if (!e0 || !e1 || !e2 || !e3) return fail(DN_ERR_HIP, "event creation failed");
-            HIP_OK(hipEventRecord(e0, c->stream));
+            for (int k = 0; k < 4; ++k)
+                if (!pool_event(c, nev + k)) return fail(DN_ERR_HIP, "event creation failed");
+            HIP_OK(hipEventRecord(c->pool[nev], c->stream));
         }
-        launch_update_v(a, c->fdoh, 1, 0, c->stream);
-        if (sample) HIP_OK(hipEventRecord(e1, c->stream));
-        launch_sources_adj(a, c->acq, nt, c->p.quelltypb, c->stream);
-        if (sample) HIP_OK(hipEventRecord(e2, c->stream));
-        launch_update_s(a, c->fdoh, 0, c->stream);
+        launch_vel(a, ctl, c->acq, c->tile, c->fdoh, 1, c->stream);
         if (sample) {
-            HIP_OK(hipEventRecord(e3, c->stream));
+            HIP_OK(hipEventRecord(c->pool[nev + 1], c->stream));
+            HIP_OK(hipEventRecord(c->pool[nev + 2], c->stream));
+        }
+        launch_str(a, ctl, c->acq, c->tile, c->fdoh, c->stream);
+        if (sample) {
+            HIP_OK(hipEventRecord(c->pool[nev + 3], c->stream));
             nev += 4;
         }
-        if (c->p.quelltypb >= 4) launch_adj_psource(a, c->acq, nt, c->stream);
-        if (c->p.free_surf) launch_surface(a, c->fdoh, c->stream);
-        launch_end_step(a, c->acq, nt, 0, c->fdoh, c->p.invmat1, nt < NT, c->stream);
-        const int nfwd = NT - nt + 1;  // forward step this adjoint step meets
-        if ((nfwd - 1) % c->p.dtinv == 0) {
-            launch_correlate(a, c->fwd, (long)(c->ntdtinv - hin), c->wc, c->wcu, c->wcr, c->p.invmat1, c->stream);
-            ++hin;
-        }
+        if (corr) ++hin;
     }
     HIP_OK(hipGetLastError());
     return finish_run(c, nev);
@@ -778,7 +926,19 @@ int dn_get_field(dn_ctx *c, int id, float *out)
 {
     if (!c || !out || id < 0 || id >= F_N) return fail(DN_ERR_ARG, "bad field request");
     if (!c->fld[id]) return fail(DN_ERR_STATE, "field not allocated in this configuration");
-    return get_plane_nr(c, c->fld[id], out);
+    int rc = get_plane_nr(c, c->fld[id], out);
+    if (rc || !c->p.free_surf || (id != F_SYY && id != F_SXY)) return rc;
+    // With a free surface the sweeps mirror the stress ghost rows on the fly;
+    // materialise them as the reference's surface routine stores them:
+    // syy[1-m] = -syy[1+m], sxy[1-m] = -sxy[m], m = 1..FDORDER/2.
+    const int nd = c->nd, ncol = c->p.nx + 2 * nd;
+    auto row = [&](int j) { return out + (size_t)(j + nd - 1) * ncol; };
+    for (int m = 1; m <= c->fdoh; ++m) {
+        float *dst = row(1 - m);
+        const float *src = id == F_SYY ? row(1 + m) : row(m);
+        for (int k = 0; k < ncol; ++k) dst[k] = -src[k];
+    }
+    return DN_OK;
 }
 
 int dn_get_psi(dn_ctx *c, int id, float *out)

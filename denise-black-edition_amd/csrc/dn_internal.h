@@ -51,8 +51,10 @@ struct HostSetup {
 struct HostMaterial {
     std::vector<float> rip, rjp;           // 1/rho at i+1/2 and j+1/2
     std::vector<float> muipjp, f, g;       // elastic: uipjp, rho vs^2, rho(vp^2-2vs^2)
-                                           // visco:   fipjp, f, g (contain DT)
-    std::vector<float> dvis, evis, dipvis; // visco: L planes each
+    // viscoelastic: relaxed moduli and relaxation times; the update rebuilds
+    // fipjp, f, g (with DT) and the per-mechanism d, e, dip from them
+    std::vector<float> vmu, vts, vpi, vtp, vmuij, vtsij;
+    float peta[MAXL + 1] = {0};
     float bip[MAXL + 1] = {0}, cip[MAXL + 1] = {0}, bjm[MAXL + 1] = {0}, cjm[MAXL + 1] = {0};
     std::vector<float> fs_fjm, fs_g, fs_dme; // visco free-surface per column
     float fs_bjm[MAXL + 1] = {0};
@@ -81,11 +83,12 @@ struct KArgs {
     float *vx, *vy, *sxx, *syy, *sxy, *vxp1, *vyp1, *ux, *uy, *uxy;
     // velocity material
     const float *rip, *rjp;
-    // stress material
+    // elastic stress material
     const float *muipjp, *f, *g;
-    // visco
+    // viscoelastic material and memory variables (L planes each)
+    const float *vmu, *vts, *vpi, *vtp, *vmuij, *vtsij;
     float *rmem, *pmem, *qmem;
-    const float *dvis, *evis, *dipvis;
+    float peta[MAXL + 1];
     float bip[MAXL + 1], cip[MAXL + 1], bjm[MAXL + 1], cjm[MAXL + 1];
     const float *fs_fjm, *fs_g, *fs_dme;
     float fs_bjm[MAXL + 1];
@@ -113,22 +116,55 @@ struct DevAcq {
     int *nan_step = nullptr;              // device scalar
 };
 
+// Work items a sweep applies to individual cells after its own update:
+// sources, adjoint sources and receivers, grouped by the tile that owns the
+// cell (CSR: off[tile]..off[tile+1] into ent, in source / trace order).
+// ent = {lane within tile, row j, kind, index}.
+enum { E_FORCE = 0, E_ADJ_FORCE = 1, E_REC_V = 2, E_EXPLOSIVE = 3, E_ADJ_P = 4, E_REC_S = 5 };
+struct Ent {
+    int lane, row, kind, idx;
+};
+struct CellList {
+    const int *off = nullptr;
+    const Ent *ent = nullptr;
+};
+
+// Tile shape of the two sweeps: blocks of tbx x by threads, each thread
+// marching ty consecutive rows of one column; a tile is tbx x (by*ty) cells.
+struct Tile {
+    int tbx = 64, by = 4, ty = 1;
+    int rows() const { return by * ty; }
+    int ntx(int nx) const { return (nx + tbx - 1) / tbx; }
+    int nty(int ny) const { return (ny + rows() - 1) / rows(); }
+};
+
+// Per-step control of the fused sweeps.
+struct StepCtl {
+    int nt;          // time step (1..NT)
+    int store_k;     // snapshot slot written this step by the sweeps, -1 none
+    int corr_k;      // snapshot slot correlated this step, -1 none
+    int write_vp1;   // forward: also write the vxp1/vyp1 planes
+    int seismo;      // receiver sampling mode this step (0 = none)
+    int mode0_gf2;   // forward with GRAD_FORM 2: write ux/uy/uxy
+    int probe;       // NaN probe of vy[NY/2][NX/2] at the start of the step
+    int quelltypb;
+    int invmat1;
+    int ntx, ntiles; // tile grid
+    CellList vlist, slist;
+    float *fwd[5];   // snapshots (compact NX*NY per slot)
+    float *wc, *wcu, *wcr;   // correlation accumulators (compact, [j][i])
+};
+
 // kernel launchers (dn_kernels.hip); all enqueue on `stream`
-void launch_update_v(const KArgs &a, int fdoh, int sw, int write_vp1, void *stream);
-void launch_update_s(const KArgs &a, int fdoh, int mode0_gf2, void *stream);
-void launch_surface(const KArgs &a, int fdoh, void *stream);
-void launch_force_sources(const KArgs &a, const DevAcq &q, int nt, void *stream);
-void launch_sources_fwd(const KArgs &a, const DevAcq &q, int nt, int qtype, int has_force, void *stream);
-void launch_sources_adj(const KArgs &a, const DevAcq &q, int nt, int quelltypb, void *stream);
-void launch_adj_psource(const KArgs &a, const DevAcq &q, int nt, void *stream);
-void launch_end_step(const KArgs &a, const DevAcq &q, int nt, int seismo, int fdoh, int invmat1,
-                     int probe, void *stream);
+void launch_vel(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, int sw,
+                void *stream);
+void launch_str(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, void *stream);
+void launch_msource(const KArgs &a, const DevAcq &q, int nt, void *stream);
 void launch_store(const KArgs &a, float *const fwd[5], long k, void *stream);
-void launch_correlate(const KArgs &a, const float *const fwd[5], long k, float *wc, float *wcu,
-                      float *wcr, int invmat1, void *stream);
 void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const float *wcr, float *gvp,
                      float *gvs, float *grho, int invmat1, void *stream);
 void launch_residual(const DevAcq &q, int comp, const float *obs, float dt, void *stream);
 void launch_l2(const DevAcq &q, int comp, void *stream);
+bool tile_supported(const Tile &t);
 
 }  // namespace dn

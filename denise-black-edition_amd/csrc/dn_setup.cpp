@@ -333,9 +333,8 @@ void prepare_material(const dn_params &p, const Layout &lay, const float *vp, co
         m.cjm[l] = (float)(1.0 - (peta[l] * 0.5));
         m.fs_bjm[l] = (float)(peta[l] / (1.0 + (peta[l] * 0.5)));
     }
-    m.dvis.assign(plane * L, 0.0f);
-    m.evis.assign(plane * L, 0.0f);
-    m.dipvis.assign(plane * L, 0.0f);
+    for (int l = 1; l <= L; ++l) m.peta[l] = peta[l];
+    for (auto *v : {&m.vmu, &m.vts, &m.vpi, &m.vtp, &m.vmuij, &m.vtsij}) v->assign(plane, 0.0f);
     for (int j = 1; j <= NY; ++j)
         for (int i = 1; i <= NX; ++i) {
             const size_t o = dev(j, i);
@@ -350,15 +349,12 @@ void prepare_material(const dn_params &p, const Layout &lay, const float *vp, co
                 pi = (float)((PI.at(j, i) * PI.at(j, i) * R.at(j, i)) / (1.0 + sumpi * tp));
             }
             const float muipjp = (float)(uipjp[(size_t)(j - 1) * NX + (i - 1)] / (1.0 + sumu * tsij));
-            m.muipjp[o] = (float)(muipjp * DT * (1.0 + L * tsij));
-            m.f[o] = (float)(mu * DT * (1.0 + L * ts));
-            m.g[o] = (float)(pi * DT * (1.0 + L * tp));
-            for (int l = 1; l <= L; ++l) {
-                const size_t ol = (size_t)(l - 1) * plane + o;
-                m.dipvis[ol] = muipjp * peta[l] * tsij;
-                m.dvis[ol] = mu * peta[l] * ts;
-                m.evis[ol] = pi * peta[l] * tp;
-            }
+            m.vmu[o] = mu;
+            m.vts[o] = ts;
+            m.vpi[o] = pi;
+            m.vtp[o] = tp;
+            m.vmuij[o] = muipjp;
+            m.vtsij[o] = tsij;
         }
     // free surface (row 1): relaxed moduli referenced to 1/TS
     const float ws_fs = (float)(2.0 * kPi * (1.0 / p.ts));

@@ -118,6 +118,8 @@ def load():
         "dn_get_pml": ([vp, C.c_int, fp], C.c_int),
         "dn_get_hc": ([vp, fp], C.c_int),
         "dn_set_kernel_sampling": ([vp, C.c_int], C.c_int),
+        "dn_set_tile": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
+        "dn_get_tile": ([vp, ip, ip, ip], C.c_int),
         "dn_get_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), ip], C.c_int),
         "dn_get_footprint": ([vp, C.POINTER(C.c_size_t)], C.c_int),
     }
@@ -263,6 +265,14 @@ class Context:
         out = np.zeros(7, np.float32)
         _check(load().dn_get_hc(self.h, _ptr(out)))
         return out
+
+    def set_tile(self, tbx, by, ty):
+        _check(load().dn_set_tile(self.h, tbx, by, ty))
+
+    def tile(self):
+        a, b, t = C.c_int(), C.c_int(), C.c_int()
+        _check(load().dn_get_tile(self.h, C.byref(a), C.byref(b), C.byref(t)))
+        return a.value, b.value, t.value
 
     def set_kernel_sampling(self, every):
         _check(load().dn_set_kernel_sampling(self.h, every))

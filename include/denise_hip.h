@@ -164,6 +164,13 @@ int dn_get_memvar(dn_ctx *ctx, int which, int l, float *out);
 int dn_get_pml(dn_ctx *ctx, int which, float *out);
 int dn_get_hc(dn_ctx *ctx, float *out7);
 
+/* Tile shape of the two fused sweeps: blocks of tbx x by threads, each
+ * thread marching ty rows.  Compiled shapes (tbx,by,ty): (64,4,1),
+ * (128,2,1), (256,1,1), (64,4,2), (128,2,4).  Results do not depend on the
+ * shape; the batch autotuner picks the fastest per grid. */
+int dn_set_tile(dn_ctx *ctx, int tbx, int by, int ty);
+int dn_get_tile(dn_ctx *ctx, int *tbx, int *by, int *ty);
+
 /* Timing of the last run: total device milliseconds (hipEvents around the
  * time loop on the context stream) and, when kernel sampling is on, the
  * average duration of the velocity and stress kernels. every = 0 turns

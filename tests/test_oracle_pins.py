@@ -1,0 +1,589 @@
+"""Pins of the CPU oracle against what the mathematics fixes (CPU only).
+
+The reference ships no tests, golden vectors or fixtures for this path, and
+its sources need mpi.h, so the oracle is pinned here by checks that are
+independent of its own formulas: closed forms, exact invariants of the
+discrete operators, and physics it must reproduce.  A plausible mistake (a
+wrong stencil offset, a swapped operand, a dropped term, a sign) breaks at
+least one of them.
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import orc
+import synth
+from cases import rel_l2
+from dnhip import Params
+
+orc.set_threads(8)
+
+
+def params(**kw):
+    base = dict(nx=64, ny=48, nt=200, dh=10.0, dt=1e-3, fdorder=8, maxrelerror=1, fw=0,
+                free_surf=0, seismo=0, ts=0.05)
+    base.update(kw)
+    return Params(**base)
+
+
+def homogeneous(nx, ny, vp=3000.0, vs=1732.0, rho=2000.0):
+    return (np.full((nx, ny), vp, np.float32), np.full((nx, ny), vs, np.float32),
+            np.full((nx, ny), rho, np.float32))
+
+
+# --------------------------------------------------------------------------
+# holbergcoeff
+# --------------------------------------------------------------------------
+def taylor_staggered(n):
+    """Coefficients c_1..c_n of the staggered first derivative
+    (1/h) sum c_m [f(x+(m-1/2)h) - f(x-(m-1/2)h)] exact to order 2n:
+    sum_m c_m (2m-1)^(2k-1) = delta_k1, k = 1..n (exact rationals)."""
+    A = [[Fraction(2 * m - 1) ** (2 * k - 1) for m in range(1, n + 1)] for k in range(1, n + 1)]
+    b = [Fraction(1 if k == 0 else 0) for k in range(n)]
+    # Gauss-Jordan in rationals
+    M = [row[:] + [b[r]] for r, row in enumerate(A)]
+    for col in range(n):
+        piv = next(r for r in range(col, n) if M[r][col] != 0)
+        M[col], M[piv] = M[piv], M[col]
+        pv = M[col][col]
+        M[col] = [x / pv for x in M[col]]
+        for r in range(n):
+            if r != col and M[r][col] != 0:
+                f = M[r][col]
+                M[r] = [x - f * y for x, y in zip(M[r], M[col])]
+    return [M[r][n] for r in range(n)]
+
+
+@pytest.mark.parametrize("fdorder", [2, 4, 6, 8, 10, 12])
+def test_taylor_row_is_the_staggered_taylor_stencil(fdorder):
+    n = fdorder // 2
+    exact = taylor_staggered(n)
+    hc = orc.holberg(fdorder, 0)
+    for m in range(1, n + 1):
+        assert hc[m] == np.float32(float(exact[m - 1])), (fdorder, m)
+    for m in range(n + 1, 7):
+        assert hc[m] == 0.0
+
+
+def phase_velocity_error(hc, n, khmax, samples=4000):
+    """max |k_num/k - 1| of the staggered operator, k_num h = 2 sum c_m sin((2m-1) kh/2)."""
+    kh = np.linspace(1e-6, khmax, samples)
+    kn = sum(float(hc[m]) * 2.0 * np.sin((2 * m - 1) * kh / 2) for m in range(1, n + 1)) / kh
+    return float(np.max(np.abs(kn - 1.0)))
+
+
+@pytest.mark.parametrize("mre,E", [(1, 0.001), (2, 0.005), (3, 0.01), (4, 0.03)])
+@pytest.mark.parametrize("fdorder", [2, 4, 6, 8, 10, 12])
+def test_holberg_rows_meet_their_error_bound(mre, E, fdorder):
+    """Every Holberg row keeps the relative phase-velocity error of the
+    staggered derivative below its design value E (0.1, 0.5, 1, 3 %) up to
+    kh = 2*pi/G, G = hc[0] grid points per shortest wavelength.  The published
+    coefficients are rounded, hence 2 % slack on E; a wrong digit in any
+    coefficient breaks the bound."""
+    n = fdorder // 2
+    hc = orc.holberg(fdorder, mre)
+    khmax = 2 * math.pi / float(hc[0])
+    err = phase_velocity_error(hc, n, khmax)
+    assert err <= 1.02 * E + 5e-5, (fdorder, mre, err)
+    # the bound is reached, not vacuous
+    assert err >= 0.9 * E, (fdorder, mre, err)
+
+
+def test_holberg_rejects_bad_order():
+    with pytest.raises(ValueError):
+        orc.holberg(7, 1)
+    with pytest.raises(ValueError):
+        orc.holberg(8, 5)
+
+
+# --------------------------------------------------------------------------
+# PML_pro (Komatitsch & Martin 2007 C-PML profiles)
+# --------------------------------------------------------------------------
+PML_NAMES = ["d_x", "K_x", "alpha_x", "a_x", "b_x", "d_x_half", "K_x_half", "alpha_x_half", "a_x_half",
+             "b_x_half", "d_y", "K_y", "alpha_y", "a_y", "b_y", "d_y_half", "K_y_half", "alpha_y_half",
+             "a_y_half", "b_y_half"]
+
+
+def cpml_closed_form(n, fw, dh, dt, damping, fpml, npower, kmax, clamp_alpha):
+    """d(x) = d0 (x/L)^N, K = 1 + (kmax-1)(x/L)^N, alpha = alpha_max (1-x/L),
+    d0 = -(N+1) V log(R) / (2L), R = 1e-3, alpha_max = pi fpml,
+    b = exp(-(d/K + alpha) dt), a = d (b-1) / (K (d + K alpha)) -- evaluated
+    in float64 at the grid points and half points of both bands."""
+    L = fw * dh
+    d0 = -(npower + 1) * damping * math.log(0.001) / (2 * L)
+    amax = math.pi * fpml
+    out = {k: np.zeros(2 * fw) for k in ("d", "K", "al", "dh", "Kh", "alh")}
+    for h in range(1, 2 * fw + 1):
+        if h <= fw:
+            x = (h - 1) * dh
+            absc, absch = L - x, L - (x + dh / 2)
+        else:
+            i = n - 2 * fw + h
+            x = (i - 1) * dh
+            absc, absch = x - ((n - 1) * dh - L), x + dh / 2 - ((n - 1) * dh - L)
+        for key, a in (("", absc), ("h", absch)):
+            if a >= 0:
+                xn = a / L
+                out["d" + key][h - 1] = d0 * xn ** npower
+                out["K" + key][h - 1] = 1 + (kmax - 1) * xn ** npower
+                out["al" + key][h - 1] = amax * (1 - xn)
+            else:
+                out["K" + key][h - 1] = 1.0
+    if clamp_alpha:
+        out["al"] = np.maximum(out["al"], 0)
+        out["alh"] = np.maximum(out["alh"], 0)
+    for key in ("", "h"):
+        d, K, al = out["d" + key], out["K" + key], out["al" + key]
+        b = np.exp(-(d / K + al) * dt)
+        # the damping gate: a is set only where the integer part of d is non-zero
+        gate = np.abs(np.trunc(d)) > 0
+        a = np.where(gate, d * (b - 1) / (K * (d + K * al) + (~gate)), 0.0)
+        out["b" + key], out["a" + key] = b, a
+    return out
+
+
+@pytest.mark.parametrize("fw,kmax", [(20, 1.0), (10, 2.0), (7, 1.5)])
+def test_cpml_profiles_closed_form(fw, kmax):
+    p = params(nx=90, ny=70, fw=fw, damping=1500.0, fpml=10.0, npower=4.0, k_max_pml=kmax, dt=1e-3)
+    vp, vs, rho = homogeneous(p.nx, p.ny)
+    o = orc.Oracle(p, vp, vs, rho, prec="f64")
+    for axis, n, clamp in (("x", p.nx, True), ("y", p.ny, False)):
+        ref = cpml_closed_form(n, fw, p.dh, p.dt, p.damping, p.fpml, p.npower, kmax, clamp)
+        base = 0 if axis == "x" else 10
+        mapping = {"d": 0, "K": 1, "al": 2, "a": 3, "b": 4, "dh": 5, "Kh": 6, "alh": 7, "ah": 8, "bh": 9}
+        for key, k in mapping.items():
+            got = o.pml(base + k)
+            # float32 inputs (dh, dt, damping...) enter the oracle's double math
+            assert np.allclose(got, ref[key], rtol=2e-6, atol=1e-9), (axis, key)
+        # shape invariants: damping grows toward the outer edge, b in (0, 1], a <= 0
+        d = o.pml(base + 0)
+        assert np.all(np.diff(d[:fw]) <= 0) and np.all(np.diff(d[fw:]) >= 0)
+        b = o.pml(base + 4)
+        assert np.all((b > 0) & (b <= 1))
+        assert np.all(o.pml(base + 3) <= 0)
+
+
+def test_cpml_alpha_clamped_only_in_x():
+    p = params(nx=60, ny=60, fw=10, fpml=10.0)
+    vp, vs, rho = homogeneous(p.nx, p.ny)
+    o = orc.Oracle(p, vp, vs, rho, prec="f64")
+    assert o.pml(7).min() >= 0.0          # alpha_x_half clamped
+    assert o.pml(17)[-1] < 0.0            # alpha_y_half at the outermost half point is not
+
+
+# --------------------------------------------------------------------------
+# update_v / update_s stencils: exact on polynomials (Taylor coefficients)
+# --------------------------------------------------------------------------
+def _padded(o):
+    nd = o.nd
+    return np.zeros((o.p.ny + 2 * nd, o.p.nx + 2 * nd))
+
+
+def _coords(o):
+    nd = o.nd
+    j = np.arange(-nd + 1, o.p.ny + nd + 1)[:, None].astype(float)
+    i = np.arange(-nd + 1, o.p.nx + nd + 1)[None, :].astype(float)
+    return j, i
+
+
+@pytest.mark.parametrize("fdorder", [2, 4, 6, 8, 10, 12])
+def test_velocity_stencils_exact_on_polynomials(fdorder):
+    """sxx at (i, j), sxy at (i+1/2, j+1/2), syy at (i, j); vx at (i+1/2, j),
+    vy at (i, j+1/2).  With polynomials of degree 2n-1 the Taylor stencils are
+    exact: vx = DT/rho (d_x sxx + d_y sxy), vy = DT/rho (d_x sxy + d_y syy)."""
+    n = fdorder // 2
+    p = params(nx=40, ny=36, nt=5, fdorder=fdorder, maxrelerror=0, dh=1.0, dt=1e-3)
+    vp, vs, rho = homogeneous(p.nx, p.ny, vp=1.0, vs=0.5, rho=2.0)
+    o = orc.Oracle(p, vp, vs, rho, prec="f64")
+    j, i = _coords(o)
+    deg = 2 * n - 1
+    c = [0.3, -0.2, 0.05, 0.01, -0.002, 3e-4, -2e-5, 1e-6, -3e-8, 1e-9, -2e-11, 1e-12][: deg + 1]
+    P = np.polynomial.Polynomial(c)
+    dP = P.deriv()
+    s = 0.07  # keep the polynomial values moderate on the grid
+    f = lambda x: P(s * x)
+    df = lambda x: s * dP(s * x)
+    x_int, y_int = i, j
+    o.set_field(orc.ORC_SXX, f(x_int) + 0 * y_int)
+    o.set_field(orc.ORC_SYY, f(2.0 * y_int) + 0 * x_int)
+    o.set_field(orc.ORC_SXY, f(x_int + 0.5) + f(1.5 * (y_int + 0.5)))
+    o.step(1)
+    vx = o.field(orc.ORC_VX)
+    vy = o.field(orc.ORC_VY)
+    nd = o.nd
+    sl = (slice(nd - 1 + n + 1, nd - 1 + p.ny - n), slice(nd - 1 + n + 1, nd - 1 + p.nx - n))
+    dt_rho = float(np.float32(p.dt)) / 2.0  # DT enters as the float32 parameter
+    exp_vx = dt_rho * (df(x_int + 0.5) + 1.5 * df(1.5 * (y_int + 0.5 - 0.5)))
+    exp_vy = dt_rho * (df(x_int + 0.5 - 0.5) * 1.0 + 2.0 * df(2.0 * (y_int + 0.5)))
+    # the coefficient table is float32 (as in the reference), so the stencil is
+    # exact up to the rounding of its coefficients (1e-8..1e-7 relative after cancellation)
+    assert np.allclose(vx[sl], (exp_vx + 0 * vx)[sl], rtol=1e-6, atol=0)
+    assert np.allclose(vy[sl], (exp_vy + 0 * vy)[sl], rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("fdorder", [2, 6, 8, 12])
+def test_stress_stencils_exact_on_polynomials(fdorder):
+    """vx at (i+1/2, j), vy at (i, j+1/2) polynomial; one stress sweep gives
+    sxx = DT[(lam+2mu) d_x vx + lam d_y vy], syy = DT[lam d_x vx + (lam+2mu) d_y vy],
+    sxy = DT mu (d_y vx + d_x vy) at (i+1/2, j+1/2)."""
+    n = fdorder // 2
+    p = params(nx=40, ny=36, nt=5, fdorder=fdorder, maxrelerror=0, dh=1.0, dt=1e-3)
+    vpv, vsv, rhov = 2.0, 1.0, 3.0
+    vp, vs, rho = homogeneous(p.nx, p.ny, vp=vpv, vs=vsv, rho=rhov)
+    mu, lam = rhov * vsv ** 2, rhov * (vpv ** 2 - 2 * vsv ** 2)
+    o = orc.Oracle(p, vp, vs, rho, prec="f64")
+    j, i = _coords(o)
+    deg = 2 * n - 1
+    c = [0.1, 0.2, -0.05, 0.01, -0.002, 3e-4, -2e-5, 1e-6, -3e-8, 1e-9, -2e-11, 1e-12][: deg + 1]
+    P = np.polynomial.Polynomial(c)
+    dP = P.deriv()
+    s = 0.06
+    f = lambda x: P(s * x)
+    df = lambda x: s * dP(s * x)
+    o.set_field(orc.ORC_VX, f(i + 0.5) + f(0.8 * j))
+    o.set_field(orc.ORC_VY, f(1.3 * i) + f(j + 0.5))
+    o.step(1)
+    nd = o.nd
+    sl = (slice(nd - 1 + n + 1, nd - 1 + p.ny - n), slice(nd - 1 + n + 1, nd - 1 + p.nx - n))
+    dvx_dx = df(i + 0.0 + 0 * j)            # at integer points
+    dvy_dy = df(j + 0.0 + 0 * i)
+    dvx_dy = 0.8 * df(0.8 * (j + 0.5)) + 0 * i   # at (i+1/2, j+1/2)
+    dvy_dx = 1.3 * df(1.3 * (i + 0.5)) + 0 * j
+    dt = float(np.float32(p.dt))
+    for fid, ref in ((orc.ORC_SXX, dt * ((lam + 2 * mu) * dvx_dx + lam * dvy_dy)),
+                     (orc.ORC_SYY, dt * (lam * dvx_dx + (lam + 2 * mu) * dvy_dy)),
+                     (orc.ORC_SXY, dt * mu * (dvx_dy + dvy_dx))):
+        assert np.allclose(o.field(fid)[sl], ref[sl], rtol=1e-7, atol=1e-10 * np.abs(ref[sl]).max()), fid
+
+
+# --------------------------------------------------------------------------
+# exact symmetries of the discrete scheme (FP32, bit for bit)
+# --------------------------------------------------------------------------
+def explosive_at(nx, ny, i0, j0, nt, dt, fc=25.0, dh=10.0):
+    src = np.zeros((8, 1), np.float32)
+    src[0, 0], src[1, 0], src[4, 0], src[5, 0], src[7, 0] = i0, j0, fc, 1.0, 1
+    sig = synth.ricker(nt, dt, fc, amp=1e3)[None, :]
+    return src, sig
+
+
+def test_mirror_symmetry_about_the_source():
+    """Homogeneous medium, explosive source on the centre column: the wavefield
+    is mirror symmetric in x exactly (vx, sxy antisymmetric about the half
+    points, vy, sxx, syy symmetric), as long as the edges have not been
+    reached (stencil light cone: 2 FDORDER/2 cells per step)."""
+    nx, ny, nt = 161, 101, 16
+    p = params(nx=nx, ny=ny, nt=nt, dt=1.5e-3)
+    vp, vs, rho = homogeneous(nx, ny)
+    i0, j0 = (nx + 1) // 2, 45
+    o = orc.Oracle(p, vp, vs, rho, prec="f32")
+    o.set_sources(*explosive_at(nx, ny, i0, j0, nt, p.dt))
+    o.forward()
+    nd = o.nd
+    col = lambda i: i + nd - 1
+    for fid, sign, half in ((orc.ORC_VX, -1, True), (orc.ORC_SXY, -1, True), (orc.ORC_VY, 1, False),
+                            (orc.ORC_SXX, 1, False), (orc.ORC_SYY, 1, False)):
+        a = o.field(fid)
+        assert np.abs(a).max() > 0
+        for i in range(1, nx + 1):
+            im = (2 * i0 - 1 - i) if half else (2 * i0 - i)
+            if 1 <= im <= nx:
+                assert np.array_equal(a[:, col(i)], sign * a[:, col(im)]), (fid, i)
+
+
+def test_transpose_symmetry_square_grid():
+    """Square homogeneous grid, source on the diagonal, no free surface or
+    CPML: swapping x and y maps vx<->vy, sxx<->syy, sxy<->sxy exactly."""
+    n, nt = 101, 14
+    p = params(nx=n, ny=n, nt=nt, dt=1.5e-3)
+    vp, vs, rho = homogeneous(n, n)
+    o = orc.Oracle(p, vp, vs, rho, prec="f32")
+    o.set_sources(*explosive_at(n, n, 51, 51, nt, p.dt))
+    o.forward()
+    assert np.array_equal(o.field(orc.ORC_VX), o.field(orc.ORC_VY).T)
+    assert np.array_equal(o.field(orc.ORC_SXX), o.field(orc.ORC_SYY).T)
+    assert np.array_equal(o.field(orc.ORC_SXY), o.field(orc.ORC_SXY).T)
+
+
+# --------------------------------------------------------------------------
+# discrete reciprocity (FP64): rho_x(B) vx_B[force x at A] = rho_x(A) vx_A[force x at B]
+# --------------------------------------------------------------------------
+def force_run(p, model, at, rec, ftype, sig, prec="f64"):
+    o = orc.Oracle(p, *model, prec=prec)
+    src = np.zeros((8, 1), np.float32)
+    src[0, 0], src[1, 0], src[4, 0], src[5, 0], src[7, 0] = at[0], at[1], 20.0, 1.0, ftype
+    o.set_sources(src, sig)
+    o.set_receivers(np.array([[rec[0]], [rec[1]]], np.int32))
+    o.forward()
+    return o
+
+
+@pytest.mark.parametrize("fdorder", [4, 8])
+def test_reciprocity_heterogeneous(fdorder):
+    """The velocity-stress operator is skew-adjoint in the energy inner
+    product (the backward stencils are the negative transposes of the forward
+    ones, the stiffness is symmetric), so force-to-velocity Green's functions
+    are reciprocal with the density weights of the velocity points.  Rigid
+    edges, heterogeneous model, FP64."""
+    nx, ny, nt = 70, 60, 260
+    p = params(nx=nx, ny=ny, nt=nt, fdorder=fdorder, dt=8e-4, seismo=1)
+    vp, vs, rho = synth.layered_model(nx, ny, 10.0, seed=5, sigma=3.0)
+    model = (vp, vs, rho)
+    sig = synth.ricker(nt, p.dt, 20.0, amp=1.0)[None, :]
+    A, B = (20, 18), (47, 40)
+    o = orc.Oracle(p, *model, prec="f64")
+    nd = o.nd
+    rip = o.field(orc.ORC_RIP)
+    rjp = o.field(orc.ORC_RJP)
+    rx = lambda c: 1.0 / rip[c[1] + nd - 1, c[0] + nd - 1]
+    ry = lambda c: 1.0 / rjp[c[1] + nd - 1, c[0] + nd - 1]
+    # x force at A -> vx at B  vs  x force at B -> vx at A
+    ab = force_run(p, model, A, B, 2, sig).seismogram(0)[0]
+    ba = force_run(p, model, B, A, 2, sig).seismogram(0)[0]
+    assert np.abs(ab).max() > 0
+    assert rel_l2(rx(B) * ab, rx(A) * ba) < 1e-10
+    # y force at A -> vx at B  vs  x force at B -> vy at A
+    ab = force_run(p, model, A, B, 3, sig).seismogram(0)[0]
+    ba = force_run(p, model, B, A, 2, sig).seismogram(1)[0]
+    assert rel_l2(rx(B) * ab, ry(A) * ba) < 1e-10
+
+
+# --------------------------------------------------------------------------
+# free surface
+# --------------------------------------------------------------------------
+def test_free_surface_invariants():
+    """After every step: syy = 0 on row 1 and the ghost rows are the odd
+    images syy[1-m] = -syy[1+m], sxy[1-m] = -sxy[m]."""
+    nx, ny, nt = 60, 40, 60
+    p = params(nx=nx, ny=ny, nt=nt, free_surf=1, fw=8, dt=1e-3)
+    vp, vs, rho = synth.layered_model(nx, ny, 10.0, seed=2, sigma=3.0)
+    o = orc.Oracle(p, vp, vs, rho, prec="f32")
+    o.set_sources(*explosive_at(nx, ny, 30, 3, nt, p.dt))
+    nd = o.nd
+    r = lambda jj: jj + nd - 1
+    for nt_ in range(1, 41):
+        o.step(nt_)
+        syy, sxy = o.field(orc.ORC_SYY), o.field(orc.ORC_SXY)
+        assert np.all(syy[r(1), nd:nd + nx] == 0)
+        for m in range(1, p.fdorder // 2 + 1):
+            assert np.array_equal(syy[r(1 - m), nd:nd + nx], -syy[r(1 + m), nd:nd + nx])
+            assert np.array_equal(sxy[r(1 - m), nd:nd + nx], -sxy[r(m), nd:nd + nx])
+    assert np.abs(o.field(orc.ORC_SXX)).max() > 0
+
+
+def test_rayleigh_wave_speed():
+    """Vertical force at the free surface of a Poisson solid (vs = vp/sqrt 3):
+    the surface wave moves at c_R = 0.9194 vs (Lamb's problem).  Measured from
+    the arrival of the peak |vy| at two surface receivers."""
+    nx, ny, dh = 320, 120, 5.0
+    vpv = 3000.0
+    vsv = vpv / math.sqrt(3.0)
+    dt = 6e-4
+    nt = 1250
+    p = params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, free_surf=1, fw=20, seismo=1, fpml=15.0,
+               damping=3000.0)
+    vp, vs, rho = homogeneous(nx, ny, vp=vpv, vs=vsv, rho=2000.0)
+    o = orc.Oracle(p, vp, vs, rho, prec="f64")
+    src = np.zeros((8, 1), np.float32)
+    src[0, 0], src[1, 0], src[4, 0], src[5, 0], src[7, 0] = 60, 1, 15.0, 1.0, 3
+    o.set_sources(src, synth.ricker(nt, dt, 15.0, amp=1.0)[None, :])
+    r1, r2 = 110, 230
+    o.set_receivers(np.array([[r1, r2], [1, 1]], np.int32))
+    o.forward()
+    s = np.abs(o.seismogram(1))
+
+    def peak_time(a):
+        k = int(np.argmax(a))
+        y0, y1, y2 = a[k - 1], a[k], a[k + 1]
+        return (k + 0.5 * (y0 - y2) / (y0 - 2 * y1 + y2)) * dt
+
+    c = (r2 - r1) * dh / (peak_time(s[1]) - peak_time(s[0]))
+    cr = 0.9194 * vsv
+    assert abs(c / cr - 1.0) < 0.02, (c, cr)
+
+
+# --------------------------------------------------------------------------
+# CPML absorbs; the scheme is stable
+# --------------------------------------------------------------------------
+def interior_energy(o, margin):
+    nd = o.nd
+    sl = (slice(nd - 1 + margin + 1, nd - 1 + o.p.ny - margin), slice(nd - 1 + margin + 1, nd - 1 + o.p.nx - margin))
+    return sum(float(np.sum(o.field(f)[sl].astype(np.float64) ** 2)) for f in (orc.ORC_VX, orc.ORC_VY))
+
+
+def test_cpml_absorbs_outgoing_waves():
+    """A pulse leaving a homogeneous box: with CPML on all sides the kinetic
+    energy left in the interior after the wave has crossed the boundary is a
+    small fraction of what rigid edges keep."""
+    nx = ny = 120
+    nt = 700
+    res = {}
+    for fw in (0, 20):
+        p = params(nx=nx, ny=ny, nt=nt, fw=fw, dt=1e-3, fpml=25.0, damping=3000.0)
+        vp, vs, rho = homogeneous(nx, ny)
+        o = orc.Oracle(p, vp, vs, rho, prec="f64")
+        o.set_sources(*explosive_at(nx, ny, 60, 60, nt, p.dt, fc=25.0))
+        o.forward(120)
+        e_mid = interior_energy(o, 20)
+        o.forward(nt)
+        res[fw] = (e_mid, interior_energy(o, 20))
+    e_rigid, e_pml = res[0][1], res[20][1]
+    assert res[20][0] > 0
+    assert e_pml < 0.02 * e_rigid, (e_pml, e_rigid)
+    assert e_pml < 0.02 * res[20][0]
+
+
+def test_long_run_stays_bounded():
+    nx, ny, nt = 50, 40, 4000
+    p = params(nx=nx, ny=ny, nt=nt, dt=1.2e-3, fw=0)
+    vp, vs, rho = synth.layered_model(nx, ny, 10.0, seed=9, sigma=2.0)
+    o = orc.Oracle(p, vp, vs, rho, prec="f64")
+    o.set_sources(*explosive_at(nx, ny, 25, 20, nt, p.dt))
+    o.forward(300)
+    e1 = interior_energy(o, 0)
+    o.forward(nt)
+    e2 = interior_energy(o, 0)
+    assert np.isfinite(e2) and e2 < 10 * e1
+
+
+# --------------------------------------------------------------------------
+# sources, receivers, wavelet, residuals: known answers
+# --------------------------------------------------------------------------
+def test_ricker_closed_form():
+    nt, dt, fc, ts = 400, 1e-3, 12.0, 0.02
+    got = orc.ricker(nt, dt, ts, fc, 2.5)
+    t = np.arange(1, nt + 1) * np.float32(dt)
+    tau = math.pi * (t - 1.5 / fc - ts) / (1.5 / fc)
+    ref = (1 - 4 * tau ** 2) * np.exp(-2 * tau ** 2) * 2.5
+    assert np.allclose(got, ref, rtol=0, atol=5e-6)
+    k = int(np.argmax(got))
+    assert abs((k + 1) * dt - (1.5 / fc + ts)) <= dt
+    # the product-side generator follows the same wavelet.c arithmetic
+    assert np.array_equal(synth.ricker(nt, dt, fc, tshift=ts, amp=2.5), got)
+
+
+def test_explosive_source_injects_the_time_derivative():
+    """psource: sxx and syy gain (s[nt+1]-s[nt-1])/DT at the source cell
+    (one-sided at nt=1 and nt=NT)."""
+    nx, ny, nt = 30, 30, 6
+    p = params(nx=nx, ny=ny, nt=nt, dt=1e-3, seismo=0)
+    vp, vs, rho = homogeneous(nx, ny)
+    o = orc.Oracle(p, vp, vs, rho, prec="f32")
+    sig = np.array([[0.5, 2.0, -1.0, 4.0, 8.0, 16.0]], np.float32)
+    src = np.zeros((8, 1), np.float32)
+    src[0, 0], src[1, 0], src[4, 0], src[7, 0] = 15, 15, 10.0, 1
+    o.set_sources(src, sig)
+    nd = o.nd
+    o.step(1)
+    assert o.field(orc.ORC_SXX)[15 + nd - 1, 15 + nd - 1] == np.float32(2.0) / np.float32(1e-3)
+    o.zero()
+    o.step(3)
+    assert o.field(orc.ORC_SYY)[15 + nd - 1, 15 + nd - 1] == (np.float32(4.0) - np.float32(2.0)) / np.float32(1e-3)
+    o.zero()
+    o.step(6)
+    assert o.field(orc.ORC_SXX)[15 + nd - 1, 15 + nd - 1] == -np.float32(8.0) / np.float32(1e-3)
+
+
+def test_receivers_sample_the_fields():
+    nx, ny, nt = 40, 30, 8
+    p = params(nx=nx, ny=ny, nt=nt, seismo=4, dt=1e-3)
+    vp, vs, rho = synth.layered_model(nx, ny, 10.0, seed=4, sigma=2.0)
+    o = orc.Oracle(p, vp, vs, rho, prec="f32")
+    o.set_sources(*explosive_at(nx, ny, 20, 15, nt, p.dt))
+    rec = np.array([[18, 21, 25], [15, 12, 17]], np.int32)
+    o.set_receivers(rec)
+    nd = o.nd
+    for nt_ in range(1, nt + 1):
+        o.step(nt_)
+        vx, vy = o.field(orc.ORC_VX), o.field(orc.ORC_VY)
+        sxx, syy = o.field(orc.ORC_SXX), o.field(orc.ORC_SYY)
+        for t in range(3):
+            jj, ii = rec[1, t] + nd - 1, rec[0, t] + nd - 1
+            assert o.seismogram(0)[t, nt_ - 1] == vx[jj, ii]
+            assert o.seismogram(1)[t, nt_ - 1] == vy[jj, ii]
+            assert o.seismogram(2)[t, nt_ - 1] == -sxx[jj, ii] - syy[jj, ii]
+
+
+def test_l2_residual_known_answer():
+    """Time-integrated, time-reversed residual: syn = 1, obs = 0 except that
+    the first observed sample is replaced by the synthetic one, so the
+    integral after k samples is DT*(k-1), stored at index NT-k."""
+    nx, ny, nt = 20, 20, 50
+    p = params(nx=nx, ny=ny, nt=nt, dt=2e-3, seismo=1, mode=1)
+    vp, vs, rho = homogeneous(nx, ny)
+    o = orc.Oracle(p, vp, vs, rho, prec="f64")
+    o.set_receivers(np.array([[5, 9], [5, 5]], np.int32))
+    # set synthetic seismograms through a zero run, then residual against obs
+    o.set_sources(np.zeros((8, 0), np.float32), np.zeros((0, nt), np.float32))
+    o.forward()
+    syn = o.seismogram(0)
+    obs = (syn - 1.0).astype(np.float32)   # residual 1 on every sample but the first
+    l2 = o.residuals(obs, obs, None, 1)
+    d = o.adjoint_sources(0)[0]
+    k = np.arange(1, nt + 1)
+    expect = np.float32(2e-3) * (k - 1)
+    assert np.allclose(d[nt - k], expect, rtol=1e-6, atol=1e-9)
+    assert l2 == pytest.approx(2 * 2 * 0.5 * float(np.sum(expect.astype(np.float64) ** 2)), rel=1e-6)
+
+
+# --------------------------------------------------------------------------
+# adjoint gradient vs finite differences of the misfit
+# --------------------------------------------------------------------------
+def test_gradient_matches_finite_differences():
+    """The vp gradient assembled from the adjoint correlation
+    (psv.c:313-387, ass_gradPSV.c) against central differences of the L2
+    misfit of calc_res.c at 12 seeded cells.  DENISE's gradient is a
+    continuous-adjoint discretisation with an unnormalised scale, so the test
+    asks for agreement up to one common positive factor: the finite
+    difference and adjoint values must be strongly correlated and their ratio
+    nearly constant across cells."""
+    nx, ny, nt, dh = 48, 40, 420, 10.0
+    p = params(nx=nx, ny=ny, nt=nt, dh=dh, dt=1e-3, fw=10, free_surf=1, seismo=1, mode=1, dtinv=1,
+               fpml=15.0, damping=2500.0)
+    vp0, vs0, rho0 = synth.layered_model(nx, ny, dh, seed=3, sigma=4.0, pert=0.02)
+    vpt = vp0.copy()
+    vpt[18:30, 10:20] *= 1.05
+    src = np.zeros((8, 1), np.float32)
+    src[0, 0], src[1, 0], src[4, 0], src[5, 0], src[7, 0] = 24, 2, 18.0, 1.0, 1
+    sig = synth.ricker(nt, p.dt, 18.0, amp=1e3)[None, :]
+    rec = np.array([[k for k in range(6, 44, 2)], [2] * 19], np.int32)
+
+    def misfit_and_run(vp, grad=False):
+        o = orc.Oracle(p, vp, vs0, rho0, prec="f64")
+        o.set_sources(src, sig)
+        o.set_receivers(rec)
+        o.forward()
+        l2 = o.residuals(obs[0], obs[1], None, 1)
+        if grad:
+            o.adjoint()
+            o.assemble()
+            return l2, o.gradient(3)
+        return l2, None
+
+    t = orc.Oracle(p, vpt, vs0, rho0, prec="f64")
+    t.set_sources(src, sig)
+    t.set_receivers(rec)
+    t.forward()
+    obs = (t.seismogram(0).astype(np.float32), t.seismogram(1).astype(np.float32))
+    _, g = misfit_and_run(vp0, grad=True)
+    rng = np.random.default_rng(12)
+    cells = [(int(a), int(b)) for a, b in zip(rng.integers(14, 34, 12), rng.integers(9, 26, 12))]
+    fd, ad = [], []
+    for (ci, cj) in cells:
+        eps = 0.5  # m/s
+        vpp, vpm = vp0.copy(), vp0.copy()
+        vpp[ci, cj] += eps
+        vpm[ci, cj] -= eps
+        lp, _ = misfit_and_run(vpp)
+        lm, _ = misfit_and_run(vpm)
+        fd.append((lp - lm) / (2 * eps))
+        ad.append(float(g[ci, cj]))
+    fd, ad = np.array(fd), np.array(ad)
+    corr = float(np.corrcoef(fd, ad)[0, 1])
+    scale = float(fd @ ad / (ad @ ad))
+    resid = float(np.linalg.norm(fd - scale * ad) / np.linalg.norm(fd))
+    print("FD vs adjoint: corr", corr, "scale", scale, "relative residual", resid)
+    assert corr > 0.95
+    assert scale > 0
+    assert resid < 0.25
