@@ -60,13 +60,18 @@ def workload(L, nt=6000, nrec=800, seed=11):
     return p, dict(vp=vp, vs=vs, rho=rho, taus=taus, taup=taup), src, sig, rec
 
 
-def run_gpu(L, steps, warmup, sample_every=20):
+def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tile=None):
     import dnhip
-    p, m, src, sig, rec = workload(L)
+    p, m, src, sig, rec = workload(L, nt=nt)
     ctx = dnhip.Context(p)
     ctx.set_model(m["vp"], m["vs"], m["rho"], m["taus"], m["taup"])
     ctx.set_sources(src, sig)
     ctx.set_receivers(rec)
+    if tile:
+        ctx.set_tile(*tile)
+    elif tune_steps > 0:
+        ctx.autotune(tune_steps)
+    tile = ctx.tile()
     for _ in range(warmup):
         ctx.forward()
     ctx.set_kernel_sampling(sample_every)
@@ -97,6 +102,7 @@ def run_gpu(L, steps, warmup, sample_every=20):
         "step_GBs": (vel_b + str_b) * cells * p.nt * steps / total / 1e9,
         "stress_bytes_per_cell": str_b, "vel_bytes_per_cell": vel_b,
         "footprint_GB": ctx.footprint() / 1e9,
+        "tile": list(tile),
     }
     ctx.close()
     return out
@@ -117,25 +123,53 @@ def cpu_baseline(L, nsteps, threads):
     return p.nx * p.ny * nsteps / dt / 1e9, dt
 
 
+def calibrate_copy(mib, reps):
+    """A plain device copy of known size, for scaling the HBM byte counters
+    (FETCH_SIZE / WRITE_SIZE) of a profiler run."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-steps", type=int, default=150)
+    ap.add_argument("--cpu-steps", type=int, default=800)
+    ap.add_argument("--cpu1-steps", type=int, default=40, help="steps of the single-thread CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-elastic", action="store_true")
+    ap.add_argument("--no-visco", action="store_true", help="profiling: elastic shot only")
+    ap.add_argument("--nt", type=int, default=6000, help="profiling: shorter shots")
+    ap.add_argument("--tune-steps", type=int, default=40)
+    ap.add_argument("--tile", default="", help="profiling: fixed tile shape tbx,by,ty (no autotune)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
+                    help="profiling: run only a device copy of this size")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy, 3)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU shot sharding is a "
                                    "design note (one shot per GPU), run with --gpus 1"}))
         sys.exit(2)
+    tile = tuple(int(v) for v in args.tile.split(",")) if args.tile else None
     L = 2
-    res = run_gpu(L, args.steps, args.warmup)
+    if args.no_visco:
+        elastic = run_gpu(0, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tile=tile)
+        print(json.dumps({"elastic": elastic}))
+        return
+    res = run_gpu(L, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tile=tile)
     print(f"# visco: {res}", file=sys.stderr, flush=True)
     elastic = None
     if not args.no_elastic:
-        elastic = run_gpu(0, 1, 1)
+        elastic = run_gpu(0, 1, 1, nt=args.nt, tune_steps=args.tune_steps, tile=tile)
         print(f"# elastic: {elastic}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
@@ -143,7 +177,11 @@ def main():
         gcps, secs = cpu_baseline(L, args.cpu_steps, threads)
         cpu = {"value": gcps, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
                "sample": f"first {args.cpu_steps} of 6000 time steps of the same 4000x1200 "
-                         f"viscoelastic shot, FP32 oracle, {secs:.1f} s wall"}
+                         f"viscoelastic shot, FP32 oracle (OpenMP), {secs:.1f} s wall"}
+        if args.cpu1_steps > 0:
+            g1, s1 = cpu_baseline(L, args.cpu1_steps, 1)
+            cpu["single_thread"] = {"value": g1, "cores": 1,
+                                    "sample": f"first {args.cpu1_steps} time steps, {s1:.1f} s wall"}
     achieved = res["s_kernel_GBs"]
     line = {
         "metric": METRIC,
@@ -158,10 +196,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded smooth-random layered model, Ricker source)",
-        "config": {"workload": "single-shot viscoelastic P/SV 4000x1200, DH=5 m, FD8 Holberg, "
-                               "DT=0.4 ms, NT=6000, L=2 (Q=50), CPML 20 + free surface, "
+        "config": {"workload": f"single-shot viscoelastic P/SV 4000x1200, DH=5 m, FD8 Holberg, "
+                               f"DT=0.4 ms, NT={args.nt}, L=2 (Q=50), CPML 20 + free surface, "
                                "Ricker 15 Hz, 800 receivers",
-                   "nx": 4000, "ny": 1200, "nt": 6000, "L": 2, "fdorder": 8,
+                   "nx": 4000, "ny": 1200, "nt": args.nt, "tile": res["tile"], "L": 2, "fdorder": 8,
                    "parallelism": "single GPU, whole-grid domain"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
@@ -170,7 +208,10 @@ def main():
                      "avg_launch_ms": res["s_kernel_ms"]},
         "cpu_baseline": cpu,
         "detail": {"visco": res, "elastic": elastic,
-                   "elastic_step_frac_of_hbm": (elastic["step_GBs"] / HBM_PEAK_GBS) if elastic else None},
+                   "elastic_step_frac_of_hbm": (elastic["step_GBs"] / HBM_PEAK_GBS) if elastic else None,
+                   "elastic_kernels_frac_of_hbm": (
+                       80 * elastic["cells_per_step"] / ((elastic["v_kernel_ms"] + elastic["s_kernel_ms"]) * 1e-3)
+                       / 1e9 / HBM_PEAK_GBS) if elastic and elastic["s_kernel_ms"] else None},
     }
     print(json.dumps(line))
 

@@ -694,6 +694,41 @@ int dn_get_tile(dn_ctx *c, int *tbx, int *by, int *ty)
     return DN_OK;
 }
 
+int dn_autotune_tile(dn_ctx *c, int nsteps, int *tbx, int *by, int *ty)
+{
+    if (!c || nsteps < 1) return fail(DN_ERR_ARG, "autotune needs a context and nsteps >= 1");
+    if (!c->model_set) return fail(DN_ERR_STATE, "dn_set_model must precede dn_autotune_tile");
+    Tile shapes[32];
+    const int n = tile_list(shapes, 32);
+    const int keep_sampling = c->sample_every;
+    c->sample_every = 0;
+    Tile best = c->tile;
+    double best_ms = -1.0;
+    for (int k = 0; k < n && k < 32; ++k) {
+        int rc = dn_set_tile(c, shapes[k].tbx, shapes[k].by, shapes[k].ty);
+        if (!rc) rc = dn_run_forward(c, nsteps);  // warm-up (code load, caches)
+        if (!rc) rc = dn_run_forward(c, nsteps);
+        if (rc) {
+            c->sample_every = keep_sampling;
+            return rc;
+        }
+        if (best_ms < 0.0 || c->run_ms < best_ms) {
+            best_ms = c->run_ms;
+            best = shapes[k];
+        }
+    }
+    c->sample_every = keep_sampling;
+    int rc = dn_set_tile(c, best.tbx, best.by, best.ty);
+    if (rc) return rc;
+    rc = zero_state(c);
+    if (rc) return rc;
+    c->fwd_complete = false;
+    if (tbx) *tbx = best.tbx;
+    if (by) *by = best.by;
+    if (ty) *ty = best.ty;
+    return DN_OK;
+}
+
 int dn_set_kernel_sampling(dn_ctx *c, int every)
 {
     if (!c || every < 0) return fail(DN_ERR_ARG, "bad sampling interval");

@@ -166,5 +166,7 @@ void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const fl
 void launch_residual(const DevAcq &q, int comp, const float *obs, float dt, void *stream);
 void launch_l2(const DevAcq &q, int comp, void *stream);
 bool tile_supported(const Tile &t);
+// compiled tile shapes (returns their count; fills at most max entries)
+int tile_list(Tile *out, int max);
 
 }  // namespace dn

@@ -152,6 +152,21 @@ __global__ void k_l2(DevAcq q, int comp)
 DN_TILES(X)
 #undef X
 
+int tile_list(Tile *out, int max)
+{
+    int n = 0;
+#define X(B, Y, T, N)              \
+    if (n < max) {                 \
+        out[n].tbx = B;            \
+        out[n].by = Y;             \
+        out[n].ty = T;             \
+    }                              \
+    ++n;
+    DN_TILES(X)
+#undef X
+    return n;
+}
+
 bool tile_supported(const Tile &t)
 {
 #define X(B, Y, T, N) \

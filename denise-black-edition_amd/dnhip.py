@@ -120,6 +120,7 @@ def load():
         "dn_set_kernel_sampling": ([vp, C.c_int], C.c_int),
         "dn_set_tile": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
         "dn_get_tile": ([vp, ip, ip, ip], C.c_int),
+        "dn_autotune_tile": ([vp, C.c_int, ip, ip, ip], C.c_int),
         "dn_get_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), ip], C.c_int),
         "dn_get_footprint": ([vp, C.POINTER(C.c_size_t)], C.c_int),
     }
@@ -272,6 +273,12 @@ class Context:
     def tile(self):
         a, b, t = C.c_int(), C.c_int(), C.c_int()
         _check(load().dn_get_tile(self.h, C.byref(a), C.byref(b), C.byref(t)))
+        return a.value, b.value, t.value
+
+    def autotune(self, nsteps=50):
+        """Pick the fastest compiled tile shape for this configuration."""
+        a, b, t = C.c_int(), C.c_int(), C.c_int()
+        _check(load().dn_autotune_tile(self.h, nsteps, C.byref(a), C.byref(b), C.byref(t)))
         return a.value, b.value, t.value
 
     def set_kernel_sampling(self, every):

@@ -171,6 +171,13 @@ int dn_get_hc(dn_ctx *ctx, float *out7);
 int dn_set_tile(dn_ctx *ctx, int tbx, int by, int ty);
 int dn_get_tile(dn_ctx *ctx, int *tbx, int *by, int *ty);
 
+/* Tile autotuner: times nsteps forward steps of this context's own
+ * configuration with every compiled tile shape, keeps the fastest and
+ * returns it (any output pointer may be NULL).  Leaves the wavefields zeroed
+ * (a following dn_run_forward starts the shot from rest).  No reference
+ * counterpart: the reference has no GPU launch shapes. */
+int dn_autotune_tile(dn_ctx *ctx, int nsteps, int *tbx, int *by, int *ty);
+
 /* Timing of the last run: total device milliseconds (hipEvents around the
  * time loop on the context stream) and, when kernel sampling is on, the
  * average duration of the velocity and stress kernels. every = 0 turns

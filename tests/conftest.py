@@ -1,4 +1,5 @@
 import os
+import subprocess
 import sys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -10,3 +11,14 @@ for sub in ("denise-black-edition_amd", "oracle", "tests"):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD MI355X GPU (runs the HIP library)")
+
+
+def pytest_sessionstart(session):
+    """Build the CPU oracle and the libraries when a fresh checkout lacks them
+    (they are build products, not tracked files)."""
+    jobs = str(min(8, os.cpu_count() or 4))
+    need = [("oracle", "liboracle_f32.so"), ("denise-black-edition_amd", "libdnhip.so")]
+    for sub, lib in need:
+        d = os.path.join(ROOT, sub)
+        if not os.path.exists(os.path.join(d, lib)):
+            subprocess.run(["make", "-C", d, "-j", jobs], check=True)

@@ -291,3 +291,21 @@ def test_tile_shapes_gradient(tile):
     run_gradient(g, case, obs)
     for which in range(6):
         assert rel_l2(g.gradient(which), o32.gradient(which)) <= GRAD_TOL, (tile, which)
+
+
+def test_autotune_keeps_results():
+    """The autotuner only changes the launch shape: the shot run after it is
+    the oracle's, and the chosen shape is a compiled one."""
+    case = make_case(nx=150, ny=90, nt=200, fdorder=8, L=2, seismo=4)
+    p, m, src, sig, rec = case
+    o = run_oracle(case)
+    g = dnhip.Context(p)
+    g.set_model(m["vp"], m["vs"], m["rho"], m["taus"], m["taup"])
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    tile = g.autotune(20)
+    assert tuple(tile) in TILES
+    assert tuple(g.tile()) == tuple(tile)
+    g.forward()
+    for comp in range(5):
+        assert np.array_equal(g.seismogram(comp), o.seismogram(comp).astype(np.float32)), comp
