@@ -587,3 +587,145 @@ def test_gradient_matches_finite_differences():
     assert corr > 0.95
     assert scale > 0
     assert resid < 0.25
+
+
+# --------------------------------------------------------------------------
+# viscoelastic update: the generalized standard linear solid
+# --------------------------------------------------------------------------
+def _linear_velocity(o, a, b, c, d, dh):
+    """vx = a x + c y at (i+1/2, j), vy = b y + d x at (i, j+1/2): uniform
+    strain rates vxx = a, vyy = b, vxy = c, vyx = d."""
+    j, i = _coords(o)
+    vx = a * (i + 0.5) * dh + c * j * dh
+    vy = b * (j + 0.5) * dh + d * i * dh
+    return vx, vy
+
+
+def test_visco_constant_strain_rate_matches_gsls():
+    """Held at constant strain rates, the memory-variable update integrates
+    the GSLS relaxation ODE  r' = -w_l (r + M tau edot): with relaxed modulus
+    M_R = M_ref / (1 + tau sum_l w0^2 tau_l^2 / (1 + w0^2 tau_l^2))
+    (the tau-method's real modulus at the reference frequency w0 = 2 pi FL[1])
+    the stress is
+        sigma(t) = M_R edot t + M_R tau edot sum_l (1 - exp(-w_l t)) / w_l,
+    applied per component (sxx: (lambda+2mu)(a+b) - 2 mu b, syy: ... - 2 mu a,
+    sxy: mu (c+d)).  The velocity field is reset every step, so only the
+    local update is exercised; trapezoidal time stepping leaves an
+    O((w DT)^2) error."""
+    nx = ny = 40
+    dh, dt, nsteps = 10.0, 1e-4, 400
+    fl = [5.0, 40.0]
+    tau_s, tau_p = 0.08, 0.05
+    p = params(nx=nx, ny=ny, nt=nsteps, dh=dh, dt=dt, L=2, fl=fl, fdorder=8, maxrelerror=0)
+    vpv, vsv, rhov = 3000.0, 1700.0, 2000.0
+    vp, vs, rho = homogeneous(nx, ny, vp=vpv, vs=vsv, rho=rhov)
+    taus = np.full((nx, ny), tau_s, np.float32)
+    taup = np.full((nx, ny), tau_p, np.float32)
+    o = orc.Oracle(p, vp, vs, rho, taus, taup, prec="f64")
+    a, b, c, d = 2.0e-3, -0.7e-3, 1.1e-3, 0.4e-3
+    vx, vy = _linear_velocity(o, a, b, c, d, dh)
+    for nt_ in range(1, nsteps + 1):
+        o.set_field(orc.ORC_VX, vx)
+        o.set_field(orc.ORC_VY, vy)
+        o.step(nt_)
+    t = nsteps * float(np.float32(dt))
+    w = [2 * math.pi * f for f in fl]
+    w0 = w[0]
+    S = sum((w0 / wl) ** 2 / (1 + (w0 / wl) ** 2) for wl in w)
+    mu = rhov * float(np.float32(vsv)) ** 2 / (1 + float(np.float32(tau_s)) * S)
+    pm = rhov * float(np.float32(vpv)) ** 2 / (1 + float(np.float32(tau_p)) * S)
+    relax = sum((1 - math.exp(-wl * t)) / wl for wl in w)
+    ts_, tp_ = float(np.float32(tau_s)), float(np.float32(tau_p))
+    exp_sxy = mu * (c + d) * t + mu * ts_ * (c + d) * relax
+    exp_sxx = (pm * (a + b) - 2 * mu * b) * t + (pm * tp_ * (a + b) - 2 * mu * ts_ * b) * relax
+    exp_syy = (pm * (a + b) - 2 * mu * a) * t + (pm * tp_ * (a + b) - 2 * mu * ts_ * a) * relax
+    nd = o.nd
+    jc, ic = ny // 2 + nd - 1, nx // 2 + nd - 1
+    got = {f: o.field(f)[jc, ic] for f in (orc.ORC_SXX, orc.ORC_SYY, orc.ORC_SXY)}
+    # uniform away from the zero ghosts
+    core = (slice(jc - 8, jc + 8), slice(ic - 8, ic + 8))
+    for f in got:
+        blk = o.field(f)[core]
+        assert np.allclose(blk, got[f], rtol=1e-9, atol=0)
+    for f, e in ((orc.ORC_SXX, exp_sxx), (orc.ORC_SYY, exp_syy), (orc.ORC_SXY, exp_sxy)):
+        assert got[f] == pytest.approx(e, rel=2e-4), (f, got[f], e)
+    # the memory terms matter at this tolerance: dropping them misses by far more
+    assert abs(mu * ts_ * (c + d) * relax) > 50 * 2e-4 * abs(exp_sxy)
+
+
+def test_visco_with_zero_tau_is_the_elastic_update():
+    """tau = 0 switches the relaxation off: the viscoelastic update, free
+    surface included, reduces to the elastic one (FP64: rounding only)."""
+    nx, ny, nt = 80, 56, 160
+    vp, vs, rho = synth.layered_model(nx, ny, 10.0, seed=5, sigma=3.0)
+    zero = np.zeros((nx, ny), np.float32)
+    common = dict(nx=nx, ny=ny, nt=nt, dt=8e-4, free_surf=1, fw=10, seismo=4)
+    pe = params(**common)
+    pv = params(L=2, fl=[5.0, 40.0], **common)
+    src, sig = explosive_at(nx, ny, 40, 4, nt, pe.dt)
+    rec = synth.receiver_line(50.0, 20.0, 20, 20.0, 10.0)
+    runs = []
+    for p, taus in ((pe, None), (pv, zero)):
+        o = orc.Oracle(p, vp, vs, rho, taus, taus, prec="f64")
+        o.set_sources(src, sig)
+        o.set_receivers(rec)
+        o.forward()
+        runs.append(o)
+    for comp in range(5):
+        assert rel_l2(runs[1].seismogram(comp), runs[0].seismogram(comp)) < 1e-10, comp
+    for f in (orc.ORC_VX, orc.ORC_VY, orc.ORC_SXX, orc.ORC_SYY, orc.ORC_SXY):
+        assert rel_l2(runs[1].field(f), runs[0].field(f)) < 1e-10, f
+
+
+# --------------------------------------------------------------------------
+# seismo_ssg div / curl, msource
+# --------------------------------------------------------------------------
+def test_div_curl_receivers_on_linear_fields():
+    """SEISMO 3: div = (vxx+vyy) DT sqrt(rho vp^2), curl = (vxy-vyx) DT
+    sqrt(rho vs^2) (src/seismo_ssg.c:25-65).  On linear velocity fields the
+    strain rates are exact, so the samples are known."""
+    nx = ny = 36
+    dh, dt = 10.0, 1e-3
+    p = params(nx=nx, ny=ny, nt=3, dh=dh, dt=dt, seismo=3, maxrelerror=0)
+    vpv, vsv, rhov = 3000.0, 1700.0, 2000.0
+    vp, vs, rho = homogeneous(nx, ny, vp=vpv, vs=vsv, rho=rhov)
+    o = orc.Oracle(p, vp, vs, rho, prec="f64")
+    o.set_receivers(np.array([[14, 18, 22], [15, 18, 20]], np.int32))
+    a, b, c, d = 2.0e-3, -0.7e-3, 1.1e-3, 0.4e-3
+    vx, vy = _linear_velocity(o, a, b, c, d, dh)
+    o.set_field(orc.ORC_VX, vx)
+    o.set_field(orc.ORC_VY, vy)
+    o.step(1)   # zero stress: the velocity update leaves the interior unchanged
+    dt32 = float(np.float32(dt))
+    div = (a + b) * dt32 * math.sqrt(rhov * vpv ** 2)
+    curl = (c - d) * dt32 * math.sqrt(rhov * float(np.float32(vsv)) ** 2)
+    assert np.allclose(o.seismogram(3)[:, 0], div, rtol=1e-6)
+    assert np.allclose(o.seismogram(4)[:, 0], curl, rtol=1e-6)
+
+
+def test_moment_tensor_source_pattern():
+    """QUELLTYP 5 (M12 = 1): the four shear-stress points around the source
+    cell (i, j), at (i +- 1/2, j +- 1/2), each gain amp/4; normal stresses are
+    untouched (src/msource.c:22-40)."""
+    nx, ny, nt = 30, 30, 6
+    p = params(nx=nx, ny=ny, nt=nt, dt=1e-3, seismo=0)
+    vp, vs, rho = homogeneous(nx, ny)
+    o = orc.Oracle(p, vp, vs, rho, prec="f64")
+    sig = np.array([[0.5, 2.0, -1.0, 4.0, 8.0, 16.0]], np.float32)
+    src = np.zeros((8, 1), np.float32)
+    src[0, 0], src[1, 0], src[4, 0], src[7, 0] = 12, 17, 10.0, 5
+    o.set_sources(src, sig)
+    o.step(3)
+    amp = (4.0 - 2.0) / float(np.float32(1e-3))
+    nd = o.nd
+    sxy = o.field(orc.ORC_SXY)
+    # sxy[j][i] sits at (i+1/2, j+1/2): the cells (i-1..i, j-1..j) surround (i, j)
+    expect = np.zeros_like(sxy)
+    for jj in (16, 17):
+        for ii in (11, 12):
+            expect[jj + nd - 1, ii + nd - 1] = 0.25 * amp
+    assert np.allclose(sxy, expect, rtol=1e-12, atol=0)
+    assert not np.any(o.field(orc.ORC_SXX)) and not np.any(o.field(orc.ORC_SYY))
+    # the pattern's centre is the source cell
+    jw, iw = np.nonzero(sxy)
+    assert np.mean(iw - (nd - 1) + 0.5) == 12 and np.mean(jw - (nd - 1) + 0.5) == 17
