@@ -130,10 +130,16 @@ def calibrate_copy(mib, reps):
     n = mib * (1 << 20) // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
+    b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(reps):
         b.copy_(a)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}))
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps + 1,
+                      "ms_per_copy": ms, "copy_GBs": 2 * n * 4 / (ms * 1e-3) / 1e9}))
 
 
 def main():

@@ -19,7 +19,7 @@ INCLUDE = os.path.join(ROOT, "include")
 def _declared(header):
     text = open(os.path.join(INCLUDE, header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(dn_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(dn_[a-z0-9_]+|dnio_[a-z0-9_]+)\s*\(", text)))
 
 
 def _headers():

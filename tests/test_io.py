@@ -1,0 +1,316 @@
+"""Host file front end (libdnio.so): the reference's .inp, workflow, source,
+receiver, model and SU formats (CPU only).
+
+Fixtures under tests/golden/marmousi are the reference's own example inputs
+(par/DENISE_marm_OBC.inp, par/FWI_workflow_marmousi.inp,
+par/source/source_OBC_VSP.dat, par/receiver/receiver_OBC.dat); the expected
+values below are read off those files by eye."""
+import math
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import dnio
+import synth
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "marmousi")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "denise-black-edition_amd", "bin", "denise_hip")
+
+
+def test_marmousi_parameter_file():
+    p = dnio.read_par(os.path.join(GOLD, "DENISE_marm_OBC.inp"))
+    expect = dict(mode=0, physics=1, nprocx=5, nprocy=3, fdorder=8, maxrelerror=1, nx=500, ny=174, dh=20.0,
+                  time=6.0, quellart=6, signal_file="./wavelet/wavelet_marmousi", ts=8.0, srcrec=1,
+                  source_file="./source/source_OBC_VSP.dat", run_multiple_shots=1, fc_spike_1=-5.0,
+                  fc_spike_2=15.0, order_spike=5, write_stf=1, readmod=1, mfile="start/marmousi_II_marine",
+                  writemod=1, L=0, tau=1.0, free_surf=1, fw=10, damping=1500.0, fpml=10.0, npower=4.0,
+                  k_max_pml=1.0, boundary=0, snap=0, snap_shot=1, idx=1, idy=1, snap_format=3,
+                  snap_file="./snap/waveform_forward", seismo=1, readrec=1, rec_file="./receiver/receiver_OBC",
+                  n_streamer=0, rec_incr_x=80.0, rec_incr_y=0.0, ndt=1, seis_format=1,
+                  seis_file_vx="su/DENISE_MARMOUSI_x.su", seis_file_vy="su/DENISE_MARMOUSI_y.su",
+                  seis_file_curl="su/2layer_rot.su", seis_file_div="su/2layer_div.su",
+                  seis_file_p="su/DENISE_MARMOUSI_p.su", log_file="log/US.log", log=1, itermax=600,
+                  jacobian="jacobian/jacobian_Test", data_dir="su/MARMOUSI_spike/DENISE_MARMOUSI", taper=0,
+                  taperlength=4, invmat1=1, grad_form=1, quelltypb=1, testshot_start=25, testshot_end=75,
+                  testshot_incr=10, sws_taper_grad_vert=0, sws_taper_grad_hor=1, exp_taper_grad_hor=2.0,
+                  srtradius=5.0, filtsize=1, tfile="taper/taper", vpupperlim=6000.0, vslowerlim=0.0,
+                  rhoupperlim=3000.0, rholowerlim=1000.0, qsupperlim=100.0, grad_method=2, pcg_beta=2,
+                  nlbfgs=20, model_filter=0, filt_size=5, dtinv=3, stepmax=6, scalefac=2.0, trkill=0,
+                  trkill_file="./trace_kill/trace_kill.dat", picks_file="./picked_times/picks_",
+                  misfit_log_file="LOG_TEST.dat", min_iter=0, grad_filter=0, filt_size_grad=10, timelapse=0,
+                  data_dir_t0="su/CAES_spike_time_0/DENISE_CAES", rtmod=0, gravity=0, ngravb=500,
+                  nzgrav=200000, grav_type=1, back_density=2, dfile="gravity/background_density.rho",
+                  rtm_shot=0)
+    for k, v in expect.items():
+        assert p.get(k) == (pytest.approx(v) if isinstance(v, float) else v), k
+    assert p.dt == np.float32(2.0e-3) and p.eps_scale == np.float32(0.01)
+    assert list(p.gradt) == [21, 25, 490, 500]
+    assert list(p.refrec)[1:3] == [0.0, 0.0]
+    assert p.nt == 3000                       # iround(TIME/DT)
+    assert p.lines >= 115
+
+
+def test_parameter_line_rules(tmp_path):
+    """Comment lines start with '#'; the first file line never counts; the
+    first character of a parameter line is dropped before the key (so a
+    one-letter key leaves the value unread); %i takes C integer syntax; FL
+    takes L numbers; FW < 0 becomes 0."""
+    lines = ["#", "XMODE = 1", "#c", "PHYS = 1", "N = 4", "NP = 3", "FD = 0x6", "MR = 2", "NX = 010",
+             "NY = 33", "DH = 2.5", "TT = 0.1", "DT = 1e-3"]
+    lines += ["QA = 1", "SF = sig", "TS = 0.5", "SR = 1", "SF = src.dat", "RM = 1", "F1 = 1", "F2 = 2",
+              "OS = 3", "WS = 0", "RM = 1", "MF = model/m", "WM = 0", "LL = 3", "FL = 1.5 2.5 3.5",
+              "TA = 0.2", "FS = 1", "FW = -4"]
+    path = tmp_path / "t.inp"
+    path.write_text("\n".join(lines) + "\n")
+    p = dnio.read_par(str(path))
+    assert p.mode == 1 and p.physics == 1 and p.fdorder == 6 and p.maxrelerror == 2
+    assert p.nprocx == 0 and p.nprocy == 3
+    assert p.nx == 8                          # octal, as %i reads it
+    assert p.ny == 33 and p.dh == 2.5 and p.nt == 100
+    assert p.get("signal_file") == "sig" and p.get("mfile") == "model/m"
+    assert p.L == 3 and list(p.fl)[:3] == [1.5, 2.5, 3.5]
+    assert p.free_surf == 1 and p.fw == 0
+
+
+def test_marmousi_workflow():
+    wf = os.path.join(GOLD, "FWI_workflow_marmousi.inp")
+    assert dnio.count_stages(wf) == 4
+    s1, s4 = dnio.read_stage(wf, 1), dnio.read_stage(wf, 4)
+    assert s1.pro == pytest.approx(0.01) and s1.time_filt == 1 and s1.fc_end == 2.0 and s1.order == 6
+    assert s1.gamma == 20.0 and s1.eprecond == 3 and s1.lnorm == 2 and s1.n_order == 0
+    assert s1.wd_damp == 0.5 and s1.offsetc_stf == -4.0 and s1.eps_stf == pytest.approx(0.1)
+    assert s1.offsetc == 10.0 and s1.scalerho == 0.5 and s1.scaleqs == 1.0
+    assert s4.fc_end == 20.0 and s4.gamma == 0.5
+    with pytest.raises(dnio.IOError_):
+        dnio.read_stage(wf, 5)
+
+
+def test_marmousi_sources():
+    src, ts = dnio.read_sources(os.path.join(GOLD, "source_OBC_VSP.dat"))
+    assert src.shape == (8, 100)
+    # columns: x, z (ignored), y, tshift, fc, amp, azimuth, type
+    assert np.allclose(src[:, 0], [800.0, 40.0, 0.0, 0.0, 10.0, 1.0, 0.0, 1.0])
+    assert src[0, 1] == 880.0 and np.all(src[7] == 1)
+    assert ts == pytest.approx(0.1)
+
+
+def test_source_file_rules(tmp_path):
+    f = tmp_path / "s.dat"
+    f.write_text("3\n100 0 50 0.01 8 2\n200 0 60 0 12 1 30 4\n-5 0 20 0 5 1 0 1\n")
+    src, ts = dnio.read_sources(str(f))
+    assert src[6, 0] == 0.0 and src[7, 0] == 0.0          # 6 values: azimuth 0, type 0
+    assert src[6, 1] == 30.0 and src[7, 1] == 4.0
+    assert ts == np.float32(1.0 / 12.0)
+    p = dnio.Par()
+    p.nx, p.ny, p.dh = 20, 10, 10.0
+    loc = dnio.split_sources(p, src, 0, 3)
+    # x = -5 m rounds to index 0 -> (0-1)/NX == 0 in C: kept at index 0
+    assert loc.shape == (8, 3) or loc.shape == (8, 2)
+    assert list(loc[0]) == [10.0, 20.0, 0.0][: loc.shape[1]] and list(loc[1][:2]) == [5.0, 6.0]
+    assert list(loc[2][:2]) == [1.0, 2.0]
+    f.write_text("1\n100 0 50 0.01\n")
+    with pytest.raises(dnio.IOError_, match="Missing parameter"):
+        dnio.read_sources(str(f))
+
+
+def test_marmousi_receivers(tmp_path):
+    p = dnio.Par()
+    p.dh = 20.0
+    p.readrec = 1
+    p.rec_file = os.path.join(GOLD, "receiver_OBC").encode()
+    rec = dnio.read_receivers(p)
+    assert rec.shape == (3, 400)
+    assert list(rec[:, 0]) == [40, 23, 0] and list(rec[:, 1]) == [41, 23, 0]
+
+
+def test_receiver_dedup_and_refrec(tmp_path):
+    (tmp_path / "r.dat").write_text("10 20\n10.4 20\n0 30\n30 40\n10 20\n")
+    p = dnio.Par()
+    p.dh = 10.0
+    p.readrec = 1
+    p.rec_file = str(tmp_path / "r").encode()
+    rec = dnio.read_receivers(p)
+    # duplicates of (1,2) removed, x index 0 dropped
+    assert rec[:2].T.tolist() == [[1, 2], [3, 4]]
+    p.refrec[1], p.refrec[2] = 5.0, 10.0
+    rec = dnio.read_receivers(p)
+    assert rec[:2].T.tolist()[0] == [2, 3]
+    (tmp_path / "r_shot_2.dat").write_text("50 50\n")
+    p.readrec = 2
+    assert dnio.read_receivers(p, 2)[:2].T.tolist() == [[6, 6]]
+
+
+def _par(nt=200, dt=1e-3, quellart=1):
+    p = dnio.Par()
+    p.nt, p.dt, p.quellart = nt, dt, quellart
+    return p
+
+
+def _loc(tshift=0.0, fc=10.0, amp=2.0):
+    loc = np.zeros((8, 1), np.float32)
+    loc[3, 0], loc[4, 0], loc[5, 0] = tshift, fc, amp
+    return loc
+
+
+def test_ricker_wavelet_matches_the_closed_form():
+    p = _par()
+    s = dnio.wavelet(p, _loc(0.01, 10.0, 2.0), 1, 0.1)[0]
+    assert np.array_equal(s, synth.ricker(200, 1e-3, 10.0, tshift=0.01, amp=2.0))
+    t = np.arange(1, 201) * 1e-3
+    tau = np.pi * (t - 1.5 / 10.0 - 0.01) / (1.5 / 10.0)
+    assert np.allclose(s, 2.0 * (1 - 4 * tau ** 2) * np.exp(-2 * tau ** 2), atol=2e-5)  # float32 tau
+
+
+@pytest.mark.parametrize("qa", [2, 4, 5, 6])
+def test_other_wavelets_closed_forms(qa):
+    p = _par(quellart=qa)
+    fc, ts0 = 25.0, 0.0045   # no sample on the window edges
+    s = dnio.wavelet(p, _loc(ts0, fc, 1.0), 1, 1.0 / fc)[0]
+    t = np.arange(1, 201) * 1e-3
+    T = 1.0 / fc
+    if qa == 2:
+        e = np.where((t < ts0) | (t > ts0 + T), 0.0,
+                     np.sin(2 * np.pi * (t - ts0) * fc) - 0.5 * np.sin(4 * np.pi * (t - ts0) * fc))
+    elif qa == 4:
+        e = np.where((t < ts0) | (t > ts0 + T), 0.0, np.sin(np.pi * (t + ts0) / T) ** 3)
+    elif qa == 5:
+        ag = np.pi ** 2 * fc ** 2
+        e = -2 * ag * (t - 1.2 / fc) * np.exp(-ag * (t - 1.2 / fc) ** 2)
+    else:
+        e = np.zeros(200)
+        e[round(ts0 / 1e-3)] = 1.0
+    assert np.allclose(s, e, atol=2e-5 * max(1.0, np.abs(e).max())), qa
+
+
+def test_wavelet_integration_n_order():
+    p = _par()
+    s0 = dnio.wavelet(p, _loc(), 1, 0.1)[0].astype(np.float64)
+    s1 = dnio.wavelet(p, _loc(), 1, 0.1, n_order=1)[0]
+    assert np.allclose(s1, np.cumsum(s0) * 1e-3, rtol=1e-5, atol=1e-9)
+
+
+def test_signal_file_wavelet(tmp_path):
+    p = _par(nt=5, quellart=3)
+    p.signal_file = str(tmp_path / "w").encode()
+    (tmp_path / "w_shot_2.dat").write_text("1.0\n2.0\n-3.0\n4.5\n0.25\n")
+    s = dnio.wavelet(p, _loc(amp=2.0), 2, 0.1)[0]
+    assert list(s) == [2.0, 4.0, -6.0, 9.0, 0.5]
+
+
+def test_model_files_round_trip(tmp_path):
+    nx, ny = 7, 5
+    p = dnio.Par()
+    p.nx, p.ny, p.readmod, p.invmat1, p.writemod = nx, ny, 1, 1, 1
+    p.mfile = str(tmp_path / "m").encode()
+    rng = np.random.default_rng(0)
+    arrs = {e: rng.uniform(1, 5, (nx, ny)).astype(np.float32) for e in ("vp", "vs", "rho")}
+    for e, a in arrs.items():
+        a.tofile(str(tmp_path / f"m.{e}"))       # file order: x outer, y inner
+    vp, vs, rho, _, _ = dnio.read_model(p)
+    assert np.array_equal(vp, arrs["vp"]) and np.array_equal(rho, arrs["rho"])
+    assert np.array_equal(np.fromfile(str(tmp_path / "m.denise.pi"), np.float32).reshape(nx, ny), arrs["vp"])
+    assert os.path.exists(str(tmp_path / "m.denise.mu.0.0"))
+    # viscoelastic: Q files must exist, their values are replaced by Q = 30
+    p.L = 2
+    for e in ("qp", "qs"):
+        np.full((nx, ny), 77.0, np.float32).tofile(str(tmp_path / f"m.{e}"))
+    vp, vs, rho, taus, taup = dnio.read_model(p)
+    assert np.all(taus == np.float32(2.0 / 30.0)) and np.all(taup == np.float32(2.0 / 30.0))
+    assert os.path.exists(str(tmp_path / "m.fdveps.rho"))
+    os.remove(str(tmp_path / "m.qs"))
+    with pytest.raises(dnio.IOError_, match="m.qs"):
+        dnio.read_model(p)
+
+
+def test_su_writer_header_and_samples(tmp_path):
+    p = dnio.Par()
+    p.dh, p.dt, p.ndt, p.seis_format = 10.0, 5e-4, 2, 1
+    p.refrec[1], p.refrec[2] = 0.0, 0.0
+    ntr, nt = 3, 9
+    sec = np.arange(ntr * nt, dtype=np.float32).reshape(ntr, nt)
+    rec = np.array([[10, 12, 14], [3, 3, 4], [0, 0, 0]], np.int32)
+    xs, ys = 150.0, 20.0
+    path = str(tmp_path / "x.su")
+    dnio.write_seis(path, p, sec, rec, xs, ys)
+    hdr, data = dnio.parse_su(path)
+    ns2 = math.floor(nt // 2 + 0.5)
+    assert data.shape == (ntr, ns2)
+    assert np.array_equal(data, sec[:, 1::2][:, :ns2])        # samples NDT, 2NDT, ...
+    xr = rec[0] * 10.0
+    yr = rec[1] * 10.0
+    assert list(hdr["gx"]) == list((xr * 1000).astype(int)) and list(hdr["gy"]) == list((yr * 1000).astype(int))
+    assert list(hdr["gelev"]) == list((yr * 1000).astype(int))
+    assert np.all(hdr["sx"] == 150000) and np.all(hdr["sy"] == 20000) and np.all(hdr["sdepth"] == 20000)
+    assert list(hdr["offset"]) == [round(math.hypot(xs - a, ys - b) * 1000) for a, b in zip(xr, yr)]
+    assert np.all(hdr["trid"] == 1) and np.all(hdr["scalel"] == -1000) and np.all(hdr["scalco"] == -1000)
+    assert np.all(hdr["ns"] == ns2) and np.all(hdr["dt"] == 1000) and np.all(hdr["d1"] == np.float32(1e-3))
+    assert list(hdr["cdp"]) == [100 + round(abs(a + xs) / 2 / 20.0) for a in xr]
+    sw = [round(math.degrees(math.atan2(a - 800.0, b - 800.0)) * 1000) for a, b in zip(xr, yr)]
+    assert list(hdr["swdep"]) == sw
+    assert np.all(hdr["tracl"] == 0) and np.all(hdr["ep"] == 0) and np.all(hdr["unass"] == 0)
+    # inseis: sample k of the file -> index k, first sample zero
+    back = dnio.read_su(path, ntr, ns2)
+    assert np.array_equal(back[:, 1:], data[:, 1:]) and np.all(back[:, 0] == 0)
+
+
+def test_ascii_and_binary_seismograms(tmp_path):
+    p = dnio.Par()
+    p.dh, p.dt, p.ndt = 10.0, 1e-3, 1
+    sec = np.array([[1.5, -2.0, 3.25]], np.float32)
+    rec = np.array([[1], [1], [0]], np.int32)
+    p.seis_format = 3
+    dnio.write_seis(str(tmp_path / "b"), p, sec, rec, 0.0, 0.0)
+    assert np.array_equal(np.fromfile(str(tmp_path / "b"), np.float32), sec[0])
+    p.seis_format = 2
+    dnio.write_seis(str(tmp_path / "a"), p, sec, rec, 0.0, 0.0)
+    assert [float(v) for v in open(str(tmp_path / "a")).read().split()] == [1.5, -2.0, 3.25]
+
+
+def test_file_names():
+    p = dnio.read_par(os.path.join(GOLD, "DENISE_marm_OBC.inp"))
+    assert dnio.seis_name(p, 0, 7) == "su/DENISE_MARMOUSI_x.su.shot7"
+    assert dnio.seis_name(p, 2, 3) == "su/DENISE_MARMOUSI_p.su.shot3"
+    p.mode = 1
+    assert dnio.seis_name(p, 1, 2, 4) == "su/DENISE_MARMOUSI_y.su.shot2.it4"
+    assert dnio.data_name(p, 0, 5) == "su/MARMOUSI_spike/DENISE_MARMOUSI_x.su.shot5"
+    assert dnio.data_name(p, 2, 1) == "su/MARMOUSI_spike/DENISE_MARMOUSI_p.su.shot1"
+
+
+def _run_cli(*args, cwd=None):
+    return subprocess.run([BIN, *args], cwd=cwd, capture_output=True, text=True, timeout=60)
+
+
+def test_cli_refuses_what_is_not_built(tmp_path):
+    """Exit status 1 and a message, before any device work."""
+    assert os.path.exists(BIN)
+    r = _run_cli(str(tmp_path / "missing.inp"))
+    assert r.returncode == 1 and "parameter file" in r.stderr
+    src = os.path.join(GOLD, "DENISE_marm_OBC.inp")
+    text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 2")
+    (tmp_path / "ac.inp").write_text(text)
+    r = _run_cli(str(tmp_path / "ac.inp"))
+    assert r.returncode == 1 and "PHYSICS" in r.stderr
+    r = _run_cli(src)    # the Marmousi example uses the filtered spike source
+    assert r.returncode == 1 and "QUELLART" in r.stderr
+    r = _run_cli()
+    assert r.returncode == 1 and "usage" in r.stderr
+
+
+def test_written_parameter_file_reads_back(tmp_path):
+    import inpfile
+    path = str(tmp_path / "p.inp")
+    inpfile.write_inp(path, NX=123, NY=45, DT=5e-4, TIME=0.25, L=2, FL="3.5 40.0", FW=7, SEISMO=4, NDT=2,
+                      DTINV=3, QUELLTYPB=7, TESTSHOT="2, 9, 3", RTM_SHOT=1, GRADT="5, 6, 7, 8")
+    p = dnio.read_par(path)
+    assert (p.nx, p.ny, p.L, p.fw, p.seismo, p.ndt, p.dtinv, p.quelltypb, p.rtm_shot) == (123, 45, 2, 7, 4, 2, 3, 7, 1)
+    assert list(p.fl)[:2] == [3.5, 40.0] and p.nt == 500 and p.lines >= 115
+    assert (p.testshot_start, p.testshot_end, p.testshot_incr) == (2, 9, 3) and list(p.gradt) == [5, 6, 7, 8]
+    assert p.get("source_file") == "source.dat" and p.get("data_dir") == "obs/d"
+    wf = str(tmp_path / "w.inp")
+    inpfile.write_workflow(wf, n_order=1)
+    s = dnio.read_stage(wf, 1)
+    assert s.lnorm == 2 and s.n_order == 1 and s.time_filt == 0
