@@ -60,18 +60,19 @@ def workload(L, nt=6000, nrec=800, seed=11):
     return p, dict(vp=vp, vs=vs, rho=rho, taus=taus, taup=taup), src, sig, rec
 
 
-def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tile=None):
+def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tiles=None):
     import dnhip
     p, m, src, sig, rec = workload(L, nt=nt)
     ctx = dnhip.Context(p)
     ctx.set_model(m["vp"], m["vs"], m["rho"], m["taus"], m["taup"])
     ctx.set_sources(src, sig)
     ctx.set_receivers(rec)
-    if tile:
-        ctx.set_tile(*tile)
+    if tiles:
+        ctx.set_tile(*tiles[0], sweep=0)
+        ctx.set_tile(*tiles[1], sweep=1)
     elif tune_steps > 0:
         ctx.autotune(tune_steps)
-    tile = ctx.tile()
+    tile = [list(ctx.tile(0)), list(ctx.tile(1))]
     for _ in range(warmup):
         ctx.forward()
     ctx.set_kernel_sampling(sample_every)
@@ -102,7 +103,7 @@ def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tile=None
         "step_GBs": (vel_b + str_b) * cells * p.nt * steps / total / 1e9,
         "stress_bytes_per_cell": str_b, "vel_bytes_per_cell": vel_b,
         "footprint_GB": ctx.footprint() / 1e9,
-        "tile": list(tile),
+        "tile": {"velocity": tile[0], "stress": tile[1]},
     }
     ctx.close()
     return out
@@ -154,7 +155,8 @@ def main():
     ap.add_argument("--no-visco", action="store_true", help="profiling: elastic shot only")
     ap.add_argument("--nt", type=int, default=6000, help="profiling: shorter shots")
     ap.add_argument("--tune-steps", type=int, default=40)
-    ap.add_argument("--tile", default="", help="profiling: fixed tile shape tbx,by,ty (no autotune)")
+    ap.add_argument("--tile-v", default="", help="profiling: fixed velocity-sweep shape tbx,by,ty,vw")
+    ap.add_argument("--tile-s", default="", help="profiling: fixed stress-sweep shape tbx,by,ty,vw")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MiB",
                     help="profiling: run only a device copy of this size")
     args = ap.parse_args()
@@ -165,17 +167,19 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU shot sharding is a "
                                    "design note (one shot per GPU), run with --gpus 1"}))
         sys.exit(2)
-    tile = tuple(int(v) for v in args.tile.split(",")) if args.tile else None
+    tiles = None
+    if args.tile_v and args.tile_s:
+        tiles = [tuple(int(v) for v in args.tile_v.split(",")), tuple(int(v) for v in args.tile_s.split(","))]
     L = 2
     if args.no_visco:
-        elastic = run_gpu(0, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tile=tile)
+        elastic = run_gpu(0, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles)
         print(json.dumps({"elastic": elastic}))
         return
-    res = run_gpu(L, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tile=tile)
+    res = run_gpu(L, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles)
     print(f"# visco: {res}", file=sys.stderr, flush=True)
     elastic = None
     if not args.no_elastic:
-        elastic = run_gpu(0, 1, 1, nt=args.nt, tune_steps=args.tune_steps, tile=tile)
+        elastic = run_gpu(0, 1, 1, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles)
         print(f"# elastic: {elastic}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:

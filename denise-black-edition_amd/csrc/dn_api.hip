@@ -67,8 +67,8 @@ struct dn_ctx {
     KArgs a{};
     // tile shape of the sweeps and the per-tile cell work lists
     // (0 forward velocity, 1 forward stress, 2 adjoint velocity, 3 adjoint stress)
-    Tile tile;
-    int ntx = 0, ntiles = 0;
+    Tile tile_v, tile_s;  // velocity / stress sweep shapes
+    int ntx_v = 0, ntiles_v = 0, ntx_s = 0, ntiles_s = 0;
     int *lst_off[4] = {};
     Ent *lst_ent[4] = {};
     // timing
@@ -223,19 +223,23 @@ Tile default_tile(int nx, int ny)
 int build_lists(dn_ctx *c)
 {
     const int NX = c->p.nx, NY = c->p.ny;
-    const Tile &t = c->tile;
-    c->ntx = t.ntx(NX);
-    c->ntiles = c->ntx * t.nty(NY);
+    c->ntx_v = c->tile_v.ntx(NX);
+    c->ntiles_v = c->ntx_v * c->tile_v.nty(NY);
+    c->ntx_s = c->tile_s.ntx(NX);
+    c->ntiles_s = c->ntx_s * c->tile_s.nty(NY);
     struct Item {
         int tile;
         Ent e;
     };
     std::vector<Item> lists[4];
+    // lists 0 and 2 belong to the velocity sweep, 1 and 3 to the stress sweep
     auto add = [&](int which, int i, int j, int kind, int idx) {
-        const int tx = (i - 1) / t.tbx, ty = (j - 1) / t.rows();
+        const Tile &t = (which == 0 || which == 2) ? c->tile_v : c->tile_s;
+        const int ntx = (which == 0 || which == 2) ? c->ntx_v : c->ntx_s;
+        const int tx = (i - 1) / t.width(), ty = (j - 1) / t.rows();
         Item it;
-        it.tile = ty * c->ntx + tx;
-        it.e.lane = (i - 1) - tx * t.tbx;
+        it.tile = ty * ntx + tx;
+        it.e.lane = (i - 1) - tx * t.width();
         it.e.row = j;
         it.e.kind = kind;
         it.e.idx = idx;
@@ -257,10 +261,11 @@ int build_lists(dn_ctx *c)
     for (int w = 0; w < 4; ++w) {
         std::stable_sort(lists[w].begin(), lists[w].end(),
                          [](const Item &x, const Item &y) { return x.tile < y.tile; });
-        std::vector<int> off((size_t)c->ntiles + 1, 0);
+        const int ntiles = (w == 0 || w == 2) ? c->ntiles_v : c->ntiles_s;
+        std::vector<int> off((size_t)ntiles + 1, 0);
         std::vector<Ent> ent(lists[w].size() + 1);
         for (auto &it : lists[w]) off[(size_t)it.tile + 1]++;
-        for (int k = 0; k < c->ntiles; ++k) off[(size_t)k + 1] += off[k];
+        for (int k = 0; k < ntiles; ++k) off[(size_t)k + 1] += off[k];
         for (size_t k = 0; k < lists[w].size(); ++k) ent[k] = lists[w][k].e;
         if (c->lst_off[w]) c->dfree(c->lst_off[w]);
         if (c->lst_ent[w]) c->dfree(c->lst_ent[w]);
@@ -279,8 +284,6 @@ int build_lists(dn_ctx *c)
 StepCtl base_ctl(dn_ctx *c, int lv, int ls)
 {
     StepCtl s{};
-    s.ntx = c->ntx;
-    s.ntiles = c->ntiles;
     s.vlist.off = c->lst_off[lv];
     s.vlist.ent = c->lst_ent[lv];
     s.slist.off = c->lst_off[ls];
@@ -501,7 +504,7 @@ int dn_ctx_create(const dn_params *p, int device, dn_ctx **out)
     if ((rc = c->dalloc((void **)&c->acq.nan_step, sizeof(int))) || (rc = c->dalloc((void **)&c->acq.l2, sizeof(double))))
         return bail(rc);
     c->acq.nt = p->nt;
-    c->tile = default_tile(p->nx, p->ny);
+    c->tile_v = c->tile_s = default_tile(p->nx, p->ny);
     if ((rc = build_lists(c))) return bail(rc);
     fill_args(c);
     if (hipDeviceSynchronize() != hipSuccess) return bail(fail(DN_ERR_HIP, "device synchronisation failed"));
@@ -671,61 +674,75 @@ int dn_set_receivers(dn_ctx *c, int ntr, const int *recpos)
     return receiver_factors(c);
 }
 
-int dn_set_tile(dn_ctx *c, int tbx, int by, int ty)
+int dn_set_tile(dn_ctx *c, int sweep, int tbx, int by, int ty, int vw)
 {
     if (!c) return fail(DN_ERR_ARG, "null context");
+    if (sweep < -1 || sweep > 1) return fail(DN_ERR_ARG, "sweep must be -1 (both), 0 (velocity) or 1 (stress)");
     Tile t;
     t.tbx = tbx;
     t.by = by;
     t.ty = ty;
+    t.vw = vw;
     if (!tile_supported(t)) return fail(DN_ERR_ARG, "tile shape not compiled into the library");
     HIP_OK(hipSetDevice(c->device));
     HIP_OK(hipStreamSynchronize(c->stream));
-    c->tile = t;
+    if (sweep != 1) c->tile_v = t;
+    if (sweep != 0) c->tile_s = t;
     return build_lists(c);
 }
 
-int dn_get_tile(dn_ctx *c, int *tbx, int *by, int *ty)
+int dn_get_tile(dn_ctx *c, int sweep, int *tbx, int *by, int *ty, int *vw)
 {
-    if (!c) return fail(DN_ERR_ARG, "null context");
-    if (tbx) *tbx = c->tile.tbx;
-    if (by) *by = c->tile.by;
-    if (ty) *ty = c->tile.ty;
+    if (!c || sweep < 0 || sweep > 1) return fail(DN_ERR_ARG, "null context or sweep not 0/1");
+    const Tile &t = sweep == 0 ? c->tile_v : c->tile_s;
+    if (tbx) *tbx = t.tbx;
+    if (by) *by = t.by;
+    if (ty) *ty = t.ty;
+    if (vw) *vw = t.vw;
     return DN_OK;
 }
 
-int dn_autotune_tile(dn_ctx *c, int nsteps, int *tbx, int *by, int *ty)
+int dn_autotune_tile(dn_ctx *c, int nsteps, int *vel_shape, int *str_shape)
 {
     if (!c || nsteps < 1) return fail(DN_ERR_ARG, "autotune needs a context and nsteps >= 1");
     if (!c->model_set) return fail(DN_ERR_STATE, "dn_set_model must precede dn_autotune_tile");
     Tile shapes[32];
     const int n = tile_list(shapes, 32);
     const int keep_sampling = c->sample_every;
-    c->sample_every = 0;
-    Tile best = c->tile;
-    double best_ms = -1.0;
-    for (int k = 0; k < n && k < 32; ++k) {
-        int rc = dn_set_tile(c, shapes[k].tbx, shapes[k].by, shapes[k].ty);
+    c->sample_every = 1;  // time every launch of both sweeps
+    Tile best_v = c->tile_v, best_s = c->tile_s;
+    double bv = -1.0, bs = -1.0;
+    int rc = DN_OK;
+    for (int k = 0; k < n && k < 32 && !rc; ++k) {
+        rc = dn_set_tile(c, -1, shapes[k].tbx, shapes[k].by, shapes[k].ty, shapes[k].vw);
         if (!rc) rc = dn_run_forward(c, nsteps);  // warm-up (code load, caches)
         if (!rc) rc = dn_run_forward(c, nsteps);
-        if (rc) {
-            c->sample_every = keep_sampling;
-            return rc;
+        if (rc) break;
+        if (bv < 0.0 || c->v_ms < bv) {
+            bv = c->v_ms;
+            best_v = shapes[k];
         }
-        if (best_ms < 0.0 || c->run_ms < best_ms) {
-            best_ms = c->run_ms;
-            best = shapes[k];
+        if (bs < 0.0 || c->s_ms < bs) {
+            bs = c->s_ms;
+            best_s = shapes[k];
         }
     }
     c->sample_every = keep_sampling;
-    int rc = dn_set_tile(c, best.tbx, best.by, best.ty);
     if (rc) return rc;
-    rc = zero_state(c);
-    if (rc) return rc;
+    c->tile_v = best_v;
+    c->tile_s = best_s;
+    if ((rc = build_lists(c))) return rc;
+    if ((rc = zero_state(c))) return rc;
     c->fwd_complete = false;
-    if (tbx) *tbx = best.tbx;
-    if (by) *by = best.by;
-    if (ty) *ty = best.ty;
+    const Tile *bt[2] = {&best_v, &best_s};
+    int *out[2] = {vel_shape, str_shape};
+    for (int w = 0; w < 2; ++w)
+        if (out[w]) {
+            out[w][0] = bt[w]->tbx;
+            out[w][1] = bt[w]->by;
+            out[w][2] = bt[w]->ty;
+            out[w][3] = bt[w]->vw;
+        }
     return DN_OK;
 }
 
@@ -768,12 +785,12 @@ int dn_run_forward(dn_ctx *c, int nsteps)
             if (!e0 || !e1 || !e2 || !pool_event(c, nev + 3)) return fail(DN_ERR_HIP, "event creation failed");
             HIP_OK(hipEventRecord(e0, c->stream));
         }
-        launch_vel(a, ctl, c->acq, c->tile, c->fdoh, 0, c->stream);
+        launch_vel(a, ctl, c->acq, c->tile_v, c->fdoh, 0, c->stream);
         if (sample) {
             HIP_OK(hipEventRecord(e1, c->stream));
             HIP_OK(hipEventRecord(c->pool[nev + 2], c->stream));
         }
-        launch_str(a, ctl, c->acq, c->tile, c->fdoh, c->stream);
+        launch_str(a, ctl, c->acq, c->tile_s, c->fdoh, c->stream);
         if (sample) {
             HIP_OK(hipEventRecord(c->pool[nev + 3], c->stream));
             nev += 4;
@@ -886,12 +903,12 @@ int dn_run_adjoint(dn_ctx *c, int nsteps)
                 if (!pool_event(c, nev + k)) return fail(DN_ERR_HIP, "event creation failed");
             HIP_OK(hipEventRecord(c->pool[nev], c->stream));
         }
-        launch_vel(a, ctl, c->acq, c->tile, c->fdoh, 1, c->stream);
+        launch_vel(a, ctl, c->acq, c->tile_v, c->fdoh, 1, c->stream);
         if (sample) {
             HIP_OK(hipEventRecord(c->pool[nev + 1], c->stream));
             HIP_OK(hipEventRecord(c->pool[nev + 2], c->stream));
         }
-        launch_str(a, ctl, c->acq, c->tile, c->fdoh, c->stream);
+        launch_str(a, ctl, c->acq, c->tile_s, c->fdoh, c->stream);
         if (sample) {
             HIP_OK(hipEventRecord(c->pool[nev + 3], c->stream));
             nev += 4;

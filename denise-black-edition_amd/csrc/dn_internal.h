@@ -130,11 +130,13 @@ struct CellList {
 };
 
 // Tile shape of the two sweeps: blocks of tbx x by threads, each thread
-// marching ty consecutive rows of one column; a tile is tbx x (by*ty) cells.
+// marching ty consecutive rows of vw adjacent columns; a tile is
+// (tbx*vw) x (by*ty) cells.
 struct Tile {
-    int tbx = 64, by = 4, ty = 1;
+    int tbx = 64, by = 4, ty = 1, vw = 1;
+    int width() const { return tbx * vw; }
     int rows() const { return by * ty; }
-    int ntx(int nx) const { return (nx + tbx - 1) / tbx; }
+    int ntx(int nx) const { return (nx + width() - 1) / width(); }
     int nty(int ny) const { return (ny + rows() - 1) / rows(); }
 };
 

@@ -143,10 +143,15 @@ __global__ void k_l2(DevAcq q, int comp)
 
 // Tile shapes compiled into the library: X(TBX, BY, TY, name), each in its
 // own dn_sweep_<name>.hip translation unit.
-#define DN_TILES(X) \
-    X(64, 4, 1, t64x4x1) X(128, 2, 1, t128x2x1) X(256, 1, 1, t256x1x1) X(64, 4, 2, t64x4x2) X(128, 2, 4, t128x2x4)
+// Compiled sweep shapes (tbx, by, ty, vw): one translation unit each
+// (dn_sweep_t*.hip scalar columns, dn_sweepv_t*.hip vector columns).
+#define DN_TILES(X)                                                                              \
+    X(64, 4, 1, 1, t64x4x1) X(128, 2, 1, 1, t128x2x1) X(256, 1, 1, 1, t256x1x1)                   \
+    X(64, 4, 2, 1, t64x4x2) X(128, 2, 4, 1, t128x2x4) X(64, 4, 1, 4, t64x4x1v4)                   \
+    X(64, 4, 2, 4, t64x4x2v4) X(64, 2, 4, 4, t64x2x4v4) X(64, 4, 1, 2, t64x4x1v2)                 \
+    X(128, 2, 2, 2, t128x2x2v2)
 
-#define X(B, Y, T, N)                                                                       \
+#define X(B, Y, T, W, N)                                                                    \
     void launch_vel_##N(const KArgs &, const StepCtl &, const DevAcq &, int, int, void *); \
     void launch_str_##N(const KArgs &, const StepCtl &, const DevAcq &, int, void *);
 DN_TILES(X)
@@ -155,11 +160,12 @@ DN_TILES(X)
 int tile_list(Tile *out, int max)
 {
     int n = 0;
-#define X(B, Y, T, N)              \
+#define X(B, Y, T, W, N)           \
     if (n < max) {                 \
         out[n].tbx = B;            \
         out[n].by = Y;             \
         out[n].ty = T;             \
+        out[n].vw = W;             \
     }                              \
     ++n;
     DN_TILES(X)
@@ -167,27 +173,40 @@ int tile_list(Tile *out, int max)
     return n;
 }
 
+#define DN_SAME(B, Y, T, W) (t.tbx == B && t.by == Y && t.ty == T && t.vw == W)
+
 bool tile_supported(const Tile &t)
 {
-#define X(B, Y, T, N) \
-    if (t.tbx == B && t.by == Y && t.ty == T) return true;
+#define X(B, Y, T, W, N) \
+    if (DN_SAME(B, Y, T, W)) return true;
     DN_TILES(X)
 #undef X
     return false;
 }
 
-void launch_vel(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, int sw, void *stream)
+// The step control carries the tile grid of the sweep being launched.
+static StepCtl with_grid(const StepCtl &c0, const Tile &t, const KArgs &a)
 {
-#define X(B, Y, T, N) \
-    if (t.tbx == B && t.by == Y && t.ty == T) return launch_vel_##N(a, c, q, fdoh, sw, stream);
+    StepCtl c = c0;
+    c.ntx = t.ntx(a.nx);
+    c.ntiles = c.ntx * t.nty(a.ny);
+    return c;
+}
+
+void launch_vel(const KArgs &a, const StepCtl &c0, const DevAcq &q, const Tile &t, int fdoh, int sw, void *stream)
+{
+    const StepCtl c = with_grid(c0, t, a);
+#define X(B, Y, T, W, N) \
+    if (DN_SAME(B, Y, T, W)) return launch_vel_##N(a, c, q, fdoh, sw, stream);
     DN_TILES(X)
 #undef X
 }
 
-void launch_str(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, void *stream)
+void launch_str(const KArgs &a, const StepCtl &c0, const DevAcq &q, const Tile &t, int fdoh, void *stream)
 {
-#define X(B, Y, T, N) \
-    if (t.tbx == B && t.by == Y && t.ty == T) return launch_str_##N(a, c, q, fdoh, stream);
+    const StepCtl c = with_grid(c0, t, a);
+#define X(B, Y, T, W, N) \
+    if (DN_SAME(B, Y, T, W)) return launch_str_##N(a, c, q, fdoh, stream);
     DN_TILES(X)
 #undef X
 }

@@ -118,9 +118,9 @@ def load():
         "dn_get_pml": ([vp, C.c_int, fp], C.c_int),
         "dn_get_hc": ([vp, fp], C.c_int),
         "dn_set_kernel_sampling": ([vp, C.c_int], C.c_int),
-        "dn_set_tile": ([vp, C.c_int, C.c_int, C.c_int], C.c_int),
-        "dn_get_tile": ([vp, ip, ip, ip], C.c_int),
-        "dn_autotune_tile": ([vp, C.c_int, ip, ip, ip], C.c_int),
+        "dn_set_tile": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
+        "dn_get_tile": ([vp, C.c_int, ip, ip, ip, ip], C.c_int),
+        "dn_autotune_tile": ([vp, C.c_int, ip, ip], C.c_int),
         "dn_get_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), ip], C.c_int),
         "dn_get_footprint": ([vp, C.POINTER(C.c_size_t)], C.c_int),
     }
@@ -267,19 +267,21 @@ class Context:
         _check(load().dn_get_hc(self.h, _ptr(out)))
         return out
 
-    def set_tile(self, tbx, by, ty):
-        _check(load().dn_set_tile(self.h, tbx, by, ty))
+    def set_tile(self, tbx, by, ty, vw=1, sweep=-1):
+        """Shape of one sweep (0 velocity, 1 stress) or of both (-1)."""
+        _check(load().dn_set_tile(self.h, sweep, tbx, by, ty, vw))
 
-    def tile(self):
-        a, b, t = C.c_int(), C.c_int(), C.c_int()
-        _check(load().dn_get_tile(self.h, C.byref(a), C.byref(b), C.byref(t)))
-        return a.value, b.value, t.value
+    def tile(self, sweep=0):
+        a, b, t, w = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(load().dn_get_tile(self.h, sweep, C.byref(a), C.byref(b), C.byref(t), C.byref(w)))
+        return a.value, b.value, t.value, w.value
 
     def autotune(self, nsteps=50):
-        """Pick the fastest compiled tile shape for this configuration."""
-        a, b, t = C.c_int(), C.c_int(), C.c_int()
-        _check(load().dn_autotune_tile(self.h, nsteps, C.byref(a), C.byref(b), C.byref(t)))
-        return a.value, b.value, t.value
+        """Pick the fastest compiled shape (tbx, by, ty, vw) of each sweep for
+        this configuration; returns (velocity shape, stress shape)."""
+        v, s = (C.c_int * 4)(), (C.c_int * 4)()
+        _check(load().dn_autotune_tile(self.h, nsteps, v, s))
+        return tuple(v), tuple(s)
 
     def set_kernel_sampling(self, every):
         _check(load().dn_set_kernel_sampling(self.h, every))

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define DN_ABI_VERSION 1
+#define DN_ABI_VERSION 2
 
 typedef enum dn_status {
     DN_OK = 0,
@@ -164,19 +164,23 @@ int dn_get_memvar(dn_ctx *ctx, int which, int l, float *out);
 int dn_get_pml(dn_ctx *ctx, int which, float *out);
 int dn_get_hc(dn_ctx *ctx, float *out7);
 
-/* Tile shape of the two fused sweeps: blocks of tbx x by threads, each
- * thread marching ty rows.  Compiled shapes (tbx,by,ty): (64,4,1),
- * (128,2,1), (256,1,1), (64,4,2), (128,2,4).  Results do not depend on the
- * shape; the batch autotuner picks the fastest per grid. */
-int dn_set_tile(dn_ctx *ctx, int tbx, int by, int ty);
-int dn_get_tile(dn_ctx *ctx, int *tbx, int *by, int *ty);
+/* Tile shape of the fused sweeps: blocks of tbx x by threads, each thread
+ * marching ty rows of vw adjacent columns (vw 1, or 2 / 4 with vector
+ * loads).  sweep: 0 velocity, 1 stress, -1 both (set only).  Compiled
+ * shapes (tbx,by,ty,vw): (64,4,1,1), (128,2,1,1), (256,1,1,1), (64,4,2,1),
+ * (128,2,4,1), (64,4,1,4), (64,4,2,4), (64,2,4,4), (64,4,1,2),
+ * (128,2,2,2).  Results do not depend on the shapes.  No reference
+ * counterpart: the reference has no GPU launch shapes. */
+int dn_set_tile(dn_ctx *ctx, int sweep, int tbx, int by, int ty, int vw);
+int dn_get_tile(dn_ctx *ctx, int sweep, int *tbx, int *by, int *ty, int *vw);
 
 /* Tile autotuner: times nsteps forward steps of this context's own
- * configuration with every compiled tile shape, keeps the fastest and
- * returns it (any output pointer may be NULL).  Leaves the wavefields zeroed
- * (a following dn_run_forward starts the shot from rest).  No reference
- * counterpart: the reference has no GPU launch shapes. */
-int dn_autotune_tile(dn_ctx *ctx, int nsteps, int *tbx, int *by, int *ty);
+ * configuration with every compiled shape (HIP events around each launch)
+ * and keeps the fastest shape of each sweep independently; vel_shape /
+ * str_shape (4 ints: tbx, by, ty, vw; may be NULL) receive them.  Leaves
+ * the wavefields zeroed (a following dn_run_forward starts from rest).  No
+ * reference counterpart. */
+int dn_autotune_tile(dn_ctx *ctx, int nsteps, int *vel_shape, int *str_shape);
 
 /* Timing of the last run: total device milliseconds (hipEvents around the
  * time loop on the context stream) and, when kernel sampling is on, the

@@ -57,7 +57,7 @@ def test_params_layout_matches_header():
 
 def test_abi_version_and_error_text():
     lib = dnhip.load()
-    assert lib.dn_abi_version() == 1
+    assert lib.dn_abi_version() == 2
     assert isinstance(lib.dn_last_error(), bytes)
 
 

@@ -255,7 +255,8 @@ def test_adjoint_requires_forward_store():
     assert ei.value.code == -5
 
 
-TILES = [(64, 4, 1), (128, 2, 1), (256, 1, 1), (64, 4, 2), (128, 2, 4)]
+TILES = [(64, 4, 1, 1), (128, 2, 1, 1), (256, 1, 1, 1), (64, 4, 2, 1), (128, 2, 4, 1), (64, 4, 1, 4),
+         (64, 4, 2, 4), (64, 2, 4, 4), (64, 4, 1, 2), (128, 2, 2, 2)]
 
 
 @pytest.mark.parametrize("tile", TILES)
@@ -279,13 +280,17 @@ def test_tile_shapes_agree(tile, L):
         assert rel_l2(g.field(fid), o.field(fid)) <= SEIS_TOL, (tile, name)
 
 
-@pytest.mark.parametrize("tile", [(128, 2, 1), (64, 4, 2), (128, 2, 4)])
+@pytest.mark.parametrize("tile", [(128, 2, 1, 1), (64, 4, 2, 1), (128, 2, 4, 1), (64, 4, 1, 4), (64, 2, 4, 4),
+                                  (128, 2, 2, 2)])
 def test_tile_shapes_gradient(tile):
+    """Every shape, also with different shapes for the two sweeps, gives the
+    oracle's gradients."""
     case, obs = _gradient_case(1, 0, 1, 2)
     p, m = case[0], case[1]
     o32 = orc.Oracle(p, m["vp"], m["vs"], m["rho"], m["taus"], m["taup"], prec="f32")
     g = dnhip.Context(p)
-    g.set_tile(*tile)
+    g.set_tile(*tile, sweep=0)
+    g.set_tile(*TILES[(TILES.index(tile) + 5) % len(TILES)], sweep=1)
     g.set_model(m["vp"], m["vs"], m["rho"], m["taus"], m["taup"])
     run_gradient(o32, case, obs)
     run_gradient(g, case, obs)
@@ -303,9 +308,9 @@ def test_autotune_keeps_results():
     g.set_model(m["vp"], m["vs"], m["rho"], m["taus"], m["taup"])
     g.set_sources(src, sig)
     g.set_receivers(rec)
-    tile = g.autotune(20)
-    assert tuple(tile) in TILES
-    assert tuple(g.tile()) == tuple(tile)
+    vt, st = g.autotune(20)
+    assert tuple(vt) in TILES and tuple(st) in TILES
+    assert tuple(g.tile(0)) == tuple(vt) and tuple(g.tile(1)) == tuple(st)
     g.forward()
     for comp in range(5):
         assert np.array_equal(g.seismogram(comp), o.seismogram(comp).astype(np.float32)), comp
