@@ -9,7 +9,8 @@ mechanisms with Q=50, CPML 20 on three sides plus free surface, Ricker
 
 value  = cell-updates per second over the K timed shots (Gcell-updates/s).
 roofline = algorithmic HBM bytes of the dominant kernel (the viscoelastic
-           stress/memory-variable update, 4*(14+6L) = 104 B per cell at L=2)
+           stress/memory-variable update: 4*(14+6L) = 104 B per cell at L=2,
+           92 B with the uniform relaxation times of this workload)
            divided by its average launch time from HIP events on the
            context stream, against the 8 TB/s MI355X HBM peak.
 cpu_baseline = the FP32 CPU oracle (the reference algorithm restated in C,
@@ -34,11 +35,13 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "Gcell-updates/s and %HBM roofline per shot; batched shots/s + p50/p99"
 
 
-def bytes_per_cell(L):
+def bytes_per_cell(L, tau_uniform=False):
     """Algorithmic HBM bytes per cell-update: velocity sweep 36 B plus the
-    stress sweep (elastic 44 B; viscoelastic 4*(14+6L) B)."""
+    stress sweep (elastic 44 B; viscoelastic 4*(14+6L) B, of which 12 B are
+    the three relaxation-time planes that the kernel does not read when the
+    relaxation times are uniform -- then 4*(11+6L) B)."""
     vel = 36
-    stress = 44 if L == 0 else 4 * (14 + 6 * L)
+    stress = 44 if L == 0 else 4 * (14 + 6 * L) - (12 if tau_uniform else 0)
     return vel, stress
 
 
@@ -89,7 +92,8 @@ def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tiles=Non
     assert np.isfinite(seis).all()
     cells = p.nx * p.ny
     total = float(sum(walls))
-    vel_b, str_b = bytes_per_cell(L)
+    tau_uniform = L > 0 and all(np.all(m[k] == m[k].flat[0]) for k in ("taus", "taup"))
+    vel_b, str_b = bytes_per_cell(L, tau_uniform)
     s_avg = float(np.mean(s_ms))
     v_avg = float(np.mean(v_ms))
     out = {
@@ -101,12 +105,88 @@ def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tiles=Non
         "v_kernel_GBs": vel_b * cells / (v_avg * 1e-3) / 1e9 if v_avg > 0 else None,
         "s_kernel_GBs": str_b * cells / (s_avg * 1e-3) / 1e9 if s_avg > 0 else None,
         "step_GBs": (vel_b + str_b) * cells * p.nt * steps / total / 1e9,
-        "stress_bytes_per_cell": str_b, "vel_bytes_per_cell": vel_b,
+        "stress_bytes_per_cell": str_b, "vel_bytes_per_cell": vel_b, "tau_uniform": bool(tau_uniform),
         "footprint_GB": ctx.footprint() / 1e9,
         "tile": {"velocity": tile[0], "stress": tile[1]},
     }
     ctx.close()
     return out
+
+
+def batch_workload(nshots=64, seed=2024):
+    """The multi-shot workload: nshots independent elastic P/SV shots with
+    NX ~ U[500,3000], NY ~ U[200,800], DH = 10 m, FDORDER in {4, 8},
+    NT ~ U[1500,5000], DT = 1 ms, each with its own seeded smooth-random
+    model and a Ricker source (fc ~ U[5,20] Hz) at the centre 20 m deep,
+    receivers every 20 m at 20 m depth."""
+    import dnhip
+    import synth
+    rng = np.random.default_rng(seed)
+    shots = []
+    for s in range(nshots):
+        nx, ny = int(rng.integers(500, 3001)), int(rng.integers(200, 801))
+        fd = int(rng.choice([4, 8]))
+        nt = int(rng.integers(1500, 5001))
+        fc = float(rng.uniform(5.0, 20.0))
+        dh, dt = 10.0, 1e-3
+        vp, vs, rho = synth.layered_model(nx, ny, dh, seed=s + 1)
+        p = dnhip.Params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=fd, maxrelerror=1, fw=20, free_surf=1,
+                         L=0, seismo=1, ts=float(np.float32(1.0 / fc)), damping=1500.0, fpml=10.0)
+        src = synth.source_table(nx * dh / 2, 20.0, dh, fc=fc)
+        sig = synth.ricker(nt, dt, fc)[None, :]
+        nrec = (nx - 2) * int(dh) // 20
+        rec = synth.receiver_line(20.0, 20.0, nrec, 20.0, dh)
+        shots.append((p, (vp, vs, rho), src, sig, rec))
+    return shots
+
+
+def run_batched(nshots, graph_steps=32):
+    """shots/s of the multi-shot engine vs the same shots one after another
+    with the plain per-shot solver (dn_run_forward), on prepared contexts
+    (models, sources, receivers resident in HBM before timing)."""
+    import dnhip
+    shots = batch_workload(nshots)
+    cells = sum(p.nx * p.ny * p.nt for p, *_ in shots)
+
+    def prepare():
+        ctxs = []
+        for p, (vp, vs, rho), src, sig, rec in shots:
+            g = dnhip.Context(p)
+            g.set_model(vp, vs, rho)
+            g.set_sources(src, sig)
+            g.set_receivers(rec)
+            ctxs.append(g)
+        return ctxs
+
+    ctxs = prepare()
+    ctxs[0].forward(50)                   # load the kernels
+    t0 = time.perf_counter()
+    for g in ctxs:
+        g.forward()
+    t_seq = time.perf_counter() - t0
+    ref = [g.seismogram(0) for g in ctxs[:4]]
+    for g in ctxs:
+        g.close()
+    ctxs = prepare()
+    eng = dnhip.Engine()
+    eng.set_graph(graph_steps)
+    t0 = time.perf_counter()
+    for g in ctxs:
+        eng.attach(g)
+    eng.run(chunk=256)
+    t_bat = time.perf_counter() - t0
+    same = all(np.array_equal(r, g.seismogram(0)) for r, g in zip(ref, ctxs[:4]))
+    for g in ctxs:
+        g.close()
+    eng.close()
+    return {
+        "shots": nshots, "cell_updates": cells,
+        "sequential_s": t_seq, "batched_s": t_bat,
+        "sequential_shots_per_s": nshots / t_seq, "batched_shots_per_s": nshots / t_bat,
+        "speedup": t_seq / t_bat,
+        "batched_gcells_per_s": cells / t_bat / 1e9, "sequential_gcells_per_s": cells / t_seq / 1e9,
+        "graph_steps": graph_steps, "results_equal_single_shot": bool(same),
+    }
 
 
 def cpu_baseline(L, nsteps, threads):
@@ -153,6 +233,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--no-visco", action="store_true", help="profiling: elastic shot only")
+    ap.add_argument("--batch-shots", type=int, default=64, help="multi-shot workload size (0: skip)")
     ap.add_argument("--nt", type=int, default=6000, help="profiling: shorter shots")
     ap.add_argument("--tune-steps", type=int, default=40)
     ap.add_argument("--tile-v", default="", help="profiling: fixed velocity-sweep shape tbx,by,ty,vw")
@@ -181,6 +262,10 @@ def main():
     if not args.no_elastic:
         elastic = run_gpu(0, 1, 1, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles)
         print(f"# elastic: {elastic}", file=sys.stderr, flush=True)
+    batched = None
+    if args.batch_shots > 0:
+        batched = run_batched(args.batch_shots)
+        print(f"# batched: {batched}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -213,10 +298,11 @@ def main():
                    "parallelism": "single GPU, whole-grid domain"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
-                     "kernel": "viscoelastic stress + memory-variable update (FD8, L=2)",
+                     "kernel": "viscoelastic stress + memory-variable update (FD8, L=2, uniform tau)",
                      "bytes_per_cell": res["stress_bytes_per_cell"],
                      "avg_launch_ms": res["s_kernel_ms"]},
         "cpu_baseline": cpu,
+        "batched": batched,
         "detail": {"visco": res, "elastic": elastic,
                    "elastic_step_frac_of_hbm": (elastic["step_GBs"] / HBM_PEAK_GBS) if elastic else None,
                    "elastic_kernels_frac_of_hbm": (

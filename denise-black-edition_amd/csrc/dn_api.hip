@@ -14,11 +14,11 @@
 #include <utility>
 #include <vector>
 
-#include "dn_internal.h"
+#include "dn_ctx.h"
 
 using namespace dn;
 
-namespace {
+namespace dn {
 
 thread_local std::string g_err;
 
@@ -28,90 +28,7 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
-#define HIP_OK(expr)                                                                         \
-    do {                                                                                     \
-        hipError_t e_ = (expr);                                                              \
-        if (e_ != hipSuccess)                                                                \
-            return fail(DN_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));      \
-    } while (0)
-
-enum { F_VX = 0, F_VY, F_SXX, F_SYY, F_SXY, F_VXP1, F_VYP1, F_UX, F_UY, F_UXY, F_N };
-
-}  // namespace
-
-struct dn_ctx {
-    dn_params p{};
-    int device = 0;
-    hipStream_t stream = nullptr;
-    Layout lay;
-    HostSetup setup;
-    int fdoh = 0, nd = 0, ntdtinv = 0;
-    bool model_set = false, fwd_complete = false;
-    std::vector<std::pair<void *, size_t>> allocs;
-    size_t bytes = 0;
-    float *fld[F_N] = {};  // plane bases
-    float *rip = nullptr, *rjp = nullptr, *muipjp = nullptr, *f = nullptr, *g = nullptr;
-    float *vp = nullptr, *vs = nullptr, *rho = nullptr;
-    float *rmem = nullptr, *pmem = nullptr, *qmem = nullptr;
-    float *vmu = nullptr, *vts = nullptr, *vpi = nullptr, *vtp = nullptr, *vmuij = nullptr, *vtsij = nullptr;
-    float *fs_fjm = nullptr, *fs_g = nullptr, *fs_dme = nullptr;
-    float *psi[9] = {};
-    float *pml[P_NVEC] = {};
-    float *fwd[5] = {};
-    float *wc = nullptr, *wcu = nullptr, *wcr = nullptr, *gvp = nullptr, *gvs = nullptr, *grho = nullptr;
-    HostMaterial mat;
-    std::vector<float> h_vp, h_vs, h_rho;  // x-major model kept for receiver factors
-    DevAcq acq;
-    std::vector<int> rec_ix, rec_iy, src_ix, src_iy, src_type;
-    int qtype = 0, has_force = 0;
-    KArgs a{};
-    // tile shape of the sweeps and the per-tile cell work lists
-    // (0 forward velocity, 1 forward stress, 2 adjoint velocity, 3 adjoint stress)
-    Tile tile_v, tile_s;  // velocity / stress sweep shapes
-    int ntx_v = 0, ntiles_v = 0, ntx_s = 0, ntiles_s = 0;
-    int *lst_off[4] = {};
-    Ent *lst_ent[4] = {};
-    // timing
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    double run_ms = 0.0, v_ms = 0.0, s_ms = 0.0;
-    int sample_every = 0, samples = 0;
-    std::vector<hipEvent_t> pool;
-
-    int dalloc(void **ptr, size_t nbytes)
-    {
-        if (nbytes == 0) nbytes = 4;
-        hipError_t e = hipMalloc(ptr, nbytes);
-        if (e != hipSuccess) {
-            *ptr = nullptr;
-            return fail(DN_ERR_OOM, "hipMalloc of " + std::to_string(nbytes) + " bytes failed: " +
-                                        hipGetErrorString(e));
-        }
-        allocs.emplace_back(*ptr, nbytes);
-        bytes += nbytes;
-        (void)hipMemset(*ptr, 0, nbytes);
-        return DN_OK;
-    }
-    void dfree(void *ptr)
-    {
-        for (auto it = allocs.begin(); it != allocs.end(); ++it)
-            if (it->first == ptr) {
-                (void)hipFree(ptr);
-                bytes -= it->second;
-                allocs.erase(it);
-                return;
-            }
-    }
-    ~dn_ctx()
-    {
-        if (device >= 0) (void)hipSetDevice(device);
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (auto &pr : allocs) (void)hipFree(pr.first);
-        for (auto ev : pool) (void)hipEventDestroy(ev);
-        if (ev0) (void)hipEventDestroy(ev0);
-        if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
+}  // namespace dn
 
 namespace {
 
@@ -172,6 +89,10 @@ void fill_args(dn_ctx *c)
     a.vtp = at(c->vtp);
     a.vmuij = at(c->vmuij);
     a.vtsij = at(c->vtsij);
+    a.tau_const = c->tau_const;
+    a.ts0 = c->ts0;
+    a.tp0 = c->tp0;
+    a.tsij0 = c->tsij0;
     for (int l = 0; l <= MAXL; ++l) {
         a.peta[l] = c->mat.peta[l];
         a.bip[l] = c->mat.bip[l];
@@ -281,7 +202,9 @@ int build_lists(dn_ctx *c)
     return DN_OK;
 }
 
-StepCtl base_ctl(dn_ctx *c, int lv, int ls)
+}  // namespace
+
+StepCtl dn::base_ctl(dn_ctx *c, int lv, int ls)
 {
     StepCtl s{};
     s.vlist.off = c->lst_off[lv];
@@ -298,6 +221,8 @@ StepCtl base_ctl(dn_ctx *c, int lv, int ls)
     s.corr_k = -1;
     return s;
 }
+
+namespace {
 
 int validate(const dn_params *p)
 {
@@ -323,7 +248,9 @@ int validate(const dn_params *p)
     return DN_OK;
 }
 
-int zero_state(dn_ctx *c)
+}  // namespace
+
+int dn::zero_state(dn_ctx *c)
 {
     const size_t pb = c->lay.plane() * sizeof(float);
     for (int k = 0; k < F_N; ++k)
@@ -338,6 +265,8 @@ int zero_state(dn_ctx *c)
     HIP_OK(hipMemsetAsync(c->acq.nan_step, 0, sizeof(int), c->stream));
     return DN_OK;
 }
+
+namespace {
 
 hipEvent_t pool_event(dn_ctx *c, size_t k)
 {
@@ -415,6 +344,41 @@ int dn_device_count(int *count)
     if (e != hipSuccess) n = 0;
     if (count) *count = n;
     return e == hipSuccess ? DN_OK : fail(DN_ERR_HIP, hipGetErrorString(e));
+}
+
+// Device bytes a context with these parameters, nsrc sources and ntr
+// receivers holds: the same allocations as dn_ctx_create, dn_set_sources and
+// dn_set_receivers (cell lists aside, which are a few bytes per point).
+static size_t plan_bytes(const dn_params &p, int nsrc, int ntr)
+{
+    Layout lay;
+    lay.init(p.nx, p.ny);
+    const size_t pb = lay.plane() * sizeof(float);
+    const bool gf2 = p.mode > 0 && p.grad_form == 2;
+    size_t b = pb * (F_N - (gf2 ? 0 : 3));
+    b += 2 * pb;                                  // rip, rjp
+    if (p.L == 0) b += 3 * pb;                    // muipjp, f, g
+    if (p.mode > 0) b += 3 * pb;                  // vp, vs, rho
+    if (p.L > 0) b += 3 * pb * p.L + 6 * pb + 3 * sizeof(float) * (p.nx + 1);
+    b += 5 * ((size_t)p.ny * 2 * p.fw * sizeof(float) + 4) + 4 * ((size_t)2 * p.fw * p.nx * sizeof(float) + 4);
+    b += P_NVEC * sizeof(float) * (2 * p.fw + 1);
+    if (p.mode > 0) {
+        const size_t ntd = (size_t)std::ceil((float)p.nt / (float)p.dtinv);
+        b += 5 * (size_t)p.nx * p.ny * ntd * sizeof(float) + 6 * (size_t)p.nx * p.ny * sizeof(float);
+    }
+    b += sizeof(int) + sizeof(double);
+    b += (size_t)nsrc * (3 * sizeof(int) + 2 * sizeof(double)) + (size_t)nsrc * p.nt * sizeof(float);
+    b += (size_t)ntr * (2 * sizeof(int) + 2 * sizeof(double)) + 9 * (size_t)ntr * p.nt * sizeof(float);
+    return b;
+}
+
+int dn_estimate_footprint(const dn_params *p, int nsrc, int ntr, size_t *bytes)
+{
+    if (!p || !bytes || nsrc < 0 || ntr < 0) return fail(DN_ERR_ARG, "bad footprint request");
+    int rc = validate(p);
+    if (rc) return rc;
+    *bytes = plan_bytes(*p, nsrc, ntr);
+    return DN_OK;
 }
 
 int dn_ctx_create(const dn_params *p, int device, dn_ctx **out)
@@ -551,6 +515,20 @@ int dn_set_model(dn_ctx *c, const float *vp, const float *vs, const float *rho, 
             return rc;
     }
     HIP_OK(hipStreamSynchronize(c->stream));
+    // uniform relaxation times over the physical grid: pass them as scalars
+    c->tau_const = 0;
+    if (c->p.L > 0) {
+        auto uniform = [&](const std::vector<float> &v, float &val) {
+            val = v[c->lay.origin()];
+            for (int j = 1; j <= c->p.ny; ++j) {
+                const float *row = v.data() + c->lay.origin() + (size_t)(j - 1) * c->lay.pitch;
+                for (int i = 0; i < c->p.nx; ++i)
+                    if (std::memcmp(&row[i], &val, sizeof(float)) != 0) return false;
+            }
+            return true;
+        };
+        c->tau_const = uniform(c->mat.vts, c->ts0) && uniform(c->mat.vtp, c->tp0) && uniform(c->mat.vtsij, c->tsij0);
+    }
     const size_t n = (size_t)c->p.nx * c->p.ny;
     c->h_vp.assign(vp, vp + n);
     c->h_vs.assign(vs, vs + n);

@@ -88,6 +88,11 @@ struct KArgs {
     // viscoelastic material and memory variables (L planes each)
     const float *vmu, *vts, *vpi, *vtp, *vmuij, *vtsij;
     float *rmem, *pmem, *qmem;
+    // relaxation times uniform over the grid (the reference's model reader
+    // sets Q = 30 everywhere): the update takes them from ts0/tp0/tsij0
+    // instead of reading the vts/vtp/vtsij planes
+    int tau_const;
+    float ts0, tp0, tsij0;
     float peta[MAXL + 1];
     float bip[MAXL + 1], cip[MAXL + 1], bjm[MAXL + 1], cjm[MAXL + 1];
     const float *fs_fjm, *fs_g, *fs_dme;
@@ -140,6 +145,13 @@ struct Tile {
     int nty(int ny) const { return (ny + rows() - 1) / rows(); }
 };
 
+// The per-step part of the sweep control (time step, snapshot slots,
+// vxp1/vyp1 output), passed separately so that a kernel can derive it from a
+// device-side step counter.
+struct StepDyn {
+    int nt, store_k, corr_k, write_vp1;
+};
+
 // Per-step control of the fused sweeps.
 struct StepCtl {
     int nt;          // time step (1..NT)
@@ -156,6 +168,26 @@ struct StepCtl {
     float *fwd[5];   // snapshots (compact NX*NY per slot)
     float *wc, *wcu, *wcr;   // correlation accumulators (compact, [j][i])
 };
+
+// One shot of a multi-shot launch: its sweep arguments, the static sweep
+// control of both sweeps (cell lists, tile grids) and its own device step
+// counter; the per-step fields are derived in the kernel from the counter.
+struct BatchShot {
+    KArgs a;
+    StepCtl cv, cs;
+    DevAcq q;
+    int *step;        // 1..nt_total: next step to run; nt_total+1: done
+    int blk0_v, blk0_s;   // first block of the shot in its group's launches
+    int nt_total, dtinv, fwdstore, vp1_form, sw;
+};
+
+// multi-shot launchers (dn_batch_*.hip)
+bool batch_tile_supported(const Tile &t, int sweep);
+void launch_batch_vel(const BatchShot *shots, const int *blk_shot, int nblocks, const Tile &t, int fdoh, int sw,
+                      void *stream);
+void launch_batch_str(const BatchShot *shots, const int *blk_shot, int nblocks, const Tile &t, int fdoh, int L,
+                      void *stream);
+void launch_batch_advance(const BatchShot *shots, int n, void *stream);
 
 // kernel launchers (dn_kernels.hip); all enqueue on `stream`
 void launch_vel(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, int sw,

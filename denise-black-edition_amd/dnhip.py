@@ -123,6 +123,14 @@ def load():
         "dn_autotune_tile": ([vp, C.c_int, ip, ip], C.c_int),
         "dn_get_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), ip], C.c_int),
         "dn_get_footprint": ([vp, C.POINTER(C.c_size_t)], C.c_int),
+        "dn_estimate_footprint": ([C.POINTER(_CParams), C.c_int, C.c_int, C.POINTER(C.c_size_t)], C.c_int),
+        "dn_engine_create": ([C.c_int, C.POINTER(vp)], C.c_int),
+        "dn_engine_destroy": ([vp], C.c_int),
+        "dn_engine_attach": ([vp, vp, C.c_int], C.c_int),
+        "dn_engine_step": ([vp, C.c_int, ip], C.c_int),
+        "dn_engine_status": ([vp, vp, ip], C.c_int),
+        "dn_engine_forget": ([vp, vp], C.c_int),
+        "dn_engine_set_graph": ([vp, C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -147,6 +155,15 @@ def _ptr(a, t=C.c_float):
 def _check(rc):
     if rc != DN_OK:
         raise DnError(rc, load().dn_last_error().decode(errors="replace"))
+
+
+def estimate_footprint(params: "Params", nsrc: int, ntr: int) -> int:
+    """Device bytes a context of these parameters will hold (no device
+    needed)."""
+    b = C.c_size_t()
+    cp = params.to_c()
+    _check(load().dn_estimate_footprint(C.byref(cp), nsrc, ntr, C.byref(b)))
+    return b.value
 
 
 def device_count():
@@ -296,3 +313,54 @@ class Context:
         b = C.c_size_t()
         _check(load().dn_get_footprint(self.h, C.byref(b)))
         return b.value
+
+
+class Engine:
+    """Multi-shot engine (dn_engine): advances many prepared contexts
+    together, one launch per sweep and shot group per time step."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(load().dn_engine_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().dn_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def attach(self, ctx: Context, phase: int = 0):
+        _check(load().dn_engine_attach(self.h, ctx.h, phase))
+
+    def step(self, nsteps: int) -> int:
+        n = C.c_int()
+        _check(load().dn_engine_step(self.h, nsteps, C.byref(n)))
+        return n.value
+
+    def run(self, chunk: int = 200) -> None:
+        """Step until no shot is attached."""
+        while self.step(chunk) > 0:
+            pass
+
+    def status(self, ctx: Context) -> int:
+        st = C.c_int()
+        _check(load().dn_engine_status(self.h, ctx.h, C.byref(st)))
+        return st.value
+
+    def forget(self, ctx: Context) -> None:
+        _check(load().dn_engine_forget(self.h, ctx.h))
+
+    def set_graph(self, steps: int) -> None:
+        _check(load().dn_engine_set_graph(self.h, steps))

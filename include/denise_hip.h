@@ -193,6 +193,40 @@ int dn_get_timing(dn_ctx *ctx, double *run_ms, double *v_kernel_ms, double *s_ke
 /* Device bytes held by the context. */
 int dn_get_footprint(dn_ctx *ctx, size_t *bytes);
 
+/* Device bytes a context with these parameters, nsrc sources and ntr
+ * receivers will hold (wavefields, material, CPML, seismograms and, for
+ * params.mode > 0, the forward snapshots), computed without a device: the
+ * admission test of a scheduler that packs shots into HBM. */
+int dn_estimate_footprint(const dn_params *p, int nsrc, int ntr, size_t *bytes);
+
+/* ---- Multi-shot engine ----------------------------------------------------
+ * No reference counterpart: the reference runs the shots of one process one
+ * after another (src/PSV/FD_PSV.c:271-354).  The engine advances many
+ * prepared contexts (model, sources and receivers set) together: each time
+ * step is one launch per sweep for all shots of the same FD order,
+ * relaxation count, direction and tile shape, so shots of any grid size and
+ * length fill the GPU together.  Results equal dn_run_forward /
+ * dn_run_adjoint of each context alone.  While attached, a context must not
+ * be used through the other entry points.  Moment-tensor shots (type 5) are
+ * not accepted. */
+typedef struct dn_engine dn_engine;
+int dn_engine_create(int device, dn_engine **out);
+int dn_engine_destroy(dn_engine *e);
+/* phase 0: forward run (stores snapshots when params.mode > 0); phase 1:
+ * adjoint run with correlation, after a finished phase 0 and
+ * dn_calc_residuals (or dn_set_adjoint_sources). */
+int dn_engine_attach(dn_engine *e, dn_ctx *ctx, int phase);
+/* Advance every attached shot by up to nsteps time steps; shots that reach
+ * their NT are detached.  *running (may be NULL) = shots still attached. */
+int dn_engine_step(dn_engine *e, int nsteps, int *running);
+/* 1 running, 0 finished, DN_ERR_UNSTABLE if its NaN probe tripped. */
+int dn_engine_status(dn_engine *e, dn_ctx *ctx, int *status);
+/* Drop the engine's record of a finished context (before destroying it). */
+int dn_engine_forget(dn_engine *e, dn_ctx *ctx);
+/* Replay runs of `steps` time steps from a captured hipGraph (0: plain
+ * launches).  The graph is re-captured when the set of shots changes. */
+int dn_engine_set_graph(dn_engine *e, int steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -117,6 +117,30 @@ def test_forward_viscoelastic(L, fdorder):
     check_forward(case, f"visc-L{L}-fd{fdorder}")
 
 
+@pytest.mark.parametrize("tile", [(64, 4, 1, 1), (64, 4, 1, 4)])
+def test_forward_viscoelastic_varying_tau(tile):
+    """Spatially varying relaxation times (the planes are read per cell);
+    uniform ones are passed as scalars -- both must give the oracle."""
+    case = make_case(nx=90, ny=64, nt=250, fdorder=8, L=2)
+    p, m, src, sig, rec = case
+    rng = np.random.default_rng(4)
+    m = dict(m)
+    m["taus"] = (0.04 * (1 + 0.5 * rng.random((p.nx, p.ny)))).astype(np.float32)
+    m["taup"] = (0.03 * (1 + 0.5 * rng.random((p.nx, p.ny)))).astype(np.float32)
+    case = (p, m, src, sig, rec)
+    o = run_oracle(case)
+    g = dnhip.Context(p)
+    g.set_tile(*tile)
+    g.set_model(m["vp"], m["vs"], m["rho"], m["taus"], m["taup"])
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    g.forward()
+    for comp in (0, 1):
+        assert np.array_equal(g.seismogram(comp), o.seismogram(comp).astype(np.float32)), comp
+    for fid, name in FIELDS:
+        assert np.array_equal(g.field(fid), o.field(fid).astype(np.float32)), name
+
+
 def test_forward_viscoelastic_no_free_surface():
     case = make_case(nx=80, ny=64, nt=250, fdorder=8, L=2, free_surf=0, src_xz=(400.0, 300.0))
     check_forward(case, "visc-nofs")
