@@ -22,6 +22,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -129,7 +130,7 @@ def batch_workload(nshots=64, seed=2024):
         nt = int(rng.integers(1500, 5001))
         fc = float(rng.uniform(5.0, 20.0))
         dh, dt = 10.0, 1e-3
-        vp, vs, rho = synth.layered_model(nx, ny, dh, seed=s + 1)
+        vp, vs, rho = synth.quick_layered_model(nx, ny, dh, seed=s + 1)
         p = dnhip.Params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=fd, maxrelerror=1, fw=20, free_surf=1,
                          L=0, seismo=1, ts=float(np.float32(1.0 / fc)), damping=1500.0, fpml=10.0)
         src = synth.source_table(nx * dh / 2, 20.0, dh, fc=fc)
@@ -189,6 +190,71 @@ def run_batched(nshots, graph_steps=32):
     }
 
 
+def serving_workload(nreq, qps, seed=77):
+    """Poisson arrivals at `qps`: 70% elastic forward modelling, 30% FWI
+    gradients (forward with snapshots every 2nd step + adjoint); grids
+    log-uniform from 256x256 to 4096x1024 (NX, NY drawn independently), NT
+    uniform in [1000, 8000], DH = 10 m, DT = 1 ms, FD8, seeded smooth-random
+    models, Ricker source (fc 10 Hz) at the centre, receivers every 20 m.
+    Observed data of a gradient job are zero traces (the work of a gradient
+    does not depend on them)."""
+    import dnhip
+    import serve
+    import synth
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    reqs = []
+    for r in range(nreq):
+        t += float(rng.exponential(1.0 / qps))
+        nx = int(round(math.exp(rng.uniform(math.log(256), math.log(4096)))))
+        ny = int(round(math.exp(rng.uniform(math.log(256), math.log(1024)))))
+        nt = int(rng.integers(1000, 8001))
+        kind = "gradient" if rng.random() < 0.3 else "forward"
+        dh, dt, fc = 10.0, 1e-3, 10.0
+        vp, vs, rho = synth.quick_layered_model(nx, ny, dh, seed=1000 + r)
+        p = dnhip.Params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=8, fw=20, free_surf=1, seismo=1,
+                         mode=1 if kind == "gradient" else 0, dtinv=2, ts=float(np.float32(1.0 / fc)))
+        src = synth.source_table(nx * dh / 2, 20.0, dh, fc=fc)
+        sig = synth.ricker(nt, dt, fc)[None, :]
+        nrec = (nx - 2) * int(dh) // 20
+        rec = synth.receiver_line(20.0, 20.0, nrec, 20.0, dh)
+        obs = ()
+        if kind == "gradient":
+            z = np.zeros((nrec, nt), np.float32)
+            obs = (z, z, None)
+        reqs.append(serve.Request(rid=r, kind=kind, params=p, model=(vp, vs, rho), srcpos=src, signals=sig,
+                                  recpos=rec, obs=obs, arrival=t))
+    return reqs
+
+
+def run_serving(nreq, qps):
+    """p50/p99 latency and throughput of the request server, and the same
+    requests one at a time with the plain solver (no overlap)."""
+    import serve
+    reqs = serving_workload(nreq, qps)
+    srv = serve.Server()
+    stats = srv.serve(reqs)
+    srv.close()
+    # the plain one-at-a-time solver on the same stream: each request starts
+    # when it has arrived and the previous one is done
+    t_free, lat, t0 = 0.0, [], time.perf_counter()
+    done = 0
+    for r in sorted(reqs, key=lambda q: q.arrival):
+        if r.error:
+            continue
+        now = time.perf_counter() - t0
+        if now < r.arrival:
+            time.sleep(r.arrival - now)
+        serve.solve_alone(r)
+        lat.append(time.perf_counter() - t0 - r.arrival)
+        done += 1
+    wall = time.perf_counter() - t0
+    stats["plain"] = {"shots_per_s": done / wall, "latency_p50_s": float(np.percentile(lat, 50)),
+                      "latency_p99_s": float(np.percentile(lat, 99)), "wall_s": wall}
+    stats["qps"] = qps
+    return stats
+
+
 def cpu_baseline(L, nsteps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
@@ -234,6 +300,8 @@ def main():
     ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--no-visco", action="store_true", help="profiling: elastic shot only")
     ap.add_argument("--batch-shots", type=int, default=64, help="multi-shot workload size (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=120, help="serving workload size (0: skip)")
+    ap.add_argument("--serve-qps", type=float, default=8.0)
     ap.add_argument("--nt", type=int, default=6000, help="profiling: shorter shots")
     ap.add_argument("--tune-steps", type=int, default=40)
     ap.add_argument("--tile-v", default="", help="profiling: fixed velocity-sweep shape tbx,by,ty,vw")
@@ -266,6 +334,10 @@ def main():
     if args.batch_shots > 0:
         batched = run_batched(args.batch_shots)
         print(f"# batched: {batched}", file=sys.stderr, flush=True)
+    serving = None
+    if args.serve_requests > 0:
+        serving = run_serving(args.serve_requests, args.serve_qps)
+        print(f"# serving: {serving}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -303,6 +375,7 @@ def main():
                      "avg_launch_ms": res["s_kernel_ms"]},
         "cpu_baseline": cpu,
         "batched": batched,
+        "serving": serving,
         "detail": {"visco": res, "elastic": elastic,
                    "elastic_step_frac_of_hbm": (elastic["step_GBs"] / HBM_PEAK_GBS) if elastic else None,
                    "elastic_kernels_frac_of_hbm": (

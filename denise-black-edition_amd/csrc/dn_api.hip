@@ -372,6 +372,17 @@ static size_t plan_bytes(const dn_params &p, int nsrc, int ntr)
     return b;
 }
 
+int dn_device_memory(int device, size_t *free_bytes, size_t *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return fail(DN_ERR_ARG, "null output pointer");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return fail(DN_ERR_ARG, "no such HIP device");
+    HIP_OK(hipSetDevice(device));
+    HIP_OK(hipMemGetInfo(free_bytes, total_bytes));
+    return DN_OK;
+}
+
 int dn_estimate_footprint(const dn_params *p, int nsrc, int ntr, size_t *bytes)
 {
     if (!p || !bytes || nsrc < 0 || ntr < 0) return fail(DN_ERR_ARG, "bad footprint request");

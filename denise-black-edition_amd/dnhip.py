@@ -124,6 +124,7 @@ def load():
         "dn_get_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), ip], C.c_int),
         "dn_get_footprint": ([vp, C.POINTER(C.c_size_t)], C.c_int),
         "dn_estimate_footprint": ([C.POINTER(_CParams), C.c_int, C.c_int, C.POINTER(C.c_size_t)], C.c_int),
+        "dn_device_memory": ([C.c_int, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
         "dn_engine_create": ([C.c_int, C.POINTER(vp)], C.c_int),
         "dn_engine_destroy": ([vp], C.c_int),
         "dn_engine_attach": ([vp, vp, C.c_int], C.c_int),
@@ -164,6 +165,13 @@ def estimate_footprint(params: "Params", nsrc: int, ntr: int) -> int:
     cp = params.to_c()
     _check(load().dn_estimate_footprint(C.byref(cp), nsrc, ntr, C.byref(b)))
     return b.value
+
+
+def device_memory(device: int = 0):
+    """(free, total) device bytes."""
+    f, t = C.c_size_t(), C.c_size_t()
+    _check(load().dn_device_memory(device, C.byref(f), C.byref(t)))
+    return f.value, t.value
 
 
 def device_count():

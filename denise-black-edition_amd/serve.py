@@ -1,0 +1,169 @@
+"""Request server for P/SV modelling and FWI-gradient jobs on one GPU.
+
+Continuous batching over the multi-shot engine (dn_engine): requests that
+arrive while others run are admitted between engine steps as soon as their
+device footprint (wavefields, CPML memory, seismograms and, for gradients,
+the stored forward snapshots -- dn_estimate_footprint) fits in the HBM
+budget; admitted shots keep their state on the device and advance together
+with the running ones.  A gradient job runs its forward phase, has its
+residuals formed on the device, then rejoins the engine for its adjoint
+phase.  A request whose footprint exceeds the whole budget is rejected.
+
+The host loop only admits, steps and collects; every time step of every
+shot is GPU work.  Latency = completion time - arrival time.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+import dnhip
+
+
+@dataclass
+class Request:
+    """One job: kind "forward" (seismograms) or "gradient" (vp/vs/rho
+    gradient of the L2 misfit against obs)."""
+    rid: int
+    kind: str
+    params: dnhip.Params
+    model: tuple                # (vp, vs, rho[, taus, taup]) x-major float32
+    srcpos: np.ndarray          # (8, nsrc)
+    signals: np.ndarray         # (nsrc, NT)
+    recpos: np.ndarray          # (2, ntr)
+    obs: tuple = ()             # gradient: (obs_vx, obs_vy, obs_p or None), ntr x NT
+    arrival: float = 0.0        # seconds after the start of the run
+    # filled in by the server
+    result: dict = field(default_factory=dict)
+    admitted: float = -1.0
+    done: float = -1.0
+    error: str = ""
+
+
+class Server:
+    def __init__(self, device: int = 0, budget_bytes: int | None = None, chunk: int = 64,
+                 graph_steps: int = 32, reserve_bytes: int = 2 << 30):
+        self.device = device
+        free, total = dnhip.device_memory(device)
+        self.budget = budget_bytes if budget_bytes is not None else max(0, free - reserve_bytes)
+        self.used = 0
+        self.chunk = chunk
+        self.engine = dnhip.Engine(device)
+        self.engine.set_graph(graph_steps)
+        self.running = {}          # Context -> (Request, phase, bytes)
+
+    @staticmethod
+    def footprint(req: Request) -> int:
+        return dnhip.estimate_footprint(req.params, req.srcpos.shape[1], req.recpos.shape[1])
+
+    def _admit(self, req: Request, now: float) -> None:
+        p = req.params
+        if req.kind == "gradient" and p.mode == 0:
+            raise ValueError("gradient requests need params.mode = 1")
+        ctx = dnhip.Context(p, self.device)
+        ctx.set_model(*req.model)
+        ctx.set_sources(req.srcpos, req.signals)
+        ctx.set_receivers(req.recpos)
+        nbytes = ctx.footprint()
+        self.engine.attach(ctx, 0)
+        self.used += nbytes
+        self.running[ctx] = (req, 0, nbytes)
+        req.admitted = now
+
+    def _finish(self, ctx, now: float) -> None:
+        req, phase, nbytes = self.running.pop(ctx)
+        st = self.engine.status(ctx)
+        if st != 0:
+            req.error = f"shot failed with status {st}"
+        elif req.kind == "gradient" and phase == 0:
+            ox, oy, op = (list(req.obs) + [None, None, None])[:3]
+            req.result["l2"] = ctx.residuals(ox, oy, op, 1)
+            self.engine.forget(ctx)
+            self.engine.attach(ctx, 1)
+            self.running[ctx] = (req, 1, nbytes)
+            return
+        elif req.kind == "gradient":
+            ctx.assemble()
+            req.result["gradient"] = [ctx.gradient(w) for w in (3, 4, 5)]
+        else:
+            comps = {1: (0, 1), 2: (2,), 3: (3, 4), 4: (0, 1, 2, 3, 4)}.get(req.params.seismo, ())
+            req.result["seismograms"] = {c: ctx.seismogram(c) for c in comps}
+        req.done = now
+        self.engine.forget(ctx)
+        ctx.close()
+        self.used -= nbytes
+
+    def serve(self, requests: list[Request], clock=time.perf_counter) -> dict:
+        """Run a request stream (arrival times relative to the start) to
+        completion; returns latency and throughput statistics."""
+        todo = sorted(requests, key=lambda r: r.arrival)
+        waiting: list[Request] = []
+        t0 = clock()
+        k = 0
+        cells = 0
+        while k < len(todo) or waiting or self.running:
+            now = clock() - t0
+            while k < len(todo) and todo[k].arrival <= now:
+                waiting.append(todo[k])
+                k += 1
+            # admission by footprint, oldest first; a request that can never
+            # fit is rejected, one that does not fit now waits
+            still = []
+            for req in waiting:
+                need = self.footprint(req)
+                if need > self.budget:
+                    req.error = f"footprint {need} B exceeds the budget {self.budget} B"
+                    req.done = now
+                elif self.used + need <= self.budget:
+                    self._admit(req, now)
+                    n = req.params.nx * req.params.ny * req.params.nt
+                    cells += n * (2 if req.kind == "gradient" else 1)
+                else:
+                    still.append(req)
+            waiting = still
+            if self.running:
+                self.engine.step(self.chunk)
+                now = clock() - t0
+                for ctx in [c for c in self.running if self.engine.status(c) != 1]:
+                    self._finish(ctx, now)
+            elif k < len(todo):
+                time.sleep(max(0.0, min(0.01, todo[k].arrival - (clock() - t0))))
+        wall = clock() - t0
+        ok = [r for r in requests if not r.error]
+        lat = np.array([r.done - r.arrival for r in ok]) if ok else np.zeros(1)
+        return {
+            "requests": len(requests), "completed": len(ok), "rejected": len(requests) - len(ok),
+            "wall_s": wall, "shots_per_s": len(ok) / wall if wall > 0 else 0.0,
+            "gcells_per_s": cells / wall / 1e9 if wall > 0 else 0.0,
+            "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+            "latency_mean_s": float(lat.mean()),
+        }
+
+    def close(self):
+        for ctx in list(self.running):
+            ctx.close()
+        self.running.clear()
+        self.engine.close()
+
+
+def solve_alone(req: Request, device: int = 0) -> dict:
+    """The same job with the plain one-shot solver (reference for the mix)."""
+    ctx = dnhip.Context(req.params, device)
+    ctx.set_model(*req.model)
+    ctx.set_sources(req.srcpos, req.signals)
+    ctx.set_receivers(req.recpos)
+    ctx.forward()
+    out = {}
+    if req.kind == "gradient":
+        ox, oy, op = (list(req.obs) + [None, None, None])[:3]
+        out["l2"] = ctx.residuals(ox, oy, op, 1)
+        ctx.adjoint()
+        ctx.assemble()
+        out["gradient"] = [ctx.gradient(w) for w in (3, 4, 5)]
+    else:
+        comps = {1: (0, 1), 2: (2,), 3: (3, 4), 4: (0, 1, 2, 3, 4)}.get(req.params.seismo, ())
+        out["seismograms"] = {c: ctx.seismogram(c) for c in comps}
+    ctx.close()
+    return out

@@ -49,6 +49,27 @@ def layered_model(nx, ny, dh, seed, vtop=1500.0, vbot=4000.0, pert=0.05, sigma=8
     return vp.astype(np.float32), vs.astype(np.float32), rho.astype(np.float32)
 
 
+def quick_layered_model(nx, ny, dh, seed, vtop=1500.0, vbot=4000.0, pert=0.05, sigma=8.0, factor=4):
+    """layered_model with the random field drawn on a grid `factor` times
+    coarser (smoothing sigma/factor there) and interpolated linearly: the
+    same kind of seeded smooth-random model at a fraction of the cost, for
+    workloads of many large models."""
+    from scipy.ndimage import zoom
+    rng = np.random.default_rng(seed)
+    cx, cy = -(-nx // factor) + 2, -(-ny // factor) + 2
+    noise = rng.standard_normal((cx, cy))
+    p = gaussian_smooth(noise, sigma / factor)
+    p = zoom(p, (nx / cx, ny / cy), order=1)[:nx, :ny]
+    p /= max(np.abs(p).max(), 1e-30)
+    z = np.arange(ny, dtype=np.float64) * dh
+    zmax = max((ny - 1) * dh, dh)
+    vp0 = vtop + (vbot - vtop) * z / zmax
+    vp = vp0[None, :] * (1.0 + pert * p)
+    vs = vp / math.sqrt(3.0)
+    rho = 310.0 * vp ** 0.25
+    return vp.astype(np.float32), vs.astype(np.float32), rho.astype(np.float32)
+
+
 def anomaly(vp, vs, rho, dh, centre_x, centre_z, radius=300.0, amp=0.10):
     """Multiply a model by (1 + amp*G(r; radius)), G a unit Gaussian bump."""
     nx, ny = vp.shape

@@ -193,6 +193,9 @@ int dn_get_timing(dn_ctx *ctx, double *run_ms, double *v_kernel_ms, double *s_ke
 /* Device bytes held by the context. */
 int dn_get_footprint(dn_ctx *ctx, size_t *bytes);
 
+/* Free and total device memory of a HIP device (hipMemGetInfo). */
+int dn_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
+
 /* Device bytes a context with these parameters, nsrc sources and ntr
  * receivers will hold (wavefields, material, CPML, seismograms and, for
  * params.mode > 0, the forward snapshots), computed without a device: the
