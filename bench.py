@@ -270,6 +270,25 @@ def cpu_baseline(L, nsteps, threads):
     return p.nx * p.ny * nsteps / dt / 1e9, dt
 
 
+def profiled_traffic(workload, sweep, tile):
+    """HBM bytes per launch of a sweep kernel at this tile shape, as measured
+    by the FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/
+    (lines "traffic_bytes_per_launch <workload> <sweep> <tile> = <bytes>");
+    None when no profile covers it."""
+    key = f"traffic_bytes_per_launch {workload} {sweep} {','.join(map(str, tile))} ="
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".md"):
+            continue
+        for line in open(os.path.join(pdir, name)):
+            line = line.strip().strip("`")
+            if line.startswith(key):
+                return float(line[len(key):].strip())
+    return None
+
+
 def calibrate_copy(mib, reps):
     """A plain device copy of known size, for scaling the HBM byte counters
     (FETCH_SIZE / WRITE_SIZE) of a profiler run."""
@@ -369,7 +388,8 @@ def main():
                    "nx": 4000, "ny": 1200, "nt": args.nt, "tile": res["tile"], "L": 2, "fdorder": 8,
                    "parallelism": "single GPU, whole-grid domain"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                     "traffic": profiled_traffic("visco", "stress", res["tile"]["stress"]),
                      "kernel": "viscoelastic stress + memory-variable update (FD8, L=2, uniform tau)",
                      "bytes_per_cell": res["stress_bytes_per_cell"],
                      "avg_launch_ms": res["s_kernel_ms"]},
