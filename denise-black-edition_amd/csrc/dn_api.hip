@@ -716,6 +716,22 @@ int dn_autotune_tile(dn_ctx *c, int nsteps, int *vel_shape, int *str_shape)
             best_s = shapes[k];
         }
     }
+    if (!rc) {
+        // second pass: plain or streaming stores, per sweep, at the best shapes
+        c->tile_v = best_v;
+        c->tile_s = best_s;
+        rc = build_lists(c);
+        double tv[2] = {0, 0}, ts[2] = {0, 0};
+        for (int h = 0; h < 2 && !rc; ++h) {
+            c->nt_v = c->nt_s = h;
+            rc = dn_run_forward(c, nsteps);
+            if (!rc) rc = dn_run_forward(c, nsteps);
+            tv[h] = c->v_ms;
+            ts[h] = c->s_ms;
+        }
+        c->nt_v = tv[1] < tv[0];
+        c->nt_s = ts[1] < ts[0];
+    }
     c->sample_every = keep_sampling;
     if (rc) return rc;
     c->tile_v = best_v;
@@ -732,6 +748,22 @@ int dn_autotune_tile(dn_ctx *c, int nsteps, int *vel_shape, int *str_shape)
             out[w][2] = bt[w]->ty;
             out[w][3] = bt[w]->vw;
         }
+    return DN_OK;
+}
+
+int dn_set_store_hint(dn_ctx *c, int sweep, int streaming)
+{
+    if (!c || sweep < -1 || sweep > 1 || streaming < 0 || streaming > 1)
+        return fail(DN_ERR_ARG, "bad store hint");
+    if (sweep != 1) c->nt_v = streaming;
+    if (sweep != 0) c->nt_s = streaming;
+    return DN_OK;
+}
+
+int dn_get_store_hint(dn_ctx *c, int sweep, int *streaming)
+{
+    if (!c || !streaming || sweep < 0 || sweep > 1) return fail(DN_ERR_ARG, "bad store hint request");
+    *streaming = sweep == 0 ? c->nt_v : c->nt_s;
     return DN_OK;
 }
 
@@ -752,7 +784,9 @@ int dn_run_forward(dn_ctx *c, int nsteps)
     int rc = zero_state(c);
     if (rc) return rc;
     c->fwd_complete = false;
-    const KArgs &a = c->a;
+    KArgs av = c->a, as = c->a;
+    av.ntstore = c->nt_v;
+    as.ntstore = c->nt_s;
     const int fwdstore = c->p.mode > 0;
     const int vp1_form = c->p.grad_form == 1 || c->p.grad_form == 2;
     StepCtl ctl = base_ctl(c, 0, 1);
@@ -774,18 +808,18 @@ int dn_run_forward(dn_ctx *c, int nsteps)
             if (!e0 || !e1 || !e2 || !pool_event(c, nev + 3)) return fail(DN_ERR_HIP, "event creation failed");
             HIP_OK(hipEventRecord(e0, c->stream));
         }
-        launch_vel(a, ctl, c->acq, c->tile_v, c->fdoh, 0, c->stream);
+        launch_vel(av, ctl, c->acq, c->tile_v, c->fdoh, 0, c->stream);
         if (sample) {
             HIP_OK(hipEventRecord(e1, c->stream));
             HIP_OK(hipEventRecord(c->pool[nev + 2], c->stream));
         }
-        launch_str(a, ctl, c->acq, c->tile_s, c->fdoh, c->stream);
+        launch_str(as, ctl, c->acq, c->tile_s, c->fdoh, c->stream);
         if (sample) {
             HIP_OK(hipEventRecord(c->pool[nev + 3], c->stream));
             nev += 4;
         }
-        if (c->qtype == 5 && c->acq.nsrc > 0) launch_msource(a, c->acq, nt, c->stream);
-        if (store && c->qtype == 5) launch_store(a, c->fwd, hin - 1, c->stream);
+        if (c->qtype == 5 && c->acq.nsrc > 0) launch_msource(c->a, c->acq, nt, c->stream);
+        if (store && c->qtype == 5) launch_store(c->a, c->fwd, hin - 1, c->stream);
         if (store) {
             ++hin;
             hin1 += c->p.dtinv;
@@ -875,7 +909,9 @@ int dn_run_adjoint(dn_ctx *c, int nsteps)
     HIP_OK(hipMemsetAsync(c->wc, 0, g, c->stream));
     HIP_OK(hipMemsetAsync(c->wcu, 0, g, c->stream));
     HIP_OK(hipMemsetAsync(c->wcr, 0, g, c->stream));
-    const KArgs &a = c->a;
+    KArgs av = c->a, as = c->a;
+    av.ntstore = c->nt_v;
+    as.ntstore = c->nt_s;
     StepCtl ctl = base_ctl(c, 2, 3);
     ctl.probe = 1;
     size_t nev = 0;
@@ -892,12 +928,12 @@ int dn_run_adjoint(dn_ctx *c, int nsteps)
                 if (!pool_event(c, nev + k)) return fail(DN_ERR_HIP, "event creation failed");
             HIP_OK(hipEventRecord(c->pool[nev], c->stream));
         }
-        launch_vel(a, ctl, c->acq, c->tile_v, c->fdoh, 1, c->stream);
+        launch_vel(av, ctl, c->acq, c->tile_v, c->fdoh, 1, c->stream);
         if (sample) {
             HIP_OK(hipEventRecord(c->pool[nev + 1], c->stream));
             HIP_OK(hipEventRecord(c->pool[nev + 2], c->stream));
         }
-        launch_str(a, ctl, c->acq, c->tile_s, c->fdoh, c->stream);
+        launch_str(as, ctl, c->acq, c->tile_s, c->fdoh, c->stream);
         if (sample) {
             HIP_OK(hipEventRecord(c->pool[nev + 3], c->stream));
             nev += 4;

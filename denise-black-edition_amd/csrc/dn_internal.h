@@ -73,6 +73,7 @@ void prepare_material(const dn_params &p, const Layout &lay, const float *vp, co
 struct KArgs {
     int nx, ny, pitch, fw, fw2;
     int free_surf, boundary, grad_form, L, nt_total;
+    int ntstore;    // write the sweep outputs with streaming (nontemporal) stores
     float dt, dh;
     float dhi_el;   // DT/DH  (update_s_elastic, seismo_ssg)
     float dhi_vis;  // 1/DH   (update_s_visc, surfaces)
@@ -179,6 +180,7 @@ struct BatchShot {
     int *step;        // 1..nt_total: next step to run; nt_total+1: done
     int blk0_v, blk0_s;   // first block of the shot in its group's launches
     int nt_total, dtinv, fwdstore, vp1_form, sw;
+    int nt_v, nt_s;   // streaming stores of the two sweeps
 };
 
 // multi-shot launchers (dn_batch_*.hip)

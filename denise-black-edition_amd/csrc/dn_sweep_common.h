@@ -48,6 +48,16 @@ struct VecT<4> {
     typedef float __attribute__((ext_vector_type(4))) type;
 };
 
+// stores of the sweep outputs: plain, or streaming (nontemporal: written
+// through without keeping the lines in the caches) when nt is set
+__device__ __forceinline__ void st1(float *p, float v, int nt)
+{
+    if (nt)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
 template <int VW>
 __device__ __forceinline__ typename VecT<VW>::type ldv(const float *p)
 {
@@ -56,10 +66,13 @@ __device__ __forceinline__ typename VecT<VW>::type ldv(const float *p)
 
 // store the first n (<= VW) elements
 template <int VW>
-__device__ __forceinline__ void stv(float *p, typename VecT<VW>::type v, int n)
+__device__ __forceinline__ void stv(float *p, typename VecT<VW>::type v, int n, int nt = 0)
 {
     if (n >= VW) {
-        *reinterpret_cast<typename VecT<VW>::type *>(p) = v;
+        if (nt)
+            __builtin_nontemporal_store(v, reinterpret_cast<typename VecT<VW>::type *>(p));
+        else
+            *reinterpret_cast<typename VecT<VW>::type *>(p) = v;
     } else {
 #pragma unroll
         for (int e = 0; e < VW; ++e)

@@ -121,6 +121,8 @@ def load():
         "dn_set_tile": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
         "dn_get_tile": ([vp, C.c_int, ip, ip, ip, ip], C.c_int),
         "dn_autotune_tile": ([vp, C.c_int, ip, ip], C.c_int),
+        "dn_set_store_hint": ([vp, C.c_int, C.c_int], C.c_int),
+        "dn_get_store_hint": ([vp, C.c_int, ip], C.c_int),
         "dn_get_timing": ([vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double), ip], C.c_int),
         "dn_get_footprint": ([vp, C.POINTER(C.c_size_t)], C.c_int),
         "dn_estimate_footprint": ([C.POINTER(_CParams), C.c_int, C.c_int, C.POINTER(C.c_size_t)], C.c_int),
@@ -307,6 +309,15 @@ class Context:
         v, s = (C.c_int * 4)(), (C.c_int * 4)()
         _check(load().dn_autotune_tile(self.h, nsteps, v, s))
         return tuple(v), tuple(s)
+
+    def set_store_hint(self, streaming, sweep=-1):
+        """0 plain, 1 streaming (nontemporal) stores of a sweep's outputs."""
+        _check(load().dn_set_store_hint(self.h, sweep, streaming))
+
+    def store_hint(self, sweep=0):
+        v = C.c_int()
+        _check(load().dn_get_store_hint(self.h, sweep, C.byref(v)))
+        return v.value
 
     def set_kernel_sampling(self, every):
         _check(load().dn_set_kernel_sampling(self.h, every))

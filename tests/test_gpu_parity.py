@@ -338,3 +338,25 @@ def test_autotune_keeps_results():
     g.forward()
     for comp in range(5):
         assert np.array_equal(g.seismogram(comp), o.seismogram(comp).astype(np.float32)), comp
+
+
+@pytest.mark.parametrize("L", [0, 2])
+@pytest.mark.parametrize("tile", [(64, 4, 1, 1), (64, 4, 1, 4)])
+def test_streaming_stores_keep_results(L, tile):
+    """Streaming (nontemporal) stores of the sweep outputs change only the
+    cache policy: results stay the oracle's."""
+    case = make_case(nx=131, ny=77, nt=200, fdorder=8, L=L, seismo=4, fw=9)
+    p, m, src, sig, rec = case
+    o = run_oracle(case)
+    g = dnhip.Context(p)
+    g.set_tile(*tile)
+    g.set_store_hint(1)
+    assert g.store_hint(0) == 1 and g.store_hint(1) == 1
+    g.set_model(m["vp"], m["vs"], m["rho"], m["taus"], m["taup"])
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    g.forward()
+    for comp in range(5):
+        assert np.array_equal(g.seismogram(comp), o.seismogram(comp).astype(np.float32)), comp
+    for fid, name in FIELDS:
+        assert np.array_equal(g.field(fid), o.field(fid).astype(np.float32)), name
