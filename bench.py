@@ -318,6 +318,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--no-visco", action="store_true", help="profiling: elastic shot only")
+    ap.add_argument("--forms-only", action="store_true",
+                    help="only the batched and served forms (no single-shot lines, no CPU baseline)")
     ap.add_argument("--batch-shots", type=int, default=64, help="multi-shot workload size (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=120, help="serving workload size (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=8.0)
@@ -339,6 +341,14 @@ def main():
     if args.tile_v and args.tile_s:
         tiles = [tuple(int(v) for v in args.tile_v.split(",")), tuple(int(v) for v in args.tile_s.split(","))]
     L = 2
+    if args.forms_only:
+        out = {}
+        if args.batch_shots > 0:
+            out["batched"] = run_batched(args.batch_shots)
+        if args.serve_requests > 0:
+            out["serving"] = run_serving(args.serve_requests, args.serve_qps)
+        print(json.dumps(out))
+        return
     if args.no_visco:
         elastic = run_gpu(0, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles)
         print(json.dumps({"elastic": elastic}))

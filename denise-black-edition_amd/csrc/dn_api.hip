@@ -196,8 +196,8 @@ int build_lists(dn_ctx *c)
         if ((rc = c->dalloc((void **)&c->lst_off[w], off.size() * sizeof(int))) ||
             (rc = c->dalloc((void **)&c->lst_ent[w], ent.size() * sizeof(Ent))))
             return rc;
-        HIP_OK(hipMemcpy(c->lst_off[w], off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(c->lst_ent[w], ent.data(), ent.size() * sizeof(Ent), hipMemcpyHostToDevice));
+        HIP_OK(h2d(c->stream, c->lst_off[w], off.data(), off.size() * sizeof(int)));
+        HIP_OK(h2d(c->stream, c->lst_ent[w], ent.data(), ent.size() * sizeof(Ent)));
     }
     return DN_OK;
 }
@@ -300,7 +300,7 @@ int finish_run(dn_ctx *c, size_t nsample_events)
         c->s_ms /= c->samples;
     }
     int nan_step = 0;
-    HIP_OK(hipMemcpy(&nan_step, c->acq.nan_step, sizeof(int), hipMemcpyDeviceToHost));
+    HIP_OK(d2h(c->stream, &nan_step, c->acq.nan_step, sizeof(int)));
     if (nan_step)
         return fail(DN_ERR_UNSTABLE, " Simulation is unstable ! (NaN at time step " + std::to_string(nan_step) + ")");
     return DN_OK;
@@ -324,8 +324,8 @@ int receiver_factors(dn_ctx *c)
             sm[t] = std::sqrt((double)muss);
         }
     }
-    HIP_OK(hipMemcpy(c->acq.rec_sqlam, sl.data(), sizeof(double) * ntr, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(c->acq.rec_sqmu, sm.data(), sizeof(double) * ntr, hipMemcpyHostToDevice));
+    HIP_OK(h2d(c->stream, c->acq.rec_sqlam, sl.data(), sizeof(double) * ntr));
+    HIP_OK(h2d(c->stream, c->acq.rec_sqmu, sm.data(), sizeof(double) * ntr));
     return DN_OK;
 }
 
@@ -366,7 +366,7 @@ static size_t plan_bytes(const dn_params &p, int nsrc, int ntr)
         const size_t ntd = (size_t)std::ceil((float)p.nt / (float)p.dtinv);
         b += 5 * (size_t)p.nx * p.ny * ntd * sizeof(float) + 6 * (size_t)p.nx * p.ny * sizeof(float);
     }
-    b += sizeof(int) + sizeof(double);
+    b += sizeof(int) + sizeof(double) * l2_scratch_doubles();
     b += (size_t)nsrc * (3 * sizeof(int) + 2 * sizeof(double)) + (size_t)nsrc * p.nt * sizeof(float);
     b += (size_t)ntr * (2 * sizeof(int) + 2 * sizeof(double)) + 9 * (size_t)ntr * p.nt * sizeof(float);
     return b;
@@ -464,7 +464,7 @@ int dn_ctx_create(const dn_params *p, int device, dn_ctx **out)
         if ((rc = c->dalloc((void **)&c->psi[k], (k < 5 ? nxb : nyb) * sizeof(float) + 4))) return bail(rc);
     for (int k = 0; k < P_NVEC; ++k) {
         if ((rc = c->dalloc((void **)&c->pml[k], sizeof(float) * (2 * p->fw + 1)))) return bail(rc);
-        if (hipMemcpy(c->pml[k], c->setup.pml[k].data(), sizeof(float) * (2 * p->fw + 1), hipMemcpyHostToDevice) != hipSuccess)
+        if (h2d(c->stream, c->pml[k], c->setup.pml[k].data(), sizeof(float) * (2 * p->fw + 1)) != hipSuccess)
             return bail(fail(DN_ERR_HIP, "upload of CPML profiles failed"));
     }
     if (p->mode > 0) {
@@ -476,13 +476,13 @@ int dn_ctx_create(const dn_params *p, int device, dn_ctx **out)
         for (auto m : grads)
             if ((rc = c->dalloc((void **)m, g))) return bail(rc);
     }
-    if ((rc = c->dalloc((void **)&c->acq.nan_step, sizeof(int))) || (rc = c->dalloc((void **)&c->acq.l2, sizeof(double))))
+    if ((rc = c->dalloc((void **)&c->acq.nan_step, sizeof(int))) || (rc = c->dalloc((void **)&c->acq.l2, sizeof(double) * l2_scratch_doubles())))
         return bail(rc);
     c->acq.nt = p->nt;
     c->tile_v = c->tile_s = default_tile(p->nx, p->ny);
     if ((rc = build_lists(c))) return bail(rc);
     fill_args(c);
-    if (hipDeviceSynchronize() != hipSuccess) return bail(fail(DN_ERR_HIP, "device synchronisation failed"));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bail(fail(DN_ERR_HIP, "stream synchronisation failed"));
     *out = c;
     return DN_OK;
 }
@@ -604,12 +604,12 @@ int dn_set_sources(dn_ctx *c, int nsrc, const float *srcpos, const float *signal
         (rc = c->dalloc((void **)&q.src_cos, sizeof(double) * nsrc)) ||
         (rc = c->dalloc((void **)&q.signals, sizeof(float) * (size_t)nsrc * c->p.nt)))
         return rc;
-    HIP_OK(hipMemcpy(q.src_ix, ix.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(q.src_iy, iy.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(q.src_type, type.data(), sizeof(int) * nsrc, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(q.src_sin, sn.data(), sizeof(double) * nsrc, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(q.src_cos, cs.data(), sizeof(double) * nsrc, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(q.signals, signals, sizeof(float) * (size_t)nsrc * c->p.nt, hipMemcpyHostToDevice));
+    HIP_OK(h2d(c->stream, q.src_ix, ix.data(), sizeof(int) * nsrc));
+    HIP_OK(h2d(c->stream, q.src_iy, iy.data(), sizeof(int) * nsrc));
+    HIP_OK(h2d(c->stream, q.src_type, type.data(), sizeof(int) * nsrc));
+    HIP_OK(h2d(c->stream, q.src_sin, sn.data(), sizeof(double) * nsrc));
+    HIP_OK(h2d(c->stream, q.src_cos, cs.data(), sizeof(double) * nsrc));
+    HIP_OK(h2d(c->stream, q.signals, signals, sizeof(float) * (size_t)nsrc * c->p.nt));
     q.nsrc = nsrc;
     c->src_ix = ix;
     c->src_iy = iy;
@@ -655,8 +655,8 @@ int dn_set_receivers(dn_ctx *c, int ntr, const int *recpos)
         if ((rc = c->dalloc((void **)&q.sec[k], tr))) return rc;
     for (int k = 0; k < 3; ++k)
         if ((rc = c->dalloc((void **)&q.adj[k], tr))) return rc;
-    HIP_OK(hipMemcpy(q.rec_ix, c->rec_ix.data(), sizeof(int) * ntr, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(q.rec_iy, c->rec_iy.data(), sizeof(int) * ntr, hipMemcpyHostToDevice));
+    HIP_OK(h2d(c->stream, q.rec_ix, c->rec_ix.data(), sizeof(int) * ntr));
+    HIP_OK(h2d(c->stream, q.rec_iy, c->rec_iy.data(), sizeof(int) * ntr));
     q.ntr = ntr;
     int rc2 = build_lists(c);
     if (rc2) return rc2;
@@ -838,7 +838,7 @@ int dn_get_seismogram(dn_ctx *c, int comp, float *out)
     if (c->acq.ntr == 0) return DN_OK;
     HIP_OK(hipSetDevice(c->device));
     HIP_OK(hipStreamSynchronize(c->stream));
-    HIP_OK(hipMemcpy(out, c->acq.sec[comp], (size_t)c->acq.ntr * c->p.nt * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_OK(d2h(c->stream, out, c->acq.sec[comp], (size_t)c->acq.ntr * c->p.nt * sizeof(float)));
     return DN_OK;
 }
 
@@ -864,7 +864,7 @@ int dn_calc_residuals(dn_ctx *c, const float *obs_vx, const float *obs_vy, const
     }
     HIP_OK(hipGetLastError());
     double v = 0.0;
-    HIP_OK(hipMemcpy(&v, q.l2, sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(d2h(c->stream, &v, q.l2, sizeof(double)));
     if (l2) *l2 = v;
     return DN_OK;
 }
@@ -878,9 +878,9 @@ int dn_set_adjoint_sources(dn_ctx *c, const float *dvx, const float *dvy, const 
     for (int k = 0; k < 3; ++k) {
         if (!c->acq.adj[k]) continue;
         if (src[k])
-            HIP_OK(hipMemcpy(c->acq.adj[k], src[k], n * sizeof(float), hipMemcpyHostToDevice));
+            HIP_OK(h2d(c->stream, c->acq.adj[k], src[k], n * sizeof(float)));
         else
-            HIP_OK(hipMemset(c->acq.adj[k], 0, n * sizeof(float)));
+            HIP_OK(hipMemsetAsync(c->acq.adj[k], 0, n * sizeof(float), c->stream));
     }
     return DN_OK;
 }
@@ -891,7 +891,7 @@ int dn_get_adjoint_sources(dn_ctx *c, int comp, float *out)
     if (c->acq.ntr == 0) return DN_OK;
     HIP_OK(hipSetDevice(c->device));
     HIP_OK(hipStreamSynchronize(c->stream));
-    HIP_OK(hipMemcpy(out, c->acq.adj[comp], (size_t)c->acq.ntr * c->p.nt * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_OK(d2h(c->stream, out, c->acq.adj[comp], (size_t)c->acq.ntr * c->p.nt * sizeof(float)));
     return DN_OK;
 }
 
@@ -965,12 +965,12 @@ int dn_get_gradient(dn_ctx *c, int which, float *out)
     const size_t n = (size_t)NX * NY;
     if (which >= 3) {
         float *src = which == 3 ? c->gvp : which == 4 ? c->gvs : c->grho;
-        HIP_OK(hipMemcpy(out, src, n * sizeof(float), hipMemcpyDeviceToHost));
+        HIP_OK(d2h(c->stream, out, src, n * sizeof(float)));
         return DN_OK;
     }
     std::vector<float> tmp(n);
     float *src = which == 0 ? c->wc : which == 1 ? c->wcu : c->wcr;
-    HIP_OK(hipMemcpy(tmp.data(), src, n * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_OK(d2h(c->stream, tmp.data(), src, n * sizeof(float)));
     for (int i = 0; i < NX; ++i)
         for (int j = 0; j < NY; ++j) out[(size_t)i * NY + j] = tmp[(size_t)j * NX + i];
     return DN_OK;
@@ -989,7 +989,7 @@ static int get_plane_nr(dn_ctx *c, const float *plane, float *out)
     HIP_OK(hipSetDevice(c->device));
     HIP_OK(hipStreamSynchronize(c->stream));
     std::vector<float> h(c->lay.plane());
-    HIP_OK(hipMemcpy(h.data(), plane, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_OK(d2h(c->stream, h.data(), plane, h.size() * sizeof(float)));
     const int nd = c->nd, NX = c->p.nx, NY = c->p.ny;
     const int ncol = NX + 2 * nd;
     size_t k = 0;
@@ -1024,7 +1024,7 @@ int dn_get_psi(dn_ctx *c, int id, float *out)
     HIP_OK(hipSetDevice(c->device));
     HIP_OK(hipStreamSynchronize(c->stream));
     const size_t n = id < 5 ? (size_t)c->p.ny * 2 * c->p.fw : (size_t)2 * c->p.fw * c->p.nx;
-    HIP_OK(hipMemcpy(out, c->psi[id], n * sizeof(float), hipMemcpyDeviceToHost));
+    HIP_OK(d2h(c->stream, out, c->psi[id], n * sizeof(float)));
     return DN_OK;
 }
 

@@ -28,6 +28,20 @@ enum { F_VX = 0, F_VY, F_SXX, F_SYY, F_SXY, F_VXP1, F_VYP1, F_UX, F_UY, F_UXY, F
 // library's internal types.
 using namespace dn;
 
+namespace dn {
+// host<->device copies ordered on a stream, returning when complete
+inline hipError_t h2d(hipStream_t s, void *dst, const void *src, size_t n)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+inline hipError_t d2h(hipStream_t s, void *dst, const void *src, size_t n)
+{
+    const hipError_t e = hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+}  // namespace dn
+
 struct dn_ctx {
     dn_params p{};
     int device = 0;
@@ -80,7 +94,11 @@ struct dn_ctx {
         }
         allocs.emplace_back(*ptr, nbytes);
         bytes += nbytes;
-        (void)hipMemset(*ptr, 0, nbytes);
+        // zeroed on the context's own stream: the library makes no
+        // null-stream or device-wide calls, so contexts can be prepared on
+        // one thread while another thread captures or runs an engine graph
+        e = hipMemsetAsync(*ptr, 0, nbytes, stream);
+        if (e != hipSuccess) return fail(DN_ERR_HIP, std::string("hipMemsetAsync failed: ") + hipGetErrorString(e));
         return DN_OK;
     }
     void dfree(void *ptr)

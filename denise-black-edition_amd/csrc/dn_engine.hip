@@ -1,11 +1,12 @@
 // Multi-shot engine: many independent shots (contexts) advanced together.
 //
 // Every time step is one velocity launch and one stress launch per group of
-// shots that share FD order, relaxation count, direction and tile shapes,
-// plus one launch that moves every shot's device step counter on.  Because
-// the kernels read the step from the counter, the same launch sequence
-// serves every step: the engine can capture a run of steps in a hipGraph
-// once and replay it.  Shots of different grid size and length share the
+// shots that share FD order, relaxation count, direction and tile shapes.
+// The kernels read the time step from one device clock plus the offset k of
+// the step in the current run of steps (shot step = clock + k - base, base
+// fixed when the shot is attached); a run of steps ends with one launch that
+// moves the clock on.  So the same launch sequence serves every run: the
+// engine can capture a run of steps in a hipGraph once and replay it.  Shots of different grid size and length share the
 // launches; a shot that has reached its NT leaves its blocks idle until the
 // host detaches it.  Shots can be attached and detached between calls
 // (continuous batching); their state stays in their contexts.
@@ -26,7 +27,7 @@ namespace {
 struct Job {
     dn_ctx *ctx;
     int phase;  // 0 forward, 1 adjoint
-    int slot;   // index of its step counter
+    long base;  // engine clock before its first step
 };
 
 struct Group {
@@ -35,6 +36,7 @@ struct Group {
     int off_v = 0, nblk_v = 0, off_s = 0, nblk_s = 0;
 };
 
+
 }  // namespace
 
 struct dn_engine {
@@ -42,14 +44,13 @@ struct dn_engine {
     hipStream_t stream = nullptr;
     std::vector<Job> jobs;              // running shots
     std::map<dn_ctx *, int> status;     // last known state per shot
-    int *d_steps = nullptr;             // step counters, one per slot
-    int cap = 0;
-    std::vector<int> free_slots;
+    int *d_clock = nullptr;             // device clock (steps run)
+    long clock = 0;                     // its value once the queued work is done
     bool dirty = true;
     std::vector<Group> groups;
     BatchShot *d_shots = nullptr;
     int n_shots = 0, cap_shots = 0;
-    int *d_blk = nullptr;
+    BatchBlock *d_blk = nullptr;
     size_t cap_blk = 0;
     int graph_steps = 0;
     hipGraphExec_t gexec = nullptr;
@@ -67,7 +68,7 @@ struct dn_engine {
         (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
         drop_graph();
-        if (d_steps) (void)hipFree(d_steps);
+        if (d_clock) (void)hipFree(d_clock);
         if (d_shots) (void)hipFree(d_shots);
         if (d_blk) (void)hipFree(d_blk);
         if (stream) (void)hipStreamDestroy(stream);
@@ -89,7 +90,7 @@ int rebuild(dn_engine *e)
             .push_back(k);
     }
     std::vector<BatchShot> shots;
-    std::vector<int> blk;
+    std::vector<BatchBlock> blk;
     e->groups.clear();
     for (auto &kv : by) {
         Group g;
@@ -99,7 +100,7 @@ int rebuild(dn_engine *e)
         g.sw = e->jobs[kv.second[0]].phase;
         g.tv = c0->tile_v;
         g.ts = c0->tile_s;
-        std::vector<int> bv, bs;
+        std::vector<BatchBlock> bv, bs;
         for (int k : kv.second) {
             const Job &j = e->jobs[k];
             dn_ctx *c = j.ctx;
@@ -116,9 +117,6 @@ int rebuild(dn_engine *e)
             b.cs.ntx = c->tile_s.ntx(c->p.nx);
             b.cs.ntiles = b.cs.ntx * c->tile_s.nty(c->p.ny);
             b.q = c->acq;
-            b.step = e->d_steps + j.slot;
-            b.blk0_v = (int)bv.size();
-            b.blk0_s = (int)bs.size();
             b.nt_total = c->p.nt;
             b.dtinv = c->p.dtinv;
             b.fwdstore = c->p.mode > 0;
@@ -128,8 +126,9 @@ int rebuild(dn_engine *e)
             b.nt_s = c->nt_s;
             const int idx = (int)shots.size();
             shots.push_back(b);
-            bv.insert(bv.end(), (size_t)b.cv.ntiles, idx);
-            bs.insert(bs.end(), (size_t)b.cs.ntiles, idx);
+            const int b0 = (int)j.base;
+            for (int t = 0; t < b.cv.ntiles; ++t) bv.push_back(BatchBlock{idx, t, b0, b.nt_total});
+            for (int t = 0; t < b.cs.ntiles; ++t) bs.push_back(BatchBlock{idx, t, b0, b.nt_total});
         }
         g.off_v = (int)blk.size();
         g.nblk_v = (int)bv.size();
@@ -149,28 +148,35 @@ int rebuild(dn_engine *e)
         if (e->d_blk) HIP_OK(hipFree(e->d_blk));
         e->d_blk = nullptr;
         e->cap_blk = std::max(blk.size(), 2 * e->cap_blk);
-        HIP_OK(hipMalloc((void **)&e->d_blk, sizeof(int) * e->cap_blk));
+        HIP_OK(hipMalloc((void **)&e->d_blk, sizeof(BatchBlock) * e->cap_blk));
     }
-    if (!shots.empty()) HIP_OK(hipMemcpy(e->d_shots, shots.data(), sizeof(BatchShot) * shots.size(), hipMemcpyHostToDevice));
-    if (!blk.empty()) HIP_OK(hipMemcpy(e->d_blk, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice));
+    if (!shots.empty()) HIP_OK(h2d(e->stream, e->d_shots, shots.data(), sizeof(BatchShot) * shots.size()));
+    if (!blk.empty()) HIP_OK(h2d(e->stream, e->d_blk, blk.data(), sizeof(BatchBlock) * blk.size()));
     e->n_shots = (int)shots.size();
     e->dirty = false;
     return DN_OK;
 }
 
-void enqueue_step(dn_engine *e)
+// step k of a run of steps
+void enqueue_step(dn_engine *e, int k)
 {
     for (const Group &g : e->groups)
-        launch_batch_vel(e->d_shots, e->d_blk + g.off_v, g.nblk_v, g.tv, g.fdoh, g.sw, e->stream);
+        launch_batch_vel(e->d_shots, e->d_blk + g.off_v, g.nblk_v, e->d_clock, k, g.tv, g.fdoh, g.sw, e->stream);
     for (const Group &g : e->groups)
-        launch_batch_str(e->d_shots, e->d_blk + g.off_s, g.nblk_s, g.ts, g.fdoh, g.L, e->stream);
-    launch_batch_advance(e->d_shots, e->n_shots, e->stream);
+        launch_batch_str(e->d_shots, e->d_blk + g.off_s, g.nblk_s, e->d_clock, k, g.ts, g.fdoh, g.L, e->stream);
+}
+
+// a run of n steps, then the clock moves on by n
+void enqueue_run(dn_engine *e, int n)
+{
+    for (int k = 0; k < n; ++k) enqueue_step(e, k);
+    launch_batch_clock(e->d_clock, n, e->stream);
 }
 
 int capture(dn_engine *e)
 {
-    HIP_OK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < e->graph_steps; ++k) enqueue_step(e);
+    HIP_OK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+    enqueue_run(e, e->graph_steps);
     hipGraph_t g = nullptr;
     HIP_OK(hipStreamEndCapture(e->stream, &g));
     e->graph = g;
@@ -178,23 +184,19 @@ int capture(dn_engine *e)
     return DN_OK;
 }
 
-// shots whose counters passed their NT leave the engine
+// shots past their NT leave the engine
 int collect(dn_engine *e)
 {
-    if (e->jobs.empty()) return DN_OK;
-    std::vector<int> steps((size_t)e->cap);
-    HIP_OK(hipMemcpy(steps.data(), e->d_steps, sizeof(int) * e->cap, hipMemcpyDeviceToHost));
     std::vector<Job> still;
     for (const Job &j : e->jobs) {
-        if (steps[(size_t)j.slot] <= j.ctx->p.nt) {
+        if (e->clock - j.base <= j.ctx->p.nt) {
             still.push_back(j);
             continue;
         }
         int nan_step = 0;
-        HIP_OK(hipMemcpy(&nan_step, j.ctx->acq.nan_step, sizeof(int), hipMemcpyDeviceToHost));
+        HIP_OK(d2h(e->stream, &nan_step, j.ctx->acq.nan_step, sizeof(int)));
         if (j.phase == 0) j.ctx->fwd_complete = j.ctx->p.mode > 0;
         e->status[j.ctx] = nan_step ? DN_ERR_UNSTABLE : 0;
-        e->free_slots.push_back(j.slot);
         e->dirty = true;
     }
     e->jobs.swap(still);
@@ -217,12 +219,12 @@ int dn_engine_create(int device, dn_engine **out)
         delete e;
         return fail(DN_ERR_HIP, "stream creation failed");
     }
-    e->cap = 4096;
-    if (hipMalloc((void **)&e->d_steps, sizeof(int) * e->cap) != hipSuccess) {
+    const int zero = 0;
+    if (hipMalloc((void **)&e->d_clock, sizeof(int)) != hipSuccess ||
+        h2d(e->stream, e->d_clock, &zero, sizeof(int)) != hipSuccess) {
         delete e;
-        return fail(DN_ERR_OOM, "step counters");
+        return fail(DN_ERR_OOM, "engine clock");
     }
-    for (int k = e->cap - 1; k >= 0; --k) e->free_slots.push_back(k);
     *out = e;
     return DN_OK;
 }
@@ -253,12 +255,14 @@ int dn_engine_attach(dn_engine *e, dn_ctx *c, int phase)
         if (j.ctx == c) return fail(DN_ERR_STATE, "context already attached");
     if (phase == 1 && (c->p.mode <= 0 || !c->fwd_complete))
         return fail(DN_ERR_STATE, "adjoint phase needs a finished forward phase with snapshots");
-    if (e->free_slots.empty()) return fail(DN_ERR_ARG, "engine full");
+    if (e->clock > (1L << 30)) return fail(DN_ERR_STATE, "engine clock exhausted; create a new engine");
     HIP_OK(hipSetDevice(e->device));
     // tile shapes the multi-shot kernels are compiled for
     int rc;
     if (!batch_tile_supported(c->tile_v, 0) && (rc = dn_set_tile(c, 0, 128, 2, 4, 1))) return rc;
-    if (!batch_tile_supported(c->tile_s, 1) && (rc = dn_set_tile(c, 1, 64, 4, 1, c->p.L == 0 ? 4 : 1))) return rc;
+    if ((!batch_tile_supported(c->tile_s, 1) || (c->p.L == 0 && c->tile_s.vw == 1)) &&
+        (rc = dn_set_tile(c, 1, 64, 4, 1, c->p.L == 0 ? 4 : 1)))
+        return rc;
     if ((rc = zero_state(c))) return rc;
     if (phase == 1) {
         const size_t g = (size_t)c->p.nx * c->p.ny * sizeof(float);
@@ -269,11 +273,7 @@ int dn_engine_attach(dn_engine *e, dn_ctx *c, int phase)
         c->fwd_complete = false;
     }
     HIP_OK(hipStreamSynchronize(c->stream));
-    const int slot = e->free_slots.back();
-    e->free_slots.pop_back();
-    const int one = 1;
-    HIP_OK(hipMemcpy(e->d_steps + slot, &one, sizeof(int), hipMemcpyHostToDevice));
-    e->jobs.push_back(Job{c, phase, slot});
+    e->jobs.push_back(Job{c, phase, e->clock - 1});
     e->status[c] = 1;
     e->dirty = true;
     return DN_OK;
@@ -286,12 +286,16 @@ int dn_engine_step(dn_engine *e, int nsteps, int *running)
     int rc;
     if (e->dirty && (rc = rebuild(e))) return rc;
     if (!e->jobs.empty() && nsteps > 0) {
-        int left = nsteps;
-        if (e->graph_steps > 0 && nsteps >= e->graph_steps) {
+        // no more steps than the longest remaining shot needs
+        long need = 0;
+        for (const Job &j : e->jobs) need = std::max<long>(need, j.ctx->p.nt - (e->clock - j.base) + 1);
+        int left = (int)std::min<long>(nsteps, need);
+        e->clock += left;
+        if (e->graph_steps > 0 && left >= e->graph_steps) {
             if (!e->gexec && (rc = capture(e))) return rc;
             for (; left >= e->graph_steps; left -= e->graph_steps) HIP_OK(hipGraphLaunch(e->gexec, e->stream));
         }
-        for (; left > 0; --left) enqueue_step(e);
+        if (left > 0) enqueue_run(e, left);
         HIP_OK(hipGetLastError());
         HIP_OK(hipStreamSynchronize(e->stream));
         if ((rc = collect(e))) return rc;

@@ -118,7 +118,7 @@ struct DevAcq {
     float *sec[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};   // ntr x NT
     float *adj[3] = {nullptr, nullptr, nullptr};                     // ntr x NT
     float *obs = nullptr;                 // scratch for observed traces
-    double *l2 = nullptr;                 // device scalar
+    double *l2 = nullptr;                 // misfit, then the partial sums of its reduction
     int *nan_step = nullptr;              // device scalar
 };
 
@@ -170,26 +170,34 @@ struct StepCtl {
     float *wc, *wcu, *wcr;   // correlation accumulators (compact, [j][i])
 };
 
-// One shot of a multi-shot launch: its sweep arguments, the static sweep
-// control of both sweeps (cell lists, tile grids) and its own device step
-// counter; the per-step fields are derived in the kernel from the counter.
+// One shot of a multi-shot launch: its sweep arguments and the static sweep
+// control of both sweeps (cell lists, tile grids); the per-step fields are
+// derived in the kernel from the engine clock.
 struct BatchShot {
     KArgs a;
     StepCtl cv, cs;
     DevAcq q;
-    int *step;        // 1..nt_total: next step to run; nt_total+1: done
-    int blk0_v, blk0_s;   // first block of the shot in its group's launches
     int nt_total, dtinv, fwdstore, vp1_form, sw;
     int nt_v, nt_s;   // streaming stores of the two sweeps
 };
 
-// multi-shot launchers (dn_batch_*.hip)
+// One block of a multi-shot launch: its shot, the tile it sweeps, and what
+// decides whether the shot runs at this step (time step = clock + k - base,
+// running while 1 <= step <= nt_total), so that a block can leave before
+// it touches the shot's arguments.
+struct alignas(16) BatchBlock {
+    int shot, tile, base, nt_total;
+};
+
+// multi-shot launchers (dn_batch_*.hip).  clock: the engine's device clock;
+// k: offset of this step from it (runs of steps are enqueued, or captured
+// in a graph, with k = 0, 1, ... and the clock moved on once at the end)
 bool batch_tile_supported(const Tile &t, int sweep);
-void launch_batch_vel(const BatchShot *shots, const int *blk_shot, int nblocks, const Tile &t, int fdoh, int sw,
-                      void *stream);
-void launch_batch_str(const BatchShot *shots, const int *blk_shot, int nblocks, const Tile &t, int fdoh, int L,
-                      void *stream);
-void launch_batch_advance(const BatchShot *shots, int n, void *stream);
+void launch_batch_vel(const BatchShot *shots, const BatchBlock *blk, int nblocks, const int *clock, int k,
+                      const Tile &t, int fdoh, int sw, void *stream);
+void launch_batch_str(const BatchShot *shots, const BatchBlock *blk, int nblocks, const int *clock, int k,
+                      const Tile &t, int fdoh, int L, void *stream);
+void launch_batch_clock(int *clock, int add, void *stream);
 
 // kernel launchers (dn_kernels.hip); all enqueue on `stream`
 void launch_vel(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, int sw,
@@ -201,6 +209,7 @@ void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const fl
                      float *gvs, float *grho, int invmat1, void *stream);
 void launch_residual(const DevAcq &q, int comp, const float *obs, float dt, void *stream);
 void launch_l2(const DevAcq &q, int comp, void *stream);
+int l2_scratch_doubles();  // doubles behind DevAcq::l2
 bool tile_supported(const Tile &t);
 // compiled tile shapes (returns their count; fills at most max entries)
 int tile_list(Tile *out, int max);

@@ -108,35 +108,84 @@ __global__ __launch_bounds__(256) void k_assemble(KArgs a, const float *wc, cons
 }
 
 // time-integrated, time-reversed L2 residual of one trace
-// (src/calc_res.c:406-452 with src/inseis.c:96-103): one thread per trace.
-__global__ void k_residual(DevAcq q, int comp, const float *obs, float dt)
+// (src/calc_res.c:406-452 with src/inseis.c:96-103).  The integration is a
+// sequential float chain per trace, so each lane owns one trace; a block of
+// 64 lanes stages 64-sample chunks of its 64 traces in LDS so that global
+// reads and writes run along the traces (coalesced) while each lane walks
+// its own trace in the reference's order.
+constexpr int RB = 64;  // traces per block = samples per chunk
+__global__ __launch_bounds__(RB) void k_residual(DevAcq q, int comp, const float *obs, float dt)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= q.ntr) return;
+    __shared__ float S[RB][RB + 1], O[RB][RB + 1];
+    const int lane = threadIdx.x, t0 = blockIdx.x * RB;
     const long ns = q.nt;
-    const float *syn = q.sec[comp] + (long)t * ns;
-    const float *ob = obs + (long)t * ns;
-    float *diff = q.adj[comp] + (long)t * ns;
+    const int nr = q.ntr - t0 < RB ? q.ntr - t0 : RB;
+    const float *syn = q.sec[comp];
+    float *diff = q.adj[comp];
+    const float syn0 = lane < nr ? syn[(long)(t0 + lane) * ns] : 0.0f;
     float intseis = 0.0f;
-    for (long jj = 1; jj <= ns; ++jj) {
-        const float d = jj == 1 ? syn[0] : ob[jj - 1];
-        intseis += dt * (syn[jj - 1] - d);
-        diff[ns - jj] = intseis;
+    for (long c0 = 0; c0 < ns; c0 += RB) {
+        const int nc = ns - c0 < RB ? (int)(ns - c0) : RB;
+        for (int r = 0; r < nr; ++r) {
+            const long g = (long)(t0 + r) * ns + c0 + lane;
+            if (lane < nc) {
+                S[r][lane] = syn[g];
+                O[r][lane] = obs[g];
+            }
+        }
+        __syncthreads();
+        if (lane < nr) {
+            for (int l = 0; l < nc; ++l) {
+                const long jj = c0 + l + 1;
+                const float d = jj == 1 ? syn0 : O[lane][l];
+                intseis += dt * (S[lane][l] - d);
+                S[lane][l] = intseis;  // sample ns - jj of the residual
+            }
+        }
+        __syncthreads();
+        for (int r = 0; r < nr; ++r)
+            if (lane < nc) diff[(long)(t0 + r) * ns + (ns - 1 - c0 - lane)] = S[r][lane];
+        __syncthreads();
     }
 }
 
-// misfit 0.5*sum(residual^2) in the reference's summation order
-// (src/calc_res.c:553-556): one sequential chain in double.
-__global__ void k_l2(DevAcq q, int comp)
+// misfit 0.5*sum(residual^2) of src/calc_res.c:553-556 in double.  Every
+// term 0.5*d*d of a float d is exact in double; the terms are summed in a
+// fixed tree order (L2_PARTS block ranges of 256 strided lanes each, then a
+// 256-lane tree over the block sums), so the result is reproducible run to
+// run and differs from the reference's single sequential chain only by
+// double rounding.
+constexpr int L2_PARTS = 256;
+__device__ __forceinline__ double tree256(double *sh, double v)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const long ns = q.nt;
-    double L2 = *q.l2;
-    for (int t = 0; t < q.ntr; ++t) {
-        const float *diff = q.adj[comp] + (long)t * ns;
-        for (long jj = ns; jj >= 1; --jj) L2 += 0.5 * (double)diff[jj - 1] * (double)diff[jj - 1];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] = sh[threadIdx.x] + sh[threadIdx.x + h];
+        __syncthreads();
     }
-    *q.l2 = L2;
+    return sh[0];
+}
+
+__global__ __launch_bounds__(256) void k_l2_part(DevAcq q, int comp)
+{
+    __shared__ double sh[256];
+    const long n = (long)q.ntr * q.nt;
+    const long lo = n * blockIdx.x / L2_PARTS, hi = n * (blockIdx.x + 1) / L2_PARTS;
+    const float *diff = q.adj[comp];
+    double s = 0.0;
+    for (long i = lo + threadIdx.x; i < hi; i += 256) s += 0.5 * (double)diff[i] * (double)diff[i];
+    const double tot = tree256(sh, s);
+    if (threadIdx.x == 0) q.l2[1 + blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(256) void k_l2_sum(DevAcq q)
+{
+    __shared__ double sh[256];
+    double s = 0.0;
+    for (int b = threadIdx.x; b < L2_PARTS; b += 256) s += q.l2[1 + b];
+    const double tot = tree256(sh, s);
+    if (threadIdx.x == 0) q.l2[0] = q.l2[0] + tot;
 }
 
 }  // namespace
@@ -235,13 +284,16 @@ void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const fl
 void launch_residual(const DevAcq &q, int comp, const float *obs, float dt, void *stream)
 {
     if (q.ntr <= 0) return;
-    dim3 g((q.ntr + 63) / 64);
-    hipLaunchKernelGGL(k_residual, g, dim3(64), 0, (hipStream_t)stream, q, comp, obs, dt);
+    dim3 g((q.ntr + RB - 1) / RB);
+    hipLaunchKernelGGL(k_residual, g, dim3(RB), 0, (hipStream_t)stream, q, comp, obs, dt);
 }
 
 void launch_l2(const DevAcq &q, int comp, void *stream)
 {
-    hipLaunchKernelGGL(k_l2, dim3(1), dim3(64), 0, (hipStream_t)stream, q, comp);
+    hipLaunchKernelGGL(k_l2_part, dim3(L2_PARTS), dim3(256), 0, (hipStream_t)stream, q, comp);
+    hipLaunchKernelGGL(k_l2_sum, dim3(1), dim3(256), 0, (hipStream_t)stream, q);
 }
+
+int l2_scratch_doubles() { return 1 + L2_PARTS; }
 
 }  // namespace dn

@@ -10,10 +10,15 @@ residuals formed on the device, then rejoins the engine for its adjoint
 phase.  A request whose footprint exceeds the whole budget is rejected.
 
 The host loop only admits, steps and collects; every time step of every
-shot is GPU work.  Latency = completion time - arrival time.
+shot is GPU work.  Admitted requests are prepared (device allocation,
+model, sources, receivers) by a worker thread while the engine steps the
+running shots, so preparation does not stall the GPU; a prepared shot joins
+at the next step boundary.  Latency = completion time - arrival time.
 """
 from __future__ import annotations
 
+import queue
+import threading
 import time
 from dataclasses import dataclass, field
 
@@ -53,24 +58,61 @@ class Server:
         self.engine = dnhip.Engine(device)
         self.engine.set_graph(graph_steps)
         self.running = {}          # Context -> (Request, phase, bytes)
+        self.preparing = 0         # admitted requests not yet attached
+        self._todo: queue.Queue = queue.Queue()
+        self._ready: queue.Queue = queue.Queue()
+        self._worker = threading.Thread(target=self._prepare_loop, daemon=True)
+        self._worker.start()
 
     @staticmethod
     def footprint(req: Request) -> int:
         return dnhip.estimate_footprint(req.params, req.srcpos.shape[1], req.recpos.shape[1])
 
-    def _admit(self, req: Request, now: float) -> None:
-        p = req.params
-        if req.kind == "gradient" and p.mode == 0:
-            raise ValueError("gradient requests need params.mode = 1")
-        ctx = dnhip.Context(p, self.device)
-        ctx.set_model(*req.model)
-        ctx.set_sources(req.srcpos, req.signals)
-        ctx.set_receivers(req.recpos)
-        nbytes = ctx.footprint()
-        self.engine.attach(ctx, 0)
-        self.used += nbytes
-        self.running[ctx] = (req, 0, nbytes)
+    def _prepare_loop(self) -> None:
+        """Worker thread: build the device state of admitted requests."""
+        while True:
+            item = self._todo.get()
+            if item is None:
+                return
+            req, nbytes = item
+            ctx = None
+            try:
+                if req.kind == "gradient" and req.params.mode == 0:
+                    raise ValueError("gradient requests need params.mode = 1")
+                ctx = dnhip.Context(req.params, self.device)
+                ctx.set_model(*req.model)
+                ctx.set_sources(req.srcpos, req.signals)
+                ctx.set_receivers(req.recpos)
+                self._ready.put((req, nbytes, ctx, None))
+            except Exception as ex:  # reported on the request
+                if ctx is not None:
+                    ctx.close()
+                self._ready.put((req, nbytes, None, ex))
+
+    def _admit(self, req: Request, need: int, now: float) -> None:
+        self.used += need
+        self.preparing += 1
         req.admitted = now
+        self._todo.put((req, need))
+
+    def _attach_ready(self, now: float, block: bool) -> None:
+        """Attach every prepared shot (waiting for one when block is set)."""
+        while self.preparing:
+            try:
+                req, nbytes, ctx, err = self._ready.get(block=block, timeout=60.0 if block else None)
+            except queue.Empty:
+                if block:
+                    raise RuntimeError("request preparation did not finish within 60 s")
+                return
+            block = False
+            self.preparing -= 1
+            if err is not None:
+                req.error = f"preparation failed: {err}"
+                req.done = now
+                self.used -= nbytes
+                continue
+            self.engine.attach(ctx, 0)
+            self.running[ctx] = (req, 0, nbytes)
 
     def _finish(self, ctx, now: float) -> None:
         req, phase, nbytes = self.running.pop(ctx)
@@ -103,7 +145,8 @@ class Server:
         t0 = clock()
         k = 0
         cells = 0
-        while k < len(todo) or waiting or self.running:
+        t_step, n_calls, shot_calls = 0.0, 0, 0
+        while k < len(todo) or waiting or self.running or self.preparing:
             now = clock() - t0
             while k < len(todo) and todo[k].arrival <= now:
                 waiting.append(todo[k])
@@ -117,18 +160,24 @@ class Server:
                     req.error = f"footprint {need} B exceeds the budget {self.budget} B"
                     req.done = now
                 elif self.used + need <= self.budget:
-                    self._admit(req, now)
+                    self._admit(req, need, now)
                     n = req.params.nx * req.params.ny * req.params.nt
                     cells += n * (2 if req.kind == "gradient" else 1)
                 else:
                     still.append(req)
             waiting = still
+            # prepared shots join; with nothing running, wait for one
+            self._attach_ready(clock() - t0, block=not self.running and self.preparing > 0)
             if self.running:
+                ts = clock()
                 self.engine.step(self.chunk)
+                t_step += clock() - ts
+                n_calls += 1
+                shot_calls += len(self.running)
                 now = clock() - t0
                 for ctx in [c for c in self.running if self.engine.status(c) != 1]:
                     self._finish(ctx, now)
-            elif k < len(todo):
+            elif k < len(todo) and not self.preparing:
                 time.sleep(max(0.0, min(0.01, todo[k].arrival - (clock() - t0))))
         wall = clock() - t0
         ok = [r for r in requests if not r.error]
@@ -139,9 +188,18 @@ class Server:
             "gcells_per_s": cells / wall / 1e9 if wall > 0 else 0.0,
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_mean_s": float(lat.mean()),
+            # time inside engine steps (GPU work plus its launch), and the
+            # mean number of shots sharing a step call
+            "engine_step_s": t_step, "mean_shots_per_call": shot_calls / n_calls if n_calls else 0.0,
         }
 
     def close(self):
+        self._todo.put(None)
+        self._worker.join()
+        while not self._ready.empty():
+            _, _, ctx, _ = self._ready.get()
+            if ctx is not None:
+                ctx.close()
         for ctx in list(self.running):
             ctx.close()
         self.running.clear()
