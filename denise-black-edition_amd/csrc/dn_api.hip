@@ -331,6 +331,52 @@ int receiver_factors(dn_ctx *c)
 
 }  // namespace
 
+namespace dn {
+// the device buffers, one entry per allocation, that dn_ctx_create,
+// dn_set_sources and dn_set_receivers make (cell lists aside)
+std::vector<size_t> plan_sizes(const dn_params &p, int nsrc, int ntr)
+{
+    Layout lay;
+    lay.init(p.nx, p.ny);
+    const size_t pb = lay.plane() * sizeof(float);
+    const bool gf2 = p.mode > 0 && p.grad_form == 2;
+    std::vector<size_t> v;
+    auto add = [&](size_t n, size_t count) {
+        for (size_t k = 0; k < count; ++k) v.push_back(n == 0 ? 4 : n);
+    };
+    add(pb, F_N - (gf2 ? 0 : 3));
+    add(pb, 2);                                   // rip, rjp
+    if (p.L == 0) add(pb, 3);                     // muipjp, f, g
+    if (p.mode > 0) add(pb, 3);                   // vp, vs, rho
+    if (p.L > 0) {
+        add(pb * p.L, 3);
+        add(pb, 6);
+        add(sizeof(float) * (p.nx + 1), 3);
+    }
+    add((size_t)p.ny * 2 * p.fw * sizeof(float) + 4, 5);
+    add((size_t)2 * p.fw * p.nx * sizeof(float) + 4, 4);
+    add(sizeof(float) * (2 * p.fw + 1), P_NVEC);
+    if (p.mode > 0) {
+        const size_t ntd = (size_t)std::ceil((float)p.nt / (float)p.dtinv);
+        add((size_t)p.nx * p.ny * ntd * sizeof(float), 5);
+        add((size_t)p.nx * p.ny * sizeof(float), 6);
+    }
+    add(sizeof(int), 1);
+    add(sizeof(double) * l2_scratch_doubles(), 1);
+    if (nsrc > 0) {
+        add(sizeof(int) * nsrc, 3);
+        add(sizeof(double) * nsrc, 2);
+        add(sizeof(float) * (size_t)nsrc * p.nt, 1);
+    }
+    if (ntr > 0) {
+        add(sizeof(int) * ntr, 2);
+        add(sizeof(double) * ntr, 2);
+        add((size_t)ntr * p.nt * sizeof(float), 9);
+    }
+    return v;
+}
+}  // namespace dn
+
 extern "C" {
 
 int dn_abi_version(void) { return DN_ABI_VERSION; }
@@ -347,28 +393,11 @@ int dn_device_count(int *count)
 }
 
 // Device bytes a context with these parameters, nsrc sources and ntr
-// receivers holds: the same allocations as dn_ctx_create, dn_set_sources and
-// dn_set_receivers (cell lists aside, which are a few bytes per point).
+// receivers holds (cell lists aside, which are a few bytes per point).
 static size_t plan_bytes(const dn_params &p, int nsrc, int ntr)
 {
-    Layout lay;
-    lay.init(p.nx, p.ny);
-    const size_t pb = lay.plane() * sizeof(float);
-    const bool gf2 = p.mode > 0 && p.grad_form == 2;
-    size_t b = pb * (F_N - (gf2 ? 0 : 3));
-    b += 2 * pb;                                  // rip, rjp
-    if (p.L == 0) b += 3 * pb;                    // muipjp, f, g
-    if (p.mode > 0) b += 3 * pb;                  // vp, vs, rho
-    if (p.L > 0) b += 3 * pb * p.L + 6 * pb + 3 * sizeof(float) * (p.nx + 1);
-    b += 5 * ((size_t)p.ny * 2 * p.fw * sizeof(float) + 4) + 4 * ((size_t)2 * p.fw * p.nx * sizeof(float) + 4);
-    b += P_NVEC * sizeof(float) * (2 * p.fw + 1);
-    if (p.mode > 0) {
-        const size_t ntd = (size_t)std::ceil((float)p.nt / (float)p.dtinv);
-        b += 5 * (size_t)p.nx * p.ny * ntd * sizeof(float) + 6 * (size_t)p.nx * p.ny * sizeof(float);
-    }
-    b += sizeof(int) + sizeof(double) * l2_scratch_doubles();
-    b += (size_t)nsrc * (3 * sizeof(int) + 2 * sizeof(double)) + (size_t)nsrc * p.nt * sizeof(float);
-    b += (size_t)ntr * (2 * sizeof(int) + 2 * sizeof(double)) + 9 * (size_t)ntr * p.nt * sizeof(float);
+    size_t b = 0;
+    for (size_t n : plan_sizes(p, nsrc, ntr)) b += n;
     return b;
 }
 
@@ -392,7 +421,17 @@ int dn_estimate_footprint(const dn_params *p, int nsrc, int ntr, size_t *bytes)
     return DN_OK;
 }
 
-int dn_ctx_create(const dn_params *p, int device, dn_ctx **out)
+static int ctx_create(const dn_params *p, int device, dn_arena *arena, dn_ctx **out);
+
+int dn_ctx_create(const dn_params *p, int device, dn_ctx **out) { return ctx_create(p, device, nullptr, out); }
+
+int dn_ctx_create_in(const dn_params *p, dn_arena *arena, dn_ctx **out)
+{
+    if (!arena) return fail(DN_ERR_ARG, "null arena");
+    return ctx_create(p, arena->device, arena, out);
+}
+
+static int ctx_create(const dn_params *p, int device, dn_arena *arena, dn_ctx **out)
 {
     if (!out) return fail(DN_ERR_ARG, "null output pointer");
     *out = nullptr;
@@ -406,6 +445,7 @@ int dn_ctx_create(const dn_params *p, int device, dn_ctx **out)
     if (!c) return fail(DN_ERR_OOM, "host allocation failed");
     c->p = *p;
     c->device = device;
+    c->arena = arena;
     if (hipSetDevice(device) != hipSuccess) {
         delete c;
         return fail(DN_ERR_HIP, "hipSetDevice failed");

@@ -7,6 +7,7 @@
 #include <utility>
 #include <vector>
 
+#include "dn_arena.h"
 #include "dn_internal.h"
 
 namespace dn {
@@ -83,11 +84,16 @@ struct dn_ctx {
     int sample_every = 0, samples = 0;
     std::vector<hipEvent_t> pool;
 
+    dn_arena *arena = nullptr;  // buffers come from this arena, else hipMalloc
     int dalloc(void **ptr, size_t nbytes)
     {
         if (nbytes == 0) nbytes = 4;
-        hipError_t e = hipMalloc(ptr, nbytes);
-        if (e != hipSuccess) {
+        hipError_t e = hipSuccess;
+        if (arena) {
+            *ptr = arena->alloc(nbytes);
+            if (!*ptr)
+                return fail(DN_ERR_OOM, "the arena has no free block of " + std::to_string(nbytes) + " bytes");
+        } else if ((e = hipMalloc(ptr, nbytes)) != hipSuccess) {
             *ptr = nullptr;
             return fail(DN_ERR_OOM, "hipMalloc of " + std::to_string(nbytes) + " bytes failed: " +
                                         hipGetErrorString(e));
@@ -101,11 +107,18 @@ struct dn_ctx {
         if (e != hipSuccess) return fail(DN_ERR_HIP, std::string("hipMemsetAsync failed: ") + hipGetErrorString(e));
         return DN_OK;
     }
+    void release(void *ptr)
+    {
+        if (arena)
+            (void)arena->release(ptr);
+        else
+            (void)hipFree(ptr);
+    }
     void dfree(void *ptr)
     {
         for (auto it = allocs.begin(); it != allocs.end(); ++it)
             if (it->first == ptr) {
-                (void)hipFree(ptr);
+                release(ptr);
                 bytes -= it->second;
                 allocs.erase(it);
                 return;
@@ -115,7 +128,7 @@ struct dn_ctx {
     {
         if (device >= 0) (void)hipSetDevice(device);
         if (stream) (void)hipStreamSynchronize(stream);
-        for (auto &pr : allocs) (void)hipFree(pr.first);
+        for (auto &pr : allocs) release(pr.first);
         for (auto ev : pool) (void)hipEventDestroy(ev);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);

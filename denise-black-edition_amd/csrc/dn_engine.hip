@@ -46,6 +46,7 @@ struct dn_engine {
     std::map<dn_ctx *, int> status;     // last known state per shot
     int *d_clock = nullptr;             // device clock (steps run)
     long clock = 0;                     // its value once the queued work is done
+    bool inflight = false;              // a launched run not yet waited for
     bool dirty = true;
     std::vector<Group> groups;
     BatchShot *d_shots = nullptr;
@@ -279,29 +280,52 @@ int dn_engine_attach(dn_engine *e, dn_ctx *c, int phase)
     return DN_OK;
 }
 
-int dn_engine_step(dn_engine *e, int nsteps, int *running)
+int dn_engine_launch(dn_engine *e, int nsteps)
 {
     if (!e || nsteps < 0) return fail(DN_ERR_ARG, "bad step request");
+    if (e->inflight) return fail(DN_ERR_STATE, "a launched run is still in flight (dn_engine_wait first)");
     HIP_OK(hipSetDevice(e->device));
     int rc;
-    if (e->dirty && (rc = rebuild(e))) return rc;
-    if (!e->jobs.empty() && nsteps > 0) {
-        // no more steps than the longest remaining shot needs
-        long need = 0;
-        for (const Job &j : e->jobs) need = std::max<long>(need, j.ctx->p.nt - (e->clock - j.base) + 1);
-        int left = (int)std::min<long>(nsteps, need);
-        e->clock += left;
-        if (e->graph_steps > 0 && left >= e->graph_steps) {
-            if (!e->gexec && (rc = capture(e))) return rc;
-            for (; left >= e->graph_steps; left -= e->graph_steps) HIP_OK(hipGraphLaunch(e->gexec, e->stream));
-        }
-        if (left > 0) enqueue_run(e, left);
-        HIP_OK(hipGetLastError());
+    const bool changed = e->dirty;
+    if (changed && (rc = rebuild(e))) return rc;
+    if (e->jobs.empty() || nsteps == 0) return DN_OK;
+    // no more steps than the longest remaining shot needs
+    long need = 0;
+    for (const Job &j : e->jobs) need = std::max<long>(need, j.ctx->p.nt - (e->clock - j.base) + 1);
+    int left = (int)std::min<long>(nsteps, need);
+    e->clock += left;
+    e->inflight = true;
+    // a graph is captured only for a set of shots that has already run
+    // unchanged once: capture and instantiation cost milliseconds of host
+    // time, which a set that changes at every call (a request stream) would
+    // pay each time while plain launches overlap the GPU work
+    if (e->graph_steps > 0 && left >= e->graph_steps && (e->gexec || !changed)) {
+        if (!e->gexec && (rc = capture(e))) return rc;
+        for (; left >= e->graph_steps; left -= e->graph_steps) HIP_OK(hipGraphLaunch(e->gexec, e->stream));
+    }
+    if (left > 0) enqueue_run(e, left);
+    HIP_OK(hipGetLastError());
+    return DN_OK;
+}
+
+int dn_engine_wait(dn_engine *e, int *running)
+{
+    if (!e) return fail(DN_ERR_ARG, "null engine");
+    if (e->inflight) {
+        HIP_OK(hipSetDevice(e->device));
+        e->inflight = false;
         HIP_OK(hipStreamSynchronize(e->stream));
+        int rc;
         if ((rc = collect(e))) return rc;
     }
     if (running) *running = (int)e->jobs.size();
     return DN_OK;
+}
+
+int dn_engine_step(dn_engine *e, int nsteps, int *running)
+{
+    const int rc = dn_engine_launch(e, nsteps);
+    return rc ? rc : dn_engine_wait(e, running);
 }
 
 int dn_engine_status(dn_engine *e, dn_ctx *c, int *status)

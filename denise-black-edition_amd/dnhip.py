@@ -100,6 +100,11 @@ def load():
         "dn_device_count": ([ip], C.c_int),
         "dn_ctx_create": ([C.POINTER(_CParams), C.c_int, C.POINTER(vp)], C.c_int),
         "dn_ctx_destroy": ([vp], C.c_int),
+        "dn_ctx_create_in": ([C.POINTER(_CParams), vp, C.POINTER(vp)], C.c_int),
+        "dn_arena_create": ([C.c_int, C.c_size_t, C.POINTER(vp)], C.c_int),
+        "dn_arena_destroy": ([vp], C.c_int),
+        "dn_arena_info": ([vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
+        "dn_arena_fits": ([vp, C.POINTER(_CParams), C.c_int, C.c_int, ip], C.c_int),
         "dn_set_model": ([vp, fp, fp, fp, fp, fp], C.c_int),
         "dn_set_sources": ([vp, C.c_int, fp, fp], C.c_int),
         "dn_set_receivers": ([vp, C.c_int, ip], C.c_int),
@@ -134,6 +139,8 @@ def load():
         "dn_engine_status": ([vp, vp, ip], C.c_int),
         "dn_engine_forget": ([vp, vp], C.c_int),
         "dn_engine_set_graph": ([vp, C.c_int], C.c_int),
+        "dn_engine_launch": ([vp, C.c_int], C.c_int),
+        "dn_engine_wait": ([vp, ip], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -182,15 +189,56 @@ def device_count():
     return n.value
 
 
-class Context:
-    """One shot's device state (dn_ctx)."""
+class Arena:
+    """Device-memory arena (dn_arena): one allocation carved into the
+    buffers of the contexts created in it; creating and destroying those
+    contexts allocates nothing on the device."""
 
-    def __init__(self, params: Params, device: int = 0):
+    def __init__(self, nbytes: int, device: int = 0):
+        h = C.c_void_p()
+        _check(load().dn_arena_create(device, nbytes, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def info(self):
+        """(free bytes, largest free block, total bytes)."""
+        f, l, t = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        _check(load().dn_arena_info(self.h, C.byref(f), C.byref(l), C.byref(t)))
+        return f.value, l.value, t.value
+
+    def fits(self, params: "Params", nsrc: int, ntr: int) -> bool:
+        """Would a context of these parameters fit in the free blocks now."""
+        cp = params.to_c()
+        r = C.c_int()
+        _check(load().dn_arena_fits(self.h, C.byref(cp), nsrc, ntr, C.byref(r)))
+        return bool(r.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(load().dn_arena_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One shot's device state (dn_ctx); with `arena`, its device buffers
+    come from that arena (on the arena's device)."""
+
+    def __init__(self, params: Params, device: int = 0, arena: Arena | None = None):
         lib = load()
         self.p = params
         self._cp = params.to_c()
         h = C.c_void_p()
-        _check(lib.dn_ctx_create(C.byref(self._cp), device, C.byref(h)))
+        if arena is not None:
+            _check(lib.dn_ctx_create_in(C.byref(self._cp), arena.h, C.byref(h)))
+            self._arena = arena   # the arena outlives its contexts
+        else:
+            _check(lib.dn_ctx_create(C.byref(self._cp), device, C.byref(h)))
         self.h = h
         nd = C.c_int()
         ntd = C.c_int()
@@ -366,6 +414,16 @@ class Engine:
     def step(self, nsteps: int) -> int:
         n = C.c_int()
         _check(load().dn_engine_step(self.h, nsteps, C.byref(n)))
+        return n.value
+
+    def launch(self, nsteps: int) -> None:
+        """Enqueue up to nsteps steps and return at once (see wait)."""
+        _check(load().dn_engine_launch(self.h, nsteps))
+
+    def wait(self) -> int:
+        """Wait for the launched run; returns the shots still attached."""
+        n = C.c_int()
+        _check(load().dn_engine_wait(self.h, C.byref(n)))
         return n.value
 
     def run(self, chunk: int = 200) -> None:

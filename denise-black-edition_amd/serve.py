@@ -49,11 +49,16 @@ class Request:
 
 class Server:
     def __init__(self, device: int = 0, budget_bytes: int | None = None, chunk: int = 64,
-                 graph_steps: int = 32, reserve_bytes: int = 2 << 30):
+                 graph_steps: int = 32, reserve_bytes: int = 2 << 30, arena: bool = True):
         self.device = device
         free, total = dnhip.device_memory(device)
         self.budget = budget_bytes if budget_bytes is not None else max(0, free - reserve_bytes)
         self.used = 0
+        # one device allocation for every request's state, made once here:
+        # contexts are then carved from it without hipMalloc / hipFree
+        self.arena = dnhip.Arena(self.budget, device) if arena and self.budget > 0 else None
+        self._requeue: list[Request] = []  # admitted, but the arena had no room after all
+        self._hold = False                 # admit nothing until a shot releases memory
         self.chunk = chunk
         self.engine = dnhip.Engine(device)
         self.engine.set_graph(graph_steps)
@@ -79,7 +84,7 @@ class Server:
             try:
                 if req.kind == "gradient" and req.params.mode == 0:
                     raise ValueError("gradient requests need params.mode = 1")
-                ctx = dnhip.Context(req.params, self.device)
+                ctx = dnhip.Context(req.params, self.device, arena=self.arena)
                 ctx.set_model(*req.model)
                 ctx.set_sources(req.srcpos, req.signals)
                 ctx.set_receivers(req.recpos)
@@ -106,6 +111,14 @@ class Server:
                 return
             block = False
             self.preparing -= 1
+            if isinstance(err, dnhip.DnError) and err.code == -3 and self.arena is not None:
+                # the arena's free blocks were too fragmented for this shot:
+                # it waits (first in line) until a running shot releases memory
+                self.used -= nbytes
+                req.admitted = -1.0
+                self._requeue.append(req)
+                self._hold = True
+                continue
             if err is not None:
                 req.error = f"preparation failed: {err}"
                 req.done = now
@@ -114,8 +127,10 @@ class Server:
             self.engine.attach(ctx, 0)
             self.running[ctx] = (req, 0, nbytes)
 
-    def _finish(self, ctx, now: float) -> None:
-        req, phase, nbytes = self.running.pop(ctx)
+    def _finish(self, ctx, entry, now: float) -> None:
+        """Results of a shot that left the engine at time `now` (runs while
+        the engine works on the next run of steps)."""
+        req, phase, nbytes = entry
         st = self.engine.status(ctx)
         if st != 0:
             req.error = f"shot failed with status {st}"
@@ -136,6 +151,7 @@ class Server:
         self.engine.forget(ctx)
         ctx.close()
         self.used -= nbytes
+        self._hold = False
 
     def serve(self, requests: list[Request], clock=time.perf_counter) -> dict:
         """Run a request stream (arrival times relative to the start) to
@@ -144,43 +160,56 @@ class Server:
         waiting: list[Request] = []
         t0 = clock()
         k = 0
-        cells = 0
-        t_step, n_calls, shot_calls = 0.0, 0, 0
-        while k < len(todo) or waiting or self.running or self.preparing:
+        t_wait, n_calls, shot_calls = 0.0, 0, 0
+        finished = []    # (ctx, entry, time) of shots whose results are not read yet
+        while k < len(todo) or waiting or self.running or self.preparing or finished:
             now = clock() - t0
             while k < len(todo) and todo[k].arrival <= now:
                 waiting.append(todo[k])
                 k += 1
             # admission by footprint, oldest first; a request that can never
             # fit is rejected, one that does not fit now waits
+            if self._requeue:
+                waiting = self._requeue + waiting
+                self._requeue = []
             still = []
             for req in waiting:
                 need = self.footprint(req)
-                if need > self.budget:
+                if self._hold:
+                    still.append(req)
+                elif need > self.budget:
                     req.error = f"footprint {need} B exceeds the budget {self.budget} B"
                     req.done = now
                 elif self.used + need <= self.budget:
                     self._admit(req, need, now)
-                    n = req.params.nx * req.params.ny * req.params.nt
-                    cells += n * (2 if req.kind == "gradient" else 1)
                 else:
                     still.append(req)
             waiting = still
-            # prepared shots join; with nothing running, wait for one
-            self._attach_ready(clock() - t0, block=not self.running and self.preparing > 0)
-            if self.running:
+            # prepared shots join; with nothing to do, wait for one
+            self._attach_ready(clock() - t0, block=not self.running and not finished and self.preparing > 0)
+            launched = bool(self.running)
+            if launched:
+                self.engine.launch(self.chunk)
+            # while the GPU runs: results of the shots finished in the last
+            # run (a gradient's forward phase turns into its adjoint phase,
+            # which joins the next run)
+            for ctx, entry, t in finished:
+                self._finish(ctx, entry, t)
+            finished = []
+            if launched:
                 ts = clock()
-                self.engine.step(self.chunk)
-                t_step += clock() - ts
+                self.engine.wait()
+                t_wait += clock() - ts
                 n_calls += 1
                 shot_calls += len(self.running)
                 now = clock() - t0
                 for ctx in [c for c in self.running if self.engine.status(c) != 1]:
-                    self._finish(ctx, now)
+                    finished.append((ctx, self.running.pop(ctx), now))
             elif k < len(todo) and not self.preparing:
                 time.sleep(max(0.0, min(0.01, todo[k].arrival - (clock() - t0))))
         wall = clock() - t0
         ok = [r for r in requests if not r.error]
+        cells = sum(r.params.nx * r.params.ny * r.params.nt * (2 if r.kind == "gradient" else 1) for r in ok)
         lat = np.array([r.done - r.arrival for r in ok]) if ok else np.zeros(1)
         return {
             "requests": len(requests), "completed": len(ok), "rejected": len(requests) - len(ok),
@@ -188,9 +217,9 @@ class Server:
             "gcells_per_s": cells / wall / 1e9 if wall > 0 else 0.0,
             "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
             "latency_mean_s": float(lat.mean()),
-            # time inside engine steps (GPU work plus its launch), and the
-            # mean number of shots sharing a step call
-            "engine_step_s": t_step, "mean_shots_per_call": shot_calls / n_calls if n_calls else 0.0,
+            # host time spent waiting for engine runs (GPU work not overlapped
+            # by host work), and the mean number of shots sharing a run
+            "engine_wait_s": t_wait, "mean_shots_per_run": shot_calls / n_calls if n_calls else 0.0,
         }
 
     def close(self):
@@ -204,6 +233,8 @@ class Server:
             ctx.close()
         self.running.clear()
         self.engine.close()
+        if self.arena is not None:
+            self.arena.close()
 
 
 def solve_alone(req: Request, device: int = 0) -> dict:

@@ -210,6 +210,28 @@ int dn_device_memory(int device, size_t *free_bytes, size_t *total_bytes);
  * admission test of a scheduler that packs shots into HBM. */
 int dn_estimate_footprint(const dn_params *p, int nsrc, int ntr, size_t *bytes);
 
+/* ---- Device-memory arena ---------------------------------------------------
+ * No reference counterpart (the reference allocates per process with
+ * malloc).  One hipMalloc of `bytes`, carved into the device buffers of the
+ * contexts created in it (best fit, 256-byte aligned, free neighbours
+ * coalesced).  Creating and destroying such contexts costs host bookkeeping
+ * only: hipMalloc / hipFree cost time that grows with the size and serialise
+ * with the process's other HIP calls, which stalls a server that admits
+ * gradient jobs of tens of GB while others run.  Thread-safe.  A context
+ * whose buffers do not fit fails with DN_ERR_OOM and leaves the arena as it
+ * was. */
+typedef struct dn_arena dn_arena;
+int dn_arena_create(int device, size_t bytes, dn_arena **out);
+/* DN_ERR_STATE while contexts still hold arena memory. */
+int dn_arena_destroy(dn_arena *a);
+int dn_arena_info(dn_arena *a, size_t *free_bytes, size_t *largest_free, size_t *total);
+/* *fits = 1 when the buffers of a context with these parameters, nsrc
+ * sources and ntr receivers would fit in the arena's free blocks now. */
+int dn_arena_fits(dn_arena *a, const dn_params *p, int nsrc, int ntr, int *fits);
+/* dn_ctx_create on the arena's device, with every device buffer (including
+ * those of dn_set_sources / dn_set_receivers) taken from the arena. */
+int dn_ctx_create_in(const dn_params *p, dn_arena *a, dn_ctx **out);
+
 /* ---- Multi-shot engine ----------------------------------------------------
  * No reference counterpart: the reference runs the shots of one process one
  * after another (src/PSV/FD_PSV.c:271-354).  The engine advances many
@@ -228,8 +250,16 @@ int dn_engine_destroy(dn_engine *e);
  * dn_calc_residuals (or dn_set_adjoint_sources). */
 int dn_engine_attach(dn_engine *e, dn_ctx *ctx, int phase);
 /* Advance every attached shot by up to nsteps time steps; shots that reach
- * their NT are detached.  *running (may be NULL) = shots still attached. */
+ * their NT are detached.  *running (may be NULL) = shots still attached.
+ * Same as dn_engine_launch followed by dn_engine_wait. */
 int dn_engine_step(dn_engine *e, int nsteps, int *running);
+/* Enqueue up to nsteps time steps of every attached shot and return at once.
+ * Until dn_engine_wait, the host may attach further contexts (they join the
+ * next launch) and read or release contexts that are not attached, so that
+ * host work overlaps the GPU run; no second launch. */
+int dn_engine_launch(dn_engine *e, int nsteps);
+/* Wait for the launched run; shots that reached their NT are detached. */
+int dn_engine_wait(dn_engine *e, int *running);
 /* 1 running, 0 finished, DN_ERR_UNSTABLE if its NaN probe tripped. */
 int dn_engine_status(dn_engine *e, dn_ctx *ctx, int *status);
 /* Drop the engine's record of a finished context (before destroying it). */

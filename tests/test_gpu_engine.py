@@ -97,6 +97,35 @@ def test_engine_shots_join_while_others_run():
         _same(x, y, case, f"join{k}")
 
 
+def test_engine_launch_wait_attach_in_flight():
+    """Asynchronous runs: a shot attached while a run is in flight joins the
+    next run; a second launch before the wait is refused; results equal the
+    shots alone."""
+    cases = _mix()[1:4]
+    alone = []
+    for case in cases:
+        g = _ctx(case)
+        g.forward()
+        alone.append(g)
+    eng = dnhip.Engine()
+    eng.set_graph(8)
+    a, b, c = (_ctx(case) for case in cases)
+    eng.attach(a)
+    eng.launch(40)
+    with pytest.raises(dnhip.DnError) as ei:
+        eng.launch(10)
+    assert ei.value.code == -5
+    eng.attach(b)              # joins the next run
+    assert eng.wait() == 2
+    eng.launch(17)
+    eng.attach(c)
+    eng.wait()
+    while eng.step(30) > 0:
+        pass
+    for k, (case, x, y) in enumerate(zip(cases, alone, (a, b, c))):
+        _same(x, y, case, f"async{k}")
+
+
 def test_engine_gradient_phases():
     """Forward (with snapshots) and adjoint phases through the engine give
     the single-shot gradient of every shot."""
