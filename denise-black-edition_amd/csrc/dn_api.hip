@@ -180,14 +180,60 @@ int build_lists(dn_ctx *c)
     if (c->p.quelltypb >= 4)
         for (int r = 0; r < nr; ++r) add(3, c->rec_ix[r], c->rec_iy[r], E_ADJ_P, r);
     for (int w = 0; w < 4; ++w) {
-        std::stable_sort(lists[w].begin(), lists[w].end(),
-                         [](const Item &x, const Item &y) { return x.tile < y.tile; });
+        // by tile, then lane; entries of one cell keep their order
+        std::stable_sort(lists[w].begin(), lists[w].end(), [](const Item &x, const Item &y) {
+            return x.tile < y.tile || (x.tile == y.tile && x.e.lane < y.e.lane);
+        });
         const int ntiles = (w == 0 || w == 2) ? c->ntiles_v : c->ntiles_s;
         std::vector<int> off((size_t)ntiles + 1, 0);
         std::vector<Ent> ent(lists[w].size() + 1);
         for (auto &it : lists[w]) off[(size_t)it.tile + 1]++;
         for (int k = 0; k < ntiles; ++k) off[(size_t)k + 1] += off[k];
         for (size_t k = 0; k < lists[w].size(); ++k) ent[k] = lists[w][k].e;
+        // lean tiles: none of the sweep body's boundary or point paths can
+        // trigger in them (the conditions below are the body's own tests,
+        // taken over the tile's column and row range)
+        {
+            const bool vel = w == 0 || w == 2;
+            const Tile &t = vel ? c->tile_v : c->tile_s;
+            const int ntx = vel ? c->ntx_v : c->ntx_s;
+            const dn_params &p = c->p;
+            c->lean_h[w].clear();
+            c->full_h[w].clear();
+            for (int k = 0; k < ntiles; ++k) {
+                const int tx = k % ntx, ty = k / ntx;
+                const int ilo = tx * t.width() + 1, ihi = ilo + t.width() - 1;
+                const int jt = ty * t.rows() + 1, jb = jt + t.rows() - 1;
+                bool lean = true;
+                if (!p.boundary && (ilo <= p.fw || ihi >= p.nx - p.fw + 1)) lean = false;  // x CPML
+                if (!p.free_surf && jt <= p.fw) lean = false;                              // top CPML
+                if (jb >= p.ny - p.fw + 1) lean = false;                                   // bottom CPML
+                if (p.free_surf && jt <= c->fdoh) lean = false;                            // surface row / ghost rows
+                if (off[(size_t)k + 1] > off[k]) lean = false;                             // sources, receivers
+                if (p.nx / 2 >= ilo && p.nx / 2 <= ihi && p.ny / 2 >= jt && p.ny / 2 <= jb) lean = false;  // NaN probe
+                (lean ? c->lean_h[w] : c->full_h[w]).push_back(k);
+            }
+            c->lst_nlean[w] = (int)c->lean_h[w].size();
+            c->lst_nfull[w] = (int)c->full_h[w].size();
+            off.insert(off.end(), c->lean_h[w].begin(), c->lean_h[w].end());
+            off.insert(off.end(), c->full_h[w].begin(), c->full_h[w].end());
+            // per-lane entry ranges of the tiles that have entries
+            const int width = t.width();
+            std::vector<int> slot((size_t)ntiles, -1), lo;
+            int ns = 0;
+            for (int k = 0; k < ntiles; ++k) {
+                const int b0 = off[k], b1 = off[(size_t)k + 1];
+                if (b1 == b0) continue;
+                slot[k] = ns++;
+                int e = b0;
+                for (int l = 0; l <= width; ++l) {
+                    while (e < b1 && lists[w][(size_t)e].e.lane < l) ++e;
+                    lo.push_back(e);
+                }
+            }
+            off.insert(off.end(), slot.begin(), slot.end());
+            off.insert(off.end(), lo.begin(), lo.end());
+        }
         if (c->lst_off[w]) c->dfree(c->lst_off[w]);
         if (c->lst_ent[w]) c->dfree(c->lst_ent[w]);
         c->lst_off[w] = nullptr;
@@ -211,6 +257,18 @@ StepCtl dn::base_ctl(dn_ctx *c, int lv, int ls)
     s.vlist.ent = c->lst_ent[lv];
     s.slist.off = c->lst_off[ls];
     s.slist.ent = c->lst_ent[ls];
+    s.vlist.nlean = c->lst_nlean[lv];
+    s.vlist.nfull = c->lst_nfull[lv];
+    s.vlist.lean = c->lst_off[lv] + (c->ntiles_v + 1);
+    s.vlist.full = s.vlist.lean + s.vlist.nlean;
+    s.vlist.slot = s.vlist.full + s.vlist.nfull;
+    s.vlist.lane_off = s.vlist.slot + c->ntiles_v;
+    s.slist.nlean = c->lst_nlean[ls];
+    s.slist.nfull = c->lst_nfull[ls];
+    s.slist.lean = c->lst_off[ls] + (c->ntiles_s + 1);
+    s.slist.full = s.slist.lean + s.slist.nlean;
+    s.slist.slot = s.slist.full + s.slist.nfull;
+    s.slist.lane_off = s.slist.slot + c->ntiles_s;
     for (int k = 0; k < 5; ++k) s.fwd[k] = c->fwd[k];
     s.wc = c->wc;
     s.wcu = c->wcu;

@@ -5,40 +5,41 @@ namespace dn {
 namespace {
 
 template <int TBX, int BY, int TY, int VW, int F>
-void bstr_f(const BatchShot *shots, const BatchBlock *blk, int nb, const int *clock, int k, int L, hipStream_t s)
+void bstr_f(const BatchShot *shots, const BatchBlock *blk, int nb, int nfull, const int *clock, int k, int L,
+            hipStream_t s)
 {
     switch (L) {
-    case 0: hipLaunchKernelGGL((kb_str<F, TY, TBX, BY, 0, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, clock, k); break;
-    case 1: hipLaunchKernelGGL((kb_str<F, TY, TBX, BY, 1, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, clock, k); break;
-    case 2: hipLaunchKernelGGL((kb_str<F, TY, TBX, BY, 2, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, clock, k); break;
-    case 3: hipLaunchKernelGGL((kb_str<F, TY, TBX, BY, 3, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, clock, k); break;
-    default: hipLaunchKernelGGL((kb_str<F, TY, TBX, BY, -1, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, clock, k); break;
+    case 0: hipLaunchKernelGGL((kb_str<F, TY, TBX, BY, 0, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, nfull, clock, k); break;
+    case 1: hipLaunchKernelGGL((kb_str<F, TY, TBX, BY, 1, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, nfull, clock, k); break;
+    case 2: hipLaunchKernelGGL((kb_str<F, TY, TBX, BY, 2, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, nfull, clock, k); break;
+    case 3: hipLaunchKernelGGL((kb_str<F, TY, TBX, BY, 3, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, nfull, clock, k); break;
+    default: hipLaunchKernelGGL((kb_str<F, TY, TBX, BY, -1, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, nfull, clock, k); break;
     }
 }
 
 template <int TBX, int BY, int TY, int VW>
-void bstr(const BatchShot *shots, const BatchBlock *blk, int nb, const int *clock, int k, int fdoh, int L,
+void bstr(const BatchShot *shots, const BatchBlock *blk, int nb, int nfull, const int *clock, int k, int fdoh, int L,
           hipStream_t s)
 {
     switch (fdoh) {
-    case 1: return bstr_f<TBX, BY, TY, VW, 1>(shots, blk, nb, clock, k, L, s);
-    case 2: return bstr_f<TBX, BY, TY, VW, 2>(shots, blk, nb, clock, k, L, s);
-    case 3: return bstr_f<TBX, BY, TY, VW, 3>(shots, blk, nb, clock, k, L, s);
-    case 4: return bstr_f<TBX, BY, TY, VW, 4>(shots, blk, nb, clock, k, L, s);
-    case 5: return bstr_f<TBX, BY, TY, VW, 5>(shots, blk, nb, clock, k, L, s);
-    default: return bstr_f<TBX, BY, TY, VW, 6>(shots, blk, nb, clock, k, L, s);
+    case 1: return bstr_f<TBX, BY, TY, VW, 1>(shots, blk, nb, nfull, clock, k, L, s);
+    case 2: return bstr_f<TBX, BY, TY, VW, 2>(shots, blk, nb, nfull, clock, k, L, s);
+    case 3: return bstr_f<TBX, BY, TY, VW, 3>(shots, blk, nb, nfull, clock, k, L, s);
+    case 4: return bstr_f<TBX, BY, TY, VW, 4>(shots, blk, nb, nfull, clock, k, L, s);
+    case 5: return bstr_f<TBX, BY, TY, VW, 5>(shots, blk, nb, nfull, clock, k, L, s);
+    default: return bstr_f<TBX, BY, TY, VW, 6>(shots, blk, nb, nfull, clock, k, L, s);
     }
 }
 
 }  // namespace
 
-void launch_batch_str(const BatchShot *shots, const BatchBlock *blk, int nblocks, const int *clock, int k,
-                      const Tile &t, int fdoh, int L, void *stream)
+void launch_batch_str(const BatchShot *shots, const BatchBlock *blk, int nblocks, int nfull, const int *clock,
+                      int k, const Tile &t, int fdoh, int L, void *stream)
 {
     if (nblocks <= 0) return;
     hipStream_t s = (hipStream_t)stream;
-    if (t.tbx == 64 && t.by == 4 && t.ty == 1 && t.vw == 4) return bstr<64, 4, 1, 4>(shots, blk, nblocks, clock, k, fdoh, L, s);
-    if (t.tbx == 64 && t.by == 4 && t.ty == 1 && t.vw == 1) return bstr<64, 4, 1, 1>(shots, blk, nblocks, clock, k, fdoh, L, s);
+    if (t.tbx == 64 && t.by == 4 && t.ty == 1 && t.vw == 4) return bstr<64, 4, 1, 4>(shots, blk, nblocks, nfull, clock, k, fdoh, L, s);
+    if (t.tbx == 64 && t.by == 4 && t.ty == 1 && t.vw == 1) return bstr<64, 4, 1, 1>(shots, blk, nblocks, nfull, clock, k, fdoh, L, s);
 }
 
 }  // namespace dn

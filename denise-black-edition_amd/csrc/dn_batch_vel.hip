@@ -9,15 +9,15 @@ namespace {
 #define DN_BATCH_VTILES(X) X(128, 2, 4, 1) X(64, 4, 2, 1)
 
 template <int TBX, int BY, int TY, int VW>
-void bvel(const BatchShot *shots, const BatchBlock *blk, int nb, const int *clock, int k, int fdoh, int sw,
-          hipStream_t s)
+void bvel(const BatchShot *shots, const BatchBlock *blk, int nb, int nfull, const int *clock, int k, int fdoh,
+          int sw, hipStream_t s)
 {
 #define DN_BV(F)                                                                                          \
     case F:                                                                                               \
         if (sw == 0)                                                                                      \
-            hipLaunchKernelGGL((kb_vel<F, TY, TBX, BY, 0, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, clock, k); \
+            hipLaunchKernelGGL((kb_vel<F, TY, TBX, BY, 0, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, nfull, clock, k); \
         else                                                                                              \
-            hipLaunchKernelGGL((kb_vel<F, TY, TBX, BY, 1, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, clock, k); \
+            hipLaunchKernelGGL((kb_vel<F, TY, TBX, BY, 1, VW>), dim3(nb), dim3(TBX, BY), 0, s, shots, blk, nb, nfull, clock, k); \
         return;
     switch (fdoh) {
         DN_BV(1) DN_BV(2) DN_BV(3) DN_BV(4) DN_BV(5) DN_BV(6)
@@ -40,13 +40,13 @@ bool batch_tile_supported(const Tile &t, int sweep)
     return (t.tbx == 64 && t.by == 4 && t.ty == 1 && (t.vw == 1 || t.vw == 4));
 }
 
-void launch_batch_vel(const BatchShot *shots, const BatchBlock *blk, int nblocks, const int *clock, int k,
-                      const Tile &t, int fdoh, int sw, void *stream)
+void launch_batch_vel(const BatchShot *shots, const BatchBlock *blk, int nblocks, int nfull, const int *clock,
+                      int k, const Tile &t, int fdoh, int sw, void *stream)
 {
     if (nblocks <= 0) return;
     hipStream_t s = (hipStream_t)stream;
 #define X(B, Y, T, W) \
-    if (t.tbx == B && t.by == Y && t.ty == T && t.vw == W) return bvel<B, Y, T, W>(shots, blk, nblocks, clock, k, fdoh, sw, s);
+    if (t.tbx == B && t.by == Y && t.ty == T && t.vw == W) return bvel<B, Y, T, W>(shots, blk, nblocks, nfull, clock, k, fdoh, sw, s);
     DN_BATCH_VTILES(X)
 #undef X
 }

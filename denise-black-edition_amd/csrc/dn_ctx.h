@@ -76,8 +76,10 @@ struct dn_ctx {
     Tile tile_v, tile_s;  // velocity / stress sweep shapes
     int nt_v = 0, nt_s = 0;  // streaming (nontemporal) stores of each sweep
     int ntx_v = 0, ntiles_v = 0, ntx_s = 0, ntiles_s = 0;
-    int *lst_off[4] = {};
+    int *lst_off[4] = {};   // per list: CSR offsets (ntiles+1), then the lean and the full tile indices
     Ent *lst_ent[4] = {};
+    int lst_nlean[4] = {}, lst_nfull[4] = {};
+    std::vector<int> lean_h[4], full_h[4];  // host copies of the tile sets
     // timing
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double run_ms = 0.0, v_ms = 0.0, s_ms = 0.0;

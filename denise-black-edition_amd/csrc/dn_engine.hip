@@ -34,6 +34,7 @@ struct Group {
     int fdoh, L, sw;
     Tile tv, ts;
     int off_v = 0, nblk_v = 0, off_s = 0, nblk_s = 0;
+    int nfull_v = 0, nfull_s = 0;  // full-tile blocks at the head of each table
 };
 
 
@@ -101,7 +102,7 @@ int rebuild(dn_engine *e)
         g.sw = e->jobs[kv.second[0]].phase;
         g.tv = c0->tile_v;
         g.ts = c0->tile_s;
-        std::vector<BatchBlock> bv, bs;
+        std::vector<BatchBlock> bv, bs, lv, ls;  // full and lean blocks of each sweep
         for (int k : kv.second) {
             const Job &j = e->jobs[k];
             dn_ctx *c = j.ctx;
@@ -128,15 +129,22 @@ int rebuild(dn_engine *e)
             const int idx = (int)shots.size();
             shots.push_back(b);
             const int b0 = (int)j.base;
-            for (int t = 0; t < b.cv.ntiles; ++t) bv.push_back(BatchBlock{idx, t, b0, b.nt_total});
-            for (int t = 0; t < b.cs.ntiles; ++t) bs.push_back(BatchBlock{idx, t, b0, b.nt_total});
+            const int wv = j.phase == 0 ? 0 : 2, ws = wv + 1;
+            for (int t : c->full_h[wv]) bv.push_back(BatchBlock{idx, t, b0, b.nt_total});
+            for (int t : c->lean_h[wv]) lv.push_back(BatchBlock{idx, t, b0, b.nt_total});
+            for (int t : c->full_h[ws]) bs.push_back(BatchBlock{idx, t, b0, b.nt_total});
+            for (int t : c->lean_h[ws]) ls.push_back(BatchBlock{idx, t, b0, b.nt_total});
         }
         g.off_v = (int)blk.size();
-        g.nblk_v = (int)bv.size();
+        g.nfull_v = (int)bv.size();
+        g.nblk_v = (int)(bv.size() + lv.size());
         blk.insert(blk.end(), bv.begin(), bv.end());
+        blk.insert(blk.end(), lv.begin(), lv.end());
         g.off_s = (int)blk.size();
-        g.nblk_s = (int)bs.size();
+        g.nfull_s = (int)bs.size();
+        g.nblk_s = (int)(bs.size() + ls.size());
         blk.insert(blk.end(), bs.begin(), bs.end());
+        blk.insert(blk.end(), ls.begin(), ls.end());
         e->groups.push_back(g);
     }
     if ((int)shots.size() > e->cap_shots) {
@@ -162,9 +170,9 @@ int rebuild(dn_engine *e)
 void enqueue_step(dn_engine *e, int k)
 {
     for (const Group &g : e->groups)
-        launch_batch_vel(e->d_shots, e->d_blk + g.off_v, g.nblk_v, e->d_clock, k, g.tv, g.fdoh, g.sw, e->stream);
+        launch_batch_vel(e->d_shots, e->d_blk + g.off_v, g.nblk_v, g.nfull_v, e->d_clock, k, g.tv, g.fdoh, g.sw, e->stream);
     for (const Group &g : e->groups)
-        launch_batch_str(e->d_shots, e->d_blk + g.off_s, g.nblk_s, e->d_clock, k, g.ts, g.fdoh, g.L, e->stream);
+        launch_batch_str(e->d_shots, e->d_blk + g.off_s, g.nblk_s, g.nfull_s, e->d_clock, k, g.ts, g.fdoh, g.L, e->stream);
 }
 
 // a run of n steps, then the clock moves on by n

@@ -133,6 +133,14 @@ struct Ent {
 struct CellList {
     const int *off = nullptr;
     const Ent *ent = nullptr;
+    // tiles of the sweep in two disjoint sets, together all tiles: lean
+    // (interior: no point, no boundary work) and full (the rest)
+    const int *lean = nullptr, *full = nullptr;
+    int nlean = 0, nfull = 0;
+    // entries of a tile are sorted by lane; for a tile with entries,
+    // lane_off[slot[tile] * (width + 1) + l] is its first entry of lane >= l
+    // (absolute index), so a thread visits only its own columns' entries
+    const int *slot = nullptr, *lane_off = nullptr;
 };
 
 // Tile shape of the two sweeps: blocks of tbx x by threads, each thread
@@ -193,10 +201,11 @@ struct alignas(16) BatchBlock {
 // k: offset of this step from it (runs of steps are enqueued, or captured
 // in a graph, with k = 0, 1, ... and the clock moved on once at the end)
 bool batch_tile_supported(const Tile &t, int sweep);
-void launch_batch_vel(const BatchShot *shots, const BatchBlock *blk, int nblocks, const int *clock, int k,
-                      const Tile &t, int fdoh, int sw, void *stream);
-void launch_batch_str(const BatchShot *shots, const BatchBlock *blk, int nblocks, const int *clock, int k,
-                      const Tile &t, int fdoh, int L, void *stream);
+// the first nfull blocks of blk are full tiles, the rest lean tiles
+void launch_batch_vel(const BatchShot *shots, const BatchBlock *blk, int nblocks, int nfull, const int *clock,
+                      int k, const Tile &t, int fdoh, int sw, void *stream);
+void launch_batch_str(const BatchShot *shots, const BatchBlock *blk, int nblocks, int nfull, const int *clock,
+                      int k, const Tile &t, int fdoh, int L, void *stream);
 void launch_batch_clock(int *clock, int add, void *stream);
 
 // kernel launchers (dn_kernels.hip); all enqueue on `stream`
