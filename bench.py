@@ -227,12 +227,12 @@ def serving_workload(nreq, qps, seed=77):
     return reqs
 
 
-def run_serving(nreq, qps):
+def run_serving(nreq, qps, graph_steps=32):
     """p50/p99 latency and throughput of the request server, and the same
     requests one at a time with the plain solver (no overlap)."""
     import serve
     reqs = serving_workload(nreq, qps)
-    srv = serve.Server()
+    srv = serve.Server(graph_steps=graph_steps)
     stats = srv.serve(reqs)
     srv.close()
     # the plain one-at-a-time solver on the same stream: each request starts
@@ -321,6 +321,9 @@ def main():
     ap.add_argument("--forms-only", action="store_true",
                     help="only the batched and served forms (no single-shot lines, no CPU baseline)")
     ap.add_argument("--batch-shots", type=int, default=64, help="multi-shot workload size (0: skip)")
+    ap.add_argument("--graph-steps", type=int, default=32,
+                    help="steps per captured hipGraph in the batched and served forms (0: plain launches, "
+                         "e.g. under a kernel-trace profiler)")
     ap.add_argument("--serve-requests", type=int, default=120, help="serving workload size (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=8.0)
     ap.add_argument("--nt", type=int, default=6000, help="profiling: shorter shots")
@@ -344,9 +347,9 @@ def main():
     if args.forms_only:
         out = {}
         if args.batch_shots > 0:
-            out["batched"] = run_batched(args.batch_shots)
+            out["batched"] = run_batched(args.batch_shots, args.graph_steps)
         if args.serve_requests > 0:
-            out["serving"] = run_serving(args.serve_requests, args.serve_qps)
+            out["serving"] = run_serving(args.serve_requests, args.serve_qps, args.graph_steps)
         print(json.dumps(out))
         return
     if args.no_visco:
@@ -361,11 +364,11 @@ def main():
         print(f"# elastic: {elastic}", file=sys.stderr, flush=True)
     batched = None
     if args.batch_shots > 0:
-        batched = run_batched(args.batch_shots)
+        batched = run_batched(args.batch_shots, args.graph_steps)
         print(f"# batched: {batched}", file=sys.stderr, flush=True)
     serving = None
     if args.serve_requests > 0:
-        serving = run_serving(args.serve_requests, args.serve_qps)
+        serving = run_serving(args.serve_requests, args.serve_qps, args.graph_steps)
         print(f"# serving: {serving}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
