@@ -166,6 +166,49 @@ def test_engine_gradient_phases():
             assert np.array_equal(g.gradient(w), single[k][w]), (k, w)
 
 
+def test_engine_lean_tiles_equal_single_shots():
+    """Grids with interior (lean) tiles for the engine's shapes, forward and
+    gradient phases: every shot equals its single-shot run bit for bit."""
+    cases = [make_case(nx=600, ny=200, nt=160, fdorder=8, L=0, seismo=4, seed=21, src_xz=(3000.0, 900.0)),
+             make_case(nx=520, ny=180, nt=140, fdorder=4, L=2, seismo=1, seed=22, src_xz=(2000.0, 700.0))]
+    alone = []
+    for case in cases:
+        g = _ctx(case)
+        g.forward()
+        alone.append(g)
+    eng = dnhip.Engine()
+    shots = [_ctx(case) for case in cases]
+    for g in shots:
+        eng.attach(g)
+    eng.run(chunk=40)
+    for k, (case, a, b) in enumerate(zip(cases, alone, shots)):
+        _same(a, b, case, f"lean{k}")
+    gc = make_case(nx=420, ny=150, nt=200, fdorder=8, L=0, mode=1, dtinv=2, fc=15.0, nrec=60, seed=23)
+    p, m, src, sig, rec = gc
+    mt = dict(m)
+    mt["vp"] = m["vp"] * 1.03
+    t = dnhip.Context(dnhip.Params(**{**p.__dict__, "mode": 0}))
+    t.set_model(mt["vp"], mt["vs"], mt["rho"], mt["taus"], mt["taup"])
+    t.set_sources(src, sig)
+    t.set_receivers(rec)
+    t.forward()
+    ox, oy = t.seismogram(0), t.seismogram(1)
+    single = _ctx(gc)
+    single.forward()
+    single.residuals(ox, oy, None, 1)
+    single.adjoint()
+    single.assemble()
+    g = _ctx(gc)
+    eng.attach(g, 0)
+    eng.run(chunk=64)
+    g.residuals(ox, oy, None, 1)
+    eng.attach(g, 1)
+    eng.run(chunk=64)
+    g.assemble()
+    for w in range(6):
+        assert np.array_equal(g.gradient(w), single.gradient(w)), w
+
+
 def test_engine_rejects_moment_tensor_and_double_attach():
     case = make_case(nx=64, ny=48, nt=50, stype=5)
     g = _ctx(case)

@@ -304,6 +304,40 @@ def test_tile_shapes_agree(tile, L):
         assert rel_l2(g.field(fid), o.field(fid)) <= SEIS_TOL, (tile, name)
 
 
+LEAN_CASES = [(t, 0) for t in TILES] + [(t, 2) for t in [(128, 2, 4, 1), (64, 4, 1, 4), (64, 4, 2, 4), (128, 2, 2, 2)]]
+
+
+@pytest.mark.parametrize("tile,L", LEAN_CASES)
+def test_lean_interior_tiles_exact(tile, L):
+    """A grid large enough for interior (lean) tiles in every shape, with an
+    explosive source and a receiver line in the interior (so full tiles sit
+    inside the lean region too) and a surface line: seismograms, wavefields,
+    CPML and memory variables equal the FP32 oracle bit for bit."""
+    import synth
+    p, m, src, sig, rec = make_case(nx=600, ny=260, nt=240, fdorder=8, L=L, seismo=4, fw=10,
+                                    src_xz=(2950.0, 1310.0), nrec=280, seed=11)
+    # the surface line plus an interior line 100 m above the source
+    rec = np.concatenate([rec, synth.receiver_line(2500.0, 30.0, 30, 1210.0, 10.0)], axis=1)
+    case = (p, m, src, sig, rec)
+    o = run_oracle(case)
+    g = dnhip.Context(p)
+    g.set_tile(*tile)
+    g.set_model(m["vp"], m["vs"], m["rho"], m["taus"], m["taup"])
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    g.forward()
+    for comp in range(5):
+        assert np.abs(o.seismogram(comp)).max() > 0, comp
+        assert np.array_equal(g.seismogram(comp), o.seismogram(comp)), (tile, L, "seis", comp)
+    for fid, name in FIELDS:
+        assert np.array_equal(g.field(fid), o.field(fid)), (tile, L, name)
+    for pid in range(9):
+        assert np.array_equal(g.psi(pid), o.psi(pid)), (tile, L, "psi", pid)
+    for l in range(1, L + 1):
+        for which in range(3):
+            assert np.array_equal(g.memvar(which, l), o.memvar(which, l)), (tile, L, "memvar", which, l)
+
+
 @pytest.mark.parametrize("tile", [(128, 2, 1, 1), (64, 4, 2, 1), (128, 2, 4, 1), (64, 4, 1, 4), (64, 2, 4, 4),
                                   (128, 2, 2, 2)])
 def test_tile_shapes_gradient(tile):
