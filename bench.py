@@ -227,14 +227,25 @@ def serving_workload(nreq, qps, seed=77):
     return reqs
 
 
-def run_serving(nreq, qps, graph_steps=32):
+def run_serving(nreq, qps, graph_steps=32, saturate_qps=0.0):
     """p50/p99 latency and throughput of the request server, and the same
-    requests one at a time with the plain solver (no overlap)."""
+    requests one at a time with the plain solver (no overlap).  With
+    saturate_qps, the server also runs the same requests arriving that fast
+    (its throughput when requests queue up; the plain solver is already
+    saturated at the base rate)."""
     import serve
     reqs = serving_workload(nreq, qps)
     srv = serve.Server(graph_steps=graph_steps)
     stats = srv.serve(reqs)
     srv.close()
+    if saturate_qps > 0:
+        burst = serving_workload(nreq, saturate_qps)
+        srv = serve.Server(graph_steps=graph_steps)
+        sat = srv.serve(burst)
+        srv.close()
+        stats["saturated"] = {k: sat[k] for k in ("shots_per_s", "gcells_per_s", "latency_p50_s", "latency_p99_s",
+                                                  "wall_s", "completed")}
+        stats["saturated"]["qps"] = saturate_qps
     # the plain one-at-a-time solver on the same stream: each request starts
     # when it has arrived and the previous one is done
     t_free, lat, t0 = 0.0, [], time.perf_counter()
@@ -321,6 +332,8 @@ def main():
     ap.add_argument("--forms-only", action="store_true",
                     help="only the batched and served forms (no single-shot lines, no CPU baseline)")
     ap.add_argument("--batch-shots", type=int, default=64, help="multi-shot workload size (0: skip)")
+    ap.add_argument("--serve-saturate-qps", type=float, default=64.0,
+                    help="also serve the same requests arriving at this rate (0: skip)")
     ap.add_argument("--graph-steps", type=int, default=32,
                     help="steps per captured hipGraph in the batched and served forms (0: plain launches, "
                          "e.g. under a kernel-trace profiler)")
@@ -349,7 +362,7 @@ def main():
         if args.batch_shots > 0:
             out["batched"] = run_batched(args.batch_shots, args.graph_steps)
         if args.serve_requests > 0:
-            out["serving"] = run_serving(args.serve_requests, args.serve_qps, args.graph_steps)
+            out["serving"] = run_serving(args.serve_requests, args.serve_qps, args.graph_steps, args.serve_saturate_qps)
         print(json.dumps(out))
         return
     if args.no_visco:
@@ -368,7 +381,7 @@ def main():
         print(f"# batched: {batched}", file=sys.stderr, flush=True)
     serving = None
     if args.serve_requests > 0:
-        serving = run_serving(args.serve_requests, args.serve_qps, args.graph_steps)
+        serving = run_serving(args.serve_requests, args.serve_qps, args.graph_steps, args.serve_saturate_qps)
         print(f"# serving: {serving}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
