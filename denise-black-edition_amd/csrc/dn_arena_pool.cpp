@@ -2,6 +2,7 @@
 // best-fit allocation over the free blocks, split on allocation, coalesced
 // with free neighbours on release.
 #include <algorithm>
+#include <cstdint>
 #include <set>
 
 #include "dn_arena.h"
@@ -47,9 +48,11 @@ void *dn_arena::alloc(size_t n)
 bool dn_arena::release(void *p)
 {
     std::lock_guard<std::mutex> lk(mu);
-    const size_t off = (size_t)((char *)p - base);
+    const std::uintptr_t a = reinterpret_cast<std::uintptr_t>(p), b = reinterpret_cast<std::uintptr_t>(base);
+    if (a < b || a >= b + size) return false;
+    const size_t off = (size_t)(a - b);
     auto u = used.find(off);
-    if ((char *)p < base || u == used.end()) return false;
+    if (u == used.end()) return false;
     size_t start = off, len = u->second;
     used.erase(u);
     free_bytes += len;
