@@ -37,13 +37,15 @@ METRIC = "Gcell-updates/s and %HBM roofline per shot; batched shots/s + p50/p99"
 
 
 def bytes_per_cell(L, tau_uniform=False):
-    """Algorithmic HBM bytes per cell-update: velocity sweep 36 B plus the
-    stress sweep (elastic 44 B; viscoelastic 4*(14+6L) B, of which 12 B are
-    the three relaxation-time planes that the kernel does not read when the
-    relaxation times are uniform -- then 4*(11+6L) B)."""
+    """Algorithmic HBM bytes per cell-update of each sweep, the per-unit
+    figures of the path's byte accounting: velocity 36 B; stress elastic
+    44 B, viscoelastic 4*(14+6L) B (104 B at L=2).  Third value: the bytes
+    the stress kernel moves -- with uniform relaxation times it does not read
+    the three relaxation-time planes, 12 B fewer."""
     vel = 36
-    stress = 44 if L == 0 else 4 * (14 + 6 * L) - (12 if tau_uniform else 0)
-    return vel, stress
+    stress = 44 if L == 0 else 4 * (14 + 6 * L)
+    moved = stress - (12 if L > 0 and tau_uniform else 0)
+    return vel, stress, moved
 
 
 def workload(L, nt=6000, nrec=800, seed=11):
@@ -94,7 +96,7 @@ def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tiles=Non
     cells = p.nx * p.ny
     total = float(sum(walls))
     tau_uniform = L > 0 and all(np.all(m[k] == m[k].flat[0]) for k in ("taus", "taup"))
-    vel_b, str_b = bytes_per_cell(L, tau_uniform)
+    vel_b, str_b, str_moved = bytes_per_cell(L, tau_uniform)
     s_avg = float(np.mean(s_ms))
     v_avg = float(np.mean(v_ms))
     out = {
@@ -107,6 +109,8 @@ def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tiles=Non
         "s_kernel_GBs": str_b * cells / (s_avg * 1e-3) / 1e9 if s_avg > 0 else None,
         "step_GBs": (vel_b + str_b) * cells * p.nt * steps / total / 1e9,
         "stress_bytes_per_cell": str_b, "vel_bytes_per_cell": vel_b, "tau_uniform": bool(tau_uniform),
+        "stress_moved_bytes_per_cell": str_moved,
+        "s_kernel_moved_GBs": str_moved * cells / (s_avg * 1e-3) / 1e9 if s_avg > 0 else None,
         "footprint_GB": ctx.footprint() / 1e9,
         "tile": {"velocity": tile[0], "stress": tile[1]},
     }
@@ -418,6 +422,8 @@ def main():
                      "traffic": profiled_traffic("visco", "stress", res["tile"]["stress"]),
                      "kernel": "viscoelastic stress + memory-variable update (FD8, L=2, uniform tau)",
                      "bytes_per_cell": res["stress_bytes_per_cell"],
+                     "moved_bytes_per_cell": res["stress_moved_bytes_per_cell"],
+                     "moved_GBs": res["s_kernel_moved_GBs"],
                      "avg_launch_ms": res["s_kernel_ms"]},
         "cpu_baseline": cpu,
         "batched": batched,
