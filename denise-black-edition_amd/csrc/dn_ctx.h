@@ -74,7 +74,7 @@ struct dn_ctx {
     // tile shape of the sweeps and the per-tile cell work lists
     // (0 forward velocity, 1 forward stress, 2 adjoint velocity, 3 adjoint stress)
     Tile tile_v, tile_s;  // velocity / stress sweep shapes
-    int nt_v = 0, nt_s = 0;  // streaming (nontemporal) stores of each sweep
+    int nt_v = 0, nt_s = 0;  // streaming access to the once-per-step planes of each sweep
     int ntx_v = 0, ntiles_v = 0, ntx_s = 0, ntiles_s = 0;
     int *lst_off[4] = {};   // per list: CSR offsets (ntiles+1), then the lean and the full tile indices
     Ent *lst_ent[4] = {};

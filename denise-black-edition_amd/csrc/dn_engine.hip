@@ -124,8 +124,6 @@ int rebuild(dn_engine *e)
             b.fwdstore = c->p.mode > 0;
             b.vp1_form = c->p.grad_form == 1 || c->p.grad_form == 2;
             b.sw = j.phase;
-            b.nt_v = c->nt_v;
-            b.nt_s = c->nt_s;
             const int idx = (int)shots.size();
             shots.push_back(b);
             const int b0 = (int)j.base;

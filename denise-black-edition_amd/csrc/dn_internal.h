@@ -73,7 +73,6 @@ void prepare_material(const dn_params &p, const Layout &lay, const float *vp, co
 struct KArgs {
     int nx, ny, pitch, fw, fw2;
     int free_surf, boundary, grad_form, L, nt_total;
-    int ntstore;    // write the sweep outputs with streaming (nontemporal) stores
     float dt, dh;
     float dhi_el;   // DT/DH  (update_s_elastic, seismo_ssg)
     float dhi_vis;  // 1/DH   (update_s_visc, surfaces)
@@ -186,7 +185,6 @@ struct BatchShot {
     StepCtl cv, cs;
     DevAcq q;
     int nt_total, dtinv, fwdstore, vp1_form, sw;
-    int nt_v, nt_s;   // streaming stores of the two sweeps
 };
 
 // One block of a multi-shot launch: its shot, the tile it sweeps, and what
@@ -209,9 +207,11 @@ void launch_batch_str(const BatchShot *shots, const BatchBlock *blk, int nblocks
 void launch_batch_clock(int *clock, int add, void *stream);
 
 // kernel launchers (dn_kernels.hip); all enqueue on `stream`
-void launch_vel(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, int sw,
+// ntl: streaming (nontemporal) access to the planes a step touches once
+// (material / coefficient loads, memory-variable stores)
+void launch_vel(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, int sw, int ntl,
                 void *stream);
-void launch_str(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, void *stream);
+void launch_str(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, int ntl, void *stream);
 void launch_msource(const KArgs &a, const DevAcq &q, int nt, void *stream);
 void launch_store(const KArgs &a, float *const fwd[5], long k, void *stream);
 void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const float *wcr, float *gvp,

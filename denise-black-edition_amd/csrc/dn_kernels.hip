@@ -201,8 +201,8 @@ __global__ __launch_bounds__(256) void k_l2_sum(DevAcq q)
     X(128, 2, 2, 2, t128x2x2v2)
 
 #define X(B, Y, T, W, N)                                                                    \
-    void launch_vel_##N(const KArgs &, const StepCtl &, const DevAcq &, int, int, void *); \
-    void launch_str_##N(const KArgs &, const StepCtl &, const DevAcq &, int, void *);
+    void launch_vel_##N(const KArgs &, const StepCtl &, const DevAcq &, int, int, int, void *); \
+    void launch_str_##N(const KArgs &, const StepCtl &, const DevAcq &, int, int, void *);
 DN_TILES(X)
 #undef X
 
@@ -242,20 +242,21 @@ static StepCtl with_grid(const StepCtl &c0, const Tile &t, const KArgs &a)
     return c;
 }
 
-void launch_vel(const KArgs &a, const StepCtl &c0, const DevAcq &q, const Tile &t, int fdoh, int sw, void *stream)
+void launch_vel(const KArgs &a, const StepCtl &c0, const DevAcq &q, const Tile &t, int fdoh, int sw, int ntl,
+                void *stream)
 {
     const StepCtl c = with_grid(c0, t, a);
 #define X(B, Y, T, W, N) \
-    if (DN_SAME(B, Y, T, W)) return launch_vel_##N(a, c, q, fdoh, sw, stream);
+    if (DN_SAME(B, Y, T, W)) return launch_vel_##N(a, c, q, fdoh, sw, ntl, stream);
     DN_TILES(X)
 #undef X
 }
 
-void launch_str(const KArgs &a, const StepCtl &c0, const DevAcq &q, const Tile &t, int fdoh, void *stream)
+void launch_str(const KArgs &a, const StepCtl &c0, const DevAcq &q, const Tile &t, int fdoh, int ntl, void *stream)
 {
     const StepCtl c = with_grid(c0, t, a);
 #define X(B, Y, T, W, N) \
-    if (DN_SAME(B, Y, T, W)) return launch_str_##N(a, c, q, fdoh, stream);
+    if (DN_SAME(B, Y, T, W)) return launch_str_##N(a, c, q, fdoh, ntl, stream);
     DN_TILES(X)
 #undef X
 }

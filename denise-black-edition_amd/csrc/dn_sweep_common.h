@@ -50,9 +50,10 @@ struct VecT<4> {
 
 // stores of the sweep outputs: plain, or streaming (nontemporal: written
 // through without keeping the lines in the caches) when nt is set
-__device__ __forceinline__ void st1(float *p, float v, int nt)
+template <bool NT>
+__device__ __forceinline__ void st1(float *p, float v)
 {
-    if (nt)
+    if constexpr (NT)
         __builtin_nontemporal_store(v, p);
     else
         *p = v;
@@ -64,12 +65,35 @@ __device__ __forceinline__ typename VecT<VW>::type ldv(const float *p)
     return *reinterpret_cast<const typename VecT<VW>::type *>(p);
 }
 
-// store the first n (<= VW) elements
-template <int VW>
-__device__ __forceinline__ void stv(float *p, typename VecT<VW>::type v, int n, int nt = 0)
+// loads of planes read once per step (material and coefficient planes):
+// streaming (nontemporal) when NT is set, so that they do not displace the
+// wavefields the next sweep reads from the caches.  NT is a template
+// parameter: with a run-time flag the compiler merges the two loads of the
+// same address and drops the nontemporal marking.
+template <bool NT>
+__device__ __forceinline__ float ld1(const float *p)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+template <int VW, bool NT>
+__device__ __forceinline__ typename VecT<VW>::type ldvn(const float *p)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(reinterpret_cast<const typename VecT<VW>::type *>(p));
+    else
+        return ldv<VW>(p);
+}
+
+// store the first n (<= VW) elements (streaming when NT is set)
+template <int VW, bool NT = false>
+__device__ __forceinline__ void stv(float *p, typename VecT<VW>::type v, int n)
 {
     if (n >= VW) {
-        if (nt)
+        if constexpr (NT)
             __builtin_nontemporal_store(v, reinterpret_cast<typename VecT<VW>::type *>(p));
         else
             *reinterpret_cast<typename VecT<VW>::type *>(p) = v;

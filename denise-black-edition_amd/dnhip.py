@@ -359,7 +359,9 @@ class Context:
         return tuple(v), tuple(s)
 
     def set_store_hint(self, streaming, sweep=-1):
-        """0 plain, 1 streaming (nontemporal) stores of a sweep's outputs."""
+        """Cache policy of a sweep: 0 plain, 1 streaming (nontemporal)
+        access to the planes a step touches once (material and coefficient
+        loads, memory-variable stores)."""
         _check(load().dn_set_store_hint(self.h, sweep, streaming))
 
     def store_hint(self, sweep=0):

@@ -174,17 +174,20 @@ int dn_get_hc(dn_ctx *ctx, float *out7);
 int dn_set_tile(dn_ctx *ctx, int sweep, int tbx, int by, int ty, int vw);
 int dn_get_tile(dn_ctx *ctx, int sweep, int *tbx, int *by, int *ty, int *vw);
 
-/* Store policy of a sweep's outputs: 0 plain stores, 1 streaming
- * (nontemporal) stores that do not keep the written lines in the caches.
- * sweep 0 velocity, 1 stress, -1 both (set only).  Results do not depend on
- * it.  No reference counterpart. */
+/* Cache policy of a sweep: 0 plain; 1 streaming (nontemporal) loads of the
+ * planes a step reads once (material and coefficient planes) and streaming
+ * stores of the memory variables, so that they do not displace from the
+ * caches the wavefields the next sweep reads.  sweep 0 velocity, 1 stress,
+ * -1 both (set only).  Results do not depend on it.  No reference
+ * counterpart. */
 int dn_set_store_hint(dn_ctx *ctx, int sweep, int streaming);
 int dn_get_store_hint(dn_ctx *ctx, int sweep, int *streaming);
 
 /* Tile autotuner: times nsteps forward steps of this context's own
  * configuration with every compiled shape (HIP events around each launch)
- * and keeps the fastest shape of each sweep independently, then the faster
- * store policy of each (dn_set_store_hint); vel_shape / str_shape (4 ints:
+ * and keeps the fastest shape of each sweep independently, then the pair
+ * of cache policies (dn_set_store_hint) with the fastest step; vel_shape /
+ * str_shape (4 ints:
  * tbx, by, ty, vw; may be NULL) receive the shapes.  Leaves
  * the wavefields zeroed (a following dn_run_forward starts from rest).  No
  * reference counterpart. */
