@@ -66,7 +66,7 @@ def workload(L, nt=6000, nrec=800, seed=11):
     return p, dict(vp=vp, vs=vs, rho=rho, taus=taus, taup=taup), src, sig, rec
 
 
-def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tiles=None):
+def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tiles=None, policy=None):
     import dnhip
     p, m, src, sig, rec = workload(L, nt=nt)
     ctx = dnhip.Context(p)
@@ -78,7 +78,11 @@ def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tiles=Non
         ctx.set_tile(*tiles[1], sweep=1)
     elif tune_steps > 0:
         ctx.autotune(tune_steps)
+    if policy is not None:
+        ctx.set_store_hint(policy[0], sweep=0)
+        ctx.set_store_hint(policy[1], sweep=1)
     tile = [list(ctx.tile(0)), list(ctx.tile(1))]
+    cache_policy = [ctx.store_hint(0), ctx.store_hint(1)]
     for _ in range(warmup):
         ctx.forward()
     ctx.set_kernel_sampling(sample_every)
@@ -109,7 +113,7 @@ def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tiles=Non
         "s_kernel_GBs": str_b * cells / (s_avg * 1e-3) / 1e9 if s_avg > 0 else None,
         "step_GBs": (vel_b + str_b) * cells * p.nt * steps / total / 1e9,
         "stress_bytes_per_cell": str_b, "vel_bytes_per_cell": vel_b, "tau_uniform": bool(tau_uniform),
-        "stress_moved_bytes_per_cell": str_moved,
+        "stress_moved_bytes_per_cell": str_moved, "cache_policy": cache_policy,
         "s_kernel_moved_GBs": str_moved * cells / (s_avg * 1e-3) / 1e9 if s_avg > 0 else None,
         "footprint_GB": ctx.footprint() / 1e9,
         "tile": {"velocity": tile[0], "stress": tile[1]},
@@ -333,6 +337,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--no-visco", action="store_true", help="profiling: elastic shot only")
+    ap.add_argument("--cache-policy", type=str, default="",
+                    help="profiling: 'v,s' streaming policy of the two sweeps (0/1), with --tile-v/--tile-s")
     ap.add_argument("--forms-only", action="store_true",
                     help="only the batched and served forms (no single-shot lines, no CPU baseline)")
     ap.add_argument("--batch-shots", type=int, default=64, help="multi-shot workload size (0: skip)")
@@ -361,6 +367,7 @@ def main():
     if args.tile_v and args.tile_s:
         tiles = [tuple(int(v) for v in args.tile_v.split(",")), tuple(int(v) for v in args.tile_s.split(","))]
     L = 2
+    policy = tuple(int(v) for v in args.cache_policy.split(",")) if args.cache_policy else None
     if args.forms_only:
         out = {}
         if args.batch_shots > 0:
@@ -370,10 +377,10 @@ def main():
         print(json.dumps(out))
         return
     if args.no_visco:
-        elastic = run_gpu(0, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles)
+        elastic = run_gpu(0, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles, policy=policy)
         print(json.dumps({"elastic": elastic}))
         return
-    res = run_gpu(L, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles)
+    res = run_gpu(L, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles, policy=policy)
     print(f"# visco: {res}", file=sys.stderr, flush=True)
     elastic = None
     if not args.no_elastic:
