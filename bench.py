@@ -15,8 +15,8 @@ roofline = algorithmic HBM bytes of the dominant kernel (the viscoelastic
            context stream, against the 8 TB/s MI355X HBM peak.
 cpu_baseline = the FP32 CPU oracle (the reference algorithm restated in C,
            OpenMP over the host cores) on the first steps of the same shot.
-The elastic FD8 4000x1200 shot (the north-star roofline target) is reported
-alongside in "elastic".
+The elastic FD8 4000x1200 shot (whose step should reach 60% of the HBM
+roofline) is reported alongside in "elastic".
 """
 from __future__ import annotations
 

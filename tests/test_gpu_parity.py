@@ -1,7 +1,7 @@
 """HIP path vs the FP32 oracle (reference CPU path restated) on seeded inputs.
 
 Bar: seismograms rel-L2 <= 1e-5 and gradients rel-L2 <= 1e-4 against the
-FP32 oracle; the error of both against the FP64
+FP32 oracle (the path's accuracy contract); the error of both against the FP64
 oracle is printed.  The kernels evaluate the reference's operation order
 without contraction, so in practice the HIP results equal the FP32 oracle
 bit for bit; `exact` in the printed line reports that.
