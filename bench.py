@@ -16,7 +16,10 @@ roofline = algorithmic HBM bytes of the dominant kernel (the viscoelastic
 cpu_baseline = the FP32 CPU oracle (the reference algorithm restated in C,
            OpenMP over the host cores) on the first steps of the same shot.
 The elastic FD8 4000x1200 shot (whose step should reach 60% of the HBM
-roofline) is reported alongside in "elastic".
+roofline) is reported alongside in "elastic"; 64 uneven shots through the
+multi-shot engine in "batched"; a request stream through the server in
+"serving"; the elastic FWI gradient job of 2000x600, NT=4000 (forward with
+the snapshot store, adjoint with the correlation) in "gradient".
 """
 from __future__ import annotations
 
@@ -274,6 +277,114 @@ def run_serving(nreq, qps, graph_steps=32, saturate_qps=0.0):
     return stats
 
 
+def gradient_workload(nt=4000):
+    """The FWI gradient job of one shot: elastic P/SV 2000 x 600, DH = 10 m,
+    DT = 1 ms, NT = 4000, FD8, CPML 20 + free surface, forward wavefield
+    stored every 2nd step (5 fields x 2000 snapshots x 1.2 M cells = 48 GB
+    in HBM), Ricker 10 Hz at the centre 20 m deep, 999 receivers every 20 m
+    recording vx and vy.  Observed data: the same shot on the true model
+    (smooth background + 10% Gaussian anomaly of radius 300 m); the
+    gradient is taken at the background."""
+    import dnhip
+    import synth
+    nx, ny, dh, dt, fc = 2000, 600, 10.0, 1e-3, 10.0
+    bg = synth.layered_model(nx, ny, dh, seed=3)
+    true = synth.anomaly(*bg, dh, 10000.0, 3000.0, radius=300.0, amp=0.10)
+    common = dict(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=8, fw=20, free_surf=1, seismo=1, dtinv=2,
+                  ts=float(np.float32(1.0 / fc)))
+    src = synth.source_table(nx * dh / 2, 20.0, dh, fc=fc)
+    sig = synth.ricker(nt, dt, fc)[None, :]
+    rec = synth.receiver_line(20.0, 20.0, 999, 20.0, dh)
+    return common, bg, true, src, sig, rec
+
+
+def run_gradient(steps, nt=4000, tune_steps=40, sample_every=7, tiles=None):
+    """Time of the gradient job's two phases on resident inputs: the forward
+    run that stores the snapshots, and the adjoint run whose sweeps
+    correlate with them (residuals and assembly timed separately).  Cell
+    updates: NX*NY*NT per phase, 2*NX*NY*NT per job.  Algorithmic bytes per
+    cell-update: forward 80 + 20/DTINV (5 snapshot fields written every
+    DTINV steps), adjoint 80 + 16 + 76/DTINV (the time integral of the
+    adjoint velocity, and snapshot + adjoint + model reads and three
+    accumulator updates every DTINV steps).  Kernel averages are sampled
+    every `sample_every`-th step; an odd interval meets storing and
+    non-storing steps alike; they come from one more job after the timed
+    ones."""
+    import dnhip
+    common, bg, true, src, sig, rec = gradient_workload(nt)
+    fwd = dnhip.Context(dnhip.Params(mode=0, **common))
+    fwd.set_model(*true)
+    fwd.set_sources(src, sig)
+    fwd.set_receivers(rec)
+    fwd.forward()
+    obs = (fwd.seismogram(0), fwd.seismogram(1))
+    fwd.close()
+    g = dnhip.Context(dnhip.Params(mode=1, **common))
+    g.set_model(*bg)
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    if tiles:
+        g.set_tile(*tiles[0], sweep=0)
+        g.set_tile(*tiles[1], sweep=1)
+    elif tune_steps > 0:
+        g.autotune(tune_steps)
+    tile = [list(g.tile(0)), list(g.tile(1))]
+    t_f, t_r, t_a, t_g, dev_f, dev_a = [], [], [], [], [], []
+    for rep in range(steps + 2):
+        # job 0: warm-up; jobs 1..steps: timed; last job: kernel averages by
+        # HIP events (kept out of the timed jobs, whose time they would add to)
+        sampled = rep == steps + 1
+        g.set_kernel_sampling(sample_every if sampled else 0)
+        t0 = time.perf_counter()
+        g.forward()
+        t1 = time.perf_counter()
+        tf = g.timing()
+        l2 = g.residuals(obs[0], obs[1], None, 1)
+        t2 = time.perf_counter()
+        g.adjoint()
+        t3 = time.perf_counter()
+        ta = g.timing()
+        g.assemble()
+        grad = g.gradient(3)
+        t4 = time.perf_counter()
+        if sampled:
+            kf = (tf["v_kernel_ms"], tf["s_kernel_ms"])
+            ka = (ta["v_kernel_ms"], ta["s_kernel_ms"])
+        elif rep > 0:
+            t_f.append(t1 - t0)
+            t_r.append(t2 - t1)
+            t_a.append(t3 - t2)
+            t_g.append(t4 - t3)
+            dev_f.append(tf["run_ms"])
+            dev_a.append(ta["run_ms"])
+    assert np.isfinite(grad).all() and np.isfinite(l2)
+    if nt == 4000:      # a short profiling run need not reach the receivers
+        assert np.abs(grad).max() > 0 and l2 > 0
+    footprint = g.footprint()
+    g.close()
+    cells = common["nx"] * common["ny"]
+    cu = cells * nt
+    dtinv = common["dtinv"]
+    b_fwd, b_adj = 80 + 20 / dtinv, 80 + 16 + 76 / dtinv
+    f_s, a_s = float(np.mean(t_f)), float(np.mean(t_a))
+    job_s = float(np.mean(np.array(t_f) + np.array(t_r) + np.array(t_a) + np.array(t_g)))
+    return {
+        "workload": "elastic FWI gradient 2000x600, DH=10 m, FD8, NT=4000, snapshots every 2nd step "
+                    "(48 GB), 999 receivers (vx, vy)",
+        "jobs": steps, "tile": {"velocity": tile[0], "stress": tile[1]}, "footprint_GB": footprint / 1e9,
+        "job_s": job_s, "gcells_per_s": 2 * cu / job_s / 1e9,
+        "forward_s": f_s, "forward_gcells_per_s": cu / f_s / 1e9,
+        "forward_device_ms": float(np.mean(dev_f)),
+        "forward_bytes_per_cell": b_fwd, "forward_GBs": b_fwd * cu / f_s / 1e9,
+        "adjoint_s": a_s, "adjoint_gcells_per_s": cu / a_s / 1e9,
+        "adjoint_device_ms": float(np.mean(dev_a)),
+        "adjoint_bytes_per_cell": b_adj, "adjoint_GBs": b_adj * cu / a_s / 1e9,
+        "residuals_s": float(np.mean(t_r)), "assemble_and_copy_s": float(np.mean(t_g)),
+        "forward_kernel_ms": list(kf), "adjoint_kernel_ms": list(ka),
+        "misfit": l2,
+    }
+
+
 def cpu_baseline(L, nsteps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
@@ -349,6 +460,8 @@ def main():
                          "e.g. under a kernel-trace profiler)")
     ap.add_argument("--serve-requests", type=int, default=120, help="serving workload size (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=8.0)
+    ap.add_argument("--grad-jobs", type=int, default=2, help="timed gradient jobs (0: skip)")
+    ap.add_argument("--grad-only", action="store_true", help="profiling: only the gradient job")
     ap.add_argument("--nt", type=int, default=6000, help="profiling: shorter shots")
     ap.add_argument("--tune-steps", type=int, default=40)
     ap.add_argument("--tile-v", default="", help="profiling: fixed velocity-sweep shape tbx,by,ty,vw")
@@ -376,6 +489,10 @@ def main():
             out["serving"] = run_serving(args.serve_requests, args.serve_qps, args.graph_steps, args.serve_saturate_qps)
         print(json.dumps(out))
         return
+    if args.grad_only:
+        print(json.dumps({"gradient": run_gradient(max(1, args.grad_jobs), nt=min(args.nt, 4000),
+                                                   tune_steps=args.tune_steps, tiles=tiles)}))
+        return
     if args.no_visco:
         elastic = run_gpu(0, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles, policy=policy)
         print(json.dumps({"elastic": elastic}))
@@ -394,6 +511,10 @@ def main():
     if args.serve_requests > 0:
         serving = run_serving(args.serve_requests, args.serve_qps, args.graph_steps, args.serve_saturate_qps)
         print(f"# serving: {serving}", file=sys.stderr, flush=True)
+    gradient = None
+    if args.grad_jobs > 0:
+        gradient = run_gradient(args.grad_jobs, tune_steps=args.tune_steps)
+        print(f"# gradient: {gradient}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -435,6 +556,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "serving": serving,
+        "gradient": gradient,
         "detail": {"visco": res, "elastic": elastic,
                    "elastic_step_frac_of_hbm": (elastic["step_GBs"] / HBM_PEAK_GBS) if elastic else None,
                    "elastic_kernels_frac_of_hbm": (
