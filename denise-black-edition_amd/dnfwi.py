@@ -1,0 +1,159 @@
+"""ctypes binding of the host FWI functions of libdnio.so
+(include/denise_fwi.h): L-BFGS search direction, parabolic step-length
+search, bounded model update -- the FWI iteration of src/PSV/FWI_PSV.c
+around the device gradients -- and `iterate`, one FWI iteration over
+dnhip contexts.
+
+Arrays are float32 of shape (nx, ny) (model-file order, x-major)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+import dnio
+
+_MISFIT = C.CFUNCTYPE(C.c_double, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                      C.POINTER(C.c_float), C.c_int)
+
+
+class UpdatePar(C.Structure):
+    """dnfwi_update_par: limits and switches of calc_mat_change_test_PSV."""
+    _fields_ = [(n, C.c_float) for n in ("vpupperlim", "vplowerlim", "vsupperlim", "vslowerlim",
+                                         "rhoupperlim", "rholowerlim", "scalerho")] + \
+               [(n, C.c_int) for n in ("inv_vp_iter", "inv_vs_iter", "inv_rho_iter", "invmat1")]
+
+
+class LineSearch(C.Structure):
+    """dnfwi_linesearch (also the state carried between iterations)."""
+    _fields_ = [("eps_scale", C.c_float), ("epst", C.c_float * 3), ("L2t", C.c_double * 3),
+                ("step1", C.c_int), ("step3", C.c_int), ("nevals", C.c_int)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    lib = dnio.load()
+    fp, vp_ = C.POINTER(C.c_float), C.c_void_p
+    sig = {
+        "dnfwi_lbfgs_create": ([C.c_int, C.c_int, C.c_int, C.POINTER(vp_)], C.c_int),
+        "dnfwi_lbfgs_destroy": ([vp_], None),
+        "dnfwi_lbfgs_reset": ([vp_], None),
+        "dnfwi_lbfgs_pointer": ([vp_], C.c_int),
+        "dnfwi_lbfgs_direction": ([vp_, C.c_int, C.c_float, C.c_float, C.c_float, fp, fp, fp, fp, fp, fp],
+                                  C.c_int),
+        "dnfwi_model_mean": ([fp, C.c_int, C.c_int], C.c_float),
+        "dnfwi_model_update": ([C.POINTER(UpdatePar), C.c_int, C.c_float, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp,
+                                fp, fp, fp], C.c_float),
+        "dnfwi_opt_step": ([C.POINTER(C.c_double), fp], C.c_float),
+        "dnfwi_step_length": ([C.POINTER(UpdatePar), C.c_int, C.c_float, C.c_float, C.c_int, C.c_double, C.c_int,
+                               C.c_int, fp, fp, fp, fp, fp, fp, _MISFIT, vp_, C.POINTER(LineSearch)], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _f(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def update_par(vpupperlim=1e9, vplowerlim=0.0, vsupperlim=1e9, vslowerlim=0.0, rhoupperlim=1e9, rholowerlim=0.0,
+               scalerho=1.0, inv_vp_iter=0, inv_vs_iter=0, inv_rho_iter=0, invmat1=1):
+    return UpdatePar(vpupperlim, vplowerlim, vsupperlim, vslowerlim, rhoupperlim, rholowerlim, scalerho,
+                     inv_vp_iter, inv_vs_iter, inv_rho_iter, invmat1)
+
+
+class LBFGS:
+    """dnfwi_lbfgs: the L-BFGS memory of one FWI run."""
+
+    def __init__(self, nx, ny, nlbfgs):
+        self.nx, self.ny = nx, ny
+        h = C.c_void_p()
+        dnio._check(load().dnfwi_lbfgs_create(nx, ny, nlbfgs, C.byref(h)))
+        self.h = h
+
+    def direction(self, it, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho, vp, vs, rho):
+        """The search direction (new arrays); the gradients are not modified."""
+        d = [_f(g).copy() for g in (g_vp, g_vs, g_rho)]
+        m = [_f(a) for a in (vp, vs, rho)]
+        for a in d + m:
+            assert a.shape == (self.nx, self.ny)
+        dnio._check(load().dnfwi_lbfgs_direction(self.h, it, c_vp, c_vs, c_rho, *[_p(a) for a in d],
+                                                 *[_p(a) for a in m]))
+        return tuple(d)
+
+    def pointer(self):
+        return load().dnfwi_lbfgs_pointer(self.h)
+
+    def reset(self):
+        load().dnfwi_lbfgs_reset(self.h)
+
+    def close(self):
+        if self.h:
+            load().dnfwi_lbfgs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def model_mean(a):
+    a = _f(a)
+    return float(load().dnfwi_model_mean(_p(a), a.shape[0], a.shape[1]))
+
+
+def model_update(par, it, eps_scale, vp, vs, rho, d_vp, d_vs, d_rho):
+    """(vp_new, vs_new, rho_new, eps_true)."""
+    m = [_f(a) for a in (vp, vs, rho)]
+    d = [_f(a) for a in (d_vp, d_vs, d_rho)]
+    out = [np.empty_like(m[0]) for _ in range(3)]
+    nx, ny = m[0].shape
+    e = load().dnfwi_model_update(C.byref(par), it, eps_scale, nx, ny, *[_p(a) for a in m + d + out])
+    return out[0], out[1], out[2], float(e)
+
+
+def opt_step(L2t, epst):
+    L = (C.c_double * 3)(*[float(v) for v in L2t])
+    e = np.ascontiguousarray(epst, dtype=np.float32)
+    return float(load().dnfwi_opt_step(L, _p(e)))
+
+
+def step_length(par, it, eps_scale0, scalefac, stepmax, L2t0, vp, vs, rho, d_vp, d_vs, d_rho, misfit, state=None):
+    """step_length_est_psv with misfit(vp, vs, rho, itest) -> float;
+    `state` (a LineSearch) carries epst/L2t between iterations."""
+    m = [_f(a) for a in (vp, vs, rho)]
+    d = [_f(a) for a in (d_vp, d_vs, d_rho)]
+    nx, ny = m[0].shape
+    err = []
+
+    def cb(_user, pv, ps, pr, itest):
+        try:
+            arrs = [np.ctypeslib.as_array(p, shape=(nx, ny)).copy() for p in (pv, ps, pr)]
+            return float(misfit(*arrs, itest))
+        except Exception as e:  # an exception cannot cross the C frame
+            err.append(e)
+            return float("nan")
+
+    fn = _MISFIT(cb)
+    st = state if state is not None else LineSearch()
+    dnio._check(load().dnfwi_step_length(C.byref(par), it, eps_scale0, scalefac, stepmax, L2t0, nx, ny,
+                                         *[_p(a) for a in m + d], fn, None, C.byref(st)))
+    if err:
+        raise err[0]
+    return st

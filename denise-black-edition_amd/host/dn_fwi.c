@@ -1,0 +1,353 @@
+/*
+ * Host side of the FWI iteration (include/denise_fwi.h): L-BFGS search
+ * direction, parabolic step-length search, bounded model update.
+ *
+ * Each function evaluates the reference's expressions in the reference's
+ * order and precision (single-precision accumulators where the reference
+ * has them, double where a double literal promotes), so the results equal
+ * the reference's bit for bit; the build compiles this file with
+ * -ffp-contract=off (no fused multiply-add).
+ *
+ * Index conventions: the reference's NR vectors are 1-based
+ * (y_LBFGS[1..NLBFGS*NLBFGS_vec]); here every array is 0-based and slot k
+ * (1..NLBFGS) starts at (k-1)*nvec.
+ */
+#include "denise_fwi.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+int dnio_fail_msg(const char *msg); /* dn_io.c */
+
+struct dnfwi_lbfgs {
+    int nx, ny, nlbfgs, pointer;
+    size_t ncell, nvec;          /* nvec = 3 * ncell: [rho | vs | vp] */
+    float *y, *s;                /* nlbfgs * nvec */
+    float *q, *r;                /* nvec */
+    float *rho_l, *alpha_l;      /* nlbfgs */
+    float *gold, *mold;          /* previous normalised gradients / model, [rho | vs | vp] */
+    int have_old;
+};
+
+int dnfwi_lbfgs_create(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out)
+{
+    if (!out || nx < 1 || ny < 1 || nlbfgs < 1) return dnio_fail_msg("dnfwi_lbfgs_create: bad size");
+    dnfwi_lbfgs *lb = calloc(1, sizeof *lb);
+    if (!lb) return dnio_fail_msg("dnfwi_lbfgs_create: out of host memory");
+    lb->nx = nx;
+    lb->ny = ny;
+    lb->nlbfgs = nlbfgs;
+    lb->ncell = (size_t)nx * ny;
+    lb->nvec = 3 * lb->ncell;
+    /* FWI_PSV.c:289-298: NR vectors, zero-initialised */
+    lb->y = calloc((size_t)nlbfgs * lb->nvec, sizeof(float));
+    lb->s = calloc((size_t)nlbfgs * lb->nvec, sizeof(float));
+    lb->q = calloc(lb->nvec, sizeof(float));
+    lb->r = calloc(lb->nvec, sizeof(float));
+    lb->gold = calloc(lb->nvec, sizeof(float));
+    lb->mold = calloc(lb->nvec, sizeof(float));
+    lb->rho_l = calloc((size_t)nlbfgs, sizeof(float));
+    lb->alpha_l = calloc((size_t)nlbfgs, sizeof(float));
+    if (!lb->y || !lb->s || !lb->q || !lb->r || !lb->gold || !lb->mold || !lb->rho_l || !lb->alpha_l) {
+        dnfwi_lbfgs_destroy(lb);
+        return dnio_fail_msg("dnfwi_lbfgs_create: out of host memory");
+    }
+    lb->pointer = 1; /* FWI_PSV.c:290 */
+    *out = lb;
+    return 0;
+}
+
+void dnfwi_lbfgs_destroy(dnfwi_lbfgs *lb)
+{
+    if (!lb) return;
+    free(lb->y);
+    free(lb->s);
+    free(lb->q);
+    free(lb->r);
+    free(lb->gold);
+    free(lb->mold);
+    free(lb->rho_l);
+    free(lb->alpha_l);
+    free(lb);
+}
+
+void dnfwi_lbfgs_reset(dnfwi_lbfgs *lb)
+{
+    if (!lb) return;
+    /* zero_LBFGS.c:13-30; the old gradient/model files of the reference stay */
+    memset(lb->y, 0, sizeof(float) * (size_t)lb->nlbfgs * lb->nvec);
+    memset(lb->s, 0, sizeof(float) * (size_t)lb->nlbfgs * lb->nvec);
+    memset(lb->q, 0, sizeof(float) * lb->nvec);
+    memset(lb->r, 0, sizeof(float) * lb->nvec);
+    memset(lb->rho_l, 0, sizeof(float) * (size_t)lb->nlbfgs);
+    memset(lb->alpha_l, 0, sizeof(float) * (size_t)lb->nlbfgs);
+    lb->pointer = 1;
+}
+
+int dnfwi_lbfgs_pointer(const dnfwi_lbfgs *lb) { return lb ? lb->pointer : 0; }
+
+/* dotp.c:12-40: single-precision sequential sum of products over n values */
+static float dotp(const float *a, const float *b, size_t n)
+{
+    float tmp = 0.0f;
+    for (size_t i = 0; i < n; i++) tmp += a[i] * b[i];
+    return tmp;
+}
+
+int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, float c_rho, float *g_vp,
+                          float *g_vs, float *g_rho, const float *vp, const float *vs, const float *rho)
+{
+    if (!lb || !g_vp || !g_vs || !g_rho || !vp || !vs || !rho || iter < 1)
+        return dnio_fail_msg("dnfwi_lbfgs_direction: null argument or iter < 1");
+    if (iter > 1 && !lb->have_old)
+        return dnio_fail_msg("dnfwi_lbfgs_direction: iter > 1 needs the previous iteration's gradients");
+    const size_t nc = lb->ncell, nv = lb->nvec;
+    const int nl = lb->nlbfgs;
+    float *g[3] = {g_rho, g_vs, g_vp};           /* vector order of the L-BFGS vectors */
+    const float *m[3] = {rho, vs, vp};
+    const float cs[3] = {c_rho, c_vs, c_vp};
+
+    /* FWI_PSV.c:455-464 */
+    if (iter > 2) lb->pointer++;
+    if (lb->pointer > nl) lb->pointer = 1;
+
+    /* store_LBFGS_PSV.c:37-89: normalisation; gradp = the normalised gradient */
+    for (int c = 0; c < 3; c++)
+        for (size_t k = 0; k < nc; k++) g[c][k] = cs[c] * g[c][k];
+
+    if (iter > 1) {
+        /* store_LBFGS_PSV.c:91-205: y, s into the current slot; q = gradient */
+        float *ys = lb->y + (size_t)(lb->pointer - 1) * nv, *ss = lb->s + (size_t)(lb->pointer - 1) * nv;
+        for (int c = 0; c < 3; c++)
+            for (size_t k = 0; k < nc; k++) {
+                ys[c * nc + k] = g[c][k] - lb->gold[c * nc + k];
+                ss[c * nc + k] = m[c][k] - lb->mold[c * nc + k];
+            }
+        for (int c = 0; c < 3; c++) memcpy(lb->q + c * nc, g[c], sizeof(float) * nc);
+
+        /* LBFGS.c:26-95 */
+        const float *yp = lb->y + (size_t)(lb->pointer - 1) * nv, *sp = lb->s + (size_t)(lb->pointer - 1) * nv;
+        float sum_nom = dotp(yp, sp, nv);
+        float sum_denom = dotp(yp, yp, nv);
+        const float gamma = sum_nom / sum_denom;
+        for (int k = 1; k <= nl; k++) {
+            sum_nom = dotp(lb->y + (size_t)(k - 1) * nv, lb->s + (size_t)(k - 1) * nv, nv);
+            lb->rho_l[k - 1] = fabs(sum_nom) > 0.0 ? (float)(1.0 / sum_nom) : 0.0f;
+        }
+        for (int k = nl; k >= 1; k--) {
+            const float *sk = lb->s + (size_t)(k - 1) * nv, *yk = lb->y + (size_t)(k - 1) * nv;
+            sum_nom = dotp(sk, lb->q, nv);
+            lb->alpha_l[k - 1] = lb->rho_l[k - 1] * sum_nom;
+            const float a = lb->alpha_l[k - 1];
+            for (size_t i = 0; i < nv; i++) lb->q[i] = lb->q[i] - a * yk[i];
+        }
+        for (size_t i = 0; i < nv; i++) lb->r[i] = gamma * lb->q[i];
+        for (int k = 1; k <= nl; k++) {
+            const float *sk = lb->s + (size_t)(k - 1) * nv, *yk = lb->y + (size_t)(k - 1) * nv;
+            sum_nom = dotp(yk, lb->r, nv);
+            const float beta = lb->rho_l[k - 1] * sum_nom;
+            const float d = lb->alpha_l[k - 1] - beta;
+            for (size_t i = 0; i < nv; i++) lb->r[i] = lb->r[i] + sk[i] * d;
+        }
+    }
+
+    /* extract_LBFGS_PSV.c:80-249: keep the model and the normalised
+     * gradients (before they are replaced by the direction) */
+    for (int c = 0; c < 3; c++) {
+        memcpy(lb->mold + c * nc, m[c], sizeof(float) * nc);
+        memcpy(lb->gold + c * nc, g[c], sizeof(float) * nc);
+    }
+    lb->have_old = 1;
+
+    if (iter > 1) {
+        /* extract_LBFGS_PSV.c:31-78: direction = r, scaled back */
+        for (int c = 0; c < 3; c++) memcpy(g[c], lb->r + c * nc, sizeof(float) * nc);
+        for (size_t k = 0; k < nc; k++) {
+            g_vp[k] = g_vp[k] * c_vp;
+            g_vs[k] = g_vs[k] * c_vs;
+            g_rho[k] = g_rho[k] * c_rho;
+        }
+    }
+    return 0;
+}
+
+float dnfwi_model_mean(const float *a, int nx, int ny)
+{
+    /* FWI_PSV.c:549-569: Vp_avg += ppi[j][i] over i, j; Vp_avg /= NXG * NYG */
+    float sum = 0.0f;
+    const size_t n = (size_t)nx * ny;
+    for (size_t k = 0; k < n; k++) sum += a[k];
+    sum /= nx * ny;
+    return sum;
+}
+
+/* calc_mat_change_test_PSV.c:41-104: max of the model (from 0) and of the
+ * direction's magnitude (the first cell's, then any larger one) */
+static void maxima(const float *m, const float *d, size_t n, float *mmax, float *dmax)
+{
+    float a = 0.0f, b = 0.0f;
+    for (size_t k = 0; k < n; k++) {
+        if (m[k] > a) a = m[k];
+        if (k == 0 || b == 0.0f) b = fabsf(d[k]);
+        else if (fabsf(d[k]) > b) b = fabsf(d[k]);
+    }
+    *mmax = a;
+    *dmax = b;
+}
+
+float dnfwi_model_update(const dnfwi_update_par *u, int iter, float eps_scale, int nx, int ny, const float *vp,
+                         const float *vs, const float *rho, const float *d_vp, const float *d_vs, const float *d_rho,
+                         float *vp_new, float *vs_new, float *rho_new)
+{
+    const size_t n = (size_t)nx * ny;
+    float pimax, gradmax, umax, gradmax_u, rhomax, gradmax_rho;
+    maxima(vp, d_vp, n, &pimax, &gradmax);
+    maxima(vs, d_vs, n, &umax, &gradmax_u);
+    maxima(rho, d_rho, n, &rhomax, &gradmax_rho);
+    /* calc_mat_change_test_PSV.c:111-147 */
+    float eps = eps_scale * (pimax / gradmax);
+    if (iter < u->inv_vp_iter) eps = 0.0f;
+    float eps_u = eps_scale * (umax / gradmax_u);
+    if (iter < u->inv_vs_iter) eps_u = 0.0f;
+    float eps_rho = eps_scale * (rhomax / gradmax_rho) * u->scalerho;
+    if (iter < u->inv_rho_iter) eps_rho = 0.0f;
+    /* calc_mat_change_test_PSV.c:157-215 */
+    if (u->invmat1 == 1 || u->invmat1 == 3) {
+        for (size_t k = 0; k < n; k++) {
+            float p1 = vp[k] - eps * d_vp[k];
+            float u1 = vs[k] - eps_u * d_vs[k];
+            float r1 = rho[k] - eps_rho * d_rho[k];
+            if (u->invmat1 == 1) {
+                if (p1 < u->vplowerlim) p1 = vp[k];
+                if (p1 > u->vpupperlim) p1 = vp[k];
+                if ((u1 < u->vslowerlim) && (u1 > 1e-6)) u1 = vs[k];
+                if (u1 > u->vsupperlim) u1 = vs[k];
+                if (r1 < u->rholowerlim) r1 = rho[k];
+                if (r1 > u->rhoupperlim) r1 = rho[k];
+            }
+            if (p1 < 0.0) p1 = vp[k];
+            if (u1 < 0.0) u1 = vs[k];
+            if (r1 < 0.0) r1 = rho[k];
+            vp_new[k] = p1;
+            vs_new[k] = u1;
+            rho_new[k] = r1;
+        }
+    } else {
+        /* INVMAT1 = 2: the reference leaves the model unchanged */
+        memcpy(vp_new, vp, sizeof(float) * n);
+        memcpy(vs_new, vs, sizeof(float) * n);
+        memcpy(rho_new, rho, sizeof(float) * n);
+    }
+    return eps_u;
+}
+
+float dnfwi_opt_step(const double *L2t, const float *epst)
+{
+    /* calc_opt_step.c:31-46: A[i] = (1, eps, eps^2), b = L2 (stored as float) */
+    float A[3][3], b[3], x[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < 3; i++) {
+        A[i][2] = (epst[i] * epst[i]);
+        A[i][1] = (epst[i]);
+        A[i][0] = (1.0);
+        b[i] = (float)(L2t[i]);
+    }
+    /* solvelin.c:30-46 (method 1): forward elimination, no pivoting */
+    for (int k = 0; k < 2; k++)
+        for (int nn = k; nn < 2; nn++) {
+            const float a = A[nn + 1][k] / A[k][k];
+            for (int m = 0; m < 3; m++) A[nn + 1][m] = A[nn + 1][m] - a * A[k][m];
+            b[nn + 1] = b[nn + 1] - a * b[k];
+        }
+    for (int k = 2; k >= 0; k--) {
+        float c = b[k];
+        for (int m = 2; m >= k + 1; m--) c = c - A[k][m] * x[m];
+        x[k] = c / A[k][k];
+    }
+    /* calc_opt_step.c:49-72 */
+    float opteps = -x[1] / (2.0 * x[2]);
+    if (((2.0 * x[2]) < 0.0) && (L2t[1] > L2t[2])) opteps = epst[2];
+    if (opteps > epst[2]) opteps = epst[2];
+    if (opteps < 0.0) opteps = epst[0];
+    return opteps;
+}
+
+int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, float scalefac, int stepmax,
+                      double L2t0, int nx, int ny, const float *vp, const float *vs, const float *rho,
+                      const float *d_vp, const float *d_vs, const float *d_rho, dnfwi_misfit_fn misfit,
+                      void *user, dnfwi_linesearch *out)
+{
+    if (!u || !misfit || !out || !vp || !vs || !rho || !d_vp || !d_vs || !d_rho || nx < 1 || ny < 1)
+        return dnio_fail_msg("dnfwi_step_length: null argument");
+    const size_t n = (size_t)nx * ny;
+    float *tv = malloc(sizeof(float) * n), *ts = malloc(sizeof(float) * n), *tr = malloc(sizeof(float) * n);
+    if (!tv || !ts || !tr) {
+        free(tv);
+        free(ts);
+        free(tr);
+        return dnio_fail_msg("dnfwi_step_length: out of host memory");
+    }
+    /* step_length_est_psv.c:37-49 */
+    int step1 = 0, step2 = 0, step3 = 0, countstep = 0, itests = 2, iteste = 2, nevals = 0;
+    float eps_scale = eps_scale0;
+    /* epst1 and L2t are the caller's arrays in the reference (FWI_PSV.c):
+     * slots the search does not reach keep the previous iteration's values */
+    float epst1[4] = {0.0f, out->epst[0], out->epst[1], out->epst[2]};
+    double L2t[4] = {0.0, L2t0, out->L2t[1], out->L2t[2]};
+    while ((step2 != 1) || (step1 != 1)) {
+        for (int itest = itests; itest <= iteste; itest++) {
+            /* step_length_est_psv.c:55-63: test model at eps_scale, its misfit */
+            dnfwi_model_update(u, iter, eps_scale, nx, ny, vp, vs, rho, d_vp, d_vs, d_rho, tv, ts, tr);
+            L2t[itest] = misfit(user, tv, ts, tr, itest);
+            nevals++;
+            epst1[itest] = eps_scale;
+            epst1[1] = 0.0f;
+        }
+        /* step_length_est_psv.c:117-149 */
+        if ((step1 == 0) && (L2t[1] <= L2t[2])) {
+            eps_scale = eps_scale / scalefac;
+            countstep++;
+        }
+        if ((step1 == 1) && (L2t[2] < L2t[3])) {
+            epst1[3] = eps_scale;
+            step2 = 1;
+        }
+        if ((step1 == 1) && (L2t[2] >= L2t[3])) {
+            epst1[3] = eps_scale;
+            eps_scale = eps_scale + (eps_scale / scalefac);
+            countstep++;
+        }
+        if ((step1 == 0) && (L2t[1] > L2t[2])) {
+            epst1[2] = eps_scale;
+            step1 = 1;
+            iteste = 3;
+            itests = 3;
+            countstep = 0;
+            eps_scale = eps_scale + (eps_scale / scalefac);
+        }
+        /* step_length_est_psv.c:151-165 */
+        step3 = 0;
+        if ((step1 == 0) && (countstep > stepmax)) {
+            step3 = 1;
+            break;
+        }
+        if ((step1 == 1) && (countstep > stepmax)) {
+            step1 = 1;
+            step2 = 1;
+        }
+    }
+    /* step_length_est_psv.c:170-179 */
+    if (step1 == 1) eps_scale = dnfwi_opt_step(L2t + 1, epst1 + 1);
+    free(tv);
+    free(ts);
+    free(tr);
+    out->eps_scale = eps_scale;
+    for (int k = 0; k < 3; k++) {
+        out->epst[k] = epst1[k + 1];
+        out->L2t[k] = L2t[k + 1];
+    }
+    out->step1 = step1;
+    out->step3 = step3;
+    out->nevals = nevals;
+    return 0;
+}

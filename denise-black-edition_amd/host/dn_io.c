@@ -31,6 +31,8 @@ static int fail(const char *fmt, ...)
 }
 
 const char *dnio_last_error(void) { return g_err; }
+/* the error entry of the library's other host sources (dn_fwi.c) */
+int dnio_fail_msg(const char *msg) { return fail("%s", msg); }
 void dnio_free(void *p) { free(p); }
 
 /* include/fd.h:17 -- iround(x) = (int)floor(x + 0.5) */

@@ -1,0 +1,107 @@
+/*
+ * denise_fwi.h -- host side of DENISE's FWI iteration over the stacked
+ * gradients the device library returns: the limited-memory BFGS search
+ * direction, the parabolic step-length search and the bounded model update.
+ * Part of libdnio.so (plain C, no device code).
+ *
+ * Vectors are the caller's plain float arrays, nx*ny values each in the
+ * reference's model-file order (x-major, y fastest: a[(i-1)*ny + (j-1)]),
+ * which is the order of the reference's loops `for i, for j` over [j][i].
+ * Every float operation follows the reference's order and precision,
+ * including its single-precision sequential dot products, so a run gives
+ * the reference's directions, step lengths and models bit for bit.
+ *
+ * Return codes: 0 ok, -1 with the message in dnio_last_error() otherwise.
+ */
+#ifndef DENISE_FWI_H
+#define DENISE_FWI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* L-BFGS memory: NLBFGS (s, y) pairs over vectors of 3*nx*ny floats laid
+ * out [rho | vs | vp], plus the previous iteration's normalised gradients
+ * and model.  Replaces y_LBFGS, s_LBFGS, q_LBFGS, r_LBFGS, rho_LBFGS,
+ * alpha_LBFGS and the JACOBIAN_p*.old files of the reference
+ * (src/PSV/FWI_PSV.c:282-300 allocation, src/PSV/store_LBFGS_PSV.c:91-205,
+ * src/PSV/extract_LBFGS_PSV.c:80-249). */
+typedef struct dnfwi_lbfgs dnfwi_lbfgs;
+
+int dnfwi_lbfgs_create(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out);
+void dnfwi_lbfgs_destroy(dnfwi_lbfgs *lb);
+/* zero_LBFGS (src/zero_LBFGS.c) and LBFGS_pointer = 1 (src/PSV/FWI_PSV.c:807-811):
+ * the memory is cleared at the start of each workflow stage after the first. */
+void dnfwi_lbfgs_reset(dnfwi_lbfgs *lb);
+/* the slot (1..NLBFGS) the last dnfwi_lbfgs_direction stored into */
+int dnfwi_lbfgs_pointer(const dnfwi_lbfgs *lb);
+
+/* One iteration's search direction (GRAD_METHOD = 2):
+ *   the slot pointer step of src/PSV/FWI_PSV.c:452-466,
+ *   store_LBFGS_PSV (src/PSV/store_LBFGS_PSV.c:37-205: normalise the
+ *     gradients by C_vp, C_vs, C_rho; for iter > 1 store y = gradient
+ *     change, s = model change, q = gradient),
+ *   LBFGS (src/LBFGS.c:26-95: the two-loop recursion over the stored slots),
+ *   extract_LBFGS_PSV (src/PSV/extract_LBFGS_PSV.c:31-249: for iter > 1 the
+ *     direction H*q, scaled by C_vp, C_vs, C_rho; then keep the model and the
+ *     normalised gradients for the next iteration).
+ * g_vp, g_vs, g_rho: the stacked gradients on entry, the direction on return
+ * (at iter = 1 the normalised gradients).  vp, vs, rho: the current model. */
+int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, float c_rho, float *g_vp,
+                          float *g_vs, float *g_rho, const float *vp, const float *vs, const float *rho);
+
+/* Model averages C_vp, C_vs, C_rho of src/PSV/FWI_PSV.c:540-580: a
+ * single-precision sequential sum in file order divided by NX*NY. */
+float dnfwi_model_mean(const float *a, int nx, int ny);
+
+/* Limits and switches of the model update (src/PSV/calc_mat_change_test_PSV.c). */
+typedef struct dnfwi_update_par {
+    float vpupperlim, vplowerlim, vsupperlim, vslowerlim, rhoupperlim, rholowerlim;
+    float scalerho;                          /* SCALERHO of the workflow stage */
+    int inv_vp_iter, inv_vs_iter, inv_rho_iter;
+    int invmat1;                             /* 1: vp, vs, rho with limits; 3: no limits */
+} dnfwi_update_par;
+
+/* calc_mat_change_test_PSV (src/PSV/calc_mat_change_test_PSV.c:10-251):
+ * new = old - eps * direction with eps = eps_scale * max(model) /
+ * max|direction| per parameter (times SCALERHO for rho; 0 before the
+ * parameter's INV_*_ITER), cells leaving the limits or turning negative
+ * keep the old value.  Returns EPSILON_u, the reference's "eps_true". */
+float dnfwi_model_update(const dnfwi_update_par *u, int iter, float eps_scale, int nx, int ny, const float *vp,
+                         const float *vs, const float *rho, const float *d_vp, const float *d_vs, const float *d_rho,
+                         float *vp_new, float *vs_new, float *rho_new);
+
+/* calc_opt_step (src/calc_opt_step.c:14-84) with solvelin's Gauss
+ * elimination (src/solvelin.c:19-48): the vertex of the parabola through
+ * (epst[k], L2t[k]), k = 0..2, kept within [epst[0], epst[2]]. */
+float dnfwi_opt_step(const double *L2t, const float *epst);
+
+/* Misfit of a test model: the sum of the L2 misfits of the test shots
+ * (src/PSV/obj_psv.c:38-140).  itest is 2 or 3, the slot of L2t it fills. */
+typedef double (*dnfwi_misfit_fn)(void *user, const float *vp, const float *vs, const float *rho, int itest);
+
+typedef struct dnfwi_linesearch {
+    float eps_scale;   /* the step length to apply (calc_opt_step's, or the last trial) */
+    float epst[3];     /* epst1[1..3] */
+    double L2t[3];     /* L2t[1..3] */
+    int step1, step3;  /* step1 = 1: a step that lowers the misfit was found; step3 = 1: search failed */
+    int nevals;        /* test forward modellings run */
+} dnfwi_linesearch;
+
+/* step_length_est_psv (src/PSV/step_length_est_psv.c:37-182): from
+ * EPS_SCALE, divide the trial step by SCALEFAC until the misfit falls below
+ * L2t0 (the current model's), then grow it by 1 + 1/SCALEFAC until the
+ * misfit rises again (at most STEPMAX tries each), then fit the parabola.
+ * Test models are the model update of (vp, vs, rho) along d at the trial step.
+ * `out` also carries epst and L2t from one iteration to the next, as the
+ * reference's arrays do (a failed search leaves slot 3 as it was): zero it
+ * before the first iteration and pass the same record every iteration. */
+int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, float scalefac, int stepmax,
+                      double L2t0, int nx, int ny, const float *vp, const float *vs, const float *rho,
+                      const float *d_vp, const float *d_vs, const float *d_rho, dnfwi_misfit_fn misfit,
+                      void *user, dnfwi_linesearch *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

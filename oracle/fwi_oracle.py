@@ -1,0 +1,234 @@
+"""CPU restatement of DENISE's FWI iteration logic (L-BFGS direction,
+parabolic step length, bounded model update) in numpy.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the host FWI
+functions of libdnio (include/denise_fwi.h).  The product never uses it.
+
+Written from the reference's functions, one per function, keeping its
+1-based slot bookkeeping and its single- and double-precision steps:
+  - store_LBFGS / LBFGS / extract_LBFGS: src/PSV/store_LBFGS_PSV.c:37-205,
+    src/LBFGS.c:26-95, src/PSV/extract_LBFGS_PSV.c:31-249, with the slot
+    pointer of src/PSV/FWI_PSV.c:452-466 and dotp of src/dotp.c:12-40
+    (a float accumulator, i.e. a sequential single-precision sum);
+  - model_mean: src/PSV/FWI_PSV.c:540-580;
+  - mat_change_test: src/PSV/calc_mat_change_test_PSV.c:10-251;
+  - opt_step: src/calc_opt_step.c:14-84 with src/solvelin.c:19-48;
+  - step_length: src/PSV/step_length_est_psv.c:37-182.
+Vectors are numpy float32 arrays of shape (nx, ny) (the model-file order).
+The pins of these functions against what the mathematics fixes (BFGS
+matrix recursion, parabola vertex, limits) are in tests/test_fwi_host.py.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+F32 = np.float32
+
+
+def dotp(a, b):
+    """src/dotp.c: tmp += vec1[i]*vec2[i] with float tmp, in index order."""
+    prod = (a.ravel().astype(F32) * b.ravel().astype(F32)).astype(F32)
+    if prod.size == 0:
+        return F32(0.0)
+    return F32(np.add.accumulate(prod, dtype=F32)[-1])
+
+
+class LBFGS:
+    """The reference's L-BFGS state: y, s of NLBFGS slots of 3*NX*NY floats
+    ([rho | vs | vp]), q, r, rho, alpha, the cyclic pointer, and the
+    previous model / normalised gradients (the JACOBIAN_*.old files)."""
+
+    def __init__(self, nx, ny, nlbfgs):
+        self.n = 3 * nx * ny
+        self.nl = nlbfgs
+        self.y = np.zeros((nlbfgs + 1, self.n), F32)   # row 0 unused: slots 1..NLBFGS
+        self.s = np.zeros((nlbfgs + 1, self.n), F32)
+        self.rho = np.zeros(nlbfgs + 1, F32)
+        self.alpha = np.zeros(nlbfgs + 1, F32)
+        self.q = np.zeros(self.n, F32)
+        self.r = np.zeros(self.n, F32)
+        self.pointer = 1
+        self.gold = None
+        self.mold = None
+
+    def reset(self):
+        """zero_LBFGS + LBFGS_pointer = 1 (FWI_PSV.c:807-811)."""
+        for a in (self.y, self.s, self.rho, self.alpha, self.q, self.r):
+            a[...] = 0
+        self.pointer = 1
+
+    def direction(self, it, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho, vp, vs, rho):
+        c_vp, c_vs, c_rho = F32(c_vp), F32(c_vs), F32(c_rho)
+        # FWI_PSV.c:455-464
+        if it > 2:
+            self.pointer += 1
+        if self.pointer > self.nl:
+            self.pointer = 1
+        # store_LBFGS_PSV.c:37-89: waveconv = C * waveconv; gradp = waveconv
+        w_vp = (c_vp * g_vp.astype(F32)).astype(F32)
+        w_vs = (c_vs * g_vs.astype(F32)).astype(F32)
+        w_rho = (c_rho * g_rho.astype(F32)).astype(F32)
+        gradp = np.concatenate([w_rho.ravel(), w_vs.ravel(), w_vp.ravel()])
+        model = np.concatenate([rho.ravel(), vs.ravel(), vp.ravel()]).astype(F32)
+        if it > 1:
+            # store_LBFGS_PSV.c:91-205
+            p = self.pointer
+            self.y[p] = (gradp - self.gold).astype(F32)
+            self.s[p] = (model - self.mold).astype(F32)
+            self.q = gradp.copy()
+            # LBFGS.c:26-95
+            gamma = F32(dotp(self.y[p], self.s[p]) / dotp(self.y[p], self.y[p]))
+            for k in range(1, self.nl + 1):
+                sn = dotp(self.y[k], self.s[k])
+                self.rho[k] = F32(1.0 / float(sn)) if abs(float(sn)) > 0.0 else F32(0.0)
+            for k in range(self.nl, 0, -1):
+                self.alpha[k] = F32(self.rho[k] * dotp(self.s[k], self.q))
+                self.q = (self.q - (self.alpha[k] * self.y[k]).astype(F32)).astype(F32)
+            self.r = (gamma * self.q).astype(F32)
+            for k in range(1, self.nl + 1):
+                beta = F32(self.rho[k] * dotp(self.y[k], self.r))
+                self.r = (self.r + (self.s[k] * F32(self.alpha[k] - beta)).astype(F32)).astype(F32)
+        # extract_LBFGS_PSV.c:80-249: old model and old normalised gradient
+        self.mold = model.copy()
+        self.gold = gradp.copy()
+        if it > 1:
+            # extract_LBFGS_PSV.c:31-78
+            nc = g_vp.size
+            d_rho = self.r[:nc].reshape(g_rho.shape)
+            d_vs = self.r[nc:2 * nc].reshape(g_vs.shape)
+            d_vp = self.r[2 * nc:].reshape(g_vp.shape)
+            return ((d_vp * c_vp).astype(F32), (d_vs * c_vs).astype(F32), (d_rho * c_rho).astype(F32))
+        return w_vp, w_vs, w_rho
+
+
+def model_mean(a):
+    """Vp_avg += ppi[j][i] (float), then Vp_avg /= NXG*NYG."""
+    s = F32(np.add.accumulate(a.ravel().astype(F32), dtype=F32)[-1])
+    return F32(s / F32(a.size))
+
+
+def _maxima(m, d):
+    mmax = F32(max(0.0, float(m.max())))
+    dmax = F32(np.abs(d).max())
+    return mmax, dmax
+
+
+def mat_change_test(par, it, eps_scale, vp, vs, rho, d_vp, d_vs, d_rho):
+    """calc_mat_change_test_PSV for one test or final step.  par: dict with
+    vpupperlim, vplowerlim, vsupperlim, vslowerlim, rhoupperlim,
+    rholowerlim, scalerho, inv_vp_iter, inv_vs_iter, inv_rho_iter, invmat1.
+    Returns (vp_new, vs_new, rho_new, eps_true)."""
+    eps_scale = F32(eps_scale)
+    pimax, gmax = _maxima(vp, d_vp)
+    umax, gmax_u = _maxima(vs, d_vs)
+    rhomax, gmax_rho = _maxima(rho, d_rho)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        eps = F32(eps_scale * F32(pimax / gmax))
+        eps_u = F32(eps_scale * F32(umax / gmax_u))
+        eps_rho = F32(F32(eps_scale * F32(rhomax / gmax_rho)) * F32(par["scalerho"]))
+    if it < par["inv_vp_iter"]:
+        eps = F32(0.0)
+    if it < par["inv_vs_iter"]:
+        eps_u = F32(0.0)
+    if it < par["inv_rho_iter"]:
+        eps_rho = F32(0.0)
+    if par["invmat1"] not in (1, 3):
+        return vp.copy(), vs.copy(), rho.copy(), eps_u
+    p1 = (vp - (eps * d_vp).astype(F32)).astype(F32)
+    u1 = (vs - (eps_u * d_vs).astype(F32)).astype(F32)
+    r1 = (rho - (eps_rho * d_rho).astype(F32)).astype(F32)
+    if par["invmat1"] == 1:
+        p1 = np.where(p1 < F32(par["vplowerlim"]), vp, p1)
+        p1 = np.where(p1 > F32(par["vpupperlim"]), vp, p1)
+        u1 = np.where((u1 < F32(par["vslowerlim"])) & (u1.astype(np.float64) > 1e-6), vs, u1)
+        u1 = np.where(u1 > F32(par["vsupperlim"]), vs, u1)
+        r1 = np.where(r1 < F32(par["rholowerlim"]), rho, r1)
+        r1 = np.where(r1 > F32(par["rhoupperlim"]), rho, r1)
+    p1 = np.where(p1 < 0, vp, p1)
+    u1 = np.where(u1 < 0, vs, u1)
+    r1 = np.where(r1 < 0, rho, r1)
+    return p1.astype(F32), u1.astype(F32), r1.astype(F32), eps_u
+
+
+def opt_step(L2t, epst):
+    """calc_opt_step with solvelin (method 1); L2t, epst: 3 values each
+    (the reference's [1..3])."""
+    A = np.zeros((4, 4), F32)
+    b = np.zeros(4, F32)
+    x = np.zeros(4, F32)
+    for i in range(1, 4):
+        e = F32(epst[i - 1])
+        A[i, 3] = F32(e * e)
+        A[i, 2] = e
+        A[i, 1] = F32(1.0)
+        b[i] = F32(L2t[i - 1])
+    for k in range(1, 3):
+        for n in range(k, 3):
+            a = F32(A[n + 1, k] / A[k, k])
+            for m in range(1, 4):
+                A[n + 1, m] = F32(A[n + 1, m] - F32(a * A[k, m]))
+            b[n + 1] = F32(b[n + 1] - F32(a * b[k]))
+    for k in range(3, 0, -1):
+        c = b[k]
+        for m in range(3, k, -1):
+            c = F32(c - F32(A[k, m] * x[m]))
+        x[k] = F32(c / A[k, k])
+    opteps = F32(-float(x[2]) / (2.0 * float(x[3])))
+    if (2.0 * float(x[3])) < 0.0 and L2t[1] > L2t[2]:
+        opteps = F32(epst[2])
+    if opteps > epst[2]:
+        opteps = F32(epst[2])
+    if opteps < 0.0:
+        opteps = F32(epst[0])
+    return F32(opteps)
+
+
+def step_length(par, it, eps_scale0, scalefac, stepmax, L2t0, vp, vs, rho, d_vp, d_vs, d_rho, misfit,
+                carry=None):
+    """step_length_est_psv.  misfit(vp, vs, rho, itest) -> float.  carry:
+    (epst[3], L2t[3]) of the previous iteration (the reference's arrays
+    persist), zeros at the first.  Returns a dict like dnfwi_linesearch."""
+    epst1 = [0.0, 0.0, 0.0, 0.0]
+    L2t = [0.0, float(L2t0), 0.0, 0.0]
+    if carry is not None:
+        epst1[1:] = [F32(v) for v in carry[0]]
+        L2t[2:] = [float(v) for v in carry[1][1:]]
+    scalefac = F32(scalefac)
+    eps_scale = F32(eps_scale0)
+    step1 = step2 = step3 = countstep = 0
+    itests = iteste = 2
+    trials = []
+    while step2 != 1 or step1 != 1:
+        for itest in range(itests, iteste + 1):
+            tv, ts, tr, _ = mat_change_test(par, it, eps_scale, vp, vs, rho, d_vp, d_vs, d_rho)
+            L2t[itest] = float(misfit(tv, ts, tr, itest))
+            trials.append(float(eps_scale))
+            epst1[itest] = F32(eps_scale)
+            epst1[1] = F32(0.0)
+        if step1 == 0 and L2t[1] <= L2t[2]:
+            eps_scale = F32(eps_scale / scalefac)
+            countstep += 1
+        if step1 == 1 and L2t[2] < L2t[3]:
+            epst1[3] = F32(eps_scale)
+            step2 = 1
+        if step1 == 1 and L2t[2] >= L2t[3]:
+            epst1[3] = F32(eps_scale)
+            eps_scale = F32(eps_scale + F32(eps_scale / scalefac))
+            countstep += 1
+        if step1 == 0 and L2t[1] > L2t[2]:
+            epst1[2] = F32(eps_scale)
+            step1 = 1
+            iteste = itests = 3
+            countstep = 0
+            eps_scale = F32(eps_scale + F32(eps_scale / scalefac))
+        step3 = 0
+        if step1 == 0 and countstep > stepmax:
+            step3 = 1
+            break
+        if step1 == 1 and countstep > stepmax:
+            step1 = 1
+            step2 = 1
+    if step1 == 1:
+        eps_scale = opt_step(L2t[1:4], epst1[1:4])
+    return {"eps_scale": F32(eps_scale), "epst": [F32(v) for v in epst1[1:4]], "L2t": L2t[1:4],
+            "step1": step1, "step3": step3, "nevals": len(trials), "trials": trials}

@@ -19,7 +19,7 @@ INCLUDE = os.path.join(ROOT, "include")
 def _declared(header):
     text = open(os.path.join(INCLUDE, header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(dn_[a-z0-9_]+|dnio_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(dn_[a-z0-9_]+|dnio_[a-z0-9_]+|dnfwi_[a-z0-9_]+)\s*\(", text)))
 
 
 def _headers():
@@ -29,7 +29,8 @@ def _headers():
 def _lib_for(header):
     pkg = os.path.join(ROOT, "denise-black-edition_amd")
     return {"denise_hip.h": os.path.join(pkg, "libdnhip.so"),
-            "denise_io.h": os.path.join(pkg, "libdnio.so")}[header]
+            "denise_io.h": os.path.join(pkg, "libdnio.so"),
+            "denise_fwi.h": os.path.join(pkg, "libdnio.so")}[header]
 
 
 @pytest.mark.parametrize("header", _headers())

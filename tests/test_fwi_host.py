@@ -1,0 +1,264 @@
+"""Host FWI iteration functions of libdnio (include/denise_fwi.h) on the
+CPU: the L-BFGS direction, the model averages, the bounded model update,
+the parabola fit and the step-length search equal the numpy restatement of
+the reference (oracle/fwi_oracle.py) bit for bit, and both are pinned
+against what the mathematics fixes: the two-loop recursion equals the
+explicit BFGS inverse-Hessian recursion over the stored pairs (in the
+reference's slot order, so a wrapped cyclic buffer is covered); the
+parabola fit returns the vertex of an exact parabola and its clamps; the
+model update moves every free cell by exactly eps_scale * max(model) for a
+unit direction and keeps cells that would leave the limits; the step
+search on an exactly parabolic misfit returns its minimiser."""
+import numpy as np
+import pytest
+
+import dnfwi
+import fwi_oracle as fo
+
+F32 = np.float32
+NX, NY = 9, 7
+
+
+def rnd(rng, lo=-1.0, hi=1.0, shape=(NX, NY)):
+    return rng.uniform(lo, hi, shape).astype(F32)
+
+
+def models(rng):
+    vp = rnd(rng, 1500.0, 4000.0)
+    vs = (vp / F32(np.sqrt(3.0))).astype(F32)
+    rho = rnd(rng, 1800.0, 2600.0)
+    return vp, vs, rho
+
+
+def run_iterations(nl, iters, seed):
+    """The same gradient/model sequence through the product and the oracle."""
+    rng = np.random.default_rng(seed)
+    lb, ob = dnfwi.LBFGS(NX, NY, nl), fo.LBFGS(NX, NY, nl)
+    vp, vs, rho = models(rng)
+    c = (fo.model_mean(vp), fo.model_mean(vs), fo.model_mean(rho))
+    out = []
+    for it in range(1, iters + 1):
+        g = [rnd(rng, -1e-3, 1e-3) for _ in range(3)]
+        d_prod = lb.direction(it, *c, *g, vp, vs, rho)
+        d_orc = ob.direction(it, *c, *g, vp, vs, rho)
+        out.append((it, g, (vp, vs, rho), d_prod, d_orc, lb.pointer(), ob.pointer))
+        # next model: a small step along the direction
+        vp = (vp - F32(0.5) * d_prod[0] / np.abs(d_prod[0]).max()).astype(F32)
+        vs = (vs - F32(0.3) * d_prod[1] / np.abs(d_prod[1]).max()).astype(F32)
+        rho = (rho - F32(0.2) * d_prod[2] / np.abs(d_prod[2]).max()).astype(F32)
+    lb.close()
+    return c, out
+
+
+def test_lbfgs_direction_matches_oracle():
+    _, out = run_iterations(nl=3, iters=8, seed=5)
+    pointers = []
+    for it, g, m, dp, do, pp, po in out:
+        pointers.append(pp)
+        assert pp == po
+        for a, b in zip(dp, do):
+            assert np.array_equal(a, b), it
+    # FWI_PSV.c:455-464: the pointer stays at 1 for iterations 1 and 2, then cycles
+    assert pointers == [1, 1, 2, 3, 1, 2, 3, 1]
+
+
+def test_lbfgs_equals_bfgs_matrix_recursion():
+    """H_k = V_k^T H_{k-1} V_k + rho_k s_k s_k^T, V_k = I - rho_k y_k s_k^T,
+    H_0 = gamma I, over the slots 1..NLBFGS in index order; the direction is
+    C * H (C g).  FP64 dense matrices against the product's FP32 result."""
+    nl = 3
+    c, out = run_iterations(nl=nl, iters=7, seed=9)
+    cv = np.array(c, np.float64)
+    nc = NX * NY
+    n = 3 * nc
+    scale = np.concatenate([np.full(nc, cv[2]), np.full(nc, cv[1]), np.full(nc, cv[0])])   # rho | vs | vp
+    slots = {k: None for k in range(1, nl + 1)}
+    prev = None
+    for it, g, m, dp, do, ptr, _ in out:
+        gn = scale * np.concatenate([g[2].ravel(), g[1].ravel(), g[0].ravel()]).astype(np.float64)
+        mv = np.concatenate([m[2].ravel(), m[1].ravel(), m[0].ravel()]).astype(np.float64)
+        if it > 1:
+            slots[ptr] = (mv - prev[1], gn - prev[0])            # (s, y)
+            s_c, y_c = slots[ptr]
+            H = np.eye(n) * (y_c @ s_c) / (y_c @ y_c)
+            for k in range(1, nl + 1):
+                if slots[k] is None:
+                    continue
+                s, y = slots[k]
+                r = 1.0 / (y @ s)
+                V = np.eye(n) - r * np.outer(y, s)
+                H = V.T @ H @ V + r * np.outer(s, s)
+            d = scale * (H @ gn)
+            got = np.concatenate([dp[2].ravel(), dp[1].ravel(), dp[0].ravel()]).astype(np.float64)
+            rel = np.linalg.norm(got - d) / np.linalg.norm(d)
+            print(f"iteration {it}: rel-L2 vs FP64 BFGS matrix {rel:.2e}")
+            assert rel < 1e-5, (it, rel)
+        else:
+            got = np.concatenate([dp[2].ravel(), dp[1].ravel(), dp[0].ravel()]).astype(np.float64)
+            assert np.allclose(got, gn, rtol=1e-6)
+        prev = (gn, mv)
+
+
+def test_lbfgs_reset_and_misuse():
+    lb = dnfwi.LBFGS(NX, NY, 2)
+    rng = np.random.default_rng(1)
+    vp, vs, rho = models(rng)
+    g = [rnd(rng) for _ in range(3)]
+    with pytest.raises(dnfwi.dnio.IOError_):
+        lb.direction(2, 1.0, 1.0, 1.0, *g, vp, vs, rho)      # no previous iteration
+    d1 = lb.direction(1, 1.0, 1.0, 1.0, *g, vp, vs, rho)
+    assert np.array_equal(d1[0], g[0])                        # iter 1: the (C = 1) gradient
+    lb.direction(2, 1.0, 1.0, 1.0, *g, vp, vs, rho)
+    lb.direction(3, 1.0, 1.0, 1.0, *g, vp, vs, rho)
+    assert lb.pointer() == 2
+    lb.reset()
+    assert lb.pointer() == 1
+    lb.close()
+
+
+def test_model_mean():
+    rng = np.random.default_rng(3)
+    a = rnd(rng, 1000.0, 5000.0, (37, 23))
+    assert dnfwi.model_mean(a) == float(fo.model_mean(a))
+    assert dnfwi.model_mean(np.full((NX, NY), 1500.0, F32)) == 1500.0
+    assert abs(dnfwi.model_mean(a) / float(a.astype(np.float64).mean()) - 1.0) < 1e-5
+
+
+PAR = dict(vpupperlim=5000.0, vplowerlim=1000.0, vsupperlim=3000.0, vslowerlim=500.0, rhoupperlim=3000.0,
+           rholowerlim=1000.0, scalerho=0.5, inv_vp_iter=0, inv_vs_iter=0, inv_rho_iter=0, invmat1=1)
+
+
+def test_model_update_matches_oracle():
+    rng = np.random.default_rng(11)
+    for trial in range(6):
+        vp, vs, rho = models(rng)
+        vs[0, :3] = 0.0                                       # fluid cells
+        d = [rnd(rng) for _ in range(3)]
+        par = dict(PAR, inv_vs_iter=2 if trial == 3 else 0, invmat1=3 if trial == 4 else 1)
+        eps = [0.01, 0.5, 2.0, 0.05, 0.3, 0.0][trial]
+        got = dnfwi.model_update(dnfwi.update_par(**par), 1, eps, vp, vs, rho, *d)
+        ref = fo.mat_change_test(par, 1, eps, vp, vs, rho, *d)
+        for a, b in zip(got[:3], ref[:3]):
+            assert np.array_equal(a, b), trial
+        assert got[3] == float(ref[3])
+
+
+def test_model_update_pins():
+    rng = np.random.default_rng(12)
+    vp, vs, rho = models(rng)
+    one = np.ones((NX, NY), F32)
+    par = dnfwi.update_par(**PAR)
+    # unit direction: every cell moves by eps_scale * max(model) (no limit reached)
+    nv, ns, nr, eps_true = dnfwi.model_update(par, 1, 0.01, vp, vs, rho, one, one, one)
+    assert np.array_equal(nv, (vp - F32(F32(0.01) * vp.max())).astype(F32))
+    assert np.array_equal(ns, (vs - F32(F32(0.01) * vs.max())).astype(F32))
+    # rho's step is scaled by SCALERHO
+    assert np.array_equal(nr, (rho - F32(F32(F32(0.01) * rho.max()) * F32(0.5))).astype(F32))
+    assert eps_true == float(F32(F32(0.01) * vs.max()))
+    # limits: a step that leaves [lower, upper] keeps the old value
+    nv, ns, nr, _ = dnfwi.model_update(par, 1, 0.9, vp, vs, rho, one, -one, one)
+    assert np.all((nv == vp) | (nv >= 1000.0))
+    assert np.array_equal(nv, np.where((vp - F32(F32(0.9) * vp.max())) < 1000.0, vp, nv))
+    assert np.all(ns <= 3000.0) and np.all(nr >= 1000.0)
+    # a parameter before its INV_*_ITER does not move
+    par2 = dnfwi.update_par(**dict(PAR, inv_vp_iter=3))
+    nv, _, _, _ = dnfwi.model_update(par2, 2, 0.01, vp, vs, rho, one, one, one)
+    assert np.array_equal(nv, vp)
+    # fluid cells (vs = 0) stay at 0 below VSLOWERLIM: the lower limit only applies above 1e-6
+    vs0 = vs.copy()
+    vs0[:, 0] = 0.0
+    dz = one.copy()
+    dz[:, 0] = 0.0
+    _, ns, _, _ = dnfwi.model_update(par, 1, 0.01, vp, vs0, rho, one, dz, one)
+    assert np.all(ns[:, 0] == 0.0)
+
+
+def test_opt_step_pins_and_oracle():
+    e = np.array([0.0, 0.01, 0.02], F32)
+
+    def para(a, b, c):
+        return [a + b * float(x) + c * float(x) * float(x) for x in e]
+
+    # vertex inside the bracket
+    assert abs(dnfwi.opt_step(para(1.0, -20.0, 1000.0), e) - 0.01) < 1e-6
+    assert abs(dnfwi.opt_step(para(5.0, -30.0, 1000.0), e) - 0.015) < 1e-6
+    # vertex beyond the last trial -> the last trial
+    assert dnfwi.opt_step(para(1.0, -100.0, 1000.0), e) == float(e[2])
+    # concave with L2[2] > L2[3] -> the last trial
+    assert dnfwi.opt_step(para(1.0, 10.0, -1000.0), e) == float(e[2])
+    # vertex at negative step -> the first trial (0)
+    assert dnfwi.opt_step(para(1.0, 20.0, 1000.0), e) == 0.0
+    rng = np.random.default_rng(4)
+    for _ in range(50):
+        ee = np.sort(rng.uniform(0.0, 0.05, 3)).astype(F32)
+        ee[0] = 0.0
+        L = list(rng.uniform(0.5, 2.0, 3))
+        assert dnfwi.opt_step(L, ee) == float(fo.opt_step(L, ee))
+
+
+def _search_both(misfit, L2t0, eps0=0.01, scalefac=2.0, stepmax=4, carry=None, seed=21):
+    rng = np.random.default_rng(seed)
+    vp, vs, rho = models(rng)
+    d = [rnd(rng, 0.5, 1.0) for _ in range(3)]
+    par = dnfwi.update_par(**dict(PAR, vplowerlim=0.0, vslowerlim=0.0, rholowerlim=0.0))
+    calls_p, calls_o = [], []
+
+    def mp(a, b, c, itest):
+        v = misfit(a, b, c, vp, d[0])
+        calls_p.append((itest, v))
+        return v
+
+    def mo(a, b, c, itest):
+        v = misfit(a, b, c, vp, d[0])
+        calls_o.append((itest, v))
+        return v
+
+    st = dnfwi.LineSearch()
+    if carry is not None:
+        st.epst[:] = carry[0]
+        st.L2t[:] = carry[1]
+    got = dnfwi.step_length(par, 1, eps0, scalefac, stepmax, L2t0, vp, vs, rho, *d, mp, state=st)
+    ref = fo.step_length(dict(PAR, vplowerlim=0.0, vslowerlim=0.0, rholowerlim=0.0), 1, eps0, scalefac, stepmax,
+                         L2t0, vp, vs, rho, *d, mo, carry=carry)
+    assert calls_p == calls_o
+    assert got.eps_scale == float(ref["eps_scale"])
+    assert list(got.epst) == [float(v) for v in ref["epst"]]
+    assert list(got.L2t) == ref["L2t"]
+    assert (got.step1, got.step3, got.nevals) == (ref["step1"], ref["step3"], ref["nevals"])
+    return got, ref, vp, d
+
+
+def test_step_length_brackets_parabola_minimum():
+    """Misfit exactly quadratic in the step: min at eps_scale = 0.03."""
+    def misfit(a, b, c, vp, dvp):
+        eps = float(vp.max()) / float(np.abs(dvp).max())
+        t = float(np.mean((vp.astype(np.float64) - a) / dvp)) / eps      # the trial's eps_scale
+        return 1.0 + 1000.0 * (t - 0.03) ** 2
+
+    got, ref, _, _ = _search_both(misfit, L2t0=1.0 + 1000.0 * 0.03 ** 2)
+    assert got.step1 == 1 and got.step3 == 0
+    assert abs(got.eps_scale - 0.03) < 1e-4
+    assert got.epst[0] == 0.0 and got.epst[1] < 0.03 < got.epst[2]
+
+
+def test_step_length_failure_keeps_old_third_slot():
+    """Every trial raises the misfit: the step is halved STEPMAX+1 times, the
+    search fails (step3 = 1) and slot 3 keeps the previous iteration's values."""
+    def misfit(a, b, c, vp, dvp):
+        return 2.0 + float(np.abs(vp.astype(np.float64) - a).sum())
+
+    got, ref, _, _ = _search_both(misfit, L2t0=2.0, stepmax=3, carry=([0.0, 0.02, 0.04], [1.5, 1.2, 1.3]))
+    assert got.step1 == 0 and got.step3 == 1 and got.nevals == 4
+    assert abs(got.eps_scale - 0.01 / 2 ** 4) < 1e-9
+    assert got.epst[2] == float(F32(0.04)) and got.L2t[2] == 1.3
+
+
+def test_step_length_no_upper_bracket():
+    """The misfit keeps falling: after STEPMAX growths the search stops with
+    the last trial in slot 3 and fits the parabola anyway."""
+    def misfit(a, b, c, vp, dvp):
+        return 1.0 - float(np.abs(vp.astype(np.float64) - a).sum()) * 1e-6
+
+    got, ref, _, _ = _search_both(misfit, L2t0=1.0, stepmax=2)
+    assert got.step1 == 1 and got.step3 == 0
+    assert got.eps_scale == float(got.epst[2])
