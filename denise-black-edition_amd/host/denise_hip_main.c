@@ -1,26 +1,41 @@
 /*
  * denise_hip -- drop-in executable for DENISE's P/SV forward modelling and
- * FWI gradient on one MI355X:
+ * full-waveform inversion on one MI355X:
  *
  *     denise_hip <model.inp> [<workflow.inp>]      (src/denise.c:60-88)
  *
  * MODE = 0: per shot, the forward run of src/PSV/FD_PSV.c:271-354 and the SU
  *           files of saveseis_glob (SEIS_FILE_*.shot<N>).
- * MODE = 1: per shot, the gradient of src/PSV/grad_obj_psv.c:130-262
- *           (forward with snapshot store, L2 residual against
- *           DATA_DIR_{x,y,p}.su.shot<N>, adjoint, ass_gradPSV), stacked over
- *           shots; writes the stacked vp/vs/rho gradients as JACOBIAN_grad_vp,
- *           JACOBIAN_grad_vs, JACOBIAN_grad_rho (float32, x-major, NX x NY)
- *           and, with RTM_SHOT = 1, the per-shot images JACOBIAN_P_image_shot_N
- *           and JACOBIAN_S_image_shot_N (src/PSV/RTM_PSV_out_shot.c).  The
- *           misfit is the L2 sum over the test shots.
+ * MODE = 1: the FWI workflow of src/PSV/FWI_PSV.c:381-836 with L-BFGS
+ *           (GRAD_METHOD = 2): for each stage of the workflow file and each
+ *           iteration up to ITERMAX,
+ *           - the gradient of src/PSV/grad_obj_psv.c:130-262 per shot
+ *             (forward with snapshot store, L2 residual against
+ *             DATA_DIR_{x,y,p}.su.shot<N>, adjoint, ass_gradPSV), stacked
+ *             over shots, written as JACOBIAN_grad_vp, _vs, _rho (float32,
+ *             x-major, NX x NY) and, with RTM_SHOT = 1, the per-shot images
+ *             JACOBIAN_P_image_shot_N and JACOBIAN_S_image_shot_N
+ *             (src/PSV/RTM_PSV_out_shot.c); the misfit is the L2 sum over
+ *             the test shots;
+ *           - the L-BFGS direction, the parabolic step-length search over
+ *             the test shots (src/PSV/step_length_est_psv.c,
+ *             src/PSV/obj_psv.c) and the model update
+ *             (src/PSV/calc_mat_change_test_PSV.c), written as
+ *             INV_MODELFILE_vp.bin, _vs.bin, _rho.bin;
+ *           - a line of MISFIT_LOG_FILE (FWI_PSV.c:735-742), and the stage
+ *             end test on the relative misfit change PRO (FWI_PSV.c:781-827)
+ *             with the models of src/PSV/model_freq_out_PSV.c (INV_MOD_OUT = 0)
+ *             or src/PSV/model_it_out_PSV.c (INV_MOD_OUT = 1).
+ *           ITERMAX = 1 gives one gradient and one model update.
  * Exit status 1 with a message on any error, as the reference's err().
  */
 #define _POSIX_C_SOURCE 200809L
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include "denise_fwi.h"
 #include "denise_hip.h"
 #include "denise_io.h"
 
@@ -52,7 +67,7 @@ static void *xmalloc(size_t n)
 static void check_supported(const dnio_par *p, const dnio_stage *st)
 {
     if (p->physics != 1) die("PHYSICS", "only 2D P/SV (PHYSICS = 1) is built");
-    if (p->mode != 0 && p->mode != 1) die("MODE", "MODE 0 (forward) and 1 (FWI gradient) are built; RTM is not");
+    if (p->mode != 0 && p->mode != 1) die("MODE", "MODE 0 (forward) and 1 (FWI) are built; RTM is not");
     if (p->boundary != 0) die("BOUNDARY", "periodic boundaries are not supported");
     if (!p->srcrec) die("SRCREC", "sources must come from SOURCE_FILE (SRCREC = 1)");
     if (p->quellart == 6) die("QUELLART", "the spike source needs the Butterworth source filter, not built");
@@ -62,17 +77,299 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (st->lnorm != 2) die("LNORM", "only the L2 misfit (LNORM = 2) is built");
         if (st->time_filt) die("TIME_FILT", "time-domain filtering is not built");
         if (st->eprecond) die("EPRECOND", "energy preconditioning is not built");
+        if (st->spatfilter) die("SPATFILTER", "spatial gradient filters are not built");
         if (st->inv_stf || st->timewin || st->normalize || st->offset_mute || st->env || st->rowi)
             die("workflow", "STF inversion, time windows, normalisation, offset mutes, envelope and ROWI are not built");
         if (p->ndt != 1) die("NDT", "FWI needs NDT = 1");
+        if (p->idx != 1 || p->idy != 1) die("IDX/IDY", "FWI runs on every grid point (IDX = IDY = 1)");
         if (p->sws_taper_circular_per_shot) die("SWS_TAPER_CIRCULAR_PER_SHOT", "per-shot gradient tapers are not built");
+        if (p->sws_taper_grad_vert || p->sws_taper_grad_hor || p->sws_taper_grad_sources || p->sws_taper_file)
+            die("SWS_TAPER_GRAD_*", "gradient tapers are not built");
         if (p->grad_form != 1 && p->grad_form != 2) die("GRAD_FORM", "1 or 2 expected");
+        if (p->grad_method != 2) die("GRAD_METHOD", "the L-BFGS method (GRAD_METHOD = 2) is built; PCG is not");
+        if (p->model_filter || p->grad_filter) die("MODEL_FILTER/GRAD_FILTER", "model and gradient filters are not built");
+        if (p->gravity) die("GRAVITY", "gravity modelling is not built");
+        if (p->nlbfgs < 1) die("NLBFGS", "at least 1 expected");
+        if (p->invmat1 != 1) die("INVMAT1", "the vp/vs/rho parameterisation (INVMAT1 = 1) is built");
     }
 }
 
 static void write_plane(const char *name, const float *a, int nx, int ny)
 {
     io_ok(dnio_write_model(name, a, nx, ny), name);
+}
+
+/* everything a shot needs besides the model */
+typedef struct run {
+    dnio_par *P;
+    dn_ctx *ctx;
+    int nsrc, nshots, n_order, ntr;
+    float *srcpos, TS;
+    int *recpos;
+    size_t ncell;
+    char name[4 * DNIO_STR];
+} run;
+
+/* receivers, sources and wavelets of shot ishot on the context
+ * (FD_PSV.c:271-300, grad_obj_psv.c:130-170, obj_psv.c:54-110) */
+static void shot_setup(run *r, int ishot)
+{
+    dnio_par *P = r->P;
+    if (P->readrec == 2) {
+        dnio_free(r->recpos);
+        r->recpos = NULL;
+        io_ok(dnio_read_receivers(P, ishot, &r->ntr, &r->recpos), "REC_FILE");
+    }
+    const int first = P->run_multiple_shots ? ishot - 1 : 0, count = P->run_multiple_shots ? 1 : r->nsrc;
+    float *loc = xmalloc(sizeof(float) * 8 * (size_t)count);
+    int nloc = 0;
+    io_ok(dnio_split_sources(P, r->srcpos, r->nsrc, first, count, loc, &nloc), "sources");
+    /* pack the kept columns into an 8 x nloc table */
+    float *sp = xmalloc(sizeof(float) * 8 * (size_t)(nloc ? nloc : 1));
+    for (int q = 0; q < 8; q++)
+        for (int l = 0; l < nloc; l++) sp[q * nloc + l] = loc[q * count + l];
+    float *sig = xmalloc(sizeof(float) * (size_t)(nloc ? nloc : 1) * P->nt);
+    io_ok(dnio_wavelet(P, sp, nloc, ishot, r->TS, r->n_order, sig), "wavelet");
+    dn_ok(dn_set_sources(r->ctx, nloc, sp, sig), "sources");
+    int *rp = xmalloc(sizeof(int) * 2 * (size_t)(r->ntr ? r->ntr : 1));
+    for (int k = 0; k < r->ntr; k++) {
+        rp[k] = r->recpos[k];
+        rp[r->ntr + k] = r->recpos[r->ntr + k];
+    }
+    dn_ok(dn_set_receivers(r->ctx, r->ntr, rp), "receivers");
+    printf(" shot %d: %d source point(s), %d receivers\n", ishot, nloc, r->ntr);
+    fflush(stdout);
+    free(loc);
+    free(sp);
+    free(sig);
+    free(rp);
+}
+
+static int is_test_shot(const dnio_par *P, int ishot)
+{
+    return ishot >= P->testshot_start && ishot <= P->testshot_end && P->testshot_incr > 0 &&
+           (ishot - P->testshot_start) % P->testshot_incr == 0;
+}
+
+/* residuals of the shot on the context against its observed data;
+ * returns its L2 misfit (0 unless a test shot) */
+static double shot_residuals(run *r, int ishot, int test)
+{
+    dnio_par *P = r->P;
+    float *obs[3] = {NULL, NULL, NULL};
+    const int qb = P->quelltypb;
+    const int use[3] = {qb == 1 || qb == 3 || qb == 5 || qb == 7, qb == 1 || qb == 2 || qb == 6 || qb == 7, qb >= 4};
+    for (int c = 0; c < 3; c++)
+        if (use[c]) {
+            obs[c] = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
+            io_ok(dnio_data_name(P, c, ishot, r->name, sizeof r->name), "file name");
+            io_ok(dnio_read_su(r->name, r->ntr, P->nt, obs[c]), r->name);
+        }
+    double l2 = 0.0;
+    dn_ok(dn_calc_residuals(r->ctx, obs[0], obs[1], obs[2], test, &l2), "residuals");
+    for (int c = 0; c < 3; c++) free(obs[c]);
+    return l2;
+}
+
+/* MODE 0: forward runs and SU output */
+static void forward_pass(run *r)
+{
+    dnio_par *P = r->P;
+    for (int ishot = 1; ishot <= r->nshots; ishot++) {
+        shot_setup(r, ishot);
+        dn_ok(dn_run_forward(r->ctx, 0), "forward run");
+        if (r->ntr > 0 && P->seismo) {
+            const float XS = r->srcpos[0 * r->nsrc + ishot - 1], YS = r->srcpos[1 * r->nsrc + ishot - 1];
+            /* saveseis_glob: SEISMO 1 vx,vy; 2 p; 3 div,curl; 4 vx,vy,p */
+            int comps[5], nc = 0;
+            if (P->seismo == 1 || P->seismo == 4) comps[nc++] = 0, comps[nc++] = 1;
+            if (P->seismo == 2 || P->seismo == 4) comps[nc++] = 2;
+            if (P->seismo == 3) comps[nc++] = 3, comps[nc++] = 4;
+            float *sec = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
+            for (int c = 0; c < nc; c++) {
+                dn_ok(dn_get_seismogram(r->ctx, comps[c], sec), "seismograms");
+                io_ok(dnio_seis_name(P, comps[c], ishot, 1, r->name, sizeof r->name), "file name");
+                io_ok(dnio_write_seis(r->name, P, sec, r->ntr, P->nt, r->recpos, XS, YS), r->name);
+                printf("   wrote %s\n", r->name);
+            }
+            free(sec);
+        }
+    }
+}
+
+/* grad_obj_psv: stacked vp/vs/rho gradients over all shots; returns the
+ * L2 misfit summed over the test shots */
+static double gradient_pass(run *r, float **g_sum)
+{
+    dnio_par *P = r->P;
+    float *g_shot = xmalloc(sizeof(float) * r->ncell);
+    for (int k = 0; k < 3; k++) memset(g_sum[k], 0, sizeof(float) * r->ncell);
+    double misfit = 0.0;
+    for (int ishot = 1; ishot <= r->nshots; ishot++) {
+        shot_setup(r, ishot);
+        dn_ok(dn_run_forward(r->ctx, 0), "forward run");
+        const int test = is_test_shot(P, ishot);
+        const double l2 = shot_residuals(r, ishot, test);
+        misfit += l2;
+        dn_ok(dn_run_adjoint(r->ctx, 0), "adjoint run");
+        dn_ok(dn_assemble_gradient(r->ctx), "gradient");
+        for (int k = 0; k < 3; k++) {
+            dn_ok(dn_get_gradient(r->ctx, 3 + k, g_shot), "gradient");
+            for (size_t q = 0; q < r->ncell; q++) g_sum[k][q] += g_shot[q];
+            if (P->rtm_shot && k < 2) {
+                snprintf(r->name, sizeof r->name, "%s_%s_image_shot_%i", P->jacobian, k == 0 ? "P" : "S", ishot);
+                write_plane(r->name, g_shot, P->nx, P->ny);
+            }
+        }
+        printf("   L2 of shot %d: %.9e%s\n", ishot, l2, test ? "" : " (not a test shot)");
+    }
+    free(g_shot);
+    return misfit;
+}
+
+/* obj_psv: forward runs of the test shots on the current model, L2 sum */
+typedef struct obj_env {
+    run *r;
+    const float *taus, *taup;
+} obj_env;
+
+static double misfit_of(void *user, const float *vp, const float *vs, const float *rho, int itest)
+{
+    obj_env *o = user;
+    run *r = o->r;
+    dn_ok(dn_set_model(r->ctx, vp, vs, rho, o->taus, o->taup), "test model");
+    double l2 = 0.0;
+    for (int ishot = 1; ishot <= r->nshots; ishot++) {
+        if (!is_test_shot(r->P, ishot)) continue;
+        printf(" test forward modelling %d for shot %d\n", itest, ishot);
+        shot_setup(r, ishot);
+        dn_ok(dn_run_forward(r->ctx, 0), "forward run");
+        l2 += shot_residuals(r, ishot, 1);
+    }
+    return l2;
+}
+
+static void write_models(const char *fmt_vp, const char *fmt_vs, const char *fmt_rho, const float *vp, const float *vs,
+                         const float *rho, int nx, int ny)
+{
+    write_plane(fmt_vp, vp, nx, ny);
+    write_plane(fmt_vs, vs, nx, ny);
+    write_plane(fmt_rho, rho, nx, ny);
+}
+
+/* MODE 1: src/PSV/FWI_PSV.c:381-836 with GRAD_METHOD = 2 */
+static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, const float *taus, const float *taup)
+{
+    dnio_par *P = r->P;
+    const int nx = P->nx, ny = P->ny;
+    int stagemax = 0;
+    io_ok(dnio_count_stages(workflow, &stagemax), "workflow file");
+    float *g[3];
+    for (int k = 0; k < 3; k++) g[k] = xmalloc(sizeof(float) * r->ncell);
+    float *nv = xmalloc(sizeof(float) * r->ncell), *ns = xmalloc(sizeof(float) * r->ncell);
+    float *nr = xmalloc(sizeof(float) * r->ncell);
+    dnfwi_lbfgs *lb = NULL;
+    io_ok(dnfwi_lbfgs_create(nx, ny, P->nlbfgs, &lb), "L-BFGS memory");
+    double *L2_hist = xmalloc(sizeof(double) * (size_t)(P->itermax + 2));
+    dnfwi_linesearch ls;
+    memset(&ls, 0, sizeof ls);
+    float c_vp = 0.0f, c_vs = 0.0f, c_rho = 0.0f;
+    int iter_true = 1, min_iter_help = P->min_iter;
+    char name[3][4 * DNIO_STR];
+    for (int nstage = 1; nstage <= stagemax; nstage++) {
+        dnio_stage st;
+        io_ok(dnio_read_stage(workflow, nstage, &st), "workflow file");
+        check_supported(P, &st);
+        dnfwi_update_par up = {P->vpupperlim, P->vplowerlim, P->vsupperlim, P->vslowerlim, P->rhoupperlim,
+                               P->rholowerlim, st.scalerho, st.inv_vp_iter, st.inv_vs_iter, st.inv_rho_iter,
+                               P->invmat1};
+        r->n_order = st.n_order;
+        for (int iter = 1; iter <= P->itermax; iter++) {
+            printf("\n TDFWI ITERATION %d of %d (stage %d of %d)\n", iter, P->itermax, nstage, stagemax);
+            fflush(stdout);
+            if (iter_true == 1) {
+                /* FWI_PSV.c:540-580: the covariances are the initial model's averages */
+                c_vp = dnfwi_model_mean(vp, nx, ny);
+                c_vs = dnfwi_model_mean(vs, nx, ny);
+                c_rho = dnfwi_model_mean(rho, nx, ny);
+                printf(" Vp_avg = %.0f \t Vs_avg = %.0f \t rho_avg = %.0f\n", c_vp, c_vs, c_rho);
+            }
+            dn_ok(dn_set_model(r->ctx, vp, vs, rho, taus, taup), "model upload");
+            const double L2sum = gradient_pass(r, g);
+            const char *suffix[3] = {"vp", "vs", "rho"};
+            for (int k = 0; k < 3; k++) {
+                snprintf(r->name, sizeof r->name, "%s_grad_%s", P->jacobian, suffix[k]);
+                write_plane(r->name, g[k], nx, ny);
+            }
+            printf(" misfit (L2, test shots): %.9e\n", L2sum);
+            io_ok(dnfwi_lbfgs_direction(lb, iter, c_vp, c_vs, c_rho, g[0], g[1], g[2], vp, vs, rho), "L-BFGS");
+            if (iter == 1) min_iter_help = P->min_iter; /* FWI_PSV.c:687-690 */
+            obj_env oe = {r, taus, taup};
+            io_ok(dnfwi_step_length(&up, iter, P->eps_scale, P->scalefac, P->stepmax, L2sum, nx, ny, vp, vs, rho, g[0],
+                                    g[1], g[2], misfit_of, &oe, &ls),
+                  "step length");
+            /* FWI_PSV.c:700-708 */
+            float eps_scale = ls.eps_scale, opteps_vp;
+            if ((iter > min_iter_help) && (ls.step1 == 0)) {
+                eps_scale = 0.0f;
+                opteps_vp = 0.0f;
+            } else {
+                opteps_vp = eps_scale;
+            }
+            /* FWI_PSV.c:727-743: the misfit log */
+            FILE *fl = fopen(P->misfit_log_file, iter_true == 1 ? "w" : "a");
+            if (!fl) die("MISFIT_LOG_FILE", P->misfit_log_file);
+            fprintf(fl, "%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %d \n", opteps_vp, ls.epst[0], ls.epst[1],
+                    ls.epst[2], ls.L2t[0], ls.L2t[1], ls.L2t[2], L2sum, nstage);
+            fclose(fl);
+            printf(" step length %e (trials %e %e %e, misfits %e %e %e)\n", opteps_vp, ls.epst[0], ls.epst[1],
+                   ls.epst[2], ls.L2t[0], ls.L2t[1], ls.L2t[2]);
+            L2_hist[iter] = L2sum;
+            /* FWI_PSV.c:749: the update, written as INV_MODELFILE_{vp,vs,rho}.bin */
+            dnfwi_model_update(&up, iter, eps_scale, nx, ny, vp, vs, rho, g[0], g[1], g[2], nv, ns, nr);
+            memcpy(vp, nv, sizeof(float) * r->ncell);
+            memcpy(vs, ns, sizeof(float) * r->ncell);
+            memcpy(rho, nr, sizeof(float) * r->ncell);
+            snprintf(name[0], sizeof name[0], "%s_vp.bin", P->inv_modelfile);
+            snprintf(name[1], sizeof name[1], "%s_vs.bin", P->inv_modelfile);
+            snprintf(name[2], sizeof name[2], "%s_rho.bin", P->inv_modelfile);
+            write_models(name[0], name[1], name[2], vp, vs, rho, nx, ny);
+            /* FWI_PSV.c:781-827: stage end on the misfit change over two
+             * iterations (iterations before the first count as misfit 0,
+             * which never ends a stage) or a failed step search */
+            int stage_end = 0;
+            if (iter > min_iter_help) {
+                const double old = iter > 2 ? L2_hist[iter - 2] : 0.0;
+                const float diff = fabs((old - L2_hist[iter]) / old);
+                if ((diff <= st.pro) || (ls.step3 == 1)) stage_end = 1;
+            }
+            if (stage_end) {
+                if (P->inv_mod_out == 0) {
+                    snprintf(name[0], sizeof name[0], "%s_vp_stage_%d.bin", P->inv_modelfile, nstage);
+                    snprintf(name[1], sizeof name[1], "%s_vs_stage_%d.bin", P->inv_modelfile, nstage);
+                    snprintf(name[2], sizeof name[2], "%s_rho_stage_%d.bin", P->inv_modelfile, nstage);
+                    write_models(name[0], name[1], name[2], vp, vs, rho, nx, ny);
+                }
+                min_iter_help = iter + P->min_iter;
+                dnfwi_lbfgs_reset(lb);
+                printf(" %s: next FWI stage\n", ls.step3 ? "step-length search failed" : "abort criterion reached");
+                break; /* FWI_PSV.c:826: before iter_true++ */
+            }
+            if (P->inv_mod_out == 1) {
+                snprintf(name[0], sizeof name[0], "%s_vp_stage_%d_it_%d.bin", P->inv_modelfile, nstage, iter);
+                snprintf(name[1], sizeof name[1], "%s_vs_stage_%d_it_%d.bin", P->inv_modelfile, nstage, iter);
+                snprintf(name[2], sizeof name[2], "%s_rho_stage_%d_it_%d.bin", P->inv_modelfile, nstage, iter);
+                write_models(name[0], name[1], name[2], vp, vs, rho, nx, ny);
+            }
+            iter_true++;
+        }
+    }
+    dnfwi_lbfgs_destroy(lb);
+    for (int k = 0; k < 3; k++) free(g[k]);
+    free(nv);
+    free(ns);
+    free(nr);
+    free(L2_hist);
 }
 
 int main(int argc, char **argv)
@@ -93,17 +390,19 @@ int main(int argc, char **argv)
     if (P.nprocx * P.nprocy > 1)
         printf(" NPROCX x NPROCY = %d x %d: the whole grid runs as one domain on one GPU\n", P.nprocx, P.nprocy);
 
-    int nsrc = 0;
-    float *srcpos = NULL, TS = P.ts;
-    io_ok(dnio_read_sources(P.source_file, &nsrc, &srcpos, &TS), "SOURCE_FILE");
-    const int nshots = P.run_multiple_shots ? nsrc : 1;
-    printf(" %d source(s), %d shot(s); NX=%d NY=%d NT=%d DH=%g DT=%g FDORDER=%d L=%d\n", nsrc, nshots, P.nx, P.ny,
+    run r;
+    memset(&r, 0, sizeof r);
+    r.P = &P;
+    r.TS = P.ts;
+    io_ok(dnio_read_sources(P.source_file, &r.nsrc, &r.srcpos, &r.TS), "SOURCE_FILE");
+    r.nshots = P.run_multiple_shots ? r.nsrc : 1;
+    printf(" %d source(s), %d shot(s); NX=%d NY=%d NT=%d DH=%g DT=%g FDORDER=%d L=%d\n", r.nsrc, r.nshots, P.nx, P.ny,
            P.nt, P.dh, P.dt, P.fdorder, P.L);
 
-    const size_t ncell = (size_t)P.nx * P.ny;
-    float *vp = xmalloc(sizeof(float) * ncell), *vs = xmalloc(sizeof(float) * ncell);
-    float *rho = xmalloc(sizeof(float) * ncell);
-    float *taus = P.L ? xmalloc(sizeof(float) * ncell) : NULL, *taup = P.L ? xmalloc(sizeof(float) * ncell) : NULL;
+    r.ncell = (size_t)P.nx * P.ny;
+    float *vp = xmalloc(sizeof(float) * r.ncell), *vs = xmalloc(sizeof(float) * r.ncell);
+    float *rho = xmalloc(sizeof(float) * r.ncell);
+    float *taus = P.L ? xmalloc(sizeof(float) * r.ncell) : NULL, *taup = P.L ? xmalloc(sizeof(float) * r.ncell) : NULL;
     io_ok(dnio_read_model(&P, vp, vs, rho, taus, taup), "model");
 
     dn_params dp;
@@ -130,114 +429,22 @@ int main(int argc, char **argv)
     dp.fpml = P.fpml;
     dp.npower = P.npower;
     dp.k_max_pml = P.k_max_pml;
-    dp.ts = TS;
+    dp.ts = r.TS;
     for (int l = 0; l < P.L && l < 8; l++) dp.fl[l] = P.fl[l];
 
-    dn_ctx *ctx = NULL;
-    dn_ok(dn_ctx_create(&dp, 0, &ctx), "device context");
-    dn_ok(dn_set_model(ctx, vp, vs, rho, taus, taup), "model upload");
+    dn_ok(dn_ctx_create(&dp, 0, &r.ctx), "device context");
+    dn_ok(dn_set_model(r.ctx, vp, vs, rho, taus, taup), "model upload");
+    if (P.seismo || P.mode == 1) io_ok(dnio_read_receivers(&P, 1, &r.ntr, &r.recpos), "REC_FILE");
+    r.n_order = st ? st->n_order : 0;
 
-    int ntr = 0, *recpos = NULL;
-    if (P.seismo || P.mode == 1) io_ok(dnio_read_receivers(&P, 1, &ntr, &recpos), "REC_FILE");
+    if (P.mode == 0)
+        forward_pass(&r);
+    else
+        fwi(&r, argv[2], vp, vs, rho, taus, taup);
 
-    float *g_sum[3] = {NULL, NULL, NULL}, *g_shot = NULL;
-    double misfit = 0.0;
-    if (P.mode == 1) {
-        for (int k = 0; k < 3; k++) g_sum[k] = xmalloc(sizeof(float) * ncell);
-        g_shot = xmalloc(sizeof(float) * ncell);
-    }
-    const int n_order = st ? st->n_order : 0;
-    char name[4 * DNIO_STR];
-    for (int ishot = 1; ishot <= nshots; ishot++) {
-        if (P.readrec == 2 && ishot > 1) {
-            dnio_free(recpos);
-            io_ok(dnio_read_receivers(&P, ishot, &ntr, &recpos), "REC_FILE");
-        }
-        const int first = P.run_multiple_shots ? ishot - 1 : 0, count = P.run_multiple_shots ? 1 : nsrc;
-        float *loc = xmalloc(sizeof(float) * 8 * (size_t)count);
-        int nloc = 0;
-        io_ok(dnio_split_sources(&P, srcpos, nsrc, first, count, loc, &nloc), "sources");
-        /* pack the kept columns into an 8 x nloc table */
-        float *sp = xmalloc(sizeof(float) * 8 * (size_t)(nloc ? nloc : 1));
-        for (int r = 0; r < 8; r++)
-            for (int l = 0; l < nloc; l++) sp[r * nloc + l] = loc[r * count + l];
-        float *sig = xmalloc(sizeof(float) * (size_t)(nloc ? nloc : 1) * P.nt);
-        io_ok(dnio_wavelet(&P, sp, nloc, ishot, TS, n_order, sig), "wavelet");
-        dn_ok(dn_set_sources(ctx, nloc, sp, sig), "sources");
-        int *rp = xmalloc(sizeof(int) * 2 * (size_t)(ntr ? ntr : 1));
-        for (int k = 0; k < ntr; k++) {
-            rp[k] = recpos[k];
-            rp[ntr + k] = recpos[ntr + k];
-        }
-        dn_ok(dn_set_receivers(ctx, ntr, rp), "receivers");
-        const float XS = srcpos[0 * nsrc + ishot - 1], YS = srcpos[1 * nsrc + ishot - 1];
-        printf(" shot %d: %d source point(s), %d receivers\n", ishot, nloc, ntr);
-        fflush(stdout);
-        dn_ok(dn_run_forward(ctx, 0), "forward run");
-
-        if (P.mode == 0 && ntr > 0 && P.seismo) {
-            /* saveseis_glob: SEISMO 1 vx,vy; 2 p; 3 div,curl; 4 vx,vy,p */
-            int comps[5], nc = 0;
-            if (P.seismo == 1 || P.seismo == 4) comps[nc++] = 0, comps[nc++] = 1;
-            if (P.seismo == 2 || P.seismo == 4) comps[nc++] = 2;
-            if (P.seismo == 3) comps[nc++] = 3, comps[nc++] = 4;
-            float *sec = xmalloc(sizeof(float) * (size_t)ntr * P.nt);
-            for (int c = 0; c < nc; c++) {
-                dn_ok(dn_get_seismogram(ctx, comps[c], sec), "seismograms");
-                io_ok(dnio_seis_name(&P, comps[c], ishot, 1, name, sizeof name), "file name");
-                io_ok(dnio_write_seis(name, &P, sec, ntr, P.nt, recpos, XS, YS), name);
-                printf("   wrote %s\n", name);
-            }
-            free(sec);
-        }
-        if (P.mode == 1) {
-            float *obs[3] = {NULL, NULL, NULL};
-            const int qb = P.quelltypb;
-            const int use[3] = {qb == 1 || qb == 3 || qb == 5 || qb == 7, qb == 1 || qb == 2 || qb == 6 || qb == 7,
-                                qb >= 4};
-            for (int c = 0; c < 3; c++)
-                if (use[c]) {
-                    obs[c] = xmalloc(sizeof(float) * (size_t)ntr * P.nt);
-                    io_ok(dnio_data_name(&P, c, ishot, name, sizeof name), "file name");
-                    io_ok(dnio_read_su(name, ntr, P.nt, obs[c]), name);
-                }
-            const int test = ishot >= P.testshot_start && ishot <= P.testshot_end && P.testshot_incr > 0 &&
-                             (ishot - P.testshot_start) % P.testshot_incr == 0;
-            double l2 = 0.0;
-            dn_ok(dn_calc_residuals(ctx, obs[0], obs[1], obs[2], test, &l2), "residuals");
-            misfit += l2;
-            for (int c = 0; c < 3; c++) free(obs[c]);
-            dn_ok(dn_run_adjoint(ctx, 0), "adjoint run");
-            dn_ok(dn_assemble_gradient(ctx), "gradient");
-            for (int k = 0; k < 3; k++) {
-                dn_ok(dn_get_gradient(ctx, 3 + k, g_shot), "gradient");
-                for (size_t q = 0; q < ncell; q++) g_sum[k][q] += g_shot[q];
-                if (P.rtm_shot && k < 2) {
-                    snprintf(name, sizeof name, "%s_%s_image_shot_%i", P.jacobian, k == 0 ? "P" : "S", ishot);
-                    write_plane(name, g_shot, P.nx, P.ny);
-                }
-            }
-            printf("   L2 of shot %d: %.9e%s\n", ishot, l2, test ? "" : " (not a test shot)");
-        }
-        free(loc);
-        free(sp);
-        free(sig);
-        free(rp);
-    }
-    if (P.mode == 1) {
-        const char *suffix[3] = {"vp", "vs", "rho"};
-        for (int k = 0; k < 3; k++) {
-            snprintf(name, sizeof name, "%s_grad_%s", P.jacobian, suffix[k]);
-            write_plane(name, g_sum[k], P.nx, P.ny);
-            printf(" wrote %s\n", name);
-            free(g_sum[k]);
-        }
-        free(g_shot);
-        printf(" misfit (L2, test shots): %.9e\n", misfit);
-    }
-    dn_ctx_destroy(ctx);
-    dnio_free(recpos);
-    dnio_free(srcpos);
+    dn_ctx_destroy(r.ctx);
+    dnio_free(r.recpos);
+    dnio_free(r.srcpos);
     free(vp);
     free(vs);
     free(rho);

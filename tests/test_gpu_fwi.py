@@ -1,0 +1,113 @@
+"""FWI iterations of the drop-in executable (MODE 1, ITERMAX = 4, L-BFGS)
+against the same iterations re-done in Python: the gradients and the test
+misfits come from the library through its Python binding (their parity with
+the FP32 oracle is tested elsewhere), and the L-BFGS direction, the
+step-length search and the model update from the numpy restatement of the
+reference (oracle/fwi_oracle.py).  The models the executable writes after
+the last iteration, and its misfit log, must equal the Python run's bit for
+bit; the misfit of the starting model must fall over the iterations."""
+import os
+
+import numpy as np
+import pytest
+
+import dnhip
+import dnio
+import fwi_oracle as fo
+import inpfile
+import synth
+from test_gpu_cli import _files, _reference_run, _run
+
+pytestmark = pytest.mark.gpu
+
+ITERMAX = 4
+
+
+def test_cli_fwi_iterations_equal_python_iterations(tmp_path):
+    tmp = str(tmp_path)
+    nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
+    bg = synth.layered_model(nx, ny, dh, seed=5, sigma=3.0)
+    true = tuple(a.copy() for a in bg)
+    true[0][40:56, 10:20] *= 1.08
+    true[1][40:56, 10:20] *= 1.08
+    sources = [(300.0, 30.0, 15.0, 1.0, 1), (650.0, 30.0, 15.0, 1.0, 1)]
+    receivers = [(40.0 + 20.0 * k, 20.0) for k in range(40)]
+    _files(tmp, nx, ny, dh, true, sources, receivers, mname="model/true")
+    inpfile.write_inp(os.path.join(tmp, "true.inp"), NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/true", SEIS_FILE_VX="obs/d_x.su", SEIS_FILE_VY="obs/d_y.su")
+    _run(tmp, "true.inp")
+    _files(tmp, nx, ny, dh, bg, sources, receivers, mname="model/start")
+    inpfile.write_inp(os.path.join(tmp, "fwi.inp"), MODE=1, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1", ITERMAX=ITERMAX, INV_MOD_OUT=1,
+                      MISFIT_LOG_FILE="misfit.dat")
+    inpfile.write_workflow(os.path.join(tmp, "wf.inp"))
+    _run(tmp, "fwi.inp", "wf.inp")
+
+    par, p, m, rec, shots, src = _reference_run(tmp, "fwi.inp")
+    st = dnio.read_stage(os.path.join(tmp, "wf.inp"), 1)
+    upd = dict(vpupperlim=par.vpupperlim, vplowerlim=par.vplowerlim, vsupperlim=par.vsupperlim,
+               vslowerlim=par.vslowerlim, rhoupperlim=par.rhoupperlim, rholowerlim=par.rholowerlim,
+               scalerho=st.scalerho, inv_vp_iter=st.inv_vp_iter, inv_vs_iter=st.inv_vs_iter,
+               inv_rho_iter=st.inv_rho_iter, invmat1=par.invmat1)
+    obs = {i: [dnio.read_su(os.path.join(tmp, f"obs/d_{c}.su.shot{i}"), rec.shape[1], nt) for c in "xy"]
+           for i in (1, 2)}
+    g = dnhip.Context(p)
+    vp, vs, rho = (a.copy() for a in m[:3])
+
+    def shot(i, model):
+        g.set_model(*model)
+        loc, sig = shots[i - 1]
+        g.set_sources(loc, sig)
+        g.set_receivers(rec[:2])
+        g.forward()
+
+    def misfit(tv, ts, tr, itest):
+        l2 = 0.0
+        for i in (1, 2):
+            shot(i, (tv, ts, tr))
+            l2 += g.residuals(obs[i][0], obs[i][1], None, 1)
+        return l2
+
+    lb = fo.LBFGS(nx, ny, par.nlbfgs)
+    c = (fo.model_mean(vp), fo.model_mean(vs), fo.model_mean(rho))
+    carry = ([0.0, 0.0, 0.0], [0.0, 0.0, 0.0])
+    lines, l2s = [], []
+    for it in range(1, ITERMAX + 1):
+        grads = [np.zeros((nx, ny), np.float32) for _ in range(3)]
+        l2sum = 0.0
+        for i in (1, 2):
+            shot(i, (vp, vs, rho))
+            l2sum += g.residuals(obs[i][0], obs[i][1], None, 1)
+            g.adjoint()
+            g.assemble()
+            for k in range(3):
+                grads[k] += g.gradient(3 + k)
+        l2s.append(l2sum)
+        d = lb.direction(it, *c, *grads, vp, vs, rho)
+        ls = fo.step_length(upd, it, par.eps_scale, par.scalefac, par.stepmax, l2sum, vp, vs, rho, *d, misfit,
+                            carry=carry)
+        carry = (ls["epst"], ls["L2t"])
+        eps = ls["eps_scale"]
+        if it > par.min_iter and ls["step1"] == 0:
+            eps = np.float32(0.0)
+        lines.append("%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %d " % (
+            eps, *ls["epst"], *ls["L2t"], l2sum, 1))
+        vp, vs, rho, _ = fo.mat_change_test(upd, it, eps, vp, vs, rho, *d)
+        print(f"[fwi] iteration {it}: misfit {l2sum:.6e}, step {float(eps):.4e}, "
+              f"{ls['nevals']} test forward runs", flush=True)
+        # the stage ends (FWI_PSV.c:781-827: misfit change over two iterations
+        # within PRO, or a failed search) before the per-iteration models are written
+        if it > par.min_iter and ((it > 2 and abs((l2s[it - 3] - l2s[it - 1]) / l2s[it - 3]) <= st.pro)
+                                  or ls["step3"] == 1):
+            break
+        for name, a in (("vp", vp), ("vs", vs), ("rho", rho)):
+            got = np.fromfile(os.path.join(tmp, f"model/inv_{name}_stage_1_it_{it}.bin"), np.float32)
+            assert np.array_equal(got.reshape(nx, ny), a), (it, name)
+    g.close()
+    for name, a in (("vp", vp), ("vs", vs), ("rho", rho)):
+        got = np.fromfile(os.path.join(tmp, f"model/inv_{name}.bin"), np.float32).reshape(nx, ny)
+        assert np.array_equal(got, a), name
+    log = open(os.path.join(tmp, "misfit.dat")).read().splitlines()
+    assert log == lines[:len(log)] and len(log) == len(lines)
+    assert l2s[-1] < l2s[0]
+    assert not np.array_equal(vp, m[0])
