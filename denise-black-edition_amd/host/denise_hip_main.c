@@ -325,7 +325,7 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
             printf(" step length %e (trials %e %e %e, misfits %e %e %e)\n", opteps_vp, ls.epst[0], ls.epst[1],
                    ls.epst[2], ls.L2t[0], ls.L2t[1], ls.L2t[2]);
             L2_hist[iter] = L2sum;
-            /* FWI_PSV.c:749: the update, written as INV_MODELFILE_{vp,vs,rho}.bin */
+            /* FWI_PSV.c:750: the update, written as INV_MODELFILE_{vp,vs,rho}.bin */
             dnfwi_model_update(&up, iter, eps_scale, nx, ny, vp, vs, rho, g[0], g[1], g[2], nv, ns, nr);
             memcpy(vp, nv, sizeof(float) * r->ncell);
             memcpy(vs, ns, sizeof(float) * r->ncell);
@@ -353,7 +353,7 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
                 min_iter_help = iter + P->min_iter;
                 dnfwi_lbfgs_reset(lb);
                 printf(" %s: next FWI stage\n", ls.step3 ? "step-length search failed" : "abort criterion reached");
-                break; /* FWI_PSV.c:826: before iter_true++ */
+                break; /* FWI_PSV.c:824, before the iter_true++ of :835 */
             }
             if (P->inv_mod_out == 1) {
                 snprintf(name[0], sizeof name[0], "%s_vp_stage_%d_it_%d.bin", P->inv_modelfile, nstage, iter);
