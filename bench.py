@@ -298,7 +298,57 @@ def gradient_workload(nt=4000):
     return common, bg, true, src, sig, rec
 
 
-def run_gradient(steps, nt=4000, tune_steps=40, sample_every=7, tiles=None):
+def run_fwi_iterations(g, model, obs, iters, nlbfgs=20):
+    """FWI iterations of the gradient job's shot (the only, and test, shot)
+    through the product path: gradient on the device, L-BFGS direction and
+    model update on the host (libdnio), step-length search with device
+    forward runs of the test shot; EPS_SCALE 0.01, SCALEFAC 2, STEPMAX 6,
+    no limits, MIN_ITER above the iteration count (a search that finds no
+    lower misfit still applies its smallest trial step, FWI_PSV.c:698-708).
+    Seconds per part and iteration: the cost of an iteration, not a
+    convergence study (one shot, no gradient tapers)."""
+    import dnfwi
+    vp, vs, rho = (a.copy() for a in model)
+    nx, ny = vp.shape
+    lb = dnfwi.LBFGS(nx, ny, nlbfgs)
+    par = dnfwi.update_par(vpupperlim=1e9, vplowerlim=0.0, vsupperlim=1e9, vslowerlim=0.0, rhoupperlim=1e9,
+                           rholowerlim=0.0, scalerho=0.5)
+    state = dnfwi.LineSearch()
+    c = (dnfwi.model_mean(vp), dnfwi.model_mean(vs), dnfwi.model_mean(rho))
+    n_fwd = [0]
+
+    def misfit(tv, ts, tr, itest):
+        g.set_model(tv, ts, tr)
+        g.forward()
+        n_fwd[0] += 1
+        return g.residuals(obs[0], obs[1], None, 1)
+
+    g.set_kernel_sampling(0)
+    out = []
+    for it in range(1, iters + 1):
+        t0 = time.perf_counter()
+        g.set_model(vp, vs, rho)
+        g.forward()
+        l2 = g.residuals(obs[0], obs[1], None, 1)
+        g.adjoint()
+        g.assemble()
+        grads = [g.gradient(3 + k) for k in range(3)]
+        t1 = time.perf_counter()
+        d = lb.direction(it, *c, *grads, vp, vs, rho)
+        t2 = time.perf_counter()
+        n_fwd[0] = 0
+        ls = dnfwi.step_length(par, it, 0.01, 2.0, 6, l2, vp, vs, rho, *d, misfit, state=state)
+        t3 = time.perf_counter()
+        vp, vs, rho, _ = dnfwi.model_update(par, it, ls.eps_scale, vp, vs, rho, *d)
+        t4 = time.perf_counter()
+        out.append({"iteration": it, "misfit": l2, "step": ls.eps_scale, "test_forward_runs": n_fwd[0],
+                    "gradient_s": t1 - t0, "lbfgs_direction_s": t2 - t1, "step_search_s": t3 - t2,
+                    "model_update_s": t4 - t3, "total_s": t4 - t0})
+    lb.close()
+    return out
+
+
+def run_gradient(steps, nt=4000, tune_steps=40, sample_every=7, tiles=None, fwi_iters=0):
     """Time of the gradient job's two phases on resident inputs: the forward
     run that stores the snapshots, and the adjoint run whose sweeps
     correlate with them (residuals and assembly timed separately).  Cell
@@ -361,6 +411,7 @@ def run_gradient(steps, nt=4000, tune_steps=40, sample_every=7, tiles=None):
     if nt == 4000:      # a short profiling run need not reach the receivers
         assert np.abs(grad).max() > 0 and l2 > 0
     footprint = g.footprint()
+    fwi = run_fwi_iterations(g, bg, obs, fwi_iters) if fwi_iters > 0 else None
     g.close()
     cells = common["nx"] * common["ny"]
     cu = cells * nt
@@ -382,6 +433,7 @@ def run_gradient(steps, nt=4000, tune_steps=40, sample_every=7, tiles=None):
         "residuals_s": float(np.mean(t_r)), "assemble_and_copy_s": float(np.mean(t_g)),
         "forward_kernel_ms": list(kf), "adjoint_kernel_ms": list(ka),
         "misfit": l2,
+        "fwi_iterations": fwi,
     }
 
 
@@ -462,6 +514,8 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=8.0)
     ap.add_argument("--grad-jobs", type=int, default=2, help="timed gradient jobs (0: skip)")
     ap.add_argument("--grad-only", action="store_true", help="profiling: only the gradient job")
+    ap.add_argument("--fwi-iters", type=int, default=2,
+                    help="FWI iterations (L-BFGS, step search, update) after the gradient jobs (0: skip)")
     ap.add_argument("--nt", type=int, default=6000, help="profiling: shorter shots")
     ap.add_argument("--tune-steps", type=int, default=40)
     ap.add_argument("--tile-v", default="", help="profiling: fixed velocity-sweep shape tbx,by,ty,vw")
@@ -491,7 +545,8 @@ def main():
         return
     if args.grad_only:
         print(json.dumps({"gradient": run_gradient(max(1, args.grad_jobs), nt=min(args.nt, 4000),
-                                                   tune_steps=args.tune_steps, tiles=tiles)}))
+                                                   tune_steps=args.tune_steps, tiles=tiles,
+                                                   fwi_iters=args.fwi_iters)}))
         return
     if args.no_visco:
         elastic = run_gpu(0, args.steps, args.warmup, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles, policy=policy)
@@ -513,7 +568,7 @@ def main():
         print(f"# serving: {serving}", file=sys.stderr, flush=True)
     gradient = None
     if args.grad_jobs > 0:
-        gradient = run_gradient(args.grad_jobs, tune_steps=args.tune_steps)
+        gradient = run_gradient(args.grad_jobs, tune_steps=args.tune_steps, fwi_iters=args.fwi_iters)
         print(f"# gradient: {gradient}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
