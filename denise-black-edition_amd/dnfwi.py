@@ -1,8 +1,8 @@
 """ctypes binding of the host FWI functions of libdnio.so
 (include/denise_fwi.h): L-BFGS search direction, parabolic step-length
 search, bounded model update -- the FWI iteration of src/PSV/FWI_PSV.c
-around the device gradients -- and `iterate`, one FWI iteration over
-dnhip contexts.
+around the device gradients -- and the adjoint source of reverse-time
+migration (MODE 2).
 
 Arrays are float32 of shape (nx, ny) (model-file order, x-major)."""
 from __future__ import annotations
@@ -52,6 +52,8 @@ def load():
         "dnfwi_opt_step": ([C.POINTER(C.c_double), fp], C.c_float),
         "dnfwi_step_length": ([C.POINTER(UpdatePar), C.c_int, C.c_float, C.c_float, C.c_int, C.c_double, C.c_int,
                                C.c_int, fp, fp, fp, fp, fp, fp, _MISFIT, vp_, C.POINTER(LineSearch)], C.c_int),
+        "dnfwi_rtm_adjoint_source": ([fp, fp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, fp,
+                                      C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -157,3 +159,15 @@ def step_length(par, it, eps_scale0, scalefac, stepmax, L2t0, vp, vs, rho, d_vp,
     if err:
         raise err[0]
     return st
+
+
+def rtm_adjoint_source(data, syn, dt, grad_form=1, testshot=1):
+    """MODE 2 adjoint source (ntr x ns, time-reversed) and its L2."""
+    d = _f(data)
+    sy = _f(syn)
+    assert d.shape == sy.shape
+    out = np.empty_like(d)
+    l2 = C.c_double()
+    dnio._check(load().dnfwi_rtm_adjoint_source(_p(d), _p(sy), d.shape[0], d.shape[1], dt, grad_form, testshot,
+                                                 _p(out), C.byref(l2)))
+    return out, l2.value

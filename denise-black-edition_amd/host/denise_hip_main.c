@@ -27,6 +27,11 @@
  *             with the models of src/PSV/model_freq_out_PSV.c (INV_MOD_OUT = 0)
  *             or src/PSV/model_it_out_PSV.c (INV_MOD_OUT = 1).
  *           ITERMAX = 1 gives one gradient and one model update.
+ * MODE = 2: reverse-time migration (src/PSV/RTM_PSV.c:265-385): the gradient
+ *           pass with the time-integrated observed data as adjoint sources
+ *           (calc_res.c:434-442), written as JACOBIAN_P_image and
+ *           JACOBIAN_S_image (src/PSV/RTM_PSV_out.c) or per shot with
+ *           RTM_SHOT = 1.
  * Exit status 1 with a message on any error, as the reference's err().
  */
 #define _POSIX_C_SOURCE 200809L
@@ -67,13 +72,13 @@ static void *xmalloc(size_t n)
 static void check_supported(const dnio_par *p, const dnio_stage *st)
 {
     if (p->physics != 1) die("PHYSICS", "only 2D P/SV (PHYSICS = 1) is built");
-    if (p->mode != 0 && p->mode != 1) die("MODE", "MODE 0 (forward) and 1 (FWI) are built; RTM is not");
+    if (p->mode < 0 || p->mode > 2) die("MODE", "MODE 0 (forward), 1 (FWI) and 2 (RTM) are built");
     if (p->boundary != 0) die("BOUNDARY", "periodic boundaries are not supported");
     if (!p->srcrec) die("SRCREC", "sources must come from SOURCE_FILE (SRCREC = 1)");
     if (p->quellart == 6) die("QUELLART", "the spike source needs the Butterworth source filter, not built");
     if (p->snap) fprintf(stderr, " denise_hip: SNAP > 0 ignored (no snapshot output)\n");
-    if (p->mode == 1) {
-        if (!st) die("workflow", "MODE 1 needs a workflow file");
+    if (p->mode == 1 || p->mode == 2) {
+        if (!st) die("workflow", "MODE 1 and 2 need a workflow file");
         if (st->lnorm != 2) die("LNORM", "only the L2 misfit (LNORM = 2) is built");
         if (st->time_filt) die("TIME_FILT", "time-domain filtering is not built");
         if (st->eprecond) die("EPRECOND", "energy preconditioning is not built");
@@ -86,6 +91,9 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (p->sws_taper_grad_vert || p->sws_taper_grad_hor || p->sws_taper_grad_sources || p->sws_taper_file)
             die("SWS_TAPER_GRAD_*", "gradient tapers are not built");
         if (p->grad_form != 1 && p->grad_form != 2) die("GRAD_FORM", "1 or 2 expected");
+        if (p->trkill) die("TRKILL", "trace killing is not built");
+    }
+    if (p->mode == 1) {
         if (p->grad_method != 2) die("GRAD_METHOD", "the L-BFGS method (GRAD_METHOD = 2) is built; PCG is not");
         if (p->model_filter || p->grad_filter) die("MODEL_FILTER/GRAD_FILTER", "model and gradient filters are not built");
         if (p->gravity) die("GRAVITY", "gravity modelling is not built");
@@ -171,6 +179,34 @@ static double shot_residuals(run *r, int ishot, int test)
     return l2;
 }
 
+/* MODE 2: the observed data as adjoint sources (calc_res.c:416-446 with
+ * MODE = 2); returns their L2 (0 unless a test shot) */
+static double shot_rtm_sources(run *r, int ishot, int test)
+{
+    dnio_par *P = r->P;
+    const int qb = P->quelltypb;
+    const int use[3] = {qb == 1 || qb == 3 || qb == 5 || qb == 7, qb == 1 || qb == 2 || qb == 6 || qb == 7, qb >= 4};
+    const size_t n = (size_t)r->ntr * P->nt;
+    float *diff[3] = {NULL, NULL, NULL}, *obs = xmalloc(sizeof(float) * n), *syn = xmalloc(sizeof(float) * n);
+    double l2 = 0.0;
+    for (int c = 0; c < 3; c++)
+        if (use[c]) {
+            double l2c = 0.0;
+            diff[c] = xmalloc(sizeof(float) * n);
+            io_ok(dnio_data_name(P, c, ishot, r->name, sizeof r->name), "file name");
+            io_ok(dnio_read_su(r->name, r->ntr, P->nt, obs), r->name);
+            dn_ok(dn_get_seismogram(r->ctx, c, syn), "seismograms");
+            io_ok(dnfwi_rtm_adjoint_source(obs, syn, r->ntr, P->nt, P->dt, P->grad_form, test, diff[c], &l2c),
+                  "RTM adjoint source");
+            l2 += l2c;
+        }
+    dn_ok(dn_set_adjoint_sources(r->ctx, diff[0], diff[1], diff[2]), "adjoint sources");
+    for (int c = 0; c < 3; c++) free(diff[c]);
+    free(obs);
+    free(syn);
+    return l2;
+}
+
 /* MODE 0: forward runs and SU output */
 static void forward_pass(run *r)
 {
@@ -198,7 +234,8 @@ static void forward_pass(run *r)
 }
 
 /* grad_obj_psv: stacked vp/vs/rho gradients over all shots; returns the
- * L2 misfit summed over the test shots */
+ * L2 misfit summed over the test shots.  MODE 2 takes the data themselves
+ * as adjoint sources (reverse-time migration). */
 static double gradient_pass(run *r, float **g_sum)
 {
     dnio_par *P = r->P;
@@ -209,7 +246,7 @@ static double gradient_pass(run *r, float **g_sum)
         shot_setup(r, ishot);
         dn_ok(dn_run_forward(r->ctx, 0), "forward run");
         const int test = is_test_shot(P, ishot);
-        const double l2 = shot_residuals(r, ishot, test);
+        const double l2 = P->mode == 2 ? shot_rtm_sources(r, ishot, test) : shot_residuals(r, ishot, test);
         misfit += l2;
         dn_ok(dn_run_adjoint(r->ctx, 0), "adjoint run");
         dn_ok(dn_assemble_gradient(r->ctx), "gradient");
@@ -372,6 +409,27 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
     free(L2_hist);
 }
 
+/* MODE 2: src/PSV/RTM_PSV.c:265-385 -- workflow stage 1, one gradient pass
+ * with the data as adjoint sources, the stacked P and S images
+ * (RTM_PSV_out.c) or, with RTM_SHOT = 1, the per-shot images */
+static void rtm(run *r)
+{
+    dnio_par *P = r->P;
+    float *g[3];
+    for (int k = 0; k < 3; k++) g[k] = xmalloc(sizeof(float) * r->ncell);
+    const double l2 = gradient_pass(r, g);
+    printf(" L2 of the data (test shots): %.9e\n", l2);
+    if (P->rtm_shot == 0) {
+        const char *img[2] = {"P", "S"};
+        for (int k = 0; k < 2; k++) {
+            snprintf(r->name, sizeof r->name, "%s_%s_image", P->jacobian, img[k]);
+            write_plane(r->name, g[k], P->nx, P->ny);
+            printf(" wrote %s\n", r->name);
+        }
+    }
+    for (int k = 0; k < 3; k++) free(g[k]);
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 2) {
@@ -381,8 +439,8 @@ int main(int argc, char **argv)
     dnio_par P;
     io_ok(dnio_read_par(argv[1], &P), "parameter file");
     dnio_stage stage, *st = NULL;
-    if (P.mode == 1) {
-        if (argc < 3) die("workflow", "MODE 1 needs a workflow file as second argument");
+    if (P.mode == 1 || P.mode == 2) {
+        if (argc < 3) die("workflow", "MODE 1 and 2 need a workflow file as second argument");
         io_ok(dnio_read_stage(argv[2], 1, &stage), "workflow file");
         st = &stage;
     }
@@ -416,13 +474,13 @@ int main(int argc, char **argv)
     dp.free_surf = P.free_surf;
     dp.boundary = P.boundary;
     dp.L = P.L;
-    dp.seismo = P.mode == 1 && P.seismo == 0 ? 1 : P.seismo;
+    dp.seismo = P.mode >= 1 && P.seismo == 0 ? 1 : P.seismo;
     dp.ndt = P.ndt;
     dp.invmat1 = P.invmat1;
     dp.grad_form = P.grad_form;
     dp.dtinv = P.dtinv > 0 ? P.dtinv : 1;
     dp.quelltypb = P.quelltypb;
-    dp.mode = P.mode;
+    dp.mode = P.mode >= 1 ? 1 : 0; /* MODE 2 runs the gradient machinery with other adjoint sources */
     dp.dh = P.dh;
     dp.dt = P.dt;
     dp.damping = P.damping;
@@ -434,13 +492,15 @@ int main(int argc, char **argv)
 
     dn_ok(dn_ctx_create(&dp, 0, &r.ctx), "device context");
     dn_ok(dn_set_model(r.ctx, vp, vs, rho, taus, taup), "model upload");
-    if (P.seismo || P.mode == 1) io_ok(dnio_read_receivers(&P, 1, &r.ntr, &r.recpos), "REC_FILE");
+    if (P.seismo || P.mode >= 1) io_ok(dnio_read_receivers(&P, 1, &r.ntr, &r.recpos), "REC_FILE");
     r.n_order = st ? st->n_order : 0;
 
     if (P.mode == 0)
         forward_pass(&r);
-    else
+    else if (P.mode == 1)
         fwi(&r, argv[2], vp, vs, rho, taus, taup);
+    else
+        rtm(&r);
 
     dn_ctx_destroy(r.ctx);
     dnio_free(r.recpos);

@@ -351,3 +351,28 @@ int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, flo
     out->nevals = nevals;
     return 0;
 }
+
+int dnfwi_rtm_adjoint_source(const float *data, const float *syn, int ntr, int ns, float dt, int grad_form,
+                             int testshot, float *diff, double *l2)
+{
+    if (!data || !syn || !diff || ntr < 0 || ns < 1 || (grad_form != 1 && grad_form != 2))
+        return dnio_fail_msg("dnfwi_rtm_adjoint_source: bad argument");
+    double L2 = 0.0;
+    for (int i = 0; i < ntr; i++) {
+        const float *d = data + (size_t)i * ns, *s = syn + (size_t)i * ns;
+        float *out = diff + (size_t)i * ns;
+        float intseis = 0.0f;
+        /* calc_res.c:416-446: invtime runs ns..1 while j runs 1..ns */
+        for (int j = 0; j < ns; j++) {
+            const float dj = j == 0 ? s[0] : d[j];
+            if (grad_form == 1) intseis += dt * dj;
+            else intseis = dj;
+            out[ns - 1 - j] = intseis;
+        }
+        /* calc_res.c:563-569 */
+        if (testshot)
+            for (int j = 0; j < ns; j++) L2 += 0.5 * out[ns - 1 - j] * out[ns - 1 - j];
+    }
+    if (l2) *l2 = L2;
+    return 0;
+}

@@ -101,6 +101,16 @@ int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, flo
                       const float *d_vp, const float *d_vs, const float *d_rho, dnfwi_misfit_fn misfit,
                       void *user, dnfwi_linesearch *out);
 
+/* Adjoint source of reverse-time migration (MODE 2), the MODE 2 branch of
+ * calc_res with LNORM = 2 (src/calc_res.c:416-446): per trace the first data
+ * sample is replaced by the synthetic one, then GRAD_FORM 1 integrates
+ * DT * data in time (single precision), GRAD_FORM 2 takes the data, and the
+ * result is stored time-reversed.  data, syn, diff: ntr x ns (trace-major).
+ * l2 (may be NULL) gets sum 0.5*diff^2 in the reference's order
+ * (src/calc_res.c:563-569), or 0 when testshot is 0. */
+int dnfwi_rtm_adjoint_source(const float *data, const float *syn, int ntr, int ns, float dt, int grad_form,
+                             int testshot, float *diff, double *l2);
+
 #ifdef __cplusplus
 }
 #endif

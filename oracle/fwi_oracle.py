@@ -13,7 +13,9 @@ Written from the reference's functions, one per function, keeping its
   - model_mean: src/PSV/FWI_PSV.c:540-580;
   - mat_change_test: src/PSV/calc_mat_change_test_PSV.c:10-251;
   - opt_step: src/calc_opt_step.c:14-84 with src/solvelin.c:19-48;
-  - step_length: src/PSV/step_length_est_psv.c:37-182.
+  - step_length: src/PSV/step_length_est_psv.c:37-182;
+  - rtm_adjoint_source: the MODE 2 branch of src/calc_res.c:416-446 and
+    its L2 sum, src/calc_res.c:563-569.
 Vectors are numpy float32 arrays of shape (nx, ny) (the model-file order).
 The pins of these functions against what the mathematics fixes (BFGS
 matrix recursion, parabola vertex, limits) are in tests/test_fwi_host.py.
@@ -232,3 +234,28 @@ def step_length(par, it, eps_scale0, scalefac, stepmax, L2t0, vp, vs, rho, d_vp,
         eps_scale = opt_step(L2t[1:4], epst1[1:4])
     return {"eps_scale": F32(eps_scale), "epst": [F32(v) for v in epst1[1:4]], "L2t": L2t[1:4],
             "step1": step1, "step3": step3, "nevals": len(trials), "trials": trials}
+
+
+def rtm_adjoint_source(data, syn, dt, grad_form=1, testshot=1):
+    """calc_res for MODE 2, LNORM 2: sectiondata[i][1] = section[i][1];
+    intseis += DT*sectiondata[i][j] (GRAD_FORM 1) or intseis =
+    sectiondata[i][j] (GRAD_FORM 2); sectiondiff[i][invtime] = intseis with
+    invtime = ns - j + 1.  Returns (diff, L2)."""
+    ntr, ns = data.shape
+    diff = np.zeros((ntr, ns), F32)
+    L2 = 0.0
+    dt = F32(dt)
+    for i in range(ntr):
+        sd = data[i].astype(F32).copy()
+        sd[0] = syn[i, 0]
+        intseis = F32(0.0)
+        for j in range(1, ns + 1):
+            if grad_form == 1:
+                intseis = F32(intseis + F32(dt * sd[j - 1]))
+            else:
+                intseis = sd[j - 1]
+            diff[i, ns - j] = intseis
+        if testshot:
+            for invtime in range(ns, 0, -1):
+                L2 += 0.5 * float(diff[i, invtime - 1]) * float(diff[i, invtime - 1])
+    return diff, L2

@@ -262,3 +262,29 @@ def test_step_length_no_upper_bracket():
     got, ref, _, _ = _search_both(misfit, L2t0=1.0, stepmax=2)
     assert got.step1 == 1 and got.step3 == 0
     assert got.eps_scale == float(got.epst[2])
+
+
+def test_rtm_adjoint_source():
+    """MODE 2 adjoint source: equal to the restatement bit for bit; pinned by
+    the integral of a constant trace (j*DT*c, time-reversed), the sample-1
+    rule (the synthetic's first sample replaces the data's) and GRAD_FORM 2
+    (the time-reversed data)."""
+    rng = np.random.default_rng(31)
+    data = rng.standard_normal((5, 300)).astype(F32)
+    syn = rng.standard_normal((5, 300)).astype(F32)
+    for gf in (1, 2):
+        for test in (0, 1):
+            got, l2 = dnfwi.rtm_adjoint_source(data, syn, 1e-3, gf, test)
+            ref, l2r = fo.rtm_adjoint_source(data, syn, 1e-3, gf, test)
+            assert np.array_equal(got, ref) and l2 == l2r
+            assert (l2 > 0) == bool(test)
+    c = np.full((2, 200), 3.0, F32)
+    got, _ = dnfwi.rtm_adjoint_source(c, c, 1e-3, 1, 1)
+    ramp = 3.0 * 1e-3 * np.arange(1, 201)
+    assert np.allclose(got[:, ::-1], ramp, rtol=2e-5)
+    syn1 = c.copy()
+    syn1[:, 0] = 7.0
+    got1, _ = dnfwi.rtm_adjoint_source(c, syn1, 1e-3, 1, 1)
+    assert np.allclose(got1[:, ::-1] - got[:, ::-1], 4.0 * 1e-3, rtol=1e-3)
+    got2, _ = dnfwi.rtm_adjoint_source(data, syn, 1e-3, 2, 1)
+    assert np.array_equal(got2[:, ::-1][:, 1:], data[:, 1:]) and np.array_equal(got2[:, -1], syn[:, 0])

@@ -111,3 +111,58 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path):
     assert log == lines[:len(log)] and len(log) == len(lines)
     assert l2s[-1] < l2s[0]
     assert not np.array_equal(vp, m[0])
+
+
+def test_cli_rtm_images_equal_library_and_oracle(tmp_path):
+    """MODE 2 (reverse-time migration, RTM_SHOT = 0): the executable's
+    stacked P and S images equal the library's gradients with the MODE 2
+    adjoint sources of the restatement (the time-integrated observed data)
+    bit for bit, and the FP32 oracle's within the gradient tolerance."""
+    import orc
+    from cases import rel_l2
+    tmp = str(tmp_path)
+    nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
+    bg = synth.layered_model(nx, ny, dh, seed=7, sigma=3.0)
+    true = tuple(a.copy() for a in bg)
+    true[0][30:70, 30:34] *= 1.15
+    sources = [(300.0, 30.0, 15.0, 1.0, 1), (650.0, 30.0, 15.0, 1.0, 1)]
+    receivers = [(40.0 + 20.0 * k, 20.0) for k in range(40)]
+    _files(tmp, nx, ny, dh, true, sources, receivers, mname="model/true")
+    inpfile.write_inp(os.path.join(tmp, "true.inp"), NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/true", SEIS_FILE_VX="obs/d_x.su", SEIS_FILE_VY="obs/d_y.su")
+    _run(tmp, "true.inp")
+    _files(tmp, nx, ny, dh, bg, sources, receivers, mname="model/mig")
+    inpfile.write_inp(os.path.join(tmp, "rtm.inp"), MODE=2, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/mig", DTINV=2, RTM_SHOT=0, TESTSHOT="1, 2, 1")
+    inpfile.write_workflow(os.path.join(tmp, "wf.inp"))
+    _run(tmp, "rtm.inp", "wf.inp")
+
+    par, p, m, rec, shots, src = _reference_run(tmp, "rtm.inp")
+    p.mode = 1
+    g = dnhip.Context(p)
+    g.set_model(*m)
+    o = orc.Oracle(p, *m, prec="f32")
+    total = [np.zeros((nx, ny), np.float32) for _ in range(2)]
+    ototal = [np.zeros((nx, ny), np.float32) for _ in range(2)]
+    for ishot, (loc, sig) in enumerate(shots, start=1):
+        obs = [dnio.read_su(os.path.join(tmp, f"obs/d_{c}.su.shot{ishot}"), rec.shape[1], nt) for c in "xy"]
+        for runner, acc in ((g, total), (o, ototal)):
+            runner.set_sources(loc, sig)
+            runner.set_receivers(rec[:2])
+            runner.forward()
+            diffs = [fo.rtm_adjoint_source(obs[c], runner.seismogram(c), par.dt, par.grad_form, 1)[0]
+                     for c in (0, 1)]
+            runner.set_adjoint_sources(diffs[0], diffs[1], None)
+            runner.adjoint()
+            runner.assemble()
+            for k in range(2):
+                acc[k] += runner.gradient(3 + k).astype(np.float32)
+    g.close()
+    o.close()
+    for k, name in enumerate(("P", "S")):
+        got = np.fromfile(os.path.join(tmp, f"jac/j_{name}_image"), np.float32).reshape(nx, ny)
+        assert np.array_equal(got, total[k]), name
+        assert np.abs(got).max() > 0
+        err = rel_l2(got, ototal[k])
+        print(f"[rtm] {name} image rel-L2 vs FP32 oracle {err:.2e}", flush=True)
+        assert err <= 1e-4, name
