@@ -873,6 +873,17 @@ int dn_get_store_hint(dn_ctx *c, int sweep, int *streaming)
     return DN_OK;
 }
 
+int dn_set_snapshots(dn_ctx *c, int first, int inc, int last, dn_snap_fn fn, void *user)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    c->snap_fn = fn;
+    c->snap_user = user;
+    c->snap_first = first;
+    c->snap_inc = inc;
+    c->snap_last = last;
+    return DN_OK;
+}
+
 int dn_set_kernel_sampling(dn_ctx *c, int every)
 {
     if (!c || every < 0) return fail(DN_ERR_ARG, "bad sampling interval");
@@ -899,7 +910,8 @@ int dn_run_forward(dn_ctx *c, int nsteps)
     ctl.probe = 1;
     size_t nev = 0;
     HIP_OK(hipEventRecord(c->ev0, c->stream));
-    int hin = 1, hin1 = 1;
+    int hin = 1, hin1 = 1, lsnap = c->snap_first, nsnap = 0;
+    std::vector<float> snapbuf;
     for (int nt = 1; nt <= nsteps; ++nt) {
         const bool store = fwdstore && nt == hin1;
         const bool sample = c->sample_every > 0 && (nt % c->sample_every) == 0;
@@ -927,6 +939,17 @@ int dn_run_forward(dn_ctx *c, int nsteps)
         if (store) {
             ++hin;
             hin1 += c->p.dtinv;
+        }
+        if (c->snap_fn && nt == lsnap && nt <= c->snap_last) {
+            // psv.c:225-231: the step's wavefields, after sources and the free surface
+            const size_t np = (size_t)(c->p.nx + 2 * c->nd) * (c->p.ny + 2 * c->nd);
+            snapbuf.resize(4 * np);
+            const int ids[4] = {F_VX, F_VY, F_SXX, F_SYY};
+            for (int k = 0; k < 4; ++k)
+                if ((rc = dn_get_field(c, ids[k], snapbuf.data() + k * np))) return rc;
+            c->snap_fn(c->snap_user, nt, ++nsnap, snapbuf.data(), snapbuf.data() + np, snapbuf.data() + 2 * np,
+                       snapbuf.data() + 3 * np);
+            lsnap += c->snap_inc > 0 ? c->snap_inc : nsteps + 1;
         }
     }
     HIP_OK(hipGetLastError());

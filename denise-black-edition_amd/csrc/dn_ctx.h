@@ -84,6 +84,10 @@ struct dn_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double run_ms = 0.0, v_ms = 0.0, s_ms = 0.0;
     int sample_every = 0, samples = 0;
+    // wavefield snapshots of dn_run_forward (dn_set_snapshots)
+    dn_snap_fn snap_fn = nullptr;
+    void *snap_user = nullptr;
+    int snap_first = 0, snap_inc = 0, snap_last = 0;
     std::vector<hipEvent_t> pool;
 
     dn_arena *arena = nullptr;  // buffers come from this arena, else hipMalloc

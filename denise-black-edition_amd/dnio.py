@@ -99,6 +99,7 @@ def load():
         "dnio_write_model": ([cp, fp, C.c_int, C.c_int], C.c_int),
         "dnio_write_seis": ([cp, C.POINTER(Par), fp, C.c_int, C.c_int, ip, C.c_float, C.c_float], C.c_int),
         "dnio_read_su": ([cp, C.c_int, C.c_int, fp], C.c_int),
+        "dnio_write_snap": ([C.POINTER(Par), C.c_int, fp, fp, fp, fp, fp, fp, fp], C.c_int),
         "dnio_seis_name": ([C.POINTER(Par), C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int], C.c_int),
         "dnio_data_name": ([C.POINTER(Par), C.c_int, C.c_int, C.c_char_p, C.c_int], C.c_int),
     }
@@ -193,6 +194,13 @@ def read_model(par):
                                   _fp(taus) if taus is not None else None,
                                   _fp(taup) if taup is not None else None))
     return vp, vs, rho, taus, taup
+
+
+def write_snap(par, nsnap, vx, vy, sxx, syy, ppi, pu, hc):
+    """One snapshot (snap_ssg): padded fields as dn_get_field gives them,
+    ppi/pu x-major (nx, ny), hc[0..6]."""
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (vx, vy, sxx, syy, ppi, pu, hc)]
+    _check(load().dnio_write_snap(C.byref(par), nsnap, *[_fp(a) for a in arrs]))
 
 
 def write_model(path, a):

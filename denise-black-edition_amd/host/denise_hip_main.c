@@ -5,7 +5,9 @@
  *     denise_hip <model.inp> [<workflow.inp>]      (src/denise.c:60-88)
  *
  * MODE = 0: per shot, the forward run of src/PSV/FD_PSV.c:271-354 and the SU
- *           files of saveseis_glob (SEIS_FILE_*.shot<N>).
+ *           files of saveseis_glob (SEIS_FILE_*.shot<N>); with SNAP > 0 shot
+ *           SNAP_SHOT only, with the wavefield snapshots of src/snap_ssg.c
+ *           (SNAP_FILE.{bin,asc}.{x,y,p,div,rot}.0.0).
  * MODE = 1: the FWI workflow of src/PSV/FWI_PSV.c:381-836 with L-BFGS
  *           (GRAD_METHOD = 2): for each stage of the workflow file and each
  *           iteration up to ITERMAX,
@@ -76,7 +78,9 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
     if (p->boundary != 0) die("BOUNDARY", "periodic boundaries are not supported");
     if (!p->srcrec) die("SRCREC", "sources must come from SOURCE_FILE (SRCREC = 1)");
     if (p->quellart == 6) die("QUELLART", "the spike source needs the Butterworth source filter, not built");
-    if (p->snap) fprintf(stderr, " denise_hip: SNAP > 0 ignored (no snapshot output)\n");
+    if (p->snap && p->mode != 0) fprintf(stderr, " denise_hip: SNAP > 0 ignored in MODE %d (snapshots of MODE 0 only)\n", p->mode);
+    if (p->snap && p->mode == 0 && p->snap_format != 2 && p->snap_format != 3)
+        die("SNAP_FORMAT", "snapshots are written as ASCII (2) or binary (3); SU (1) is not implemented, as in the reference");
     if (p->mode == 1 || p->mode == 2) {
         if (!st) die("workflow", "MODE 1 and 2 need a workflow file");
         if (st->lnorm != 2) die("LNORM", "only the L2 misfit (LNORM = 2) is built");
@@ -115,6 +119,8 @@ typedef struct run {
     float *srcpos, TS;
     int *recpos;
     size_t ncell;
+    const float *vp, *vs; /* the model as read: ppi, pu of the snapshot writer */
+    float hc[7];
     char name[4 * DNIO_STR];
 } run;
 
@@ -207,11 +213,32 @@ static double shot_rtm_sources(run *r, int ishot, int test)
     return l2;
 }
 
-/* MODE 0: forward runs and SU output */
+/* include/fd.h: iround(x) = (int)floor(x + 0.5) of a float quotient */
+static int iround_f(float x) { return (int)floor(x + 0.5); }
+
+static void snap_writer(void *user, int nt, int nsnap, const float *vx, const float *vy, const float *sxx,
+                        const float *syy)
+{
+    run *r = user;
+    printf("   snapshot %d at T = %f s\n", nsnap, nt * r->P->dt);
+    io_ok(dnio_write_snap(r->P, nsnap, vx, vy, sxx, syy, r->vp, r->vs, r->hc), "snapshot");
+}
+
+/* MODE 0: forward runs and SU output; with SNAP > 0 only shot SNAP_SHOT
+ * (FD_PSV.c:264-268), with its snapshots (psv.c:75-77, 225-231) */
 static void forward_pass(run *r)
 {
     dnio_par *P = r->P;
-    for (int ishot = 1; ishot <= r->nshots; ishot++) {
+    int first = 1, last = r->nshots;
+    if (P->snap) {
+        if (P->snap_shot < 1 || P->snap_shot > r->nshots) die("SNAP_SHOT", "no such shot");
+        first = last = P->snap_shot;
+        dn_ok(dn_get_hc(r->ctx, r->hc), "FD coefficients");
+        dn_ok(dn_set_snapshots(r->ctx, iround_f(P->tsnap1 / P->dt), iround_f(P->tsnapinc / P->dt),
+                               iround_f(P->tsnap2 / P->dt), snap_writer, r),
+              "snapshots");
+    }
+    for (int ishot = first; ishot <= last; ishot++) {
         shot_setup(r, ishot);
         dn_ok(dn_run_forward(r->ctx, 0), "forward run");
         if (r->ntr > 0 && P->seismo) {
@@ -462,6 +489,8 @@ int main(int argc, char **argv)
     float *rho = xmalloc(sizeof(float) * r.ncell);
     float *taus = P.L ? xmalloc(sizeof(float) * r.ncell) : NULL, *taup = P.L ? xmalloc(sizeof(float) * r.ncell) : NULL;
     io_ok(dnio_read_model(&P, vp, vs, rho, taus, taup), "model");
+    r.vp = vp;
+    r.vs = vs;
 
     dn_params dp;
     memset(&dp, 0, sizeof dp);

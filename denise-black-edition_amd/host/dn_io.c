@@ -663,3 +663,99 @@ int dnio_data_name(const dnio_par *p, int comp, int ishot, char *out, int n)
     snprintf(out, (size_t)n, "%s%s.su.shot%d", p->data_dir, suf[comp], ishot);
     return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Wavefield snapshots (src/snap_ssg.c:10-184, src/writedsk.c:15-43)
+ * ---------------------------------------------------------------------- */
+static int snap_put(FILE *f, float amp, int format)
+{
+    if (format == 2) return fprintf(f, "%e\n", amp) < 0;
+    return fwrite(&amp, sizeof(float), 1, f) != 1;
+}
+
+int dnio_write_snap(const dnio_par *p, int nsnap, const float *vx, const float *vy, const float *sxx,
+                    const float *syy, const float *ppi, const float *pu, const float *hc)
+{
+    if (!p || !vx || !vy || !sxx || !syy || !ppi || !pu || !hc) return fail("dnio_write_snap: null argument");
+    if (p->snap_format != 2 && p->snap_format != 3)
+        return fail(" Sorry, SU-format for snapshots not implemented yet. (SNAP_FORMAT = %d)", p->snap_format);
+    if (p->snap < 1 || p->snap > 4) return fail("dnio_write_snap: SNAP = %d", p->snap);
+    const int NX = p->nx, NY = p->ny, nd = p->fdorder / 2 + 1, fdoh = p->fdorder / 2;
+    const int idx = p->idx > 0 ? p->idx : 1, idy = p->idy > 0 ? p->idy : 1;
+    const size_t ncol = (size_t)NX + 2 * nd;
+#define NR(a, j, i) (a)[((size_t)((j) + nd - 1)) * ncol + (size_t)((i) + nd - 1)]
+#define MOD(a, j, i) (a)[(size_t)((i)-1) * NY + (size_t)((j)-1)]
+    const char *ext = p->snap_format == 2 ? ".asc" : ".bin";
+    const char *mode = nsnap == 1 ? "w" : "a";
+    char name[2][2 * DNIO_STR + 32];
+    const float dhi = 1.0 / p->dh;
+    int bad = 0;
+    if (p->snap == 1 || p->snap == 4) {
+        snprintf(name[0], sizeof name[0], "%s%s.x.0.0", p->snap_file, ext);
+        snprintf(name[1], sizeof name[1], "%s%s.y.0.0", p->snap_file, ext);
+        FILE *fx = fopen(name[0], mode), *fy = fopen(name[1], mode);
+        if (!fx || !fy) {
+            if (fx) fclose(fx);
+            if (fy) fclose(fy);
+            return fail("cannot open snapshot file %s", !fx ? name[0] : name[1]);
+        }
+        for (int i = 1; i <= NX; i += idx)
+            for (int j = 1; j <= NY; j += idy) {
+                bad |= snap_put(fx, NR(vx, j, i), p->snap_format);
+                bad |= snap_put(fy, NR(vy, j, i), p->snap_format);
+            }
+        fclose(fx);
+        fclose(fy);
+    }
+    if (p->snap == 2) {
+        snprintf(name[0], sizeof name[0], "%s%s.p.0.0", p->snap_file, ext);
+        FILE *fp = fopen(name[0], mode);
+        if (!fp) return fail("cannot open snapshot file %s", name[0]);
+        for (int i = 1; i <= NX; i += idx)
+            for (int j = 1; j <= NY; j += idy) {
+                const float amp = -NR(sxx, j, i) - NR(syy, j, i);
+                bad |= snap_put(fp, amp, p->snap_format);
+            }
+        fclose(fp);
+    }
+    if (p->snap == 3 || p->snap == 4) {
+        /* curl and divergence (Dougherty and Stephen, PAGEOPH 1988), snap_ssg.c:120-178 */
+        snprintf(name[0], sizeof name[0], "%s%s.div.0.0", p->snap_file, ext);
+        snprintf(name[1], sizeof name[1], "%s%s.rot.0.0", p->snap_file, ext);
+        FILE *fd = fopen(name[0], mode), *fr = fopen(name[1], mode);
+        if (!fd || !fr) {
+            if (fd) fclose(fd);
+            if (fr) fclose(fr);
+            return fail("cannot open snapshot file %s", !fd ? name[0] : name[1]);
+        }
+        for (int i = 1; i <= NX; i += idx)
+            for (int j = 1; j <= NY; j += idy) {
+                float vyx = 0, vxy = 0;
+                for (int m = 1; m <= fdoh; m++) {
+                    vyx += hc[m] * (NR(vy, j, i + m) - NR(vy, j, i - m + 1));
+                    vxy += hc[m] * (NR(vx, j + m, i) - NR(vx, j - m + 1, i));
+                }
+                vyx *= dhi;
+                vxy *= dhi;
+                const float curl = (vxy - vyx) * sqrt(MOD(pu, j, i));
+                bad |= snap_put(fr, curl, p->snap_format);
+            }
+        for (int i = 1; i <= NX; i += idx)
+            for (int j = 1; j <= NY; j += idy) {
+                float vxx = 0, vyy = 0;
+                for (int m = 1; m <= fdoh; m++) {
+                    vxx += hc[m] * (NR(vx, j, i + m - 1) - NR(vx, j, i - m));
+                    vyy += hc[m] * (NR(vy, j + m - 1, i) - NR(vy, j - m, i));
+                }
+                vxx *= dhi;
+                vyy *= dhi;
+                const float div = (vxx + vyy) * sqrt(MOD(ppi, j, i));
+                bad |= snap_put(fd, div, p->snap_format);
+            }
+        fclose(fd);
+        fclose(fr);
+    }
+#undef NR
+#undef MOD
+    return bad ? fail("write error in snapshot files %s*", p->snap_file) : 0;
+}

@@ -198,6 +198,16 @@ int dn_autotune_tile(dn_ctx *ctx, int nsteps, int *vel_shape, int *str_shape);
  * average duration of the velocity and stress kernels. every = 0 turns
  * sampling off; every = k times one launch of each in k steps. */
 int dn_set_kernel_sampling(dn_ctx *ctx, int every);
+
+/* Wavefield snapshots during dn_run_forward (src/PSV/psv.c:75-77, 225-231):
+ * after time step `first` and every `inc` steps after it, up to `last`,
+ * the run waits for the device and calls fn with the step, the snapshot
+ * count of this run (from 1) and vx, vy, sxx, syy in the dn_get_field
+ * layout (valid during the call).  inc <= 0 gives one snapshot; fn = NULL
+ * switches snapshots off.  The multi-shot engine takes none. */
+typedef void (*dn_snap_fn)(void *user, int nt, int nsnap, const float *vx, const float *vy, const float *sxx,
+                           const float *syy);
+int dn_set_snapshots(dn_ctx *ctx, int first, int inc, int last, dn_snap_fn fn, void *user);
 int dn_get_timing(dn_ctx *ctx, double *run_ms, double *v_kernel_ms, double *s_kernel_ms,
                   int *samples);
 

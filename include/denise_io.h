@@ -165,6 +165,16 @@ int dnio_read_su(const char *path, int ntr, int ns, float *section);
 /* File names the reference uses: saveseis_glob (src/saveseis_glob.c:35-50)
  * comp 0 vx, 1 vy, 2 p, 3 div, 4 curl; observed data of inseis sws 1/2/11
  * (src/inseis.c:21-61) comp 0 vx, 1 vy, 2 p. */
+/* One wavefield snapshot (src/snap_ssg.c:10-184, src/writedsk.c): SNAP 1
+ * vx, vy; 2 -sxx-syy; 3 div*sqrt(ppi) and curl*sqrt(pu) of the velocity by
+ * the FD stencil hc; 4 both 1 and 3; every IDX-th column and IDY-th row;
+ * SNAP_FORMAT 2 ASCII "%e", 3 raw float32 (1 is refused, as by writedsk).
+ * Files SNAP_FILE<.asc|.bin>.{x,y,p,div,rot}.0.0 (one rank's POS), created
+ * at nsnap = 1 and appended to after.  vx, vy, sxx, syy: the padded NR layout
+ * of dn_get_field; ppi, pu: the model as read (x-major NX x NY; vp and vs
+ * for INVMAT1 = 1); hc: hc[0..6] (dn_get_hc). */
+int dnio_write_snap(const dnio_par *p, int nsnap, const float *vx, const float *vy, const float *sxx,
+                    const float *syy, const float *ppi, const float *pu, const float *hc);
 int dnio_seis_name(const dnio_par *p, int comp, int ishot, int iter, char *out, int n);
 int dnio_data_name(const dnio_par *p, int comp, int ishot, char *out, int n);
 
