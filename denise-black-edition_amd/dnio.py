@@ -99,6 +99,7 @@ def load():
         "dnio_write_model": ([cp, fp, C.c_int, C.c_int], C.c_int),
         "dnio_write_seis": ([cp, C.POINTER(Par), fp, C.c_int, C.c_int, ip, C.c_float, C.c_float], C.c_int),
         "dnio_read_su": ([cp, C.c_int, C.c_int, fp], C.c_int),
+        "dnio_tdfilt": ([fp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_float, C.c_float], C.c_int),
         "dnio_write_snap": ([C.POINTER(Par), C.c_int, fp, fp, fp, fp, fp, fp, fp], C.c_int),
         "dnio_seis_name": ([C.POINTER(Par), C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_int], C.c_int),
         "dnio_data_name": ([C.POINTER(Par), C.c_int, C.c_int, C.c_char_p, C.c_int], C.c_int),
@@ -194,6 +195,16 @@ def read_model(par):
                                   _fp(taus) if taus is not None else None,
                                   _fp(taup) if taup is not None else None))
     return vp, vs, rho, taus, taup
+
+
+def tdfilt(data, dt, order, fc2, fc1=0.0):
+    """Butterworth low-pass at fc2 (and high-pass at fc1 > 0) of each row
+    of data (ntr x ns); returns the filtered float32 copy."""
+    d = np.array(data, np.float32, copy=True, order="C")
+    if d.ndim == 1:
+        d = d[None, :]
+    _check(load().dnio_tdfilt(_fp(d), d.shape[0], d.shape[1], dt, order, fc2, fc1))
+    return d
 
 
 def write_snap(par, nsnap, vx, vy, sxx, syy, ppi, pu, hc):

@@ -77,14 +77,12 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
     if (p->mode < 0 || p->mode > 2) die("MODE", "MODE 0 (forward), 1 (FWI) and 2 (RTM) are built");
     if (p->boundary != 0) die("BOUNDARY", "periodic boundaries are not supported");
     if (!p->srcrec) die("SRCREC", "sources must come from SOURCE_FILE (SRCREC = 1)");
-    if (p->quellart == 6) die("QUELLART", "the spike source needs the Butterworth source filter, not built");
     if (p->snap && p->mode != 0) fprintf(stderr, " denise_hip: SNAP > 0 ignored in MODE %d (snapshots of MODE 0 only)\n", p->mode);
     if (p->snap && p->mode == 0 && p->snap_format != 2 && p->snap_format != 3)
         die("SNAP_FORMAT", "snapshots are written as ASCII (2) or binary (3); SU (1) is not implemented, as in the reference");
     if (p->mode == 1 || p->mode == 2) {
         if (!st) die("workflow", "MODE 1 and 2 need a workflow file");
         if (st->lnorm != 2) die("LNORM", "only the L2 misfit (LNORM = 2) is built");
-        if (st->time_filt) die("TIME_FILT", "time-domain filtering is not built");
         if (st->eprecond) die("EPRECOND", "energy preconditioning is not built");
         if (st->spatfilter) die("SPATFILTER", "spatial gradient filters are not built");
         if (st->inv_stf || st->timewin || st->normalize || st->offset_mute || st->env || st->rowi)
@@ -120,6 +118,8 @@ typedef struct run {
     int *recpos;
     size_t ncell;
     const float *vp, *vs; /* the model as read: ppi, pu of the snapshot writer */
+    int time_filt, order; /* TIME_FILT of the current workflow stage: Butterworth ORDER, */
+    float fc, fc_start;   /* low-pass FC (= FC_END of the stage), high-pass FC_START */
     float hc[7];
     char name[4 * DNIO_STR];
 } run;
@@ -144,6 +144,11 @@ static void shot_setup(run *r, int ishot)
         for (int l = 0; l < nloc; l++) sp[q * nloc + l] = loc[q * count + l];
     float *sig = xmalloc(sizeof(float) * (size_t)(nloc ? nloc : 1) * P->nt);
     io_ok(dnio_wavelet(P, sp, nloc, ishot, r->TS, r->n_order, sig), "wavelet");
+    /* grad_obj_psv.c:140-156 (and FD_PSV.c:327-333, obj_psv.c:106-120): the
+     * band-limited spike, then the stage's time-domain filter */
+    if (P->quellart == 6)
+        io_ok(dnio_tdfilt(sig, nloc, P->nt, P->dt, P->order_spike, P->fc_spike_2, P->fc_spike_1), "spike filter");
+    if (r->time_filt) io_ok(dnio_tdfilt(sig, nloc, P->nt, P->dt, r->order, r->fc, r->fc_start), "TIME_FILT");
     dn_ok(dn_set_sources(r->ctx, nloc, sp, sig), "sources");
     int *rp = xmalloc(sizeof(int) * 2 * (size_t)(r->ntr ? r->ntr : 1));
     for (int k = 0; k < r->ntr; k++) {
@@ -178,6 +183,8 @@ static double shot_residuals(run *r, int ishot, int test)
             obs[c] = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
             io_ok(dnio_data_name(P, c, ishot, r->name, sizeof r->name), "file name");
             io_ok(dnio_read_su(r->name, r->ntr, P->nt, obs[c]), r->name);
+            /* calc_res_PSV.c:31-32 etc.: the observed data get the stage's filter */
+            if (r->time_filt) io_ok(dnio_tdfilt(obs[c], r->ntr, P->nt, P->dt, r->order, r->fc, r->fc_start), "TIME_FILT");
         }
     double l2 = 0.0;
     dn_ok(dn_calc_residuals(r->ctx, obs[0], obs[1], obs[2], test, &l2), "residuals");
@@ -201,6 +208,7 @@ static double shot_rtm_sources(run *r, int ishot, int test)
             diff[c] = xmalloc(sizeof(float) * n);
             io_ok(dnio_data_name(P, c, ishot, r->name, sizeof r->name), "file name");
             io_ok(dnio_read_su(r->name, r->ntr, P->nt, obs), r->name);
+            if (r->time_filt) io_ok(dnio_tdfilt(obs, r->ntr, P->nt, P->dt, r->order, r->fc, r->fc_start), "TIME_FILT");
             dn_ok(dn_get_seismogram(r->ctx, c, syn), "seismograms");
             io_ok(dnfwi_rtm_adjoint_source(obs, syn, r->ntr, P->nt, P->dt, P->grad_form, test, diff[c], &l2c),
                   "RTM adjoint source");
@@ -348,6 +356,11 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
                                P->rholowerlim, st.scalerho, st.inv_vp_iter, st.inv_vs_iter, st.inv_rho_iter,
                                P->invmat1};
         r->n_order = st.n_order;
+        /* FWI_PSV.c:413: FC = FC_END for the stage */
+        r->time_filt = st.time_filt;
+        r->order = st.order;
+        r->fc = st.fc_end;
+        r->fc_start = st.fc_start;
         for (int iter = 1; iter <= P->itermax; iter++) {
             printf("\n TDFWI ITERATION %d of %d (stage %d of %d)\n", iter, P->itermax, nstage, stagemax);
             fflush(stdout);
@@ -383,8 +396,13 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
             /* FWI_PSV.c:727-743: the misfit log */
             FILE *fl = fopen(P->misfit_log_file, iter_true == 1 ? "w" : "a");
             if (!fl) die("MISFIT_LOG_FILE", P->misfit_log_file);
-            fprintf(fl, "%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %d \n", opteps_vp, ls.epst[0], ls.epst[1],
-                    ls.epst[2], ls.L2t[0], ls.L2t[1], ls.L2t[2], L2sum, nstage);
+            if (st.time_filt == 0)
+                fprintf(fl, "%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %d \n", opteps_vp, ls.epst[0],
+                        ls.epst[1], ls.epst[2], ls.L2t[0], ls.L2t[1], ls.L2t[2], L2sum, nstage);
+            else
+                fprintf(fl, "%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %f \t %f \t %d \n", opteps_vp,
+                        ls.epst[0], ls.epst[1], ls.epst[2], ls.L2t[0], ls.L2t[1], ls.L2t[2], L2sum, st.fc_start,
+                        r->fc, nstage);
             fclose(fl);
             printf(" step length %e (trials %e %e %e, misfits %e %e %e)\n", opteps_vp, ls.epst[0], ls.epst[1],
                    ls.epst[2], ls.L2t[0], ls.L2t[1], ls.L2t[2]);
@@ -439,9 +457,13 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
 /* MODE 2: src/PSV/RTM_PSV.c:265-385 -- workflow stage 1, one gradient pass
  * with the data as adjoint sources, the stacked P and S images
  * (RTM_PSV_out.c) or, with RTM_SHOT = 1, the per-shot images */
-static void rtm(run *r)
+static void rtm(run *r, const dnio_stage *st)
 {
     dnio_par *P = r->P;
+    r->time_filt = st->time_filt; /* RTM_PSV.c:283: FC = FC_END */
+    r->order = st->order;
+    r->fc = st->fc_end;
+    r->fc_start = st->fc_start;
     float *g[3];
     for (int k = 0; k < 3; k++) g[k] = xmalloc(sizeof(float) * r->ncell);
     const double l2 = gradient_pass(r, g);
@@ -529,7 +551,7 @@ int main(int argc, char **argv)
     else if (P.mode == 1)
         fwi(&r, argv[2], vp, vs, rho, taus, taup);
     else
-        rtm(&r);
+        rtm(&r, st);
 
     dn_ctx_destroy(r.ctx);
     dnio_free(r.recpos);

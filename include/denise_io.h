@@ -175,6 +175,13 @@ int dnio_read_su(const char *path, int ntr, int ns, float *section);
  * for INVMAT1 = 1); hc: hc[0..6] (dn_get_hc). */
 int dnio_write_snap(const dnio_par *p, int nsnap, const float *vx, const float *vy, const float *sxx,
                     const float *syy, const float *ppi, const float *pu, const float *hc);
+/* Butterworth filtering of ntr traces of ns samples in place
+ * (src/apply_tdfilt.c, src/timedomain_filt.c, the vendored libcseife's
+ * seife_lpb / seife_hpb): low-pass with corner fc2 and, when fc1 > 0,
+ * high-pass with corner fc1, both of the given order; each pass filters in
+ * double and stores float.  Used for the spike source (QUELLART 6: ORDER_SPIKE,
+ * FC_SPIKE_2, FC_SPIKE_1) and for TIME_FILT stages (ORDER, FC_END, FC_START). */
+int dnio_tdfilt(float *data, int ntr, int ns, float dt, int order, float fc2, float fc1);
 int dnio_seis_name(const dnio_par *p, int comp, int ishot, int iter, char *out, int n);
 int dnio_data_name(const dnio_par *p, int comp, int ishot, char *out, int n);
 

@@ -16,6 +16,7 @@ import dnio
 import fwi_oracle as fo
 import inpfile
 import synth
+import tdfilt_oracle as tf
 from test_gpu_cli import _files, _reference_run, _run
 
 pytestmark = pytest.mark.gpu
@@ -23,7 +24,10 @@ pytestmark = pytest.mark.gpu
 ITERMAX = 4
 
 
-def test_cli_fwi_iterations_equal_python_iterations(tmp_path):
+@pytest.mark.parametrize("time_filt", [0, 1])
+def test_cli_fwi_iterations_equal_python_iterations(tmp_path, time_filt):
+    """time_filt = 1: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END
+    12 Hz, ORDER 4) filters the source signals and the observed data."""
     tmp = str(tmp_path)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
     bg = synth.layered_model(nx, ny, dh, seed=5, sigma=3.0)
@@ -40,10 +44,13 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path):
     inpfile.write_inp(os.path.join(tmp, "fwi.inp"), MODE=1, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
                       MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1", ITERMAX=ITERMAX, INV_MOD_OUT=1,
                       MISFIT_LOG_FILE="misfit.dat")
-    inpfile.write_workflow(os.path.join(tmp, "wf.inp"))
+    fcs, fce, order = (4.0, 12.0, 4) if time_filt else (0.0, 0.0, 6)
+    inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=time_filt, fc_start=fcs, fc_end=fce, order=order)
     _run(tmp, "fwi.inp", "wf.inp")
 
     par, p, m, rec, shots, src = _reference_run(tmp, "fwi.inp")
+    if time_filt:
+        shots = [(loc, tf.apply_tdfilt(sig, par.dt, order, fce, fcs)) for loc, sig in shots]
     st = dnio.read_stage(os.path.join(tmp, "wf.inp"), 1)
     upd = dict(vpupperlim=par.vpupperlim, vplowerlim=par.vplowerlim, vsupperlim=par.vsupperlim,
                vslowerlim=par.vslowerlim, rhoupperlim=par.rhoupperlim, rholowerlim=par.rholowerlim,
@@ -51,6 +58,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path):
                inv_rho_iter=st.inv_rho_iter, invmat1=par.invmat1)
     obs = {i: [dnio.read_su(os.path.join(tmp, f"obs/d_{c}.su.shot{i}"), rec.shape[1], nt) for c in "xy"]
            for i in (1, 2)}
+    if time_filt:
+        obs = {i: [tf.apply_tdfilt(o, par.dt, order, fce, fcs) for o in v] for i, v in obs.items()}
     g = dnhip.Context(p)
     vp, vs, rho = (a.copy() for a in m[:3])
 
@@ -90,8 +99,12 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path):
         eps = ls["eps_scale"]
         if it > par.min_iter and ls["step1"] == 0:
             eps = np.float32(0.0)
-        lines.append("%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %d " % (
-            eps, *ls["epst"], *ls["L2t"], l2sum, 1))
+        if time_filt:
+            lines.append("%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %f \t %f \t %d " % (
+                eps, *ls["epst"], *ls["L2t"], l2sum, st.fc_start, st.fc_end, 1))
+        else:
+            lines.append("%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %d " % (
+                eps, *ls["epst"], *ls["L2t"], l2sum, 1))
         vp, vs, rho, _ = fo.mat_change_test(upd, it, eps, vp, vs, rho, *d)
         print(f"[fwi] iteration {it}: misfit {l2sum:.6e}, step {float(eps):.4e}, "
               f"{ls['nevals']} test forward runs", flush=True)
@@ -166,3 +179,34 @@ def test_cli_rtm_images_equal_library_and_oracle(tmp_path):
         err = rel_l2(got, ototal[k])
         print(f"[rtm] {name} image rel-L2 vs FP32 oracle {err:.2e}", flush=True)
         assert err <= 1e-4, name
+
+
+def test_cli_spike_source_seismograms(tmp_path):
+    """QUELLART 6 (the Marmousi example's source): a unit spike band-limited
+    by the Butterworth filter (ORDER_SPIKE 5, FC_SPIKE_2 15 Hz, FC_SPIKE_1
+    2 Hz); the executable's seismograms equal the library's with the
+    oracle-filtered spike bit for bit."""
+    tmp = str(tmp_path)
+    nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
+    model = synth.layered_model(nx, ny, dh, seed=11, sigma=3.0)
+    sources = [(300.0, 30.0, 15.0, 1.0, 1), (650.0, 30.0, 15.0, 2.0, 1)]
+    receivers = [(40.0 + 20.0 * k, 20.0) for k in range(40)]
+    _files(tmp, nx, ny, dh, model, sources, receivers, mname="model/m")
+    with open(os.path.join(tmp, "source.dat"), "w") as f:    # tshift 0.05 s: spike at sample 51
+        f.write("2\n300.00 0.0 30.00 0.05 15.00 1.00 0.0 1\n650.00 0.0 30.00 0.05 15.00 2.00 0.0 1\n")
+    inpfile.write_inp(os.path.join(tmp, "spike.inp"), NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt, MFILE="model/m",
+                      QUELLART=6, FC_SPIKE_1=2.0, FC_SPIKE_2=15.0, ORDER_SPIKE=5)
+    _run(tmp, "spike.inp")
+    par, p, m, rec, shots, src = _reference_run(tmp, "spike.inp")
+    g = dnhip.Context(p)
+    g.set_model(*m)
+    for ishot, (loc, sig) in enumerate(shots, start=1):
+        assert sig.max() == 0.0 or np.count_nonzero(sig) <= sig.shape[0]   # raw spikes
+        g.set_sources(loc, tf.apply_tdfilt(sig, par.dt, 5, 15.0, 2.0))
+        g.set_receivers(rec[:2])
+        g.forward()
+        for c, name in ((0, "x"), (1, "y")):
+            got = dnio.read_su(os.path.join(tmp, f"su/out_{name}.su.shot{ishot}"), rec.shape[1], nt)
+            assert np.array_equal(got[:, 1:], g.seismogram(c)[:, 1:]), (ishot, name)
+            assert np.abs(got).max() > 0
+    g.close()
