@@ -873,6 +873,37 @@ int dn_get_store_hint(dn_ctx *c, int sweep, int *streaming)
     return DN_OK;
 }
 
+int dn_set_energy_weights(dn_ctx *c, int eprecond)
+{
+    if (!c || (eprecond != 0 && eprecond != 1 && eprecond != 3))
+        return fail(DN_ERR_ARG, "EPRECOND 0, 1 or 3 expected");
+    if (eprecond && c->p.mode <= 0) return fail(DN_ERR_STATE, "energy weights need params.mode > 0");
+    HIP_OK(hipSetDevice(c->device));
+    const size_t n = (size_t)c->p.nx * c->p.ny * sizeof(float);
+    for (int w = 0; w < 2 && eprecond; ++w)
+        if (!c->ew[w]) {
+            int rc = c->dalloc((void **)&c->ew[w], n);
+            if (rc) return rc;
+            HIP_OK(hipMemsetAsync(c->ew[w], 0, n, c->stream));
+        }
+    c->eprecond = eprecond;
+    return DN_OK;
+}
+
+int dn_get_energy(dn_ctx *c, int which, float *out)
+{
+    if (!c || !out || which < 0 || which > 1) return fail(DN_ERR_ARG, "bad energy request");
+    if (!c->ew[which]) return fail(DN_ERR_STATE, "energy weights are off (dn_set_energy_weights)");
+    HIP_OK(hipSetDevice(c->device));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    const int NX = c->p.nx, NY = c->p.ny;
+    std::vector<float> tmp((size_t)NX * NY);
+    HIP_OK(d2h(c->stream, tmp.data(), c->ew[which], tmp.size() * sizeof(float)));
+    for (int i = 0; i < NX; ++i)
+        for (int j = 0; j < NY; ++j) out[(size_t)i * NY + j] = tmp[(size_t)j * NX + i];
+    return DN_OK;
+}
+
 int dn_set_snapshots(dn_ctx *c, int first, int inc, int last, dn_snap_fn fn, void *user)
 {
     if (!c) return fail(DN_ERR_ARG, "null context");
@@ -909,6 +940,8 @@ int dn_run_forward(dn_ctx *c, int nsteps)
     ctl.seismo = c->p.seismo;
     ctl.probe = 1;
     size_t nev = 0;
+    if (c->eprecond)  // grad_obj_psv.c:71-76: the energies start at zero for each shot
+        for (int w = 0; w < 2; ++w) HIP_OK(hipMemsetAsync(c->ew[w], 0, (size_t)c->p.nx * c->p.ny * sizeof(float), c->stream));
     HIP_OK(hipEventRecord(c->ev0, c->stream));
     int hin = 1, hin1 = 1, lsnap = c->snap_first, nsnap = 0;
     std::vector<float> snapbuf;
@@ -936,6 +969,7 @@ int dn_run_forward(dn_ctx *c, int nsteps)
         }
         if (c->qtype == 5 && c->acq.nsrc > 0) launch_msource(c->a, c->acq, nt, c->stream);
         if (store && c->qtype == 5) launch_store(c->a, c->fwd, hin - 1, c->stream);
+        if (store && c->eprecond) launch_energy(c->a, c->ew[0], c->stream);  // psv.c:300-303
         if (store) {
             ++hin;
             hin1 += c->p.dtinv;
@@ -1063,6 +1097,7 @@ int dn_run_adjoint(dn_ctx *c, int nsteps)
             HIP_OK(hipEventRecord(c->pool[nev + 3], c->stream));
             nev += 4;
         }
+        if (corr && c->eprecond == 1) launch_energy(c->a, c->ew[1], c->stream);  // psv.c:380-383
         if (corr) ++hin;
     }
     HIP_OK(hipGetLastError());

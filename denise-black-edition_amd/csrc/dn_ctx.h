@@ -84,6 +84,11 @@ struct dn_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double run_ms = 0.0, v_ms = 0.0, s_ms = 0.0;
     int sample_every = 0, samples = 0;
+    // energy preconditioning (dn_set_energy_weights): sum of vx^2+vy^2 of the
+    // forward field at its store steps (0) and of the adjoint field at its
+    // correlation steps (1), NX x NY in the gradient accumulators' layout
+    int eprecond = 0;
+    float *ew[2] = {nullptr, nullptr};
     // wavefield snapshots of dn_run_forward (dn_set_snapshots)
     dn_snap_fn snap_fn = nullptr;
     void *snap_user = nullptr;

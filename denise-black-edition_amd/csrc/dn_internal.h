@@ -213,6 +213,7 @@ void launch_vel(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t
                 void *stream);
 void launch_str(const KArgs &a, const StepCtl &c, const DevAcq &q, const Tile &t, int fdoh, int ntl, void *stream);
 void launch_msource(const KArgs &a, const DevAcq &q, int nt, void *stream);
+void launch_energy(const KArgs &a, float *W, void *stream);
 void launch_store(const KArgs &a, float *const fwd[5], long k, void *stream);
 void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const float *wcr, float *gvp,
                      float *gvs, float *grho, int invmat1, void *stream);

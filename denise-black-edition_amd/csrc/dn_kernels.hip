@@ -64,6 +64,18 @@ __global__ __launch_bounds__(256) void k_store(KArgs a, float *f0, float *f1, fl
     }
 }
 
+// src/eprecond.c:9-18: W += vx*vx + vy*vy over the grid
+__global__ __launch_bounds__(256) void k_energy(KArgs a, float *W)
+{
+    const long n = (long)a.nx * a.ny;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)(t % a.nx) + 1, j = (int)(t / a.nx) + 1;
+    const long o = (long)(j - 1) * a.pitch + (i - 1);
+    const float vx = a.vx[o], vy = a.vy[o];
+    W[t] = W[t] + ((vx * vx) + (vy * vy));
+}
+
 // src/PSV/ass_gradPSV.c:10-155 (gradients stored x-major like the files)
 __global__ __launch_bounds__(256) void k_assemble(KArgs a, const float *wc, const float *wcu,
                                                   const float *wcr, float *gvp, float *gvs, float *grho,
@@ -271,6 +283,12 @@ void launch_store(const KArgs &a, float *const fwd[5], long k, void *stream)
     const long n = (long)a.nx * a.ny;
     hipLaunchKernelGGL(k_store, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a, fwd[0],
                        fwd[1], fwd[2], fwd[3], fwd[4], k);
+}
+
+void launch_energy(const KArgs &a, float *W, void *stream)
+{
+    const long n = (long)a.nx * a.ny;
+    hipLaunchKernelGGL(k_energy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a, W);
 }
 
 void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const float *wcr, float *gvp,

@@ -52,6 +52,9 @@ def load():
         "dnfwi_opt_step": ([C.POINTER(C.c_double), fp], C.c_float),
         "dnfwi_step_length": ([C.POINTER(UpdatePar), C.c_int, C.c_float, C.c_float, C.c_int, C.c_double, C.c_int,
                                C.c_int, fp, fp, fp, fp, fp, fp, _MISFIT, vp_, C.POINTER(LineSearch)], C.c_int),
+        "dnfwi_energy_precond": ([C.c_int, C.c_int, C.c_int, C.c_float, fp, fp, C.c_float, C.c_float, C.c_float,
+                                  fp, fp, fp, fp], C.c_int),
+        "dnfwi_taper_hor": ([C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, fp, fp], C.c_int),
         "dnfwi_rtm_adjoint_source": ([fp, fp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, fp,
                                       C.POINTER(C.c_double)], C.c_int),
     }
@@ -171,3 +174,21 @@ def rtm_adjoint_source(data, syn, dt, grad_form=1, testshot=1):
     dnio._check(load().dnfwi_rtm_adjoint_source(_p(d), _p(sy), d.shape[0], d.shape[1], dt, grad_form, testshot,
                                                  _p(out), C.byref(l2)))
     return out, l2.value
+
+
+def energy_precond(eprecond, dh, Ws, Wr, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho):
+    """(We, g_vp, g_vs, g_rho) after dnfwi_energy_precond (new arrays)."""
+    ws = _f(Ws)
+    wr = _f(Wr) if Wr is not None else np.zeros_like(ws)
+    g = [_f(a).copy() for a in (g_vp, g_vs, g_rho)]
+    We = np.empty_like(ws)
+    nx, ny = ws.shape
+    dnio._check(load().dnfwi_energy_precond(eprecond, nx, ny, dh, _p(ws), _p(wr), c_vp, c_vs, c_rho,
+                                            *[_p(a) for a in g], _p(We)))
+    return We, g[0], g[1], g[2]
+
+
+def taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper):
+    coeff = np.empty((nx, ny), np.float32)
+    dnio._check(load().dnfwi_taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper, _p(coeff), None))
+    return coeff

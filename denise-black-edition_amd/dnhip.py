@@ -123,6 +123,9 @@ def load():
         "dn_get_pml": ([vp, C.c_int, fp], C.c_int),
         "dn_get_hc": ([vp, fp], C.c_int),
         "dn_set_kernel_sampling": ([vp, C.c_int], C.c_int),
+        "dn_set_energy_weights": ([vp, C.c_int], C.c_int),
+        "dn_get_energy": ([vp, C.c_int, fp], C.c_int),
+        "dn_set_snapshots": ([vp, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p], C.c_int),
         "dn_set_tile": ([vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int], C.c_int),
         "dn_get_tile": ([vp, C.c_int, ip, ip, ip, ip], C.c_int),
         "dn_autotune_tile": ([vp, C.c_int, ip, ip], C.c_int),
@@ -368,6 +371,15 @@ class Context:
         v = C.c_int()
         _check(load().dn_get_store_hint(self.h, sweep, C.byref(v)))
         return v.value
+
+    def set_energy_weights(self, eprecond):
+        """EPRECOND 1 or 3: sum vx^2+vy^2 of the forward (and adjoint) field."""
+        _check(load().dn_set_energy_weights(self.h, eprecond))
+
+    def energy(self, which=0):
+        out = np.zeros((self.p.nx, self.p.ny), np.float32)
+        _check(load().dn_get_energy(self.h, which, _ptr(out)))
+        return out
 
     def set_kernel_sampling(self, every):
         _check(load().dn_set_kernel_sampling(self.h, every))

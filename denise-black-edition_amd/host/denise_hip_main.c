@@ -83,15 +83,16 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
     if (p->mode == 1 || p->mode == 2) {
         if (!st) die("workflow", "MODE 1 and 2 need a workflow file");
         if (st->lnorm != 2) die("LNORM", "only the L2 misfit (LNORM = 2) is built");
-        if (st->eprecond) die("EPRECOND", "energy preconditioning is not built");
+        if (st->eprecond != 0 && st->eprecond != 1 && st->eprecond != 3)
+            die("EPRECOND", "energy preconditioning 1 and 3 are built");
         if (st->spatfilter) die("SPATFILTER", "spatial gradient filters are not built");
         if (st->inv_stf || st->timewin || st->normalize || st->offset_mute || st->env || st->rowi)
             die("workflow", "STF inversion, time windows, normalisation, offset mutes, envelope and ROWI are not built");
         if (p->ndt != 1) die("NDT", "FWI needs NDT = 1");
         if (p->idx != 1 || p->idy != 1) die("IDX/IDY", "FWI runs on every grid point (IDX = IDY = 1)");
         if (p->sws_taper_circular_per_shot) die("SWS_TAPER_CIRCULAR_PER_SHOT", "per-shot gradient tapers are not built");
-        if (p->sws_taper_grad_vert || p->sws_taper_grad_hor || p->sws_taper_grad_sources || p->sws_taper_file)
-            die("SWS_TAPER_GRAD_*", "gradient tapers are not built");
+        if (p->sws_taper_grad_vert || p->sws_taper_grad_sources || p->sws_taper_file)
+            die("SWS_TAPER_GRAD_*", "of the gradient tapers the horizontal one (SWS_TAPER_GRAD_HOR) is built");
         if (p->grad_form != 1 && p->grad_form != 2) die("GRAD_FORM", "1 or 2 expected");
         if (p->trkill) die("TRKILL", "trace killing is not built");
     }
@@ -117,7 +118,9 @@ typedef struct run {
     float *srcpos, TS;
     int *recpos;
     size_t ncell;
-    const float *vp, *vs; /* the model as read: ppi, pu of the snapshot writer */
+    const float *vp, *vs, *rho; /* the model as read (ppi, pu of the snapshot writer) */
+    int eprecond;         /* EPRECOND of the current workflow stage */
+    float c_vp, c_vs, c_rho; /* the initial model's averages (FWI_PSV.c:540-580) */
     int time_filt, order; /* TIME_FILT of the current workflow stage: Butterworth ORDER, */
     float fc, fc_start;   /* low-pass FC (= FC_END of the stage), high-pass FC_START */
     float hc[7];
@@ -274,7 +277,13 @@ static void forward_pass(run *r)
 static double gradient_pass(run *r, float **g_sum)
 {
     dnio_par *P = r->P;
-    float *g_shot = xmalloc(sizeof(float) * r->ncell);
+    float *gs[3], *ws = NULL, *wr = NULL;
+    for (int k = 0; k < 3; k++) gs[k] = xmalloc(sizeof(float) * r->ncell);
+    if (r->eprecond) {
+        ws = xmalloc(sizeof(float) * r->ncell);
+        wr = xmalloc(sizeof(float) * r->ncell);
+    }
+    dn_ok(dn_set_energy_weights(r->ctx, r->eprecond), "energy weights");
     for (int k = 0; k < 3; k++) memset(g_sum[k], 0, sizeof(float) * r->ncell);
     double misfit = 0.0;
     for (int ishot = 1; ishot <= r->nshots; ishot++) {
@@ -285,18 +294,45 @@ static double gradient_pass(run *r, float **g_sum)
         misfit += l2;
         dn_ok(dn_run_adjoint(r->ctx, 0), "adjoint run");
         dn_ok(dn_assemble_gradient(r->ctx), "gradient");
+        for (int k = 0; k < 3; k++) dn_ok(dn_get_gradient(r->ctx, 3 + k, gs[k]), "gradient");
+        if (r->eprecond) {
+            /* grad_obj_psv.c:219-238: energy weights of this shot */
+            dn_ok(dn_get_energy(r->ctx, 0, ws), "energy");
+            if (r->eprecond == 1) dn_ok(dn_get_energy(r->ctx, 1, wr), "energy");
+            io_ok(dnfwi_energy_precond(r->eprecond, P->nx, P->ny, P->dh, ws, wr, r->c_vp, r->c_vs, r->c_rho, gs[0],
+                                       gs[1], gs[2], NULL),
+                  "energy preconditioning");
+        }
         for (int k = 0; k < 3; k++) {
-            dn_ok(dn_get_gradient(r->ctx, 3 + k, g_shot), "gradient");
-            for (size_t q = 0; q < r->ncell; q++) g_sum[k][q] += g_shot[q];
-            if (P->rtm_shot && k < 2) {
+            for (size_t q = 0; q < r->ncell; q++) g_sum[k][q] += gs[k][q];
+            if (P->rtm_shot && k < 2) { /* grad_obj_psv.c:263-266 */
                 snprintf(r->name, sizeof r->name, "%s_%s_image_shot_%i", P->jacobian, k == 0 ? "P" : "S", ishot);
-                write_plane(r->name, g_shot, P->nx, P->ny);
+                write_plane(r->name, gs[k], P->nx, P->ny);
             }
         }
         printf("   L2 of shot %d: %.9e%s\n", ishot, l2, test ? "" : " (not a test shot)");
     }
-    free(g_shot);
+    for (int k = 0; k < 3; k++) free(gs[k]);
+    free(ws);
+    free(wr);
     return misfit;
+}
+
+/* precond_PSV (src/PSV/precond_PSV.c:13-60) with the horizontal taper only:
+ * the stacked vp, vs and rho gradients times the taper, which is written to
+ * TFILE_coeff_hor.bin (taper_grad.c:312-316) */
+static void precondition(run *r, float **g)
+{
+    dnio_par *P = r->P;
+    if (!P->sws_taper_grad_hor) return;
+    float *coeff = xmalloc(sizeof(float) * r->ncell);
+    io_ok(dnfwi_taper_hor(P->nx, P->ny, P->dh, P->gradt[0], P->gradt[1], P->exp_taper_grad_hor, coeff, NULL),
+          "gradient taper");
+    for (int k = 0; k < 3; k++)
+        for (size_t q = 0; q < r->ncell; q++) g[k][q] *= coeff[q];
+    snprintf(r->name, sizeof r->name, "%s_coeff_hor.bin", P->tfile);
+    write_plane(r->name, coeff, P->nx, P->ny);
+    free(coeff);
 }
 
 /* obj_psv: forward runs of the test shots on the current model, L2 sum */
@@ -361,6 +397,7 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
         r->order = st.order;
         r->fc = st.fc_end;
         r->fc_start = st.fc_start;
+        r->eprecond = st.eprecond;
         for (int iter = 1; iter <= P->itermax; iter++) {
             printf("\n TDFWI ITERATION %d of %d (stage %d of %d)\n", iter, P->itermax, nstage, stagemax);
             fflush(stdout);
@@ -370,9 +407,13 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
                 c_vs = dnfwi_model_mean(vs, nx, ny);
                 c_rho = dnfwi_model_mean(rho, nx, ny);
                 printf(" Vp_avg = %.0f \t Vs_avg = %.0f \t rho_avg = %.0f\n", c_vp, c_vs, c_rho);
+                r->c_vp = c_vp;
+                r->c_vs = c_vs;
+                r->c_rho = c_rho;
             }
             dn_ok(dn_set_model(r->ctx, vp, vs, rho, taus, taup), "model upload");
             const double L2sum = gradient_pass(r, g);
+            precondition(r, g); /* FWI_PSV.c:628-634: smooth_grad (off) and precond_PSV */
             const char *suffix[3] = {"vp", "vs", "rho"};
             for (int k = 0; k < 3; k++) {
                 snprintf(r->name, sizeof r->name, "%s_grad_%s", P->jacobian, suffix[k]);
@@ -460,6 +501,10 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
 static void rtm(run *r, const dnio_stage *st)
 {
     dnio_par *P = r->P;
+    r->eprecond = st->eprecond;
+    r->c_vp = dnfwi_model_mean(r->vp, P->nx, P->ny); /* RTM_PSV.c:316-352 */
+    r->c_vs = dnfwi_model_mean(r->vs, P->nx, P->ny);
+    r->c_rho = dnfwi_model_mean(r->rho, P->nx, P->ny);
     r->time_filt = st->time_filt; /* RTM_PSV.c:283: FC = FC_END */
     r->order = st->order;
     r->fc = st->fc_end;
@@ -467,6 +512,7 @@ static void rtm(run *r, const dnio_stage *st)
     float *g[3];
     for (int k = 0; k < 3; k++) g[k] = xmalloc(sizeof(float) * r->ncell);
     const double l2 = gradient_pass(r, g);
+    precondition(r, g); /* RTM_PSV.c:381 */
     printf(" L2 of the data (test shots): %.9e\n", l2);
     if (P->rtm_shot == 0) {
         const char *img[2] = {"P", "S"};
@@ -513,6 +559,7 @@ int main(int argc, char **argv)
     io_ok(dnio_read_model(&P, vp, vs, rho, taus, taup), "model");
     r.vp = vp;
     r.vs = vs;
+    r.rho = rho;
 
     dn_params dp;
     memset(&dp, 0, sizeof dp);

@@ -376,3 +376,66 @@ int dnfwi_rtm_adjoint_source(const float *data, const float *syn, int ntr, int n
     if (l2) *l2 = L2;
     return 0;
 }
+
+int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws, const float *Wr, float c_vp,
+                         float c_vs, float c_rho, float *g_vp, float *g_vs, float *g_rho, float *We)
+{
+    if ((eprecond != 1 && eprecond != 3) || !Ws || (eprecond == 1 && !Wr) || !g_vp || !g_vs || !g_rho || nx < 1 ||
+        ny < 1)
+        return dnio_fail_msg("dnfwi_energy_precond: bad argument");
+    const size_t n = (size_t)nx * ny;
+    float *w = We ? We : malloc(sizeof(float) * n);
+    if (!w) return dnio_fail_msg("dnfwi_energy_precond: out of host memory");
+    /* eprecond1.c:13-46 */
+    const float xmin = dh, xmax = nx * dh, eps_we = 0.005f;
+    float maxwe = 0.0f;
+    for (int i = 1; i <= nx; i++)
+        for (int j = 1; j <= ny; j++) {
+            const size_t k = (size_t)(i - 1) * ny + (j - 1);
+            if (eprecond == 1) {
+                w[k] = sqrt(Ws[k] * Wr[k]);
+            } else {
+                const float x = i * dh, y = j * dh;
+                w[k] = sqrt(Ws[k]) * (asinh((xmax - x) / y) - asinh((xmin - x) / y));
+            }
+            if (w[k] > maxwe) maxwe = w[k];
+        }
+    /* eprecond1.c:50-58 */
+    for (size_t k = 0; k < n; k++) w[k] = w[k] + (eps_we * maxwe);
+    /* grad_obj_psv.c:224-237 */
+    for (size_t k = 0; k < n; k++) {
+        g_vp[k] = g_vp[k] / (w[k] * c_vp * c_vp);
+        g_vs[k] = g_vs[k] / (w[k] * c_vs * c_vs);
+        if (c_vs == 0.0) g_vs[k] = 0.0f;
+        g_rho[k] = g_rho[k] / (w[k] * c_rho * c_rho);
+    }
+    if (!We) free(w);
+    return 0;
+}
+
+int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_taper, float *coeff, float *g)
+{
+    if (!coeff || nx < 1 || ny < 1) return dnio_fail_msg("dnfwi_taper_hor: bad argument");
+    const int taperlength = gradt2 - gradt1, ifw = gradt2 - gradt1 + 1;
+    float *window = malloc(sizeof(float) * (size_t)(ifw > 0 ? ifw : 1));
+    if (!window) return dnio_fail_msg("dnfwi_taper_hor: out of host memory");
+    const float a = 3; /* damping coefficient */
+    for (int i = 1; i <= ifw; i++) {
+        window[i - 1] = 1.0f;
+        if (i <= taperlength)
+            window[i - 1] = exp(-(1.0 / 2.0) * (a * (i - taperlength) / (taperlength / 2.0)) *
+                                (a * (i - taperlength) / (taperlength / 2.0)));
+    }
+    int h = 1;
+    for (int j = 1; j <= ny; j++) {
+        float grad_tap = 0.0f;
+        if ((j >= gradt1) && (j <= gradt2)) grad_tap = window[h - 1];
+        if (j > gradt2) grad_tap = pow((float)(j * dh), exp_taper);
+        for (int i = 1; i <= nx; i++) coeff[(size_t)(i - 1) * ny + (j - 1)] = grad_tap;
+        if (j >= gradt1) h++;
+    }
+    free(window);
+    if (g)
+        for (size_t k = 0; k < (size_t)nx * ny; k++) g[k] *= coeff[k];
+    return 0;
+}

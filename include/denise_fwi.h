@@ -101,6 +101,25 @@ int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, flo
                       const float *d_vp, const float *d_vs, const float *d_rho, dnfwi_misfit_fn misfit,
                       void *user, dnfwi_linesearch *out);
 
+/* Energy preconditioning of one shot's gradients (src/eprecond1.c:9-60 and
+ * src/PSV/grad_obj_psv.c:219-238): the weights We = sqrt(Ws*Wr)
+ * (EPRECOND 1) or sqrt(Ws) * (asinh((xmax-x)/y) - asinh((xmin-x)/y)) with
+ * x = i*DH, y = j*DH, xmin = DH, xmax = NX*DH (EPRECOND 3, the forward field
+ * and an approximate receiver Green's function, Plessix and Mulder 2004),
+ * regularised by 0.005 * max(We); then g_vp /= We*C_vp*C_vp, g_vs /=
+ * We*C_vs*C_vs (0 when C_vs = 0), g_rho /= We*C_rho*C_rho.  Ws, Wr, g_*:
+ * x-major NX x NY (Wr only for EPRECOND 1); We (may be NULL) gets the
+ * weights. */
+int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws, const float *Wr, float c_vp,
+                         float c_vs, float c_rho, float *g_vp, float *g_vs, float *g_rho, float *We);
+
+/* Horizontal gradient taper (src/taper_grad.c:188-322, SWS_TAPER_GRAD_HOR):
+ * 0 above depth index GRADT1, a Gaussian ramp exp(-(3 (k - L)/(L/2))^2 / 2),
+ * k = 1..L+1, L = GRADT2 - GRADT1, from GRADT1 to GRADT2, and
+ * (j*DH)^EXP_TAPER_GRAD_HOR below GRADT2; coeff (NX x NY, x-major) gets the
+ * taper, and g (may be NULL) is multiplied by it. */
+int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_taper, float *coeff, float *g);
+
 /* Adjoint source of reverse-time migration (MODE 2), the MODE 2 branch of
  * calc_res with LNORM = 2 (src/calc_res.c:416-446): per trace the first data
  * sample is replaced by the synthetic one, then GRAD_FORM 1 integrates

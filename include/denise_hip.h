@@ -199,6 +199,16 @@ int dn_autotune_tile(dn_ctx *ctx, int nsteps, int *vel_shape, int *str_shape);
  * sampling off; every = k times one launch of each in k steps. */
 int dn_set_kernel_sampling(dn_ctx *ctx, int every);
 
+/* Energy preconditioning (EPRECOND, src/eprecond.c, psv.c:300-303, 380-383):
+ * with eprecond 1 or 3 the runs sum vx^2 + vy^2 over the grid, from zero at
+ * each forward run: which 0 (Ws) the forward field at its snapshot-store
+ * steps, which 1 (Wr, eprecond 1 only) the adjoint field at its
+ * correlation steps.  dn_get_energy returns NX x NY, x-major.  The weights
+ * (src/eprecond1.c) and the gradient scaling are the host's
+ * (dnfwi_energy_precond).  0 switches it off.  Not taken by the engine. */
+int dn_set_energy_weights(dn_ctx *ctx, int eprecond);
+int dn_get_energy(dn_ctx *ctx, int which, float *out);
+
 /* Wavefield snapshots during dn_run_forward (src/PSV/psv.c:75-77, 225-231):
  * after time step `first` and every `inc` steps after it, up to `last`,
  * the run waits for the device and calls fn with the step, the snapshot

@@ -15,7 +15,10 @@ Written from the reference's functions, one per function, keeping its
   - opt_step: src/calc_opt_step.c:14-84 with src/solvelin.c:19-48;
   - step_length: src/PSV/step_length_est_psv.c:37-182;
   - rtm_adjoint_source: the MODE 2 branch of src/calc_res.c:416-446 and
-    its L2 sum, src/calc_res.c:563-569.
+    its L2 sum, src/calc_res.c:563-569;
+  - energy_precond: src/eprecond1.c:9-60 with the gradient scaling of
+    src/PSV/grad_obj_psv.c:219-238;
+  - taper_hor: src/taper_grad.c:188-322 (SWS_TAPER_GRAD_HOR).
 Vectors are numpy float32 arrays of shape (nx, ny) (the model-file order).
 The pins of these functions against what the mathematics fixes (BFGS
 matrix recursion, parabola vertex, limits) are in tests/test_fwi_host.py.
@@ -259,3 +262,59 @@ def rtm_adjoint_source(data, syn, dt, grad_form=1, testshot=1):
             for invtime in range(ns, 0, -1):
                 L2 += 0.5 * float(diff[i, invtime - 1]) * float(diff[i, invtime - 1])
     return diff, L2
+
+
+def energy_precond(eprecond, dh, Ws, Wr, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho):
+    """Returns (We, g_vp, g_vs, g_rho) as new float32 arrays; (nx, ny) inputs."""
+    import math
+    nx, ny = Ws.shape
+    dh = F32(dh)
+    xmin, xmax = dh, F32(F32(nx) * dh)
+    We = np.zeros((nx, ny), F32)
+    maxwe = F32(0.0)
+    for i in range(1, nx + 1):
+        for j in range(1, ny + 1):
+            if eprecond == 1:
+                w = F32(math.sqrt(float(F32(Ws[i - 1, j - 1] * Wr[i - 1, j - 1]))))
+            else:
+                x, y = F32(F32(i) * dh), F32(F32(j) * dh)
+                w = F32(math.sqrt(float(Ws[i - 1, j - 1])) *
+                        (math.asinh(float(F32((xmax - x) / y))) - math.asinh(float(F32((xmin - x) / y)))))
+            We[i - 1, j - 1] = w
+            if w > maxwe:
+                maxwe = w
+    We = (We + F32(F32(0.005) * maxwe)).astype(F32)
+    c_vp, c_vs, c_rho = F32(c_vp), F32(c_vs), F32(c_rho)
+    gv = (g_vp / ((We * c_vp).astype(F32) * c_vp).astype(F32)).astype(F32)
+    gs = (g_vs / ((We * c_vs).astype(F32) * c_vs).astype(F32)).astype(F32)
+    if c_vs == 0.0:
+        gs[...] = 0.0
+    gr = (g_rho / ((We * c_rho).astype(F32) * c_rho).astype(F32)).astype(F32)
+    return We, gv, gs, gr
+
+
+def taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper):
+    """The taper coefficients, (nx, ny) float32."""
+    import math
+    L = gradt2 - gradt1
+    ifw = L + 1
+    a = F32(3.0)
+    window = []
+    for i in range(1, ifw + 1):
+        w = F32(1.0)
+        if i <= L:
+            q = float(F32(a * F32(i - L))) / (L / 2.0)
+            w = F32(math.exp(-(1.0 / 2.0) * q * q))
+        window.append(w)
+    coeff = np.zeros((nx, ny), F32)
+    h = 1
+    for j in range(1, ny + 1):
+        t = F32(0.0)
+        if gradt1 <= j <= gradt2:
+            t = window[h - 1]
+        if j > gradt2:
+            t = F32(math.pow(float(F32(F32(j) * F32(dh))), float(F32(exp_taper))))
+        coeff[:, j - 1] = t
+        if j >= gradt1:
+            h += 1
+    return coeff

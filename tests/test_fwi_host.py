@@ -288,3 +288,54 @@ def test_rtm_adjoint_source():
     assert np.allclose(got1[:, ::-1] - got[:, ::-1], 4.0 * 1e-3, rtol=1e-3)
     got2, _ = dnfwi.rtm_adjoint_source(data, syn, 1e-3, 2, 1)
     assert np.array_equal(got2[:, ::-1][:, 1:], data[:, 1:]) and np.array_equal(got2[:, -1], syn[:, 0])
+
+
+@pytest.mark.parametrize("eprecond", [1, 3])
+def test_energy_precond(eprecond):
+    """dnfwi_energy_precond equals the restatement bit for bit; pins: for
+    EPRECOND 3 the asinh factor is the integral over the receiver line
+    x' in [DH, NX*DH] of 1/sqrt((x'-x)^2 + y^2) (quadrature); for EPRECOND 1
+    on constant energies We = sqrt(Ws*Wr) * 1.005; the gradients are divided
+    by We*C^2."""
+    from scipy.integrate import quad
+    rng = np.random.default_rng(eprecond)
+    nx, ny, dh = 13, 9, 10.0
+    Ws = rng.uniform(0.5, 2.0, (nx, ny)).astype(F32)
+    Wr = rng.uniform(0.5, 2.0, (nx, ny)).astype(F32)
+    g = [rnd(rng, shape=(nx, ny)) for _ in range(3)]
+    got = dnfwi.energy_precond(eprecond, dh, Ws, Wr, 2500.0, 1400.0, 2100.0, *g)
+    ref = fo.energy_precond(eprecond, dh, Ws, Wr if eprecond == 1 else None, 2500.0, 1400.0, 2100.0, *g)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    We = got[0]
+    if eprecond == 3:
+        raw = (We - F32(0.005) * (We.max() / 1.005)) / np.sqrt(Ws.astype(np.float64))
+        for i, j in ((1, 1), (6, 4), (13, 9)):
+            x, y = i * dh, j * dh
+            val, _ = quad(lambda xp: 1.0 / np.sqrt((xp - x) ** 2 + y ** 2), dh, nx * dh)
+            assert abs(raw[i - 1, j - 1] - val) < 1e-4 * val
+    else:
+        c = dnfwi.energy_precond(1, dh, np.full((nx, ny), 4.0, F32), np.full((nx, ny), 9.0, F32), 1.0, 1.0, 1.0,
+                                 *g)[0]
+        assert np.allclose(c, 6.0 * 1.005, rtol=1e-6)
+    assert np.allclose(got[1], g[0] / (We.astype(np.float64) * 2500.0 ** 2), rtol=1e-5)
+    _, _, gs0, _ = dnfwi.energy_precond(eprecond, dh, Ws, Wr, 2500.0, 0.0, 2100.0, *g)
+    assert np.all(gs0 == 0.0)
+
+
+def test_taper_hor():
+    """dnfwi_taper_hor equals the restatement bit for bit; pins: 0 above
+    GRADT1, 1 at GRADT2, the Gaussian ramp exp(-(3(k-L)/(L/2))^2/2) in between,
+    (j*DH)^EXP below."""
+    nx, ny, dh, g1, g2, ex = 7, 40, 10.0, 10, 20, 0.5
+    got = dnfwi.taper_hor(nx, ny, dh, g1, g2, ex)
+    assert np.array_equal(got, fo.taper_hor(nx, ny, dh, g1, g2, ex))
+    L = g2 - g1
+    assert np.all(got[:, :g1 - 1] == 0.0)
+    assert np.all(got[:, g2 - 1] == 1.0)
+    for j in range(g1, g2 + 1):
+        k = j - g1 + 1
+        want = np.exp(-0.5 * (3 * (k - L) / (L / 2)) ** 2) if k <= L else 1.0
+        assert np.allclose(got[:, j - 1], want, rtol=1e-6, atol=1e-12)
+    for j in range(g2 + 1, ny + 1):
+        assert np.allclose(got[:, j - 1], (j * dh) ** ex, rtol=1e-6)

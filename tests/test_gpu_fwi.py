@@ -24,11 +24,17 @@ pytestmark = pytest.mark.gpu
 ITERMAX = 4
 
 
-@pytest.mark.parametrize("time_filt", [0, 1])
-def test_cli_fwi_iterations_equal_python_iterations(tmp_path, time_filt):
-    """time_filt = 1: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END
-    12 Hz, ORDER 4) filters the source signals and the observed data."""
+@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond"])
+def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
+    """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
+    ORDER 4) filters the source signals and the observed data.  precond:
+    energy preconditioning EPRECOND 3 of each shot's gradients (device
+    energy sums, host weights) and the horizontal gradient taper
+    (SWS_TAPER_GRAD_HOR, GRADT 3..12, exponent 2) on the stacked ones."""
+    time_filt = int(variant == "time_filt")
+    precond = variant == "precond"
     tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
     bg = synth.layered_model(nx, ny, dh, seed=5, sigma=3.0)
     true = tuple(a.copy() for a in bg)
@@ -43,9 +49,11 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, time_filt):
     _files(tmp, nx, ny, dh, bg, sources, receivers, mname="model/start")
     inpfile.write_inp(os.path.join(tmp, "fwi.inp"), MODE=1, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
                       MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1", ITERMAX=ITERMAX, INV_MOD_OUT=1,
-                      MISFIT_LOG_FILE="misfit.dat")
+                      MISFIT_LOG_FILE="misfit.dat", SWS_TAPER_GRAD_HOR=int(precond), GRADT="3, 12, 60, 62",
+                      EXP_TAPER_GRAD_HOR=2.0)
     fcs, fce, order = (4.0, 12.0, 4) if time_filt else (0.0, 0.0, 6)
-    inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=time_filt, fc_start=fcs, fc_end=fce, order=order)
+    inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=time_filt, fc_start=fcs, fc_end=fce, order=order,
+                           eprecond=3 if precond else 0)
     _run(tmp, "fwi.inp", "wf.inp")
 
     par, p, m, rec, shots, src = _reference_run(tmp, "fwi.inp")
@@ -61,6 +69,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, time_filt):
     if time_filt:
         obs = {i: [tf.apply_tdfilt(o, par.dt, order, fce, fcs) for o in v] for i, v in obs.items()}
     g = dnhip.Context(p)
+    if precond:
+        g.set_energy_weights(3)
     vp, vs, rho = (a.copy() for a in m[:3])
 
     def shot(i, model):
@@ -89,8 +99,16 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, time_filt):
             l2sum += g.residuals(obs[i][0], obs[i][1], None, 1)
             g.adjoint()
             g.assemble()
+            gs = [g.gradient(3 + k) for k in range(3)]
+            if precond:
+                gs = list(fo.energy_precond(3, par.dh, g.energy(0), None, *c, *gs)[1:])
             for k in range(3):
-                grads[k] += g.gradient(3 + k)
+                grads[k] += gs[k]
+        if precond:
+            coeff = fo.taper_hor(nx, ny, par.dh, 3, 12, 2.0)
+            grads = [(a * coeff).astype(np.float32) for a in grads]
+            got = np.fromfile(os.path.join(tmp, "taper/t_coeff_hor.bin"), np.float32).reshape(nx, ny)
+            assert np.array_equal(got, coeff)
         l2s.append(l2sum)
         d = lb.direction(it, *c, *grads, vp, vs, rho)
         ls = fo.step_length(upd, it, par.eps_scale, par.scalefac, par.stepmax, l2sum, vp, vs, rho, *d, misfit,
@@ -209,4 +227,47 @@ def test_cli_spike_source_seismograms(tmp_path):
             got = dnio.read_su(os.path.join(tmp, f"su/out_{name}.su.shot{ishot}"), rec.shape[1], nt)
             assert np.array_equal(got[:, 1:], g.seismogram(c)[:, 1:]), (ishot, name)
             assert np.abs(got).max() > 0
+    g.close()
+
+
+@pytest.mark.parametrize("eprecond", [1, 3])
+def test_energy_weights_equal_field_sums(eprecond):
+    """dn_set_energy_weights: Ws is the float sum of vx^2 + vy^2 of the
+    forward field after each snapshot-store step (1, 1+DTINV, ...), Wr (with
+    EPRECOND 1) that of the adjoint field after each correlation step --
+    equal to the same sums over fields taken from shorter runs."""
+    from cases import make_case
+    nx, ny, nt = 96, 64, 60
+    p, m, src, sig, rec = make_case(nx, ny, nt, mode=1, dtinv=2)
+    g = dnhip.Context(p)
+    g.set_model(m["vp"], m["vs"], m["rho"])
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    g.set_energy_weights(eprecond)
+    nd = g.nd
+
+    def v2():
+        vx = g.field(dnhip.F_VX)[nd:nd + ny, nd:nd + nx].T
+        vy = g.field(dnhip.F_VY)[nd:nd + ny, nd:nd + nx].T
+        return ((vx * vx).astype(np.float32) + (vy * vy).astype(np.float32)).astype(np.float32)
+
+    g.forward()
+    obs = [np.zeros((rec.shape[1], nt), np.float32)] * 2
+    g.residuals(obs[0], obs[1], None, 1)
+    g.adjoint()
+    ws, wr = g.energy(0), (g.energy(1) if eprecond == 1 else None)
+    want = np.zeros((nx, ny), np.float32)
+    for k in range(1, nt + 1, 2):
+        g.forward(k)
+        want = (want + v2()).astype(np.float32)
+    assert np.array_equal(ws, want) and ws.max() > 0
+    if eprecond == 1:
+        g.forward()
+        g.residuals(obs[0], obs[1], None, 1)
+        want = np.zeros((nx, ny), np.float32)
+        for k in range(1, nt + 1):
+            if (nt - k) % 2 == 0:              # the adjoint step meets a stored forward step
+                g.adjoint(k)
+                want = (want + v2()).astype(np.float32)
+        assert np.array_equal(wr, want)
     g.close()
