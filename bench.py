@@ -300,13 +300,14 @@ def gradient_workload(nt=4000):
 
 def run_fwi_iterations(g, model, obs, iters, nlbfgs=20):
     """FWI iterations of the gradient job's shot (the only, and test, shot)
-    through the product path: gradient on the device, L-BFGS direction and
-    model update on the host (libdnio), step-length search with device
-    forward runs of the test shot; EPS_SCALE 0.01, SCALEFAC 2, STEPMAX 6,
-    no limits, MIN_ITER above the iteration count (a search that finds no
-    lower misfit still applies its smallest trial step, FWI_PSV.c:698-708).
-    Seconds per part and iteration: the cost of an iteration, not a
-    convergence study (one shot, no gradient tapers)."""
+    through the product path: gradient on the device with energy
+    preconditioning (EPRECOND 3: device energy sums, host weights), the
+    horizontal gradient taper (rows 3..10 ramp, (j*DH)^1 below), L-BFGS
+    direction and model update on the host (libdnio), step-length search
+    with device forward runs of the test shot; EPS_SCALE 0.01, SCALEFAC 2,
+    STEPMAX 6, no limits, MIN_ITER above the iteration count (a search that
+    finds no lower misfit still applies its smallest trial step,
+    FWI_PSV.c:698-708).  Seconds per part and iteration, and the misfit."""
     import dnfwi
     vp, vs, rho = (a.copy() for a in model)
     nx, ny = vp.shape
@@ -315,6 +316,9 @@ def run_fwi_iterations(g, model, obs, iters, nlbfgs=20):
                            rholowerlim=0.0, scalerho=0.5)
     state = dnfwi.LineSearch()
     c = (dnfwi.model_mean(vp), dnfwi.model_mean(vs), dnfwi.model_mean(rho))
+    g.set_energy_weights(3)
+    dh = float(g.p.dh)
+    coeff = dnfwi.taper_hor(nx, ny, dh, 3, 10, 1.0)
     n_fwd = [0]
 
     def misfit(tv, ts, tr, itest):
@@ -333,6 +337,8 @@ def run_fwi_iterations(g, model, obs, iters, nlbfgs=20):
         g.adjoint()
         g.assemble()
         grads = [g.gradient(3 + k) for k in range(3)]
+        grads = [(a * coeff).astype(np.float32)
+                 for a in dnfwi.energy_precond(3, dh, g.energy(0), None, *c, *grads)[1:]]
         t1 = time.perf_counter()
         d = lb.direction(it, *c, *grads, vp, vs, rho)
         t2 = time.perf_counter()
@@ -341,7 +347,8 @@ def run_fwi_iterations(g, model, obs, iters, nlbfgs=20):
         t3 = time.perf_counter()
         vp, vs, rho, _ = dnfwi.model_update(par, it, ls.eps_scale, vp, vs, rho, *d)
         t4 = time.perf_counter()
-        out.append({"iteration": it, "misfit": l2, "step": ls.eps_scale, "test_forward_runs": n_fwd[0],
+        out.append({"iteration": it, "misfit": l2, "step": ls.eps_scale, "search_ok": bool(ls.step1),
+                    "test_forward_runs": n_fwd[0],
                     "gradient_s": t1 - t0, "lbfgs_direction_s": t2 - t1, "step_search_s": t3 - t2,
                     "model_update_s": t4 - t3, "total_s": t4 - t0})
     lb.close()
