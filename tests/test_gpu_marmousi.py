@@ -69,3 +69,72 @@ def test_shipped_marmousi_example_forward(tmp_path):
             assert err <= 1e-5 and np.abs(data).max() > 0
         o.close()
     g.close()
+
+
+def test_shipped_marmousi_fwi_workflow(tmp_path):
+    """The shipped FWI workflow (tests/golden/marmousi/FWI_workflow_marmousi.inp:
+    four stages, TIME_FILT low-pass 2/5/10/20 Hz, EPRECOND 3, LNORM 2) with the
+    shipped parameter file (L-BFGS, horizontal gradient taper GRADT 21..25,
+    EPS_SCALE 0.01, DTINV 3), changed only in MODE = 1, ITERMAX = 2 and the
+    shot subset (shots 1, 50, 100, all test shots).  Observed data: MODE 0 on
+    a true model (the starting model plus a fast lens).  The run ends
+    normally, logs one line per iteration in the TIME_FILT format with the
+    stage number, writes finite models, the stacked gradients and the taper,
+    and its first iteration lowers the misfit or ends the stage."""
+    import re
+    tmp = str(tmp_path)
+    for d in ("start", "su/MARMOUSI_spike", "log", "source", "receiver", "jacobian", "taper", "model"):
+        os.makedirs(os.path.join(tmp, d), exist_ok=True)
+    shutil.copy(os.path.join(GOLD, "receiver_OBC.dat"), os.path.join(tmp, "receiver"))
+    shutil.copy(os.path.join(GOLD, "FWI_workflow_marmousi.inp"), tmp)
+    lines = open(os.path.join(GOLD, "source_OBC_VSP.dat")).read().splitlines()
+    with open(os.path.join(tmp, "source", "source_OBC_VSP.dat"), "w") as f:
+        f.write("3\n" + "\n".join([lines[1], lines[50], lines[100]]) + "\n")
+    text = open(os.path.join(GOLD, "DENISE_marm_OBC.inp")).read()
+
+    def setv(t, key, value):
+        t2, n = re.subn(r"(\(" + re.escape(key) + r"\)\s*=\s*)[^\n]*", lambda m: m.group(1) + value, t)
+        assert n == 1, key
+        return t2
+
+    par0 = dnio.read_par(os.path.join(GOLD, "DENISE_marm_OBC.inp"))
+    nx, ny, dh = par0.nx, par0.ny, par0.dh
+    vp, vs, rho = synth.layered_model(nx, ny, dh, seed=4, sigma=4.0)
+    vp[:, :20], vs[:, :20], rho[:, :20] = 1500.0, 0.0, 1000.0
+    tvp, tvs = vp.copy(), vs.copy()
+    tvp[200:300, 60:80] *= 1.1
+    tvs[200:300, 60:80] *= 1.1
+    for name, model in (("true", (tvp, tvs, rho)), ("marmousi_II_marine", (vp, vs, rho))):
+        for ext, a in zip(("vp", "vs", "rho"), model):
+            np.ascontiguousarray(a, np.float32).tofile(os.path.join(tmp, f"start/{name}.{ext}"))
+    # observed data: MODE 0 on the true model, moved to DATA_DIR
+    t0 = setv(text, "MODE", "0")
+    t0 = t0.replace("MFILE = start/marmousi_II_marine", "MFILE = start/true")
+    open(os.path.join(tmp, "true.inp"), "w").write(t0)
+    _run(tmp, "true.inp")
+    for s in (1, 2, 3):
+        for c in ("x", "y"):
+            os.replace(os.path.join(tmp, f"su/DENISE_MARMOUSI_{c}.su.shot{s}"),
+                       os.path.join(tmp, f"su/MARMOUSI_spike/DENISE_MARMOUSI_{c}.su.shot{s}"))
+    t1 = setv(text, "MODE", "1")
+    t1 = setv(t1, "ITERMAX", "2")
+    t1 = setv(t1, "TESTSHOT_START,TESTSHOT_END,TESTSHOT_INCR", "1, 3, 1")
+    open(os.path.join(tmp, "fwi.inp"), "w").write(t1)
+    out = _run(tmp, "fwi.inp", "FWI_workflow_marmousi.inp")
+    log = [ln.split() for ln in open(os.path.join(tmp, "LOG_TEST.dat")).read().splitlines()]
+    print(f"[marmousi-fwi] {len(log)} iterations; misfits {[float(r[7]) for r in log]}", flush=True)
+    assert 1 <= len(log) <= 8 and all(len(r) == 11 for r in log)
+    stages = [int(r[10]) for r in log]
+    assert stages == sorted(stages) and stages[0] == 1
+    assert [float(r[9]) for r in log if int(r[10]) == 1][0] == 2.0       # FC of stage 1
+    assert float(log[0][7]) > 0
+    if len(log) > 1 and stages[1] == 1:
+        assert float(log[1][7]) < float(log[0][7])
+    for ext in ("vp", "vs", "rho"):
+        a = np.fromfile(os.path.join(tmp, f"model/modelTest_{ext}.bin"), np.float32)
+        assert a.size == nx * ny and np.isfinite(a).all()
+        g = np.fromfile(os.path.join(tmp, f"jacobian/jacobian_Test_grad_{ext}"), np.float32)
+        assert g.size == nx * ny and np.isfinite(g).all()
+    taper = np.fromfile(os.path.join(tmp, "taper/taper_coeff_hor.bin"), np.float32).reshape(nx, ny)
+    assert np.all(taper[:, :20] == 0.0) and np.all(taper[:, 24] == 1.0)
+    assert "TDFWI ITERATION 1 of 2 (stage 1 of 4)" in out
