@@ -55,6 +55,10 @@ def load():
         "dnfwi_energy_precond": ([C.c_int, C.c_int, C.c_int, C.c_float, fp, fp, C.c_float, C.c_float, C.c_float,
                                   fp, fp, fp, fp], C.c_int),
         "dnfwi_taper_hor": ([C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, fp, fp], C.c_int),
+        "dnfwi_pcg_create": ([C.c_int, C.c_int, C.POINTER(vp_)], C.c_int),
+        "dnfwi_pcg_destroy": ([vp_], None),
+        "dnfwi_pcg_reset": ([vp_], None),
+        "dnfwi_pcg_direction": ([vp_, C.c_int, fp, fp, fp, fp], C.c_int),
         "dnfwi_rtm_adjoint_source": ([fp, fp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, fp,
                                       C.POINTER(C.c_double)], C.c_int),
     }
@@ -192,3 +196,33 @@ def taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper):
     coeff = np.empty((nx, ny), np.float32)
     dnio._check(load().dnfwi_taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper, _p(coeff), None))
     return coeff
+
+
+class PCG:
+    """dnfwi_pcg: the conjugate-gradient memory of one FWI run (GRAD_METHOD 1)."""
+
+    def __init__(self, nx, ny):
+        h = C.c_void_p()
+        dnio._check(load().dnfwi_pcg_create(nx, ny, C.byref(h)))
+        self.h = h
+
+    def direction(self, pcg_beta, g_vp, g_vs, g_rho):
+        """(d_vp, d_vs, d_rho, betas): new arrays."""
+        d = [_f(a).copy() for a in (g_vp, g_vs, g_rho)]
+        b = np.zeros(3, np.float32)
+        dnio._check(load().dnfwi_pcg_direction(self.h, pcg_beta, *[_p(a) for a in d], _p(b)))
+        return d[0], d[1], d[2], b
+
+    def reset(self):
+        load().dnfwi_pcg_reset(self.h)
+
+    def close(self):
+        if self.h:
+            load().dnfwi_pcg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -9,7 +9,7 @@
  *           SNAP_SHOT only, with the wavefield snapshots of src/snap_ssg.c
  *           (SNAP_FILE.{bin,asc}.{x,y,p,div,rot}.0.0).
  * MODE = 1: the FWI workflow of src/PSV/FWI_PSV.c:381-836 with L-BFGS
- *           (GRAD_METHOD = 2): for each stage of the workflow file and each
+ *           (GRAD_METHOD = 2) or PCG (1): for each stage of the workflow file and each
  *           iteration up to ITERMAX,
  *           - the gradient of src/PSV/grad_obj_psv.c:130-262 per shot
  *             (forward with snapshot store, L2 residual against
@@ -19,7 +19,7 @@
  *             JACOBIAN_P_image_shot_N and JACOBIAN_S_image_shot_N
  *             (src/PSV/RTM_PSV_out_shot.c); the misfit is the L2 sum over
  *             the test shots;
- *           - the L-BFGS direction, the parabolic step-length search over
+ *           - the L-BFGS or PCG direction, the parabolic step-length search over
  *             the test shots (src/PSV/step_length_est_psv.c,
  *             src/PSV/obj_psv.c) and the model update
  *             (src/PSV/calc_mat_change_test_PSV.c), written as
@@ -97,7 +97,8 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (p->trkill) die("TRKILL", "trace killing is not built");
     }
     if (p->mode == 1) {
-        if (p->grad_method != 2) die("GRAD_METHOD", "the L-BFGS method (GRAD_METHOD = 2) is built; PCG is not");
+        if (p->grad_method != 1 && p->grad_method != 2) die("GRAD_METHOD", "PCG (1) or L-BFGS (2) expected");
+        if (p->grad_method == 1 && (p->pcg_beta < 1 || p->pcg_beta > 4)) die("PCG_BETA", "1 to 4 expected");
         if (p->model_filter || p->grad_filter) die("MODEL_FILTER/GRAD_FILTER", "model and gradient filters are not built");
         if (p->gravity) die("GRAVITY", "gravity modelling is not built");
         if (p->nlbfgs < 1) die("NLBFGS", "at least 1 expected");
@@ -377,7 +378,11 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
     float *nv = xmalloc(sizeof(float) * r->ncell), *ns = xmalloc(sizeof(float) * r->ncell);
     float *nr = xmalloc(sizeof(float) * r->ncell);
     dnfwi_lbfgs *lb = NULL;
-    io_ok(dnfwi_lbfgs_create(nx, ny, P->nlbfgs, &lb), "L-BFGS memory");
+    dnfwi_pcg *pc = NULL;
+    if (P->grad_method == 2)
+        io_ok(dnfwi_lbfgs_create(nx, ny, P->nlbfgs, &lb), "L-BFGS memory");
+    else
+        io_ok(dnfwi_pcg_create(nx, ny, &pc), "PCG memory");
     double *L2_hist = xmalloc(sizeof(double) * (size_t)(P->itermax + 2));
     dnfwi_linesearch ls;
     memset(&ls, 0, sizeof ls);
@@ -420,7 +425,11 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
                 write_plane(r->name, g[k], nx, ny);
             }
             printf(" misfit (L2, test shots): %.9e\n", L2sum);
-            io_ok(dnfwi_lbfgs_direction(lb, iter, c_vp, c_vs, c_rho, g[0], g[1], g[2], vp, vs, rho), "L-BFGS");
+            /* FWI_PSV.c:636-676: PCG or L-BFGS search direction */
+            if (lb)
+                io_ok(dnfwi_lbfgs_direction(lb, iter, c_vp, c_vs, c_rho, g[0], g[1], g[2], vp, vs, rho), "L-BFGS");
+            else
+                io_ok(dnfwi_pcg_direction(pc, P->pcg_beta, g[0], g[1], g[2], NULL), "PCG");
             if (iter == 1) min_iter_help = P->min_iter; /* FWI_PSV.c:687-690 */
             obj_env oe = {r, taus, taup};
             io_ok(dnfwi_step_length(&up, iter, P->eps_scale, P->scalefac, P->stepmax, L2sum, nx, ny, vp, vs, rho, g[0],
@@ -474,7 +483,8 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
                     write_models(name[0], name[1], name[2], vp, vs, rho, nx, ny);
                 }
                 min_iter_help = iter + P->min_iter;
-                dnfwi_lbfgs_reset(lb);
+                if (lb) dnfwi_lbfgs_reset(lb); /* FWI_PSV.c:802-811 */
+                if (pc) dnfwi_pcg_reset(pc);
                 printf(" %s: next FWI stage\n", ls.step3 ? "step-length search failed" : "abort criterion reached");
                 break; /* FWI_PSV.c:824, before the iter_true++ of :835 */
             }
@@ -488,6 +498,7 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
         }
     }
     dnfwi_lbfgs_destroy(lb);
+    dnfwi_pcg_destroy(pc);
     for (int k = 0; k < 3; k++) free(g[k]);
     free(nv);
     free(ns);

@@ -439,3 +439,89 @@ int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_
         for (size_t k = 0; k < (size_t)nx * ny; k++) g[k] *= coeff[k];
     return 0;
 }
+
+struct dnfwi_pcg {
+    size_t ncell;
+    float *old, *dir, *nw; /* PCG_old, PCG_dir, PCG_new: [vp | vs | rho] */
+};
+
+int dnfwi_pcg_create(int nx, int ny, dnfwi_pcg **out)
+{
+    if (!out || nx < 1 || ny < 1) return dnio_fail_msg("dnfwi_pcg_create: bad size");
+    dnfwi_pcg *pc = calloc(1, sizeof *pc);
+    if (!pc) return dnio_fail_msg("dnfwi_pcg_create: out of host memory");
+    pc->ncell = (size_t)nx * ny;
+    pc->old = calloc(3 * pc->ncell, sizeof(float));
+    pc->dir = calloc(3 * pc->ncell, sizeof(float));
+    pc->nw = calloc(3 * pc->ncell, sizeof(float));
+    if (!pc->old || !pc->dir || !pc->nw) {
+        dnfwi_pcg_destroy(pc);
+        return dnio_fail_msg("dnfwi_pcg_create: out of host memory");
+    }
+    *out = pc;
+    return 0;
+}
+
+void dnfwi_pcg_destroy(dnfwi_pcg *pc)
+{
+    if (!pc) return;
+    free(pc->old);
+    free(pc->dir);
+    free(pc->nw);
+    free(pc);
+}
+
+void dnfwi_pcg_reset(dnfwi_pcg *pc)
+{
+    if (!pc) return;
+    memset(pc->old, 0, sizeof(float) * 3 * pc->ncell);
+    memset(pc->dir, 0, sizeof(float) * 3 * pc->ncell);
+    memset(pc->nw, 0, sizeof(float) * 3 * pc->ncell);
+}
+
+int dnfwi_pcg_direction(dnfwi_pcg *pc, int pcg_beta, float *g_vp, float *g_vs, float *g_rho, float *beta_out)
+{
+    if (!pc || !g_vp || !g_vs || !g_rho || pcg_beta < 1 || pcg_beta > 4)
+        return dnio_fail_msg("dnfwi_pcg_direction: bad argument (PCG_BETA 1..4)");
+    const size_t nc = pc->ncell;
+    float *g[3] = {g_vp, g_vs, g_rho};
+    /* descent + store_PCG_PSV: PCG_new = -g */
+    for (int c = 0; c < 3; c++)
+        for (size_t k = 0; k < nc; k++) pc->nw[c * nc + k] = -g[c][k];
+    /* PCG.c:21-78 */
+    float beta[3];
+    for (int c = 0; c < 3; c++) {
+        const float *n = pc->nw + c * nc, *o = pc->old + c * nc, *d = pc->dir + c * nc;
+        float betaz = 0.0f, betan = 0.0f;
+        for (size_t k = 0; k < nc; k++) {
+            if (pcg_beta == 1) {
+                betaz += n[k] * n[k];
+                betan += o[k] * o[k];
+            } else if (pcg_beta == 2) {
+                betaz += n[k] * (n[k] - o[k]);
+                betan += o[k] * o[k];
+            } else if (pcg_beta == 3) {
+                betaz += -n[k] * (n[k] - o[k]);
+                betan += d[k] * (n[k] - o[k]);
+            } else {
+                betaz += -n[k] * n[k];
+                betan += d[k] * (n[k] - o[k]);
+            }
+        }
+        beta[c] = 0.0f;
+        if (betan != 0.0f) beta[c] = betaz / betan;
+        if ((beta[c] < 0.0) && (pcg_beta == 2)) beta[c] = 0.0f;
+    }
+    /* PCG.c:82-93, extract_PCG_PSV, store_PCG_PSV(PCG_old), descent */
+    for (int c = 0; c < 3; c++) {
+        float *d = pc->dir + c * nc;
+        const float *n = pc->nw + c * nc;
+        for (size_t k = 0; k < nc; k++) {
+            d[k] = n[k] + beta[c] * d[k];
+            g[c][k] = -d[k];
+        }
+        memcpy(pc->old + c * nc, n, sizeof(float) * nc);
+        if (beta_out) beta_out[c] = beta[c];
+    }
+    return 0;
+}

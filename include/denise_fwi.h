@@ -50,6 +50,23 @@ int dnfwi_lbfgs_pointer(const dnfwi_lbfgs *lb);
 int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, float c_rho, float *g_vp,
                           float *g_vs, float *g_rho, const float *vp, const float *vs, const float *rho);
 
+/* Preconditioned conjugate gradients (GRAD_METHOD = 1): PCG_old and PCG_dir
+ * of 3*nx*ny floats [vp | vs | rho] (src/PSV/FWI_PSV.c:304-316), zero at
+ * the start and after each stage (zero_PCG, FWI_PSV.c:802-805). */
+typedef struct dnfwi_pcg dnfwi_pcg;
+int dnfwi_pcg_create(int nx, int ny, dnfwi_pcg **out);
+void dnfwi_pcg_destroy(dnfwi_pcg *pc);
+void dnfwi_pcg_reset(dnfwi_pcg *pc);
+
+/* One iteration's direction (FWI_PSV.c:637-662): descent (-g),
+ * store_PCG_PSV, PCG (src/PCG.c:11-105: per class beta from single-precision
+ * sums, PCG_BETA 1 Fletcher-Reeves, 2 Polak-Ribiere with beta >= 0,
+ * 3 Hestenes-Stiefel, 4 Dai-Yuan; beta = 0 when its denominator is 0;
+ * dir = -g + beta dir), extract_PCG_PSV, and descent again: the gradients
+ * g_vp, g_vs, g_rho are replaced by -dir, the direction the model update
+ * subtracts.  beta_out (may be NULL) gets the three betas. */
+int dnfwi_pcg_direction(dnfwi_pcg *pc, int pcg_beta, float *g_vp, float *g_vs, float *g_rho, float *beta_out);
+
 /* Model averages C_vp, C_vs, C_rho of src/PSV/FWI_PSV.c:540-580: a
  * single-precision sequential sum in file order divided by NX*NY. */
 float dnfwi_model_mean(const float *a, int nx, int ny);

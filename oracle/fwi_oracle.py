@@ -18,7 +18,9 @@ Written from the reference's functions, one per function, keeping its
     its L2 sum, src/calc_res.c:563-569;
   - energy_precond: src/eprecond1.c:9-60 with the gradient scaling of
     src/PSV/grad_obj_psv.c:219-238;
-  - taper_hor: src/taper_grad.c:188-322 (SWS_TAPER_GRAD_HOR).
+  - taper_hor: src/taper_grad.c:188-322 (SWS_TAPER_GRAD_HOR);
+  - PCG: src/PCG.c:11-105 with descent, store_PCG_PSV and extract_PCG_PSV
+    as src/PSV/FWI_PSV.c:637-662 calls them.
 Vectors are numpy float32 arrays of shape (nx, ny) (the model-file order).
 The pins of these functions against what the mathematics fixes (BFGS
 matrix recursion, parabola vertex, limits) are in tests/test_fwi_host.py.
@@ -318,3 +320,46 @@ def taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper):
         if j >= gradt1:
             h += 1
     return coeff
+
+
+class PCG:
+    """PCG_old / PCG_dir of [vp | vs | rho]; direction() returns the
+    arrays the model update subtracts (-PCG_dir) and the three betas."""
+
+    def __init__(self, nx, ny):
+        self.n = nx * ny
+        self.old = np.zeros(3 * self.n, F32)
+        self.dir = np.zeros(3 * self.n, F32)
+
+    def reset(self):
+        self.old[...] = 0
+        self.dir[...] = 0
+
+    def direction(self, pcg_beta, g_vp, g_vs, g_rho):
+        shape = g_vp.shape
+        new = np.concatenate([-g_vp.ravel(), -g_vs.ravel(), -g_rho.ravel()]).astype(F32)
+        betas = []
+        for k in range(3):
+            sl = slice(k * self.n, (k + 1) * self.n)
+            nw, od, dr = new[sl], self.old[sl], self.dir[sl]
+            if pcg_beta == 1:
+                tz, tn = (nw * nw).astype(F32), (od * od).astype(F32)
+            elif pcg_beta == 2:
+                tz, tn = (nw * (nw - od).astype(F32)).astype(F32), (od * od).astype(F32)
+            elif pcg_beta == 3:
+                tz, tn = ((-nw) * (nw - od).astype(F32)).astype(F32), (dr * (nw - od).astype(F32)).astype(F32)
+            else:
+                tz, tn = ((-nw) * nw).astype(F32), (dr * (nw - od).astype(F32)).astype(F32)
+            bz = np.add.accumulate(tz, dtype=F32)[-1]
+            bn = np.add.accumulate(tn, dtype=F32)[-1]
+            b = F32(0.0)
+            if bn != 0:
+                b = F32(bz / bn)
+            if b < 0.0 and pcg_beta == 2:
+                b = F32(0.0)
+            betas.append(b)
+            self.dir[sl] = (nw + (b * dr).astype(F32)).astype(F32)
+        self.old = new.copy()
+        d = -self.dir
+        return (d[:self.n].reshape(shape), d[self.n:2 * self.n].reshape(shape), d[2 * self.n:].reshape(shape),
+                np.array(betas, F32))

@@ -339,3 +339,42 @@ def test_taper_hor():
         assert np.allclose(got[:, j - 1], want, rtol=1e-6, atol=1e-12)
     for j in range(g2 + 1, ny + 1):
         assert np.allclose(got[:, j - 1], (j * dh) ** ex, rtol=1e-6)
+
+
+@pytest.mark.parametrize("pcg_beta", [1, 2, 3, 4])
+def test_pcg_direction_and_conjugacy(pcg_beta):
+    """dnfwi_pcg_direction equals the restatement bit for bit over several
+    iterations (three classes, random gradients); pin: on a quadratic
+    misfit of the vp class, 1/2 m^T A m - b^T m with A symmetric positive
+    definite (n = 6 cells), exact line searches along the directions reach
+    the minimum in n iterations for every beta formula (linear conjugate
+    gradients), and iteration 1 is steepest descent."""
+    rng = np.random.default_rng(pcg_beta)
+    pc, po = dnfwi.PCG(NX, NY), fo.PCG(NX, NY)
+    for it in range(5):
+        g = [rnd(rng) for _ in range(3)]
+        a = pc.direction(pcg_beta, *g)
+        b = po.direction(pcg_beta, *g)
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y), it
+        if it == 0:
+            assert np.array_equal(a[0], g[0]) and np.all(a[3] == 0)
+    pc.close()
+    # the pin
+    nx, ny = 3, 2
+    n = nx * ny
+    M = rng.standard_normal((n, n))
+    A = M @ M.T + n * np.eye(n)
+    bvec = rng.standard_normal(n)
+    m = np.zeros(n)
+    cg = dnfwi.PCG(nx, ny)
+    zero = np.zeros((nx, ny), F32)
+    g0 = np.linalg.norm(A @ m - bvec)
+    for it in range(n):
+        g = (A @ m - bvec).astype(F32).reshape(nx, ny)
+        d = cg.direction(pcg_beta, g, zero, zero)[0].astype(np.float64).ravel()
+        p = -d                                   # the model moves along -direction
+        eps = -float((A @ m - bvec) @ p) / float(p @ A @ p)
+        m = m + eps * p
+    cg.close()
+    assert np.linalg.norm(A @ m - bvec) < 1e-4 * g0

@@ -24,15 +24,18 @@ pytestmark = pytest.mark.gpu
 ITERMAX = 4
 
 
-@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond"])
+@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
     ORDER 4) filters the source signals and the observed data.  precond:
     energy preconditioning EPRECOND 3 of each shot's gradients (device
     energy sums, host weights) and the horizontal gradient taper
-    (SWS_TAPER_GRAD_HOR, GRADT 3..12, exponent 2) on the stacked ones."""
+    (SWS_TAPER_GRAD_HOR, GRADT 3..12, exponent 2) on the stacked ones.
+    pcg: the same preconditioning with conjugate gradients (GRAD_METHOD 1,
+    PCG_BETA 2) instead of L-BFGS."""
     time_filt = int(variant == "time_filt")
-    precond = variant == "precond"
+    precond = variant in ("precond", "pcg")
+    pcg = variant == "pcg"
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
@@ -50,7 +53,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     inpfile.write_inp(os.path.join(tmp, "fwi.inp"), MODE=1, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
                       MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1", ITERMAX=ITERMAX, INV_MOD_OUT=1,
                       MISFIT_LOG_FILE="misfit.dat", SWS_TAPER_GRAD_HOR=int(precond), GRADT="3, 12, 60, 62",
-                      EXP_TAPER_GRAD_HOR=2.0)
+                      EXP_TAPER_GRAD_HOR=2.0, GRAD_METHOD=1 if pcg else 2, PCG_BETA=2)
     fcs, fce, order = (4.0, 12.0, 4) if time_filt else (0.0, 0.0, 6)
     inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=time_filt, fc_start=fcs, fc_end=fce, order=order,
                            eprecond=3 if precond else 0)
@@ -87,7 +90,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
             l2 += g.residuals(obs[i][0], obs[i][1], None, 1)
         return l2
 
-    lb = fo.LBFGS(nx, ny, par.nlbfgs)
+    lb = fo.PCG(nx, ny) if pcg else fo.LBFGS(nx, ny, par.nlbfgs)
     c = (fo.model_mean(vp), fo.model_mean(vs), fo.model_mean(rho))
     carry = ([0.0, 0.0, 0.0], [0.0, 0.0, 0.0])
     lines, l2s = [], []
@@ -110,7 +113,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
             got = np.fromfile(os.path.join(tmp, "taper/t_coeff_hor.bin"), np.float32).reshape(nx, ny)
             assert np.array_equal(got, coeff)
         l2s.append(l2sum)
-        d = lb.direction(it, *c, *grads, vp, vs, rho)
+        d = lb.direction(2, *grads)[:3] if pcg else lb.direction(it, *c, *grads, vp, vs, rho)
         ls = fo.step_length(upd, it, par.eps_scale, par.scalefac, par.stepmax, l2sum, vp, vs, rho, *d, misfit,
                             carry=carry)
         carry = (ls["epst"], ls["L2t"])
