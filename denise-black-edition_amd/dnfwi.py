@@ -59,6 +59,8 @@ def load():
         "dnfwi_pcg_destroy": ([vp_], None),
         "dnfwi_pcg_reset": ([vp_], None),
         "dnfwi_pcg_direction": ([vp_, C.c_int, fp, fp, fp, fp], C.c_int),
+        "dnfwi_taper_vert": ([C.c_int, C.c_int, C.c_float, C.POINTER(C.c_int), C.c_int, fp, C.c_int,
+                              C.POINTER(C.c_int), fp, fp], C.c_int),
         "dnfwi_rtm_adjoint_source": ([fp, fp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, fp,
                                       C.POINTER(C.c_double)], C.c_int),
     }
@@ -226,3 +228,17 @@ class PCG:
             self.close()
         except Exception:
             pass
+
+
+def taper_vert(nx, ny, dh, gradt, srcpos, recpos, g=None):
+    """(coeff, g tapered) of dnfwi_taper_vert; srcpos 8 x nsrc (metres),
+    recpos 2 x ntr (grid indices)."""
+    gt = (C.c_int * 4)(*[int(v) for v in gradt])
+    sp = np.ascontiguousarray(srcpos, np.float32)
+    rp = np.ascontiguousarray(recpos[:2], np.int32)
+    coeff = np.empty((nx, ny), np.float32)
+    gg = _f(g).copy() if g is not None else None
+    dnio._check(load().dnfwi_taper_vert(nx, ny, dh, gt, sp.shape[1], _p(sp), rp.shape[1],
+                                        rp.ctypes.data_as(C.POINTER(C.c_int)), _p(coeff),
+                                        _p(gg) if gg is not None else None))
+    return coeff, gg

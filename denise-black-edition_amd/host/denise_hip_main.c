@@ -91,8 +91,8 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (p->ndt != 1) die("NDT", "FWI needs NDT = 1");
         if (p->idx != 1 || p->idy != 1) die("IDX/IDY", "FWI runs on every grid point (IDX = IDY = 1)");
         if (p->sws_taper_circular_per_shot) die("SWS_TAPER_CIRCULAR_PER_SHOT", "per-shot gradient tapers are not built");
-        if (p->sws_taper_grad_vert || p->sws_taper_grad_sources || p->sws_taper_file)
-            die("SWS_TAPER_GRAD_*", "of the gradient tapers the horizontal one (SWS_TAPER_GRAD_HOR) is built");
+        if (p->sws_taper_grad_sources)
+            die("SWS_TAPER_GRAD_SOURCES", "the cylindrical source taper is not built (vertical, horizontal and file tapers are)");
         if (p->grad_form != 1 && p->grad_form != 2) die("GRAD_FORM", "1 or 2 expected");
         if (p->trkill) die("TRKILL", "trace killing is not built");
     }
@@ -319,20 +319,39 @@ static double gradient_pass(run *r, float **g_sum)
     return misfit;
 }
 
-/* precond_PSV (src/PSV/precond_PSV.c:13-60) with the horizontal taper only:
- * the stacked vp, vs and rho gradients times the taper, which is written to
- * TFILE_coeff_hor.bin (taper_grad.c:312-316) */
+/* precond_PSV (src/PSV/precond_PSV.c:13-60): per stacked gradient (vp, vs,
+ * rho) the vertical taper (taper_grad.c:46-186, TFILE_coeff_vert.bin), the
+ * horizontal taper (188-322, TFILE_coeff_hor.bin) and the taper read from
+ * TFILE.bin, TFILE_u.bin, TFILE_rho.bin (578-640), in that order */
 static void precondition(run *r, float **g)
 {
     dnio_par *P = r->P;
-    if (!P->sws_taper_grad_hor) return;
+    if (!P->sws_taper_grad_vert && !P->sws_taper_grad_hor && !P->sws_taper_file) return;
     float *coeff = xmalloc(sizeof(float) * r->ncell);
-    io_ok(dnfwi_taper_hor(P->nx, P->ny, P->dh, P->gradt[0], P->gradt[1], P->exp_taper_grad_hor, coeff, NULL),
-          "gradient taper");
-    for (int k = 0; k < 3; k++)
-        for (size_t q = 0; q < r->ncell; q++) g[k][q] *= coeff[q];
-    snprintf(r->name, sizeof r->name, "%s_coeff_hor.bin", P->tfile);
-    write_plane(r->name, coeff, P->nx, P->ny);
+    for (int k = 0; k < 3; k++) {
+        if (P->sws_taper_grad_vert) {
+            io_ok(dnfwi_taper_vert(P->nx, P->ny, P->dh, P->gradt, r->nsrc, r->srcpos, r->ntr, r->recpos, coeff, g[k]),
+                  "vertical gradient taper");
+            snprintf(r->name, sizeof r->name, "%s_coeff_vert.bin", P->tfile);
+            write_plane(r->name, coeff, P->nx, P->ny);
+        }
+        if (P->sws_taper_grad_hor) {
+            io_ok(dnfwi_taper_hor(P->nx, P->ny, P->dh, P->gradt[0], P->gradt[1], P->exp_taper_grad_hor, coeff, g[k]),
+                  "horizontal gradient taper");
+            snprintf(r->name, sizeof r->name, "%s_coeff_hor.bin", P->tfile);
+            write_plane(r->name, coeff, P->nx, P->ny);
+        }
+        if (P->sws_taper_file) {
+            const char *suffix[3] = {"", "_u", "_rho"};
+            snprintf(r->name, sizeof r->name, "%s%s.bin", P->tfile, suffix[k]);
+            FILE *f = fopen(r->name, "rb");
+            if (!f) die("SWS_TAPER_FILE", r->name);
+            const size_t got = fread(coeff, sizeof(float), r->ncell, f);
+            fclose(f);
+            if (got != r->ncell) die("SWS_TAPER_FILE", "taper file shorter than NX*NY floats");
+            for (size_t q = 0; q < r->ncell; q++) g[k][q] *= coeff[q];
+        }
+    }
     free(coeff);
 }
 

@@ -525,3 +525,43 @@ int dnfwi_pcg_direction(dnfwi_pcg *pc, int pcg_beta, float *g_vp, float *g_vs, f
     }
     return 0;
 }
+
+int dnfwi_taper_vert(int nx, int ny, float dh, const int *gradt, int nsrc, const float *srcpos, int ntr,
+                     const int *recpos, float *coeff, float *g)
+{
+    if (!coeff || !gradt || nx < 1 || ny < 1 || (nsrc > 0 && !srcpos) || (ntr > 0 && !recpos))
+        return dnio_fail_msg("dnfwi_taper_vert: bad argument");
+    const int tl = gradt[1] - gradt[0], tl2 = gradt[3] - gradt[2], ifw = gradt[3] - gradt[0];
+    float *window = malloc(sizeof(float) * (size_t)(ifw > 0 ? ifw : 1));
+    if (!window) return dnio_fail_msg("dnfwi_taper_vert: out of host memory");
+    const float a = 3; /* damping coefficient */
+    for (int i = 1; i <= ifw; i++) {
+        window[i - 1] = 1.0f;
+        if (i <= tl)
+            window[i - 1] = exp(-(1.0 / 2.0) * (a * (i - tl) / (tl / 2.0)) * (a * (i - tl) / (tl / 2.0)));
+        if (i >= (ifw - tl2))
+            window[i - 1] = exp(-(1.0 / 2.0) * (a * (i - (ifw - tl2)) / (tl2 / 2.0)) *
+                                (a * (i - (ifw - tl2)) / (tl2 / 2.0)));
+    }
+    for (int j = 1; j <= ny; j++) {
+        int h = 1;
+        for (int i = 1; i <= nx; i++) {
+            float grad_tap = 0.0f;
+            if ((i > gradt[0]) && (i < gradt[3])) grad_tap = window[h++ - 1];
+            coeff[(size_t)(i - 1) * ny + (j - 1)] = grad_tap;
+        }
+    }
+    free(window);
+    if (!g) return 0;
+    for (size_t k = 0; k < (size_t)nx * ny; k++) g[k] *= coeff[k];
+    /* taper_grad.c:120-170: the source and receiver cells */
+    for (int n = 0; n < nsrc; n++) {
+        const int i = (int)floor(srcpos[0 * nsrc + n] / dh + 0.5), j = (int)floor(srcpos[1 * nsrc + n] / dh + 0.5);
+        if (i >= 1 && i <= nx && j >= 1 && j <= ny) g[(size_t)(i - 1) * ny + (j - 1)] = 0.0f;
+    }
+    for (int n = 0; n < ntr; n++) {
+        const int i = recpos[n], j = recpos[ntr + n];
+        if (i >= 1 && i <= nx && j >= 1 && j <= ny) g[(size_t)(i - 1) * ny + (j - 1)] = 0.0f;
+    }
+    return 0;
+}

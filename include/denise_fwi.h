@@ -137,6 +137,17 @@ int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws
  * taper, and g (may be NULL) is multiplied by it. */
 int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_taper, float *coeff, float *g);
 
+/* Vertical gradient taper (src/taper_grad.c:46-186, SWS_TAPER_GRAD_VERT):
+ * along x, 0 up to column GRADT1 and from GRADT4, Gaussian ramps
+ * exp(-(3 (k - L1)/(L1/2))^2 / 2) up from GRADT1 (L1 = GRADT2 - GRADT1) and
+ * down to GRADT4 (L2 = GRADT4 - GRADT3), 1 between; coeff (x-major) gets the
+ * taper; g (may be NULL) is multiplied by it and then set to 0 at the grid
+ * cells of all nsrc sources (srcpos: the 8 x nsrc table of
+ * dnio_read_sources, x and y in metres, cell iround(x/DH), iround(y/DH)) and
+ * of the ntr receivers (recpos: 2 x ntr, 1-based grid indices). */
+int dnfwi_taper_vert(int nx, int ny, float dh, const int *gradt, int nsrc, const float *srcpos, int ntr,
+                     const int *recpos, float *coeff, float *g);
+
 /* Adjoint source of reverse-time migration (MODE 2), the MODE 2 branch of
  * calc_res with LNORM = 2 (src/calc_res.c:416-446): per trace the first data
  * sample is replaced by the synthetic one, then GRAD_FORM 1 integrates

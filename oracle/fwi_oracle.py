@@ -19,6 +19,7 @@ Written from the reference's functions, one per function, keeping its
   - energy_precond: src/eprecond1.c:9-60 with the gradient scaling of
     src/PSV/grad_obj_psv.c:219-238;
   - taper_hor: src/taper_grad.c:188-322 (SWS_TAPER_GRAD_HOR);
+  - taper_vert: src/taper_grad.c:46-186 (SWS_TAPER_GRAD_VERT);
   - PCG: src/PCG.c:11-105 with descent, store_PCG_PSV and extract_PCG_PSV
     as src/PSV/FWI_PSV.c:637-662 calls them.
 Vectors are numpy float32 arrays of shape (nx, ny) (the model-file order).
@@ -363,3 +364,40 @@ class PCG:
         d = -self.dir
         return (d[:self.n].reshape(shape), d[self.n:2 * self.n].reshape(shape), d[2 * self.n:].reshape(shape),
                 np.array(betas, F32))
+
+
+def taper_vert(nx, ny, dh, gradt, srcpos, recpos, g):
+    """(coeff, tapered g); srcpos rows 0/1 = x/y in metres, recpos rows 0/1 =
+    grid indices."""
+    import math
+    g1, g2, g3, g4 = [int(v) for v in gradt]
+    tl, tl2, ifw = g2 - g1, g4 - g3, g4 - g1
+    a = F32(3.0)
+    window = []
+    for i in range(1, ifw + 1):
+        w = F32(1.0)
+        if i <= tl:
+            q = float(F32(a * F32(i - tl))) / (tl / 2.0)
+            w = F32(math.exp(-(1.0 / 2.0) * q * q))
+        if i >= ifw - tl2:
+            q = float(F32(a * F32(i - (ifw - tl2)))) / (tl2 / 2.0)
+            w = F32(math.exp(-(1.0 / 2.0) * q * q))
+        window.append(w)
+    row = np.zeros(nx, F32)
+    h = 1
+    for i in range(1, nx + 1):
+        if g1 < i < g4:
+            row[i - 1] = window[h - 1]
+            h += 1
+    coeff = np.repeat(row[:, None], ny, axis=1).astype(F32)
+    out = (g * coeff).astype(F32)
+    for n in range(srcpos.shape[1]):
+        i = int(math.floor(float(F32(F32(srcpos[0, n]) / F32(dh))) + 0.5))
+        j = int(math.floor(float(F32(F32(srcpos[1, n]) / F32(dh))) + 0.5))
+        if 1 <= i <= nx and 1 <= j <= ny:
+            out[i - 1, j - 1] = 0.0
+    for n in range(recpos.shape[1]):
+        i, j = int(recpos[0, n]), int(recpos[1, n])
+        if 1 <= i <= nx and 1 <= j <= ny:
+            out[i - 1, j - 1] = 0.0
+    return coeff, out

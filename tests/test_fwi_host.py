@@ -378,3 +378,34 @@ def test_pcg_direction_and_conjugacy(pcg_beta):
         m = m + eps * p
     cg.close()
     assert np.linalg.norm(A @ m - bvec) < 1e-4 * g0
+
+
+def test_taper_vert():
+    """dnfwi_taper_vert equals the restatement bit for bit; pins: 0 up to
+    GRADT1 and from GRADT4, 1 between the ramps, both Gaussian ramps in
+    closed form, and the gradient is 0 at every source and receiver cell."""
+    nx, ny, dh, gt = 60, 9, 10.0, (5, 15, 40, 50)
+    rng = np.random.default_rng(8)
+    g = rnd(rng, shape=(nx, ny))
+    src = np.zeros((8, 3), F32)
+    src[0], src[1] = [100.0, 204.0, 305.0], [20.0, 30.0, 41.0]
+    rec = np.array([[12, 20, 33], [2, 2, 2]], np.int32)
+    coeff, got = dnfwi.taper_vert(nx, ny, dh, gt, src, rec, g)
+    rc, rg = fo.taper_vert(nx, ny, dh, gt, src, rec, g)
+    assert np.array_equal(coeff, rc) and np.array_equal(got, rg)
+    assert np.all(coeff[:5] == 0.0) and np.all(coeff[49:] == 0.0)
+    assert np.all(coeff[15:39] == 1.0)
+    for i in range(6, 50):
+        k = i - 5
+        want = 1.0
+        if k <= 10:
+            want = np.exp(-0.5 * (3 * (k - 10) / 5.0) ** 2)
+        if k >= 45 - 10:
+            want = np.exp(-0.5 * (3 * (k - 35) / 5.0) ** 2)
+        assert abs(coeff[i - 1, 0] - want) < 1e-6 * max(want, 1e-30) + 1e-30, i
+    for (x, y) in ((10, 2), (20, 3), (31, 4), (12, 2), (20, 2), (33, 2)):
+        assert got[x - 1, y - 1] == 0.0
+    mask = np.ones_like(g, bool)
+    for (x, y) in ((10, 2), (20, 3), (31, 4), (12, 2), (20, 2), (33, 2)):
+        mask[x - 1, y - 1] = False
+    assert np.array_equal(got[mask], (g * coeff)[mask])

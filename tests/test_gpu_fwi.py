@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 ITERMAX = 4
 
 
-@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg"])
+@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
     ORDER 4) filters the source signals and the observed data.  precond:
@@ -32,10 +32,13 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     energy sums, host weights) and the horizontal gradient taper
     (SWS_TAPER_GRAD_HOR, GRADT 3..12, exponent 2) on the stacked ones.
     pcg: the same preconditioning with conjugate gradients (GRAD_METHOD 1,
-    PCG_BETA 2) instead of L-BFGS."""
+    PCG_BETA 2) instead of L-BFGS.  tapers: the vertical taper
+    (SWS_TAPER_GRAD_VERT, columns 10..86, source and receiver cells zeroed)
+    and taper files (SWS_TAPER_FILE: TFILE.bin, _u, _rho)."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
     pcg = variant == "pcg"
+    tapers = variant == "tapers"
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
@@ -52,8 +55,15 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     _files(tmp, nx, ny, dh, bg, sources, receivers, mname="model/start")
     inpfile.write_inp(os.path.join(tmp, "fwi.inp"), MODE=1, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
                       MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1", ITERMAX=ITERMAX, INV_MOD_OUT=1,
-                      MISFIT_LOG_FILE="misfit.dat", SWS_TAPER_GRAD_HOR=int(precond), GRADT="3, 12, 60, 62",
-                      EXP_TAPER_GRAD_HOR=2.0, GRAD_METHOD=1 if pcg else 2, PCG_BETA=2)
+                      MISFIT_LOG_FILE="misfit.dat", SWS_TAPER_GRAD_HOR=int(precond),
+                      GRADT="10, 25, 70, 86" if tapers else "3, 12, 60, 62",
+                      EXP_TAPER_GRAD_HOR=2.0, GRAD_METHOD=1 if pcg else 2, PCG_BETA=2,
+                      SWS_TAPER_GRAD_VERT=int(tapers), SWS_TAPER_FILE=int(tapers))
+    if tapers:
+        trng = np.random.default_rng(5)
+        tfiles = {sfx: trng.uniform(0.5, 1.5, (nx, ny)).astype(np.float32) for sfx in ("", "_u", "_rho")}
+        for sfx, a in tfiles.items():
+            a.tofile(os.path.join(tmp, f"taper/t{sfx}.bin"))
     fcs, fce, order = (4.0, 12.0, 4) if time_filt else (0.0, 0.0, 6)
     inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=time_filt, fc_start=fcs, fc_end=fce, order=order,
                            eprecond=3 if precond else 0)
@@ -107,6 +117,11 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                 gs = list(fo.energy_precond(3, par.dh, g.energy(0), None, *c, *gs)[1:])
             for k in range(3):
                 grads[k] += gs[k]
+        if tapers:
+            gt = list(par.gradt)
+            assert gt == [10, 25, 70, 86]
+            grads = [fo.taper_vert(nx, ny, par.dh, gt, src, rec, a)[1] for a in grads]
+            grads = [(a * tfiles[sfx]).astype(np.float32) for a, sfx in zip(grads, ("", "_u", "_rho"))]
         if precond:
             coeff = fo.taper_hor(nx, ny, par.dh, 3, 12, 2.0)
             grads = [(a * coeff).astype(np.float32) for a in grads]
