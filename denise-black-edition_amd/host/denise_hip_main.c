@@ -94,7 +94,7 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (p->sws_taper_grad_sources)
             die("SWS_TAPER_GRAD_SOURCES", "the cylindrical source taper is not built (vertical, horizontal and file tapers are)");
         if (p->grad_form != 1 && p->grad_form != 2) die("GRAD_FORM", "1 or 2 expected");
-        if (p->trkill) die("TRKILL", "trace killing is not built");
+        if (p->trkill && p->readrec == 2) die("TRKILL", "trace killing needs one receiver file (READREC = 1)");
     }
     if (p->mode == 1) {
         if (p->grad_method != 1 && p->grad_method != 2) die("GRAD_METHOD", "PCG (1) or L-BFGS (2) expected");
@@ -124,6 +124,7 @@ typedef struct run {
     float c_vp, c_vs, c_rho; /* the initial model's averages (FWI_PSV.c:540-580) */
     int time_filt, order; /* TIME_FILT of the current workflow stage: Butterworth ORDER, */
     float fc, fc_start;   /* low-pass FC (= FC_END of the stage), high-pass FC_START */
+    int *kill;            /* TRKILL: nsrc x ntr flags of TRKILL_FILE, or NULL */
     float hc[7];
     char name[4 * DNIO_STR];
 } run;
@@ -174,6 +175,41 @@ static int is_test_shot(const dnio_par *P, int ishot)
            (ishot - P->testshot_start) % P->testshot_incr == 0;
 }
 
+/* TRKILL (calc_res.c:61-87, 308, 544): the traces flagged for this shot
+ * get no adjoint source and no misfit; the misfit of the others is summed
+ * in the reference's order (per component, trace, time reversed) */
+static double kill_traces(run *r, int ishot, int test, float **diff)
+{
+    dnio_par *P = r->P;
+    const int *flag = r->kill + (size_t)(ishot - 1) * r->ntr;
+    double l2 = 0.0;
+    for (int c = 0; c < 3; c++) {
+        if (!diff[c]) continue;
+        for (int i = 0; i < r->ntr; i++) {
+            float *d = diff[c] + (size_t)i * P->nt;
+            if (flag[i] == 1) {
+                memset(d, 0, sizeof(float) * P->nt);
+                continue;
+            }
+            if (test)
+                for (int k = P->nt - 1; k >= 0; k--) l2 += 0.5 * d[k] * d[k];
+        }
+    }
+    dn_ok(dn_set_adjoint_sources(r->ctx, diff[0], diff[1], diff[2]), "adjoint sources");
+    return l2;
+}
+
+static int *read_kill(const char *path, int nsrc, int ntr)
+{
+    FILE *f = fopen(path, "r");
+    if (!f) die("TRKILL_FILE", " Trace kill file could not be opened!");
+    int *k = xmalloc(sizeof(int) * (size_t)nsrc * (ntr ? ntr : 1));
+    for (size_t q = 0; q < (size_t)nsrc * ntr; q++)
+        if (fscanf(f, "%d", &k[q]) != 1) die("TRKILL_FILE", "fewer than NSRC x NTR flags");
+    fclose(f);
+    return k;
+}
+
 /* residuals of the shot on the context against its observed data;
  * returns its L2 misfit (0 unless a test shot) */
 static double shot_residuals(run *r, int ishot, int test)
@@ -192,6 +228,16 @@ static double shot_residuals(run *r, int ishot, int test)
         }
     double l2 = 0.0;
     dn_ok(dn_calc_residuals(r->ctx, obs[0], obs[1], obs[2], test, &l2), "residuals");
+    if (r->kill) {
+        float *diff[3] = {NULL, NULL, NULL};
+        for (int c = 0; c < 3; c++)
+            if (use[c]) {
+                diff[c] = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
+                dn_ok(dn_get_adjoint_sources(r->ctx, c, diff[c]), "adjoint sources");
+            }
+        l2 = kill_traces(r, ishot, test, diff);
+        for (int c = 0; c < 3; c++) free(diff[c]);
+    }
     for (int c = 0; c < 3; c++) free(obs[c]);
     return l2;
 }
@@ -218,7 +264,10 @@ static double shot_rtm_sources(run *r, int ishot, int test)
                   "RTM adjoint source");
             l2 += l2c;
         }
-    dn_ok(dn_set_adjoint_sources(r->ctx, diff[0], diff[1], diff[2]), "adjoint sources");
+    if (r->kill)
+        l2 = kill_traces(r, ishot, test, diff);
+    else
+        dn_ok(dn_set_adjoint_sources(r->ctx, diff[0], diff[1], diff[2]), "adjoint sources");
     for (int c = 0; c < 3; c++) free(diff[c]);
     free(obs);
     free(syn);
@@ -622,6 +671,7 @@ int main(int argc, char **argv)
     dn_ok(dn_set_model(r.ctx, vp, vs, rho, taus, taup), "model upload");
     if (P.seismo || P.mode >= 1) io_ok(dnio_read_receivers(&P, 1, &r.ntr, &r.recpos), "REC_FILE");
     r.n_order = st ? st->n_order : 0;
+    if (P.mode >= 1 && P.trkill) r.kill = read_kill(P.trkill_file, r.nsrc, r.ntr);
 
     if (P.mode == 0)
         forward_pass(&r);
@@ -631,6 +681,7 @@ int main(int argc, char **argv)
         rtm(&r, st);
 
     dn_ctx_destroy(r.ctx);
+    free(r.kill);
     dnio_free(r.recpos);
     dnio_free(r.srcpos);
     free(vp);

@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 ITERMAX = 4
 
 
-@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers"])
+@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
     ORDER 4) filters the source signals and the observed data.  precond:
@@ -34,11 +34,13 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     pcg: the same preconditioning with conjugate gradients (GRAD_METHOD 1,
     PCG_BETA 2) instead of L-BFGS.  tapers: the vertical taper
     (SWS_TAPER_GRAD_VERT, columns 10..86, source and receiver cells zeroed)
-    and taper files (SWS_TAPER_FILE: TFILE.bin, _u, _rho)."""
+    and taper files (SWS_TAPER_FILE: TFILE.bin, _u, _rho).  trkill: every
+    third trace of shot 1 and every fifth of shot 2 killed (TRKILL)."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
     pcg = variant == "pcg"
     tapers = variant == "tapers"
+    trkill = variant == "trkill"
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
@@ -58,7 +60,14 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                       MISFIT_LOG_FILE="misfit.dat", SWS_TAPER_GRAD_HOR=int(precond),
                       GRADT="10, 25, 70, 86" if tapers else "3, 12, 60, 62",
                       EXP_TAPER_GRAD_HOR=2.0, GRAD_METHOD=1 if pcg else 2, PCG_BETA=2,
-                      SWS_TAPER_GRAD_VERT=int(tapers), SWS_TAPER_FILE=int(tapers))
+                      SWS_TAPER_GRAD_VERT=int(tapers), SWS_TAPER_FILE=int(tapers), TRKILL=int(trkill),
+                      TRKILL_FILE="kill.dat")
+    kill = np.zeros((2, len(receivers)), np.int32)
+    kill[0, ::3] = 1
+    kill[1, ::5] = 1
+    with open(os.path.join(tmp, "kill.dat"), "w") as f:
+        for row in kill:
+            f.write(" ".join(str(v) for v in row) + "\n")
     if tapers:
         trng = np.random.default_rng(5)
         tfiles = {sfx: trng.uniform(0.5, 1.5, (nx, ny)).astype(np.float32) for sfx in ("", "_u", "_rho")}
@@ -93,11 +102,28 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
         g.set_receivers(rec[:2])
         g.forward()
 
+    def residuals(i):
+        l2 = g.residuals(obs[i][0], obs[i][1], None, 1)
+        if not trkill:
+            return l2
+        diffs = []
+        l2 = 0.0
+        for comp in (0, 1):
+            d = g.adjoint_sources(comp)
+            d[kill[i - 1] == 1] = 0.0
+            for tr in range(d.shape[0]):
+                if kill[i - 1, tr] != 1:
+                    for k in range(d.shape[1] - 1, -1, -1):
+                        l2 += 0.5 * float(d[tr, k]) * float(d[tr, k])
+            diffs.append(d)
+        g.set_adjoint_sources(diffs[0], diffs[1], None)
+        return l2
+
     def misfit(tv, ts, tr, itest):
         l2 = 0.0
         for i in (1, 2):
             shot(i, (tv, ts, tr))
-            l2 += g.residuals(obs[i][0], obs[i][1], None, 1)
+            l2 += residuals(i)
         return l2
 
     lb = fo.PCG(nx, ny) if pcg else fo.LBFGS(nx, ny, par.nlbfgs)
@@ -109,7 +135,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
         l2sum = 0.0
         for i in (1, 2):
             shot(i, (vp, vs, rho))
-            l2sum += g.residuals(obs[i][0], obs[i][1], None, 1)
+            l2sum += residuals(i)
             g.adjoint()
             g.assemble()
             gs = [g.gradient(3 + k) for k in range(3)]
