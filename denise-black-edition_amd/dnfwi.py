@@ -61,6 +61,8 @@ def load():
         "dnfwi_pcg_direction": ([vp_, C.c_int, fp, fp, fp, fp], C.c_int),
         "dnfwi_taper_vert": ([C.c_int, C.c_int, C.c_float, C.POINTER(C.c_int), C.c_int, fp, C.c_int,
                               C.POINTER(C.c_int), fp, fp], C.c_int),
+        "dnfwi_taper_sources": ([C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, fp, fp, fp],
+                                C.c_int),
         "dnfwi_rtm_adjoint_source": ([fp, fp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, fp,
                                       C.POINTER(C.c_double)], C.c_int),
     }
@@ -241,4 +243,14 @@ def taper_vert(nx, ny, dh, gradt, srcpos, recpos, g=None):
     dnio._check(load().dnfwi_taper_vert(nx, ny, dh, gt, sp.shape[1], _p(sp), rp.shape[1],
                                         rp.ctypes.data_as(C.POINTER(C.c_int)), _p(coeff),
                                         _p(gg) if gg is not None else None))
+    return coeff, gg
+
+
+def taper_sources(nx, ny, dh, srtradius, srtshape, filtsize, srcpos, g=None):
+    """(coeff, g tapered) of dnfwi_taper_sources; srcpos 8 x nsrc (metres)."""
+    sp = np.ascontiguousarray(srcpos, np.float32)
+    coeff = np.empty((nx, ny), np.float32)
+    gg = _f(g).copy() if g is not None else None
+    dnio._check(load().dnfwi_taper_sources(nx, ny, dh, srtradius, srtshape, filtsize, sp.shape[1], _p(sp), _p(coeff),
+                                           _p(gg) if gg is not None else None))
     return coeff, gg

@@ -91,8 +91,6 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (p->ndt != 1) die("NDT", "FWI needs NDT = 1");
         if (p->idx != 1 || p->idy != 1) die("IDX/IDY", "FWI runs on every grid point (IDX = IDY = 1)");
         if (p->sws_taper_circular_per_shot) die("SWS_TAPER_CIRCULAR_PER_SHOT", "per-shot gradient tapers are not built");
-        if (p->sws_taper_grad_sources)
-            die("SWS_TAPER_GRAD_SOURCES", "the cylindrical source taper is not built (vertical, horizontal and file tapers are)");
         if (p->grad_form != 1 && p->grad_form != 2) die("GRAD_FORM", "1 or 2 expected");
         if (p->trkill && p->readrec == 2) die("TRKILL", "trace killing needs one receiver file (READREC = 1)");
     }
@@ -370,12 +368,13 @@ static double gradient_pass(run *r, float **g_sum)
 
 /* precond_PSV (src/PSV/precond_PSV.c:13-60): per stacked gradient (vp, vs,
  * rho) the vertical taper (taper_grad.c:46-186, TFILE_coeff_vert.bin), the
- * horizontal taper (188-322, TFILE_coeff_hor.bin) and the taper read from
- * TFILE.bin, TFILE_u.bin, TFILE_rho.bin (578-640), in that order */
+ * horizontal taper (188-322, TFILE_coeff_hor.bin), the cylindrical taper
+ * around the sources (328-577, TFILE_coeff_sources.bin) and the taper read
+ * from TFILE.bin, TFILE_u.bin, TFILE_rho.bin (578-640), in that order */
 static void precondition(run *r, float **g)
 {
     dnio_par *P = r->P;
-    if (!P->sws_taper_grad_vert && !P->sws_taper_grad_hor && !P->sws_taper_file) return;
+    if (!P->sws_taper_grad_vert && !P->sws_taper_grad_hor && !P->sws_taper_grad_sources && !P->sws_taper_file) return;
     float *coeff = xmalloc(sizeof(float) * r->ncell);
     for (int k = 0; k < 3; k++) {
         if (P->sws_taper_grad_vert) {
@@ -388,6 +387,13 @@ static void precondition(run *r, float **g)
             io_ok(dnfwi_taper_hor(P->nx, P->ny, P->dh, P->gradt[0], P->gradt[1], P->exp_taper_grad_hor, coeff, g[k]),
                   "horizontal gradient taper");
             snprintf(r->name, sizeof r->name, "%s_coeff_hor.bin", P->tfile);
+            write_plane(r->name, coeff, P->nx, P->ny);
+        }
+        if (P->sws_taper_grad_sources) {
+            io_ok(dnfwi_taper_sources(P->nx, P->ny, P->dh, P->srtradius, P->srtshape, P->filtsize, r->nsrc, r->srcpos,
+                                      coeff, g[k]),
+                  "source gradient taper");
+            snprintf(r->name, sizeof r->name, "%s_coeff_sources.bin", P->tfile);
             write_plane(r->name, coeff, P->nx, P->ny);
         }
         if (P->sws_taper_file) {

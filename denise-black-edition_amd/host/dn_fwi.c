@@ -565,3 +565,88 @@ int dnfwi_taper_vert(int nx, int ny, float dh, const int *gradt, int nsrc, const
     }
     return 0;
 }
+
+int dnfwi_taper_sources(int nx, int ny, float dh, float srtradius, int srtshape, int filtsize, int nsrc,
+                        const float *srcpos, float *coeff, float *g)
+{
+    if (!coeff || nx < 1 || ny < 1 || (nsrc > 0 && !srcpos) || (srtshape != 1 && srtshape != 2))
+        return dnio_fail_msg("dnfwi_taper_sources: bad argument (SRTSHAPE 1 or 2)");
+    int gp = (int)(ceil(2.0 * srtradius / dh));
+    if (gp < 5) gp = 5;
+    const size_t n = (size_t)nx * ny;
+    float *m = malloc(sizeof(float) * (size_t)gp * gp), *msum = malloc(sizeof(float) * n);
+    if (!m || !msum) {
+        free(m);
+        free(msum);
+        return dnio_fail_msg("dnfwi_taper_sources: out of host memory");
+    }
+#define M(j, i) m[(size_t)((j)-1) * gp + ((i)-1)]
+    /* taper_grad.c:346-371: the stencil m[j][i] */
+    const float a = 1.0;
+    const float maxrad = sqrt(2.0 * srtradius * srtradius);
+    for (int j = 1; j <= gp; j++)
+        for (int i = 1; i <= gp; i++) {
+            const float x = ((float)i - ((float)gp) / 2.0 - 0.5) * dh;
+            const float y = ((float)j - ((float)gp) / 2.0 - 0.5) * dh;
+            const float rad = sqrt(x * x + y * y);
+            if (srtshape == 1)
+                M(j, i) = erf(a * rad / maxrad);
+            else
+                M(j, i) = rad > 0 ? log(rad) : 0.0;
+        }
+    /* minimum_m / maximum_m (util.c:38-70): over all values */
+    float minm = M(1, 1);
+    for (int i = 1; i <= gp; i++)
+        for (int j = 1; j <= gp; j++)
+            if (M(j, i) < minm) minm = M(j, i);
+    for (int j = 1; j <= gp; j++)
+        for (int i = 1; i <= gp; i++) M(j, i) -= minm;
+    const float edge[4] = {M(1, gp / 2), M(gp / 2, 1), M(gp / 2, gp), M(gp, gp / 2)};
+    float maxm = edge[0];
+    for (int k = 1; k < 4; k++)
+        if (edge[k] > maxm) maxm = edge[k];
+    for (int j = 1; j <= gp; j++)
+        for (int i = 1; i <= gp; i++) {
+            M(j, i) /= maxm;
+            if (M(j, i) > 1.0) M(j, i) = 1.0;
+        }
+    /* taper_grad.c:387-419: the minimum over the sources' placements */
+    const int ijc = (int)(ceil((float)gp / 2));
+    for (size_t k = 0; k < n; k++) msum[k] = 1.0f;
+    for (int s = 0; s < nsrc; s++) {
+        const int i = (int)floor(srcpos[0 * nsrc + s] / dh + 0.5), j = (int)floor(srcpos[1 * nsrc + s] / dh + 0.5);
+        for (int iy = 1; iy <= gp; iy++)
+            for (int ix = 1; ix <= gp; ix++) {
+                const int xx = i + ix - ijc, yy = j + iy - ijc;
+                if ((xx < 1) || (xx > nx) || (yy < 1) || (yy > ny)) continue;
+                float *v = msum + (size_t)(xx - 1) * ny + (yy - 1);
+                if (*v > M(iy, ix)) *v = M(iy, ix);
+            }
+    }
+#undef M
+    /* taper_grad.c:456-479 */
+    float mn = msum[0];
+    for (size_t k = 0; k < n; k++)
+        if (msum[k] < mn) mn = msum[k];
+    for (size_t k = 0; k < n; k++) msum[k] -= mn;
+    float mx = msum[0];
+    for (size_t k = 0; k < n; k++)
+        if (msum[k] > mx) mx = msum[k];
+    for (size_t k = 0; k < n; k++) coeff[k] = msum[k] / mx;
+    if (g)
+        for (size_t k = 0; k < n; k++) g[k] *= coeff[k];
+    /* taper_grad.c:488-512: the boxes around the sources */
+    for (int s = 0; s < nsrc; s++) {
+        const int i1 = (int)floor(srcpos[0 * nsrc + s] / dh + 0.5), j1 = (int)floor(srcpos[1 * nsrc + s] / dh + 0.5);
+        for (int i = i1 - filtsize; i <= i1 + filtsize; i++)
+            for (int j = j1 - filtsize; j <= j1 + filtsize; j++) {
+                if (i < 1 || i > nx || j < 1 || j > ny) continue;
+                const size_t k = (size_t)(i - 1) * ny + (j - 1);
+                if (g) g[k] = 0.0f;
+                coeff[k] = 0.0f;
+            }
+    }
+    free(m);
+    free(msum);
+    return 0;
+}

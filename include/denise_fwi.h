@@ -148,6 +148,16 @@ int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_
 int dnfwi_taper_vert(int nx, int ny, float dh, const int *gradt, int nsrc, const float *srcpos, int ntr,
                      const int *recpos, float *coeff, float *g);
 
+/* Cylindrical gradient taper around the sources (src/taper_grad.c:328-577,
+ * SWS_TAPER_GRAD_SOURCES): a stencil of ceil(2 SRTRADIUS/DH) (at least 5)
+ * points with erf(r/rmax) (SRTSHAPE 1) or log(r) (2), shifted to 0, scaled
+ * by its edge maximum and clipped at 1, placed at every source's cell; the
+ * minimum over the sources, rescaled to [0, 1], is the taper (coeff,
+ * x-major), then 0 in the (2 FILTSIZE + 1)^2 box around each source.  g (may
+ * be NULL) is multiplied by the taper before the boxes are set. */
+int dnfwi_taper_sources(int nx, int ny, float dh, float srtradius, int srtshape, int filtsize, int nsrc,
+                        const float *srcpos, float *coeff, float *g);
+
 /* Adjoint source of reverse-time migration (MODE 2), the MODE 2 branch of
  * calc_res with LNORM = 2 (src/calc_res.c:416-446): per trace the first data
  * sample is replaced by the synthetic one, then GRAD_FORM 1 integrates

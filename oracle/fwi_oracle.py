@@ -20,6 +20,7 @@ Written from the reference's functions, one per function, keeping its
     src/PSV/grad_obj_psv.c:219-238;
   - taper_hor: src/taper_grad.c:188-322 (SWS_TAPER_GRAD_HOR);
   - taper_vert: src/taper_grad.c:46-186 (SWS_TAPER_GRAD_VERT);
+  - taper_sources: src/taper_grad.c:328-577 (SWS_TAPER_GRAD_SOURCES);
   - PCG: src/PCG.c:11-105 with descent, store_PCG_PSV and extract_PCG_PSV
     as src/PSV/FWI_PSV.c:637-662 calls them.
 Vectors are numpy float32 arrays of shape (nx, ny) (the model-file order).
@@ -400,4 +401,51 @@ def taper_vert(nx, ny, dh, gradt, srcpos, recpos, g):
         i, j = int(recpos[0, n]), int(recpos[1, n])
         if 1 <= i <= nx and 1 <= j <= ny:
             out[i - 1, j - 1] = 0.0
+    return coeff, out
+
+
+def taper_sources(nx, ny, dh, srtradius, srtshape, filtsize, srcpos, g):
+    """(coeff, tapered g) of the cylindrical source taper."""
+    import math
+    dh = F32(dh)
+    gp = int(math.ceil(2.0 * float(F32(srtradius)) / float(dh)))
+    gp = max(gp, 5)
+    maxrad = F32(math.sqrt(2.0 * float(F32(srtradius)) * float(F32(srtradius))))
+    m = np.zeros((gp + 1, gp + 1), F32)          # 1-based [j][i]
+    for j in range(1, gp + 1):
+        for i in range(1, gp + 1):
+            x = F32((float(F32(i)) - float(F32(gp)) / 2.0 - 0.5) * float(dh))
+            y = F32((float(F32(j)) - float(F32(gp)) / 2.0 - 0.5) * float(dh))
+            rad = F32(math.sqrt(float(F32(F32(x * x) + F32(y * y)))))
+            if srtshape == 1:
+                m[j, i] = F32(math.erf(float(F32(F32(F32(1.0) * rad) / maxrad))))
+            else:
+                m[j, i] = F32(math.log(float(rad))) if rad > 0 else F32(0.0)
+    core = m[1:, 1:]
+    core -= F32(core.min())
+    edge = [m[1, gp // 2], m[gp // 2, 1], m[gp // 2, gp], m[gp, gp // 2]]
+    maxm = F32(max(edge))
+    core /= maxm
+    core[core > 1.0] = 1.0
+    ijc = int(math.ceil(float(F32(F32(gp) / F32(2)))))
+    msum = np.ones((nx, ny), F32)
+    cells = []
+    for n in range(srcpos.shape[1]):
+        i = int(math.floor(float(F32(F32(srcpos[0, n]) / dh)) + 0.5))
+        j = int(math.floor(float(F32(F32(srcpos[1, n]) / dh)) + 0.5))
+        cells.append((i, j))
+        for iy in range(1, gp + 1):
+            for ix in range(1, gp + 1):
+                xx, yy = i + ix - ijc, j + iy - ijc
+                if 1 <= xx <= nx and 1 <= yy <= ny:
+                    msum[xx - 1, yy - 1] = min(msum[xx - 1, yy - 1], m[iy, ix])
+    msum = (msum - F32(msum.min())).astype(F32)
+    coeff = (msum / F32(msum.max())).astype(F32)
+    out = (g * coeff).astype(F32)
+    for (i1, j1) in cells:
+        for i in range(i1 - filtsize, i1 + filtsize + 1):
+            for j in range(j1 - filtsize, j1 + filtsize + 1):
+                if 1 <= i <= nx and 1 <= j <= ny:
+                    out[i - 1, j - 1] = 0.0
+                    coeff[i - 1, j - 1] = 0.0
     return coeff, out

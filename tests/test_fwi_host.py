@@ -409,3 +409,27 @@ def test_taper_vert():
     for (x, y) in ((10, 2), (20, 3), (31, 4), (12, 2), (20, 2), (33, 2)):
         mask[x - 1, y - 1] = False
     assert np.array_equal(got[mask], (g * coeff)[mask])
+
+
+@pytest.mark.parametrize("srtshape", [1, 2])
+def test_taper_sources(srtshape):
+    """dnfwi_taper_sources equals the restatement bit for bit; pins: the
+    taper lies in [0, 1], is 1 away from the sources and 0 in the FILTSIZE
+    box around each; for SRTSHAPE 1 it rises with the distance from a source
+    like erf(r/rmax) (monotone along a row through the source)."""
+    nx, ny, dh = 50, 30, 10.0
+    rng = np.random.default_rng(srtshape)
+    g = rnd(rng, shape=(nx, ny))
+    src = np.zeros((8, 2), F32)
+    src[0], src[1] = [120.0, 352.0], [100.0, 150.0]
+    coeff, got = dnfwi.taper_sources(nx, ny, dh, 40.0, srtshape, 1, src, g)
+    rc, rg = fo.taper_sources(nx, ny, dh, 40.0, srtshape, 1, src, g)
+    assert np.array_equal(coeff, rc) and np.array_equal(got, rg)
+    assert coeff.min() >= 0.0 and coeff.max() <= 1.0
+    assert np.all(coeff[:, 25:] == 1.0) and np.all(coeff[45:, :] == 1.0)
+    for (i, j) in ((12, 10), (35, 15)):
+        assert np.all(coeff[i - 2:i + 1, j - 2:j + 1] == 0.0)
+        assert np.all(got[i - 2:i + 1, j - 2:j + 1] == 0.0)
+    if srtshape == 1:
+        row = coeff[12:20, 9]          # x from the box edge outwards at the source depth
+        assert np.all(np.diff(row) >= 0.0)

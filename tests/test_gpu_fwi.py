@@ -33,8 +33,9 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     (SWS_TAPER_GRAD_HOR, GRADT 3..12, exponent 2) on the stacked ones.
     pcg: the same preconditioning with conjugate gradients (GRAD_METHOD 1,
     PCG_BETA 2) instead of L-BFGS.  tapers: the vertical taper
-    (SWS_TAPER_GRAD_VERT, columns 10..86, source and receiver cells zeroed)
-    and taper files (SWS_TAPER_FILE: TFILE.bin, _u, _rho).  trkill: every
+    (SWS_TAPER_GRAD_VERT, columns 10..86, source and receiver cells zeroed),
+    the cylindrical taper around the sources (SRTSHAPE 1, SRTRADIUS 40 m,
+    FILTSIZE 1) and taper files (SWS_TAPER_FILE: TFILE.bin, _u, _rho).  trkill: every
     third trace of shot 1 and every fifth of shot 2 killed (TRKILL)."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
@@ -61,6 +62,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                       GRADT="10, 25, 70, 86" if tapers else "3, 12, 60, 62",
                       EXP_TAPER_GRAD_HOR=2.0, GRAD_METHOD=1 if pcg else 2, PCG_BETA=2,
                       SWS_TAPER_GRAD_VERT=int(tapers), SWS_TAPER_FILE=int(tapers), TRKILL=int(trkill),
+                      SWS_TAPER_GRAD_SOURCES=int(tapers), SRTSHAPE=1, SRTRADIUS=40.0, FILTSIZE=1,
                       TRKILL_FILE="kill.dat")
     kill = np.zeros((2, len(receivers)), np.int32)
     kill[0, ::3] = 1
@@ -147,6 +149,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
             gt = list(par.gradt)
             assert gt == [10, 25, 70, 86]
             grads = [fo.taper_vert(nx, ny, par.dh, gt, src, rec, a)[1] for a in grads]
+            grads = [fo.taper_sources(nx, ny, par.dh, 40.0, 1, 1, src, a)[1] for a in grads]
             grads = [(a * tfiles[sfx]).astype(np.float32) for a, sfx in zip(grads, ("", "_u", "_rho"))]
         if precond:
             coeff = fo.taper_hor(nx, ny, par.dh, 3, 12, 2.0)
