@@ -90,7 +90,6 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
             die("workflow", "STF inversion, time windows, normalisation, offset mutes, envelope and ROWI are not built");
         if (p->ndt != 1) die("NDT", "FWI needs NDT = 1");
         if (p->idx != 1 || p->idy != 1) die("IDX/IDY", "FWI runs on every grid point (IDX = IDY = 1)");
-        if (p->sws_taper_circular_per_shot) die("SWS_TAPER_CIRCULAR_PER_SHOT", "per-shot gradient tapers are not built");
         if (p->grad_form != 1 && p->grad_form != 2) die("GRAD_FORM", "1 or 2 expected");
         if (p->trkill && p->readrec == 2) die("TRKILL", "trace killing needs one receiver file (READREC = 1)");
     }
@@ -350,6 +349,16 @@ static double gradient_pass(run *r, float **g_sum)
             io_ok(dnfwi_energy_precond(r->eprecond, P->nx, P->ny, P->dh, ws, wr, r->c_vp, r->c_vs, r->c_rho, gs[0],
                                        gs[1], gs[2], NULL),
                   "energy preconditioning");
+        }
+        if (P->sws_taper_circular_per_shot) {
+            /* grad_obj_psv.c:241-250, taper_grad_shot.c: the source taper of this shot's source alone */
+            float col[8], *coeff = xmalloc(sizeof(float) * r->ncell);
+            for (int q = 0; q < 8; q++) col[q] = r->srcpos[(size_t)q * r->nsrc + (ishot - 1)];
+            for (int k = 0; k < 3; k++)
+                io_ok(dnfwi_taper_sources(P->nx, P->ny, P->dh, P->srtradius, P->srtshape, P->filtsize, 1, col, coeff,
+                                          gs[k]),
+                      "per-shot gradient taper");
+            free(coeff);
         }
         for (int k = 0; k < 3; k++) {
             for (size_t q = 0; q < r->ncell; q++) g_sum[k][q] += gs[k][q];

@@ -35,7 +35,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     PCG_BETA 2) instead of L-BFGS.  tapers: the vertical taper
     (SWS_TAPER_GRAD_VERT, columns 10..86, source and receiver cells zeroed),
     the cylindrical taper around the sources (SRTSHAPE 1, SRTRADIUS 40 m,
-    FILTSIZE 1) and taper files (SWS_TAPER_FILE: TFILE.bin, _u, _rho).  trkill: every
+    FILTSIZE 1; also per shot, SWS_TAPER_CIRCULAR_PER_SHOT) and taper files
+    (SWS_TAPER_FILE: TFILE.bin, _u, _rho).  trkill: every
     third trace of shot 1 and every fifth of shot 2 killed (TRKILL)."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
@@ -63,6 +64,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                       EXP_TAPER_GRAD_HOR=2.0, GRAD_METHOD=1 if pcg else 2, PCG_BETA=2,
                       SWS_TAPER_GRAD_VERT=int(tapers), SWS_TAPER_FILE=int(tapers), TRKILL=int(trkill),
                       SWS_TAPER_GRAD_SOURCES=int(tapers), SRTSHAPE=1, SRTRADIUS=40.0, FILTSIZE=1,
+                      SWS_TAPER_CIRCULAR_PER_SHOT=int(tapers),
                       TRKILL_FILE="kill.dat")
     kill = np.zeros((2, len(receivers)), np.int32)
     kill[0, ::3] = 1
@@ -143,6 +145,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
             gs = [g.gradient(3 + k) for k in range(3)]
             if precond:
                 gs = list(fo.energy_precond(3, par.dh, g.energy(0), None, *c, *gs)[1:])
+            if tapers:        # the per-shot circular taper: this shot's source alone
+                gs = [fo.taper_sources(nx, ny, par.dh, 40.0, 1, 1, src[:, i - 1:i], a)[1] for a in gs]
             for k in range(3):
                 grads[k] += gs[k]
         if tapers:
