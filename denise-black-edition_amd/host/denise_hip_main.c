@@ -86,8 +86,9 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (st->eprecond != 0 && st->eprecond != 1 && st->eprecond != 3)
             die("EPRECOND", "energy preconditioning 1 and 3 are built");
         if (st->spatfilter) die("SPATFILTER", "spatial gradient filters are not built");
-        if (st->inv_stf || st->timewin || st->normalize || st->offset_mute || st->env || st->rowi)
-            die("workflow", "STF inversion, time windows, normalisation, offset mutes, envelope and ROWI are not built");
+        if (st->inv_stf || st->timewin || st->normalize || st->env || st->rowi)
+            die("workflow", "STF inversion, time windows, normalisation, envelope and ROWI are not built");
+        if (st->offset_mute < 0 || st->offset_mute > 2) die("OFFSET_MUTE", "0, 1 or 2 expected");
         if (p->ndt != 1) die("NDT", "FWI needs NDT = 1");
         if (p->idx != 1 || p->idy != 1) die("IDX/IDY", "FWI runs on every grid point (IDX = IDY = 1)");
         if (p->grad_form != 1 && p->grad_form != 2) die("GRAD_FORM", "1 or 2 expected");
@@ -122,6 +123,9 @@ typedef struct run {
     int time_filt, order; /* TIME_FILT of the current workflow stage: Butterworth ORDER, */
     float fc, fc_start;   /* low-pass FC (= FC_END of the stage), high-pass FC_START */
     int *kill;            /* TRKILL: nsrc x ntr flags of TRKILL_FILE, or NULL */
+    int offset_mute;      /* OFFSET_MUTE of the current stage, with OFFSETC */
+    float offsetc;
+    int *flag;            /* this shot's traces to leave out (ntr) */
     float hc[7];
     char name[4 * DNIO_STR];
 } run;
@@ -175,10 +179,33 @@ static int is_test_shot(const dnio_par *P, int ishot)
 /* TRKILL (calc_res.c:61-87, 308, 544): the traces flagged for this shot
  * get no adjoint source and no misfit; the misfit of the others is summed
  * in the reference's order (per component, trace, time reversed) */
+/* the traces calc_res leaves out for this shot (calc_res.c:302-316,
+ * 542-560): TRKILL flags, and OFFSET_MUTE 1 (offset >= OFFSETC) or 2
+ * (offset <= OFFSETC); 1 when any trace can be left out */
+static int shot_flags(run *r, int ishot)
+{
+    dnio_par *P = r->P;
+    if (!r->kill && !r->offset_mute) return 0;
+    if (!r->flag) r->flag = xmalloc(sizeof(int) * (size_t)(r->ntr ? r->ntr : 1));
+    const float xs = r->srcpos[0 * r->nsrc + ishot - 1], ys = r->srcpos[1 * r->nsrc + ishot - 1];
+    for (int i = 0; i < r->ntr; i++) {
+        int f = r->kill ? r->kill[(size_t)(ishot - 1) * r->ntr + i] == 1 : 0;
+        if (r->offset_mute) {
+            const float xr = r->recpos[i] * P->dh, yr = r->recpos[r->ntr + i] * P->dh;
+            const float offset = sqrt(((xs - xr) * (xs - xr)) + ((ys - yr) * (ys - yr)));
+            if ((r->offset_mute == 1) && (offset >= r->offsetc)) f = 1;
+            if ((r->offset_mute == 2) && (offset <= r->offsetc)) f = 1;
+        }
+        r->flag[i] = f;
+    }
+    return 1;
+}
+
 static double kill_traces(run *r, int ishot, int test, float **diff)
 {
     dnio_par *P = r->P;
-    const int *flag = r->kill + (size_t)(ishot - 1) * r->ntr;
+    (void)ishot;
+    const int *flag = r->flag;
     double l2 = 0.0;
     for (int c = 0; c < 3; c++) {
         if (!diff[c]) continue;
@@ -225,7 +252,7 @@ static double shot_residuals(run *r, int ishot, int test)
         }
     double l2 = 0.0;
     dn_ok(dn_calc_residuals(r->ctx, obs[0], obs[1], obs[2], test, &l2), "residuals");
-    if (r->kill) {
+    if (shot_flags(r, ishot)) {
         float *diff[3] = {NULL, NULL, NULL};
         for (int c = 0; c < 3; c++)
             if (use[c]) {
@@ -261,7 +288,7 @@ static double shot_rtm_sources(run *r, int ishot, int test)
                   "RTM adjoint source");
             l2 += l2c;
         }
-    if (r->kill)
+    if (shot_flags(r, ishot))
         l2 = kill_traces(r, ishot, test, diff);
     else
         dn_ok(dn_set_adjoint_sources(r->ctx, diff[0], diff[1], diff[2]), "adjoint sources");
@@ -486,6 +513,8 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
         r->fc = st.fc_end;
         r->fc_start = st.fc_start;
         r->eprecond = st.eprecond;
+        r->offset_mute = st.offset_mute;
+        r->offsetc = st.offsetc;
         for (int iter = 1; iter <= P->itermax; iter++) {
             printf("\n TDFWI ITERATION %d of %d (stage %d of %d)\n", iter, P->itermax, nstage, stagemax);
             fflush(stdout);
@@ -596,6 +625,8 @@ static void rtm(run *r, const dnio_stage *st)
 {
     dnio_par *P = r->P;
     r->eprecond = st->eprecond;
+    r->offset_mute = st->offset_mute;
+    r->offsetc = st->offsetc;
     r->c_vp = dnfwi_model_mean(r->vp, P->nx, P->ny); /* RTM_PSV.c:316-352 */
     r->c_vs = dnfwi_model_mean(r->vs, P->nx, P->ny);
     r->c_rho = dnfwi_model_mean(r->rho, P->nx, P->ny);
@@ -697,6 +728,7 @@ int main(int argc, char **argv)
 
     dn_ctx_destroy(r.ctx);
     free(r.kill);
+    free(r.flag);
     dnio_free(r.recpos);
     dnio_free(r.srcpos);
     free(vp);

@@ -37,7 +37,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     the cylindrical taper around the sources (SRTSHAPE 1, SRTRADIUS 40 m,
     FILTSIZE 1; also per shot, SWS_TAPER_CIRCULAR_PER_SHOT) and taper files
     (SWS_TAPER_FILE: TFILE.bin, _u, _rho).  trkill: every
-    third trace of shot 1 and every fifth of shot 2 killed (TRKILL)."""
+    third trace of shot 1 and every fifth of shot 2 killed (TRKILL), and the
+    offsets of 350 m and more muted (OFFSET_MUTE 1)."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
     pcg = variant == "pcg"
@@ -79,7 +80,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
             a.tofile(os.path.join(tmp, f"taper/t{sfx}.bin"))
     fcs, fce, order = (4.0, 12.0, 4) if time_filt else (0.0, 0.0, 6)
     inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=time_filt, fc_start=fcs, fc_end=fce, order=order,
-                           eprecond=3 if precond else 0)
+                           eprecond=3 if precond else 0, offset_mute=1 if trkill else 0, offsetc=350.0)
     _run(tmp, "fwi.inp", "wf.inp")
 
     par, p, m, rec, shots, src = _reference_run(tmp, "fwi.inp")
@@ -106,17 +107,27 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
         g.set_receivers(rec[:2])
         g.forward()
 
+    def flags(i):
+        f = kill[i - 1] == 1
+        xs, ys = np.float32(src[0, i - 1]), np.float32(src[1, i - 1])
+        xr = (rec[0].astype(np.float32) * np.float32(par.dh)).astype(np.float32)
+        yr = (rec[1].astype(np.float32) * np.float32(par.dh)).astype(np.float32)
+        off = np.sqrt((((xs - xr) * (xs - xr)).astype(np.float32) +
+                       ((ys - yr) * (ys - yr)).astype(np.float32)).astype(np.float64)).astype(np.float32)
+        return f | (off >= np.float32(350.0))
+
     def residuals(i):
         l2 = g.residuals(obs[i][0], obs[i][1], None, 1)
         if not trkill:
             return l2
+        fl = flags(i)
         diffs = []
         l2 = 0.0
         for comp in (0, 1):
             d = g.adjoint_sources(comp)
-            d[kill[i - 1] == 1] = 0.0
+            d[fl] = 0.0
             for tr in range(d.shape[0]):
-                if kill[i - 1, tr] != 1:
+                if not fl[tr]:
                     for k in range(d.shape[1] - 1, -1, -1):
                         l2 += 0.5 * float(d[tr, k]) * float(d[tr, k])
             diffs.append(d)
