@@ -83,8 +83,6 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
     if (p->mode == 1 || p->mode == 2) {
         if (!st) die("workflow", "MODE 1 and 2 need a workflow file");
         if (st->lnorm != 2) die("LNORM", "only the L2 misfit (LNORM = 2) is built");
-        if (st->eprecond != 0 && st->eprecond != 1 && st->eprecond != 3)
-            die("EPRECOND", "energy preconditioning 1 and 3 are built");
         if (st->spatfilter) die("SPATFILTER", "spatial gradient filters are not built");
         if (st->inv_stf || st->timewin || st->normalize || st->env || st->rowi)
             die("workflow", "STF inversion, time windows, normalisation, envelope and ROWI are not built");
@@ -512,7 +510,8 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
         r->order = st.order;
         r->fc = st.fc_end;
         r->fc_start = st.fc_start;
-        r->eprecond = st.eprecond;
+        /* grad_obj_psv.c:71, 219: the P/SV path acts on EPRECOND 1 and 3 only */
+        r->eprecond = (st.eprecond == 1 || st.eprecond == 3) ? st.eprecond : 0;
         r->offset_mute = st.offset_mute;
         r->offsetc = st.offsetc;
         for (int iter = 1; iter <= P->itermax; iter++) {
@@ -624,7 +623,7 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
 static void rtm(run *r, const dnio_stage *st)
 {
     dnio_par *P = r->P;
-    r->eprecond = st->eprecond;
+    r->eprecond = (st->eprecond == 1 || st->eprecond == 3) ? st->eprecond : 0;
     r->offset_mute = st->offset_mute;
     r->offsetc = st->offsetc;
     r->c_vp = dnfwi_model_mean(r->vp, P->nx, P->ny); /* RTM_PSV.c:316-352 */
