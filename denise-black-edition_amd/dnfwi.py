@@ -55,6 +55,7 @@ def load():
         "dnfwi_energy_precond": ([C.c_int, C.c_int, C.c_int, C.c_float, fp, fp, C.c_float, C.c_float, C.c_float,
                                   fp, fp, fp, fp], C.c_int),
         "dnfwi_taper_hor": ([C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, fp, fp], C.c_int),
+        "dnfwi_median_filter": ([C.c_int, C.c_int, C.c_int, fp], C.c_int),
         "dnfwi_pcg_create": ([C.c_int, C.c_int, C.POINTER(vp_)], C.c_int),
         "dnfwi_pcg_destroy": ([vp_], None),
         "dnfwi_pcg_reset": ([vp_], None),
@@ -200,6 +201,13 @@ def taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper):
     coeff = np.empty((nx, ny), np.float32)
     dnio._check(load().dnfwi_taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper, _p(coeff), None))
     return coeff
+
+
+def median_filter(a, filt_size):
+    """dnfwi_median_filter on a copy of the (nx, ny) array a (MODEL_FILTER 1)."""
+    out = np.array(a, np.float32, copy=True, order="C")
+    dnio._check(load().dnfwi_median_filter(out.shape[0], out.shape[1], filt_size, _p(out)))
+    return out
 
 
 class PCG:

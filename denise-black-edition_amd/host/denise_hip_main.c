@@ -95,7 +95,7 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
     if (p->mode == 1) {
         if (p->grad_method != 1 && p->grad_method != 2) die("GRAD_METHOD", "PCG (1) or L-BFGS (2) expected");
         if (p->grad_method == 1 && (p->pcg_beta < 1 || p->pcg_beta > 4)) die("PCG_BETA", "1 to 4 expected");
-        if (p->model_filter || p->grad_filter) die("MODEL_FILTER/GRAD_FILTER", "model and gradient filters are not built");
+        if (p->grad_filter) die("GRAD_FILTER", "the gradient filter is not built");
         if (p->gravity) die("GRAVITY", "gravity modelling is not built");
         if (p->nlbfgs < 1) die("NLBFGS", "at least 1 expected");
         if (p->invmat1 != 1) die("INVMAT1", "the vp/vs/rho parameterisation (INVMAT1 = 1) is built");
@@ -577,6 +577,14 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
             snprintf(name[1], sizeof name[1], "%s_vs.bin", P->inv_modelfile);
             snprintf(name[2], sizeof name[2], "%s_rho.bin", P->inv_modelfile);
             write_models(name[0], name[1], name[2], vp, vs, rho, nx, ny);
+            /* FWI_PSV.c:752, smooth_model.c:37: MODEL_FILTER 1 replaces the
+             * updated models (not the files just written) by their medians */
+            if (P->model_filter == 1 && P->filt_size != 0) {
+                float *mods[3] = {vp, vs, rho};
+                for (int k = 0; k < 3; k++)
+                    if (dnfwi_median_filter(nx, ny, P->filt_size, mods[k])) die("MODEL_FILTER", dnio_last_error());
+                printf(" models smoothed with a %d-point median filter\n", abs(P->filt_size) | 1);
+            }
             /* FWI_PSV.c:781-827: stage end on the misfit change over two
              * iterations (iterations before the first count as misfit 0,
              * which never ends a stage) or a failed step search */

@@ -440,6 +440,47 @@ int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_
     return 0;
 }
 
+int dnfwi_median_filter(int nx, int ny, int filt_size, float *a)
+{
+    if (!a || nx < 1 || ny < 1) return dnio_fail_msg("dnfwi_median_filter: bad argument");
+    if (filt_size == 0) return 0;
+    int fs = abs(filt_size);
+    if (fs % 2 == 0) fs++;
+    const int hfs = fs / 2, n = fs * fs;
+    const size_t ncell = (size_t)nx * ny;
+    float *src = malloc(sizeof(float) * ncell), *win = malloc(sizeof(float) * (size_t)n);
+    if (!src || !win) {
+        free(src);
+        free(win);
+        return dnio_fail_msg("dnfwi_median_filter: out of host memory");
+    }
+    memcpy(src, a, sizeof(float) * ncell);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            /* the window row by row (y offset outer, x offset inner), each
+             * value inserted into a descending run; with an odd count the
+             * middle element is the median whatever the order of ties */
+            int k = 0;
+            for (int dj = -hfs; dj <= hfs; dj++) {
+                const int jj = j + dj < 0 ? 0 : (j + dj >= ny ? ny - 1 : j + dj);
+                for (int di = -hfs; di <= hfs; di++) {
+                    const int ii = i + di < 0 ? 0 : (i + di >= nx ? nx - 1 : i + di);
+                    const float v = src[(size_t)ii * ny + jj];
+                    int m = k++;
+                    while (m > 0 && win[m - 1] < v) {
+                        win[m] = win[m - 1];
+                        m--;
+                    }
+                    win[m] = v;
+                }
+            }
+            a[(size_t)i * ny + j] = win[n / 2];
+        }
+    free(src);
+    free(win);
+    return 0;
+}
+
 struct dnfwi_pcg {
     size_t ncell;
     float *old, *dir, *nw; /* PCG_old, PCG_dir, PCG_new: [vp | vs | rho] */

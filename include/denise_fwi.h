@@ -137,6 +137,14 @@ int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws
  * taper, and g (may be NULL) is multiplied by it. */
 int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_taper, float *coeff, float *g);
 
+/* Median model filter (src/smooth_model.c:37-196 with src/median2d.c,
+ * MODEL_FILTER 1, applied by FWI_PSV.c:752 to vp, vs and rho after each
+ * model update): every cell of the x-major nx x ny array a is replaced by
+ * the median of the |filt_size| x |filt_size| window around it (an even
+ * size is widened by one), the array extended at its edges by repeating the
+ * edge cells.  filt_size 0 leaves a unchanged. */
+int dnfwi_median_filter(int nx, int ny, int filt_size, float *a);
+
 /* Vertical gradient taper (src/taper_grad.c:46-186, SWS_TAPER_GRAD_VERT):
  * along x, 0 up to column GRADT1 and from GRADT4, Gaussian ramps
  * exp(-(3 (k - L1)/(L1/2))^2 / 2) up from GRADT1 (L1 = GRADT2 - GRADT1) and

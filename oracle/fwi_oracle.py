@@ -23,6 +23,8 @@ Written from the reference's functions, one per function, keeping its
   - taper_sources: src/taper_grad.c:328-577 (SWS_TAPER_GRAD_SOURCES);
   - PCG: src/PCG.c:11-105 with descent, store_PCG_PSV and extract_PCG_PSV
     as src/PSV/FWI_PSV.c:637-662 calls them.
+  - median_filter: src/smooth_model.c:37-196 with src/median2d.c:9-44
+    (MODEL_FILTER 1, called by src/PSV/FWI_PSV.c:752).
 Vectors are numpy float32 arrays of shape (nx, ny) (the model-file order).
 The pins of these functions against what the mathematics fixes (BFGS
 matrix recursion, parabola vertex, limits) are in tests/test_fwi_host.py.
@@ -295,6 +297,21 @@ def energy_precond(eprecond, dh, Ws, Wr, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho):
         gs[...] = 0.0
     gr = (g_rho / ((We * c_rho).astype(F32) * c_rho).astype(F32)).astype(F32)
     return We, gv, gs, gr
+
+
+def median_filter(a, filt_size):
+    """Median of the |filt_size|^2 window (an even size widened by one)
+    around each cell of the (nx, ny) array, the array extended by repeating
+    its edge cells (smooth_model.c:40-136); filt_size 0 returns a copy."""
+    a = np.asarray(a, np.float32)
+    if filt_size == 0:
+        return a.copy()
+    fs = abs(filt_size)
+    fs += 1 - fs % 2
+    h = fs // 2
+    padded = np.pad(a, h, mode="edge")
+    win = np.lib.stride_tricks.sliding_window_view(padded, (fs, fs))
+    return np.median(win.reshape(a.shape[0], a.shape[1], fs * fs), axis=2).astype(np.float32)
 
 
 def taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper):

@@ -433,3 +433,48 @@ def test_taper_sources(srtshape):
     if srtshape == 1:
         row = coeff[12:20, 9]          # x from the box edge outwards at the source depth
         assert np.all(np.diff(row) >= 0.0)
+
+
+def _median_brute(a, fs):
+    """The median filter by its definition: clamp every window index into
+    the array, sort the fs*fs values, take the middle one."""
+    nx, ny = a.shape
+    h = fs // 2
+    out = np.empty_like(a)
+    for i in range(nx):
+        for j in range(ny):
+            v = sorted(float(a[min(max(i + di, 0), nx - 1), min(max(j + dj, 0), ny - 1)])
+                       for di in range(-h, h + 1) for dj in range(-h, h + 1))
+            out[i, j] = v[len(v) // 2]
+    return out
+
+
+@pytest.mark.parametrize("filt_size", [0, 1, 2, 3, -4, 5, 7])
+def test_median_filter(filt_size):
+    """dnfwi_median_filter equals the restatement bit for bit; pins: the
+    definition (clamped window, sorted, middle value) on a small array,
+    an even size acting as the next odd one, a negative size as its
+    absolute value, 0 and 1 leaving the array unchanged."""
+    rng = np.random.default_rng(11 + abs(filt_size))
+    a = rng.uniform(1500.0, 4500.0, (37, 23)).astype(np.float32)
+    a[5:9, 3:6] = 2000.0          # ties
+    got = dnfwi.median_filter(a, filt_size)
+    assert np.array_equal(got, fo.median_filter(a, filt_size))
+    fs = abs(filt_size) + (1 - abs(filt_size) % 2) if filt_size else 1
+    small = a[:9, :7]
+    assert np.array_equal(fo.median_filter(small, filt_size), _median_brute(small, fs))
+    if filt_size in (0, 1):
+        assert np.array_equal(got, a)
+
+
+def test_median_filter_properties():
+    """Constant fields and straight step edges pass unchanged, an isolated
+    spike is removed, and the input array of the binding is not modified."""
+    a = np.full((20, 15), 3000.0, np.float32)
+    assert np.array_equal(dnfwi.median_filter(a, 5), a)
+    step = np.where(np.arange(20)[:, None] < 8, 2000.0, 3500.0).astype(np.float32) * np.ones((1, 15), np.float32)
+    assert np.array_equal(dnfwi.median_filter(step, 5), step)
+    spike = a.copy()
+    spike[10, 7] = 9000.0
+    assert np.array_equal(dnfwi.median_filter(spike, 3), a)
+    assert spike[10, 7] == 9000.0

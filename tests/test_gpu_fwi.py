@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 ITERMAX = 4
 
 
-@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill"])
+@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill", "model_filter"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
     ORDER 4) filters the source signals and the observed data.  precond:
@@ -38,12 +38,15 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     FILTSIZE 1; also per shot, SWS_TAPER_CIRCULAR_PER_SHOT) and taper files
     (SWS_TAPER_FILE: TFILE.bin, _u, _rho).  trkill: every
     third trace of shot 1 and every fifth of shot 2 killed (TRKILL), and the
-    offsets of 350 m and more muted (OFFSET_MUTE 1)."""
+    offsets of 350 m and more muted (OFFSET_MUTE 1).  model_filter: the
+    updated models replaced by their 5 x 5 medians (MODEL_FILTER 1,
+    FILT_SIZE 4, widened to 5) after INV_MODELFILE is written."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
     pcg = variant == "pcg"
     tapers = variant == "tapers"
     trkill = variant == "trkill"
+    mfilt = variant == "model_filter"
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
@@ -66,7 +69,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                       SWS_TAPER_GRAD_VERT=int(tapers), SWS_TAPER_FILE=int(tapers), TRKILL=int(trkill),
                       SWS_TAPER_GRAD_SOURCES=int(tapers), SRTSHAPE=1, SRTRADIUS=40.0, FILTSIZE=1,
                       SWS_TAPER_CIRCULAR_PER_SHOT=int(tapers),
-                      TRKILL_FILE="kill.dat")
+                      TRKILL_FILE="kill.dat", MODEL_FILTER=int(mfilt), FILT_SIZE=4)
     kill = np.zeros((2, len(receivers)), np.int32)
     kill[0, ::3] = 1
     kill[1, ::5] = 1
@@ -186,6 +189,9 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
             lines.append("%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %d " % (
                 eps, *ls["epst"], *ls["L2t"], l2sum, 1))
         vp, vs, rho, _ = fo.mat_change_test(upd, it, eps, vp, vs, rho, *d)
+        written = (vp, vs, rho)
+        if mfilt:
+            vp, vs, rho = (fo.median_filter(a, 4) for a in written)
         print(f"[fwi] iteration {it}: misfit {l2sum:.6e}, step {float(eps):.4e}, "
               f"{ls['nevals']} test forward runs", flush=True)
         # the stage ends (FWI_PSV.c:781-827: misfit change over two iterations
@@ -197,12 +203,15 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
             got = np.fromfile(os.path.join(tmp, f"model/inv_{name}_stage_1_it_{it}.bin"), np.float32)
             assert np.array_equal(got.reshape(nx, ny), a), (it, name)
     g.close()
-    for name, a in (("vp", vp), ("vs", vs), ("rho", rho)):
+    for name, a in zip(("vp", "vs", "rho"), written):
         got = np.fromfile(os.path.join(tmp, f"model/inv_{name}.bin"), np.float32).reshape(nx, ny)
         assert np.array_equal(got, a), name
     log = open(os.path.join(tmp, "misfit.dat")).read().splitlines()
     assert log == lines[:len(log)] and len(log) == len(lines)
-    assert l2s[-1] < l2s[0]
+    if mfilt:   # a 5 x 5 median also flattens this small model's layering, so no misfit decrease is asked
+        assert not np.array_equal(vp, written[0])
+    else:
+        assert l2s[-1] < l2s[0]
     assert not np.array_equal(vp, m[0])
 
 
