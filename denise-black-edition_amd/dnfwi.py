@@ -56,6 +56,8 @@ def load():
                                   fp, fp, fp, fp], C.c_int),
         "dnfwi_taper_hor": ([C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, fp, fp], C.c_int),
         "dnfwi_median_filter": ([C.c_int, C.c_int, C.c_int, fp], C.c_int),
+        "dnfwi_gauss_filter": ([C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp,
+                                C.POINTER(C.c_int)], C.c_int),
         "dnfwi_pcg_create": ([C.c_int, C.c_int, C.POINTER(vp_)], C.c_int),
         "dnfwi_pcg_destroy": ([vp_], None),
         "dnfwi_pcg_reset": ([vp_], None),
@@ -208,6 +210,16 @@ def median_filter(a, filt_size):
     out = np.array(a, np.float32, copy=True, order="C")
     dnio._check(load().dnfwi_median_filter(out.shape[0], out.shape[1], filt_size, _p(out)))
     return out
+
+
+def gauss_filter(g, dh, wd_damp, c_vs, fc_end):
+    """dnfwi_gauss_filter on a copy of the (nx, ny) gradient g (SPATFILTER 3);
+    returns (filtered, width)."""
+    out = np.array(g, np.float32, copy=True, order="C")
+    w = C.c_int(0)
+    dnio._check(load().dnfwi_gauss_filter(out.shape[0], out.shape[1], dh, wd_damp, c_vs, fc_end, _p(out),
+                                          C.byref(w)))
+    return out, w.value
 
 
 class PCG:

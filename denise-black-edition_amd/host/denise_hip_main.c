@@ -83,7 +83,10 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
     if (p->mode == 1 || p->mode == 2) {
         if (!st) die("workflow", "MODE 1 and 2 need a workflow file");
         if (st->lnorm != 2) die("LNORM", "only the L2 misfit (LNORM = 2) is built");
-        if (st->spatfilter) die("SPATFILTER", "spatial gradient filters are not built");
+        /* smooth_grad.c:17-33: 1 wavenumber damping, 2 smooth2, 3 Gaussian, 4 velocity-dependent Gaussian */
+        if (st->spatfilter == 1 || st->spatfilter == 2 || st->spatfilter == 4)
+            die("SPATFILTER", "the Gaussian gradient filter (3) is built; 1, 2 and 4 are not");
+        if (st->spatfilter == 3 && !(st->fc_end > 0.0f)) die("SPATFILTER", "SPATFILTER 3 needs FC_END > 0");
         if (st->inv_stf || st->timewin || st->normalize || st->env || st->rowi)
             die("workflow", "STF inversion, time windows, normalisation, envelope and ROWI are not built");
         if (st->offset_mute < 0 || st->offset_mute > 2) die("OFFSET_MUTE", "0, 1 or 2 expected");
@@ -95,7 +98,7 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
     if (p->mode == 1) {
         if (p->grad_method != 1 && p->grad_method != 2) die("GRAD_METHOD", "PCG (1) or L-BFGS (2) expected");
         if (p->grad_method == 1 && (p->pcg_beta < 1 || p->pcg_beta > 4)) die("PCG_BETA", "1 to 4 expected");
-        if (p->grad_filter) die("GRAD_FILTER", "the gradient filter is not built");
+        /* GRAD_FILTER is read by the SH inversion only (FWI_SH.c:635); P/SV ignores it */
         if (p->gravity) die("GRAVITY", "gravity modelling is not built");
         if (p->nlbfgs < 1) die("NLBFGS", "at least 1 expected");
         if (p->invmat1 != 1) die("INVMAT1", "the vp/vs/rho parameterisation (INVMAT1 = 1) is built");
@@ -529,7 +532,13 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
             }
             dn_ok(dn_set_model(r->ctx, vp, vs, rho, taus, taup), "model upload");
             const double L2sum = gradient_pass(r, g);
-            precondition(r, g); /* FWI_PSV.c:628-634: smooth_grad (off) and precond_PSV */
+            if (st.spatfilter == 3) { /* FWI_PSV.c:628-631: smooth_grad before precond_PSV */
+                int w = 0;
+                for (int k = 0; k < 3; k++)
+                    io_ok(dnfwi_gauss_filter(nx, ny, P->dh, st.wd_damp, c_vs, st.fc_end, g[k], &w), "SPATFILTER 3");
+                printf(" gradients smoothed with a %d-point Gaussian (%.2f m)\n", w, w * P->dh);
+            }
+            precondition(r, g); /* FWI_PSV.c:634: precond_PSV */
             const char *suffix[3] = {"vp", "vs", "rho"};
             for (int k = 0; k < 3; k++) {
                 snprintf(r->name, sizeof r->name, "%s_grad_%s", P->jacobian, suffix[k]);

@@ -440,6 +440,56 @@ int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_
     return 0;
 }
 
+int dnfwi_gauss_filter(int nx, int ny, float dh, float wd_damp, float c_vs, float fc_end, float *g, int *width)
+{
+    if (!g || nx < 1 || ny < 1 || !(dh > 0.0f) || !(fc_end > 0.0f))
+        return dnio_fail_msg("dnfwi_gauss_filter: bad argument (FC_END and DH must be positive)");
+    const float lam = c_vs / fc_end;
+    int fs = (int)round((wd_damp * lam) / dh);
+    if (width) *width = fs;
+    if (fs == 0) return 0;
+    if (fs % 2 == 0) fs += fs > 0 ? 1 : -1;
+    if (width) *width = fs;
+    const int hfs = abs(fs) / 2, n = 2 * hfs + 1;
+    const float sigma = hfs / 2; /* integer division, as the reference */
+    const float s = 2.0 * sigma * sigma;
+    const size_t ncell = (size_t)nx * ny;
+    float *k = malloc(sizeof(float) * (size_t)n * n), *src = malloc(sizeof(float) * ncell);
+    if (!k || !src) {
+        free(k);
+        free(src);
+        return dnio_fail_msg("dnfwi_gauss_filter: out of host memory");
+    }
+    /* k[(di + hfs) * n + (dj + hfs)]: di along x, dj along y */
+    float sum = 0.0f;
+    for (int di = -hfs; di <= hfs; di++)
+        for (int dj = -hfs; dj <= hfs; dj++) {
+            const float e = -((di * di) / s) - ((dj * dj) / s);
+            k[(di + hfs) * n + dj + hfs] = exp(e);
+            sum += k[(di + hfs) * n + dj + hfs];
+        }
+    /* the reference normalises over 1..width, which is nothing for a
+     * negative width */
+    if (fs > 0)
+        for (int q = 0; q < n * n; q++) k[q] /= sum;
+    memcpy(src, g, sizeof(float) * ncell);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            float conv = 0.0f;
+            for (int di = -hfs; di <= hfs; di++) {
+                const int ii = i + di < 0 ? 0 : (i + di >= nx ? nx - 1 : i + di);
+                for (int dj = -hfs; dj <= hfs; dj++) {
+                    const int jj = j + dj < 0 ? 0 : (j + dj >= ny ? ny - 1 : j + dj);
+                    conv += src[(size_t)ii * ny + jj] * k[(di + hfs) * n + dj + hfs];
+                }
+            }
+            g[(size_t)i * ny + j] = conv;
+        }
+    free(k);
+    free(src);
+    return 0;
+}
+
 int dnfwi_median_filter(int nx, int ny, int filt_size, float *a)
 {
     if (!a || nx < 1 || ny < 1) return dnio_fail_msg("dnfwi_median_filter: bad argument");

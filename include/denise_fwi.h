@@ -137,6 +137,17 @@ int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws
  * taper, and g (may be NULL) is multiplied by it. */
 int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_taper, float *coeff, float *g);
 
+/* Gaussian gradient filter (src/gauss_filt.c:9-208, SPATFILTER 3, applied by
+ * FWI_PSV.c:628-631 through smooth_grad.c to the three stacked gradients
+ * before the tapers): the width is round(WD_DAMP * (C_vs / FC_END) / DH)
+ * points, an even width widened by one away from 0, half width h = |width|/2;
+ * the kernel exp(-(di^2 / s) - (dj^2 / s)) with s = 2 sigma^2, sigma = h / 2
+ * in integer division, divided by its sum (the reference starts that sum
+ * uninitialised; here it starts at 0); the x-major nx x ny array g is
+ * replaced by its convolution with the kernel, edges repeated.  Width 0
+ * leaves g unchanged.  *width (may be NULL) gets the width used. */
+int dnfwi_gauss_filter(int nx, int ny, float dh, float wd_damp, float c_vs, float fc_end, float *g, int *width);
+
 /* Median model filter (src/smooth_model.c:37-196 with src/median2d.c,
  * MODEL_FILTER 1, applied by FWI_PSV.c:752 to vp, vs and rho after each
  * model update): every cell of the x-major nx x ny array a is replaced by

@@ -23,6 +23,8 @@ Written from the reference's functions, one per function, keeping its
   - taper_sources: src/taper_grad.c:328-577 (SWS_TAPER_GRAD_SOURCES);
   - PCG: src/PCG.c:11-105 with descent, store_PCG_PSV and extract_PCG_PSV
     as src/PSV/FWI_PSV.c:637-662 calls them.
+  - gauss_filter: src/gauss_filt.c:9-208 (SPATFILTER 3, through
+    src/smooth_grad.c:17-33 at src/PSV/FWI_PSV.c:628-631);
   - median_filter: src/smooth_model.c:37-196 with src/median2d.c:9-44
     (MODEL_FILTER 1, called by src/PSV/FWI_PSV.c:752).
 Vectors are numpy float32 arrays of shape (nx, ny) (the model-file order).
@@ -297,6 +299,43 @@ def energy_precond(eprecond, dh, Ws, Wr, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho):
         gs[...] = 0.0
     gr = (g_rho / ((We * c_rho).astype(F32) * c_rho).astype(F32)).astype(F32)
     return We, gv, gs, gr
+
+
+def gauss_filter(g, dh, wd_damp, c_vs, fc_end):
+    """The Gaussian gradient smoother: (filtered (nx, ny) float32, width).
+    Kernel values are float32 of exp (C libm through math.exp) of a float32
+    exponent; the convolution adds, per cell, the products of the
+    edge-extended array and the kernel in the order x offset outer, y offset
+    inner, in float32 (gauss_filt.c:117-147)."""
+    import math
+    g = np.asarray(g, np.float32)
+    lam = F32(F32(c_vs) / F32(fc_end))
+    x = float(F32(F32(F32(wd_damp) * lam) / F32(dh)))
+    fs = int(math.copysign(math.floor(abs(x) + 0.5), x))      # C round(): halves away from 0
+    if fs == 0:
+        return g.copy(), 0
+    if fs % 2 == 0:
+        fs += 1 if fs > 0 else -1
+    h = abs(fs) // 2
+    sigma = F32(h // 2)
+    s = F32(2.0 * float(sigma) * float(sigma))
+    k = np.empty((2 * h + 1, 2 * h + 1), np.float32)
+    tot = F32(0.0)
+    for di in range(-h, h + 1):
+        for dj in range(-h, h + 1):
+            with np.errstate(invalid="ignore", divide="ignore"):
+                e = F32(-(F32(di * di) / s) - (F32(dj * dj) / s))
+            k[di + h, dj + h] = F32(math.exp(float(e))) if not np.isnan(e) else np.float32(np.nan)
+            tot = F32(tot + k[di + h, dj + h])
+    if fs > 0:
+        k = (k / tot).astype(np.float32)
+    nx, ny = g.shape
+    pad = np.pad(g, h, mode="edge")
+    conv = np.zeros_like(g)
+    for di in range(-h, h + 1):
+        for dj in range(-h, h + 1):
+            conv = (conv + pad[h + di:h + di + nx, h + dj:h + dj + ny] * k[di + h, dj + h]).astype(np.float32)
+    return conv, fs
 
 
 def median_filter(a, filt_size):

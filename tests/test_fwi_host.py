@@ -478,3 +478,47 @@ def test_median_filter_properties():
     spike[10, 7] = 9000.0
     assert np.array_equal(dnfwi.median_filter(spike, 3), a)
     assert spike[10, 7] == 9000.0
+
+
+@pytest.mark.parametrize("wd_damp,c_vs,fc_end,dh", [(0.5, 2000.0, 10.0, 10.0), (0.3, 1800.0, 7.0, 5.0),
+                                                    (0.25, 2400.0, 12.0, 10.0), (0.01, 2000.0, 10.0, 10.0),
+                                                    (-0.45, 2000.0, 10.0, 10.0)])
+def test_gauss_filter(wd_damp, c_vs, fc_end, dh):
+    """dnfwi_gauss_filter equals the restatement bit for bit (widths 11, 15,
+    5 (from 4), 0 and -9 (no normalisation, as in the reference))."""
+    rng = np.random.default_rng(3)
+    g = rng.standard_normal((41, 29)).astype(np.float32) * np.float32(1e-9)
+    got, w = dnfwi.gauss_filter(g, dh, wd_damp, c_vs, fc_end)
+    want, w2 = fo.gauss_filter(g, dh, wd_damp, c_vs, fc_end)
+    assert w == w2
+    assert np.array_equal(got, want)
+    if w == 0:
+        assert np.array_equal(got, g)
+
+
+def test_gauss_filter_pins():
+    """Width round(WD_DAMP C_vs / FC_END / DH) = 10 -> 11, h = 5, sigma = 2
+    (integer h/2), s = 8: a unit spike away from the edges spreads into the
+    normalised product of the 1-D Gaussians exp(-k^2/8), k = -5..5; a
+    constant field stays constant; the filter is linear; width 4 -> 5 has
+    sigma 1 (s = 2).  A near-half width rounds away from zero (2.5 -> 3)."""
+    g = np.zeros((31, 27), np.float32)
+    g[15, 13] = 1.0
+    out, w = fo.gauss_filter(g, 10.0, 0.5, 2000.0, 10.0)
+    assert w == 11
+    k1 = np.exp(-np.arange(-5, 6) ** 2 / 8.0)
+    want = np.outer(k1, k1) / k1.sum() ** 2
+    np.testing.assert_allclose(out[10:21, 8:19], want, rtol=2e-6, atol=1e-9)
+    assert np.count_nonzero(out) == 121
+    out5, w5 = fo.gauss_filter(g, 10.0, 0.2, 2000.0, 10.0)
+    k1 = np.exp(-np.arange(-2, 3) ** 2 / 2.0)
+    assert w5 == 5
+    np.testing.assert_allclose(out5[13:18, 11:16], np.outer(k1, k1) / k1.sum() ** 2, rtol=2e-6)
+    c = np.full((20, 17), 3.5, np.float32)
+    np.testing.assert_allclose(fo.gauss_filter(c, 10.0, 0.5, 2000.0, 10.0)[0], c, rtol=1e-5)
+    rng = np.random.default_rng(9)
+    a, b = (rng.standard_normal((25, 19)).astype(np.float32) for _ in range(2))
+    fa, fb, fab = (fo.gauss_filter(x, 10.0, 0.5, 2000.0, 10.0)[0] for x in (a, b, a + b))
+    np.testing.assert_allclose(fab, fa + fb, atol=1e-5)
+    assert fo.gauss_filter(a, 10.0, 0.25, 1000.0, 10.0)[1] == 3      # 2.5 -> 3 (odd already)
+    assert dnfwi.gauss_filter(a, 10.0, 0.25, 1000.0, 10.0)[1] == 3

@@ -24,7 +24,7 @@ pytestmark = pytest.mark.gpu
 ITERMAX = 4
 
 
-@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill", "model_filter"])
+@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill", "model_filter", "spatfilter"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
     ORDER 4) filters the source signals and the observed data.  precond:
@@ -40,13 +40,17 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     third trace of shot 1 and every fifth of shot 2 killed (TRKILL), and the
     offsets of 350 m and more muted (OFFSET_MUTE 1).  model_filter: the
     updated models replaced by their 5 x 5 medians (MODEL_FILTER 1,
-    FILT_SIZE 4, widened to 5) after INV_MODELFILE is written."""
+    FILT_SIZE 4, widened to 5) after INV_MODELFILE is written.  spatfilter:
+    the stacked gradients smoothed by the Gaussian filter (SPATFILTER 3,
+    WD_DAMP 0.5, FC_END 10 Hz without time filtering: an 11-point kernel at
+    the start model's mean Vs) before the L-BFGS direction."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
     pcg = variant == "pcg"
     tapers = variant == "tapers"
     trkill = variant == "trkill"
     mfilt = variant == "model_filter"
+    spat = variant == "spatfilter"
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
@@ -81,9 +85,10 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
         tfiles = {sfx: trng.uniform(0.5, 1.5, (nx, ny)).astype(np.float32) for sfx in ("", "_u", "_rho")}
         for sfx, a in tfiles.items():
             a.tofile(os.path.join(tmp, f"taper/t{sfx}.bin"))
-    fcs, fce, order = (4.0, 12.0, 4) if time_filt else (0.0, 0.0, 6)
+    fcs, fce, order = (4.0, 12.0, 4) if time_filt else (0.0, 10.0 if spat else 0.0, 6)
     inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=time_filt, fc_start=fcs, fc_end=fce, order=order,
-                           eprecond=3 if precond else 0, offset_mute=1 if trkill else 0, offsetc=350.0)
+                           eprecond=3 if precond else 0, offset_mute=1 if trkill else 0, offsetc=350.0,
+                           spatfilter=3 if spat else 0, wd_damp=0.5)
     _run(tmp, "fwi.inp", "wf.inp")
 
     par, p, m, rec, shots, src = _reference_run(tmp, "fwi.inp")
@@ -163,6 +168,10 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                 gs = [fo.taper_sources(nx, ny, par.dh, 40.0, 1, 1, src[:, i - 1:i], a)[1] for a in gs]
             for k in range(3):
                 grads[k] += gs[k]
+        if spat:
+            sm = [fo.gauss_filter(a, par.dh, 0.5, float(c[1]), fce) for a in grads]
+            assert sm[0][1] % 2 == 1 and sm[0][1] >= 5, sm[0][1]     # sigma = (width // 2) // 2 >= 1
+            grads = [a for a, _ in sm]
         if tapers:
             gt = list(par.gradt)
             assert gt == [10, 25, 70, 86]
