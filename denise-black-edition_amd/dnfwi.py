@@ -56,6 +56,8 @@ def load():
                                   fp, fp, fp, fp], C.c_int),
         "dnfwi_taper_hor": ([C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, fp, fp], C.c_int),
         "dnfwi_median_filter": ([C.c_int, C.c_int, C.c_int, fp], C.c_int),
+        "dnfwi_time_window": ([C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp, fp, fp],
+                              C.c_int),
         "dnfwi_gauss_filter": ([C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp,
                                 C.POINTER(C.c_int)], C.c_int),
         "dnfwi_pcg_create": ([C.c_int, C.c_int, C.POINTER(vp_)], C.c_int),
@@ -220,6 +222,16 @@ def gauss_filter(g, dh, wd_damp, c_vs, fc_end):
     dnio._check(load().dnfwi_gauss_filter(out.shape[0], out.shape[1], dh, wd_damp, c_vs, fc_end, _p(out),
                                           C.byref(w)))
     return out, w.value
+
+
+def time_window(timewin, diff, dt, gamma, twlength_minus, twlength_plus, picks=None, offset=None):
+    """dnfwi_time_window on a copy of the (ntr, ns) time-reversed residuals."""
+    out = np.array(diff, np.float32, copy=True, order="C")
+    pk = None if picks is None else np.ascontiguousarray(picks, np.float32)
+    of = None if offset is None else np.ascontiguousarray(offset, np.float32)
+    dnio._check(load().dnfwi_time_window(timewin, out.shape[0], out.shape[1], dt, gamma, twlength_minus,
+                                         twlength_plus, _p(pk), _p(of), _p(out)))
+    return out
 
 
 class PCG:

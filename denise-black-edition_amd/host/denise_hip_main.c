@@ -87,8 +87,9 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (st->spatfilter == 1 || st->spatfilter == 2 || st->spatfilter == 4)
             die("SPATFILTER", "the Gaussian gradient filter (3) is built; 1, 2 and 4 are not");
         if (st->spatfilter == 3 && !(st->fc_end > 0.0f)) die("SPATFILTER", "SPATFILTER 3 needs FC_END > 0");
-        if (st->inv_stf || st->timewin || st->normalize || st->env || st->rowi)
-            die("workflow", "STF inversion, time windows, normalisation, envelope and ROWI are not built");
+        if (st->inv_stf || st->normalize || st->env || st->rowi)
+            die("workflow", "STF inversion, normalisation, envelope and ROWI are not built");
+        if (st->timewin == 3) die("TIMEWIN", "time windows 1, 2, 4 and 5 are built; the STA/LTA picker (3) is not");
         if (st->offset_mute < 0 || st->offset_mute > 2) die("OFFSET_MUTE", "0, 1 or 2 expected");
         if (p->ndt != 1) die("NDT", "FWI needs NDT = 1");
         if (p->idx != 1 || p->idy != 1) die("IDX/IDY", "FWI runs on every grid point (IDX = IDY = 1)");
@@ -127,6 +128,9 @@ typedef struct run {
     int offset_mute;      /* OFFSET_MUTE of the current stage, with OFFSETC */
     float offsetc;
     int *flag;            /* this shot's traces to leave out (ntr) */
+    int timewin;          /* TIMEWIN of the current stage, with GAMMA, TWLENGTH_MINUS/PLUS */
+    float gamma, twlength_minus, twlength_plus;
+    float *picks, *offs;  /* this shot's picked times (TIMEWIN 1, 4) and offsets (ntr) */
     float hc[7];
     char name[4 * DNIO_STR];
 } run;
@@ -186,14 +190,31 @@ static int is_test_shot(const dnio_par *P, int ishot)
 static int shot_flags(run *r, int ishot)
 {
     dnio_par *P = r->P;
-    if (!r->kill && !r->offset_mute) return 0;
+    const int tw = r->timewin == 1 || r->timewin == 2 || r->timewin == 4 || r->timewin == 5;
+    if (!r->kill && !r->offset_mute && !tw) return 0;
     if (!r->flag) r->flag = xmalloc(sizeof(int) * (size_t)(r->ntr ? r->ntr : 1));
+    if (!r->offs) r->offs = xmalloc(sizeof(float) * (size_t)(r->ntr ? r->ntr : 1));
+    if (!r->picks) r->picks = xmalloc(sizeof(float) * (size_t)(r->ntr ? r->ntr : 1));
+    if (r->timewin == 1 || r->timewin == 4) {
+        /* time_window.c:29-44 (TIMEWIN 1: PICKS_FILE<shot>.dat) and 49-61
+         * (TIMEWIN 4: PICKS_FILE shot<shot>.<POS1><POS2>, one domain: .00) */
+        if (r->timewin == 1)
+            snprintf(r->name, sizeof r->name, "%s%i.dat", P->picks_file, ishot);
+        else
+            snprintf(r->name, sizeof r->name, "%sshot%d.%d%d", P->picks_file, ishot, 0, 0);
+        FILE *f = fopen(r->name, "r");
+        if (!f) die(r->name, " picks_?.dat could not be opened !");
+        for (int i = 0; i < r->ntr; i++)
+            if (fscanf(f, r->timewin == 1 ? "%f" : "%e", &r->picks[i]) != 1) die(r->name, "fewer than NTR picked times");
+        fclose(f);
+    }
     const float xs = r->srcpos[0 * r->nsrc + ishot - 1], ys = r->srcpos[1 * r->nsrc + ishot - 1];
     for (int i = 0; i < r->ntr; i++) {
         int f = r->kill ? r->kill[(size_t)(ishot - 1) * r->ntr + i] == 1 : 0;
-        if (r->offset_mute) {
+        if (r->offset_mute || r->timewin == 5) { /* calc_res.c:302-316 */
             const float xr = r->recpos[i] * P->dh, yr = r->recpos[r->ntr + i] * P->dh;
             const float offset = sqrt(((xs - xr) * (xs - xr)) + ((ys - yr) * (ys - yr)));
+            r->offs[i] = offset;
             if ((r->offset_mute == 1) && (offset >= r->offsetc)) f = 1;
             if ((r->offset_mute == 2) && (offset <= r->offsetc)) f = 1;
         }
@@ -212,10 +233,15 @@ static double kill_traces(run *r, int ishot, int test, float **diff)
         if (!diff[c]) continue;
         for (int i = 0; i < r->ntr; i++) {
             float *d = diff[c] + (size_t)i * P->nt;
-            if (flag[i] == 1) {
-                memset(d, 0, sizeof(float) * P->nt);
-                continue;
-            }
+            if (flag[i] == 1) memset(d, 0, sizeof(float) * P->nt);
+        }
+        /* calc_res.c:447-450 and 526-537: TIMEWIN on the residuals before the misfit */
+        io_ok(dnfwi_time_window(r->timewin, r->ntr, P->nt, P->dt, r->gamma, r->twlength_minus, r->twlength_plus,
+                                r->picks, r->offs, diff[c]),
+              "TIMEWIN");
+        for (int i = 0; i < r->ntr; i++) {
+            const float *d = diff[c] + (size_t)i * P->nt;
+            if (flag[i] == 1) continue;
             if (test)
                 for (int k = P->nt - 1; k >= 0; k--) l2 += 0.5 * d[k] * d[k];
         }
@@ -516,6 +542,10 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
         /* grad_obj_psv.c:71, 219: the P/SV path acts on EPRECOND 1 and 3 only */
         r->eprecond = (st.eprecond == 1 || st.eprecond == 3) ? st.eprecond : 0;
         r->offset_mute = st.offset_mute;
+        r->timewin = st.timewin;
+        r->gamma = st.gamma;
+        r->twlength_minus = st.twlength_minus;
+        r->twlength_plus = st.twlength_plus;
         r->offsetc = st.offsetc;
         for (int iter = 1; iter <= P->itermax; iter++) {
             printf("\n TDFWI ITERATION %d of %d (stage %d of %d)\n", iter, P->itermax, nstage, stagemax);
@@ -642,6 +672,10 @@ static void rtm(run *r, const dnio_stage *st)
     dnio_par *P = r->P;
     r->eprecond = (st->eprecond == 1 || st->eprecond == 3) ? st->eprecond : 0;
     r->offset_mute = st->offset_mute;
+    r->timewin = st->timewin;
+    r->gamma = st->gamma;
+    r->twlength_minus = st->twlength_minus;
+    r->twlength_plus = st->twlength_plus;
     r->offsetc = st->offsetc;
     r->c_vp = dnfwi_model_mean(r->vp, P->nx, P->ny); /* RTM_PSV.c:316-352 */
     r->c_vs = dnfwi_model_mean(r->vs, P->nx, P->ny);
@@ -745,6 +779,8 @@ int main(int argc, char **argv)
     dn_ctx_destroy(r.ctx);
     free(r.kill);
     free(r.flag);
+    free(r.picks);
+    free(r.offs);
     dnio_free(r.recpos);
     dnio_free(r.srcpos);
     free(vp);

@@ -440,6 +440,34 @@ int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_
     return 0;
 }
 
+int dnfwi_time_window(int timewin, int ntr, int ns, float dt, float gamma, float twlength_minus,
+                      float twlength_plus, const float *picks, const float *offset, float *diff)
+{
+    if ((ntr > 0 && !diff) || ntr < 0 || ns < 0) return dnio_fail_msg("dnfwi_time_window: bad argument");
+    if ((timewin == 1 || timewin == 4) && !picks) return dnio_fail_msg("dnfwi_time_window: TIMEWIN 1/4 need picks");
+    if (timewin == 5 && !offset) return dnio_fail_msg("dnfwi_time_window: TIMEWIN 5 needs offsets");
+    for (int i = 0; i < ntr; i++) {
+        float *d = diff + (size_t)i * ns;
+        if (timewin == 5) {
+            const float t0 = fabs(offset[i] / twlength_minus);
+            for (int k = 0; k < ns; k++) {
+                const float t = (ns - k) * dt;
+                if (t >= t0) d[k] *= exp(-gamma * pow(t - t0, 2));
+            }
+        } else if (timewin == 1 || timewin == 2 || timewin == 4) {
+            const float pt = timewin == 2 ? -dt : picks[i] + twlength_plus;
+            const float pt1 = timewin == 2 ? twlength_plus : picks[i];
+            for (int k = 0; k < ns; k++) {
+                const float t = (float)((ns - k) * dt);
+                const float dump = t - pt;
+                const float taper = exp(-gamma * dump * dump);
+                if (timewin == 2 ? t >= pt1 : t >= pt) d[k] = d[k] * taper;
+            }
+        }
+    }
+    return 0;
+}
+
 int dnfwi_gauss_filter(int nx, int ny, float dh, float wd_damp, float c_vs, float fc_end, float *g, int *width)
 {
     if (!g || nx < 1 || ny < 1 || !(dh > 0.0f) || !(fc_end > 0.0f))

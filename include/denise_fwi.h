@@ -148,6 +148,21 @@ int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_
  * leaves g unchanged.  *width (may be NULL) gets the width used. */
 int dnfwi_gauss_filter(int nx, int ny, float dh, float wd_damp, float c_vs, float fc_end, float *g, int *width);
 
+/* Time windows of one shot's residuals, in place on diff (ntr traces of ns
+ * time-reversed samples: diff[i * ns + k] belongs to time (ns - k) DT, the
+ * order of dn_get_adjoint_sources), as src/calc_res.c:447-450 (TIMEWIN 5)
+ * and src/time_window.c:7-146 (TIMEWIN 1, 2, 4) apply them.
+ *   TIMEWIN 1, 4: samples at t >= pick + TWLENGTH_PLUS are multiplied by
+ *     exp(-GAMMA (t - pick - TWLENGTH_PLUS)^2); picks: ntr times in s.
+ *   TIMEWIN 2: samples at t >= TWLENGTH_PLUS are multiplied by
+ *     exp(-GAMMA (t + DT)^2) (the reference's constant window).
+ *   TIMEWIN 5: surface waves damped, samples at t >= t0 = |offset /
+ *     TWLENGTH_MINUS| multiplied by exp(-GAMMA (t - t0)^2); offset: ntr
+ *     source-receiver distances in m.
+ * Other values leave diff unchanged. */
+int dnfwi_time_window(int timewin, int ntr, int ns, float dt, float gamma, float twlength_minus,
+                      float twlength_plus, const float *picks, const float *offset, float *diff);
+
 /* Median model filter (src/smooth_model.c:37-196 with src/median2d.c,
  * MODEL_FILTER 1, applied by FWI_PSV.c:752 to vp, vs and rho after each
  * model update): every cell of the x-major nx x ny array a is replaced by

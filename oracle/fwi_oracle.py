@@ -23,6 +23,8 @@ Written from the reference's functions, one per function, keeping its
   - taper_sources: src/taper_grad.c:328-577 (SWS_TAPER_GRAD_SOURCES);
   - PCG: src/PCG.c:11-105 with descent, store_PCG_PSV and extract_PCG_PSV
     as src/PSV/FWI_PSV.c:637-662 calls them.
+  - time_window: src/time_window.c:7-146 (TIMEWIN 1, 2, 4) and
+    src/calc_res.c:447-450 (TIMEWIN 5);
   - gauss_filter: src/gauss_filt.c:9-208 (SPATFILTER 3, through
     src/smooth_grad.c:17-33 at src/PSV/FWI_PSV.c:628-631);
   - median_filter: src/smooth_model.c:37-196 with src/median2d.c:9-44
@@ -32,6 +34,8 @@ The pins of these functions against what the mathematics fixes (BFGS
 matrix recursion, parabola vertex, limits) are in tests/test_fwi_host.py.
 """
 from __future__ import annotations
+
+import math
 
 import numpy as np
 
@@ -299,6 +303,32 @@ def energy_precond(eprecond, dh, Ws, Wr, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho):
         gs[...] = 0.0
     gr = (g_rho / ((We * c_rho).astype(F32) * c_rho).astype(F32)).astype(F32)
     return We, gv, gs, gr
+
+
+def time_window(timewin, diff, dt, gamma, twlength_minus, twlength_plus, picks=None, offset=None):
+    """The windowed (ntr, ns) time-reversed residuals; column k holds time
+    (ns - k) DT (calc_res.c:423, 441).  Exponents in float32, exp in
+    double, as the C expressions do."""
+    d = np.array(diff, np.float32, copy=True)
+    ntr, ns = d.shape
+    t = (np.arange(ns, 0, -1).astype(np.float32) * F32(dt)).astype(np.float32)
+    g32 = F32(gamma)
+    for i in range(ntr):
+        if timewin == 5:
+            t0 = F32(abs(F32(F32(offset[i]) / F32(twlength_minus))))
+            x = (t - t0).astype(np.float32).astype(np.float64)
+            fac = np.array([math.exp(-float(g32) * (v * v)) for v in x], np.float64)
+            on = t >= t0
+            d[i, on] = (d[i, on].astype(np.float64) * fac[on]).astype(np.float32)
+        elif timewin in (1, 2, 4):
+            pt = F32(-F32(dt)) if timewin == 2 else F32(F32(picks[i]) + F32(twlength_plus))
+            pt1 = F32(twlength_plus) if timewin == 2 else F32(picks[i])
+            dump = (t - pt).astype(np.float32)
+            ex = ((-g32 * dump).astype(np.float32) * dump).astype(np.float32)
+            taper = np.array([math.exp(float(v)) for v in ex], np.float64).astype(np.float32)
+            on = t >= (pt1 if timewin == 2 else pt)
+            d[i, on] = (d[i, on] * taper[on]).astype(np.float32)
+    return d
 
 
 def gauss_filter(g, dh, wd_damp, c_vs, fc_end):

@@ -522,3 +522,34 @@ def test_gauss_filter_pins():
     np.testing.assert_allclose(fab, fa + fb, atol=1e-5)
     assert fo.gauss_filter(a, 10.0, 0.25, 1000.0, 10.0)[1] == 3      # 2.5 -> 3 (odd already)
     assert dnfwi.gauss_filter(a, 10.0, 0.25, 1000.0, 10.0)[1] == 3
+
+
+@pytest.mark.parametrize("timewin", [0, 1, 2, 4, 5])
+def test_time_window(timewin):
+    """dnfwi_time_window equals the restatement bit for bit; pins: samples
+    before the window start are untouched, the factor is 1 at the start
+    and follows exp(-GAMMA (t - start)^2) after it (TIMEWIN 1, 4: start =
+    pick + TWLENGTH_PLUS; 5: start = |offset / TWLENGTH_MINUS|), TIMEWIN 2
+    multiplies from TWLENGTH_PLUS on by exp(-GAMMA (t + DT)^2), and column
+    k of the time-reversed traces is time (ns - k) DT."""
+    rng = np.random.default_rng(20 + timewin)
+    ntr, ns, dt, gamma = 7, 300, 1e-3, 200.0
+    d = (rng.standard_normal((ntr, ns)) + 2.0).astype(np.float32)
+    picks = rng.uniform(0.02, 0.15, ntr).astype(np.float32)
+    offset = rng.uniform(50.0, 900.0, ntr).astype(np.float32)
+    twm, twp = 3000.0, 0.04
+    got = dnfwi.time_window(timewin, d, dt, gamma, twm, twp, picks, offset)
+    want = fo.time_window(timewin, d, dt, gamma, twm, twp, picks, offset)
+    assert np.array_equal(got, want)
+    t = np.arange(ns, 0, -1) * dt
+    fac = got.astype(np.float64) / d
+    for i in range(ntr):
+        if timewin in (1, 4, 5):
+            start = picks[i] + twp if timewin != 5 else offset[i] / twm
+            ref = np.where(t >= start, np.exp(-gamma * (t - start) ** 2), 1.0)
+        elif timewin == 2:
+            ref = np.where(t >= twp, np.exp(-gamma * (t + dt) ** 2), 1.0)
+        else:
+            ref = np.ones(ns)
+        np.testing.assert_allclose(fac[i], ref, rtol=3e-5, atol=1e-6)
+        assert np.array_equal(got[i, t < start - 1e-5], d[i, t < start - 1e-5]) if timewin in (1, 4, 5) else True

@@ -24,7 +24,8 @@ pytestmark = pytest.mark.gpu
 ITERMAX = 4
 
 
-@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill", "model_filter", "spatfilter"])
+@pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill", "model_filter", "spatfilter", "timewin",
+                                     "surface"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
     ORDER 4) filters the source signals and the observed data.  precond:
@@ -43,7 +44,11 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     FILT_SIZE 4, widened to 5) after INV_MODELFILE is written.  spatfilter:
     the stacked gradients smoothed by the Gaussian filter (SPATFILTER 3,
     WD_DAMP 0.5, FC_END 10 Hz without time filtering: an 11-point kernel at
-    the start model's mean Vs) before the L-BFGS direction."""
+    the start model's mean Vs) before the L-BFGS direction.  timewin: the
+    residuals damped by exp(-GAMMA (t - pick - TWLENGTH_PLUS)^2) after each
+    trace's picked time (TIMEWIN 1, PICKS_FILE per shot, GAMMA 100,
+    TWLENGTH_PLUS 0.05 s).  surface: surface waves damped after offset /
+    1500 m/s (TIMEWIN 5, TWLENGTH_MINUS 1500)."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
     pcg = variant == "pcg"
@@ -51,6 +56,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     trkill = variant == "trkill"
     mfilt = variant == "model_filter"
     spat = variant == "spatfilter"
+    timewin = {"timewin": 1, "surface": 5}.get(variant, 0)
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
@@ -73,7 +79,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                       SWS_TAPER_GRAD_VERT=int(tapers), SWS_TAPER_FILE=int(tapers), TRKILL=int(trkill),
                       SWS_TAPER_GRAD_SOURCES=int(tapers), SRTSHAPE=1, SRTRADIUS=40.0, FILTSIZE=1,
                       SWS_TAPER_CIRCULAR_PER_SHOT=int(tapers),
-                      TRKILL_FILE="kill.dat", MODEL_FILTER=int(mfilt), FILT_SIZE=4)
+                      TRKILL_FILE="kill.dat", MODEL_FILTER=int(mfilt), FILT_SIZE=4, PICKS_FILE="picks/picks_")
     kill = np.zeros((2, len(receivers)), np.int32)
     kill[0, ::3] = 1
     kill[1, ::5] = 1
@@ -88,7 +94,15 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     fcs, fce, order = (4.0, 12.0, 4) if time_filt else (0.0, 10.0 if spat else 0.0, 6)
     inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=time_filt, fc_start=fcs, fc_end=fce, order=order,
                            eprecond=3 if precond else 0, offset_mute=1 if trkill else 0, offsetc=350.0,
-                           spatfilter=3 if spat else 0, wd_damp=0.5)
+                           spatfilter=3 if spat else 0, wd_damp=0.5, timewin=timewin, gamma=100.0,
+                           twlength_minus=1500.0, twlength_plus=0.05)
+    os.makedirs(os.path.join(tmp, "picks"), exist_ok=True)
+    picks = {}
+    for i in (1, 2):     # a direct-wave pick per trace, written with 6 decimals
+        txt = ["%.6f" % (abs(x - sources[i - 1][0]) / 2200.0 + 0.03) for x, _ in receivers]
+        with open(os.path.join(tmp, f"picks/picks_{i}.dat"), "w") as f:
+            f.write("\n".join(txt) + "\n")
+        picks[i] = np.array([np.float32(t) for t in txt], np.float32)
     _run(tmp, "fwi.inp", "wf.inp")
 
     par, p, m, rec, shots, src = _reference_run(tmp, "fwi.inp")
@@ -124,16 +138,25 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                        ((ys - yr) * (ys - yr)).astype(np.float32)).astype(np.float64)).astype(np.float32)
         return f | (off >= np.float32(350.0))
 
+    def offsets(i):
+        xs, ys = np.float32(src[0, i - 1]), np.float32(src[1, i - 1])
+        xr = (rec[0].astype(np.float32) * np.float32(par.dh)).astype(np.float32)
+        yr = (rec[1].astype(np.float32) * np.float32(par.dh)).astype(np.float32)
+        return np.sqrt((((xs - xr) * (xs - xr)).astype(np.float32) +
+                        ((ys - yr) * (ys - yr)).astype(np.float32)).astype(np.float64)).astype(np.float32)
+
     def residuals(i):
         l2 = g.residuals(obs[i][0], obs[i][1], None, 1)
-        if not trkill:
+        if not trkill and not timewin:
             return l2
-        fl = flags(i)
+        fl = flags(i) if trkill else np.zeros(rec.shape[1], bool)
         diffs = []
         l2 = 0.0
         for comp in (0, 1):
             d = g.adjoint_sources(comp)
             d[fl] = 0.0
+            if timewin:
+                d = fo.time_window(timewin, d, par.dt, 100.0, 1500.0, 0.05, picks[i], offsets(i))
             for tr in range(d.shape[0]):
                 if not fl[tr]:
                     for k in range(d.shape[1] - 1, -1, -1):
