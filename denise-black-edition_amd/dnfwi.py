@@ -56,6 +56,8 @@ def load():
                                   fp, fp, fp, fp], C.c_int),
         "dnfwi_taper_hor": ([C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, fp, fp], C.c_int),
         "dnfwi_median_filter": ([C.c_int, C.c_int, C.c_int, fp], C.c_int),
+        "dnfwi_residual_gc": ([fp, fp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.POINTER(C.c_int), fp,
+                               C.POINTER(C.c_double)], C.c_int),
         "dnfwi_time_window": ([C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp, fp, fp],
                               C.c_int),
         "dnfwi_gauss_filter": ([C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp,
@@ -85,7 +87,7 @@ def _f(a):
 
 
 def _p(a):
-    return a.ctypes.data_as(C.POINTER(C.c_float))
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def update_par(vpupperlim=1e9, vplowerlim=0.0, vsupperlim=1e9, vslowerlim=0.0, rhoupperlim=1e9, rholowerlim=0.0,
@@ -222,6 +224,19 @@ def gauss_filter(g, dh, wd_damp, c_vs, fc_end):
     dnio._check(load().dnfwi_gauss_filter(out.shape[0], out.shape[1], dh, wd_damp, c_vs, fc_end, _p(out),
                                           C.byref(w)))
     return out, w.value
+
+
+def residual_gc(data, syn, dt, grad_form, testshot, skip=None, l2=0.0):
+    """dnfwi_residual_gc: (time-reversed adjoint sources (ntr, ns), l2)."""
+    data = np.ascontiguousarray(data, np.float32)
+    syn = np.ascontiguousarray(syn, np.float32)
+    out = np.empty_like(syn)
+    sk = None if skip is None else np.ascontiguousarray(skip, np.int32)
+    v = C.c_double(l2)
+    dnio._check(load().dnfwi_residual_gc(_p(data), _p(syn), syn.shape[0], syn.shape[1], dt, grad_form, testshot,
+                                         None if sk is None else sk.ctypes.data_as(C.POINTER(C.c_int)), _p(out),
+                                         C.byref(v)))
+    return out, v.value
 
 
 def time_window(timewin, diff, dt, gamma, twlength_minus, twlength_plus, picks=None, offset=None):

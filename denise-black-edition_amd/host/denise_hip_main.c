@@ -82,7 +82,9 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         die("SNAP_FORMAT", "snapshots are written as ASCII (2) or binary (3); SU (1) is not implemented, as in the reference");
     if (p->mode == 1 || p->mode == 2) {
         if (!st) die("workflow", "MODE 1 and 2 need a workflow file");
-        if (st->lnorm != 2) die("LNORM", "only the L2 misfit (LNORM = 2) is built");
+        if (st->lnorm != 2 && st->lnorm != 5) die("LNORM", "the L2 (2) and global-correlation (5) misfits are built");
+        if (st->lnorm == 5 && (p->mode != 1 || st->n_order != 0))
+            die("LNORM", "the global-correlation misfit is built for FWI (MODE 1) without N_ORDER");
         /* smooth_grad.c:17-33: 1 wavenumber damping, 2 smooth2, 3 Gaussian, 4 velocity-dependent Gaussian */
         if (st->spatfilter == 1 || st->spatfilter == 2 || st->spatfilter == 4)
             die("SPATFILTER", "the Gaussian gradient filter (3) is built; 1, 2 and 4 are not");
@@ -131,6 +133,7 @@ typedef struct run {
     int timewin;          /* TIMEWIN of the current stage, with GAMMA, TWLENGTH_MINUS/PLUS */
     float gamma, twlength_minus, twlength_plus;
     float *picks, *offs;  /* this shot's picked times (TIMEWIN 1, 4) and offsets (ntr) */
+    int lnorm;            /* LNORM of the current stage */
     float hc[7];
     char name[4 * DNIO_STR];
 } run;
@@ -278,6 +281,31 @@ static double shot_residuals(run *r, int ishot, int test)
             if (r->time_filt) io_ok(dnio_tdfilt(obs[c], r->ntr, P->nt, P->dt, r->order, r->fc, r->fc_start), "TIME_FILT");
         }
     double l2 = 0.0;
+    if (r->lnorm == 5) {
+        /* calc_res.c with LNORM 5: residuals and misfit on the host from the
+         * device's seismograms, then the time windows, as RTM's sources */
+        const int fl = shot_flags(r, ishot);
+        float *diff[3] = {NULL, NULL, NULL}, *syn = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
+        for (int c = 0; c < 3; c++)
+            if (use[c]) {
+                diff[c] = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
+                dn_ok(dn_get_seismogram(r->ctx, c, syn), "seismograms");
+                io_ok(dnfwi_residual_gc(obs[c], syn, r->ntr, P->nt, P->dt, P->grad_form, test, fl ? r->flag : NULL,
+                                        diff[c], &l2),
+                      "LNORM 5 residuals");
+                if (fl)
+                    io_ok(dnfwi_time_window(r->timewin, r->ntr, P->nt, P->dt, r->gamma, r->twlength_minus,
+                                            r->twlength_plus, r->picks, r->offs, diff[c]),
+                          "TIMEWIN");
+            }
+        dn_ok(dn_set_adjoint_sources(r->ctx, diff[0], diff[1], diff[2]), "adjoint sources");
+        for (int c = 0; c < 3; c++) {
+            free(diff[c]);
+            free(obs[c]);
+        }
+        free(syn);
+        return l2;
+    }
     dn_ok(dn_calc_residuals(r->ctx, obs[0], obs[1], obs[2], test, &l2), "residuals");
     if (shot_flags(r, ishot)) {
         float *diff[3] = {NULL, NULL, NULL};
@@ -543,6 +571,7 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
         r->eprecond = (st.eprecond == 1 || st.eprecond == 3) ? st.eprecond : 0;
         r->offset_mute = st.offset_mute;
         r->timewin = st.timewin;
+        r->lnorm = st.lnorm;
         r->gamma = st.gamma;
         r->twlength_minus = st.twlength_minus;
         r->twlength_plus = st.twlength_plus;

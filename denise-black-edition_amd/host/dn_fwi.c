@@ -440,6 +440,65 @@ int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_
     return 0;
 }
 
+int dnfwi_residual_gc(const float *data, const float *syn, int ntr, int ns, float dt, int grad_form, int testshot,
+                      const int *skip, float *diff, double *l2)
+{
+    if (!data || !syn || !diff || !l2 || ntr < 0 || ns < 1 || (grad_form != 1 && grad_form != 2))
+        return dnio_fail_msg("dnfwi_residual_gc: bad argument");
+    const float eps = 0.0f; /* EPS_LNORM, calc_res.c:36 */
+    float *id = NULL, *is = NULL;
+    if (grad_form == 1) {
+        id = malloc(sizeof(float) * (size_t)ns);
+        is = malloc(sizeof(float) * (size_t)ns);
+        if (!id || !is) {
+            free(id);
+            free(is);
+            return dnio_fail_msg("dnfwi_residual_gc: out of host memory");
+        }
+    }
+    memset(diff, 0, sizeof(float) * (size_t)ntr * ns); /* calc_res.c:41-43 */
+    double L2 = *l2;
+    for (int i = 0; i < ntr; i++) {
+        if (skip && skip[i] == 1) continue;
+        const float *dt_ = data + (size_t)i * ns, *st_ = syn + (size_t)i * ns;
+        const float *d = dt_, *sy = st_;
+        if (grad_form == 1) {
+            float a = 0.0f, b = 0.0f;
+            for (int j = 0; j < ns; j++) {
+                a += st_[j];
+                is[j] = dt * a;
+                b += dt_[j];
+                id[j] = dt * b;
+            }
+            d = id;
+            sy = is;
+        }
+        float abs_d = 0.0f, abs_s = 0.0f, dms = 0.0f;
+        for (int j = 0; j < ns; j++) {
+            abs_d += d[j] * d[j];
+            abs_s += sy[j] * sy[j];
+            dms += sy[j] * d[j];
+        }
+        abs_d = sqrt(abs_d);
+        abs_s = sqrt(abs_s);
+        float *out = diff + (size_t)i * ns;
+        for (int j = 0; j < ns; j++) {
+            const float dj = (grad_form == 2 && j == 0) ? st_[0] : d[j];
+            const float sj = sy[j];
+            const float tmp = abs_s * abs_s * abs_s * abs_d, tmp1 = abs_s * abs_d;
+            out[ns - 1 - j] = ((sj * dms) / (tmp + eps)) - (dj / (tmp1 + eps));
+            if (testshot) {
+                const float t2 = abs_d * abs_s;
+                L2 -= (dj * sj) / (t2 + eps);
+            }
+        }
+    }
+    *l2 = L2;
+    free(id);
+    free(is);
+    return 0;
+}
+
 int dnfwi_time_window(int timewin, int ntr, int ns, float dt, float gamma, float twlength_minus,
                       float twlength_plus, const float *picks, const float *offset, float *diff)
 {

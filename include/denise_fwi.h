@@ -148,6 +148,20 @@ int dnfwi_taper_hor(int nx, int ny, float dh, int gradt1, int gradt2, float exp_
  * leaves g unchanged.  *width (may be NULL) gets the width used. */
 int dnfwi_gauss_filter(int nx, int ny, float dh, float wd_damp, float c_vs, float fc_end, float *g, int *width);
 
+/* Global-correlation residuals of one component of one shot (LNORM 5,
+ * src/calc_res.c:130-157, 330-360, 465-489): per trace, with d the
+ * observed and s the synthetic trace (GRAD_FORM 1: their running integrals
+ * DT * sum; GRAD_FORM 2: the traces, the first observed sample replaced
+ * by the synthetic one as calc_res.c:414 does, after the norms), the norms
+ * |d|, |s| and the product d.s as float sums, the adjoint source
+ * s (d.s) / (|s|^3 |d|) - d / (|s| |d|) time-reversed into diff (ntr x ns,
+ * the order of dn_set_adjoint_sources), and, for a test shot, *l2 (in/out,
+ * running like the reference's L2 over components) decreased by
+ * d s / (|d| |s|) per sample.  Traces with skip[i] = 1 (may be NULL) get
+ * zeros and no misfit. */
+int dnfwi_residual_gc(const float *data, const float *syn, int ntr, int ns, float dt, int grad_form, int testshot,
+                      const int *skip, float *diff, double *l2);
+
 /* Time windows of one shot's residuals, in place on diff (ntr traces of ns
  * time-reversed samples: diff[i * ns + k] belongs to time (ns - k) DT, the
  * order of dn_get_adjoint_sources), as src/calc_res.c:447-450 (TIMEWIN 5)

@@ -23,6 +23,7 @@ Written from the reference's functions, one per function, keeping its
   - taper_sources: src/taper_grad.c:328-577 (SWS_TAPER_GRAD_SOURCES);
   - PCG: src/PCG.c:11-105 with descent, store_PCG_PSV and extract_PCG_PSV
     as src/PSV/FWI_PSV.c:637-662 calls them.
+  - residual_gc: src/calc_res.c:130-157, 330-360, 414, 465-489 (LNORM 5);
   - time_window: src/time_window.c:7-146 (TIMEWIN 1, 2, 4) and
     src/calc_res.c:447-450 (TIMEWIN 5);
   - gauss_filter: src/gauss_filt.c:9-208 (SPATFILTER 3, through
@@ -303,6 +304,43 @@ def energy_precond(eprecond, dh, Ws, Wr, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho):
         gs[...] = 0.0
     gr = (g_rho / ((We * c_rho).astype(F32) * c_rho).astype(F32)).astype(F32)
     return We, gv, gs, gr
+
+
+def _seqsum(x):
+    """Sequential float32 sum (a float accumulator in index order)."""
+    return np.cumsum(np.asarray(x, np.float32), dtype=np.float32)[-1] if len(x) else F32(0.0)
+
+
+def residual_gc(data, syn, dt, grad_form, testshot, skip=None, l2=0.0):
+    """LNORM 5 residuals (time-reversed (ntr, ns)) and the running misfit."""
+    data = np.asarray(data, np.float32)
+    syn = np.asarray(syn, np.float32)
+    ntr, ns = syn.shape
+    out = np.zeros((ntr, ns), np.float32)
+    L2 = float(l2)
+    for i in range(ntr):
+        if skip is not None and skip[i] == 1:
+            continue
+        if grad_form == 1:
+            d = (np.cumsum(data[i], dtype=np.float32) * F32(dt)).astype(np.float32)
+            s = (np.cumsum(syn[i], dtype=np.float32) * F32(dt)).astype(np.float32)
+        else:
+            d, s = data[i], syn[i]
+        abs_d = F32(math.sqrt(float(_seqsum((d * d).astype(np.float32)))))
+        abs_s = F32(math.sqrt(float(_seqsum((s * s).astype(np.float32)))))
+        dms = _seqsum((s * d).astype(np.float32))
+        dj = d.copy()
+        if grad_form == 2:
+            dj[0] = syn[i, 0]                      # calc_res.c:414
+        tmp = F32(F32(F32(abs_s * abs_s) * abs_s) * abs_d)
+        tmp1 = F32(abs_s * abs_d)
+        r = ((s * dms).astype(np.float32) / tmp).astype(np.float32) - (dj / tmp1).astype(np.float32)
+        out[i] = r.astype(np.float32)[::-1]
+        if testshot:
+            terms = ((dj * s).astype(np.float32) / F32(abs_d * abs_s)).astype(np.float32)
+            for v in terms.tolist():
+                L2 -= v
+    return out, L2
 
 
 def time_window(timewin, diff, dt, gamma, twlength_minus, twlength_plus, picks=None, offset=None):

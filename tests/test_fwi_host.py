@@ -553,3 +553,59 @@ def test_time_window(timewin):
             ref = np.ones(ns)
         np.testing.assert_allclose(fac[i], ref, rtol=3e-5, atol=1e-6)
         assert np.array_equal(got[i, t < start - 1e-5], d[i, t < start - 1e-5]) if timewin in (1, 4, 5) else True
+
+
+@pytest.mark.parametrize("grad_form", [1, 2])
+def test_residual_gc(grad_form):
+    """dnfwi_residual_gc (LNORM 5) equals the restatement bit for bit, with
+    skipped traces zero and the misfit running from a given start."""
+    rng = np.random.default_rng(30 + grad_form)
+    ntr, ns, dt = 6, 250, 1e-3
+    syn = rng.standard_normal((ntr, ns)).astype(np.float32)
+    data = (syn * np.float32(0.7) + rng.standard_normal((ntr, ns)).astype(np.float32) * np.float32(0.3)).astype(np.float32)
+    skip = np.array([0, 1, 0, 0, 1, 0], np.int32)
+    got, l2 = dnfwi.residual_gc(data, syn, dt, grad_form, 1, skip, l2=0.25)
+    want, l2w = fo.residual_gc(data, syn, dt, grad_form, 1, skip, l2=0.25)
+    assert np.array_equal(got, want)
+    assert l2 == l2w
+    assert not got[1].any() and not got[4].any()
+    g0, l0 = dnfwi.residual_gc(data, syn, dt, grad_form, 0)
+    assert l0 == 0.0 and np.array_equal(g0[0], got[0])
+
+
+def test_residual_gc_pins():
+    """What the mathematics fixes: the misfit -sum d.s/(|d||s|) is -1 per
+    trace for data proportional to the synthetics (scale-free; traces that
+    start at 0, as seismograms do), where the
+    residual vanishes; the residual is the derivative of the misfit with
+    respect to the synthetics (GRAD_FORM 2, central differences in float64,
+    first samples equal so that the sample-1 replacement does not enter),
+    time-reversed; GRAD_FORM 1 is GRAD_FORM 2 on the running integrals."""
+    rng = np.random.default_rng(41)
+    ntr, ns, dt = 3, 64, 1e-3
+    syn = rng.standard_normal((ntr, ns)).astype(np.float32)
+    syn[:, 0] = 0.0           # traces start at rest, so the sample-1 replacement changes nothing
+    r, l2 = fo.residual_gc(syn * np.float32(3.0), syn, dt, 2, 1)
+    assert abs(l2 + ntr) < 1e-5
+    assert np.abs(r).max() < 1e-5 * np.abs(syn).max()
+    data = rng.standard_normal((ntr, ns)).astype(np.float32)
+    data[:, 0] = syn[:, 0]
+    r, _ = fo.residual_gc(data, syn, dt, 2, 1)
+
+    def misfit(s):
+        d = data.astype(np.float64)
+        return -sum(float(d[i] @ s[i]) / (np.linalg.norm(d[i]) * np.linalg.norm(s[i])) for i in range(ntr))
+
+    s64 = syn.astype(np.float64)
+    for i, j in [(0, 5), (1, 30), (2, 63), (0, 0)]:
+        h = 1e-4
+        sp, sm = s64.copy(), s64.copy()
+        sp[i, j] += h
+        sm[i, j] -= h
+        fd = (misfit(sp) - misfit(sm)) / (2 * h)
+        assert abs(r[i, ns - 1 - j] - fd) < 2e-4 * max(1.0, abs(fd)), (i, j, r[i, ns - 1 - j], fd)
+    i1 = (np.cumsum(data, axis=1, dtype=np.float32) * np.float32(dt)).astype(np.float32)
+    s1 = (np.cumsum(syn, axis=1, dtype=np.float32) * np.float32(dt)).astype(np.float32)
+    r1, l1 = fo.residual_gc(data, syn, dt, 1, 1)
+    r2, l2b = fo.residual_gc(i1, s1, dt, 2, 1)
+    assert np.array_equal(r1[:, :-1], r2[:, :-1]) and abs(l1 - l2b) < 1e-4

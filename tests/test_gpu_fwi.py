@@ -25,7 +25,7 @@ ITERMAX = 4
 
 
 @pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill", "model_filter", "spatfilter", "timewin",
-                                     "surface"])
+                                     "surface", "gc"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
     ORDER 4) filters the source signals and the observed data.  precond:
@@ -48,7 +48,9 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     residuals damped by exp(-GAMMA (t - pick - TWLENGTH_PLUS)^2) after each
     trace's picked time (TIMEWIN 1, PICKS_FILE per shot, GAMMA 100,
     TWLENGTH_PLUS 0.05 s).  surface: surface waves damped after offset /
-    1500 m/s (TIMEWIN 5, TWLENGTH_MINUS 1500)."""
+    1500 m/s (TIMEWIN 5, TWLENGTH_MINUS 1500).  gc: the global-correlation
+    misfit (LNORM 5, GRAD_FORM 1) with residuals computed on the host from
+    the device's seismograms."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
     pcg = variant == "pcg"
@@ -57,6 +59,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     mfilt = variant == "model_filter"
     spat = variant == "spatfilter"
     timewin = {"timewin": 1, "surface": 5}.get(variant, 0)
+    gc = variant == "gc"
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
@@ -95,7 +98,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=time_filt, fc_start=fcs, fc_end=fce, order=order,
                            eprecond=3 if precond else 0, offset_mute=1 if trkill else 0, offsetc=350.0,
                            spatfilter=3 if spat else 0, wd_damp=0.5, timewin=timewin, gamma=100.0,
-                           twlength_minus=1500.0, twlength_plus=0.05)
+                           twlength_minus=1500.0, twlength_plus=0.05, lnorm=5 if gc else 2)
     os.makedirs(os.path.join(tmp, "picks"), exist_ok=True)
     picks = {}
     for i in (1, 2):     # a direct-wave pick per trace, written with 6 decimals
@@ -146,6 +149,13 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                         ((ys - yr) * (ys - yr)).astype(np.float32)).astype(np.float64)).astype(np.float32)
 
     def residuals(i):
+        if gc:
+            diffs, l2 = [], 0.0
+            for comp in (0, 1):
+                d, l2 = fo.residual_gc(obs[i][comp], g.seismogram(comp), par.dt, par.grad_form, 1, None, l2)
+                diffs.append(d)
+            g.set_adjoint_sources(diffs[0], diffs[1], None)
+            return l2
         l2 = g.residuals(obs[i][0], obs[i][1], None, 1)
         if not trkill and not timewin:
             return l2
