@@ -89,8 +89,8 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (st->spatfilter == 1 || st->spatfilter == 2 || st->spatfilter == 4)
             die("SPATFILTER", "the Gaussian gradient filter (3) is built; 1, 2 and 4 are not");
         if (st->spatfilter == 3 && !(st->fc_end > 0.0f)) die("SPATFILTER", "SPATFILTER 3 needs FC_END > 0");
-        if (st->inv_stf || st->normalize || st->env || st->rowi)
-            die("workflow", "STF inversion, normalisation, envelope and ROWI are not built");
+        /* NORMALIZE acts in the STF inversion only (stf.c:176-186) and ENV with LNORM 6 only */
+        if (st->inv_stf || st->rowi) die("workflow", "STF inversion and ROWI are not built");
         if (st->timewin == 3) die("TIMEWIN", "time windows 1, 2, 4 and 5 are built; the STA/LTA picker (3) is not");
         if (st->offset_mute < 0 || st->offset_mute > 2) die("OFFSET_MUTE", "0, 1 or 2 expected");
         if (p->ndt != 1) die("NDT", "FWI needs NDT = 1");
