@@ -56,6 +56,8 @@ def load():
                                   fp, fp, fp, fp], C.c_int),
         "dnfwi_taper_hor": ([C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, fp, fp], C.c_int),
         "dnfwi_median_filter": ([C.c_int, C.c_int, C.c_int, fp], C.c_int),
+        "dnfwi_gauss_filter_var": ([C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp, fp],
+                                   C.c_int),
         "dnfwi_residual_gc": ([fp, fp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.POINTER(C.c_int), fp,
                                C.POINTER(C.c_double)], C.c_int),
         "dnfwi_time_window": ([C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, fp, fp, fp],
@@ -207,6 +209,15 @@ def taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper):
     coeff = np.empty((nx, ny), np.float32)
     dnio._check(load().dnfwi_taper_hor(nx, ny, dh, gradt1, gradt2, exp_taper, _p(coeff), None))
     return coeff
+
+
+def gauss_filter_var(g, vel, dh, wd_damp, wd_damp1, fc_end):
+    """dnfwi_gauss_filter_var on a copy of the (nx, ny) gradient g (SPATFILTER 4)."""
+    out = np.array(g, np.float32, copy=True, order="C")
+    v = np.ascontiguousarray(vel, np.float32)
+    dnio._check(load().dnfwi_gauss_filter_var(out.shape[0], out.shape[1], dh, wd_damp, wd_damp1, fc_end, _p(v),
+                                              _p(out)))
+    return out
 
 
 def median_filter(a, filt_size):

@@ -86,9 +86,10 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (st->lnorm == 5 && (p->mode != 1 || st->n_order != 0))
             die("LNORM", "the global-correlation misfit is built for FWI (MODE 1) without N_ORDER");
         /* smooth_grad.c:17-33: 1 wavenumber damping, 2 smooth2, 3 Gaussian, 4 velocity-dependent Gaussian */
-        if (st->spatfilter == 1 || st->spatfilter == 2 || st->spatfilter == 4)
-            die("SPATFILTER", "the Gaussian gradient filter (3) is built; 1, 2 and 4 are not");
-        if (st->spatfilter == 3 && !(st->fc_end > 0.0f)) die("SPATFILTER", "SPATFILTER 3 needs FC_END > 0");
+        if (st->spatfilter == 1 || st->spatfilter == 2)
+            die("SPATFILTER", "the Gaussian gradient filters (3, 4) are built; 1 and 2 are not");
+        if ((st->spatfilter == 3 || st->spatfilter == 4) && !(st->fc_end > 0.0f))
+            die("SPATFILTER", "SPATFILTER 3 and 4 need FC_END > 0");
         /* NORMALIZE acts in the STF inversion only (stf.c:176-186) and ENV with LNORM 6 only */
         if (st->inv_stf || st->rowi) die("workflow", "STF inversion and ROWI are not built");
         if (st->timewin == 3) die("TIMEWIN", "time windows 1, 2, 4 and 5 are built; the STA/LTA picker (3) is not");
@@ -596,6 +597,12 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
                 for (int k = 0; k < 3; k++)
                     io_ok(dnfwi_gauss_filter(nx, ny, P->dh, st.wd_damp, c_vs, st.fc_end, g[k], &w), "SPATFILTER 3");
                 printf(" gradients smoothed with a %d-point Gaussian (%.2f m)\n", w, w * P->dh);
+            }
+            if (st.spatfilter == 4) { /* gauss_filt_var.c with the current Vs model (FWI_PSV.c:629-631) */
+                for (int k = 0; k < 3; k++)
+                    io_ok(dnfwi_gauss_filter_var(nx, ny, P->dh, st.wd_damp, st.wd_damp1, st.fc_end, vs, g[k]),
+                          "SPATFILTER 4");
+                printf(" gradients smoothed with the velocity-dependent Gaussian\n");
             }
             precondition(r, g); /* FWI_PSV.c:634: precond_PSV */
             const char *suffix[3] = {"vp", "vs", "rho"};

@@ -577,6 +577,43 @@ int dnfwi_gauss_filter(int nx, int ny, float dh, float wd_damp, float c_vs, floa
     return 0;
 }
 
+int dnfwi_gauss_filter_var(int nx, int ny, float dh, float wd_damp, float wd_damp1, float fc_end, const float *vel,
+                           float *g)
+{
+    if (!g || !vel || nx < 1 || ny < 1 || !(dh > 0.0f) || !(fc_end > 0.0f))
+        return dnio_fail_msg("dnfwi_gauss_filter_var: bad argument (FC_END and DH must be positive)");
+    const size_t ncell = (size_t)nx * ny;
+    float *src = malloc(sizeof(float) * ncell);
+    if (!src) return dnio_fail_msg("dnfwi_gauss_filter_var: out of host memory");
+    memcpy(src, g, sizeof(float) * ncell);
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            const float lam = vel[(size_t)i * ny + j] / fc_end;
+            int wx = (int)round((wd_damp * lam) / dh), wz = (int)round((wd_damp1 * lam) / dh);
+            if (wz <= 0) continue; /* g keeps the cell's value */
+            if (wx % 2 == 0) wx += 1;
+            if (wz % 2 == 0) wz += 1;
+            if (wx < 6) wx = 6;
+            if (wz < 6) wz = 6;
+            const int hx = abs(wx) / 2, hz = abs(wz) / 2;
+            const float sigmax = hx / 2, sigmaz = hz / 2; /* integer division, as the reference */
+            const float sx = 2.0 * sigmax * sigmax, sz = 2.0 * sigmaz * sigmaz;
+            float conv = 0.0f, sum = 0.0f;
+            for (int di = -hx; di <= hx; di++) {
+                const int ii = i + di < 0 ? 0 : (i + di >= nx ? nx - 1 : i + di);
+                for (int dj = -hz; dj <= hz; dj++) {
+                    const int jj = j + dj < 0 ? 0 : (j + dj >= ny ? ny - 1 : j + dj);
+                    const float k = exp(-((di * di) / sx) - ((dj * dj) / sz));
+                    conv += src[(size_t)ii * ny + jj] * k;
+                    sum += k;
+                }
+            }
+            g[(size_t)i * ny + j] = conv / sum;
+        }
+    free(src);
+    return 0;
+}
+
 int dnfwi_median_filter(int nx, int ny, int filt_size, float *a)
 {
     if (!a || nx < 1 || ny < 1) return dnio_fail_msg("dnfwi_median_filter: bad argument");

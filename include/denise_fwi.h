@@ -177,6 +177,20 @@ int dnfwi_residual_gc(const float *data, const float *syn, int ntr, int ns, floa
 int dnfwi_time_window(int timewin, int ntr, int ns, float dt, float gamma, float twlength_minus,
                       float twlength_plus, const float *picks, const float *offset, float *diff);
 
+/* Velocity-dependent Gaussian gradient filter (src/gauss_filt_var.c:12-240,
+ * SPATFILTER 4, with the current Vs model as vel, x-major like g): at each
+ * cell, with lam = vel / FC_END, widths wx = round(WD_DAMP lam / DH) and
+ * wz = round(WD_DAMP1 lam / DH); if wz > 0 both are made odd by adding one
+ * to an even width and then raised to at least 6, h = w/2, sigma = h/2 in
+ * integer division, s = 2 sigma^2, and the cell becomes
+ * sum g k / sum k over the (2hx+1) x (2hz+1) window with
+ * k = exp(-(di^2/sx) - (dj^2/sz)), edges repeated; cells with wz <= 0 are
+ * unchanged.  The reference pads the edges by the half width of the maximum
+ * velocity's WD_DAMP window only, so a wider WD_DAMP1 window reads outside
+ * its array; here the edge cells repeat there too. */
+int dnfwi_gauss_filter_var(int nx, int ny, float dh, float wd_damp, float wd_damp1, float fc_end, const float *vel,
+                           float *g);
+
 /* Median model filter (src/smooth_model.c:37-196 with src/median2d.c,
  * MODEL_FILTER 1, applied by FWI_PSV.c:752 to vp, vs and rho after each
  * model update): every cell of the x-major nx x ny array a is replaced by

@@ -28,6 +28,7 @@ Written from the reference's functions, one per function, keeping its
     src/calc_res.c:447-450 (TIMEWIN 5);
   - gauss_filter: src/gauss_filt.c:9-208 (SPATFILTER 3, through
     src/smooth_grad.c:17-33 at src/PSV/FWI_PSV.c:628-631);
+  - gauss_filter_var: src/gauss_filt_var.c:12-240 (SPATFILTER 4);
   - median_filter: src/smooth_model.c:37-196 with src/median2d.c:9-44
     (MODEL_FILTER 1, called by src/PSV/FWI_PSV.c:752).
 Vectors are numpy float32 arrays of shape (nx, ny) (the model-file order).
@@ -404,6 +405,44 @@ def gauss_filter(g, dh, wd_damp, c_vs, fc_end):
         for dj in range(-h, h + 1):
             conv = (conv + pad[h + di:h + di + nx, h + dj:h + dj + ny] * k[di + h, dj + h]).astype(np.float32)
     return conv, fs
+
+
+def _c_round(x):
+    return int(math.copysign(math.floor(abs(x) + 0.5), x))
+
+
+def gauss_filter_var(g, vel, dh, wd_damp, wd_damp1, fc_end):
+    """The velocity-dependent Gaussian smoother, cell by cell in plain
+    loops (small arrays): windows from the local wavelength, kernel sums in
+    float32 in the order x offset outer, y offset inner."""
+    g = np.asarray(g, np.float32)
+    vel = np.asarray(vel, np.float32)
+    nx, ny = g.shape
+    out = g.copy()
+    for i in range(nx):
+        for j in range(ny):
+            lam = F32(vel[i, j] / F32(fc_end))
+            wx = _c_round(float(F32(F32(F32(wd_damp) * lam) / F32(dh))))
+            wz = _c_round(float(F32(F32(F32(wd_damp1) * lam) / F32(dh))))
+            if wz <= 0:
+                continue
+            wx += 1 - wx % 2
+            wz += 1 - wz % 2
+            wx, wz = max(wx, 6), max(wz, 6)
+            hx, hz = abs(wx) // 2, abs(wz) // 2
+            sx = F32(2.0 * float(hx // 2) ** 2)
+            sz = F32(2.0 * float(hz // 2) ** 2)
+            conv = F32(0.0)
+            tot = F32(0.0)
+            for di in range(-hx, hx + 1):
+                ii = min(max(i + di, 0), nx - 1)
+                for dj in range(-hz, hz + 1):
+                    jj = min(max(j + dj, 0), ny - 1)
+                    k = F32(math.exp(float(F32(-(F32(di * di) / sx) - (F32(dj * dj) / sz)))))
+                    conv = F32(conv + F32(g[ii, jj] * k))
+                    tot = F32(tot + k)
+            out[i, j] = F32(conv / tot)
+    return out
 
 
 def median_filter(a, filt_size):

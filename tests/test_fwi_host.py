@@ -609,3 +609,37 @@ def test_residual_gc_pins():
     r1, l1 = fo.residual_gc(data, syn, dt, 1, 1)
     r2, l2b = fo.residual_gc(i1, s1, dt, 2, 1)
     assert np.array_equal(r1[:, :-1], r2[:, :-1]) and abs(l1 - l2b) < 1e-4
+
+
+def test_gauss_filter_var():
+    """dnfwi_gauss_filter_var (SPATFILTER 4) equals the restatement bit for
+    bit on a model with varying velocity and a WD_DAMP1 window wider than
+    the WD_DAMP one."""
+    rng = np.random.default_rng(8)
+    nx, ny = 16, 12
+    g = rng.standard_normal((nx, ny)).astype(np.float32)
+    vel = np.linspace(1200.0, 2600.0, nx * ny, dtype=np.float32).reshape(nx, ny)
+    got = dnfwi.gauss_filter_var(g, vel, 10.0, 0.4, 0.7, 12.0)
+    want = fo.gauss_filter_var(g, vel, 10.0, 0.4, 0.7, 12.0)
+    assert np.array_equal(got, want)
+
+
+def test_gauss_filter_var_pins():
+    """Uniform velocity 2000 m/s, FC_END 10 Hz, DH 10 m: WD_DAMP 0.5 gives
+    x width 10 -> 11 (h 5, sigma 2), WD_DAMP1 0.2 gives y width 4 -> 5,
+    raised to 6 (h 3, sigma 1); a unit spike spreads into the normalised
+    product exp(-k^2/8) x exp(-l^2/2); constant fields stay constant;
+    WD_DAMP1 0 (y width 0) leaves every cell unchanged."""
+    nx, ny = 25, 17
+    vel = np.full((nx, ny), 2000.0, np.float32)
+    g = np.zeros((nx, ny), np.float32)
+    g[12, 8] = 1.0
+    out = fo.gauss_filter_var(g, vel, 10.0, 0.5, 0.2, 10.0)
+    kx, kz = np.exp(-np.arange(-5, 6) ** 2 / 8.0), np.exp(-np.arange(-3, 4) ** 2 / 2.0)
+    want = np.outer(kx, kz) / (kx.sum() * kz.sum())
+    np.testing.assert_allclose(out[7:18, 5:12], want, rtol=3e-6, atol=1e-9)
+    assert np.count_nonzero(out) == 11 * 7
+    c = np.full((nx, ny), -2.5, np.float32)
+    np.testing.assert_allclose(dnfwi.gauss_filter_var(c, vel, 10.0, 0.5, 0.2, 10.0), c, rtol=2e-6)
+    r = np.random.default_rng(2).standard_normal((nx, ny)).astype(np.float32)
+    assert np.array_equal(dnfwi.gauss_filter_var(r, vel, 10.0, 0.5, 0.0, 10.0), r)

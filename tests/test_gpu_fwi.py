@@ -25,7 +25,7 @@ ITERMAX = 4
 
 
 @pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill", "model_filter", "spatfilter", "timewin",
-                                     "surface", "gc"])
+                                     "surface", "gc", "spatfilter4"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
     ORDER 4) filters the source signals and the observed data.  precond:
@@ -50,7 +50,9 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     TWLENGTH_PLUS 0.05 s).  surface: surface waves damped after offset /
     1500 m/s (TIMEWIN 5, TWLENGTH_MINUS 1500).  gc: the global-correlation
     misfit (LNORM 5, GRAD_FORM 1) with residuals computed on the host from
-    the device's seismograms."""
+    the device's seismograms.  spatfilter4: the velocity-dependent Gaussian
+    (SPATFILTER 4, WD_DAMP 0.5, WD_DAMP1 0.3, FC_END 10 Hz) with the current
+    Vs model."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
     pcg = variant == "pcg"
@@ -58,6 +60,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     trkill = variant == "trkill"
     mfilt = variant == "model_filter"
     spat = variant == "spatfilter"
+    spat4 = variant == "spatfilter4"
     timewin = {"timewin": 1, "surface": 5}.get(variant, 0)
     gc = variant == "gc"
     tmp = str(tmp_path)
@@ -94,10 +97,11 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
         tfiles = {sfx: trng.uniform(0.5, 1.5, (nx, ny)).astype(np.float32) for sfx in ("", "_u", "_rho")}
         for sfx, a in tfiles.items():
             a.tofile(os.path.join(tmp, f"taper/t{sfx}.bin"))
-    fcs, fce, order = (4.0, 12.0, 4) if time_filt else (0.0, 10.0 if spat else 0.0, 6)
+    fcs, fce, order = (4.0, 12.0, 4) if time_filt else (0.0, 10.0 if spat or spat4 else 0.0, 6)
     inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=time_filt, fc_start=fcs, fc_end=fce, order=order,
                            eprecond=3 if precond else 0, offset_mute=1 if trkill else 0, offsetc=350.0,
-                           spatfilter=3 if spat else 0, wd_damp=0.5, timewin=timewin, gamma=100.0,
+                           spatfilter=3 if spat else (4 if spat4 else 0), wd_damp=0.5,
+                           wd_damp1=0.3 if spat4 else 0.5, timewin=timewin, gamma=100.0,
                            twlength_minus=1500.0, twlength_plus=0.05, lnorm=5 if gc else 2)
     os.makedirs(os.path.join(tmp, "picks"), exist_ok=True)
     picks = {}
@@ -205,6 +209,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
             sm = [fo.gauss_filter(a, par.dh, 0.5, float(c[1]), fce) for a in grads]
             assert sm[0][1] % 2 == 1 and sm[0][1] >= 5, sm[0][1]     # sigma = (width // 2) // 2 >= 1
             grads = [a for a, _ in sm]
+        if spat4:
+            grads = [fo.gauss_filter_var(a, vs, par.dh, 0.5, 0.3, fce) for a in grads]
         if tapers:
             gt = list(par.gradt)
             assert gt == [10, 25, 70, 86]
