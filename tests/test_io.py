@@ -316,3 +316,22 @@ def test_written_parameter_file_reads_back(tmp_path):
     inpfile.write_workflow(wf, n_order=1)
     s = dnio.read_stage(wf, 1)
     assert s.lnorm == 2 and s.n_order == 1 and s.time_filt == 0
+
+
+@pytest.mark.parametrize("stage,key", [(dict(lnorm=1), "LNORM"), (dict(lnorm=5, n_order=1), "LNORM"),
+                                       (dict(spatfilter=2), "SPATFILTER"), (dict(timewin=3), "TIMEWIN"),
+                                       (dict(spatfilter=3, fc_end=0.0), "SPATFILTER")])
+def test_cli_refuses_workflow_switches_not_built(tmp_path, stage, key):
+    """FWI stages with a misfit, gradient filter or time window that is not
+    built stop with exit status 1 and name the switch, before any device
+    work; a stage of built switches (LNORM 5, SPATFILTER 4, TIMEWIN 5)
+    passes the check and stops at the missing source file."""
+    import inpfile
+    inp, wf = str(tmp_path / "f.inp"), str(tmp_path / "w.inp")
+    inpfile.write_inp(inp, MODE=1, SOURCE_FILE="missing_sources.dat")
+    inpfile.write_workflow(wf, **stage)
+    r = _run_cli(inp, wf, cwd=str(tmp_path))
+    assert r.returncode == 1 and key in r.stderr, r.stderr
+    inpfile.write_workflow(wf, lnorm=5, spatfilter=4, fc_end=10.0, timewin=5)
+    r = _run_cli(inp, wf, cwd=str(tmp_path))
+    assert r.returncode == 1 and "SOURCE_FILE" in r.stderr, r.stderr
