@@ -1019,7 +1019,7 @@ int dn_calc_residuals(dn_ctx *c, const float *obs_vx, const float *obs_vy, const
     for (int comp = 0; comp < 3; ++comp) {
         if (!use[comp] || !obs[comp] || q.ntr == 0) continue;
         HIP_OK(hipMemcpyAsync(q.obs, obs[comp], n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        launch_residual(q, comp, q.obs, c->p.dt, c->stream);
+        launch_residual(q, comp, q.obs, c->p.dt, comp < 2 ? c->res_w[comp] : 1.0f, c->stream);
         if (testshot) launch_l2(q, comp, c->stream);
         HIP_OK(hipStreamSynchronize(c->stream));
     }
@@ -1027,6 +1027,15 @@ int dn_calc_residuals(dn_ctx *c, const float *obs_vx, const float *obs_vy, const
     double v = 0.0;
     HIP_OK(d2h(c->stream, &v, q.l2, sizeof(double)));
     if (l2) *l2 = v;
+    return DN_OK;
+}
+
+int dn_set_residual_weights(dn_ctx *c, float w_vx, float w_vy)
+{
+    if (!c) return fail(DN_ERR_ARG, "null context");
+    if (!(std::isfinite(w_vx) && std::isfinite(w_vy))) return fail(DN_ERR_ARG, "residual weights must be finite");
+    c->res_w[0] = w_vx;
+    c->res_w[1] = w_vy;
     return DN_OK;
 }
 

@@ -68,6 +68,7 @@ struct dn_ctx {
     DevAcq acq;
     std::vector<int> rec_ix, rec_iy, src_ix, src_iy, src_type;
     int qtype = 0, has_force = 0;
+    float res_w[2] = {1.0f, 1.0f};  // COMP_WEIGHT factors of the vx / vy residuals
     int tau_const = 0;
     float ts0 = 0.0f, tp0 = 0.0f, tsij0 = 0.0f;
     KArgs a{};

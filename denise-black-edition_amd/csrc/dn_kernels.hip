@@ -124,9 +124,12 @@ __global__ __launch_bounds__(256) void k_assemble(KArgs a, const float *wc, cons
 // sequential float chain per trace, so each lane owns one trace; a block of
 // 64 lanes stages 64-sample chunks of its 64 traces in LDS so that global
 // reads and writes run along the traces (coalesced) while each lane walks
-// its own trace in the reference's order.
+// its own trace in the reference's order.  w: the component weight of
+// src/calc_res.c:100-123 (COMP_WEIGHT, QUELLTYPB 5-7), applied to the
+// synthetic and the observed samples before the residual as there (float
+// products); 1 leaves every sample unchanged.
 constexpr int RB = 64;  // traces per block = samples per chunk
-__global__ __launch_bounds__(RB) void k_residual(DevAcq q, int comp, const float *obs, float dt)
+__global__ __launch_bounds__(RB) void k_residual(DevAcq q, int comp, const float *obs, float dt, float w)
 {
     __shared__ float S[RB][RB + 1], O[RB][RB + 1];
     const int lane = threadIdx.x, t0 = blockIdx.x * RB;
@@ -134,15 +137,15 @@ __global__ __launch_bounds__(RB) void k_residual(DevAcq q, int comp, const float
     const int nr = q.ntr - t0 < RB ? q.ntr - t0 : RB;
     const float *syn = q.sec[comp];
     float *diff = q.adj[comp];
-    const float syn0 = lane < nr ? syn[(long)(t0 + lane) * ns] : 0.0f;
+    const float syn0 = lane < nr ? syn[(long)(t0 + lane) * ns] * w : 0.0f;
     float intseis = 0.0f;
     for (long c0 = 0; c0 < ns; c0 += RB) {
         const int nc = ns - c0 < RB ? (int)(ns - c0) : RB;
         for (int r = 0; r < nr; ++r) {
             const long g = (long)(t0 + r) * ns + c0 + lane;
             if (lane < nc) {
-                S[r][lane] = syn[g];
-                O[r][lane] = obs[g];
+                S[r][lane] = syn[g] * w;
+                O[r][lane] = obs[g] * w;
             }
         }
         __syncthreads();
@@ -300,11 +303,11 @@ void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const fl
                        invmat1);
 }
 
-void launch_residual(const DevAcq &q, int comp, const float *obs, float dt, void *stream)
+void launch_residual(const DevAcq &q, int comp, const float *obs, float dt, float w, void *stream)
 {
     if (q.ntr <= 0) return;
     dim3 g((q.ntr + RB - 1) / RB);
-    hipLaunchKernelGGL(k_residual, g, dim3(RB), 0, (hipStream_t)stream, q, comp, obs, dt);
+    hipLaunchKernelGGL(k_residual, g, dim3(RB), 0, (hipStream_t)stream, q, comp, obs, dt, w);
 }
 
 void launch_l2(const DevAcq &q, int comp, void *stream)

@@ -111,6 +111,7 @@ def load():
         "dn_run_forward": ([vp, C.c_int], C.c_int),
         "dn_get_seismogram": ([vp, C.c_int, fp], C.c_int),
         "dn_calc_residuals": ([vp, fp, fp, fp, C.c_int, C.POINTER(C.c_double)], C.c_int),
+        "dn_set_residual_weights": ([vp, C.c_float, C.c_float], C.c_int),
         "dn_set_adjoint_sources": ([vp, fp, fp, fp], C.c_int),
         "dn_get_adjoint_sources": ([vp, C.c_int, fp], C.c_int),
         "dn_run_adjoint": ([vp, C.c_int], C.c_int),
@@ -295,6 +296,10 @@ class Context:
         arrs = [_f32(a) for a in (obs_vx, obs_vy, obs_p)]
         _check(load().dn_calc_residuals(self.h, *[_ptr(a) for a in arrs], testshot, C.byref(l2)))
         return l2.value
+
+    def set_residual_weights(self, w_vx=1.0, w_vy=1.0):
+        """COMP_WEIGHT factors of the vx / vy residuals (calc_res.c:100-123)."""
+        _check(load().dn_set_residual_weights(self.h, w_vx, w_vy))
 
     def set_adjoint_sources(self, dvx=None, dvy=None, dp=None):
         arrs = [_f32(a) for a in (dvx, dvy, dp)]

@@ -98,6 +98,8 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if (p->idx != 1 || p->idy != 1) die("IDX/IDY", "FWI runs on every grid point (IDX = IDY = 1)");
         if (p->grad_form != 1 && p->grad_form != 2) die("GRAD_FORM", "1 or 2 expected");
         if (p->trkill && p->readrec == 2) die("TRKILL", "trace killing needs one receiver file (READREC = 1)");
+        /* calc_res_PSV.c:39-61: differences against a second data set */
+        if (p->timelapse == 1) die("TIMELAPSE", "time-lapse residuals (TIMELAPSE = 1) are not built");
     }
     if (p->mode == 1) {
         if (p->grad_method != 1 && p->grad_method != 2) die("GRAD_METHOD", "PCG (1) or L-BFGS (2) expected");
@@ -131,6 +133,7 @@ typedef struct run {
     int offset_mute;      /* OFFSET_MUTE of the current stage, with OFFSETC */
     float offsetc;
     int *flag;            /* this shot's traces to leave out (ntr) */
+    int flag_cap;         /* traces flag, offs and picks hold */
     int timewin;          /* TIMEWIN of the current stage, with GAMMA, TWLENGTH_MINUS/PLUS */
     float gamma, twlength_minus, twlength_plus;
     float *picks, *offs;  /* this shot's picked times (TIMEWIN 1, 4) and offsets (ntr) */
@@ -196,9 +199,18 @@ static int shot_flags(run *r, int ishot)
     dnio_par *P = r->P;
     const int tw = r->timewin == 1 || r->timewin == 2 || r->timewin == 4 || r->timewin == 5;
     if (!r->kill && !r->offset_mute && !tw) return 0;
-    if (!r->flag) r->flag = xmalloc(sizeof(int) * (size_t)(r->ntr ? r->ntr : 1));
-    if (!r->offs) r->offs = xmalloc(sizeof(float) * (size_t)(r->ntr ? r->ntr : 1));
-    if (!r->picks) r->picks = xmalloc(sizeof(float) * (size_t)(r->ntr ? r->ntr : 1));
+    /* READREC 2 reads a receiver file per shot: the buffers follow the
+     * largest ntr so far */
+    if (!r->flag || r->ntr > r->flag_cap) {
+        const size_t n = (size_t)(r->ntr ? r->ntr : 1);
+        free(r->flag);
+        free(r->offs);
+        free(r->picks);
+        r->flag = xmalloc(sizeof(int) * n);
+        r->offs = xmalloc(sizeof(float) * n);
+        r->picks = xmalloc(sizeof(float) * n);
+        r->flag_cap = (int)n;
+    }
     if (r->timewin == 1 || r->timewin == 4) {
         /* time_window.c:29-44 (TIMEWIN 1: PICKS_FILE<shot>.dat) and 49-61
          * (TIMEWIN 4: PICKS_FILE shot<shot>.<POS1><POS2>, one domain: .00) */
@@ -265,6 +277,24 @@ static int *read_kill(const char *path, int nsrc, int ntr)
     return k;
 }
 
+/* COMP_WEIGHT (calc_res_PSV.c:65-67, 118-120; calc_res.c:100-123): with
+ * QUELLTYPB 5 and 7 the vx traces, with 6 and 7 the vy traces, synthetic
+ * and observed, are scaled by C_vp*C_rho before their residuals */
+static float comp_weight(const run *r, int c)
+{
+    const int qb = r->P->quelltypb;
+    const int weighted = (c == 0 && (qb == 5 || qb == 7)) || (c == 1 && (qb == 6 || qb == 7));
+    if (!weighted) return 1.0f;
+    const float tmp = r->c_vp * r->c_rho;
+    return tmp;
+}
+
+static void scale_traces(float *a, size_t n, float w)
+{
+    if (w == 1.0f) return;
+    for (size_t k = 0; k < n; k++) a[k] *= w;
+}
+
 /* residuals of the shot on the context against its observed data;
  * returns its L2 misfit (0 unless a test shot) */
 static double shot_residuals(run *r, int ishot, int test)
@@ -291,10 +321,15 @@ static double shot_residuals(run *r, int ishot, int test)
             if (use[c]) {
                 diff[c] = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
                 dn_ok(dn_get_seismogram(r->ctx, c, syn), "seismograms");
+                scale_traces(obs[c], (size_t)r->ntr * P->nt, comp_weight(r, c));
+                scale_traces(syn, (size_t)r->ntr * P->nt, comp_weight(r, c));
                 io_ok(dnfwi_residual_gc(obs[c], syn, r->ntr, P->nt, P->dt, P->grad_form, test, fl ? r->flag : NULL,
                                         diff[c], &l2),
                       "LNORM 5 residuals");
-                if (fl)
+                /* calc_res.c:526-537: TIMEWIN 1, 2 and 4 act on every misfit;
+                 * TIMEWIN 5 sits in the LNORM 2 branch (calc_res.c:447-450)
+                 * and leaves LNORM 5 residuals alone */
+                if (fl && r->timewin != 5)
                     io_ok(dnfwi_time_window(r->timewin, r->ntr, P->nt, P->dt, r->gamma, r->twlength_minus,
                                             r->twlength_plus, r->picks, r->offs, diff[c]),
                           "TIMEWIN");
@@ -307,6 +342,7 @@ static double shot_residuals(run *r, int ishot, int test)
         free(syn);
         return l2;
     }
+    dn_ok(dn_set_residual_weights(r->ctx, comp_weight(r, 0), comp_weight(r, 1)), "residual weights");
     dn_ok(dn_calc_residuals(r->ctx, obs[0], obs[1], obs[2], test, &l2), "residuals");
     if (shot_flags(r, ishot)) {
         float *diff[3] = {NULL, NULL, NULL};
@@ -340,6 +376,8 @@ static double shot_rtm_sources(run *r, int ishot, int test)
             io_ok(dnio_read_su(r->name, r->ntr, P->nt, obs), r->name);
             if (r->time_filt) io_ok(dnio_tdfilt(obs, r->ntr, P->nt, P->dt, r->order, r->fc, r->fc_start), "TIME_FILT");
             dn_ok(dn_get_seismogram(r->ctx, c, syn), "seismograms");
+            scale_traces(obs, n, comp_weight(r, c));
+            scale_traces(syn, n, comp_weight(r, c));
             io_ok(dnfwi_rtm_adjoint_source(obs, syn, r->ntr, P->nt, P->dt, P->grad_form, test, diff[c], &l2c),
                   "RTM adjoint source");
             l2 += l2c;

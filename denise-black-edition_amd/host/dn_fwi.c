@@ -15,6 +15,7 @@
 #include "denise_fwi.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -538,6 +539,16 @@ int dnfwi_gauss_filter(int nx, int ny, float dh, float wd_damp, float c_vs, floa
     if (fs % 2 == 0) fs += fs > 0 ? 1 : -1;
     if (width) *width = fs;
     const int hfs = abs(fs) / 2, n = 2 * hfs + 1;
+    /* |width| 1 or 3 gives sigma = 0 and a 0/0 kernel (NaN everywhere, as
+     * in the reference): refused */
+    if (hfs < 2) {
+        char msg[200];
+        snprintf(msg, sizeof msg,
+                 "dnfwi_gauss_filter: SPATFILTER 3 filter width %d (|width| below 5) gives a zero Gaussian "
+                 "sigma and a 0/0 kernel; raise WD_DAMP",
+                 fs);
+        return dnio_fail_msg(msg);
+    }
     const float sigma = hfs / 2; /* integer division, as the reference */
     const float s = 2.0 * sigma * sigma;
     const size_t ncell = (size_t)nx * ny;

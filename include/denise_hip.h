@@ -133,6 +133,14 @@ int dn_get_seismogram(dn_ctx *ctx, int comp, float *out);
 int dn_calc_residuals(dn_ctx *ctx, const float *obs_vx, const float *obs_vy,
                       const float *obs_p, int testshot, double *l2);
 
+/* Component weights of the residuals (COMP_WEIGHT, src/calc_res.c:100-123,
+ * set by src/PSV/calc_res_PSV.c:65-67 and 118-120 for QUELLTYPB 5, 6, 7):
+ * dn_calc_residuals multiplies the synthetic and the observed samples of
+ * vx by w_vx and of vy by w_vy (float products) before it forms their
+ * residuals; the reference's weight is C_vp*C_rho of the weighted
+ * component, 1 otherwise.  Default 1, 1 (no change). */
+int dn_set_residual_weights(dn_ctx *ctx, float w_vx, float w_vy);
+
 /* Adjoint sources supplied directly (ntr x NT each, NULL = zero):
  * sectionvxdiff / sectionvydiff / sectionpdiff of the reference. */
 int dn_set_adjoint_sources(dn_ctx *ctx, const float *dvx, const float *dvy, const float *dp);

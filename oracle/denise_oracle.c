@@ -185,6 +185,7 @@ struct orc_ctx {
     arr gvp, gvs, grho;                 /* assembled per-shot gradients */
     int qtype;                          /* global QUELLTYP */
     int nan_step;
+    float resw[2];                      /* COMP_WEIGHT factors of vx / vy (0 = none set) */
 };
 
 #define PSIX(a, j, h) ((a)[(size_t)((j) - 1) * (size_t)c->fw2 + (size_t)((h) - 1)])
@@ -1156,16 +1157,22 @@ int orc_adjoint(orc_ctx *c, int nsteps) { return run_psv(c, 1, nsteps); }
  * with the observed-trace reading of src/inseis.c:96-103 (sample 1 is zero
  * on input and replaced by the synthetic at calc_res.c:419) and the misfit
  * sum :553-556.  obs arrays hold the SU samples 1..NT per trace. */
-static double calc_res_comp(orc_ctx *c, const real *syn, const float *obs, real *diff, double L2, int testshot)
+static double calc_res_comp(orc_ctx *c, const real *syn, const float *obs, real *diff, double L2, int testshot,
+                            float w)
 {
     const int ns = c->P.nt;
     const real DT = (real)c->P.dt;
     real *data = (real *)malloc(sizeof(real) * (size_t)ns);
+    real *sw = (real *)malloc(sizeof(real) * (size_t)ns);
     for (int i = 0; i < c->ntr; i++) {
-        const real *s = &syn[(size_t)i * ns];
+        /* calc_res.c:100-123: with COMP_WEIGHT the synthetic and observed
+         * samples are multiplied by tmp = C_vp*C_rho (float) first */
+        const real tmp = (real)w;
+        const real *s = sw;
+        for (int j = 0; j < ns; j++) sw[j] = syn[(size_t)i * ns + j] * tmp;
         real *df = &diff[(size_t)i * ns];
         data[0] = 0.0;
-        for (int j = 1; j < ns; j++) data[j] = obs[(size_t)i * ns + j];
+        for (int j = 1; j < ns; j++) data[j] = obs[(size_t)i * ns + j] * tmp;
         for (int j = 0; j < ns; j++) df[j] = 0.0;
         real intseis = 0.0;
         int invtime = ns;
@@ -1182,7 +1189,16 @@ static double calc_res_comp(orc_ctx *c, const real *syn, const float *obs, real 
         }
     }
     free(data);
+    free(sw);
     return L2;
+}
+
+/* COMP_WEIGHT of src/calc_res.c:100-123 for the vx and vy residuals */
+int orc_set_res_weights(orc_ctx *c, float w_vx, float w_vy)
+{
+    c->resw[0] = w_vx;
+    c->resw[1] = w_vy;
+    return 0;
 }
 
 /* src/PSV/calc_res_PSV.c:8-184 (vx, vy, p branches by QUELLTYPB) */
@@ -1190,9 +1206,12 @@ double orc_calc_res(orc_ctx *c, const float *obs_vx, const float *obs_vy, const 
 {
     const int QB = c->P.quelltypb;
     double L2 = 0.0;
-    if ((QB == 1 || QB == 3 || QB == 5 || QB == 7) && obs_vx) L2 = calc_res_comp(c, c->sec[0], obs_vx, c->adj[0], L2, testshot);
-    if ((QB == 1 || QB == 2 || QB == 6 || QB == 7) && obs_vy) L2 = calc_res_comp(c, c->sec[1], obs_vy, c->adj[1], L2, testshot);
-    if (QB >= 4 && obs_p) L2 = calc_res_comp(c, c->sec[2], obs_p, c->adj[2], L2, testshot);
+    const float wx = c->resw[0] != 0.0f ? c->resw[0] : 1.0f, wy = c->resw[1] != 0.0f ? c->resw[1] : 1.0f;
+    if ((QB == 1 || QB == 3 || QB == 5 || QB == 7) && obs_vx)
+        L2 = calc_res_comp(c, c->sec[0], obs_vx, c->adj[0], L2, testshot, wx);
+    if ((QB == 1 || QB == 2 || QB == 6 || QB == 7) && obs_vy)
+        L2 = calc_res_comp(c, c->sec[1], obs_vy, c->adj[1], L2, testshot, wy);
+    if (QB >= 4 && obs_p) L2 = calc_res_comp(c, c->sec[2], obs_p, c->adj[2], L2, testshot, 1.0f);
     return L2;
 }
 

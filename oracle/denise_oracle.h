@@ -71,6 +71,7 @@ int orc_set_field(orc_ctx *c, int id, const void *in);
 /* calc_res() LNORM=2 on vx / vy; returns the L2 term; fills the adjoint sources. */
 double orc_calc_res(orc_ctx *c, const float *obs_vx, const float *obs_vy,
                     const float *obs_p, int testshot);
+int orc_set_res_weights(orc_ctx *c, float w_vx, float w_vy);
 int orc_set_adjoint(orc_ctx *c, const float *dvx, const float *dvy, const float *dp);
 /* psv() mode 1 (adjoint + correlation) for nsteps time steps. */
 int orc_adjoint(orc_ctx *c, int nsteps);

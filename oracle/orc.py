@@ -49,6 +49,7 @@ def load(prec="f32"):
         "orc_zero": ([vp], C.c_int),
         "orc_set_field": ([vp, C.c_int, vp], C.c_int),
         "orc_calc_res": ([vp, fp, fp, fp, C.c_int], C.c_double),
+        "orc_set_res_weights": ([vp, C.c_float, C.c_float], C.c_int),
         "orc_set_adjoint": ([vp, fp, fp, fp], C.c_int),
         "orc_adjoint": ([vp, C.c_int], C.c_int),
         "orc_assemble": ([vp], C.c_int),
@@ -169,6 +170,9 @@ class Oracle:
     def residuals(self, obs_vx=None, obs_vy=None, obs_p=None, testshot=1):
         arrs = [_f32(a) for a in (obs_vx, obs_vy, obs_p)]
         return self.lib.orc_calc_res(self.h, *[_p(a) for a in arrs], testshot)
+
+    def set_residual_weights(self, w_vx=1.0, w_vy=1.0):
+        self._rc(self.lib.orc_set_res_weights(self.h, w_vx, w_vy))
 
     def set_adjoint_sources(self, dvx=None, dvy=None, dp=None):
         arrs = [_f32(a) for a in (dvx, dvy, dp)]

@@ -496,6 +496,16 @@ def test_gauss_filter(wd_damp, c_vs, fc_end, dh):
         assert np.array_equal(got, g)
 
 
+@pytest.mark.parametrize("wd_damp", [0.05, 0.15, -0.05, -0.15])
+def test_gauss_filter_refuses_zero_sigma(wd_damp):
+    """Widths 1 and 3 (here round(WD_DAMP 2000 / 10 / 10) = 1, 3 and their
+    negatives) give hfs <= 1, sigma = hfs / 2 = 0 and a 0/0 kernel, NaN in
+    every gradient cell in the reference: the filter refuses them."""
+    g = np.ones((20, 15), np.float32)
+    with pytest.raises(Exception, match="zero Gaussian"):
+        dnfwi.gauss_filter(g, 10.0, wd_damp, 2000.0, 10.0)
+
+
 def test_gauss_filter_pins():
     """Width round(WD_DAMP C_vs / FC_END / DH) = 10 -> 11, h = 5, sigma = 2
     (integer h/2), s = 8: a unit spike away from the edges spreads into the
@@ -521,7 +531,8 @@ def test_gauss_filter_pins():
     fa, fb, fab = (fo.gauss_filter(x, 10.0, 0.5, 2000.0, 10.0)[0] for x in (a, b, a + b))
     np.testing.assert_allclose(fab, fa + fb, atol=1e-5)
     assert fo.gauss_filter(a, 10.0, 0.25, 1000.0, 10.0)[1] == 3      # 2.5 -> 3 (odd already)
-    assert dnfwi.gauss_filter(a, 10.0, 0.25, 1000.0, 10.0)[1] == 3
+    with pytest.raises(Exception, match="filter width 3"):          # sigma 0: refused
+        dnfwi.gauss_filter(a, 10.0, 0.25, 1000.0, 10.0)
 
 
 @pytest.mark.parametrize("timewin", [0, 1, 2, 4, 5])

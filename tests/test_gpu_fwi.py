@@ -25,7 +25,7 @@ ITERMAX = 4
 
 
 @pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill", "model_filter", "spatfilter", "timewin",
-                                     "surface", "gc", "spatfilter4"])
+                                     "surface", "gc", "spatfilter4", "gc_surface", "weights"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
     ORDER 4) filters the source signals and the observed data.  precond:
@@ -52,7 +52,12 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     misfit (LNORM 5, GRAD_FORM 1) with residuals computed on the host from
     the device's seismograms.  spatfilter4: the velocity-dependent Gaussian
     (SPATFILTER 4, WD_DAMP 0.5, WD_DAMP1 0.3, FC_END 10 Hz) with the current
-    Vs model."""
+    Vs model.  gc_surface: LNORM 5 with TIMEWIN 5, which the reference
+    applies in its LNORM 2 branch only (calc_res.c:447-450): the same
+    iterations as LNORM 5 without a time window.  weights: QUELLTYPB 7 with
+    SEISMO 4 (vx, vy and p residuals; the pressure adjoint source), whose vx
+    and vy synthetic and observed samples are scaled by C_vp*C_rho of the
+    start model before the residuals (COMP_WEIGHT, calc_res.c:100-123)."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
     pcg = variant == "pcg"
@@ -61,8 +66,10 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     mfilt = variant == "model_filter"
     spat = variant == "spatfilter"
     spat4 = variant == "spatfilter4"
-    timewin = {"timewin": 1, "surface": 5}.get(variant, 0)
-    gc = variant == "gc"
+    timewin = {"timewin": 1, "surface": 5, "gc_surface": 5}.get(variant, 0)
+    gc = variant in ("gc", "gc_surface")
+    weights = variant == "weights"
+    qb, seismo = (7, 4) if weights else (1, 1)
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
@@ -74,10 +81,11 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     receivers = [(40.0 + 20.0 * k, 20.0) for k in range(40)]
     _files(tmp, nx, ny, dh, true, sources, receivers, mname="model/true")
     inpfile.write_inp(os.path.join(tmp, "true.inp"), NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
-                      MFILE="model/true", SEIS_FILE_VX="obs/d_x.su", SEIS_FILE_VY="obs/d_y.su")
+                      MFILE="model/true", SEIS_FILE_VX="obs/d_x.su", SEIS_FILE_VY="obs/d_y.su",
+                      SEIS_FILE_P="obs/d_p.su", SEISMO=seismo)
     _run(tmp, "true.inp")
     _files(tmp, nx, ny, dh, bg, sources, receivers, mname="model/start")
-    inpfile.write_inp(os.path.join(tmp, "fwi.inp"), MODE=1, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+    inpfile.write_inp(os.path.join(tmp, "fwi.inp"), MODE=1, QUELLTYPB=qb, SEISMO=seismo, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
                       MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1", ITERMAX=ITERMAX, INV_MOD_OUT=1,
                       MISFIT_LOG_FILE="misfit.dat", SWS_TAPER_GRAD_HOR=int(precond),
                       GRADT="10, 25, 70, 86" if tapers else "3, 12, 60, 62",
@@ -120,8 +128,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                vslowerlim=par.vslowerlim, rhoupperlim=par.rhoupperlim, rholowerlim=par.rholowerlim,
                scalerho=st.scalerho, inv_vp_iter=st.inv_vp_iter, inv_vs_iter=st.inv_vs_iter,
                inv_rho_iter=st.inv_rho_iter, invmat1=par.invmat1)
-    obs = {i: [dnio.read_su(os.path.join(tmp, f"obs/d_{c}.su.shot{i}"), rec.shape[1], nt) for c in "xy"]
-           for i in (1, 2)}
+    obs = {i: [dnio.read_su(os.path.join(tmp, f"obs/d_{c}.su.shot{i}"), rec.shape[1], nt)
+               for c in ("xyp" if weights else "xy")] for i in (1, 2)}
     if time_filt:
         obs = {i: [tf.apply_tdfilt(o, par.dt, order, fce, fcs) for o in v] for i, v in obs.items()}
     g = dnhip.Context(p)
@@ -160,6 +168,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                 diffs.append(d)
             g.set_adjoint_sources(diffs[0], diffs[1], None)
             return l2
+        if weights:
+            return g.residuals(obs[i][0], obs[i][1], obs[i][2], 1)
         l2 = g.residuals(obs[i][0], obs[i][1], None, 1)
         if not trkill and not timewin:
             return l2
@@ -188,6 +198,9 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
 
     lb = fo.PCG(nx, ny) if pcg else fo.LBFGS(nx, ny, par.nlbfgs)
     c = (fo.model_mean(vp), fo.model_mean(vs), fo.model_mean(rho))
+    if weights:
+        w = float(np.float32(c[0]) * np.float32(c[2]))
+        g.set_residual_weights(w, w)
     carry = ([0.0, 0.0, 0.0], [0.0, 0.0, 0.0])
     lines, l2s = [], []
     for it in range(1, ITERMAX + 1):
@@ -390,3 +403,70 @@ def test_energy_weights_equal_field_sums(eprecond):
                 want = (want + v2()).astype(np.float32)
         assert np.array_equal(wr, want)
     g.close()
+
+
+def test_cli_fwi_per_shot_receiver_files_of_different_lengths(tmp_path):
+    """READREC 2 (one receiver file per shot, receiver.c:39-42) with the
+    offset mute (OFFSET_MUTE 1): shot 2 has twice the receivers of shot 1,
+    so the per-trace flags and offsets must follow the larger count.  The
+    first iteration's misfit in the log equals the library run through the
+    Python binding with each shot's own receivers and muted traces."""
+    tmp = str(tmp_path)
+    nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 300
+    bg = synth.layered_model(nx, ny, dh, seed=9, sigma=3.0)
+    true = tuple(a.copy() for a in bg)
+    true[0][40:56, 10:20] *= 1.08
+    sources = [(300.0, 30.0, 15.0, 1.0, 1), (650.0, 30.0, 15.0, 1.0, 1)]
+    recs = {1: [(60.0 + 40.0 * k, 20.0) for k in range(20)], 2: [(40.0 + 20.0 * k, 20.0) for k in range(40)]}
+    _files(tmp, nx, ny, dh, true, sources, recs[2], mname="model/true")
+    for i, rl in recs.items():
+        with open(os.path.join(tmp, f"receiver_shot_{i}.dat"), "w") as f:
+            for (x, y) in rl:
+                f.write(f"{x:.3f}\t{y:.3f}\n")
+    common = dict(NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt, READREC=2)
+    inpfile.write_inp(os.path.join(tmp, "true.inp"), MFILE="model/true", SEIS_FILE_VX="obs/d_x.su",
+                      SEIS_FILE_VY="obs/d_y.su", **common)
+    _run(tmp, "true.inp")
+    _files(tmp, nx, ny, dh, bg, sources, recs[2], mname="model/start")
+    inpfile.write_inp(os.path.join(tmp, "fwi.inp"), MODE=1, MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1",
+                      ITERMAX=1, MISFIT_LOG_FILE="misfit.dat", **common)
+    inpfile.write_workflow(os.path.join(tmp, "wf.inp"), offset_mute=1, offsetc=350.0)
+    _run(tmp, "fwi.inp", "wf.inp")
+
+    par, p, m, _, shots, src = _reference_run(tmp, "fwi.inp")
+    g = dnhip.Context(p)
+    g.set_model(*m[:3])
+    l2 = 0.0
+    for i in (1, 2):
+        cwd = os.getcwd()
+        os.chdir(tmp)
+        try:
+            rec = dnio.read_receivers(par, i)[:2]
+        finally:
+            os.chdir(cwd)
+        assert rec.shape[1] == len(recs[i])
+        loc, sig = shots[i - 1]
+        g.set_sources(loc, sig)
+        g.set_receivers(rec)
+        g.forward()
+        obs = [dnio.read_su(os.path.join(tmp, f"obs/d_{c}.su.shot{i}"), rec.shape[1], nt) for c in "xy"]
+        g.residuals(obs[0], obs[1], None, 1)
+        xs, ys = np.float32(src[0, i - 1]), np.float32(src[1, i - 1])
+        xr = (rec[0].astype(np.float32) * np.float32(dh)).astype(np.float32)
+        yr = (rec[1].astype(np.float32) * np.float32(dh)).astype(np.float32)
+        off = np.sqrt((((xs - xr) * (xs - xr)).astype(np.float32) +
+                       ((ys - yr) * (ys - yr)).astype(np.float32)).astype(np.float64)).astype(np.float32)
+        fl = off >= np.float32(350.0)
+        assert fl.any() and not fl.all()
+        l2_shot = 0.0
+        for comp in (0, 1):
+            d = g.adjoint_sources(comp)
+            for tr in range(d.shape[0]):
+                if not fl[tr]:
+                    for k in range(d.shape[1] - 1, -1, -1):
+                        l2_shot += 0.5 * float(d[tr, k]) * float(d[tr, k])
+        l2 += l2_shot
+    g.close()
+    log = open(os.path.join(tmp, "misfit.dat")).read().split()
+    print(f"[readrec2] misfit log {log[7]} python {l2:e}", flush=True)
+    assert log[7] == "%e" % l2

@@ -243,6 +243,64 @@ def test_gradient(grad_form, L, invmat1, dtinv):
         assert e32 <= GRAD_TOL, f"{tag} {name}: {e32}"
 
 
+@pytest.mark.parametrize("quelltypb", [2, 3, 4, 5, 6, 7])
+def test_gradient_adjoint_source_types(quelltypb):
+    """QUELLTYPB 2-7 against the oracle: the adjoint forces in y (2, 6, 7)
+    and in x (3, 5, 7) of src/PSV/update_v_PML_PSV.c:755-768, the adjoint
+    pressure source of src/psource.c:41-52 (4-7) driven by the p residuals
+    of src/PSV/calc_res_PSV.c:133-180 (SEISMO 4 records p), and for 5-7 the
+    COMP_WEIGHT scaling of the weighted component's synthetic and observed
+    samples by C_vp*C_rho (src/calc_res.c:100-123)."""
+    case = make_case(nx=72, ny=56, nt=500, fdorder=8, mode=1, seismo=4, quelltypb=quelltypb, dtinv=2,
+                     fc=15.0, nrec=30)
+    p, m, src, sig, rec = case
+    mt = {k: (v.copy() if v is not None else None) for k, v in m.items()}
+    mt["vp"][30:42, 8:16] *= 1.08
+    mt["vs"][30:42, 8:16] *= 1.08
+    t = orc.Oracle(p, mt["vp"], mt["vs"], mt["rho"], prec="f32")
+    t.set_sources(src, sig)
+    t.set_receivers(rec)
+    t.forward()
+    obs = [t.seismogram(c).astype(np.float32) for c in (0, 1, 2)]
+    w = np.float32(np.float32(m["vp"].mean()) * np.float32(m["rho"].mean()))
+    wx = float(w) if quelltypb in (5, 7) else 1.0
+    wy = float(w) if quelltypb in (6, 7) else 1.0
+    use = [quelltypb in (1, 3, 5, 7), quelltypb in (1, 2, 6, 7), quelltypb >= 4]
+    runs = {}
+    for name, r in (("f32", orc.Oracle(p, m["vp"], m["vs"], m["rho"], prec="f32")),
+                    ("gpu", dnhip.Context(p))):
+        if name == "gpu":
+            r.set_model(m["vp"], m["vs"], m["rho"])
+        r.set_residual_weights(wx, wy)
+        r.set_sources(src, sig)
+        r.set_receivers(rec)
+        r.forward()
+        l2 = r.residuals(*[o if u else None for o, u in zip(obs, use)], 1)
+        r.adjoint()
+        r.assemble()
+        runs[name] = (r, l2)
+    (o32, l2o), (g, l2g) = runs["f32"], runs["gpu"]
+    tag = f"grad-quelltypb{quelltypb}"
+    print(f"[{tag}] weights {wx:.6e} {wy:.6e}; L2 oracle {l2o:.9e} gpu {l2g:.9e}")
+    assert l2o > 0
+    assert abs(l2g - l2o) <= 1e-5 * abs(l2o)
+    for comp in (0, 1, 2):
+        ref = o32.adjoint_sources(comp)
+        got = g.adjoint_sources(comp)
+        if not use[comp]:
+            assert not np.any(got) and not np.any(ref), comp
+            continue
+        assert np.abs(ref).max() > 0
+        assert rel_l2(got, ref) <= SEIS_TOL, (tag, comp)
+    for which, name in enumerate(["lam", "mu", "rho", "vp", "vs", "rho_grad"]):
+        ref = o32.gradient(which)
+        got = g.gradient(which)
+        e32 = rel_l2(got, ref)
+        print(f"[{tag}] {name}: relL2 vs f32 {e32:.2e} exact={bool(np.array_equal(got, ref))}")
+        assert np.abs(ref).max() > 0
+        assert e32 <= GRAD_TOL, f"{tag} {name}: {e32}"
+
+
 def test_nan_probe_reports_unstable():
     case = make_case(nx=64, ny=48, nt=120, fdorder=4)
     p, m, src, sig, rec = case

@@ -302,6 +302,21 @@ def test_cli_refuses_what_is_not_built(tmp_path):
     assert r.returncode == 1 and "usage" in r.stderr
 
 
+def test_cli_refuses_timelapse(tmp_path):
+    """TIMELAPSE 1 (residuals of differences to a second data set,
+    calc_res_PSV.c:39-61) is not built: exit status 1 naming the switch,
+    before any device work; QUELLTYPB 7 (weighted components) passes."""
+    import inpfile
+    inp, wf = str(tmp_path / "f.inp"), str(tmp_path / "w.inp")
+    inpfile.write_workflow(wf)
+    inpfile.write_inp(inp, MODE=1, SOURCE_FILE="missing_sources.dat", TIMELAPSE=1)
+    r = _run_cli(inp, wf, cwd=str(tmp_path))
+    assert r.returncode == 1 and "TIMELAPSE" in r.stderr, r.stderr
+    inpfile.write_inp(inp, MODE=1, SOURCE_FILE="missing_sources.dat", QUELLTYPB=7, SEISMO=4)
+    r = _run_cli(inp, wf, cwd=str(tmp_path))
+    assert r.returncode == 1 and "SOURCE_FILE" in r.stderr, r.stderr
+
+
 def test_written_parameter_file_reads_back(tmp_path):
     import inpfile
     path = str(tmp_path / "p.inp")
