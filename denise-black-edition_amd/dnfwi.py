@@ -44,6 +44,7 @@ def load():
         "dnfwi_lbfgs_destroy": ([vp_], None),
         "dnfwi_lbfgs_reset": ([vp_], None),
         "dnfwi_lbfgs_pointer": ([vp_], C.c_int),
+        "dnfwi_lbfgs_save": ([vp_, C.c_char_p, fp, fp, fp], C.c_int),
         "dnfwi_lbfgs_direction": ([vp_, C.c_int, C.c_float, C.c_float, C.c_float, fp, fp, fp, fp, fp, fp],
                                   C.c_int),
         "dnfwi_model_mean": ([fp, C.c_int, C.c_int], C.c_float),
@@ -116,6 +117,11 @@ class LBFGS:
         dnio._check(load().dnfwi_lbfgs_direction(self.h, it, c_vp, c_vs, c_rho, *[_p(a) for a in d],
                                                  *[_p(a) for a in m]))
         return tuple(d)
+
+    def save(self, jacobian, d_vp, d_vs, d_rho):
+        """Write the reference's JACOBIAN_*.old files of the last direction."""
+        d = [_f(a) for a in (d_vp, d_vs, d_rho)]
+        dnio._check(load().dnfwi_lbfgs_save(self.h, jacobian.encode(), *[_p(a) for a in d]))
 
     def pointer(self):
         return load().dnfwi_lbfgs_pointer(self.h)

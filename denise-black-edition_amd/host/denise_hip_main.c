@@ -14,12 +14,17 @@
  *           - the gradient of src/PSV/grad_obj_psv.c:130-262 per shot
  *             (forward with snapshot store, L2 residual against
  *             DATA_DIR_{x,y,p}.su.shot<N>, adjoint, ass_gradPSV), stacked
- *             over shots, written as JACOBIAN_grad_vp, _vs, _rho (float32,
- *             x-major, NX x NY) and, with RTM_SHOT = 1, the per-shot images
- *             JACOBIAN_P_image_shot_N and JACOBIAN_S_image_shot_N
- *             (src/PSV/RTM_PSV_out_shot.c); the misfit is the L2 sum over
- *             the test shots;
- *           - the L-BFGS or PCG direction, the parabolic step-length search over
+ *             over shots, written after preconditioning as JACOBIAN_grad_vp,
+ *             _vs, _rho (float32, x-major, NX x NY; files the reference does
+ *             not write, kept for inspection) and, with RTM_SHOT = 1, the
+ *             per-shot images JACOBIAN_P_image_shot_N and
+ *             JACOBIAN_S_image_shot_N (src/PSV/RTM_PSV_out_shot.c); the
+ *             misfit is the L2 sum over the test shots;
+ *           - the L-BFGS or PCG direction; with L-BFGS the files of
+ *             src/PSV/extract_LBFGS_PSV.c:80-249 (JACOBIAN_p_vp.old, _p.old,
+ *             _c.old, _p_vs.old, _p_u.old, _c_u.old, _p_mrho.old, _p_rho.old,
+ *             _c_rho.old, each also as .old.0.0);
+ *           - the parabolic step-length search over
  *             the test shots (src/PSV/step_length_est_psv.c,
  *             src/PSV/obj_psv.c) and the model update
  *             (src/PSV/calc_mat_change_test_PSV.c), written as
@@ -650,9 +655,11 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
             }
             printf(" misfit (L2, test shots): %.9e\n", L2sum);
             /* FWI_PSV.c:636-676: PCG or L-BFGS search direction */
-            if (lb)
+            if (lb) {
                 io_ok(dnfwi_lbfgs_direction(lb, iter, c_vp, c_vs, c_rho, g[0], g[1], g[2], vp, vs, rho), "L-BFGS");
-            else
+                /* extract_LBFGS_PSV.c:80-249: JACOBIAN_p*.old, _c*.old */
+                io_ok(dnfwi_lbfgs_save(lb, P->jacobian, g[0], g[1], g[2]), "L-BFGS files");
+            } else
                 io_ok(dnfwi_pcg_direction(pc, P->pcg_beta, g[0], g[1], g[2], NULL), "PCG");
             if (iter == 1) min_iter_help = P->min_iter; /* FWI_PSV.c:687-690 */
             obj_env oe = {r, taus, taup};

@@ -13,6 +13,7 @@
  * (1..NLBFGS) starts at (k-1)*nvec.
  */
 #include "denise_fwi.h"
+#include "denise_io.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -168,6 +169,30 @@ int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, flo
             g_vp[k] = g_vp[k] * c_vp;
             g_vs[k] = g_vs[k] * c_vs;
             g_rho[k] = g_rho[k] * c_rho;
+        }
+    }
+    return 0;
+}
+
+int dnfwi_lbfgs_save(const dnfwi_lbfgs *lb, const char *jacobian, const float *d_vp, const float *d_vs,
+                     const float *d_rho)
+{
+    if (!lb || !jacobian || !d_vp || !d_vs || !d_rho) return dnio_fail_msg("dnfwi_lbfgs_save: null argument");
+    if (!lb->have_old) return dnio_fail_msg("dnfwi_lbfgs_save: no direction taken yet");
+    const size_t nc = lb->ncell;
+    /* extract_LBFGS_PSV.c:80-249: per parameter the model, the normalised
+     * gradient, then the direction; vp, vs, rho ([rho | vs | vp] slots 2, 1, 0) */
+    static const char *const names[3][3] = {
+        {"p_vp", "p", "c"}, {"p_vs", "p_u", "c_u"}, {"p_mrho", "p_rho", "c_rho"}};
+    static const int slot[3] = {2, 1, 0};
+    const float *dir[3] = {d_vp, d_vs, d_rho};
+    char name[1024];
+    for (int k = 0; k < 3; k++) {
+        const float *arr[3] = {lb->mold + (size_t)slot[k] * nc, lb->gold + (size_t)slot[k] * nc, dir[k]};
+        for (int f = 0; f < 3; f++) {
+            if ((size_t)snprintf(name, sizeof name, "%s_%s.old", jacobian, names[k][f]) >= sizeof name)
+                return dnio_fail_msg("dnfwi_lbfgs_save: JACOBIAN path too long");
+            if (dnio_write_model(name, arr[f], lb->nx, lb->ny)) return -1;
         }
     }
     return 0;

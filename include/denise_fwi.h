@@ -50,6 +50,19 @@ int dnfwi_lbfgs_pointer(const dnfwi_lbfgs *lb);
 int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, float c_rho, float *g_vp,
                           float *g_vs, float *g_rho, const float *vp, const float *vs, const float *rho);
 
+/* The files the reference's extract_LBFGS_PSV writes after each direction
+ * (src/PSV/extract_LBFGS_PSV.c:80-249, read back by store_LBFGS_PSV.c:91-160),
+ * in its order, each as <jacobian>_<name>.old.0.0 (the one rank's part) and
+ * <jacobian>_<name>.old (mergemod's whole-model file, identical on one rank),
+ * float32 NX*NY in model-file order:
+ *   _p_vp.old / _p_vs.old / _p_mrho.old   the model the direction was taken at,
+ *   _p.old / _p_u.old / _p_rho.old        its normalised gradients C*g,
+ *   _c.old / _c_u.old / _c_rho.old        the direction d_vp, d_vs, d_rho that
+ *                                          dnfwi_lbfgs_direction returned.
+ * Call after dnfwi_lbfgs_direction. */
+int dnfwi_lbfgs_save(const dnfwi_lbfgs *lb, const char *jacobian, const float *d_vp, const float *d_vs,
+                     const float *d_rho);
+
 /* Preconditioned conjugate gradients (GRAD_METHOD = 1): PCG_old and PCG_dir
  * of 3*nx*ny floats [vp | vs | rho] (src/PSV/FWI_PSV.c:304-316), zero at
  * the start and after each stage (zero_PCG, FWI_PSV.c:802-805). */

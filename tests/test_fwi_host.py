@@ -62,6 +62,43 @@ def test_lbfgs_direction_matches_oracle():
     assert pointers == [1, 1, 2, 3, 1, 2, 3, 1]
 
 
+def test_lbfgs_files_hold_model_gradient_and_direction(tmp_path):
+    """dnfwi_lbfgs_save after each direction writes the nine files of
+    extract_LBFGS_PSV.c:80-249 (model, normalised gradient C*g, direction
+    per parameter), each also as the one rank's .old.0.0, in model-file
+    order; their values equal the oracle's L-BFGS state ([rho | vs | vp]
+    vectors) at iterations 1 (direction = C*g) and 3."""
+    rng = np.random.default_rng(9)
+    lb, ob = dnfwi.LBFGS(NX, NY, 2), fo.LBFGS(NX, NY, 2)
+    vp, vs, rho = models(rng)
+    c = (fo.model_mean(vp), fo.model_mean(vs), fo.model_mean(rho))
+    jac = str(tmp_path / "jac")
+    with pytest.raises(RuntimeError):       # nothing to save before the first direction
+        lb.save(jac, vp, vs, rho)
+    nc = NX * NY
+    names = (("p_vp", "p", "c"), ("p_vs", "p_u", "c_u"), ("p_mrho", "p_rho", "c_rho"))
+    for it in (1, 2, 3):
+        g = [rnd(rng, -1e-3, 1e-3) for _ in range(3)]
+        d = lb.direction(it, *c, *g, vp, vs, rho)
+        do = ob.direction(it, *c, *g, vp, vs, rho)
+        lb.save(jac, *d)
+        for k, slot in enumerate((2, 1, 0)):
+            want = (ob.mold[slot * nc:(slot + 1) * nc], ob.gold[slot * nc:(slot + 1) * nc], do[k].ravel())
+            for n, w in zip(names[k], want):
+                for sfx in (".old", ".old.0.0"):
+                    got = np.fromfile(f"{jac}_{n}{sfx}", np.float32)
+                    assert np.array_equal(got, w), (it, n, sfx)
+        # the model and normalised gradient are the inputs, C * g in float32
+        assert np.array_equal(np.fromfile(f"{jac}_p_vs.old", np.float32), vs.ravel())
+        assert np.array_equal(np.fromfile(f"{jac}_p_rho.old", np.float32), (F32(c[2]) * g[2]).ravel())
+        if it == 1:
+            assert np.array_equal(np.fromfile(f"{jac}_c.old", np.float32), (F32(c[0]) * g[0]).ravel())
+        vp = (vp - F32(0.5) * d[0] / np.abs(d[0]).max()).astype(F32)
+        vs = (vs - F32(0.3) * d[1] / np.abs(d[1]).max()).astype(F32)
+        rho = (rho - F32(0.2) * d[2] / np.abs(d[2]).max()).astype(F32)
+    lb.close()
+
+
 def test_lbfgs_equals_bfgs_matrix_recursion():
     """H_k = V_k^T H_{k-1} V_k + rho_k s_k s_k^T, V_k = I - rho_k y_k s_k^T,
     H_0 = gamma I, over the slots 1..NLBFGS in index order; the direction is

@@ -237,6 +237,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
             assert np.array_equal(got, coeff)
         l2s.append(l2sum)
         d = lb.direction(2, *grads)[:3] if pcg else lb.direction(it, *c, *grads, vp, vs, rho)
+        if not pcg:     # what extract_LBFGS_PSV leaves in the JACOBIAN_*.old files
+            saved = (lb.mold.copy(), lb.gold.copy(), [a.copy() for a in d])
         ls = fo.step_length(upd, it, par.eps_scale, par.scalefac, par.stepmax, l2sum, vp, vs, rho, *d, misfit,
                             carry=carry)
         carry = (ls["epst"], ls["L2t"])
@@ -269,11 +271,36 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
         assert np.array_equal(got, a), name
     log = open(os.path.join(tmp, "misfit.dat")).read().splitlines()
     assert log == lines[:len(log)] and len(log) == len(lines)
+    _check_lbfgs_files(os.path.join(tmp, "jac/j"), nx, ny, None if pcg else saved)
     if mfilt:   # a 5 x 5 median also flattens this small model's layering, so no misfit decrease is asked
         assert not np.array_equal(vp, written[0])
     else:
         assert l2s[-1] < l2s[0]
     assert not np.array_equal(vp, m[0])
+
+
+def _check_lbfgs_files(jac, nx, ny, saved):
+    """The reference's L-BFGS files after the last direction
+    (extract_LBFGS_PSV.c:80-249): per parameter the model the direction was
+    taken at (_p_vp, _p_vs, _p_mrho), its normalised gradient C*g (_p, _p_u,
+    _p_rho) and the direction (_c, _c_u, _c_rho), each as .old and as the
+    one rank's .old.0.0, equal bit for bit to the Python run's L-BFGS state
+    ([rho | vs | vp] vectors).  PCG writes none of them."""
+    names = (("p_vp", "p", "c"), ("p_vs", "p_u", "c_u"), ("p_mrho", "p_rho", "c_rho"))
+    if saved is None:
+        for row in names:
+            for n in row:
+                assert not os.path.exists(f"{jac}_{n}.old"), n
+        return
+    mold, gold, d = saved
+    nc = nx * ny
+    for k, slot in enumerate((2, 1, 0)):          # vp, vs, rho
+        want = (mold[slot * nc:(slot + 1) * nc], gold[slot * nc:(slot + 1) * nc], d[k].ravel())
+        for n, w in zip(names[k], want):
+            for sfx in (".old", ".old.0.0"):
+                got = np.fromfile(f"{jac}_{n}{sfx}", np.float32)
+                assert got.size == nc and np.array_equal(got, w.astype(np.float32)), n + sfx
+        assert np.abs(want[1]).max() > 0
 
 
 def test_cli_rtm_images_equal_library_and_oracle(tmp_path):
