@@ -25,7 +25,6 @@ from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -125,38 +124,12 @@ def run_gpu(L, steps, warmup, sample_every=20, nt=6000, tune_steps=40, tiles=Non
     return out
 
 
-def batch_workload(nshots=64, seed=2024):
-    """The multi-shot workload: nshots independent elastic P/SV shots with
-    NX ~ U[500,3000], NY ~ U[200,800], DH = 10 m, FDORDER in {4, 8},
-    NT ~ U[1500,5000], DT = 1 ms, each with its own seeded smooth-random
-    model and a Ricker source (fc ~ U[5,20] Hz) at the centre 20 m deep,
-    receivers every 20 m at 20 m depth."""
-    import dnhip
-    import synth
-    rng = np.random.default_rng(seed)
-    shots = []
-    for s in range(nshots):
-        nx, ny = int(rng.integers(500, 3001)), int(rng.integers(200, 801))
-        fd = int(rng.choice([4, 8]))
-        nt = int(rng.integers(1500, 5001))
-        fc = float(rng.uniform(5.0, 20.0))
-        dh, dt = 10.0, 1e-3
-        vp, vs, rho = synth.quick_layered_model(nx, ny, dh, seed=s + 1)
-        p = dnhip.Params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=fd, maxrelerror=1, fw=20, free_surf=1,
-                         L=0, seismo=1, ts=float(np.float32(1.0 / fc)), damping=1500.0, fpml=10.0)
-        src = synth.source_table(nx * dh / 2, 20.0, dh, fc=fc)
-        sig = synth.ricker(nt, dt, fc)[None, :]
-        nrec = (nx - 2) * int(dh) // 20
-        rec = synth.receiver_line(20.0, 20.0, nrec, 20.0, dh)
-        shots.append((p, (vp, vs, rho), src, sig, rec))
-    return shots
-
-
 def run_batched(nshots, graph_steps=32):
     """shots/s of the multi-shot engine vs the same shots one after another
     with the plain per-shot solver (dn_run_forward), on prepared contexts
     (models, sources, receivers resident in HBM before timing)."""
     import dnhip
+    from workloads import batch_workload
     shots = batch_workload(nshots)
     cells = sum(p.nx * p.ny * p.nt for p, *_ in shots)
 
@@ -201,43 +174,6 @@ def run_batched(nshots, graph_steps=32):
     }
 
 
-def serving_workload(nreq, qps, seed=77):
-    """Poisson arrivals at `qps`: 70% elastic forward modelling, 30% FWI
-    gradients (forward with snapshots every 2nd step + adjoint); grids
-    log-uniform from 256x256 to 4096x1024 (NX, NY drawn independently), NT
-    uniform in [1000, 8000], DH = 10 m, DT = 1 ms, FD8, seeded smooth-random
-    models, Ricker source (fc 10 Hz) at the centre, receivers every 20 m.
-    Observed data of a gradient job are zero traces (the work of a gradient
-    does not depend on them)."""
-    import dnhip
-    import serve
-    import synth
-    rng = np.random.default_rng(seed)
-    t = 0.0
-    reqs = []
-    for r in range(nreq):
-        t += float(rng.exponential(1.0 / qps))
-        nx = int(round(math.exp(rng.uniform(math.log(256), math.log(4096)))))
-        ny = int(round(math.exp(rng.uniform(math.log(256), math.log(1024)))))
-        nt = int(rng.integers(1000, 8001))
-        kind = "gradient" if rng.random() < 0.3 else "forward"
-        dh, dt, fc = 10.0, 1e-3, 10.0
-        vp, vs, rho = synth.quick_layered_model(nx, ny, dh, seed=1000 + r)
-        p = dnhip.Params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=8, fw=20, free_surf=1, seismo=1,
-                         mode=1 if kind == "gradient" else 0, dtinv=2, ts=float(np.float32(1.0 / fc)))
-        src = synth.source_table(nx * dh / 2, 20.0, dh, fc=fc)
-        sig = synth.ricker(nt, dt, fc)[None, :]
-        nrec = (nx - 2) * int(dh) // 20
-        rec = synth.receiver_line(20.0, 20.0, nrec, 20.0, dh)
-        obs = ()
-        if kind == "gradient":
-            z = np.zeros((nrec, nt), np.float32)
-            obs = (z, z, None)
-        reqs.append(serve.Request(rid=r, kind=kind, params=p, model=(vp, vs, rho), srcpos=src, signals=sig,
-                                  recpos=rec, obs=obs, arrival=t))
-    return reqs
-
-
 def run_serving(nreq, qps, graph_steps=32, saturate_qps=0.0):
     """p50/p99 latency and throughput of the request server, and the same
     requests one at a time with the plain solver (no overlap).  With
@@ -245,6 +181,7 @@ def run_serving(nreq, qps, graph_steps=32, saturate_qps=0.0):
     (its throughput when requests queue up; the plain solver is already
     saturated at the base rate)."""
     import serve
+    from workloads import serving_workload
     reqs = serving_workload(nreq, qps)
     srv = serve.Server(graph_steps=graph_steps)
     stats = srv.serve(reqs)
@@ -444,6 +381,78 @@ def run_gradient(steps, nt=4000, tune_steps=40, sample_every=7, tiles=None, fwi_
     }
 
 
+def config1_workload():
+    """The reference's CPU-runnable case: elastic P/SV 400 x 200, DH = 10 m,
+    FD8 Holberg, DT = 1 ms, NT = 2000, CPML 20 on three sides plus free
+    surface, vp linear 1500 -> 4000 m/s with a seeded smooth +-5%
+    perturbation, vs = vp/sqrt(3), rho = 310 vp^0.25, one explosive Ricker
+    source of 10 Hz at x = 2000 m, z = 20 m, 200 receivers every 20 m at
+    z = 20 m."""
+    import dnhip
+    import synth
+    nx, ny, dh, dt, nt, fc = 400, 200, 10.0, 1e-3, 2000, 10.0
+    vp, vs, rho = synth.layered_model(nx, ny, dh, seed=1)
+    p = dnhip.Params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=8, maxrelerror=1, fw=20, free_surf=1,
+                     L=0, seismo=1, ts=float(np.float32(1.0 / fc)), damping=1500.0, fpml=10.0)
+    src = synth.source_table(2000.0, 20.0, dh, fc=fc)
+    sig = synth.ricker(nt, dt, fc)[None, :]
+    rec = synth.receiver_line(20.0, 20.0, 200, 20.0, dh)
+    return p, (vp, vs, rho), src, sig, rec
+
+
+def run_config1(threads, shots=5):
+    """Config 1 on the GPU (shots/s and Gcell-updates/s over `shots` whole
+    shots after one warm-up) and on the CPU oracle with 1 and `threads`
+    threads (one whole shot each), with the seismograms compared."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import dnhip
+    import orc
+    p, (vp, vs, rho), src, sig, rec = config1_workload()
+    g = dnhip.Context(p)
+    g.set_model(vp, vs, rho)
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    g.forward()
+    t0 = time.perf_counter()
+    for _ in range(shots):
+        g.forward()
+    t_gpu = (time.perf_counter() - t0) / shots
+    seis = g.seismogram(0)
+    g.close()
+    cells = p.nx * p.ny * p.nt
+    out = {"workload": "elastic P/SV 400x200, DH=10 m, FD8, DT=1 ms, NT=2000, CPML 20 + free surface, "
+                       "Ricker 10 Hz, 200 receivers",
+           "gpu_ms_per_shot": t_gpu * 1e3, "gpu_gcells_per_s": cells / t_gpu / 1e9}
+    for n in (1, threads):
+        orc.set_threads(n)
+        o = orc.Oracle(p, vp, vs, rho, prec="f32")
+        o.set_sources(src, sig)
+        o.set_receivers(rec)
+        t0 = time.perf_counter()
+        o.forward()
+        dt = time.perf_counter() - t0
+        out[f"cpu_{n}_threads_s"] = dt
+        out[f"cpu_{n}_threads_gcells_per_s"] = cells / dt / 1e9
+        if n == 1:
+            out["gpu_equals_cpu_oracle"] = bool(np.array_equal(seis, o.seismogram(0)))
+        o.close()
+    return out
+
+
+def host_cpu():
+    """The host CPU model and thread counts (lscpu's 'Model name' from
+    /proc/cpuinfo)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(L, nsteps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import orc
@@ -505,6 +514,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=800)
     ap.add_argument("--cpu1-steps", type=int, default=40, help="steps of the single-thread CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the 400x200 config (GPU and whole-shot CPU runs on 1 and all threads)")
     ap.add_argument("--no-elastic", action="store_true")
     ap.add_argument("--no-visco", action="store_true", help="profiling: elastic shot only")
     ap.add_argument("--cache-policy", type=str, default="",
@@ -579,15 +590,21 @@ def main():
         print(f"# gradient: {gradient}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # one GPU's share of the box's host: OMP_NUM_THREADS (16 on the GPU boxes)
+        host = host_cpu()
+        threads = int(host["omp_num_threads"] or 0) or min(16, os.cpu_count() or 1)
         gcps, secs = cpu_baseline(L, args.cpu_steps, threads)
         cpu = {"value": gcps, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
                "sample": f"first {args.cpu_steps} of 6000 time steps of the same 4000x1200 "
-                         f"viscoelastic shot, FP32 oracle (OpenMP), {secs:.1f} s wall"}
+                         f"viscoelastic shot, FP32 oracle (OpenMP), {secs:.1f} s wall",
+               "host": host}
         if args.cpu1_steps > 0:
             g1, s1 = cpu_baseline(L, args.cpu1_steps, 1)
             cpu["single_thread"] = {"value": g1, "cores": 1,
                                     "sample": f"first {args.cpu1_steps} time steps, {s1:.1f} s wall"}
+        if not args.no_config1:
+            cpu["config1"] = run_config1(threads)
+            print(f"# config1: {cpu['config1']}", file=sys.stderr, flush=True)
     achieved = res["s_kernel_GBs"]
     line = {
         "metric": METRIC,

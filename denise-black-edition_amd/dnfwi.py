@@ -75,6 +75,8 @@ def load():
                                 C.c_int),
         "dnfwi_rtm_adjoint_source": ([fp, fp, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, fp,
                                       C.POINTER(C.c_double)], C.c_int),
+        "dnfwi_stf": ([fp, fp, C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_float, fp, fp], C.c_int),
+        "dnfwi_stf_npad": ([C.c_int], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -318,3 +320,20 @@ def taper_sources(nx, ny, dh, srtradius, srtshape, filtsize, srcpos, g=None):
     dnio._check(load().dnfwi_taper_sources(nx, ny, dh, srtradius, srtshape, filtsize, sp.shape[1], _p(sp), _p(coeff),
                                            _p(gg) if gg is not None else None))
     return coeff, gg
+
+
+def stf(obs, syn, mute, normalize, eps_stf, spike):
+    """dnfwi_stf: the source wavelet (ns floats) of one shot from its observed
+    and synthetic vy sections (ntr, ns).  Returns (wavelet, obs, syn) with
+    the sections as the function left them (muted, normalised)."""
+    obs = _f(obs).copy()
+    syn = _f(syn).copy()
+    ntr, ns = obs.shape
+    m = None if mute is None else np.ascontiguousarray(mute, dtype=np.int32)
+    out = np.zeros(ns, np.float32)
+    sp = _f(spike)
+    assert sp.size >= ns
+    dnio._check(load().dnfwi_stf(_p(obs), _p(syn), ntr, ns,
+                                 None if m is None else m.ctypes.data_as(C.POINTER(C.c_int)), normalize, eps_stf,
+                                 _p(sp), _p(out)))
+    return out, obs, syn

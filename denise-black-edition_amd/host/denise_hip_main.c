@@ -96,7 +96,11 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         if ((st->spatfilter == 3 || st->spatfilter == 4) && !(st->fc_end > 0.0f))
             die("SPATFILTER", "SPATFILTER 3 and 4 need FC_END > 0");
         /* NORMALIZE acts in the STF inversion only (stf.c:176-186) and ENV with LNORM 6 only */
-        if (st->inv_stf || st->rowi) die("workflow", "STF inversion and ROWI are not built");
+        if (st->rowi) die("ROWI", "ROWI is not built");
+        /* stf.c:70-74: INV_STF 2 windows the data with the STA/LTA picker */
+        if (st->inv_stf != 0 && st->inv_stf != 1) die("INV_STF", "0 or 1 (Wiener deconvolution) is built; 2 needs the STA/LTA picker");
+        if (st->inv_stf && p->mode != 1) die("INV_STF", "the STF inversion runs in FWI (MODE 1)");
+        if (st->inv_stf && (st->normalize < 0 || st->normalize > 2)) die("NORMALIZE", "0, 1 or 2 expected with INV_STF");
         if (st->timewin == 3) die("TIMEWIN", "time windows 1, 2, 4 and 5 are built; the STA/LTA picker (3) is not");
         if (st->offset_mute < 0 || st->offset_mute > 2) die("OFFSET_MUTE", "0, 1 or 2 expected");
         if (p->ndt != 1) die("NDT", "FWI needs NDT = 1");
@@ -143,13 +147,16 @@ typedef struct run {
     float gamma, twlength_minus, twlength_plus;
     float *picks, *offs;  /* this shot's picked times (TIMEWIN 1, 4) and offsets (ntr) */
     int lnorm;            /* LNORM of the current stage */
+    int inv_stf, normalize; /* INV_STF, NORMALIZE, OFFSETC_STF, EPS_STF of the current stage */
+    float offsetc_stf, eps_stf;
+    int stf_now;          /* invert the source wavelets in this gradient pass (iter 1) */
     float hc[7];
     char name[4 * DNIO_STR];
 } run;
 
 /* receivers, sources and wavelets of shot ishot on the context
  * (FD_PSV.c:271-300, grad_obj_psv.c:130-170, obj_psv.c:54-110) */
-static void shot_setup(run *r, int ishot)
+static void shot_setup(run *r, int ishot, float *first_signal)
 {
     dnio_par *P = r->P;
     if (P->readrec == 2) {
@@ -171,8 +178,17 @@ static void shot_setup(run *r, int ishot)
      * band-limited spike, then the stage's time-domain filter */
     if (P->quellart == 6)
         io_ok(dnio_tdfilt(sig, nloc, P->nt, P->dt, P->order_spike, P->fc_spike_2, P->fc_spike_1), "spike filter");
-    if (r->time_filt) io_ok(dnio_tdfilt(sig, nloc, P->nt, P->dt, r->order, r->fc, r->fc_start), "TIME_FILT");
+    /* grad_obj_psv.c:151, obj_psv.c:116: no TIME_FILT of the source with
+     * INV_STF (the inverted wavelet was fitted to the filtered data) */
+    if (r->time_filt && !r->inv_stf)
+        io_ok(dnio_tdfilt(sig, nloc, P->nt, P->dt, r->order, r->fc, r->fc_start), "TIME_FILT");
     dn_ok(dn_set_sources(r->ctx, nloc, sp, sig), "sources");
+    if (first_signal) {
+        if (nloc > 0)
+            memcpy(first_signal, sig, sizeof(float) * P->nt);
+        else
+            memset(first_signal, 0, sizeof(float) * P->nt);
+    }
     int *rp = xmalloc(sizeof(int) * 2 * (size_t)(r->ntr ? r->ntr : 1));
     for (int k = 0; k < r->ntr; k++) {
         rp[k] = r->recpos[k];
@@ -363,6 +379,56 @@ static double shot_residuals(run *r, int ishot, int test)
     return l2;
 }
 
+/* INV_STF 1 (grad_obj_psv.c:127-129 -> src/PSV/stf_psv.c:13-66): a forward
+ * run of this shot with the band-limited spike (QUELLART 6, its
+ * ORDER_SPIKE filter only), the observed vy data with the stage's filter,
+ * and the Wiener deconvolution of src/stf.c; the wavelet is written as
+ * SIGNAL_FILE_shot_<ishot>.dat (%e per line), and from here on every
+ * wavelet of the run is read from those files (QUELLART 3, as the
+ * reference leaves it after stf_psv) */
+static void stf_shot(run *r, int ishot)
+{
+    dnio_par *P = r->P;
+    const size_t n = (size_t)r->ntr * P->nt;
+    float *spike = xmalloc(sizeof(float) * P->nt), *stf = xmalloc(sizeof(float) * P->nt);
+    P->quellart = 6;
+    const int tf = r->time_filt;
+    r->time_filt = 0;
+    shot_setup(r, ishot, spike);
+    r->time_filt = tf;
+    dn_ok(dn_run_forward(r->ctx, 0), "STF forward run");
+    float *syn = xmalloc(sizeof(float) * n), *obs = xmalloc(sizeof(float) * n);
+    dn_ok(dn_get_seismogram(r->ctx, 1, syn), "seismograms");
+    io_ok(dnio_data_name(P, 1, ishot, r->name, sizeof r->name), "file name");
+    io_ok(dnio_read_su(r->name, r->ntr, P->nt, obs), r->name);
+    if (r->time_filt) io_ok(dnio_tdfilt(obs, r->ntr, P->nt, P->dt, r->order, r->fc, r->fc_start), "TIME_FILT");
+    /* stf.c:96-166: TRKILL, OFFSET_MUTE and OFFSETC_STF leave traces out */
+    int *mute = xmalloc(sizeof(int) * (size_t)(r->ntr ? r->ntr : 1));
+    const float xs = r->srcpos[0 * r->nsrc + ishot - 1], ys = r->srcpos[1 * r->nsrc + ishot - 1];
+    for (int i = 0; i < r->ntr; i++) {
+        int f = r->kill ? r->kill[(size_t)(ishot - 1) * r->ntr + i] == 1 : 0;
+        const float xr = r->recpos[i] * P->dh, yr = r->recpos[r->ntr + i] * P->dh;
+        const float offset = sqrt(((xs - xr) * (xs - xr)) + ((ys - yr) * (ys - yr)));
+        if ((r->offset_mute == 1) && (offset >= r->offsetc)) f = 1;
+        if ((r->offset_mute == 2) && (offset <= r->offsetc)) f = 1;
+        if (r->offsetc_stf > 0.0f && fabs(offset) >= r->offsetc_stf) f = 1;
+        mute[i] = f;
+    }
+    io_ok(dnfwi_stf(obs, syn, r->ntr, P->nt, mute, r->normalize, r->eps_stf, spike, stf), "STF inversion");
+    snprintf(r->name, sizeof r->name, "%s_shot_%i.dat", P->signal_file, ishot);
+    FILE *f = fopen(r->name, "w");
+    if (!f) die("SIGNAL_FILE", r->name);
+    for (int j = 0; j < P->nt; j++) fprintf(f, "%e\n", stf[j]);
+    fclose(f);
+    printf("   source wavelet of shot %d inverted (Wiener deconvolution), written to %s\n", ishot, r->name);
+    P->quellart = 3;
+    free(spike);
+    free(stf);
+    free(syn);
+    free(obs);
+    free(mute);
+}
+
 /* MODE 2: the observed data as adjoint sources (calc_res.c:416-446 with
  * MODE = 2); returns their L2 (0 unless a test shot) */
 static double shot_rtm_sources(run *r, int ishot, int test)
@@ -423,7 +489,7 @@ static void forward_pass(run *r)
               "snapshots");
     }
     for (int ishot = first; ishot <= last; ishot++) {
-        shot_setup(r, ishot);
+        shot_setup(r, ishot, NULL);
         dn_ok(dn_run_forward(r->ctx, 0), "forward run");
         if (r->ntr > 0 && P->seismo) {
             const float XS = r->srcpos[0 * r->nsrc + ishot - 1], YS = r->srcpos[1 * r->nsrc + ishot - 1];
@@ -460,7 +526,8 @@ static double gradient_pass(run *r, float **g_sum)
     for (int k = 0; k < 3; k++) memset(g_sum[k], 0, sizeof(float) * r->ncell);
     double misfit = 0.0;
     for (int ishot = 1; ishot <= r->nshots; ishot++) {
-        shot_setup(r, ishot);
+        if (r->stf_now) stf_shot(r, ishot); /* grad_obj_psv.c:127-129 */
+        shot_setup(r, ishot, NULL);
         dn_ok(dn_run_forward(r->ctx, 0), "forward run");
         const int test = is_test_shot(P, ishot);
         const double l2 = P->mode == 2 ? shot_rtm_sources(r, ishot, test) : shot_residuals(r, ishot, test);
@@ -560,7 +627,7 @@ static double misfit_of(void *user, const float *vp, const float *vs, const floa
     for (int ishot = 1; ishot <= r->nshots; ishot++) {
         if (!is_test_shot(r->P, ishot)) continue;
         printf(" test forward modelling %d for shot %d\n", itest, ishot);
-        shot_setup(r, ishot);
+        shot_setup(r, ishot, NULL);
         dn_ok(dn_run_forward(r->ctx, 0), "forward run");
         l2 += shot_residuals(r, ishot, 1);
     }
@@ -620,6 +687,10 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
         r->twlength_minus = st.twlength_minus;
         r->twlength_plus = st.twlength_plus;
         r->offsetc = st.offsetc;
+        r->inv_stf = st.inv_stf;
+        r->normalize = st.normalize;
+        r->offsetc_stf = st.offsetc_stf;
+        r->eps_stf = st.eps_stf;
         for (int iter = 1; iter <= P->itermax; iter++) {
             printf("\n TDFWI ITERATION %d of %d (stage %d of %d)\n", iter, P->itermax, nstage, stagemax);
             fflush(stdout);
@@ -634,7 +705,9 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
                 r->c_rho = c_rho;
             }
             dn_ok(dn_set_model(r->ctx, vp, vs, rho, taus, taup), "model upload");
+            r->stf_now = r->inv_stf && iter == 1; /* grad_obj_psv.c:127 */
             const double L2sum = gradient_pass(r, g);
+            r->stf_now = 0;
             if (st.spatfilter == 3) { /* FWI_PSV.c:628-631: smooth_grad before precond_PSV */
                 int w = 0;
                 for (int k = 0; k < 3; k++)

@@ -80,6 +80,25 @@ void dnfwi_pcg_reset(dnfwi_pcg *pc);
  * subtracts.  beta_out (may be NULL) gets the three betas. */
 int dnfwi_pcg_direction(dnfwi_pcg *pc, int pcg_beta, float *g_vp, float *g_vs, float *g_rho, float *beta_out);
 
+/* Source-time-function inversion of one shot (INV_STF 1: src/PSV/stf_psv.c:13-66
+ * calling src/stf.c:13-262): the Wiener deconvolution that turns the spike
+ * wavelet into the source wavelet that best explains the observed vy traces.
+ *   obs, syn: ntr x ns observed and synthetic vy traces (trace-major), the
+ *     synthetics from a forward run with `spike` as source signal; both are
+ *     modified in place: traces with mute[i] != 0 (TRKILL, OFFSET_MUTE,
+ *     OFFSETC_STF; mute may be NULL) are zeroed, then NORMALIZE 1 divides
+ *     each trace by its largest |sample| and NORMALIZE 2 scales the observed
+ *     trace to the synthetic one's (samples 2..ns, src/util.c:489-579);
+ *   spike: ns samples, the first source's signal of the synthetic run;
+ *   eps_stf: EPS_STF, the water level relative to the mean energy;
+ *   stf_out: ns samples of the wavelet, which the reference writes as
+ *     SIGNAL_FILE_shot_<N>.dat and reads back with QUELLART 3.
+ * Transforms of length dnfwi_stf_npad(ns) = 2^(ceil(log2 ns) + 2), in double
+ * precision (the build's own FFT; equal to FFTW's up to double rounding). */
+int dnfwi_stf(float *obs, float *syn, int ntr, int ns, const int *mute, int normalize, float eps_stf,
+              const float *spike, float *stf_out);
+int dnfwi_stf_npad(int ns);
+
 /* Model averages C_vp, C_vs, C_rho of src/PSV/FWI_PSV.c:540-580: a
  * single-precision sequential sum in file order divided by NX*NY. */
 float dnfwi_model_mean(const float *a, int nx, int ny);

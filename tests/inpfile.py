@@ -62,10 +62,10 @@ def write_inp(path, **overrides):
 
 def write_workflow(path, n_order=0, lnorm=2, time_filt=0, fc_start=0.0, fc_end=0.0, order=6, eprecond=0,
                    offset_mute=0, offsetc=10.0, spatfilter=0, wd_damp=0.5, wd_damp1=0.5, timewin=0, gamma=1.0, twlength_minus=0.0,
-                   twlength_plus=0.0):
+                   twlength_plus=0.0, inv_stf=0, offsetc_stf=-4.0, eps_stf=0.1, normalize=0):
     row = (f"0.01 {time_filt} {fc_start} {fc_end} {order} {timewin} {gamma} {twlength_minus} {twlength_plus} "
            f"0 0 0 0 {spatfilter} {wd_damp} {wd_damp1} "
-           f"{eprecond} {lnorm} 0 0 "
-           f"-4.0 0.1 0 {offset_mute} {offsetc} 0.5 1.0 0 0 {n_order}")
+           f"{eprecond} {lnorm} 0 {inv_stf} "
+           f"{offsetc_stf} {eps_stf} {normalize} {offset_mute} {offsetc} 0.5 1.0 0 0 {n_order}")
     with open(path, "w") as f:
         f.write(WORKFLOW_HEADER + "\n" + row + "\n")

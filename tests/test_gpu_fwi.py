@@ -25,7 +25,7 @@ ITERMAX = 4
 
 
 @pytest.mark.parametrize("variant", ["plain", "time_filt", "precond", "pcg", "tapers", "trkill", "model_filter", "spatfilter", "timewin",
-                                     "surface", "gc", "spatfilter4", "gc_surface", "weights"])
+                                     "surface", "gc", "spatfilter4", "gc_surface", "weights", "stf"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """time_filt: a band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz,
     ORDER 4) filters the source signals and the observed data.  precond:
@@ -57,7 +57,13 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     iterations as LNORM 5 without a time window.  weights: QUELLTYPB 7 with
     SEISMO 4 (vx, vy and p residuals; the pressure adjoint source), whose vx
     and vy synthetic and observed samples are scaled by C_vp*C_rho of the
-    start model before the residuals (COMP_WEIGHT, calc_res.c:100-123)."""
+    start model before the residuals (COMP_WEIGHT, calc_res.c:100-123).
+    stf: the source wavelets inverted at iteration 1 (INV_STF 1, NORMALIZE
+    2, EPS_STF 0.1): per shot a run with the band-limited spike (ORDER_SPIKE
+    5, FC_SPIKE_2 15 Hz), the Wiener deconvolution of the observed vy data
+    (oracle/stf_oracle.py), the wavelet file SIGNAL_FILE_shot_<N>.dat equal
+    to it within its printed precision, and every later source signal read
+    from that file (QUELLART 3) without the stage filter."""
     time_filt = int(variant == "time_filt")
     precond = variant in ("precond", "pcg")
     pcg = variant == "pcg"
@@ -69,9 +75,11 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     timewin = {"timewin": 1, "surface": 5, "gc_surface": 5}.get(variant, 0)
     gc = variant in ("gc", "gc_surface")
     weights = variant == "weights"
+    stf = variant == "stf"
     qb, seismo = (7, 4) if weights else (1, 1)
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
+    os.makedirs(os.path.join(tmp, "wavelet"), exist_ok=True)
     nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
     bg = synth.layered_model(nx, ny, dh, seed=5, sigma=3.0)
     true = tuple(a.copy() for a in bg)
@@ -110,7 +118,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                            eprecond=3 if precond else 0, offset_mute=1 if trkill else 0, offsetc=350.0,
                            spatfilter=3 if spat else (4 if spat4 else 0), wd_damp=0.5,
                            wd_damp1=0.3 if spat4 else 0.5, timewin=timewin, gamma=100.0,
-                           twlength_minus=1500.0, twlength_plus=0.05, lnorm=5 if gc else 2)
+                           twlength_minus=1500.0, twlength_plus=0.05, lnorm=5 if gc else 2,
+                           inv_stf=int(stf), normalize=2 if stf else 0, eps_stf=0.1)
     os.makedirs(os.path.join(tmp, "picks"), exist_ok=True)
     picks = {}
     for i in (1, 2):     # a direct-wave pick per trace, written with 6 decimals
@@ -136,6 +145,26 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     if precond:
         g.set_energy_weights(3)
     vp, vs, rho = (a.copy() for a in m[:3])
+    if stf:
+        # stf_psv.c at iteration 1: the spike run, the deconvolution, the file
+        for i in (1, 2):
+            loc = shots[i - 1][0]
+            spike = np.zeros(nt, np.float32)
+            spike[int(np.floor(loc[3, 0] / np.float32(par.dt) + 0.5))] = np.float32(1.0) * loc[5, 0]
+            spike = tf.apply_tdfilt(spike[None, :], par.dt, 5, 15.0, -5.0)
+            g.set_model(vp, vs, rho)
+            g.set_sources(loc, spike)
+            g.set_receivers(rec[:2])
+            g.forward()
+            import stf_oracle as so
+            from cases import rel_l2
+            want = so.stf(obs[i][1], g.seismogram(1), None, 2, 0.1, spike[0])
+            got = np.loadtxt(os.path.join(tmp, f"wavelet/w_shot_{i}.dat"), dtype=np.float64).astype(np.float32)
+            err = rel_l2(got, want)
+            print(f"[fwi stf] shot {i}: wavelet file vs restatement rel-L2 {err:.2e}", flush=True)
+            assert got.shape == (nt,) and err < 1e-6
+            assert np.max(np.abs(got - want)) <= 1e-6 * np.abs(want).max()
+            shots[i - 1] = (loc, (got * loc[5, 0])[None, :].astype(np.float32))
 
     def shot(i, model):
         g.set_model(*model)

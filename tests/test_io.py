@@ -335,7 +335,8 @@ def test_written_parameter_file_reads_back(tmp_path):
 
 @pytest.mark.parametrize("stage,key", [(dict(lnorm=1), "LNORM"), (dict(lnorm=5, n_order=1), "LNORM"),
                                        (dict(spatfilter=2), "SPATFILTER"), (dict(timewin=3), "TIMEWIN"),
-                                       (dict(spatfilter=3, fc_end=0.0), "SPATFILTER")])
+                                       (dict(spatfilter=3, fc_end=0.0), "SPATFILTER"), (dict(inv_stf=2), "INV_STF"),
+                                       (dict(inv_stf=1, normalize=3), "NORMALIZE")])
 def test_cli_refuses_workflow_switches_not_built(tmp_path, stage, key):
     """FWI stages with a misfit, gradient filter or time window that is not
     built stop with exit status 1 and name the switch, before any device
@@ -347,6 +348,6 @@ def test_cli_refuses_workflow_switches_not_built(tmp_path, stage, key):
     inpfile.write_workflow(wf, **stage)
     r = _run_cli(inp, wf, cwd=str(tmp_path))
     assert r.returncode == 1 and key in r.stderr, r.stderr
-    inpfile.write_workflow(wf, lnorm=5, spatfilter=4, fc_end=10.0, timewin=5)
+    inpfile.write_workflow(wf, lnorm=5, spatfilter=4, fc_end=10.0, timewin=5, inv_stf=1, normalize=2)
     r = _run_cli(inp, wf, cwd=str(tmp_path))
     assert r.returncode == 1 and "SOURCE_FILE" in r.stderr, r.stderr
