@@ -49,6 +49,8 @@ void fill_args(dn_ctx *c)
     a.fw = p.fw;
     a.fw2 = 2 * p.fw;
     a.free_surf = p.free_surf;
+    a.physics = p.physics == 0 ? 1 : p.physics;
+    a.mirror = p.free_surf && a.physics == 1;
     a.boundary = p.boundary;
     a.grad_form = p.grad_form;
     a.L = p.L;
@@ -80,6 +82,7 @@ void fill_args(dn_ctx *c)
     a.vp = at(c->vp);
     a.vs = at(c->vs);
     a.rho = at(c->rho);
+    for (int k = 0; k < 8; ++k) a.ani[k] = at(c->ani[k]);
     a.rmem = at(c->rmem);
     a.pmem = at(c->pmem);
     a.qmem = at(c->qmem);
@@ -298,6 +301,10 @@ int validate(const dn_params *p)
     if (p->mode > 0 && p->dtinv < 1) return fail(DN_ERR_ARG, "DTINV must be >= 1");
     if (p->mode > 0 && p->grad_form != 1 && p->grad_form != 2) return fail(DN_ERR_ARG, "GRAD_FORM must be 1 or 2");
     if (!(p->dh > 0.0f) || !(p->dt > 0.0f)) return fail(DN_ERR_ARG, "DH and DT must be positive");
+    if (p->physics != 0 && p->physics != 1 && p->physics != 3 && p->physics != 4)
+        return fail(DN_ERR_ARG, "PHYSICS must be 1 (P/SV), 3 (VTI) or 4 (TTI)");
+    if ((p->physics == 3 || p->physics == 4) && (p->L != 0 || p->mode != 0))
+        return fail(DN_ERR_ARG, "VTI/TTI runs are elastic forward modelling: L = 0 and MODE 0");
     if (p->L > 0) {
         for (int l = 0; l < p->L; ++l)
             if (!(p->fl[l] > 0.0f)) return fail(DN_ERR_ARG, "relaxation frequencies FL must be positive");
@@ -404,7 +411,9 @@ std::vector<size_t> plan_sizes(const dn_params &p, int nsrc, int ntr)
     };
     add(pb, F_N - (gf2 ? 0 : 3));
     add(pb, 2);                                   // rip, rjp
-    if (p.L == 0) add(pb, 3);                     // muipjp, f, g
+    const int nani = aniso_planes(p.physics);
+    if (nani) add(pb, nani);                      // stiffness planes
+    else if (p.L == 0) add(pb, 3);                // muipjp, f, g
     if (p.mode > 0) add(pb, 3);                   // vp, vs, rho
     if (p.L > 0) {
         add(pb * p.L, 3);
@@ -535,7 +544,10 @@ static int ctx_create(const dn_params *p, int device, dn_arena *arena, dn_ctx **
         if ((rc = c->dalloc((void **)&c->fld[k], pb))) return bail(rc);
     }
     if ((rc = c->dalloc((void **)&c->rip, pb)) || (rc = c->dalloc((void **)&c->rjp, pb))) return bail(rc);
-    if (p->L == 0) {
+    if (aniso_planes(p->physics)) {
+        for (int k = 0; k < aniso_planes(p->physics); ++k)
+            if ((rc = c->dalloc((void **)&c->ani[k], pb))) return bail(rc);
+    } else if (p->L == 0) {
         float **mats[] = {&c->muipjp, &c->f, &c->g};
         for (auto m : mats)
             if ((rc = c->dalloc((void **)m, pb))) return bail(rc);
@@ -595,6 +607,7 @@ int dn_set_model(dn_ctx *c, const float *vp, const float *vs, const float *rho, 
                  const float *taup)
 {
     if (!c || !vp || !vs || !rho) return fail(DN_ERR_ARG, "null argument");
+    if (aniso_planes(c->p.physics)) return fail(DN_ERR_ARG, "VTI/TTI models are set with dn_set_model_aniso");
     if (c->p.L > 0 && (!taus || !taup)) return fail(DN_ERR_ARG, "viscoelastic model needs taus and taup");
     HIP_OK(hipSetDevice(c->device));
     std::string err;
@@ -659,6 +672,37 @@ int dn_set_model(dn_ctx *c, const float *vp, const float *vs, const float *rho, 
     c->model_set = true;
     c->fwd_complete = false;
     return receiver_factors(c);
+}
+
+int dn_set_model_aniso(dn_ctx *c, const float *rho, const float *c11, const float *c13, const float *c33,
+                       const float *c44, const float *theta)
+{
+    if (!c || !rho || !c11 || !c13 || !c33 || !c44) return fail(DN_ERR_ARG, "null argument");
+    const int np = aniso_planes(c->p.physics);
+    if (!np) return fail(DN_ERR_ARG, "dn_set_model_aniso needs params.physics 3 (VTI) or 4 (TTI)");
+    if (c->p.physics == 4 && !theta) return fail(DN_ERR_ARG, "a TTI model needs theta");
+    HIP_OK(hipSetDevice(c->device));
+    std::string err;
+    int rc;
+    if (c->p.physics == 3 && (rc = check_stability_vti(c->p, c->setup.hc, rho, c11, c13, c33, c44, err)))
+        return fail(rc, err);
+    prepare_material_aniso(c->p, c->lay, rho, c11, c13, c33, c44, theta, c->mat);
+    const size_t pl = c->lay.plane();
+    if ((rc = upload(c, c->rip, c->mat.rip.data(), pl)) || (rc = upload(c, c->rjp, c->mat.rjp.data(), pl)))
+        return rc;
+    for (int k = 0; k < np; ++k)
+        if ((rc = upload(c, c->ani[k], c->mat.ani[k].data(), pl))) return rc;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->mat.rip.clear(); c->mat.rip.shrink_to_fit();
+    c->mat.rjp.clear(); c->mat.rjp.shrink_to_fit();
+    for (auto &v : c->mat.ani) {
+        v.clear();
+        v.shrink_to_fit();
+    }
+    fill_args(c);
+    c->model_set = true;
+    c->fwd_complete = false;
+    return DN_OK;
 }
 
 int dn_set_sources(dn_ctx *c, int nsrc, const float *srcpos, const float *signals)
@@ -771,6 +815,8 @@ int dn_set_tile(dn_ctx *c, int sweep, int tbx, int by, int ty, int vw)
     t.ty = ty;
     t.vw = vw;
     if (!tile_supported(t)) return fail(DN_ERR_ARG, "tile shape not compiled into the library");
+    if (sweep != 0 && vw != 1 && aniso_planes(c->p.physics))
+        return fail(DN_ERR_ARG, "the VTI/TTI stress sweep is compiled for scalar columns (vw = 1) only");
     HIP_OK(hipSetDevice(c->device));
     HIP_OK(hipStreamSynchronize(c->stream));
     if (sweep != 1) c->tile_v = t;
@@ -800,8 +846,15 @@ int dn_autotune_tile(dn_ctx *c, int nsteps, int *vel_shape, int *str_shape)
     Tile best_v = c->tile_v, best_s = c->tile_s;
     double bv = -1.0, bs = -1.0;
     int rc = DN_OK;
+    // the VTI/TTI stress sweep has scalar columns only: with a vector shape
+    // the velocity sweep is timed beside the default stress shape
+    const bool scalar_s = aniso_planes(c->p.physics) > 0;
     for (int k = 0; k < n && k < 32 && !rc; ++k) {
-        rc = dn_set_tile(c, -1, shapes[k].tbx, shapes[k].by, shapes[k].ty, shapes[k].vw);
+        const bool both = !(scalar_s && shapes[k].vw != 1);
+        if (both)
+            rc = dn_set_tile(c, -1, shapes[k].tbx, shapes[k].by, shapes[k].ty, shapes[k].vw);
+        else if (!(rc = dn_set_tile(c, 0, shapes[k].tbx, shapes[k].by, shapes[k].ty, shapes[k].vw)))
+            rc = dn_set_tile(c, 1, 64, 4, 1, 1);
         if (!rc) rc = dn_run_forward(c, nsteps);  // warm-up (code load, caches)
         if (!rc) rc = dn_run_forward(c, nsteps);
         if (rc) break;
@@ -809,7 +862,7 @@ int dn_autotune_tile(dn_ctx *c, int nsteps, int *vel_shape, int *str_shape)
             bv = c->v_ms;
             best_v = shapes[k];
         }
-        if (bs < 0.0 || c->s_ms < bs) {
+        if (both && (bs < 0.0 || c->s_ms < bs)) {
             bs = c->s_ms;
             best_s = shapes[k];
         }
@@ -1173,7 +1226,7 @@ int dn_get_field(dn_ctx *c, int id, float *out)
     if (!c || !out || id < 0 || id >= F_N) return fail(DN_ERR_ARG, "bad field request");
     if (!c->fld[id]) return fail(DN_ERR_STATE, "field not allocated in this configuration");
     int rc = get_plane_nr(c, c->fld[id], out);
-    if (rc || !c->p.free_surf || (id != F_SYY && id != F_SXY)) return rc;
+    if (rc || !c->a.mirror || (id != F_SYY && id != F_SXY)) return rc;
     // With a free surface the sweeps mirror the stress ghost rows on the fly;
     // materialise them as the reference's surface routine stores them:
     // syy[1-m] = -syy[1+m], sxy[1-m] = -sxy[m], m = 1..FDORDER/2.

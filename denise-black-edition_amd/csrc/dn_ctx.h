@@ -59,6 +59,7 @@ struct dn_ctx {
     float *rmem = nullptr, *pmem = nullptr, *qmem = nullptr;
     float *vmu = nullptr, *vts = nullptr, *vpi = nullptr, *vtp = nullptr, *vmuij = nullptr, *vtsij = nullptr;
     float *fs_fjm = nullptr, *fs_g = nullptr, *fs_dme = nullptr;
+    float *ani[8] = {};  // anisotropic stiffness planes (physics 3 / 4)
     float *psi[9] = {};
     float *pml[P_NVEC] = {};
     float *fwd[5] = {};

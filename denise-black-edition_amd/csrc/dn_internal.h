@@ -59,7 +59,13 @@ struct HostMaterial {
     std::vector<float> fs_fjm, fs_g, fs_dme; // visco free-surface per column
     float fs_bjm[MAXL + 1] = {0};
     std::vector<float> vp, vs, rho;        // raw model in device layout
+    // anisotropic stiffness planes: VTI c11, c13, c33, c44h; TTI d11, d13,
+    // d15, d15h, d33, d35, d35h, d55h
+    std::vector<float> ani[8];
 };
+
+// number of stiffness planes of an anisotropic run (0 for isotropic P/SV)
+inline int aniso_planes(int physics) { return physics == 3 ? 4 : (physics == 4 ? 8 : 0); }
 
 // set-up math (dn_setup.cpp)
 int holberg(int fdorder, int maxrelerror, float hc[7], std::string &err);
@@ -68,11 +74,19 @@ int check_stability(const dn_params &p, const float hc[7], const float *vp, cons
                     const float *rho, const float *taus, const float *taup, std::string &err);
 void prepare_material(const dn_params &p, const Layout &lay, const float *vp, const float *vs,
                       const float *rho, const float *taus, const float *taup, HostMaterial &m);
+int check_stability_vti(const dn_params &p, const float hc[7], const float *rho, const float *c11,
+                        const float *c13, const float *c33, const float *c44, std::string &err);
+void prepare_material_aniso(const dn_params &p, const Layout &lay, const float *rho, const float *c11,
+                            const float *c13, const float *c33, const float *c44, const float *theta,
+                            HostMaterial &m);
 
 // Kernel argument block (passed by value).
 struct KArgs {
     int nx, ny, pitch, fw, fw2;
     int free_surf, boundary, grad_form, L, nt_total;
+    // mirror: the velocity sweep forms the free-surface ghost rows of syy and
+    // sxy (isotropic P/SV with FREE_SURF); the VTI/TTI loops leave them 0
+    int mirror, physics;
     float dt, dh;
     float dhi_el;   // DT/DH  (update_s_elastic, seismo_ssg)
     float dhi_vis;  // 1/DH   (update_s_visc, surfaces)
@@ -99,6 +113,8 @@ struct KArgs {
     float fs_bjm[MAXL + 1];
     // raw model (FWI)
     const float *vp, *vs, *rho;
+    // anisotropic stiffness planes (HostMaterial::ani order)
+    const float *ani[8];
     // CPML memory and profiles (profiles indexed 1..2FW)
     float *psi_sxx_x, *psi_sxy_x, *psi_vxx, *psi_vyx, *psi_vxxs;
     float *psi_syy_y, *psi_sxy_y, *psi_vyy, *psi_vxy;

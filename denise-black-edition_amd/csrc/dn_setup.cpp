@@ -387,4 +387,124 @@ void prepare_material(const dn_params &p, const Layout &lay, const float *vp, co
     }
 }
 
+
+// Stability limit of the VTI driver, src/VTI/checkfd_ssg_VTI.c:33-125: the
+// largest of sqrt(c11/rho), sqrt(c13/rho), sqrt(c33/rho), sqrt(c44/rho)
+// (NaN for a negative c13, which no comparison picks) as vmax.
+int check_stability_vti(const dn_params &p, const float hc[7], const float *rho, const float *c11,
+                        const float *c13, const float *c33, const float *c44, std::string &err)
+{
+    const size_t n = (size_t)p.nx * p.ny;
+    float cmax = 0.0f;
+    for (size_t k = 0; k < n; ++k) {
+        for (const float *cm : {c11, c13, c33, c44}) {
+            const float c = (float)std::sqrt(cm[k] / rho[k]);
+            if (cmax < c) cmax = c;
+        }
+    }
+    const float gamma = std::fabs(hc[1]) + std::fabs(hc[2]) + std::fabs(hc[3]) + std::fabs(hc[4]) +
+                        std::fabs(hc[5]) + std::fabs(hc[6]);
+    const float dtstab = (float)(p.dh / (std::sqrt(2.0) * gamma * cmax));
+    if (p.dt > dtstab) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, " The simulation will get unstable, choose smaller DT. (DT=%e > %e)",
+                      (double)p.dt, (double)dtstab);
+        err = buf;
+        return DN_ERR_CFL;
+    }
+    return DN_OK;
+}
+
+// Anisotropic material: the one-rank periodic ghosts of matcopy_elastic_VTI
+// (src/VTI/matcopy_elastic_VTI.c:32-126), av_rho (src/av_rho.c:5-28) and
+// av_harm (src/av_harm.c:13-19) of c44 (VTI, src/VTI/FD_VTI.c:180-197) or of
+// the rotated d15, d35, d55 (TTI, src/TTI/FD_TTI.c:180-205); the rotation of
+// src/TTI/rot_el_tensor_TTI.c:18-40 with its float locals, theta in degrees
+// as read by src/TTI/readmod_elastic_TTI.c:84.
+void prepare_material_aniso(const dn_params &p, const Layout &lay, const float *rho, const float *c11,
+                            const float *c13, const float *c33, const float *c44, const float *theta,
+                            HostMaterial &m)
+{
+    const int NX = p.nx, NY = p.ny;
+    const size_t plane = lay.plane();
+    auto dev = [&](int j, int i) { return lay.origin() + (size_t)lay.off(j, i); };
+    auto xm = [&](int j, int i) { return (size_t)(i - 1) * NY + (j - 1); };
+    Ring R(NX, NY);
+    R.load(rho);
+    m.rip.assign(plane, 0.0f);
+    m.rjp.assign(plane, 0.0f);
+    const int np = aniso_planes(p.physics);
+    for (int k = 0; k < 8; ++k) m.ani[k].assign(k < np ? plane : 0, 0.0f);
+    for (int j = 1; j <= NY; ++j)
+        for (int i = 1; i <= NX; ++i) {
+            const size_t o = dev(j, i);
+            float rj = (float)(1.0 / (0.5 * (R.at(j, i) + R.at(j + 1, i))));
+            float ri = (float)(1.0 / (0.5 * (R.at(j, i) + R.at(j, i + 1))));
+            if (R.at(j, i) < 1e-4 && R.at(j + 1, i) < 1e-4) rj = 0.0f;
+            if (R.at(j, i) < 1e-4 && R.at(j, i + 1) < 1e-4) ri = 0.0f;
+            m.rjp[o] = rj;
+            m.rip[o] = ri;
+        }
+    auto harm = [&](Ring &M, std::vector<float> &out) {
+        for (int j = 1; j <= NY; ++j)
+            for (int i = 1; i <= NX; ++i) {
+                const float a = M.at(j, i), b = M.at(j, i + 1), c = M.at(j + 1, i), d = M.at(j + 1, i + 1);
+                float v = (float)(4.0 / ((1.0 / a) + (1.0 / b) + (1.0 / c) + (1.0 / d)));
+                if (a == 0.0f || b == 0.0f || c == 0.0f || d == 0.0f) v = 0.0f;
+                out[dev(j, i)] = v;
+            }
+    };
+    if (p.physics == 3) {
+        for (int j = 1; j <= NY; ++j)
+            for (int i = 1; i <= NX; ++i) {
+                m.ani[0][dev(j, i)] = c11[xm(j, i)];
+                m.ani[1][dev(j, i)] = c13[xm(j, i)];
+                m.ani[2][dev(j, i)] = c33[xm(j, i)];
+            }
+        Ring C44(NX, NY);
+        C44.load(c44);
+        harm(C44, m.ani[3]);
+        return;
+    }
+    // TTI: rotated tensor in x-major order, then the ghosts and averages of d15, d35, d55
+    const size_t n = (size_t)NX * NY;
+    std::vector<float> d15(n), d35(n), d55(n);
+    for (int j = 1; j <= NY; ++j)
+        for (int i = 1; i <= NX; ++i) {
+            const size_t k = xm(j, i);
+            const float th = (float)(theta[k] * kPi / 180.0);
+            const float cos2 = (float)(std::cos((double)th) * std::cos((double)th));
+            const float sin2 = (float)(std::sin((double)th) * std::sin((double)th));
+            const float cos4 = cos2 * cos2;
+            const float sin4 = sin2 * sin2;
+            const float sincos = (float)(std::sin((double)th) * std::cos((double)th));
+            const float cos2sin2 = cos2 * sin2;
+            const float cos2msin2 = cos2 - sin2;
+            const float a11 = c11[k], a13 = c13[k], a33 = c33[k], a44 = c44[k];
+            const size_t o = dev(j, i);
+            m.ani[0][o] = (float)((double)(cos2 * (a11 * cos2 + a13 * sin2) + sin2 * (a13 * cos2 + a33 * sin2)) +
+                                  4.0 * (double)a44 * (double)cos2sin2);
+            m.ani[1][o] = (float)((double)(a11 * cos2sin2 + a13 * sin4 + a13 * cos4 + a33 * cos2sin2) -
+                                  4.0 * (double)a44 * (double)cos2sin2);
+            d15[k] = (float)((double)((a13 - a11) * cos2 * sincos + (a33 - a13) * sin2 * sincos) +
+                             2.0 * (double)sincos * (double)a44 * (double)cos2msin2);
+            m.ani[4][o] = (float)((double)(sin2 * (a11 * sin2 + a13 * cos2) + cos2 * (a33 * cos2 + a13 * sin2)) +
+                                  4.0 * (double)a44 * (double)cos2sin2);
+            d35[k] = (float)((double)((a13 - a11) * sin2 * sincos + (a33 - a13) * cos2 * sincos) -
+                             2.0 * (double)sincos * (double)a44 * (double)cos2msin2);
+            d55[k] = (float)(((double)a44 * (1.0 - 2.0 * (double)sin2) * (1.0 - 2.0 * (double)sin2) +
+                              (double)((a33 - a13) * cos2sin2)) -
+                             (double)((a13 - a11) * cos2sin2));
+            m.ani[2][o] = d15[k];
+            m.ani[5][o] = d35[k];
+        }
+    Ring D15(NX, NY), D35(NX, NY), D55(NX, NY);
+    D15.load(d15.data());
+    D35.load(d35.data());
+    D55.load(d55.data());
+    harm(D15, m.ani[3]);
+    harm(D35, m.ani[6]);
+    harm(D55, m.ani[7]);
+}
+
 }  // namespace dn

@@ -38,7 +38,7 @@ class _CParams(C.Structure):
         "nx", "ny", "nt", "fdorder", "maxrelerror", "fw", "free_surf", "boundary", "L",
         "seismo", "ndt", "invmat1", "grad_form", "dtinv", "quelltypb", "mode")] + \
         [(n, C.c_float) for n in ("dh", "dt", "damping", "fpml", "npower", "k_max_pml", "ts")] + \
-        [("fl", C.c_float * 8)]
+        [("fl", C.c_float * 8), ("physics", C.c_int)]
 
 
 @dataclass
@@ -68,6 +68,7 @@ class Params:
     k_max_pml: float = 1.0
     ts: float = 0.1
     fl: list = field(default_factory=list)
+    physics: int = 1
 
     def to_c(self, cls=_CParams):
         c = cls()
@@ -106,6 +107,7 @@ def load():
         "dn_arena_info": ([vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
         "dn_arena_fits": ([vp, C.POINTER(_CParams), C.c_int, C.c_int, ip], C.c_int),
         "dn_set_model": ([vp, fp, fp, fp, fp, fp], C.c_int),
+        "dn_set_model_aniso": ([vp, fp, fp, fp, fp, fp, fp], C.c_int),
         "dn_set_sources": ([vp, C.c_int, fp, fp], C.c_int),
         "dn_set_receivers": ([vp, C.c_int, ip], C.c_int),
         "dn_run_forward": ([vp, C.c_int], C.c_int),
@@ -271,6 +273,12 @@ class Context:
     def set_model(self, vp, vs, rho, taus=None, taup=None):
         arrs = [_f32(a) for a in (vp, vs, rho, taus, taup)]
         _check(load().dn_set_model(self.h, *[_ptr(a) for a in arrs]))
+
+    def set_model_aniso(self, rho, c11, c13, c33, c44, theta=None):
+        """VTI (params.physics 3) or TTI (4, theta: tilt in degrees) model,
+        each NX x NY x-major."""
+        arrs = [_f32(a) for a in (rho, c11, c13, c33, c44, theta)]
+        _check(load().dn_set_model_aniso(self.h, *[_ptr(a) for a in arrs]))
 
     def set_sources(self, srcpos, signals):
         srcpos = _f32(srcpos)

@@ -78,7 +78,12 @@ static void *xmalloc(size_t n)
  * silently computing something else */
 static void check_supported(const dnio_par *p, const dnio_stage *st)
 {
-    if (p->physics != 1) die("PHYSICS", "only 2D P/SV (PHYSICS = 1) is built");
+    if (p->physics != 1 && p->physics != 3 && p->physics != 4)
+        die("PHYSICS", "2D P/SV (1), VTI (3) and TTI (4) are built; AC (2) and SH (5) are not");
+    /* physics_VTI.c / physics_TTI.c: MODE 0 forward modelling and MODE 2
+     * RTM; the FWI branch is commented out there */
+    if (p->physics != 1 && p->mode != 0) die("MODE", "VTI and TTI are built for forward modelling (MODE 0)");
+    if (p->physics != 1 && p->snap) die("SNAP", "wavefield snapshots of the VTI/TTI loop (snap_ssg_VTI) are not built");
     if (p->mode < 0 || p->mode > 2) die("MODE", "MODE 0 (forward), 1 (FWI) and 2 (RTM) are built");
     if (p->boundary != 0) die("BOUNDARY", "periodic boundaries are not supported");
     if (!p->srcrec) die("SRCREC", "sources must come from SOURCE_FILE (SRCREC = 1)");
@@ -882,10 +887,21 @@ int main(int argc, char **argv)
            P.nt, P.dh, P.dt, P.fdorder, P.L);
 
     r.ncell = (size_t)P.nx * P.ny;
+    const int aniso = P.physics == 3 || P.physics == 4;
     float *vp = xmalloc(sizeof(float) * r.ncell), *vs = xmalloc(sizeof(float) * r.ncell);
     float *rho = xmalloc(sizeof(float) * r.ncell);
-    float *taus = P.L ? xmalloc(sizeof(float) * r.ncell) : NULL, *taup = P.L ? xmalloc(sizeof(float) * r.ncell) : NULL;
-    io_ok(dnio_read_model(&P, vp, vs, rho, taus, taup), "model");
+    float *taus = P.L && !aniso ? xmalloc(sizeof(float) * r.ncell) : NULL;
+    float *taup = P.L && !aniso ? xmalloc(sizeof(float) * r.ncell) : NULL;
+    float *cani[4] = {NULL, NULL, NULL, NULL}, *theta = NULL;
+    if (aniso) {
+        /* c11, c13, c33, c44 (+ theta for TTI); the VTI/TTI loops update the
+         * stresses elastically whatever L is (VTI.c:137-149) */
+        for (int k = 0; k < 4; k++) cani[k] = xmalloc(sizeof(float) * r.ncell);
+        if (P.physics == 4) theta = xmalloc(sizeof(float) * r.ncell);
+        io_ok(dnio_read_model_aniso(&P, rho, cani[0], cani[1], cani[2], cani[3], theta), "model");
+    } else {
+        io_ok(dnio_read_model(&P, vp, vs, rho, taus, taup), "model");
+    }
     r.vp = vp;
     r.vs = vs;
     r.rho = rho;
@@ -900,7 +916,8 @@ int main(int argc, char **argv)
     dp.fw = P.fw;
     dp.free_surf = P.free_surf;
     dp.boundary = P.boundary;
-    dp.L = P.L;
+    dp.L = aniso ? 0 : P.L;
+    dp.physics = P.physics;
     dp.seismo = P.mode >= 1 && P.seismo == 0 ? 1 : P.seismo;
     dp.ndt = P.ndt;
     dp.invmat1 = P.invmat1;
@@ -918,7 +935,10 @@ int main(int argc, char **argv)
     for (int l = 0; l < P.L && l < 8; l++) dp.fl[l] = P.fl[l];
 
     dn_ok(dn_ctx_create(&dp, 0, &r.ctx), "device context");
-    dn_ok(dn_set_model(r.ctx, vp, vs, rho, taus, taup), "model upload");
+    if (aniso)
+        dn_ok(dn_set_model_aniso(r.ctx, rho, cani[0], cani[1], cani[2], cani[3], theta), "model upload");
+    else
+        dn_ok(dn_set_model(r.ctx, vp, vs, rho, taus, taup), "model upload");
     if (P.seismo || P.mode >= 1) io_ok(dnio_read_receivers(&P, 1, &r.ntr, &r.recpos), "REC_FILE");
     r.n_order = st ? st->n_order : 0;
     if (P.mode >= 1 && P.trkill) r.kill = read_kill(P.trkill_file, r.nsrc, r.ntr);
@@ -942,5 +962,7 @@ int main(int argc, char **argv)
     free(rho);
     free(taus);
     free(taup);
+    for (int k = 0; k < 4; k++) free(cani[k]);
+    free(theta);
     return 0;
 }

@@ -535,6 +535,47 @@ int dnio_read_model(const dnio_par *p, float *vp, float *vs, float *rho, float *
     return 0;
 }
 
+/* VTI (PHYSICS 3) and TTI (PHYSICS 4) models: readmod_elastic_VTI.c:27-79 /
+ * readmod_elastic_TTI.c:27-85 read MFILE.c11, .c13, .c33, .c44, (.theta),
+ * .rho in the same x-major order and write the copies MFILE.denise.c11,
+ * .c13, .c33, .c44 (VTI: merged file and the one-rank part .0.0,
+ * readmod_elastic_VTI.c:96-118) or, for TTI, also .theta in radians, with
+ * the parts removed after the merge (readmod_elastic_TTI.c:96-151). */
+int dnio_read_model_aniso(const dnio_par *p, float *rho, float *c11, float *c13, float *c33, float *c44,
+                          float *theta)
+{
+    if (!p || !rho || !c11 || !c13 || !c33 || !c44) return fail("null argument");
+    if (p->physics != 3 && p->physics != 4) return fail("PHYSICS = %d: 3 (VTI) or 4 (TTI) expected", p->physics);
+    if (p->physics == 4 && !theta) return fail("TTI needs a theta output");
+    if (!p->readmod) return fail("READMOD = 0 (built-in model functions) is not supported; give model files");
+    const size_t n = (size_t)p->nx * p->ny;
+    if (read_model_file(p->mfile, "c11", c11, n) || read_model_file(p->mfile, "c13", c13, n) ||
+        read_model_file(p->mfile, "c33", c33, n) || read_model_file(p->mfile, "c44", c44, n) ||
+        (p->physics == 4 && read_model_file(p->mfile, "theta", theta, n)) ||
+        read_model_file(p->mfile, "rho", rho, n))
+        return -1;
+    const char *ext[5] = {"denise.c11", "denise.c13", "denise.c33", "denise.c44", "denise.theta"};
+    float *rad = NULL;
+    if (p->physics == 4) {
+        rad = (float *)malloc(sizeof(float) * n);
+        if (!rad) return fail("out of host memory");
+        for (size_t k = 0; k < n; k++) rad[k] = (float)(theta[k] * 3.141592653589793 / 180.0);
+    }
+    const float *arr[5] = {c11, c13, c33, c44, rad};
+    char name[2 * DNIO_STR], part[2 * DNIO_STR + 8];
+    int rc = 0;
+    for (int k = 0; k < (p->physics == 4 ? 5 : 4) && !rc; k++) {
+        snprintf(name, sizeof name, "%s.%s", p->mfile, ext[k]);
+        rc = dnio_write_model(name, arr[k], p->nx, p->ny);
+        if (!rc && p->physics == 4) {
+            snprintf(part, sizeof part, "%s.0.0", name);
+            remove(part);
+        }
+    }
+    free(rad);
+    return rc;
+}
+
 /* ------------------------------------------------------------------------
  * seismograms (src/outseis_glob.c:8-303, src/inseis.c:8-110,
  * include/segy.h:429-710)

@@ -124,3 +124,28 @@ def stable_dt(vmax, dh, fdorder=8, maxrelerror=1, safety=0.9):
     """Largest DT under the stability limit DH/(sqrt(2)*sum|hc|*vmax)."""
     hc_sum = {2: 1.0010, 4: 1.1846, 6: 1.2835, 8: 1.3458, 10: 1.3880, 12: 1.4171}
     return safety * dh / (math.sqrt(2.0) * hc_sum[fdorder] * vmax)
+
+
+def anisotropic_model(nx, ny, dh, seed, eps=0.2, delta=0.1, tilt=None, sigma=8.0):
+    """Seeded VTI model from the layered construction: vertical speeds vp0,
+    vs0 and rho of layered_model, Thomsen epsilon and delta scaled by a
+    smooth random field in [0.5, 1.5]; c33 = rho vp0^2, c44 = rho vs0^2,
+    c11 = c33 (1 + 2 eps), c13 = sqrt(2 delta c33 (c33 - c44) + (c33 - c44)^2)
+    - c44.  With tilt (degrees), also a smooth random tilt field tilt +- 10.
+    Returns (rho, c11, c13, c33, c44, theta or None), x-major float32."""
+    vp, vs, rho = layered_model(nx, ny, dh, seed, sigma=sigma)
+    rng = np.random.default_rng(seed + 1000)
+    f = gaussian_smooth(rng.standard_normal((nx, ny)), sigma)
+    f = 1.0 + 0.5 * f / max(np.abs(f).max(), 1e-30)
+    r = rho.astype(np.float64)
+    c33 = r * vp.astype(np.float64) ** 2
+    c44 = r * vs.astype(np.float64) ** 2
+    e, d = eps * f, delta * f
+    c11 = c33 * (1.0 + 2.0 * e)
+    c13 = np.sqrt(2.0 * d * c33 * (c33 - c44) + (c33 - c44) ** 2) - c44
+    theta = None
+    if tilt is not None:
+        g = gaussian_smooth(rng.standard_normal((nx, ny)), sigma)
+        theta = (tilt + 10.0 * g / max(np.abs(g).max(), 1e-30)).astype(np.float32)
+    return (rho, c11.astype(np.float32), c13.astype(np.float32), c33.astype(np.float32),
+            c44.astype(np.float32), theta)

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define DN_ABI_VERSION 2
+#define DN_ABI_VERSION 3
 
 typedef enum dn_status {
     DN_OK = 0,
@@ -71,6 +71,9 @@ typedef struct dn_params {
     float damping, fpml, npower, k_max_pml;   /* CPML (src/PML_pro.c)          */
     float ts;                /* TS = 1/max(fc) (src/sources.c:92)              */
     float fl[8];             /* FL[1..L]                                       */
+    int physics;             /* PHYSICS: 1 (or 0) isotropic P/SV, 3 VTI, 4 TTI
+                                (src/denise.c:116-141); 3 and 4 model elastic
+                                forward runs (MODE 0, L = 0)                  */
 } dn_params;
 
 typedef struct dn_ctx dn_ctx;
@@ -103,6 +106,18 @@ int dn_ctx_destroy(dn_ctx *ctx);
  * above the stability limit. */
 int dn_set_model(dn_ctx *ctx, const float *vp, const float *vs, const float *rho,
                  const float *taus, const float *taup);
+
+/* Anisotropic model upload (params.physics 3 or 4).  Replaces
+ * readmod_elastic_VTI (src/VTI/readmod_elastic_VTI.c:60-79: MFILE.c11, .c13,
+ * .c33, .c44, .rho) with matcopy_elastic_VTI, av_harm (c44 at the shear
+ * points) and av_rho (src/VTI/FD_VTI.c:180-197), and for TTI
+ * readmod_elastic_TTI (src/TTI/readmod_elastic_TTI.c:64-85, plus MFILE.theta
+ * in degrees), rot_el_tensor_TTI and av_harm of the rotated d15, d35, d55
+ * (src/TTI/FD_TTI.c:180-205).  All NX*NY x-major; theta is NULL for VTI.
+ * VTI runs checkfd_ssg_VTI (src/VTI/checkfd_ssg_VTI.c:33-125: DN_ERR_CFL);
+ * the reference's TTI driver runs no stability check (FD_TTI.c:187). */
+int dn_set_model_aniso(dn_ctx *ctx, const float *rho, const float *c11, const float *c13,
+                       const float *c33, const float *c44, const float *theta);
 
 /* Sources of the shot.  srcpos: 8 x nsrc rows as srcpos_loc
  * (src/splitsrc.c:28-36): [0] x grid index, [1] y grid index, [2] source

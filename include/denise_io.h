@@ -145,6 +145,12 @@ int dnio_wavelet(const dnio_par *p, const float *srcpos_loc, int nsrc, int ishot
  * the copies the reference writes (WRITEMOD: MFILE.denise.{pi,mu,rho};
  * always for L > 0: MFILE.fdveps.{pi,mu,rho}). */
 int dnio_read_model(const dnio_par *p, float *vp, float *vs, float *rho, float *taus, float *taup);
+/* VTI / TTI model files (PHYSICS 3 / 4): MFILE.c11, .c13, .c33, .c44, .rho and,
+ * for TTI, .theta in degrees (src/VTI/readmod_elastic_VTI.c:27-118,
+ * src/TTI/readmod_elastic_TTI.c:27-151, with their model copies); NX*NY
+ * x-major each, theta NULL for VTI. */
+int dnio_read_model_aniso(const dnio_par *p, float *rho, float *c11, float *c13, float *c33, float *c44,
+                          float *theta);
 
 /* writemod + mergemod on one rank (src/writemod.c, src/mergemod.c): a
  * float32 x-major NX x NY file `name` and its per-rank copy `name.0.0`. */

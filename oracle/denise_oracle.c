@@ -186,6 +186,10 @@ struct orc_ctx {
     int qtype;                          /* global QUELLTYP */
     int nan_step;
     float resw[2];                      /* COMP_WEIGHT factors of vx / vy (0 = none set) */
+    /* anisotropic P/SV (PHYSICS 3 VTI, 4 TTI; 1 = isotropic) */
+    int physics;
+    arr c11, c13, c33, c44, c44h, theta;
+    arr d11, d13, d15, d15h, d33, d35, d35h, d55, d55h;
 };
 
 #define PSIX(a, j, h) ((a)[(size_t)((j) - 1) * (size_t)c->fw2 + (size_t)((h) - 1)])
@@ -459,7 +463,9 @@ void orc_destroy(orc_ctx *c)
     arr *all[] = {&c->rho, &c->pi, &c->u, &c->taus, &c->taup, &c->uipjp, &c->rip, &c->rjp,
                   &c->tausipjp, &c->fipjp, &c->f, &c->g, &c->vx, &c->vy, &c->sxx, &c->syy,
                   &c->sxy, &c->vxp1, &c->vyp1, &c->ux, &c->uy, &c->uxy, &c->wconv, &c->wconv_u,
-                  &c->wconv_rho, &c->wlam, &c->wmu, &c->wrho_s, &c->gvp, &c->gvs, &c->grho};
+                  &c->wconv_rho, &c->wlam, &c->wmu, &c->wrho_s, &c->gvp, &c->gvs, &c->grho,
+                  &c->c11, &c->c13, &c->c33, &c->c44, &c->c44h, &c->theta, &c->d11, &c->d13,
+                  &c->d15, &c->d15h, &c->d33, &c->d35, &c->d35h, &c->d55, &c->d55h};
     for (size_t k = 0; k < sizeof all / sizeof all[0]; k++) free(all[k]->p);
     for (int l = 0; l < 9; l++) {
         free(c->d[l].p); free(c->e[l].p); free(c->dip[l].p);
@@ -474,13 +480,13 @@ void orc_destroy(orc_ctx *c)
     free(c);
 }
 
-orc_ctx *orc_create(const orc_params *p, const float *vp, const float *vs, const float *rho,
-                    const float *taus, const float *taup)
+/* Context with the grid, FD coefficients, CPML profiles, wavefields and
+ * material arrays allocated (alloc_PSV, PML_pro, holbergcoeff of
+ * src/PSV/FD_PSV.c:170-199); the model is loaded by the callers. */
+static orc_ctx *ctx_new(const orc_params *p)
 {
-    if (!p || !vp || !vs || !rho) { seterr("null argument"); return NULL; }
     if (p->nx < 1 || p->ny < 1 || p->nt < 2) { seterr("bad grid/NT"); return NULL; }
     if (p->L < 0 || p->L > 8) { seterr("L out of range"); return NULL; }
-    if (p->L > 0 && (!taus || !taup)) { seterr("visco model needs taus/taup"); return NULL; }
     if (p->fw < 0 || p->fw > p->nx || p->fw > p->ny) { seterr("FW out of range"); return NULL; }
     if (p->dtinv < 1) { seterr("DTINV must be >= 1"); return NULL; }
     orc_ctx *c = (orc_ctx *)calloc(1, sizeof *c);
@@ -516,7 +522,20 @@ orc_ctx *orc_create(const orc_params *p, const float *vp, const float *vs, const
     c->psi_vxxs = calloc(nxb, sizeof(real));
     c->psi_syy_y = calloc(nyb, sizeof(real)); c->psi_sxy_y = calloc(nyb, sizeof(real));
     c->psi_vyy = calloc(nyb, sizeof(real)); c->psi_vxy = calloc(nyb, sizeof(real));
+    c->dtinv_help = (int *)calloc((size_t)p->nt + 2, sizeof(int));
+    c->nan_step = 0;
+    c->physics = 1;
+    return c;
+}
 
+orc_ctx *orc_create(const orc_params *p, const float *vp, const float *vs, const float *rho,
+                    const float *taus, const float *taup)
+{
+    if (!p || !vp || !vs || !rho) { seterr("null argument"); return NULL; }
+    if (p->L > 0 && (!taus || !taup)) { seterr("visco model needs taus/taup"); return NULL; }
+    orc_ctx *c = ctx_new(p);
+    if (!c) return NULL;
+    const int NX = p->nx, NY = p->ny;
     /* model files: loop i=1..NX outer, j=1..NY inner (readmod_elastic_PSV.c:72-76) */
     for (int i = 1; i <= NX; i++)
         for (int j = 1; j <= NY; j++) {
@@ -542,8 +561,103 @@ orc_ctx *orc_create(const orc_params *p, const float *vp, const float *vs, const
         av_tau(c);
         prepare_visc(c);
     }
-    c->dtinv_help = (int *)calloc((size_t)p->nt + 2, sizeof(int));
-    c->nan_step = 0;
+    return c;
+}
+
+/* ------------------------------------------------------------------------- */
+/* anisotropic P/SV: VTI (PHYSICS 3) and TTI (PHYSICS 4)                      */
+/* ------------------------------------------------------------------------- */
+
+/* src/av_harm.c:9-24: harmonic mean of the four cells around (i+1/2, j+1/2),
+ * 0 where any of them is 0 */
+static void av_harm(orc_ctx *c, arr m, arr mh)
+{
+    for (int j = 1; j <= c->P.ny; j++)
+        for (int i = 1; i <= c->P.nx; i++) {
+            real v = (real)(4.0 / ((1.0 / AT(m, j, i)) + (1.0 / AT(m, j, i + 1)) + (1.0 / AT(m, j + 1, i)) +
+                                   (1.0 / AT(m, j + 1, i + 1))));
+            if (AT(m, j, i) == 0.0 || AT(m, j, i + 1) == 0.0 || AT(m, j + 1, i) == 0.0 || AT(m, j + 1, i + 1) == 0.0)
+                v = 0.0;
+            AT(mh, j, i) = v;
+        }
+}
+
+/* src/TTI/rot_el_tensor_TTI.c:18-40: the VTI tensor c11, c13, c33, c44
+ * rotated by the tilt angle theta (radians); float locals as there */
+static void rot_el_tensor(orc_ctx *c)
+{
+    for (int j = 1; j <= c->P.ny; j++)
+        for (int i = 1; i <= c->P.nx; i++) {
+            const real th = AT(c->theta, j, i);
+            real cos2 = (real)(cos(th) * cos(th));
+            real sin2 = (real)(sin(th) * sin(th));
+            real cos4 = cos2 * cos2;
+            real sin4 = sin2 * sin2;
+            real sincos = (real)(sin(th) * cos(th));
+            real cos2sin2 = cos2 * sin2;
+            real cos2msin2 = cos2 - sin2;
+            const real c11 = AT(c->c11, j, i), c13 = AT(c->c13, j, i), c33 = AT(c->c33, j, i), c44 = AT(c->c44, j, i);
+            AT(c->d11, j, i) = (real)(cos2 * (c11 * cos2 + c13 * sin2) + sin2 * (c13 * cos2 + c33 * sin2) +
+                                      4.0 * c44 * cos2sin2);
+            AT(c->d13, j, i) = (real)(c11 * cos2sin2 + c13 * sin4 + c13 * cos4 + c33 * cos2sin2 - 4.0 * c44 * cos2sin2);
+            AT(c->d15, j, i) = (real)((c13 - c11) * cos2 * sincos + (c33 - c13) * sin2 * sincos +
+                                      2.0 * sincos * c44 * cos2msin2);
+            AT(c->d33, j, i) = (real)(sin2 * (c11 * sin2 + c13 * cos2) + cos2 * (c33 * cos2 + c13 * sin2) +
+                                      4.0 * c44 * cos2sin2);
+            AT(c->d35, j, i) = (real)((c13 - c11) * sin2 * sincos + (c33 - c13) * cos2 * sincos -
+                                      2.0 * sincos * c44 * cos2msin2);
+            AT(c->d55, j, i) = (real)(c44 * (1.0 - 2.0 * sin2) * (1.0 - 2.0 * sin2) + (c33 - c13) * cos2sin2 -
+                                      (c13 - c11) * cos2sin2);
+        }
+}
+
+/* Model set-up of src/VTI/FD_VTI.c:180-197 (readmod_elastic_VTI.c:60-79,
+ * matcopy_elastic_VTI with one rank, av_harm, av_rho) and of
+ * src/TTI/FD_TTI.c:180-205 (readmod_elastic_TTI.c:64-85 with theta given in
+ * degrees, rot_el_tensor_TTI, matcopy of d15, d35, d55, av_harm of each). */
+orc_ctx *orc_create_aniso(const orc_params *p, int physics, const float *rho, const float *c11,
+                          const float *c13, const float *c33, const float *c44, const float *theta)
+{
+    if (!p || !rho || !c11 || !c13 || !c33 || !c44) { seterr("null argument"); return NULL; }
+    if (physics != 3 && physics != 4) { seterr("physics must be 3 (VTI) or 4 (TTI)"); return NULL; }
+    if (physics == 4 && !theta) { seterr("TTI needs theta"); return NULL; }
+    if (p->L != 0 || p->mode != 0) { seterr("anisotropic runs are elastic forward modelling (L = 0, MODE 0)"); return NULL; }
+    orc_ctx *c = ctx_new(p);
+    if (!c) return NULL;
+    c->physics = physics;
+    const int NX = p->nx, NY = p->ny, nd = c->nd;
+#define NEWA() arr_new(-nd + 1, NY + nd, -nd + 1, NX + nd)
+    c->c11 = NEWA(); c->c13 = NEWA(); c->c33 = NEWA(); c->c44 = NEWA(); c->c44h = NEWA();
+    if (physics == 4) {
+        c->theta = NEWA(); c->d11 = NEWA(); c->d13 = NEWA(); c->d15 = NEWA(); c->d15h = NEWA();
+        c->d33 = NEWA(); c->d35 = NEWA(); c->d35h = NEWA(); c->d55 = NEWA(); c->d55h = NEWA();
+    }
+#undef NEWA
+    for (int i = 1; i <= NX; i++)
+        for (int j = 1; j <= NY; j++) {
+            size_t k = (size_t)(i - 1) * NY + (j - 1);
+            AT(c->c11, j, i) = c11[k];
+            AT(c->c13, j, i) = c13[k];
+            AT(c->c33, j, i) = c33[k];
+            AT(c->c44, j, i) = c44[k];
+            AT(c->rho, j, i) = rho[k];
+            if (physics == 4) AT(c->theta, j, i) = (real)(theta[k] * PI_D / 180.0);
+        }
+    matcopy_periodic(c, c->rho);
+    matcopy_periodic(c, c->c11);
+    if (physics == 3) {
+        matcopy_periodic(c, c->c44);
+        av_harm(c, c->c44, c->c44h);
+    } else {
+        rot_el_tensor(c);
+        matcopy_periodic(c, c->d15);
+        matcopy_periodic(c, c->d35);
+        matcopy_periodic(c, c->d55);
+        av_harm(c, c->d15, c->d15h);
+        av_harm(c, c->d35, c->d35h);
+        av_harm(c, c->d55, c->d55h);
+    }
+    av_rho(c);
     return c;
 }
 
@@ -830,6 +944,42 @@ static void update_s_visc(orc_ctx *c, int mode)
 
 /* time derivative of the source signal by central differences, one-sided at
  * the ends (src/psource.c:28-31, src/msource.c:27-30) */
+/* src/VTI/update_s_elastic_PML_VTI.c:13-637 (FD8 block :304-396): the
+ * derivatives and CPML of the isotropic update, then
+ * sxy += c44h (vyx + vxy), sxx += c11 vxx + c13 vyy, syy += c13 vxx + c33 vyy */
+static void update_s_vti(orc_ctx *c)
+{
+    const orc_params *P = &c->P;
+    const real dhi = (real)P->dt / (real)P->dh;
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= P->ny; j++) {
+        for (int i = 1; i <= P->nx; i++) {
+            real vxx, vyy, vxy, vyx;
+            stress_derivs(c, j, i, dhi, &vxx, &vyy, &vxy, &vyx);
+            AT(c->sxy, j, i) += AT(c->c44h, j, i) * (vyx + vxy);
+            AT(c->sxx, j, i) += AT(c->c11, j, i) * vxx + AT(c->c13, j, i) * vyy;
+            AT(c->syy, j, i) += AT(c->c13, j, i) * vxx + AT(c->c33, j, i) * vyy;
+        }
+    }
+}
+
+/* src/TTI/update_s_elastic_PML_TTI.c:13-635 (FD8 block :304-395) */
+static void update_s_tti(orc_ctx *c)
+{
+    const orc_params *P = &c->P;
+    const real dhi = (real)P->dt / (real)P->dh;
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= P->ny; j++) {
+        for (int i = 1; i <= P->nx; i++) {
+            real vxx, vyy, vxy, vyx;
+            stress_derivs(c, j, i, dhi, &vxx, &vyy, &vxy, &vyx);
+            AT(c->sxx, j, i) += AT(c->d11, j, i) * vxx + AT(c->d13, j, i) * vyy + AT(c->d15h, j, i) * (vyx + vxy);
+            AT(c->syy, j, i) += AT(c->d13, j, i) * vxx + AT(c->d33, j, i) * vyy + AT(c->d35h, j, i) * (vyx + vxy);
+            AT(c->sxy, j, i) += AT(c->d15, j, i) * vxx + AT(c->d35, j, i) * vyy + AT(c->d55h, j, i) * (vyx + vxy);
+        }
+    }
+}
+
 static real source_amp(const orc_ctx *c, int l, int nt)
 {
     const int NT = c->P.nt;
@@ -1014,6 +1164,23 @@ static void seismo_ssg(orc_ctx *c, int nt)
     }
 }
 
+/* src/VTI/seismo_ssg_VTI.c:8-108 (also called by src/TTI/TTI.c:212): the
+ * div/curl branches are commented out there, so SEISMO 3 samples nothing and
+ * SEISMO 4 samples vx, vy and p */
+static void seismo_ssg_aniso(orc_ctx *c, int nt)
+{
+    const orc_params *P = &c->P;
+    for (int itr = 0; itr < c->ntr; itr++) {
+        int i = c->recpos[itr], j = c->recpos[c->ntr + itr];
+        size_t o = (size_t)itr * P->nt + (nt - 1);
+        if (P->seismo == 1 || P->seismo == 4) {
+            c->sec[0][o] = AT(c->vx, j, i);
+            c->sec[1][o] = AT(c->vy, j, i);
+        }
+        if (P->seismo == 2 || P->seismo == 4) c->sec[2][o] = -AT(c->sxx, j, i) - AT(c->syy, j, i);
+    }
+}
+
 /* forward snapshot store, src/PSV/psv.c:240-309 (IDXI=IDYI=1) */
 static void store_snapshot(orc_ctx *c, int k)
 {
@@ -1080,6 +1247,33 @@ static void correlate(orc_ctx *c, int k)
         }
 }
 
+/* One time step after the NaN probe: src/PSV/psv.c:104-222 (velocity
+ * update, stress update, explosive / adjoint pressure / moment-tensor
+ * sources, free surface, receivers).  The VTI and TTI loops
+ * (src/VTI/VTI.c:109-215, src/TTI/TTI.c:109-213) call their own stress
+ * update and no free-surface routine: with FREE_SURF the top CPML is off
+ * and the ghost rows above row 1 stay 0. */
+static void step_body(orc_ctx *c, int nt, int mode)
+{
+    const orc_params *P = &c->P;
+    update_v(c, nt, mode == 1 ? 1 : 0);
+    if (c->physics == 3) update_s_vti(c);
+    else if (c->physics == 4) update_s_tti(c);
+    else if (P->L) update_s_visc(c, mode);
+    else update_s_elastic(c, mode);
+    if (c->qtype == 1 && mode == 0) psource(c, nt, 0);
+    if (P->quelltypb >= 4 && mode == 1) psource(c, nt, 1);
+    if (c->qtype == 5 && mode == 0) msource(c, nt);
+    if (P->free_surf && c->physics == 1) {
+        if (P->L) surface_visc(c);
+        else surface_elastic(c);
+    }
+    if (P->seismo && mode == 0) {
+        if (c->physics == 1) seismo_ssg(c, nt);
+        else seismo_ssg_aniso(c, nt);
+    }
+}
+
 /* psv() mode 0 / 1, src/PSV/psv.c:17-390 */
 static int run_psv(orc_ctx *c, int mode, int nsteps)
 {
@@ -1098,17 +1292,7 @@ static int run_psv(orc_ctx *c, int mode, int nsteps)
             seterr("Simulation is unstable at time step %d", nt);
             return -2;
         }
-        update_v(c, nt, mode == 1 ? 1 : 0);
-        if (P->L) update_s_visc(c, mode);
-        else update_s_elastic(c, mode);
-        if (c->qtype == 1 && mode == 0) psource(c, nt, 0);
-        if (P->quelltypb >= 4 && mode == 1) psource(c, nt, 1);
-        if (c->qtype == 5 && mode == 0) msource(c, nt);
-        if (P->free_surf) {
-            if (P->L) surface_visc(c);
-            else surface_elastic(c);
-        }
-        if (P->seismo && mode == 0) seismo_ssg(c, nt);
+        step_body(c, nt, mode);
         if (nt == hin1 && mode == 0 && P->mode > 0) {
             store_snapshot(c, hin - 1);
             hin++;
@@ -1132,17 +1316,7 @@ int orc_step(orc_ctx *c, int nt, int mode)
 {
     const orc_params *P = &c->P;
     if (nt < 1 || nt > P->nt) { seterr("step %d out of range", nt); return -1; }
-    update_v(c, nt, mode == 1 ? 1 : 0);
-    if (P->L) update_s_visc(c, mode);
-    else update_s_elastic(c, mode);
-    if (c->qtype == 1 && mode == 0) psource(c, nt, 0);
-    if (P->quelltypb >= 4 && mode == 1) psource(c, nt, 1);
-    if (c->qtype == 5 && mode == 0) msource(c, nt);
-    if (P->free_surf) {
-        if (P->L) surface_visc(c);
-        else surface_elastic(c);
-    }
-    if (P->seismo && mode == 0) seismo_ssg(c, nt);
+    step_body(c, nt, mode);
     return 0;
 }
 
@@ -1326,6 +1500,18 @@ int orc_get_field(const orc_ctx *c, int id, void *out)
     case ORC_FIPJP: a = &c->fipjp; break;
     case ORC_F: a = &c->f; break;
     case ORC_G: a = &c->g; break;
+    case ORC_C11: a = &c->c11; break;
+    case ORC_C13: a = &c->c13; break;
+    case ORC_C33: a = &c->c33; break;
+    case ORC_C44H: a = &c->c44h; break;
+    case ORC_D11: a = &c->d11; break;
+    case ORC_D13: a = &c->d13; break;
+    case ORC_D15: a = &c->d15; break;
+    case ORC_D15H: a = &c->d15h; break;
+    case ORC_D33: a = &c->d33; break;
+    case ORC_D35: a = &c->d35; break;
+    case ORC_D35H: a = &c->d35h; break;
+    case ORC_D55H: a = &c->d55h; break;
     default: seterr("bad field id %d", id); return -1;
     }
     memcpy(out, a->p, arr_len(*a) * sizeof(real));

@@ -49,11 +49,21 @@ enum {
     ORC_VX = 0, ORC_VY, ORC_SXX, ORC_SYY, ORC_SXY, ORC_VXP1, ORC_VYP1,
     ORC_UX, ORC_UY, ORC_UXY,
     ORC_RIP, ORC_RJP, ORC_UIPJP, ORC_PRHO, ORC_PPI, ORC_PU,
-    ORC_FIPJP, ORC_F, ORC_G, ORC_NFIELD
+    ORC_FIPJP, ORC_F, ORC_G,
+    /* anisotropic material: VTI c11, c13, c33, harmonic c44; TTI rotated
+     * d11, d13, d15, d15h, d33, d35, d35h, d55h */
+    ORC_C11, ORC_C13, ORC_C33, ORC_C44H,
+    ORC_D11, ORC_D13, ORC_D15, ORC_D15H, ORC_D33, ORC_D35, ORC_D35H, ORC_D55H,
+    ORC_NFIELD
 };
 
 orc_ctx *orc_create(const orc_params *p, const float *vp, const float *vs,
                     const float *rho, const float *taus, const float *taup);
+/* Anisotropic elastic forward modelling (L = 0, MODE 0): physics 3 VTI
+ * (rho, c11, c13, c33, c44; theta unused) or 4 TTI (theta: tilt in degrees,
+ * as in the model file MFILE.theta).  All NX*NY x-major. */
+orc_ctx *orc_create_aniso(const orc_params *p, int physics, const float *rho, const float *c11,
+                          const float *c13, const float *c33, const float *c44, const float *theta);
 void orc_destroy(orc_ctx *c);
 const char *orc_error(void);
 void orc_set_threads(int n);

