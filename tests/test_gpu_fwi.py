@@ -306,8 +306,10 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     elif stf and len(l2s) == 1 and ls["step3"] == 1:
         # with the wavelet matched to the data the first direction of this
         # small problem raises the misfit at every trial step: the search
-        # fails and the stage ends in both runs alike (equal logs above)
+        # fails, the model stays, and the stage ends in both runs alike
+        # (equal logs and model files above)
         assert float(eps) == 0.0
+        return
     else:
         assert l2s[-1] < l2s[0]
     assert not np.array_equal(vp, m[0])
