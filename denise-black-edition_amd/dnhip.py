@@ -113,6 +113,7 @@ def load():
         "dn_run_forward": ([vp, C.c_int], C.c_int),
         "dn_get_seismogram": ([vp, C.c_int, fp], C.c_int),
         "dn_calc_residuals": ([vp, fp, fp, fp, C.c_int, C.POINTER(C.c_double)], C.c_int),
+        "dn_calc_residuals_noalloc": ([vp, fp, fp, fp, C.c_int, C.POINTER(C.c_double)], C.c_int),
         "dn_set_residual_weights": ([vp, C.c_float, C.c_float], C.c_int),
         "dn_set_adjoint_sources": ([vp, fp, fp, fp], C.c_int),
         "dn_get_adjoint_sources": ([vp, C.c_int, fp], C.c_int),
@@ -303,6 +304,14 @@ class Context:
         l2 = C.c_double(0.0)
         arrs = [_f32(a) for a in (obs_vx, obs_vy, obs_p)]
         _check(load().dn_calc_residuals(self.h, *[_ptr(a) for a in arrs], testshot, C.byref(l2)))
+        return l2.value
+
+    def residuals_noalloc(self, obs_vx=None, obs_vy=None, obs_p=None, testshot=1):
+        """residuals() through dn_calc_residuals_noalloc (no device
+        allocation: static device scratch, ntr * NT <= DN_NOALLOC_MAX_SAMPLES)."""
+        l2 = C.c_double(0.0)
+        arrs = [_f32(a) for a in (obs_vx, obs_vy, obs_p)]
+        _check(load().dn_calc_residuals_noalloc(self.h, *[_ptr(a) for a in arrs], testshot, C.byref(l2)))
         return l2.value
 
     def set_residual_weights(self, w_vx=1.0, w_vy=1.0):

@@ -148,6 +148,21 @@ int dn_get_seismogram(dn_ctx *ctx, int comp, float *out);
 int dn_calc_residuals(dn_ctx *ctx, const float *obs_vx, const float *obs_vy,
                       const float *obs_p, int testshot, double *l2);
 
+/* dn_calc_residuals that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and adjoint sources as dn_calc_residuals, but the
+ * device scratch of the call -- the observed traces of one component staged
+ * on the device and the misfit with its partial sums -- lives in one fixed
+ * static device array of libdnhip.so (about 32 MiB per GPU, held until the
+ * process ends) instead of the context's buffers; no hipMalloc, hipFree or
+ * allocating library call.  ntr * NT must not exceed DN_NOALLOC_MAX_SAMPLES
+ * (else DN_ERR_ARG).  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (from several threads) race. */
+#define DN_NOALLOC_MAX_SAMPLES 8388608
+int dn_calc_residuals_noalloc(dn_ctx *ctx, const float *obs_vx, const float *obs_vy,
+                              const float *obs_p, int testshot, double *l2);
+
 /* Component weights of the residuals (COMP_WEIGHT, src/calc_res.c:100-123,
  * set by src/PSV/calc_res_PSV.c:65-67 and 118-120 for QUELLTYPB 5, 6, 7):
  * dn_calc_residuals multiplies the synthetic and the observed samples of
