@@ -19,7 +19,9 @@ The elastic FD8 4000x1200 shot (whose step should reach 60% of the HBM
 roofline) is reported alongside in "elastic"; 64 uneven shots through the
 multi-shot engine in "batched"; a request stream through the server in
 "serving"; the elastic FWI gradient job of 2000x600, NT=4000 (forward with
-the snapshot store, adjoint with the correlation) in "gradient".
+the snapshot store, adjoint with the correlation) in "gradient"; the
+drop-in executable over the shipped Marmousi example's 100 shots, with the
+multi-shot engine and shot by shot, in "executable".
 """
 from __future__ import annotations
 
@@ -439,6 +441,66 @@ def run_config1(threads, shots=5):
     return out
 
 
+def run_executable_marmousi(keep_shots=100):
+    """The drop-in executable on the shipped Marmousi example's parameter,
+    source and receiver files (500 x 174, DH 20 m, DT 2 ms, 6 s = 3000
+    steps, FD8, CPML 10 + free surface, the band-limited spike source, 400
+    OBC receivers; a synthetic layered model under 400 m of water stands
+    where MFILE points, the example's model is not shipped): MODE 0 over
+    all its shots, once through the multi-shot engine (the default) and once
+    shot by shot (DENISE_HIP_SEQUENTIAL=1).  Wall time of each process
+    (start-up, model and SU file I/O included) and the SU files of three
+    shots compared byte for byte."""
+    import filecmp
+    import shutil
+    import subprocess
+    import tempfile
+    import synth
+    gold = os.path.join(ROOT, "tests", "golden", "marmousi")
+    exe = os.path.join(ROOT, "denise-black-edition_amd", "bin", "denise_hip")
+    base = tempfile.mkdtemp(prefix="dnbench_", dir="/tmp")
+    out = {"workload": "shipped Marmousi example, MODE 0: 500x174, NT=3000, FD8, QUELLART 6, 400 receivers"}
+    try:
+        lines = open(os.path.join(gold, "source_OBC_VSP.dat")).read().splitlines()
+        shots = lines[1:1 + keep_shots]
+        vp, vs, rho = synth.layered_model(500, 174, 20.0, seed=4, sigma=4.0)
+        vp[:, :20], vs[:, :20], rho[:, :20] = 1500.0, 0.0, 1000.0
+        walls = {}
+        for label, env in (("batched", {}), ("sequential", {"DENISE_HIP_SEQUENTIAL": "1"})):
+            d = os.path.join(base, label)
+            for sub in ("start", "su", "log", "source", "receiver"):
+                os.makedirs(os.path.join(d, sub), exist_ok=True)
+            shutil.copy(os.path.join(gold, "DENISE_marm_OBC.inp"), d)
+            shutil.copy(os.path.join(gold, "receiver_OBC.dat"), os.path.join(d, "receiver"))
+            with open(os.path.join(d, "source", "source_OBC_VSP.dat"), "w") as f:
+                f.write(f"{len(shots)}\n" + "\n".join(shots) + "\n")
+            for ext, a in (("vp", vp), ("vs", vs), ("rho", rho)):
+                np.ascontiguousarray(a, np.float32).tofile(os.path.join(d, f"start/marmousi_II_marine.{ext}"))
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "DENISE_marm_OBC.inp"], cwd=d, capture_output=True, text=True, timeout=600,
+                               env=dict(os.environ, **env))
+            walls[label] = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(f"denise_hip ({label}) failed: {r.stderr[-500:]}")
+        out["shots"] = len(shots)
+        out["batched_s"] = walls["batched"]
+        out["sequential_s"] = walls["sequential"]
+        out["speedup"] = walls["sequential"] / walls["batched"]
+        cells = 500 * 174 * 3000 * len(shots)
+        out["batched_gcells_per_s"] = cells / walls["batched"] / 1e9
+        out["sequential_gcells_per_s"] = cells / walls["sequential"] / 1e9
+        same = True
+        for ishot in sorted({1, (len(shots) + 1) // 2, len(shots)}):
+            for comp in ("x", "y"):
+                name = f"su/DENISE_MARMOUSI_{comp}.su.shot{ishot}"
+                same &= filecmp.cmp(os.path.join(base, "batched", name), os.path.join(base, "sequential", name),
+                                    shallow=False)
+        out["su_files_equal"] = bool(same)
+    finally:
+        shutil.rmtree(base, ignore_errors=True)
+    return out
+
+
 def host_cpu():
     """The host CPU model and thread counts (lscpu's 'Model name' from
     /proc/cpuinfo)."""
@@ -532,6 +594,9 @@ def main():
     ap.add_argument("--serve-qps", type=float, default=8.0)
     ap.add_argument("--grad-jobs", type=int, default=2, help="timed gradient jobs (0: skip)")
     ap.add_argument("--grad-only", action="store_true", help="profiling: only the gradient job")
+    ap.add_argument("--executable-only", action="store_true", help="only the executable on the Marmousi example")
+    ap.add_argument("--no-executable", action="store_true",
+                    help="skip the drop-in executable on the shipped Marmousi example (100 shots, MODE 0)")
     ap.add_argument("--fwi-iters", type=int, default=2,
                     help="FWI iterations (L-BFGS, step search, update) after the gradient jobs (0: skip)")
     ap.add_argument("--nt", type=int, default=6000, help="profiling: shorter shots")
@@ -561,6 +626,9 @@ def main():
             out["serving"] = run_serving(args.serve_requests, args.serve_qps, args.graph_steps, args.serve_saturate_qps)
         print(json.dumps(out))
         return
+    if args.executable_only:
+        print(json.dumps({"executable": run_executable_marmousi()}))
+        return
     if args.grad_only:
         print(json.dumps({"gradient": run_gradient(max(1, args.grad_jobs), nt=min(args.nt, 4000),
                                                    tune_steps=args.tune_steps, tiles=tiles,
@@ -588,6 +656,10 @@ def main():
     if args.grad_jobs > 0:
         gradient = run_gradient(args.grad_jobs, tune_steps=args.tune_steps, fwi_iters=args.fwi_iters)
         print(f"# gradient: {gradient}", file=sys.stderr, flush=True)
+    executable = None
+    if not args.no_executable:
+        executable = run_executable_marmousi()
+        print(f"# executable: {executable}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         # one GPU's share of the box's host: OMP_NUM_THREADS (16 on the GPU boxes)
@@ -636,6 +708,7 @@ def main():
         "batched": batched,
         "serving": serving,
         "gradient": gradient,
+        "executable": executable,
         "detail": {"visco": res, "elastic": elastic,
                    "elastic_step_frac_of_hbm": (elastic["step_GBs"] / HBM_PEAK_GBS) if elastic else None,
                    "elastic_kernels_frac_of_hbm": (

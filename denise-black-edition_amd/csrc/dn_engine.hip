@@ -124,6 +124,7 @@ int rebuild(dn_engine *e)
             b.fwdstore = c->p.mode > 0;
             b.vp1_form = c->p.grad_form == 1 || c->p.grad_form == 2;
             b.sw = j.phase;
+            b.ew = j.phase == 0 ? (c->eprecond ? c->ew[0] : nullptr) : (c->eprecond == 1 ? c->ew[1] : nullptr);
             const int idx = (int)shots.size();
             shots.push_back(b);
             const int b0 = (int)j.base;
@@ -258,7 +259,6 @@ int dn_engine_attach(dn_engine *e, dn_ctx *c, int phase)
     if (c->device != e->device) return fail(DN_ERR_ARG, "context and engine are on different devices");
     if (!c->model_set) return fail(DN_ERR_STATE, "dn_set_model must precede dn_engine_attach");
     if (c->qtype == 5) return fail(DN_ERR_ARG, "moment-tensor shots run through dn_run_forward, not the engine");
-    if (c->eprecond) return fail(DN_ERR_ARG, "energy-weighted shots run through dn_run_forward/dn_run_adjoint, not the engine");
     if (c->a.physics != 1) return fail(DN_ERR_ARG, "VTI/TTI shots run through dn_run_forward, not the engine");
     for (const Job &j : e->jobs)
         if (j.ctx == c) return fail(DN_ERR_STATE, "context already attached");
@@ -280,6 +280,10 @@ int dn_engine_attach(dn_engine *e, dn_ctx *c, int phase)
         HIP_OK(hipMemsetAsync(c->wcr, 0, g, c->stream));
     } else {
         c->fwd_complete = false;
+        // grad_obj_psv.c:71-76: the energies start at zero for each shot
+        if (c->eprecond)
+            for (int w = 0; w < 2; ++w)
+                HIP_OK(hipMemsetAsync(c->ew[w], 0, (size_t)c->p.nx * c->p.ny * sizeof(float), c->stream));
     }
     HIP_OK(hipStreamSynchronize(c->stream));
     e->jobs.push_back(Job{c, phase, e->clock - 1});

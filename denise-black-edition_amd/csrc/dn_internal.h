@@ -201,6 +201,10 @@ struct BatchShot {
     StepCtl cv, cs;
     DevAcq q;
     int nt_total, dtinv, fwdstore, vp1_form, sw;
+    // energy weights (EPRECOND, src/eprecond.c:9-18): vx^2+vy^2 summed at
+    // the store steps (forward, Ws) or correlation steps (adjoint with
+    // EPRECOND 1, Wr) into this NX x NY plane, or nullptr
+    float *ew;
 };
 
 // One block of a multi-shot launch: its shot, the tile it sweeps, and what

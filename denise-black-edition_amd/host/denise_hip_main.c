@@ -157,6 +157,13 @@ typedef struct run {
     int stf_now;          /* invert the source wavelets in this gradient pass (iter 1) */
     float hc[7];
     char name[4 * DNIO_STR];
+    /* multi-shot passes: the shots of a pass run together through the
+     * engine, one context each, carved from one device arena */
+    int batch;
+    dn_params dp;                                  /* parameters of the shot contexts */
+    const float *m_vp, *m_vs, *m_rho, *m_taus, *m_taup; /* the model of the next pass */
+    dn_arena *arena;
+    dn_engine *eng;
 } run;
 
 /* receivers, sources and wavelets of shot ishot on the context
@@ -481,6 +488,124 @@ static void snap_writer(void *user, int nt, int nsnap, const float *vx, const fl
 
 /* MODE 0: forward runs and SU output; with SNAP > 0 only shot SNAP_SHOT
  * (FD_PSV.c:264-268), with its snapshots (psv.c:75-77, 225-231) */
+/* ---- multi-shot passes ------------------------------------------------
+ * FD_PSV.c:271-354, grad_obj_psv.c:56-262 and obj_psv.c:54-124 loop over the
+ * shots one by one.  Here the shots of a pass run together through the
+ * multi-shot engine (dn_engine_*), one device context per shot, in groups
+ * that fit the device arena; every host step around the time loops
+ * (sources and wavelets, residuals and misfits, gradient weights, tapers,
+ * stacking, output) runs per shot in the reference's shot order, so the
+ * results equal the one-context loop bit for bit. */
+
+/* shots go through the engine unless the run needs what it does not take:
+ * VTI/TTI, moment-tensor sources, snapshots, per-shot receiver files, a
+ * single shot; DENISE_HIP_SEQUENTIAL=1 keeps the one-context loop */
+static int batching_applies(const run *r)
+{
+    const dnio_par *P = r->P;
+    const char *env = getenv("DENISE_HIP_SEQUENTIAL");
+    if (env && env[0] && env[0] != '0') return 0;
+    if (P->physics != 1 || P->snap || P->readrec == 2 || r->nshots < 2) return 0;
+    for (int l = 0; l < r->nsrc; l++)
+        if ((int)r->srcpos[7 * r->nsrc + l] == 5) return 0;
+    return 1;
+}
+
+/* the arena and engine of the multi-shot passes: room for every shot of the
+ * largest pass (mode: params.mode of its contexts) and the test shots, up
+ * to the free device memory less a reserve */
+static void batch_init(run *r)
+{
+    if (!r->batch) return;
+    dn_params p = r->dp;
+    size_t fp = 0, fp0 = 0, fr = 0, tot = 0;
+    p.mode = r->dp.mode;
+    dn_ok(dn_estimate_footprint(&p, r->nsrc, r->ntr, &fp), "footprint");
+    p.mode = 0;
+    dn_ok(dn_estimate_footprint(&p, r->nsrc, r->ntr, &fp0), "footprint");
+    dn_ok(dn_device_memory(0, &fr, &tot), "device memory");
+    int ntest = 0;
+    for (int ishot = 1; ishot <= r->nshots; ishot++) ntest += is_test_shot(r->P, ishot);
+    const size_t slack = 1 << 20; /* cell lists and alignment */
+    size_t need = (size_t)r->nshots * (fp + slack) + (size_t)ntest * (fp0 + slack);
+    const size_t reserve = (size_t)4 << 30;
+    size_t cap = fr > reserve ? (fr - reserve) / 10 * 9 : 0;
+    /* DENISE_HIP_ARENA_BYTES: a smaller arena (the passes then run in
+     * several groups of shots) */
+    const char *lim = getenv("DENISE_HIP_ARENA_BYTES");
+    if (lim && atoll(lim) > 0 && (size_t)atoll(lim) < cap) cap = (size_t)atoll(lim);
+    if (need > cap) need = cap;
+    if (need < fp + slack) { /* not even one shot: the one-context loop */
+        r->batch = 0;
+        return;
+    }
+    dn_ok(dn_arena_create(0, need, &r->arena), "device arena");
+    dn_ok(dn_engine_create(0, &r->eng), "multi-shot engine");
+    printf(" multi-shot passes: %d shots through the engine, %.2f GB device arena\n", r->nshots, need / 1e9);
+}
+
+static void batch_free(run *r)
+{
+    if (r->eng) dn_engine_destroy(r->eng);
+    if (r->arena) dn_arena_destroy(r->arena);
+    r->eng = NULL;
+    r->arena = NULL;
+}
+
+/* a context for one shot of a pass on the pass's model, or NULL when the
+ * arena has no room left for it */
+static dn_ctx *shot_ctx(run *r, int mode)
+{
+    dn_params p = r->dp;
+    p.mode = mode;
+    int fits = 0;
+    dn_ok(dn_arena_fits(r->arena, &p, r->nsrc, r->ntr, &fits), "arena");
+    if (!fits) return NULL;
+    dn_ctx *c = NULL;
+    dn_ok(dn_ctx_create_in(&p, r->arena, &c), "shot context");
+    dn_ok(dn_set_model(c, r->m_vp, r->m_vs, r->m_rho, r->m_taus, r->m_taup), "model upload");
+    if (mode > 0 && r->eprecond) dn_ok(dn_set_energy_weights(c, r->eprecond), "energy weights");
+    return c;
+}
+
+/* one phase (0 forward, 1 adjoint) of n prepared shots through the engine */
+static void engine_phase(run *r, dn_ctx **cs, int n, int phase)
+{
+    const char *what = phase ? "adjoint run" : "forward run";
+    printf(" engine: %d shot(s), %s\n", n, what);
+    fflush(stdout);
+    for (int k = 0; k < n; k++) dn_ok(dn_engine_attach(r->eng, cs[k], phase), what);
+    int running = n;
+    while (running > 0) dn_ok(dn_engine_step(r->eng, 512, &running), what);
+    for (int k = 0; k < n; k++) {
+        int st = 0;
+        dn_ok(dn_engine_status(r->eng, cs[k], &st), what);
+        if (st == DN_ERR_UNSTABLE) die(what, " Simulation is unstable !");
+        dn_ok(dn_engine_forget(r->eng, cs[k]), what);
+    }
+}
+
+/* saveseis_glob for one shot from the context's seismograms */
+static void write_shot_seis(run *r, int ishot)
+{
+    dnio_par *P = r->P;
+    if (r->ntr <= 0 || !P->seismo) return;
+    const float XS = r->srcpos[0 * r->nsrc + ishot - 1], YS = r->srcpos[1 * r->nsrc + ishot - 1];
+    /* saveseis_glob: SEISMO 1 vx,vy; 2 p; 3 div,curl; 4 vx,vy,p */
+    int comps[5], nc = 0;
+    if (P->seismo == 1 || P->seismo == 4) comps[nc++] = 0, comps[nc++] = 1;
+    if (P->seismo == 2 || P->seismo == 4) comps[nc++] = 2;
+    if (P->seismo == 3) comps[nc++] = 3, comps[nc++] = 4;
+    float *sec = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
+    for (int c = 0; c < nc; c++) {
+        dn_ok(dn_get_seismogram(r->ctx, comps[c], sec), "seismograms");
+        io_ok(dnio_seis_name(P, comps[c], ishot, 1, r->name, sizeof r->name), "file name");
+        io_ok(dnio_write_seis(r->name, P, sec, r->ntr, P->nt, r->recpos, XS, YS), r->name);
+        printf("   wrote %s\n", r->name);
+    }
+    free(sec);
+}
+
 static void forward_pass(run *r)
 {
     dnio_par *P = r->P;
@@ -493,26 +618,73 @@ static void forward_pass(run *r)
                                iround_f(P->tsnap2 / P->dt), snap_writer, r),
               "snapshots");
     }
-    for (int ishot = first; ishot <= last; ishot++) {
-        shot_setup(r, ishot, NULL);
-        dn_ok(dn_run_forward(r->ctx, 0), "forward run");
-        if (r->ntr > 0 && P->seismo) {
-            const float XS = r->srcpos[0 * r->nsrc + ishot - 1], YS = r->srcpos[1 * r->nsrc + ishot - 1];
-            /* saveseis_glob: SEISMO 1 vx,vy; 2 p; 3 div,curl; 4 vx,vy,p */
-            int comps[5], nc = 0;
-            if (P->seismo == 1 || P->seismo == 4) comps[nc++] = 0, comps[nc++] = 1;
-            if (P->seismo == 2 || P->seismo == 4) comps[nc++] = 2;
-            if (P->seismo == 3) comps[nc++] = 3, comps[nc++] = 4;
-            float *sec = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
-            for (int c = 0; c < nc; c++) {
-                dn_ok(dn_get_seismogram(r->ctx, comps[c], sec), "seismograms");
-                io_ok(dnio_seis_name(P, comps[c], ishot, 1, r->name, sizeof r->name), "file name");
-                io_ok(dnio_write_seis(r->name, P, sec, r->ntr, P->nt, r->recpos, XS, YS), r->name);
-                printf("   wrote %s\n", r->name);
-            }
-            free(sec);
+    if (!r->batch) {
+        for (int ishot = first; ishot <= last; ishot++) {
+            shot_setup(r, ishot, NULL);
+            dn_ok(dn_run_forward(r->ctx, 0), "forward run");
+            write_shot_seis(r, ishot);
+        }
+        return;
+    }
+    dn_ctx *own = r->ctx, **cs = xmalloc(sizeof(dn_ctx *) * (size_t)r->nshots);
+    for (int k0 = first; k0 <= last;) {
+        int n = 0;
+        for (int ishot = k0; ishot <= last; ishot++) {
+            dn_ctx *c = shot_ctx(r, 0);
+            if (!c) break;
+            cs[n++] = r->ctx = c;
+            shot_setup(r, ishot, NULL);
+        }
+        if (n == 0) die("device memory", "one shot does not fit the device arena");
+        engine_phase(r, cs, n, 0);
+        for (int q = 0; q < n; q++) {
+            r->ctx = cs[q];
+            write_shot_seis(r, k0 + q);
+            dn_ctx_destroy(cs[q]);
+        }
+        k0 += n;
+    }
+    r->ctx = own;
+    free(cs);
+}
+
+/* grad_obj_psv: stacked vp/vs/rho gradients over all shots; returns the
+ * L2 misfit summed over the test shots.  MODE 2 takes the data themselves
+ * as adjoint sources (reverse-time migration). */
+/* the per-shot part of grad_obj_psv.c:215-266 after the adjoint run:
+ * ass_gradPSV, energy weights, the per-shot source taper, stacking and the
+ * per-shot images */
+static void shot_gradient(run *r, int ishot, double l2, int test, float **gs, float *ws, float *wr, float **g_sum)
+{
+    dnio_par *P = r->P;
+    dn_ok(dn_assemble_gradient(r->ctx), "gradient");
+    for (int k = 0; k < 3; k++) dn_ok(dn_get_gradient(r->ctx, 3 + k, gs[k]), "gradient");
+    if (r->eprecond) {
+        /* grad_obj_psv.c:219-238: energy weights of this shot */
+        dn_ok(dn_get_energy(r->ctx, 0, ws), "energy");
+        if (r->eprecond == 1) dn_ok(dn_get_energy(r->ctx, 1, wr), "energy");
+        io_ok(dnfwi_energy_precond(r->eprecond, P->nx, P->ny, P->dh, ws, wr, r->c_vp, r->c_vs, r->c_rho, gs[0],
+                                   gs[1], gs[2], NULL),
+              "energy preconditioning");
+    }
+    if (P->sws_taper_circular_per_shot) {
+        /* grad_obj_psv.c:241-250, taper_grad_shot.c: the source taper of this shot's source alone */
+        float col[8], *coeff = xmalloc(sizeof(float) * r->ncell);
+        for (int q = 0; q < 8; q++) col[q] = r->srcpos[(size_t)q * r->nsrc + (ishot - 1)];
+        for (int k = 0; k < 3; k++)
+            io_ok(dnfwi_taper_sources(P->nx, P->ny, P->dh, P->srtradius, P->srtshape, P->filtsize, 1, col, coeff,
+                                      gs[k]),
+                  "per-shot gradient taper");
+        free(coeff);
+    }
+    for (int k = 0; k < 3; k++) {
+        for (size_t q = 0; q < r->ncell; q++) g_sum[k][q] += gs[k][q];
+        if (P->rtm_shot && k < 2) { /* grad_obj_psv.c:263-266 */
+            snprintf(r->name, sizeof r->name, "%s_%s_image_shot_%i", P->jacobian, k == 0 ? "P" : "S", ishot);
+            write_plane(r->name, gs[k], P->nx, P->ny);
         }
     }
+    printf("   L2 of shot %d: %.9e%s\n", ishot, l2, test ? "" : " (not a test shot)");
 }
 
 /* grad_obj_psv: stacked vp/vs/rho gradients over all shots; returns the
@@ -527,45 +699,52 @@ static double gradient_pass(run *r, float **g_sum)
         ws = xmalloc(sizeof(float) * r->ncell);
         wr = xmalloc(sizeof(float) * r->ncell);
     }
-    dn_ok(dn_set_energy_weights(r->ctx, r->eprecond), "energy weights");
     for (int k = 0; k < 3; k++) memset(g_sum[k], 0, sizeof(float) * r->ncell);
     double misfit = 0.0;
-    for (int ishot = 1; ishot <= r->nshots; ishot++) {
-        if (r->stf_now) stf_shot(r, ishot); /* grad_obj_psv.c:127-129 */
-        shot_setup(r, ishot, NULL);
-        dn_ok(dn_run_forward(r->ctx, 0), "forward run");
-        const int test = is_test_shot(P, ishot);
-        const double l2 = P->mode == 2 ? shot_rtm_sources(r, ishot, test) : shot_residuals(r, ishot, test);
-        misfit += l2;
-        dn_ok(dn_run_adjoint(r->ctx, 0), "adjoint run");
-        dn_ok(dn_assemble_gradient(r->ctx), "gradient");
-        for (int k = 0; k < 3; k++) dn_ok(dn_get_gradient(r->ctx, 3 + k, gs[k]), "gradient");
-        if (r->eprecond) {
-            /* grad_obj_psv.c:219-238: energy weights of this shot */
-            dn_ok(dn_get_energy(r->ctx, 0, ws), "energy");
-            if (r->eprecond == 1) dn_ok(dn_get_energy(r->ctx, 1, wr), "energy");
-            io_ok(dnfwi_energy_precond(r->eprecond, P->nx, P->ny, P->dh, ws, wr, r->c_vp, r->c_vs, r->c_rho, gs[0],
-                                       gs[1], gs[2], NULL),
-                  "energy preconditioning");
+    if (!r->batch) {
+        dn_ok(dn_set_energy_weights(r->ctx, r->eprecond), "energy weights");
+        for (int ishot = 1; ishot <= r->nshots; ishot++) {
+            if (r->stf_now) stf_shot(r, ishot); /* grad_obj_psv.c:127-129 */
+            shot_setup(r, ishot, NULL);
+            dn_ok(dn_run_forward(r->ctx, 0), "forward run");
+            const int test = is_test_shot(P, ishot);
+            const double l2 = P->mode == 2 ? shot_rtm_sources(r, ishot, test) : shot_residuals(r, ishot, test);
+            misfit += l2;
+            dn_ok(dn_run_adjoint(r->ctx, 0), "adjoint run");
+            shot_gradient(r, ishot, l2, test, gs, ws, wr, g_sum);
         }
-        if (P->sws_taper_circular_per_shot) {
-            /* grad_obj_psv.c:241-250, taper_grad_shot.c: the source taper of this shot's source alone */
-            float col[8], *coeff = xmalloc(sizeof(float) * r->ncell);
-            for (int q = 0; q < 8; q++) col[q] = r->srcpos[(size_t)q * r->nsrc + (ishot - 1)];
-            for (int k = 0; k < 3; k++)
-                io_ok(dnfwi_taper_sources(P->nx, P->ny, P->dh, P->srtradius, P->srtshape, P->filtsize, 1, col, coeff,
-                                          gs[k]),
-                      "per-shot gradient taper");
-            free(coeff);
-        }
-        for (int k = 0; k < 3; k++) {
-            for (size_t q = 0; q < r->ncell; q++) g_sum[k][q] += gs[k][q];
-            if (P->rtm_shot && k < 2) { /* grad_obj_psv.c:263-266 */
-                snprintf(r->name, sizeof r->name, "%s_%s_image_shot_%i", P->jacobian, k == 0 ? "P" : "S", ishot);
-                write_plane(r->name, gs[k], P->nx, P->ny);
+    } else {
+        dn_ctx *own = r->ctx, **cs = xmalloc(sizeof(dn_ctx *) * (size_t)r->nshots);
+        double *l2s = xmalloc(sizeof(double) * (size_t)r->nshots);
+        for (int k0 = 1; k0 <= r->nshots;) {
+            int n = 0;
+            for (int ishot = k0; ishot <= r->nshots; ishot++) {
+                dn_ctx *c = shot_ctx(r, 1);
+                if (!c) break;
+                cs[n++] = r->ctx = c;
+                if (r->stf_now) stf_shot(r, ishot); /* grad_obj_psv.c:127-129 */
+                shot_setup(r, ishot, NULL);
             }
+            if (n == 0) die("device memory", "one gradient shot does not fit the device arena");
+            engine_phase(r, cs, n, 0);
+            for (int q = 0; q < n; q++) {
+                const int ishot = k0 + q, test = is_test_shot(P, ishot);
+                r->ctx = cs[q];
+                l2s[q] = P->mode == 2 ? shot_rtm_sources(r, ishot, test) : shot_residuals(r, ishot, test);
+                misfit += l2s[q];
+            }
+            engine_phase(r, cs, n, 1);
+            for (int q = 0; q < n; q++) {
+                const int ishot = k0 + q;
+                r->ctx = cs[q];
+                shot_gradient(r, ishot, l2s[q], is_test_shot(P, ishot), gs, ws, wr, g_sum);
+                dn_ctx_destroy(cs[q]);
+            }
+            k0 += n;
         }
-        printf("   L2 of shot %d: %.9e%s\n", ishot, l2, test ? "" : " (not a test shot)");
+        r->ctx = own;
+        free(cs);
+        free(l2s);
     }
     for (int k = 0; k < 3; k++) free(gs[k]);
     free(ws);
@@ -627,15 +806,47 @@ static double misfit_of(void *user, const float *vp, const float *vs, const floa
 {
     obj_env *o = user;
     run *r = o->r;
-    dn_ok(dn_set_model(r->ctx, vp, vs, rho, o->taus, o->taup), "test model");
     double l2 = 0.0;
-    for (int ishot = 1; ishot <= r->nshots; ishot++) {
-        if (!is_test_shot(r->P, ishot)) continue;
-        printf(" test forward modelling %d for shot %d\n", itest, ishot);
-        shot_setup(r, ishot, NULL);
-        dn_ok(dn_run_forward(r->ctx, 0), "forward run");
-        l2 += shot_residuals(r, ishot, 1);
+    if (!r->batch) {
+        dn_ok(dn_set_model(r->ctx, vp, vs, rho, o->taus, o->taup), "test model");
+        for (int ishot = 1; ishot <= r->nshots; ishot++) {
+            if (!is_test_shot(r->P, ishot)) continue;
+            printf(" test forward modelling %d for shot %d\n", itest, ishot);
+            shot_setup(r, ishot, NULL);
+            dn_ok(dn_run_forward(r->ctx, 0), "forward run");
+            l2 += shot_residuals(r, ishot, 1);
+        }
+        return l2;
     }
+    /* obj_psv.c:124 per test shot, the test shots together through the engine */
+    dn_ctx *own = r->ctx, **cs = xmalloc(sizeof(dn_ctx *) * (size_t)r->nshots);
+    int *ids = xmalloc(sizeof(int) * (size_t)r->nshots);
+    r->m_vp = vp;
+    r->m_vs = vs;
+    r->m_rho = rho;
+    for (int k0 = 1; k0 <= r->nshots;) {
+        int n = 0, ishot = k0;
+        for (; ishot <= r->nshots; ishot++) {
+            if (!is_test_shot(r->P, ishot)) continue;
+            dn_ctx *c = shot_ctx(r, 0);
+            if (!c) break;
+            printf(" test forward modelling %d for shot %d\n", itest, ishot);
+            ids[n] = ishot;
+            cs[n++] = r->ctx = c;
+            shot_setup(r, ishot, NULL);
+        }
+        if (n == 0 && ishot <= r->nshots) die("device memory", "one test shot does not fit the device arena");
+        if (n) engine_phase(r, cs, n, 0);
+        for (int q = 0; q < n; q++) {
+            r->ctx = cs[q];
+            l2 += shot_residuals(r, ids[q], 1);
+            dn_ctx_destroy(cs[q]);
+        }
+        k0 = ishot;
+    }
+    r->ctx = own;
+    free(cs);
+    free(ids);
     return l2;
 }
 
@@ -710,6 +921,9 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
                 r->c_rho = c_rho;
             }
             dn_ok(dn_set_model(r->ctx, vp, vs, rho, taus, taup), "model upload");
+            r->m_vp = vp;
+            r->m_vs = vs;
+            r->m_rho = rho;
             r->stf_now = r->inv_stf && iter == 1; /* grad_obj_psv.c:127 */
             const double L2sum = gradient_pass(r, g);
             r->stf_now = 0;
@@ -934,7 +1148,16 @@ int main(int argc, char **argv)
     dp.ts = r.TS;
     for (int l = 0; l < P.L && l < 8; l++) dp.fl[l] = P.fl[l];
 
-    dn_ok(dn_ctx_create(&dp, 0, &r.ctx), "device context");
+    r.dp = dp;
+    r.m_vp = vp;
+    r.m_vs = vs;
+    r.m_rho = rho;
+    r.m_taus = taus;
+    r.m_taup = taup;
+    r.batch = batching_applies(&r);
+    dn_params dmain = dp;
+    if (r.batch) dmain.mode = 0; /* the shots run in contexts of their own */
+    dn_ok(dn_ctx_create(&dmain, 0, &r.ctx), "device context");
     if (aniso)
         dn_ok(dn_set_model_aniso(r.ctx, rho, cani[0], cani[1], cani[2], cani[3], theta), "model upload");
     else
@@ -942,6 +1165,7 @@ int main(int argc, char **argv)
     if (P.seismo || P.mode >= 1) io_ok(dnio_read_receivers(&P, 1, &r.ntr, &r.recpos), "REC_FILE");
     r.n_order = st ? st->n_order : 0;
     if (P.mode >= 1 && P.trkill) r.kill = read_kill(P.trkill_file, r.nsrc, r.ntr);
+    batch_init(&r);
 
     if (P.mode == 0)
         forward_pass(&r);
@@ -950,6 +1174,7 @@ int main(int argc, char **argv)
     else
         rtm(&r, st);
 
+    batch_free(&r);
     dn_ctx_destroy(r.ctx);
     free(r.kill);
     free(r.flag);

@@ -243,7 +243,7 @@ int dn_set_kernel_sampling(dn_ctx *ctx, int every);
  * steps, which 1 (Wr, eprecond 1 only) the adjoint field at its
  * correlation steps.  dn_get_energy returns NX x NY, x-major.  The weights
  * (src/eprecond1.c) and the gradient scaling are the host's
- * (dnfwi_energy_precond).  0 switches it off.  Not taken by the engine. */
+ * (dnfwi_energy_precond).  0 switches it off.  The engine sums them too. */
 int dn_set_energy_weights(dn_ctx *ctx, int eprecond);
 int dn_get_energy(dn_ctx *ctx, int which, float *out);
 
@@ -300,9 +300,10 @@ int dn_ctx_create_in(const dn_params *p, dn_arena *a, dn_ctx **out);
  * step is one launch per sweep for all shots of the same FD order,
  * relaxation count, direction and tile shape, so shots of any grid size and
  * length fill the GPU together.  Results equal dn_run_forward /
- * dn_run_adjoint of each context alone.  While attached, a context must not
- * be used through the other entry points.  Moment-tensor shots (type 5) are
- * not accepted. */
+ * dn_run_adjoint of each context alone, energy weights (dn_set_energy_weights)
+ * included.  While attached, a context must not be used through the other
+ * entry points.  Moment-tensor shots (type 5) and VTI/TTI shots are not
+ * accepted. */
 typedef struct dn_engine dn_engine;
 int dn_engine_create(int device, dn_engine **out);
 int dn_engine_destroy(dn_engine *e);
