@@ -21,7 +21,8 @@ multi-shot engine in "batched"; a request stream through the server in
 "serving"; the elastic FWI gradient job of 2000x600, NT=4000 (forward with
 the snapshot store, adjoint with the correlation) in "gradient"; the
 drop-in executable over the shipped Marmousi example's 100 shots, with the
-multi-shot engine and shot by shot, in "executable".
+multi-shot engine and shot by shot, in "executable"; a VTI and a TTI shot of
+the same 4000 x 1200 grid in "detail.aniso".
 """
 from __future__ import annotations
 
@@ -441,6 +442,49 @@ def run_config1(threads, shots=5):
     return out
 
 
+def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20):
+    """One VTI (physics 3) or TTI (4) elastic shot on the 4000 x 1200 grid of
+    the headline workload (DH=5 m, FD8, DT=0.4 ms, CPML 20 + top switch,
+    Ricker 15 Hz, 800 receivers), seeded layered model with Thomsen
+    epsilon/delta 0.1-0.3 / 0.05-0.15 (TTI: tilt 30 +- 10 degrees).
+    Algorithmic bytes per cell-update: velocity 36 B, stress VTI 48 B
+    (5 wavefields + 4 stiffness planes read, 3 stresses written), TTI 64 B
+    (8 stiffness planes)."""
+    import dnhip
+    import synth
+    nx, ny, dh, dt, fc = 4000, 1200, 5.0, 4.0e-4, 15.0
+    rho, c11, c13, c33, c44, theta = synth.anisotropic_model(nx, ny, dh, 11, tilt=30.0 if physics == 4 else None)
+    p = dnhip.Params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=8, maxrelerror=1, fw=20, free_surf=1, seismo=1,
+                     ts=float(np.float32(1.0 / fc)), damping=1500.0, fpml=fc, physics=physics)
+    ctx = dnhip.Context(p)
+    ctx.set_model_aniso(rho, c11, c13, c33, c44, theta)
+    ctx.set_sources(synth.source_table(nx * dh / 2, 2 * dh, dh, fc=fc), synth.ricker(nt, dt, fc)[None, :])
+    ctx.set_receivers(synth.receiver_line(2 * dh, 25.0, 800, 2 * dh, dh))
+    if tune_steps > 0:
+        ctx.autotune(tune_steps)
+    ctx.forward()
+    ctx.set_kernel_sampling(sample_every)
+    t0 = time.perf_counter()
+    ctx.forward()
+    wall = time.perf_counter() - t0
+    tm = ctx.timing()
+    assert np.isfinite(ctx.seismogram(0)).all()
+    cells = nx * ny
+    sb = 48 if physics == 3 else 64
+    out = {"workload": f"{'VTI' if physics == 3 else 'TTI'} elastic 4000x1200, DH=5 m, FD8, NT={nt}",
+           "gcells_per_s": cells * nt / wall / 1e9, "ms_per_shot": wall * 1e3,
+           "tile": {"velocity": list(ctx.tile(0)), "stress": list(ctx.tile(1))},
+           "v_kernel_ms": tm["v_kernel_ms"], "s_kernel_ms": tm["s_kernel_ms"],
+           "stress_bytes_per_cell": sb, "vel_bytes_per_cell": 36}
+    if tm["s_kernel_ms"] > 0:
+        out["s_kernel_GBs"] = sb * cells / (tm["s_kernel_ms"] * 1e-3) / 1e9
+        out["v_kernel_GBs"] = 36 * cells / (tm["v_kernel_ms"] * 1e-3) / 1e9
+        out["s_kernel_frac_of_hbm"] = out["s_kernel_GBs"] / HBM_PEAK_GBS
+    out["step_frac_of_hbm"] = (sb + 36) * cells * nt / wall / 1e9 / HBM_PEAK_GBS
+    ctx.close()
+    return out
+
+
 def run_executable_marmousi(keep_shots=100):
     """The drop-in executable on the shipped Marmousi example's parameter,
     source and receiver files (500 x 174, DH 20 m, DT 2 ms, 6 s = 3000
@@ -482,10 +526,18 @@ def run_executable_marmousi(keep_shots=100):
             walls[label] = time.perf_counter() - t0
             if r.returncode != 0:
                 raise RuntimeError(f"denise_hip ({label}) failed: {r.stderr[-500:]}")
+            loops = [ln for ln in r.stdout.splitlines() if ln.startswith(" time loops:")]
+            walls[label + "_loops"] = float(loops[-1].split()[2]) if loops else None
         out["shots"] = len(shots)
         out["batched_s"] = walls["batched"]
         out["sequential_s"] = walls["sequential"]
         out["speedup"] = walls["sequential"] / walls["batched"]
+        # the time loops alone (engine phases / dn_run_forward calls), without
+        # start-up, host set-up per shot and SU file output
+        out["batched_loops_s"] = walls["batched_loops"]
+        out["sequential_loops_s"] = walls["sequential_loops"]
+        if walls["batched_loops"] and walls["sequential_loops"]:
+            out["loops_speedup"] = walls["sequential_loops"] / walls["batched_loops"]
         cells = 500 * 174 * 3000 * len(shots)
         out["batched_gcells_per_s"] = cells / walls["batched"] / 1e9
         out["sequential_gcells_per_s"] = cells / walls["sequential"] / 1e9
@@ -595,6 +647,9 @@ def main():
     ap.add_argument("--grad-jobs", type=int, default=2, help="timed gradient jobs (0: skip)")
     ap.add_argument("--grad-only", action="store_true", help="profiling: only the gradient job")
     ap.add_argument("--executable-only", action="store_true", help="only the executable on the Marmousi example")
+    ap.add_argument("--no-aniso", action="store_true", help="skip the VTI and TTI shots")
+    ap.add_argument("--aniso-only", type=int, default=0, metavar="PHYSICS",
+                    help="profiling: only the VTI (3) or TTI (4) shot")
     ap.add_argument("--no-executable", action="store_true",
                     help="skip the drop-in executable on the shipped Marmousi example (100 shots, MODE 0)")
     ap.add_argument("--fwi-iters", type=int, default=2,
@@ -626,6 +681,9 @@ def main():
             out["serving"] = run_serving(args.serve_requests, args.serve_qps, args.graph_steps, args.serve_saturate_qps)
         print(json.dumps(out))
         return
+    if args.aniso_only:
+        print(json.dumps({"aniso": run_aniso(args.aniso_only, nt=args.nt, tune_steps=args.tune_steps)}))
+        return
     if args.executable_only:
         print(json.dumps({"executable": run_executable_marmousi()}))
         return
@@ -656,6 +714,11 @@ def main():
     if args.grad_jobs > 0:
         gradient = run_gradient(args.grad_jobs, tune_steps=args.tune_steps, fwi_iters=args.fwi_iters)
         print(f"# gradient: {gradient}", file=sys.stderr, flush=True)
+    aniso = None
+    if not args.no_aniso:
+        aniso = {"vti": run_aniso(3, nt=args.nt, tune_steps=args.tune_steps),
+                 "tti": run_aniso(4, nt=args.nt, tune_steps=args.tune_steps)}
+        print(f"# aniso: {aniso}", file=sys.stderr, flush=True)
     executable = None
     if not args.no_executable:
         executable = run_executable_marmousi()
@@ -709,7 +772,7 @@ def main():
         "serving": serving,
         "gradient": gradient,
         "executable": executable,
-        "detail": {"visco": res, "elastic": elastic,
+        "detail": {"visco": res, "elastic": elastic, "aniso": aniso,
                    "elastic_step_frac_of_hbm": (elastic["step_GBs"] / HBM_PEAK_GBS) if elastic else None,
                    "elastic_kernels_frac_of_hbm": (
                        80 * elastic["cells_per_step"] / ((elastic["v_kernel_ms"] + elastic["s_kernel_ms"]) * 1e-3)

@@ -213,7 +213,7 @@ __global__ __launch_bounds__(256) void k_l2_sum(DevAcq q)
     X(64, 4, 1, 1, t64x4x1) X(128, 2, 1, 1, t128x2x1) X(256, 1, 1, 1, t256x1x1)                   \
     X(64, 4, 2, 1, t64x4x2) X(128, 2, 4, 1, t128x2x4) X(64, 4, 1, 4, t64x4x1v4)                   \
     X(64, 4, 2, 4, t64x4x2v4) X(64, 2, 4, 4, t64x2x4v4) X(64, 4, 1, 2, t64x4x1v2)                 \
-    X(128, 2, 2, 2, t128x2x2v2)
+    X(128, 2, 2, 2, t128x2x2v2) X(64, 8, 2, 1, t64x8x2) X(64, 4, 2, 2, t64x4x2v2)
 
 #define X(B, Y, T, W, N)                                                                    \
     void launch_vel_##N(const KArgs &, const StepCtl &, const DevAcq &, int, int, int, void *); \

@@ -46,6 +46,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "denise_fwi.h"
 #include "denise_hip.h"
@@ -65,6 +66,32 @@ static void io_ok(int rc, const char *what)
 static void dn_ok(int rc, const char *what)
 {
     if (rc) die(what, dn_last_error());
+}
+
+/* seconds spent in the time loops of the run (single-shot runs and engine
+ * phases), printed at the end */
+static double g_loop_s = 0.0;
+
+static double now_s(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
+
+/* dn_run_forward / dn_run_adjoint of the one-context loop, timed */
+static void run_forward(dn_ctx *c, const char *what)
+{
+    const double t0 = now_s();
+    dn_ok(dn_run_forward(c, 0), what);
+    g_loop_s += now_s() - t0;
+}
+
+static void run_adjoint(dn_ctx *c)
+{
+    const double t0 = now_s();
+    dn_ok(dn_run_adjoint(c, 0), "adjoint run");
+    g_loop_s += now_s() - t0;
 }
 
 static void *xmalloc(size_t n)
@@ -408,7 +435,7 @@ static void stf_shot(run *r, int ishot)
     r->time_filt = 0;
     shot_setup(r, ishot, spike);
     r->time_filt = tf;
-    dn_ok(dn_run_forward(r->ctx, 0), "STF forward run");
+    run_forward(r->ctx, "STF forward run");
     float *syn = xmalloc(sizeof(float) * n), *obs = xmalloc(sizeof(float) * n);
     dn_ok(dn_get_seismogram(r->ctx, 1, syn), "seismograms");
     io_ok(dnio_data_name(P, 1, ishot, r->name, sizeof r->name), "file name");
@@ -574,9 +601,11 @@ static void engine_phase(run *r, dn_ctx **cs, int n, int phase)
     const char *what = phase ? "adjoint run" : "forward run";
     printf(" engine: %d shot(s), %s\n", n, what);
     fflush(stdout);
+    const double t0 = now_s();
     for (int k = 0; k < n; k++) dn_ok(dn_engine_attach(r->eng, cs[k], phase), what);
     int running = n;
     while (running > 0) dn_ok(dn_engine_step(r->eng, 512, &running), what);
+    g_loop_s += now_s() - t0;
     for (int k = 0; k < n; k++) {
         int st = 0;
         dn_ok(dn_engine_status(r->eng, cs[k], &st), what);
@@ -621,7 +650,7 @@ static void forward_pass(run *r)
     if (!r->batch) {
         for (int ishot = first; ishot <= last; ishot++) {
             shot_setup(r, ishot, NULL);
-            dn_ok(dn_run_forward(r->ctx, 0), "forward run");
+            run_forward(r->ctx, "forward run");
             write_shot_seis(r, ishot);
         }
         return;
@@ -706,11 +735,11 @@ static double gradient_pass(run *r, float **g_sum)
         for (int ishot = 1; ishot <= r->nshots; ishot++) {
             if (r->stf_now) stf_shot(r, ishot); /* grad_obj_psv.c:127-129 */
             shot_setup(r, ishot, NULL);
-            dn_ok(dn_run_forward(r->ctx, 0), "forward run");
+            run_forward(r->ctx, "forward run");
             const int test = is_test_shot(P, ishot);
             const double l2 = P->mode == 2 ? shot_rtm_sources(r, ishot, test) : shot_residuals(r, ishot, test);
             misfit += l2;
-            dn_ok(dn_run_adjoint(r->ctx, 0), "adjoint run");
+            run_adjoint(r->ctx);
             shot_gradient(r, ishot, l2, test, gs, ws, wr, g_sum);
         }
     } else {
@@ -813,7 +842,7 @@ static double misfit_of(void *user, const float *vp, const float *vs, const floa
             if (!is_test_shot(r->P, ishot)) continue;
             printf(" test forward modelling %d for shot %d\n", itest, ishot);
             shot_setup(r, ishot, NULL);
-            dn_ok(dn_run_forward(r->ctx, 0), "forward run");
+            run_forward(r->ctx, "forward run");
             l2 += shot_residuals(r, ishot, 1);
         }
         return l2;
@@ -1174,6 +1203,7 @@ int main(int argc, char **argv)
     else
         rtm(&r, st);
 
+    printf(" time loops: %.6f s\n", g_loop_s);
     batch_free(&r);
     dn_ctx_destroy(r.ctx);
     free(r.kill);

@@ -206,8 +206,9 @@ int dn_get_hc(dn_ctx *ctx, float *out7);
  * marching ty rows of vw adjacent columns (vw 1, or 2 / 4 with vector
  * loads).  sweep: 0 velocity, 1 stress, -1 both (set only).  Compiled
  * shapes (tbx,by,ty,vw): (64,4,1,1), (128,2,1,1), (256,1,1,1), (64,4,2,1),
- * (128,2,4,1), (64,4,1,4), (64,4,2,4), (64,2,4,4), (64,4,1,2),
- * (128,2,2,2).  Results do not depend on the shapes.  No reference
+ * (128,2,4,1), (64,8,2,1), (64,4,1,4), (64,4,2,4), (64,2,4,4), (64,4,1,2),
+ * (128,2,2,2), (64,4,2,2); VTI/TTI stress sweeps take vw = 1 only.
+ * Results do not depend on the shapes.  No reference
  * counterpart: the reference has no GPU launch shapes. */
 int dn_set_tile(dn_ctx *ctx, int sweep, int tbx, int by, int ty, int vw);
 int dn_get_tile(dn_ctx *ctx, int sweep, int *tbx, int *by, int *ty, int *vw);

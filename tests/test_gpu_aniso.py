@@ -111,8 +111,8 @@ def test_source_types(physics, stype):
     g.close()
 
 
-SCALAR = [(64, 4, 1, 1), (128, 2, 1, 1), (256, 1, 1, 1), (64, 4, 2, 1), (128, 2, 4, 1)]
-VECTOR = [(64, 4, 1, 4), (64, 4, 2, 4), (64, 2, 4, 4), (64, 4, 1, 2), (128, 2, 2, 2)]
+SCALAR = [(64, 4, 1, 1), (128, 2, 1, 1), (256, 1, 1, 1), (64, 4, 2, 1), (128, 2, 4, 1), (64, 8, 2, 1)]
+VECTOR = [(64, 4, 1, 4), (64, 4, 2, 4), (64, 2, 4, 4), (64, 4, 1, 2), (128, 2, 2, 2), (64, 4, 2, 2)]
 
 
 @pytest.mark.parametrize("physics", [3, 4])

@@ -337,8 +337,8 @@ def test_adjoint_requires_forward_store():
     assert ei.value.code == -5
 
 
-TILES = [(64, 4, 1, 1), (128, 2, 1, 1), (256, 1, 1, 1), (64, 4, 2, 1), (128, 2, 4, 1), (64, 4, 1, 4),
-         (64, 4, 2, 4), (64, 2, 4, 4), (64, 4, 1, 2), (128, 2, 2, 2)]
+TILES = [(64, 4, 1, 1), (128, 2, 1, 1), (256, 1, 1, 1), (64, 4, 2, 1), (128, 2, 4, 1), (64, 8, 2, 1),
+         (64, 4, 1, 4), (64, 4, 2, 4), (64, 2, 4, 4), (64, 4, 1, 2), (128, 2, 2, 2), (64, 4, 2, 2)]
 
 
 @pytest.mark.parametrize("tile", TILES)
