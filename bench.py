@@ -443,21 +443,28 @@ def run_config1(threads, shots=5):
 
 
 def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20):
-    """One VTI (physics 3) or TTI (4) elastic shot on the 4000 x 1200 grid of
-    the headline workload (DH=5 m, FD8, DT=0.4 ms, CPML 20 + top switch,
-    Ricker 15 Hz, 800 receivers), seeded layered model with Thomsen
-    epsilon/delta 0.1-0.3 / 0.05-0.15 (TTI: tilt 30 +- 10 degrees).
-    Algorithmic bytes per cell-update: velocity 36 B, stress VTI 48 B
-    (5 wavefields + 4 stiffness planes read, 3 stresses written), TTI 64 B
-    (8 stiffness planes)."""
+    """One acoustic (physics 2), VTI (3) or TTI (4) shot on the 4000 x 1200
+    grid of the headline workload (DH=5 m, FD8, DT=0.4 ms, CPML 20 + free
+    surface / top switch, Ricker 15 Hz, 800 receivers); the acoustic shot on
+    the seeded layered vp/rho model, VTI/TTI on a seeded layered model with
+    Thomsen epsilon/delta 0.1-0.3 / 0.05-0.15 (TTI: tilt 30 +- 10 degrees).
+    Algorithmic bytes per cell-update: acoustic velocity 28 B (p, rip, rjp,
+    vx, vy read; vx, vy written), pressure 20 B (vx, vy, p, modulus read; p
+    written); elastic velocity 36 B, stress VTI 48 B (5 wavefields + 4
+    stiffness planes read, 3 stresses written), TTI 64 B (8 stiffness
+    planes)."""
     import dnhip
     import synth
     nx, ny, dh, dt, fc = 4000, 1200, 5.0, 4.0e-4, 15.0
-    rho, c11, c13, c33, c44, theta = synth.anisotropic_model(nx, ny, dh, 11, tilt=30.0 if physics == 4 else None)
     p = dnhip.Params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=8, maxrelerror=1, fw=20, free_surf=1, seismo=1,
                      ts=float(np.float32(1.0 / fc)), damping=1500.0, fpml=fc, physics=physics)
     ctx = dnhip.Context(p)
-    ctx.set_model_aniso(rho, c11, c13, c33, c44, theta)
+    if physics == 2:
+        vp, _, rho = synth.layered_model(nx, ny, dh, 11)
+        ctx.set_model(vp, None, rho)
+    else:
+        rho, c11, c13, c33, c44, theta = synth.anisotropic_model(nx, ny, dh, 11, tilt=30.0 if physics == 4 else None)
+        ctx.set_model_aniso(rho, c11, c13, c33, c44, theta)
     ctx.set_sources(synth.source_table(nx * dh / 2, 2 * dh, dh, fc=fc), synth.ricker(nt, dt, fc)[None, :])
     ctx.set_receivers(synth.receiver_line(2 * dh, 25.0, 800, 2 * dh, dh))
     if tune_steps > 0:
@@ -470,17 +477,20 @@ def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20):
     tm = ctx.timing()
     assert np.isfinite(ctx.seismogram(0)).all()
     cells = nx * ny
-    sb = 48 if physics == 3 else 64
-    out = {"workload": f"{'VTI' if physics == 3 else 'TTI'} elastic 4000x1200, DH=5 m, FD8, NT={nt}",
+    sb = {2: 20, 3: 48, 4: 64}[physics]
+    vb = 28 if physics == 2 else 36
+    name = {2: "acoustic", 3: "VTI elastic", 4: "TTI elastic"}[physics]
+    out = {"workload": f"{name} 4000x1200, DH=5 m, FD8, NT={nt}",
            "gcells_per_s": cells * nt / wall / 1e9, "ms_per_shot": wall * 1e3,
            "tile": {"velocity": list(ctx.tile(0)), "stress": list(ctx.tile(1))},
            "v_kernel_ms": tm["v_kernel_ms"], "s_kernel_ms": tm["s_kernel_ms"],
-           "stress_bytes_per_cell": sb, "vel_bytes_per_cell": 36}
+           "stress_bytes_per_cell": sb, "vel_bytes_per_cell": vb}
     if tm["s_kernel_ms"] > 0:
         out["s_kernel_GBs"] = sb * cells / (tm["s_kernel_ms"] * 1e-3) / 1e9
-        out["v_kernel_GBs"] = 36 * cells / (tm["v_kernel_ms"] * 1e-3) / 1e9
+        out["v_kernel_GBs"] = vb * cells / (tm["v_kernel_ms"] * 1e-3) / 1e9
         out["s_kernel_frac_of_hbm"] = out["s_kernel_GBs"] / HBM_PEAK_GBS
-    out["step_frac_of_hbm"] = (sb + 36) * cells * nt / wall / 1e9 / HBM_PEAK_GBS
+        out["v_kernel_frac_of_hbm"] = out["v_kernel_GBs"] / HBM_PEAK_GBS
+    out["step_frac_of_hbm"] = (sb + vb) * cells * nt / wall / 1e9 / HBM_PEAK_GBS
     ctx.close()
     return out
 
@@ -647,9 +657,9 @@ def main():
     ap.add_argument("--grad-jobs", type=int, default=2, help="timed gradient jobs (0: skip)")
     ap.add_argument("--grad-only", action="store_true", help="profiling: only the gradient job")
     ap.add_argument("--executable-only", action="store_true", help="only the executable on the Marmousi example")
-    ap.add_argument("--no-aniso", action="store_true", help="skip the VTI and TTI shots")
+    ap.add_argument("--no-aniso", action="store_true", help="skip the acoustic, VTI and TTI shots")
     ap.add_argument("--aniso-only", type=int, default=0, metavar="PHYSICS",
-                    help="profiling: only the VTI (3) or TTI (4) shot")
+                    help="profiling: only the acoustic (2), VTI (3) or TTI (4) shot")
     ap.add_argument("--no-executable", action="store_true",
                     help="skip the drop-in executable on the shipped Marmousi example (100 shots, MODE 0)")
     ap.add_argument("--fwi-iters", type=int, default=2,
@@ -716,7 +726,8 @@ def main():
         print(f"# gradient: {gradient}", file=sys.stderr, flush=True)
     aniso = None
     if not args.no_aniso:
-        aniso = {"vti": run_aniso(3, nt=args.nt, tune_steps=args.tune_steps),
+        aniso = {"acoustic": run_aniso(2, nt=args.nt, tune_steps=args.tune_steps),
+                 "vti": run_aniso(3, nt=args.nt, tune_steps=args.tune_steps),
                  "tti": run_aniso(4, nt=args.nt, tune_steps=args.tune_steps)}
         print(f"# aniso: {aniso}", file=sys.stderr, flush=True)
     executable = None

@@ -50,7 +50,9 @@ void fill_args(dn_ctx *c)
     a.fw2 = 2 * p.fw;
     a.free_surf = p.free_surf;
     a.physics = p.physics == 0 ? 1 : p.physics;
-    a.mirror = p.free_surf && a.physics == 1;
+    // the P/SV and acoustic loops mirror the ghost rows above the free
+    // surface (surface_*_PML_PSV.c, surface_acoustic_PML_AC.c); VTI/TTI do not
+    a.mirror = p.free_surf && (a.physics == 1 || a.physics == 2);
     a.boundary = p.boundary;
     a.grad_form = p.grad_form;
     a.L = p.L;
@@ -83,6 +85,9 @@ void fill_args(dn_ctx *c)
     a.vs = at(c->vs);
     a.rho = at(c->rho);
     for (int k = 0; k < 8; ++k) a.ani[k] = at(c->ani[k]);
+    // acoustic: the pressure lives in the sxx plane and is both normal
+    // stress of the velocity sweep; sxy stays 0
+    if (a.physics == 2) a.syy = a.sxx;
     a.rmem = at(c->rmem);
     a.pmem = at(c->pmem);
     a.qmem = at(c->qmem);
@@ -301,10 +306,10 @@ int validate(const dn_params *p)
     if (p->mode > 0 && p->dtinv < 1) return fail(DN_ERR_ARG, "DTINV must be >= 1");
     if (p->mode > 0 && p->grad_form != 1 && p->grad_form != 2) return fail(DN_ERR_ARG, "GRAD_FORM must be 1 or 2");
     if (!(p->dh > 0.0f) || !(p->dt > 0.0f)) return fail(DN_ERR_ARG, "DH and DT must be positive");
-    if (p->physics != 0 && p->physics != 1 && p->physics != 3 && p->physics != 4)
-        return fail(DN_ERR_ARG, "PHYSICS must be 1 (P/SV), 3 (VTI) or 4 (TTI)");
-    if ((p->physics == 3 || p->physics == 4) && (p->L != 0 || p->mode != 0))
-        return fail(DN_ERR_ARG, "VTI/TTI runs are elastic forward modelling: L = 0 and MODE 0");
+    if (p->physics != 0 && p->physics != 1 && p->physics != 2 && p->physics != 3 && p->physics != 4)
+        return fail(DN_ERR_ARG, "PHYSICS must be 1 (P/SV), 2 (acoustic), 3 (VTI) or 4 (TTI)");
+    if ((p->physics == 2 || p->physics == 3 || p->physics == 4) && (p->L != 0 || p->mode != 0))
+        return fail(DN_ERR_ARG, "acoustic, VTI and TTI runs are forward modelling: L = 0 and MODE 0");
     if (p->L > 0) {
         for (int l = 0; l < p->L; ++l)
             if (!(p->fl[l] > 0.0f)) return fail(DN_ERR_ARG, "relaxation frequencies FL must be positive");
@@ -603,9 +608,40 @@ int dn_ctx_destroy(dn_ctx *ctx)
     return DN_OK;
 }
 
+// acoustic model (params.physics 2): readmod_AC's vp (INVMAT1 1) or
+// modulus (3) and rho, x-major; vs unused
+static int set_model_ac(dn_ctx *c, const float *vp, const float *rho)
+{
+    if (!vp || !rho) return fail(DN_ERR_ARG, "null argument");
+    HIP_OK(hipSetDevice(c->device));
+    std::string err;
+    int rc = check_stability_ac(c->p, c->setup.hc, vp, rho, err);
+    if (rc) return fail(rc, err);
+    prepare_material_ac(c->p, c->lay, vp, rho, c->mat);
+    const size_t pl = c->lay.plane();
+    if ((rc = upload(c, c->rip, c->mat.rip.data(), pl)) || (rc = upload(c, c->rjp, c->mat.rjp.data(), pl)) ||
+        (rc = upload(c, c->ani[0], c->mat.ani[0].data(), pl)))
+        return rc;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (auto *v : {&c->mat.rip, &c->mat.rjp, &c->mat.ani[0]}) {
+        v->clear();
+        v->shrink_to_fit();
+    }
+    // seismo_AC's div and curl factors both come from vp (src/AC/ac.c:199-201)
+    const size_t n = (size_t)c->p.nx * c->p.ny;
+    c->h_vp.assign(vp, vp + n);
+    c->h_vs.assign(vp, vp + n);
+    c->h_rho.assign(rho, rho + n);
+    fill_args(c);
+    c->model_set = true;
+    c->fwd_complete = false;
+    return receiver_factors(c);
+}
+
 int dn_set_model(dn_ctx *c, const float *vp, const float *vs, const float *rho, const float *taus,
                  const float *taup)
 {
+    if (c && c->p.physics == 2) return set_model_ac(c, vp, rho);
     if (!c || !vp || !vs || !rho) return fail(DN_ERR_ARG, "null argument");
     if (aniso_planes(c->p.physics)) return fail(DN_ERR_ARG, "VTI/TTI models are set with dn_set_model_aniso");
     if (c->p.L > 0 && (!taus || !taup)) return fail(DN_ERR_ARG, "viscoelastic model needs taus and taup");
@@ -679,7 +715,7 @@ int dn_set_model_aniso(dn_ctx *c, const float *rho, const float *c11, const floa
 {
     if (!c || !rho || !c11 || !c13 || !c33 || !c44) return fail(DN_ERR_ARG, "null argument");
     const int np = aniso_planes(c->p.physics);
-    if (!np) return fail(DN_ERR_ARG, "dn_set_model_aniso needs params.physics 3 (VTI) or 4 (TTI)");
+    if (np < 4) return fail(DN_ERR_ARG, "dn_set_model_aniso needs params.physics 3 (VTI) or 4 (TTI)");
     if (c->p.physics == 4 && !theta) return fail(DN_ERR_ARG, "a TTI model needs theta");
     HIP_OK(hipSetDevice(c->device));
     std::string err;
@@ -1020,7 +1056,8 @@ int dn_run_forward(dn_ctx *c, int nsteps)
             HIP_OK(hipEventRecord(c->pool[nev + 3], c->stream));
             nev += 4;
         }
-        if (c->qtype == 5 && c->acq.nsrc > 0) launch_msource(c->a, c->acq, nt, c->stream);
+        // the acoustic loop has no moment-tensor source (src/AC/ac.c:155-170)
+        if (c->qtype == 5 && c->acq.nsrc > 0 && c->a.physics != 2) launch_msource(c->a, c->acq, nt, c->stream);
         if (store && c->qtype == 5) launch_store(c->a, c->fwd, hin - 1, c->stream);
         if (store && c->eprecond) launch_energy(c->a, c->ew[0], c->stream);  // psv.c:300-303
         if (store) {
@@ -1225,8 +1262,10 @@ int dn_get_field(dn_ctx *c, int id, float *out)
 {
     if (!c || !out || id < 0 || id >= F_N) return fail(DN_ERR_ARG, "bad field request");
     if (!c->fld[id]) return fail(DN_ERR_STATE, "field not allocated in this configuration");
-    int rc = get_plane_nr(c, c->fld[id], out);
-    if (rc || !c->a.mirror || (id != F_SYY && id != F_SXY)) return rc;
+    // acoustic contexts keep the pressure in the sxx plane, read as syy too
+    int rc = get_plane_nr(c, (id == F_SYY && c->a.physics == 2) ? c->fld[F_SXX] : c->fld[id], out);
+    if (rc || !c->a.mirror || (id != F_SYY && id != F_SXY && !(id == F_SXX && c->a.physics == 2))) return rc;
+    if (id == F_SXX) id = F_SYY;  // the acoustic pressure: mirrored like syy (surface_acoustic_PML_AC.c:20-25)
     // With a free surface the sweeps mirror the stress ghost rows on the fly;
     // materialise them as the reference's surface routine stores them:
     // syy[1-m] = -syy[1+m], sxy[1-m] = -sxy[m], m = 1..FDORDER/2.

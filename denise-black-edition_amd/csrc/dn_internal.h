@@ -64,8 +64,9 @@ struct HostMaterial {
     std::vector<float> ani[8];
 };
 
-// number of stiffness planes of an anisotropic run (0 for isotropic P/SV)
-inline int aniso_planes(int physics) { return physics == 3 ? 4 : (physics == 4 ? 8 : 0); }
+// material planes of the acoustic (physics 2: the modulus g) and the
+// anisotropic runs (3 VTI, 4 TTI: the stiffness planes); 0 for isotropic P/SV
+inline int aniso_planes(int physics) { return physics == 2 ? 1 : (physics == 3 ? 4 : (physics == 4 ? 8 : 0)); }
 
 // set-up math (dn_setup.cpp)
 int holberg(int fdorder, int maxrelerror, float hc[7], std::string &err);
@@ -74,6 +75,8 @@ int check_stability(const dn_params &p, const float hc[7], const float *vp, cons
                     const float *rho, const float *taus, const float *taup, std::string &err);
 void prepare_material(const dn_params &p, const Layout &lay, const float *vp, const float *vs,
                       const float *rho, const float *taus, const float *taup, HostMaterial &m);
+int check_stability_ac(const dn_params &p, const float hc[7], const float *vp, const float *rho, std::string &err);
+void prepare_material_ac(const dn_params &p, const Layout &lay, const float *vp, const float *rho, HostMaterial &m);
 int check_stability_vti(const dn_params &p, const float hc[7], const float *rho, const float *c11,
                         const float *c13, const float *c33, const float *c44, std::string &err);
 void prepare_material_aniso(const dn_params &p, const Layout &lay, const float *rho, const float *c11,

@@ -507,4 +507,55 @@ void prepare_material_aniso(const dn_params &p, const Layout &lay, const float *
     harm(D55, m.ani[7]);
 }
 
+
+// Stability limit of the acoustic driver, src/AC/checkfd_acoustic.c:33-68:
+// vmax = vp (INVMAT1 1) or sqrt(pi/rho) (INVMAT1 3).
+int check_stability_ac(const dn_params &p, const float hc[7], const float *vp, const float *rho, std::string &err)
+{
+    const size_t n = (size_t)p.nx * p.ny;
+    float cmax = 0.0f;
+    for (size_t k = 0; k < n; ++k) {
+        const float c = p.invmat1 == 3 ? (float)std::sqrt(vp[k] / rho[k]) : vp[k];
+        if (cmax < c) cmax = c;
+    }
+    const float gamma = std::fabs(hc[1]) + std::fabs(hc[2]) + std::fabs(hc[3]) + std::fabs(hc[4]) +
+                        std::fabs(hc[5]) + std::fabs(hc[6]);
+    const float dtstab = (float)(p.dh / (std::sqrt(2.0) * gamma * cmax));
+    if (p.dt > dtstab) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, " The simulation will get unstable, choose smaller DT. (DT=%e > %e)",
+                      (double)p.dt, (double)dtstab);
+        err = buf;
+        return DN_ERR_CFL;
+    }
+    return DN_OK;
+}
+
+// Acoustic material (src/AC/FD_AC.c:184-207): the one-rank periodic ghosts
+// of matcopy_acoustic_AC, av_rho (src/av_rho.c:5-28), and the modulus of
+// the pressure update, g = rho vp^2 (INVMAT1 1) or the given modulus (3)
+// (update_s_acoustic_PML_AC.c:303-307).
+void prepare_material_ac(const dn_params &p, const Layout &lay, const float *vp, const float *rho, HostMaterial &m)
+{
+    const int NX = p.nx, NY = p.ny;
+    const size_t plane = lay.plane();
+    auto dev = [&](int j, int i) { return lay.origin() + (size_t)lay.off(j, i); };
+    Ring R(NX, NY), PI(NX, NY);
+    R.load(rho);
+    PI.load(vp);
+    m.rip.assign(plane, 0.0f);
+    m.rjp.assign(plane, 0.0f);
+    for (int k = 0; k < 8; ++k) m.ani[k].assign(k < 1 ? plane : 0, 0.0f);
+    for (int j = 1; j <= NY; ++j)
+        for (int i = 1; i <= NX; ++i) {
+            const size_t o = dev(j, i);
+            float rj = (float)(1.0 / (0.5 * (R.at(j, i) + R.at(j + 1, i))));
+            float ri = (float)(1.0 / (0.5 * (R.at(j, i) + R.at(j, i + 1))));
+            if (R.at(j, i) < 1e-4 && R.at(j + 1, i) < 1e-4) rj = 0.0f;
+            if (R.at(j, i) < 1e-4 && R.at(j, i + 1) < 1e-4) ri = 0.0f;
+            m.rjp[o] = rj;
+            m.rip[o] = ri;
+            m.ani[0][o] = p.invmat1 == 3 ? PI.at(j, i) : R.at(j, i) * PI.at(j, i) * PI.at(j, i);
+        }
+}
 }  // namespace dn

@@ -105,12 +105,14 @@ static void *xmalloc(size_t n)
  * silently computing something else */
 static void check_supported(const dnio_par *p, const dnio_stage *st)
 {
-    if (p->physics != 1 && p->physics != 3 && p->physics != 4)
-        die("PHYSICS", "2D P/SV (1), VTI (3) and TTI (4) are built; AC (2) and SH (5) are not");
-    /* physics_VTI.c / physics_TTI.c: MODE 0 forward modelling and MODE 2
-     * RTM; the FWI branch is commented out there */
-    if (p->physics != 1 && p->mode != 0) die("MODE", "VTI and TTI are built for forward modelling (MODE 0)");
-    if (p->physics != 1 && p->snap) die("SNAP", "wavefield snapshots of the VTI/TTI loop (snap_ssg_VTI) are not built");
+    if (p->physics < 1 || p->physics > 4)
+        die("PHYSICS", "2D P/SV (1), acoustic (2), VTI (3) and TTI (4) are built; SH (5) is not");
+    /* physics_AC.c / physics_VTI.c / physics_TTI.c: of their modes, forward
+     * modelling (MODE 0) is built */
+    if (p->physics != 1 && p->mode != 0)
+        die("MODE", "the acoustic, VTI and TTI loops are built for forward modelling (MODE 0)");
+    if (p->physics != 1 && p->snap)
+        die("SNAP", "wavefield snapshots of the acoustic and VTI/TTI loops (snap_AC, snap_ssg_VTI) are not built");
     if (p->mode < 0 || p->mode > 2) die("MODE", "MODE 0 (forward), 1 (FWI) and 2 (RTM) are built");
     if (p->boundary != 0) die("BOUNDARY", "periodic boundaries are not supported");
     if (!p->srcrec) die("SRCREC", "sources must come from SOURCE_FILE (SRCREC = 1)");
@@ -1133,8 +1135,9 @@ int main(int argc, char **argv)
     const int aniso = P.physics == 3 || P.physics == 4;
     float *vp = xmalloc(sizeof(float) * r.ncell), *vs = xmalloc(sizeof(float) * r.ncell);
     float *rho = xmalloc(sizeof(float) * r.ncell);
-    float *taus = P.L && !aniso ? xmalloc(sizeof(float) * r.ncell) : NULL;
-    float *taup = P.L && !aniso ? xmalloc(sizeof(float) * r.ncell) : NULL;
+    const int elastic_only = aniso || P.physics == 2;
+    float *taus = P.L && !elastic_only ? xmalloc(sizeof(float) * r.ncell) : NULL;
+    float *taup = P.L && !elastic_only ? xmalloc(sizeof(float) * r.ncell) : NULL;
     float *cani[4] = {NULL, NULL, NULL, NULL}, *theta = NULL;
     if (aniso) {
         /* c11, c13, c33, c44 (+ theta for TTI); the VTI/TTI loops update the
@@ -1142,6 +1145,10 @@ int main(int argc, char **argv)
         for (int k = 0; k < 4; k++) cani[k] = xmalloc(sizeof(float) * r.ncell);
         if (P.physics == 4) theta = xmalloc(sizeof(float) * r.ncell);
         io_ok(dnio_read_model_aniso(&P, rho, cani[0], cani[1], cani[2], cani[3], theta), "model");
+    } else if (P.physics == 2) {
+        /* vp (or the modulus for INVMAT1 3) and rho; the acoustic loop has
+         * no relaxation mechanisms (ac.c) */
+        io_ok(dnio_read_model_ac(&P, vp, rho), "model");
     } else {
         io_ok(dnio_read_model(&P, vp, vs, rho, taus, taup), "model");
     }
@@ -1159,7 +1166,7 @@ int main(int argc, char **argv)
     dp.fw = P.fw;
     dp.free_surf = P.free_surf;
     dp.boundary = P.boundary;
-    dp.L = aniso ? 0 : P.L;
+    dp.L = elastic_only ? 0 : P.L;
     dp.physics = P.physics;
     dp.seismo = P.mode >= 1 && P.seismo == 0 ? 1 : P.seismo;
     dp.ndt = P.ndt;

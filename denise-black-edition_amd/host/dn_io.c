@@ -576,6 +576,29 @@ int dnio_read_model_aniso(const dnio_par *p, float *rho, float *c11, float *c13,
     return rc;
 }
 
+/* Acoustic models (PHYSICS 2): readmod_AC.c:27-93 reads MFILE.vp (INVMAT1 1)
+ * or MFILE.lam (INVMAT1 3) and MFILE.rho in the same x-major order and
+ * always writes the copies MFILE.fdveps.pi and .fdveps.rho (merged file and
+ * the one-rank part .0.0). */
+int dnio_read_model_ac(const dnio_par *p, float *vp, float *rho)
+{
+    if (!p || !vp || !rho) return fail("null argument");
+    if (p->physics != 2) return fail("PHYSICS = %d: 2 (acoustic) expected", p->physics);
+    if (!p->readmod) return fail("READMOD = 0 (built-in model functions) is not supported; give model files");
+    if (p->invmat1 != 1 && p->invmat1 != 3) return fail("INVMAT1 = %d: 1 or 3 expected", p->invmat1);
+    const size_t n = (size_t)p->nx * p->ny;
+    if (read_model_file(p->mfile, p->invmat1 == 1 ? "vp" : "lam", vp, n) || read_model_file(p->mfile, "rho", rho, n))
+        return -1;
+    const char *ext[2] = {"fdveps.pi", "fdveps.rho"};
+    const float *arr[2] = {vp, rho};
+    char name[2 * DNIO_STR];
+    for (int k = 0; k < 2; k++) {
+        snprintf(name, sizeof name, "%s.%s", p->mfile, ext[k]);
+        if (dnio_write_model(name, arr[k], p->nx, p->ny)) return -1;
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------------
  * seismograms (src/outseis_glob.c:8-303, src/inseis.c:8-110,
  * include/segy.h:429-710)

@@ -71,9 +71,9 @@ typedef struct dn_params {
     float damping, fpml, npower, k_max_pml;   /* CPML (src/PML_pro.c)          */
     float ts;                /* TS = 1/max(fc) (src/sources.c:92)              */
     float fl[8];             /* FL[1..L]                                       */
-    int physics;             /* PHYSICS: 1 (or 0) isotropic P/SV, 3 VTI, 4 TTI
-                                (src/denise.c:116-141); 3 and 4 model elastic
-                                forward runs (MODE 0, L = 0)                  */
+    int physics;             /* PHYSICS: 1 (or 0) isotropic P/SV, 2 acoustic,
+                                3 VTI, 4 TTI (src/denise.c:116-141); 2, 3 and
+                                4 model forward runs (MODE 0, L = 0)          */
 } dn_params;
 
 typedef struct dn_ctx dn_ctx;
@@ -103,7 +103,10 @@ int dn_ctx_destroy(dn_ctx *ctx);
  * periodic ghosts), av_mue, av_rho, av_tau, prepare_update_s_visc_PSV
  * (src/PSV/FD_PSV.c:203-249).  vp/vs/rho (INVMAT1=3: lambda/mu/rho) and, for
  * L>0, taus/taup: NX*NY x-major.  Runs checkfd_ssg_*: DN_ERR_CFL if DT is
- * above the stability limit. */
+ * above the stability limit.  Acoustic contexts (params.physics 2) take vp
+ * (INVMAT1=3: the modulus) and rho only, vs/taus/taup ignored (may be
+ * NULL): readmod_AC, matcopy_acoustic_AC, av_rho and checkfd_acoustic
+ * (src/AC/FD_AC.c:184-207). */
 int dn_set_model(dn_ctx *ctx, const float *vp, const float *vs, const float *rho,
                  const float *taus, const float *taup);
 

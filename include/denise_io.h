@@ -151,6 +151,10 @@ int dnio_read_model(const dnio_par *p, float *vp, float *vs, float *rho, float *
  * x-major each, theta NULL for VTI. */
 int dnio_read_model_aniso(const dnio_par *p, float *rho, float *c11, float *c13, float *c33, float *c44,
                           float *theta);
+/* acoustic model files (PHYSICS 2): MFILE.vp (INVMAT1 1) or MFILE.lam (3)
+ * and MFILE.rho, NX*NY x-major; writes MFILE.fdveps.pi and .fdveps.rho as
+ * src/AC/readmod_AC.c:27-93 does. */
+int dnio_read_model_ac(const dnio_par *p, float *vp, float *rho);
 
 /* writemod + mergemod on one rank (src/writemod.c, src/mergemod.c): a
  * float32 x-major NX x NY file `name` and its per-rank copy `name.0.0`. */

@@ -944,6 +944,168 @@ static void update_s_visc(orc_ctx *c, int mode)
 
 /* time derivative of the source signal by central differences, one-sided at
  * the ends (src/psource.c:28-31, src/msource.c:27-30) */
+/* ------------------------------------------------------------------------- */
+/* acoustic (PHYSICS 2): pressure p (kept in the sxx array) and velocities    */
+/* ------------------------------------------------------------------------- */
+
+static real source_amp(const orc_ctx *c, int l, int nt);
+
+/* src/AC/update_v_PML_AC.c:11-676 (FD8 block :328-420; forces :646-656) */
+static void update_v_ac(orc_ctx *c, int nt)
+{
+    const orc_params *P = &c->P;
+    const int NX = P->nx, NY = P->ny, FW = P->fw, fdoh = c->fdoh;
+    const real DT = (real)P->dt, DH = (real)P->dh;
+    const real *hc = c->hc;
+    real *const *pm = c->pml;
+    const arr p = c->sxx;
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= NY; j++) {
+        for (int i = 1; i <= NX; i++) {
+            real p_x = hc[1] * (AT(p, j, i + 1) - AT(p, j, i));
+            real p_y = hc[1] * (AT(p, j + 1, i) - AT(p, j, i));
+            for (int m = 2; m <= fdoh; m++) {
+                p_x = p_x + hc[m] * (AT(p, j, i + m) - AT(p, j, i - m + 1));
+                p_y = p_y + hc[m] * (AT(p, j + m, i) - AT(p, j - m + 1, i));
+            }
+            if (!P->boundary && i <= FW) {
+                real *ps = &PSIX(c->psi_sxx_x, j, i);
+                *ps = pm[PML_BXH][i] * *ps + pm[PML_AXH][i] * p_x;
+                p_x = p_x / pm[PML_KXH][i] + *ps;
+            }
+            if (!P->boundary && i >= NX - FW + 1) {
+                int h1 = i - NX + 2 * FW;
+                real *ps = &PSIX(c->psi_sxx_x, j, h1);
+                *ps = pm[PML_BXH][h1] * *ps + pm[PML_AXH][h1] * p_x;
+                p_x = p_x / pm[PML_KXH][h1] + *ps;
+            }
+            if (!P->free_surf && j <= FW) {
+                real *ps = &PSIY(c->psi_syy_y, j, i);
+                *ps = pm[PML_BYH][j] * *ps + pm[PML_AYH][j] * p_y;
+                p_y = p_y / pm[PML_KYH][j] + *ps;
+            }
+            if (j >= NY - FW + 1) {
+                int h1 = j - NY + 2 * FW;
+                real *ps = &PSIY(c->psi_syy_y, h1, i);
+                *ps = pm[PML_BYH][h1] * *ps + pm[PML_AYH][h1] * p_y;
+                p_y = p_y / pm[PML_KYH][h1] + *ps;
+            }
+            real rip = AT(c->rip, j, i), rjp = AT(c->rjp, j, i);
+            AT(c->vxp1, j, i) = rip * p_x / DH;
+            AT(c->vyp1, j, i) = rjp * p_y / DH;
+            AT(c->vx, j, i) += DT * rip * p_x / DH;
+            AT(c->vy, j, i) += DT * rjp * p_y / DH;
+        }
+    }
+    for (int l = 0; l < c->nsrc; l++) {
+        int i = (int)c->srcpos[0 * c->nsrc + l], j = (int)c->srcpos[1 * c->nsrc + l];
+        real azi = (real)(c->srcpos[6 * c->nsrc + l] * PI_D / 180);
+        int type = (int)c->srcpos[7 * c->nsrc + l];
+        real s = c->signals[(size_t)l * P->nt + (nt - 1)];
+        c->qtype = type; /* update_v_PML_AC.c:646 sets the global QUELLTYP */
+        if (type == 2) AT(c->vx, j, i) += s;
+        if (type == 3) AT(c->vy, j, i) += s;
+        if (type == 4) {
+            AT(c->vx, j, i) = (real)(AT(c->vx, j, i) + -sin(azi) * s);
+            AT(c->vy, j, i) = (real)(AT(c->vy, j, i) + cos(azi) * s);
+        }
+    }
+}
+
+/* src/AC/update_s_acoustic_PML_AC.c:13-496 (FD8 block :249-316):
+ * p += g (vxx + vyy), g = rho vp^2 (INVMAT1 1) or the modulus (3) */
+static void update_p_ac(orc_ctx *c)
+{
+    const orc_params *P = &c->P;
+    const real dhi = (real)P->dt / (real)P->dh;
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= P->ny; j++) {
+        for (int i = 1; i <= P->nx; i++) {
+            real vxx, vyy, vxy, vyx;
+            stress_derivs(c, j, i, dhi, &vxx, &vyy, &vxy, &vyx);
+            real g = P->invmat1 == 3 ? AT(c->pi, j, i) : AT(c->rho, j, i) * AT(c->pi, j, i) * AT(c->pi, j, i);
+            AT(c->sxx, j, i) += g * (vxx + vyy);
+        }
+    }
+}
+
+/* src/AC/psource_AC.c:9-51 (explosive source on p) */
+static void psource_ac(orc_ctx *c, int nt)
+{
+    for (int l = 0; l < c->nsrc; l++) {
+        int i = (int)c->srcpos[0 * c->nsrc + l], j = (int)c->srcpos[1 * c->nsrc + l];
+        AT(c->sxx, j, i) += source_amp(c, l, nt);
+    }
+}
+
+/* src/AC/surface_acoustic_PML_AC.c:10-27: p = 0 at row 1, ghost rows mirrored */
+static void surface_ac(orc_ctx *c)
+{
+    for (int i = 1; i <= c->P.nx; i++) {
+        AT(c->sxx, 1, i) = 0;
+        for (int m = 1; m <= c->fdoh; m++) AT(c->sxx, 1 - m, i) = -AT(c->sxx, 1 + m, i);
+    }
+}
+
+/* src/AC/seismo_AC.c:8-108 (called from src/AC/ac.c:199-201 with u = pi):
+ * SEISMO 1 vx, vy; 2 p; 3 div, curl; 4 all five */
+static void seismo_ac(orc_ctx *c, int nt)
+{
+    const orc_params *P = &c->P;
+    const real dhi = (real)P->dt / (real)P->dh;
+    const real *hc = c->hc;
+    for (int itr = 0; itr < c->ntr; itr++) {
+        int i = c->recpos[itr], j = c->recpos[c->ntr + itr];
+        size_t o = (size_t)itr * P->nt + (nt - 1);
+        if (P->seismo == 3 || P->seismo == 4) {
+            real vxx = 0, vyy = 0, vyx = 0, vxy = 0;
+            for (int m = 1; m <= c->fdoh; m++) {
+                vxx += hc[m] * (AT(c->vx, j, i + m - 1) - AT(c->vx, j, i - m));
+                vyy += hc[m] * (AT(c->vy, j + m - 1, i) - AT(c->vy, j - m, i));
+                vyx += hc[m] * (AT(c->vy, j, i + m) - AT(c->vy, j, i - m + 1));
+                vxy += hc[m] * (AT(c->vx, j + m, i) - AT(c->vx, j - m + 1, i));
+            }
+            vxx *= dhi; vyy *= dhi; vyx *= dhi; vxy *= dhi;
+            if (P->invmat1 == 3) {
+                c->sec[3][o] = (real)((vxx + vyy) * sqrt(AT(c->pi, j, i)));
+                c->sec[4][o] = (real)((vxy - vyx) * sqrt(AT(c->pi, j, i)));
+            } else {
+                real muss = AT(c->pi, j, i) * AT(c->pi, j, i) * AT(c->rho, j, i);
+                real lamss = AT(c->pi, j, i) * AT(c->pi, j, i) * AT(c->rho, j, i);
+                c->sec[3][o] = (real)((vxx + vyy) * sqrt(lamss));
+                c->sec[4][o] = (real)((vxy - vyx) * sqrt(muss));
+            }
+        }
+        if (P->seismo == 1 || P->seismo == 4) {
+            c->sec[0][o] = AT(c->vx, j, i);
+            c->sec[1][o] = AT(c->vy, j, i);
+        }
+        if (P->seismo == 2 || P->seismo == 4) c->sec[2][o] = AT(c->sxx, j, i);
+    }
+}
+
+/* Model set-up of src/AC/FD_AC.c:184-207: readmod_AC (MFILE.vp, .rho),
+ * matcopy_acoustic_AC (one-rank periodic ghosts of rho and vp), av_rho */
+orc_ctx *orc_create_ac(const orc_params *p, const float *vp, const float *rho)
+{
+    if (!p || !vp || !rho) { seterr("null argument"); return NULL; }
+    if (p->L != 0 || p->mode != 0) { seterr("acoustic runs are forward modelling (L = 0, MODE 0)"); return NULL; }
+    orc_ctx *c = ctx_new(p);
+    if (!c) return NULL;
+    c->physics = 2;
+    const int NX = p->nx, NY = p->ny;
+    for (int i = 1; i <= NX; i++)
+        for (int j = 1; j <= NY; j++) {
+            size_t k = (size_t)(i - 1) * NY + (j - 1);
+            AT(c->pi, j, i) = vp[k];
+            AT(c->rho, j, i) = rho[k];
+        }
+    matcopy_periodic(c, c->rho);
+    matcopy_periodic(c, c->pi);
+    av_rho(c);
+    return c;
+}
+
 /* src/VTI/update_s_elastic_PML_VTI.c:13-637 (FD8 block :304-396): the
  * derivatives and CPML of the isotropic update, then
  * sxy += c44h (vyx + vxy), sxx += c11 vxx + c13 vyy, syy += c13 vxx + c33 vyy */
@@ -1256,6 +1418,14 @@ static void correlate(orc_ctx *c, int k)
 static void step_body(orc_ctx *c, int nt, int mode)
 {
     const orc_params *P = &c->P;
+    if (c->physics == 2) { /* src/AC/ac.c:96-205 */
+        update_v_ac(c, nt);
+        update_p_ac(c);
+        if (c->qtype == 1) psource_ac(c, nt);
+        if (P->free_surf) surface_ac(c);
+        if (P->seismo) seismo_ac(c, nt);
+        return;
+    }
     update_v(c, nt, mode == 1 ? 1 : 0);
     if (c->physics == 3) update_s_vti(c);
     else if (c->physics == 4) update_s_tti(c);

@@ -64,6 +64,9 @@ orc_ctx *orc_create(const orc_params *p, const float *vp, const float *vs,
  * as in the model file MFILE.theta).  All NX*NY x-major. */
 orc_ctx *orc_create_aniso(const orc_params *p, int physics, const float *rho, const float *c11,
                           const float *c13, const float *c33, const float *c44, const float *theta);
+/* Acoustic forward modelling (PHYSICS 2, L = 0, MODE 0): vp and rho, NX*NY
+ * x-major; the pressure is field ORC_SXX. */
+orc_ctx *orc_create_ac(const orc_params *p, const float *vp, const float *rho);
 void orc_destroy(orc_ctx *c);
 const char *orc_error(void);
 void orc_set_threads(int n);

@@ -42,6 +42,7 @@ def load(prec="f32"):
     sig = {
         "orc_create": ([C.POINTER(_OParams), fp, fp, fp, fp, fp], vp),
         "orc_create_aniso": ([C.POINTER(_OParams), C.c_int, fp, fp, fp, fp, fp, fp], vp),
+        "orc_create_ac": ([C.POINTER(_OParams), fp, fp], vp),
         "orc_destroy": ([vp], None),
         "orc_error": ([], C.c_char_p),
         "orc_set_threads": ([C.c_int], None),
@@ -110,7 +111,8 @@ class Oracle:
     """psv() restated on the CPU; prec 'f32' (reference FP32 path) or 'f64'."""
 
     def __init__(self, params, vp, vs, rho, taus=None, taup=None, prec="f32", aniso=None):
-        """aniso: None (isotropic vp, vs, rho) or (physics, c11, c13, c33,
+        """aniso: None (isotropic vp, vs, rho), "ac" (acoustic: vp and rho,
+        vs ignored; the pressure is field ORC_SXX) or (physics, c11, c13, c33,
         c44, theta) for VTI (physics 3, theta None) / TTI (physics 4, theta in
         degrees); then vp and vs are ignored."""
         self.lib = load(prec)
@@ -125,7 +127,10 @@ class Oracle:
             else:
                 setattr(cp, name, getattr(params, name))
         self._cp = cp
-        if aniso is None:
+        if aniso == "ac":
+            arrs = [_f32(a) for a in (vp, rho)]
+            self.h = self.lib.orc_create_ac(C.byref(cp), *[_p(a) for a in arrs])
+        elif aniso is None:
             arrs = [_f32(a) for a in (vp, vs, rho, taus, taup)]
             self.h = self.lib.orc_create(C.byref(cp), *[_p(a) for a in arrs])
         else:

@@ -290,10 +290,15 @@ def test_cli_refuses_what_is_not_built(tmp_path):
     r = _run_cli(str(tmp_path / "missing.inp"))
     assert r.returncode == 1 and "parameter file" in r.stderr
     src = os.path.join(GOLD, "DENISE_marm_OBC.inp")
-    text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 2")
+    text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 5")
+    (tmp_path / "sh.inp").write_text(text)
+    r = _run_cli(str(tmp_path / "sh.inp"))
+    assert r.returncode == 1 and "PHYSICS" in r.stderr
+    # the acoustic loop is built for forward modelling only
+    text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 2").replace("_(MODE) = 0", "_(MODE) = 1")
     (tmp_path / "ac.inp").write_text(text)
     r = _run_cli(str(tmp_path / "ac.inp"))
-    assert r.returncode == 1 and "PHYSICS" in r.stderr
+    assert r.returncode == 1 and "MODE" in r.stderr
     # the Marmousi example (filtered spike source, QUELLART 6) passes the
     # checks and stops at its source file, which the fixtures do not hold
     r = _run_cli(src)
