@@ -442,7 +442,7 @@ def run_config1(threads, shots=5):
     return out
 
 
-def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20):
+def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20, tiles=None):
     """One acoustic (physics 2), VTI (3) or TTI (4) shot on the 4000 x 1200
     grid of the headline workload (DH=5 m, FD8, DT=0.4 ms, CPML 20 + free
     surface / top switch, Ricker 15 Hz, 800 receivers); the acoustic shot on
@@ -467,7 +467,10 @@ def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20):
         ctx.set_model_aniso(rho, c11, c13, c33, c44, theta)
     ctx.set_sources(synth.source_table(nx * dh / 2, 2 * dh, dh, fc=fc), synth.ricker(nt, dt, fc)[None, :])
     ctx.set_receivers(synth.receiver_line(2 * dh, 25.0, 800, 2 * dh, dh))
-    if tune_steps > 0:
+    if tiles:
+        ctx.set_tile(*tiles[0], sweep=0)
+        ctx.set_tile(*tiles[1], sweep=1)
+    elif tune_steps > 0:
         ctx.autotune(tune_steps)
     ctx.forward()
     ctx.set_kernel_sampling(sample_every)
@@ -692,7 +695,7 @@ def main():
         print(json.dumps(out))
         return
     if args.aniso_only:
-        print(json.dumps({"aniso": run_aniso(args.aniso_only, nt=args.nt, tune_steps=args.tune_steps)}))
+        print(json.dumps({"aniso": run_aniso(args.aniso_only, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles)}))
         return
     if args.executable_only:
         print(json.dumps({"executable": run_executable_marmousi()}))

@@ -851,6 +851,8 @@ int dn_set_tile(dn_ctx *c, int sweep, int tbx, int by, int ty, int vw)
     t.ty = ty;
     t.vw = vw;
     if (!tile_supported(t)) return fail(DN_ERR_ARG, "tile shape not compiled into the library");
+    if (vw != 1 && c->p.physics == 2)
+        return fail(DN_ERR_ARG, "the acoustic sweeps are compiled for scalar columns (vw = 1) only");
     if (sweep != 0 && vw != 1 && aniso_planes(c->p.physics))
         return fail(DN_ERR_ARG, "the VTI/TTI stress sweep is compiled for scalar columns (vw = 1) only");
     HIP_OK(hipSetDevice(c->device));
@@ -886,6 +888,7 @@ int dn_autotune_tile(dn_ctx *c, int nsteps, int *vel_shape, int *str_shape)
     // the velocity sweep is timed beside the default stress shape
     const bool scalar_s = aniso_planes(c->p.physics) > 0;
     for (int k = 0; k < n && k < 32 && !rc; ++k) {
+        if (c->p.physics == 2 && shapes[k].vw != 1) continue;  // acoustic: scalar columns only
         const bool both = !(scalar_s && shapes[k].vw != 1);
         if (both)
             rc = dn_set_tile(c, -1, shapes[k].tbx, shapes[k].by, shapes[k].ty, shapes[k].vw);

@@ -1012,6 +1012,47 @@ static void update_v_ac(orc_ctx *c, int nt)
     }
 }
 
+/* the pressure sweep's derivatives, src/AC/update_s_acoustic_PML_AC.c:253-298:
+ * vxx and vyy only, CPML on psi_vxx and psi_vyy */
+static void pressure_derivs(orc_ctx *c, int j, int i, real dhi, real *pvxx, real *pvyy)
+{
+    const orc_params *P = &c->P;
+    const int NX = P->nx, NY = P->ny, FW = P->fw, fdoh = c->fdoh;
+    const real *hc = c->hc;
+    real *const *pm = c->pml;
+    real vxx = hc[1] * (AT(c->vx, j, i) - AT(c->vx, j, i - 1));
+    real vyy = hc[1] * (AT(c->vy, j, i) - AT(c->vy, j - 1, i));
+    for (int m = 2; m <= fdoh; m++) {
+        vxx = vxx + hc[m] * (AT(c->vx, j, i + m - 1) - AT(c->vx, j, i - m));
+        vyy = vyy + hc[m] * (AT(c->vy, j + m - 1, i) - AT(c->vy, j - m, i));
+    }
+    vxx = vxx * dhi;
+    vyy = vyy * dhi;
+    if (!P->boundary && i <= FW) {
+        real *a = &PSIX(c->psi_vxx, j, i);
+        *a = pm[PML_BX][i] * *a + pm[PML_AX][i] * vxx;
+        vxx = vxx / pm[PML_KX][i] + *a;
+    }
+    if (!P->boundary && i >= NX - FW + 1) {
+        int h1 = i - NX + 2 * FW;
+        real *a = &PSIX(c->psi_vxx, j, h1);
+        *a = pm[PML_BX][h1] * *a + pm[PML_AX][h1] * vxx;
+        vxx = vxx / pm[PML_KX][h1] + *a;
+    }
+    if (!P->free_surf && j <= FW) {
+        real *a = &PSIY(c->psi_vyy, j, i);
+        *a = pm[PML_BY][j] * *a + pm[PML_AY][j] * vyy;
+        vyy = vyy / pm[PML_KY][j] + *a;
+    }
+    if (j >= NY - FW + 1) {
+        int h1 = j - NY + 2 * FW;
+        real *a = &PSIY(c->psi_vyy, h1, i);
+        *a = pm[PML_BY][h1] * *a + pm[PML_AY][h1] * vyy;
+        vyy = vyy / pm[PML_KY][h1] + *a;
+    }
+    *pvxx = vxx; *pvyy = vyy;
+}
+
 /* src/AC/update_s_acoustic_PML_AC.c:13-496 (FD8 block :249-316):
  * p += g (vxx + vyy), g = rho vp^2 (INVMAT1 1) or the modulus (3) */
 static void update_p_ac(orc_ctx *c)
@@ -1021,8 +1062,8 @@ static void update_p_ac(orc_ctx *c)
 #pragma omp parallel for schedule(static)
     for (int j = 1; j <= P->ny; j++) {
         for (int i = 1; i <= P->nx; i++) {
-            real vxx, vyy, vxy, vyx;
-            stress_derivs(c, j, i, dhi, &vxx, &vyy, &vxy, &vyx);
+            real vxx, vyy;
+            pressure_derivs(c, j, i, dhi, &vxx, &vyy);
             real g = P->invmat1 == 3 ? AT(c->pi, j, i) : AT(c->rho, j, i) * AT(c->pi, j, i) * AT(c->pi, j, i);
             AT(c->sxx, j, i) += g * (vxx + vyy);
         }
