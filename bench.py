@@ -443,16 +443,18 @@ def run_config1(threads, shots=5):
 
 
 def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20, tiles=None):
-    """One acoustic (physics 2), VTI (3) or TTI (4) shot on the 4000 x 1200
-    grid of the headline workload (DH=5 m, FD8, DT=0.4 ms, CPML 20 + free
-    surface / top switch, Ricker 15 Hz, 800 receivers); the acoustic shot on
-    the seeded layered vp/rho model, VTI/TTI on a seeded layered model with
-    Thomsen epsilon/delta 0.1-0.3 / 0.05-0.15 (TTI: tilt 30 +- 10 degrees).
-    Algorithmic bytes per cell-update: acoustic velocity 28 B (p, rip, rjp,
-    vx, vy read; vx, vy written), pressure 20 B (vx, vy, p, modulus read; p
-    written); elastic velocity 36 B, stress VTI 48 B (5 wavefields + 4
-    stiffness planes read, 3 stresses written), TTI 64 B (8 stiffness
-    planes)."""
+    """One acoustic (physics 2), VTI (3), TTI (4) or SH (5) shot on the
+    4000 x 1200 grid of the headline workload (DH=5 m, FD8, DT=0.4 ms, CPML
+    20 + free surface / top switch, Ricker 15 Hz, 800 receivers); the
+    acoustic and SH shots on the seeded layered vp/vs/rho model, VTI/TTI on
+    a seeded layered model with Thomsen epsilon/delta 0.1-0.3 / 0.05-0.15
+    (TTI: tilt 30 +- 10 degrees).  Algorithmic bytes per cell-update:
+    acoustic velocity 28 B (p, rip, rjp, vx, vy read; vx, vy written),
+    pressure 20 B (vx, vy, p, modulus read; p written); SH velocity 20 B
+    (sxz, syz, 1/rho, vz read; vz written), stress 28 B (vz, sxz, syz, uip,
+    ujp read; sxz, syz written); elastic velocity 36 B, stress VTI 48 B (5
+    wavefields + 4 stiffness planes read, 3 stresses written), TTI 64 B (8
+    stiffness planes)."""
     import dnhip
     import synth
     nx, ny, dh, dt, fc = 4000, 1200, 5.0, 4.0e-4, 15.0
@@ -462,6 +464,9 @@ def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20, tiles=None):
     if physics == 2:
         vp, _, rho = synth.layered_model(nx, ny, dh, 11)
         ctx.set_model(vp, None, rho)
+    elif physics == 5:
+        _, vs, rho = synth.layered_model(nx, ny, dh, 11)
+        ctx.set_model(None, vs, rho)
     else:
         rho, c11, c13, c33, c44, theta = synth.anisotropic_model(nx, ny, dh, 11, tilt=30.0 if physics == 4 else None)
         ctx.set_model_aniso(rho, c11, c13, c33, c44, theta)
@@ -480,9 +485,9 @@ def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20, tiles=None):
     tm = ctx.timing()
     assert np.isfinite(ctx.seismogram(0)).all()
     cells = nx * ny
-    sb = {2: 20, 3: 48, 4: 64}[physics]
-    vb = 28 if physics == 2 else 36
-    name = {2: "acoustic", 3: "VTI elastic", 4: "TTI elastic"}[physics]
+    sb = {2: 20, 3: 48, 4: 64, 5: 28}[physics]
+    vb = {2: 28, 5: 20}.get(physics, 36)
+    name = {2: "acoustic", 3: "VTI elastic", 4: "TTI elastic", 5: "SH elastic"}[physics]
     out = {"workload": f"{name} 4000x1200, DH=5 m, FD8, NT={nt}",
            "gcells_per_s": cells * nt / wall / 1e9, "ms_per_shot": wall * 1e3,
            "tile": {"velocity": list(ctx.tile(0)), "stress": list(ctx.tile(1))},
@@ -660,9 +665,9 @@ def main():
     ap.add_argument("--grad-jobs", type=int, default=2, help="timed gradient jobs (0: skip)")
     ap.add_argument("--grad-only", action="store_true", help="profiling: only the gradient job")
     ap.add_argument("--executable-only", action="store_true", help="only the executable on the Marmousi example")
-    ap.add_argument("--no-aniso", action="store_true", help="skip the acoustic, VTI and TTI shots")
+    ap.add_argument("--no-aniso", action="store_true", help="skip the acoustic, VTI, TTI and SH shots")
     ap.add_argument("--aniso-only", type=int, default=0, metavar="PHYSICS",
-                    help="profiling: only the acoustic (2), VTI (3) or TTI (4) shot")
+                    help="profiling: only the acoustic (2), VTI (3), TTI (4) or SH (5) shot")
     ap.add_argument("--no-executable", action="store_true",
                     help="skip the drop-in executable on the shipped Marmousi example (100 shots, MODE 0)")
     ap.add_argument("--fwi-iters", type=int, default=2,
@@ -731,7 +736,8 @@ def main():
     if not args.no_aniso:
         aniso = {"acoustic": run_aniso(2, nt=args.nt, tune_steps=args.tune_steps),
                  "vti": run_aniso(3, nt=args.nt, tune_steps=args.tune_steps),
-                 "tti": run_aniso(4, nt=args.nt, tune_steps=args.tune_steps)}
+                 "tti": run_aniso(4, nt=args.nt, tune_steps=args.tune_steps),
+                 "sh": run_aniso(5, nt=args.nt, tune_steps=args.tune_steps)}
         print(f"# aniso: {aniso}", file=sys.stderr, flush=True)
     executable = None
     if not args.no_executable:

@@ -176,11 +176,15 @@ int build_lists(dn_ctx *c)
     };
     const int ns = (int)c->src_ix.size(), nr = (int)c->rec_ix.size();
     const int seismo = c->p.seismo;
+    // SH: QUELLTYP 1 is the force on vz (update_v_PML_SH.c:560-568); the SH
+    // loop has no explosive source
+    const bool sh = c->p.physics == 5;
     for (int l = 0; l < ns; ++l)
-        if (c->src_type[l] >= 2 && c->src_type[l] <= 4) add(0, c->src_ix[l], c->src_iy[l], E_FORCE, l);
+        if (sh ? c->src_type[l] == 1 : (c->src_type[l] >= 2 && c->src_type[l] <= 4))
+            add(0, c->src_ix[l], c->src_iy[l], E_FORCE, l);
     if (seismo == 1 || seismo == 4)
         for (int r = 0; r < nr; ++r) add(0, c->rec_ix[r], c->rec_iy[r], E_REC_V, r);
-    if (c->qtype == 1)
+    if (c->qtype == 1 && !sh)
         for (int l = 0; l < ns; ++l) add(1, c->src_ix[l], c->src_iy[l], E_EXPLOSIVE, l);
     if (seismo >= 2)
         for (int r = 0; r < nr; ++r) add(1, c->rec_ix[r], c->rec_iy[r], E_REC_S, r);
@@ -306,10 +310,12 @@ int validate(const dn_params *p)
     if (p->mode > 0 && p->dtinv < 1) return fail(DN_ERR_ARG, "DTINV must be >= 1");
     if (p->mode > 0 && p->grad_form != 1 && p->grad_form != 2) return fail(DN_ERR_ARG, "GRAD_FORM must be 1 or 2");
     if (!(p->dh > 0.0f) || !(p->dt > 0.0f)) return fail(DN_ERR_ARG, "DH and DT must be positive");
-    if (p->physics != 0 && p->physics != 1 && p->physics != 2 && p->physics != 3 && p->physics != 4)
-        return fail(DN_ERR_ARG, "PHYSICS must be 1 (P/SV), 2 (acoustic), 3 (VTI) or 4 (TTI)");
-    if ((p->physics == 2 || p->physics == 3 || p->physics == 4) && (p->L != 0 || p->mode != 0))
-        return fail(DN_ERR_ARG, "acoustic, VTI and TTI runs are forward modelling: L = 0 and MODE 0");
+    if (p->physics < 0 || p->physics > 5)
+        return fail(DN_ERR_ARG, "PHYSICS must be 1 (P/SV), 2 (acoustic), 3 (VTI), 4 (TTI) or 5 (SH)");
+    if (p->physics >= 2 && (p->L != 0 || p->mode != 0))
+        return fail(DN_ERR_ARG, "acoustic, VTI, TTI and SH runs are forward modelling: L = 0 and MODE 0");
+    // sh.c:133-135 samples vz into every section; outseis_SHfor.c writes SEISMO 1 only
+    if (p->physics == 5 && p->seismo > 1) return fail(DN_ERR_ARG, "SH runs record vz (SEISMO 0 or 1)");
     if (p->L > 0) {
         for (int l = 0; l < p->L; ++l)
             if (!(p->fl[l] > 0.0f)) return fail(DN_ERR_ARG, "relaxation frequencies FL must be positive");
@@ -638,10 +644,38 @@ static int set_model_ac(dn_ctx *c, const float *vp, const float *rho)
     return receiver_factors(c);
 }
 
+// SH model (params.physics 5): readmod_elastic_SH's vs (INVMAT1 1) or mu
+// (INVMAT1 3) and rho; checkfd_elast_SH's DT limit is the acoustic one with
+// the shear velocity (checkfd_elast_SH.c:30-75), then matcopy_elastic_SH,
+// av_mu_SH and inv_rho_SH (src/SH/FD_SH.c:175-199).
+static int set_model_sh(dn_ctx *c, const float *vs, const float *rho)
+{
+    if (!vs || !rho) return fail(DN_ERR_ARG, "null argument");
+    HIP_OK(hipSetDevice(c->device));
+    std::string err;
+    int rc = check_stability_ac(c->p, c->setup.hc, vs, rho, err);
+    if (rc) return fail(rc, err);
+    prepare_material_sh(c->p, c->lay, vs, rho, c->mat);
+    const size_t pl = c->lay.plane();
+    if ((rc = upload(c, c->rip, c->mat.rip.data(), pl)) || (rc = upload(c, c->ani[0], c->mat.ani[0].data(), pl)) ||
+        (rc = upload(c, c->ani[1], c->mat.ani[1].data(), pl)))
+        return rc;
+    HIP_OK(hipStreamSynchronize(c->stream));
+    for (auto *v : {&c->mat.rip, &c->mat.ani[0], &c->mat.ani[1]}) {
+        v->clear();
+        v->shrink_to_fit();
+    }
+    fill_args(c);
+    c->model_set = true;
+    c->fwd_complete = false;
+    return DN_OK;
+}
+
 int dn_set_model(dn_ctx *c, const float *vp, const float *vs, const float *rho, const float *taus,
                  const float *taup)
 {
     if (c && c->p.physics == 2) return set_model_ac(c, vp, rho);
+    if (c && c->p.physics == 5) return set_model_sh(c, vs, rho);
     if (!c || !vp || !vs || !rho) return fail(DN_ERR_ARG, "null argument");
     if (aniso_planes(c->p.physics)) return fail(DN_ERR_ARG, "VTI/TTI models are set with dn_set_model_aniso");
     if (c->p.L > 0 && (!taus || !taup)) return fail(DN_ERR_ARG, "viscoelastic model needs taus and taup");
@@ -851,8 +885,8 @@ int dn_set_tile(dn_ctx *c, int sweep, int tbx, int by, int ty, int vw)
     t.ty = ty;
     t.vw = vw;
     if (!tile_supported(t)) return fail(DN_ERR_ARG, "tile shape not compiled into the library");
-    if (vw != 1 && c->p.physics == 2)
-        return fail(DN_ERR_ARG, "the acoustic sweeps are compiled for scalar columns (vw = 1) only");
+    if (vw != 1 && scalar_only(c->p.physics))
+        return fail(DN_ERR_ARG, "the acoustic and SH sweeps are compiled for scalar columns (vw = 1) only");
     if (sweep != 0 && vw != 1 && aniso_planes(c->p.physics))
         return fail(DN_ERR_ARG, "the VTI/TTI stress sweep is compiled for scalar columns (vw = 1) only");
     HIP_OK(hipSetDevice(c->device));
@@ -888,7 +922,7 @@ int dn_autotune_tile(dn_ctx *c, int nsteps, int *vel_shape, int *str_shape)
     // the velocity sweep is timed beside the default stress shape
     const bool scalar_s = aniso_planes(c->p.physics) > 0;
     for (int k = 0; k < n && k < 32 && !rc; ++k) {
-        if (c->p.physics == 2 && shapes[k].vw != 1) continue;  // acoustic: scalar columns only
+        if (scalar_only(c->p.physics) && shapes[k].vw != 1) continue;  // acoustic / SH: scalar columns only
         const bool both = !(scalar_s && shapes[k].vw != 1);
         if (both)
             rc = dn_set_tile(c, -1, shapes[k].tbx, shapes[k].by, shapes[k].ty, shapes[k].vw);
@@ -1059,8 +1093,9 @@ int dn_run_forward(dn_ctx *c, int nsteps)
             HIP_OK(hipEventRecord(c->pool[nev + 3], c->stream));
             nev += 4;
         }
-        // the acoustic loop has no moment-tensor source (src/AC/ac.c:155-170)
-        if (c->qtype == 5 && c->acq.nsrc > 0 && c->a.physics != 2) launch_msource(c->a, c->acq, nt, c->stream);
+        // the acoustic and SH loops have no moment-tensor source (src/AC/ac.c:155-170, src/SH/sh.c:80-130)
+        if (c->qtype == 5 && c->acq.nsrc > 0 && c->a.physics != 2 && c->a.physics != 5)
+            launch_msource(c->a, c->acq, nt, c->stream);
         if (store && c->qtype == 5) launch_store(c->a, c->fwd, hin - 1, c->stream);
         if (store && c->eprecond) launch_energy(c->a, c->ew[0], c->stream);  // psv.c:300-303
         if (store) {

@@ -259,7 +259,7 @@ int dn_engine_attach(dn_engine *e, dn_ctx *c, int phase)
     if (c->device != e->device) return fail(DN_ERR_ARG, "context and engine are on different devices");
     if (!c->model_set) return fail(DN_ERR_STATE, "dn_set_model must precede dn_engine_attach");
     if (c->qtype == 5) return fail(DN_ERR_ARG, "moment-tensor shots run through dn_run_forward, not the engine");
-    if (c->a.physics != 1) return fail(DN_ERR_ARG, "acoustic and VTI/TTI shots run through dn_run_forward, not the engine");
+    if (c->a.physics != 1) return fail(DN_ERR_ARG, "acoustic, SH and VTI/TTI shots run through dn_run_forward, not the engine");
     for (const Job &j : e->jobs)
         if (j.ctx == c) return fail(DN_ERR_STATE, "context already attached");
     if (phase == 1 && (c->p.mode <= 0 || !c->fwd_complete))

@@ -64,9 +64,16 @@ struct HostMaterial {
     std::vector<float> ani[8];
 };
 
-// material planes of the acoustic (physics 2: the modulus g) and the
-// anisotropic runs (3 VTI, 4 TTI: the stiffness planes); 0 for isotropic P/SV
-inline int aniso_planes(int physics) { return physics == 2 ? 1 : (physics == 3 ? 4 : (physics == 4 ? 8 : 0)); }
+// material planes of the acoustic (physics 2: the modulus g), the
+// anisotropic (3 VTI, 4 TTI: the stiffness planes) and the SH runs (5: the
+// shear moduli uip, ujp); 0 for isotropic P/SV
+inline int aniso_planes(int physics)
+{
+    return physics == 2 ? 1 : (physics == 3 ? 4 : (physics == 4 ? 8 : (physics == 5 ? 2 : 0)));
+}
+
+// the physics whose sweeps have scalar columns only in both sweeps
+inline bool scalar_only(int physics) { return physics == 2 || physics == 5; }
 
 // set-up math (dn_setup.cpp)
 int holberg(int fdorder, int maxrelerror, float hc[7], std::string &err);
@@ -77,6 +84,7 @@ void prepare_material(const dn_params &p, const Layout &lay, const float *vp, co
                       const float *rho, const float *taus, const float *taup, HostMaterial &m);
 int check_stability_ac(const dn_params &p, const float hc[7], const float *vp, const float *rho, std::string &err);
 void prepare_material_ac(const dn_params &p, const Layout &lay, const float *vp, const float *rho, HostMaterial &m);
+void prepare_material_sh(const dn_params &p, const Layout &lay, const float *vs, const float *rho, HostMaterial &m);
 int check_stability_vti(const dn_params &p, const float hc[7], const float *rho, const float *c11,
                         const float *c13, const float *c33, const float *c44, std::string &err);
 void prepare_material_aniso(const dn_params &p, const Layout &lay, const float *rho, const float *c11,

@@ -81,14 +81,16 @@ struct PmlPoint {
     float d = 0.0f, K = 1.0f, al = 0.0f, a = 0.0f, b = 0.0f;
 };
 
-void pml_point(float absc, float thick, float d0, float npower, float kmax, float alpha_max,
+// sh: the SH profile of src/SH/PML_pro_SH.c:130-285 adds 0.1 alpha_max to
+// alpha inside the layer.
+void pml_point(float absc, float thick, float d0, float npower, float kmax, float alpha_max, bool sh,
                PmlPoint &p)
 {
     if (absc >= 0.0f) {
         float an = absc / thick;
         p.d = (float)(d0 * std::pow((double)an, (double)npower));
         p.K = (float)(1.0 + (kmax - 1.0) * std::pow((double)an, (double)npower));
-        p.al = (float)(alpha_max * (1.0 - an));
+        p.al = sh ? (float)(alpha_max * (1.0 - an) + 0.1 * alpha_max) : (float)(alpha_max * (1.0 - an));
     }
 }
 
@@ -100,9 +102,11 @@ void pml_finish(PmlPoint &p, float dt)
 
 }  // namespace
 
-// src/PML_pro.c:65-407 for a single whole-grid domain.
+// src/PML_pro.c:65-407 (SH: src/SH/PML_pro_SH.c:65-407) for a single
+// whole-grid domain.
 void pml_profiles(const dn_params &p, HostSetup &s)
 {
+    const bool sh = p.physics == 5;
     const int FW = p.fw;
     for (auto &v : s.pml) v.assign((size_t)2 * FW + 1, 0.0f);
     if (FW <= 0) return;
@@ -131,8 +135,8 @@ void pml_profiles(const dn_params &p, HostSetup &s)
         for (int i = 1; i <= FW; ++i) {
             PmlPoint g, hh;
             const float val = DH * (i - 1);
-            pml_point(o_near - val, thick, d0, npower, kmax, alpha_max, g);
-            pml_point((float)(o_near - (val + DH / 2.0)), thick, d0, npower, kmax, alpha_max, hh);
+            pml_point(o_near - val, thick, d0, npower, kmax, alpha_max, sh, g);
+            pml_point((float)(o_near - (val + DH / 2.0)), thick, d0, npower, kmax, alpha_max, sh, hh);
             if (x) {
                 if (g.al < 0.0f) g.al = 0.0f;
                 if (hh.al < 0.0f) hh.al = 0.0f;
@@ -145,8 +149,8 @@ void pml_profiles(const dn_params &p, HostSetup &s)
             const int h = i - N + 2 * FW;
             PmlPoint g, hh;
             const float val = DH * (i - 1);
-            pml_point(val - o_far, thick, d0, npower, kmax, alpha_max, g);
-            pml_point((float)(val + DH / 2.0 - o_far), thick, d0, npower, kmax, alpha_max, hh);
+            pml_point(val - o_far, thick, d0, npower, kmax, alpha_max, sh, g);
+            pml_point((float)(val + DH / 2.0 - o_far), thick, d0, npower, kmax, alpha_max, sh, hh);
             if (x) {
                 if (g.al < 0.0f) g.al = 0.0f;
                 if (hh.al < 0.0f) hh.al = 0.0f;
@@ -556,6 +560,40 @@ void prepare_material_ac(const dn_params &p, const Layout &lay, const float *vp,
             m.rjp[o] = rj;
             m.rip[o] = ri;
             m.ani[0][o] = p.invmat1 == 3 ? PI.at(j, i) : R.at(j, i) * PI.at(j, i) * PI.at(j, i);
+        }
+}
+
+// SH material (src/SH/FD_SH.c:175-199): the one-rank periodic ghosts of
+// matcopy_elastic_SH, av_mu_SH (src/SH/av_mu_SH.c:14-50: harmonic means of
+// the shear modulus at the sxz and syz points, in double; for INVMAT1 1 the
+// modulus rho vs^2 in float and 0 next to a fluid cell) and inv_rho_SH
+// (src/SH/inv_rho_SH.c:12-24: 1/rho at the vz points, 0 below 1e-4).
+void prepare_material_sh(const dn_params &p, const Layout &lay, const float *vs, const float *rho, HostMaterial &m)
+{
+    const int NX = p.nx, NY = p.ny;
+    const size_t plane = lay.plane();
+    auto dev = [&](int j, int i) { return lay.origin() + (size_t)lay.off(j, i); };
+    Ring R(NX, NY), U(NX, NY);
+    R.load(rho);
+    U.load(vs);
+    m.rip.assign(plane, 0.0f);
+    m.rjp.clear();
+    for (int k = 0; k < 8; ++k) m.ani[k].assign(k < 2 ? plane : 0, 0.0f);
+    const bool mod = p.invmat1 == 3;
+    auto modulus = [&](int j, int i) { return mod ? U.at(j, i) : R.at(j, i) * U.at(j, i) * U.at(j, i); };
+    for (int j = 1; j <= NY; ++j)
+        for (int i = 1; i <= NX; ++i) {
+            const size_t o = dev(j, i);
+            const float u1 = modulus(j, i), u2 = modulus(j, i + 1), u3 = modulus(j + 1, i);
+            float uip = (float)(2.0 / ((1.0 / u1) + (1.0 / u2)));
+            float ujp = (float)(2.0 / ((1.0 / u1) + (1.0 / u3)));
+            if (!mod && (u1 == 0.0f || u2 == 0.0f)) uip = 0.0f;
+            if (!mod && (u1 == 0.0f || u3 == 0.0f)) ujp = 0.0f;
+            m.ani[0][o] = uip;
+            m.ani[1][o] = ujp;
+            float ri = (float)(1.0 / R.at(j, i));
+            if (R.at(j, i) < 1e-4) ri = 0.0f;
+            m.rip[o] = ri;
         }
 }
 }  // namespace dn

@@ -105,14 +105,18 @@ static void *xmalloc(size_t n)
  * silently computing something else */
 static void check_supported(const dnio_par *p, const dnio_stage *st)
 {
-    if (p->physics < 1 || p->physics > 4)
-        die("PHYSICS", "2D P/SV (1), acoustic (2), VTI (3) and TTI (4) are built; SH (5) is not");
-    /* physics_AC.c / physics_VTI.c / physics_TTI.c: of their modes, forward
-     * modelling (MODE 0) is built */
+    if (p->physics < 1 || p->physics > 5)
+        die("PHYSICS", "2D P/SV (1), acoustic (2), VTI (3), TTI (4) and SH (5) are built");
+    /* physics_AC.c / physics_VTI.c / physics_TTI.c / physics_SH.c: of their
+     * modes, forward modelling (MODE 0) is built */
     if (p->physics != 1 && p->mode != 0)
-        die("MODE", "the acoustic, VTI and TTI loops are built for forward modelling (MODE 0)");
+        die("MODE", "the acoustic, VTI, TTI and SH loops are built for forward modelling (MODE 0)");
     if (p->physics != 1 && p->snap)
-        die("SNAP", "wavefield snapshots of the acoustic and VTI/TTI loops (snap_AC, snap_ssg_VTI) are not built");
+        die("SNAP", "wavefield snapshots of the acoustic, VTI/TTI and SH loops (snap_AC, snap_ssg_VTI, snap) are not "
+                    "built");
+    /* FD_SH.c:171-203: with L > 0 the SH set-up reads the viscoelastic model
+     * and prepares its coefficients (readmod_visc_SH, prepare_update_s_visc_SH) */
+    if (p->physics == 5 && p->L > 0) die("L", "the viscoelastic SH set-up is not built (SH runs with L = 0)");
     if (p->mode < 0 || p->mode > 2) die("MODE", "MODE 0 (forward), 1 (FWI) and 2 (RTM) are built");
     if (p->boundary != 0) die("BOUNDARY", "periodic boundaries are not supported");
     if (!p->srcrec) die("SRCREC", "sources must come from SOURCE_FILE (SRCREC = 1)");
@@ -622,11 +626,16 @@ static void write_shot_seis(run *r, int ishot)
     dnio_par *P = r->P;
     if (r->ntr <= 0 || !P->seismo) return;
     const float XS = r->srcpos[0 * r->nsrc + ishot - 1], YS = r->srcpos[1 * r->nsrc + ishot - 1];
-    /* saveseis_glob: SEISMO 1 vx,vy; 2 p; 3 div,curl; 4 vx,vy,p */
+    /* saveseis_glob: SEISMO 1 vx,vy; 2 p; 3 div,curl; 4 vx,vy,p;
+     * saveseis_glob_SH: SEISMO 1 vz to SEIS_FILE_VY, nothing else */
     int comps[5], nc = 0;
-    if (P->seismo == 1 || P->seismo == 4) comps[nc++] = 0, comps[nc++] = 1;
-    if (P->seismo == 2 || P->seismo == 4) comps[nc++] = 2;
-    if (P->seismo == 3) comps[nc++] = 3, comps[nc++] = 4;
+    if (P->physics == 5) {
+        if (P->seismo == 1) comps[nc++] = 1;
+    } else {
+        if (P->seismo == 1 || P->seismo == 4) comps[nc++] = 0, comps[nc++] = 1;
+        if (P->seismo == 2 || P->seismo == 4) comps[nc++] = 2;
+        if (P->seismo == 3) comps[nc++] = 3, comps[nc++] = 4;
+    }
     float *sec = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
     for (int c = 0; c < nc; c++) {
         dn_ok(dn_get_seismogram(r->ctx, comps[c], sec), "seismograms");
@@ -1135,7 +1144,7 @@ int main(int argc, char **argv)
     const int aniso = P.physics == 3 || P.physics == 4;
     float *vp = xmalloc(sizeof(float) * r.ncell), *vs = xmalloc(sizeof(float) * r.ncell);
     float *rho = xmalloc(sizeof(float) * r.ncell);
-    const int elastic_only = aniso || P.physics == 2;
+    const int elastic_only = aniso || P.physics == 2 || P.physics == 5;
     float *taus = P.L && !elastic_only ? xmalloc(sizeof(float) * r.ncell) : NULL;
     float *taup = P.L && !elastic_only ? xmalloc(sizeof(float) * r.ncell) : NULL;
     float *cani[4] = {NULL, NULL, NULL, NULL}, *theta = NULL;
@@ -1149,6 +1158,8 @@ int main(int argc, char **argv)
         /* vp (or the modulus for INVMAT1 3) and rho; the acoustic loop has
          * no relaxation mechanisms (ac.c) */
         io_ok(dnio_read_model_ac(&P, vp, rho), "model");
+    } else if (P.physics == 5) {
+        io_ok(dnio_read_model_sh(&P, vs, rho), "model"); /* vs (or mu) and rho */
     } else {
         io_ok(dnio_read_model(&P, vp, vs, rho, taus, taup), "model");
     }
@@ -1169,6 +1180,8 @@ int main(int argc, char **argv)
     dp.L = elastic_only ? 0 : P.L;
     dp.physics = P.physics;
     dp.seismo = P.mode >= 1 && P.seismo == 0 ? 1 : P.seismo;
+    /* outseis_SHfor.c writes SH seismograms for SEISMO 1 only */
+    if (P.physics == 5) dp.seismo = P.seismo == 1;
     dp.ndt = P.ndt;
     dp.invmat1 = P.invmat1;
     dp.grad_form = P.grad_form;

@@ -599,6 +599,30 @@ int dnio_read_model_ac(const dnio_par *p, float *vp, float *rho)
     return 0;
 }
 
+/* SH models (PHYSICS 5): readmod_elastic_SH.c:21-98 reads MFILE.vs (INVMAT1
+ * 1) or MFILE.mu (INVMAT1 3) and MFILE.rho in the same x-major order and,
+ * with WRITEMOD, writes MFILE.denise.mu and .denise.rho (merged file and the
+ * one-rank part .0.0). */
+int dnio_read_model_sh(const dnio_par *p, float *vs, float *rho)
+{
+    if (!p || !vs || !rho) return fail("null argument");
+    if (p->physics != 5) return fail("PHYSICS = %d: 5 (SH) expected", p->physics);
+    if (!p->readmod) return fail("READMOD = 0 (built-in model functions) is not supported; give model files");
+    if (p->invmat1 != 1 && p->invmat1 != 3) return fail("INVMAT1 = %d: 1 or 3 expected", p->invmat1);
+    const size_t n = (size_t)p->nx * p->ny;
+    if (read_model_file(p->mfile, p->invmat1 == 1 ? "vs" : "mu", vs, n) || read_model_file(p->mfile, "rho", rho, n))
+        return -1;
+    if (!p->writemod) return 0;
+    const char *ext[2] = {"denise.mu", "denise.rho"};
+    const float *arr[2] = {vs, rho};
+    char name[2 * DNIO_STR];
+    for (int k = 0; k < 2; k++) {
+        snprintf(name, sizeof name, "%s.%s", p->mfile, ext[k]);
+        if (dnio_write_model(name, arr[k], p->nx, p->ny)) return -1;
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------------
  * seismograms (src/outseis_glob.c:8-303, src/inseis.c:8-110,
  * include/segy.h:429-710)

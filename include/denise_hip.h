@@ -72,8 +72,9 @@ typedef struct dn_params {
     float ts;                /* TS = 1/max(fc) (src/sources.c:92)              */
     float fl[8];             /* FL[1..L]                                       */
     int physics;             /* PHYSICS: 1 (or 0) isotropic P/SV, 2 acoustic,
-                                3 VTI, 4 TTI (src/denise.c:116-141); 2, 3 and
-                                4 model forward runs (MODE 0, L = 0)          */
+                                3 VTI, 4 TTI, 5 SH (src/denise.c:116-141);
+                                2-5 model forward runs (MODE 0, L = 0; SH:
+                                SEISMO 0 or 1, vz recorded as component 1) */
 } dn_params;
 
 typedef struct dn_ctx dn_ctx;
@@ -106,7 +107,10 @@ int dn_ctx_destroy(dn_ctx *ctx);
  * above the stability limit.  Acoustic contexts (params.physics 2) take vp
  * (INVMAT1=3: the modulus) and rho only, vs/taus/taup ignored (may be
  * NULL): readmod_AC, matcopy_acoustic_AC, av_rho and checkfd_acoustic
- * (src/AC/FD_AC.c:184-207). */
+ * (src/AC/FD_AC.c:184-207).  SH contexts (params.physics 5) take vs
+ * (INVMAT1=3: mu) and rho only, vp/taus/taup ignored: readmod_elastic_SH,
+ * matcopy_elastic_SH, av_mu_SH, inv_rho_SH and checkfd_elast_SH
+ * (src/SH/FD_SH.c:171-199). */
 int dn_set_model(dn_ctx *ctx, const float *vp, const float *vs, const float *rho,
                  const float *taus, const float *taup);
 

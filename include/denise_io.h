@@ -155,6 +155,10 @@ int dnio_read_model_aniso(const dnio_par *p, float *rho, float *c11, float *c13,
  * and MFILE.rho, NX*NY x-major; writes MFILE.fdveps.pi and .fdveps.rho as
  * src/AC/readmod_AC.c:27-93 does. */
 int dnio_read_model_ac(const dnio_par *p, float *vp, float *rho);
+/* SH model files (PHYSICS 5): MFILE.vs (INVMAT1 1) or MFILE.mu (3) and
+ * MFILE.rho, NX*NY x-major; with WRITEMOD writes MFILE.denise.mu and
+ * .denise.rho as src/SH/readmod_elastic_SH.c:21-98 does. */
+int dnio_read_model_sh(const dnio_par *p, float *vs, float *rho);
 
 /* writemod + mergemod on one rank (src/writemod.c, src/mergemod.c): a
  * float32 x-major NX x NY file `name` and its per-rank copy `name.0.0`. */

@@ -188,6 +188,7 @@ struct orc_ctx {
     float resw[2];                      /* COMP_WEIGHT factors of vx / vy (0 = none set) */
     /* anisotropic P/SV (PHYSICS 3 VTI, 4 TTI; 1 = isotropic) */
     int physics;
+    arr uip, ujp; /* SH shear moduli at the sxz / syz points (av_mu_SH) */
     arr c11, c13, c33, c44, c44h, theta;
     arr d11, d13, d15, d15h, d33, d35, d35h, d55, d55h;
 };
@@ -203,7 +204,10 @@ static int pml_gate(real d)
     return abs(v) > 1.0e-6;
 }
 
-/* src/PML_pro.c:65-407 (single domain: NXG=NX, NYG=NY) */
+/* src/PML_pro.c:65-407 (single domain: NXG=NX, NYG=NY); the SH loop's
+ * src/SH/PML_pro_SH.c:65-407 adds 0.1 alpha_max to alpha inside the layer */
+#define PML_ALPHA(an) \
+    (c->physics == 5 ? (real)(alpha_max * (1.0 - (an)) + 0.1 * alpha_max) : (real)(alpha_max * (1.0 - (an))))
 static void pml_profiles(orc_ctx *c)
 {
     const orc_params *P = &c->P;
@@ -239,14 +243,14 @@ static void pml_profiles(orc_ctx *c)
             an = absc / thick_x;
             dx[i] = (real)(d0_x * pow(an, npower));
             Kx[i] = (real)(1.0 + (kmax - 1.0) * pow(an, npower));
-            alx[i] = (real)(alpha_max * (1.0 - an));
+            alx[i] = PML_ALPHA(an);
         }
         absc = (real)(xoL - (xval + DH / 2.0));
         if (absc >= 0.0) {
             an = absc / thick_x;
             dxh[i] = (real)(d0_x * pow(an, npower));
             Kxh[i] = (real)(1.0 + (kmax - 1.0) * pow(an, npower));
-            alxh[i] = (real)(alpha_max * (1.0 - an));
+            alxh[i] = PML_ALPHA(an);
         }
         if (alx[i] < 0.0) alx[i] = 0.0;
         if (alxh[i] < 0.0) alxh[i] = 0.0;
@@ -269,14 +273,14 @@ static void pml_profiles(orc_ctx *c)
             an = absc / thick_x;
             dx[h] = (real)(d0_x * pow(an, npower));
             Kx[h] = (real)(1.0 + (kmax - 1.0) * pow(an, npower));
-            alx[h] = (real)(alpha_max * (1.0 - an));
+            alx[h] = PML_ALPHA(an);
         }
         absc = (real)(xval + DH / 2.0 - xoR);
         if (absc >= 0.0) {
             an = absc / thick_x;
             dxh[h] = (real)(d0_x * pow(an, npower));
             Kxh[h] = (real)(1.0 + (kmax - 1.0) * pow(an, npower));
-            alxh[h] = (real)(alpha_max * (1.0 - an));
+            alxh[h] = PML_ALPHA(an);
         }
         if (alx[h] < 0.0) alx[h] = 0.0;
         if (alxh[h] < 0.0) alxh[h] = 0.0;
@@ -300,14 +304,14 @@ static void pml_profiles(orc_ctx *c)
             an = absc / thick_y;
             dy[i] = (real)(d0_y * pow(an, npower));
             Ky[i] = (real)(1.0 + (kmax - 1.0) * pow(an, npower));
-            aly[i] = (real)(alpha_max * (1.0 - an));
+            aly[i] = PML_ALPHA(an);
         }
         absc = (real)(yoT - (yval + DH / 2.0));
         if (absc >= 0.0) {
             an = absc / thick_y;
             dyh[i] = (real)(d0_y * pow(an, npower));
             Kyh[i] = (real)(1.0 + (kmax - 1.0) * pow(an, npower));
-            alyh[i] = (real)(alpha_max * (1.0 - an));
+            alyh[i] = PML_ALPHA(an);
         }
         by[i] = (real)exp((double)(-(dy[i] / Ky[i] + aly[i]) * DT));
         byh[i] = (real)exp((double)(-(dyh[i] / Kyh[i] + alyh[i]) * DT));
@@ -328,14 +332,14 @@ static void pml_profiles(orc_ctx *c)
             an = absc / thick_y;
             dy[h] = (real)(d0_y * pow(an, npower));
             Ky[h] = (real)(1.0 + (kmax - 1.0) * pow(an, npower));
-            aly[h] = (real)(alpha_max * (1.0 - an));
+            aly[h] = PML_ALPHA(an);
         }
         absc = (real)(yval + DH / 2.0 - yoB);
         if (absc >= 0.0) {
             an = absc / thick_y;
             dyh[h] = (real)(d0_x * pow(an, npower));
             Kyh[h] = (real)(1.0 + (kmax - 1.0) * pow(an, npower));
-            alyh[h] = (real)(alpha_max * (1.0 - an));
+            alyh[h] = PML_ALPHA(an);
         }
         by[h] = (real)exp((double)(-(dy[h] / Ky[h] + aly[h]) * DT));
         byh[h] = (real)exp((double)(-(dyh[h] / Kyh[h] + alyh[h]) * DT));
@@ -465,7 +469,7 @@ void orc_destroy(orc_ctx *c)
                   &c->sxy, &c->vxp1, &c->vyp1, &c->ux, &c->uy, &c->uxy, &c->wconv, &c->wconv_u,
                   &c->wconv_rho, &c->wlam, &c->wmu, &c->wrho_s, &c->gvp, &c->gvs, &c->grho,
                   &c->c11, &c->c13, &c->c33, &c->c44, &c->c44h, &c->theta, &c->d11, &c->d13,
-                  &c->d15, &c->d15h, &c->d33, &c->d35, &c->d35h, &c->d55, &c->d55h};
+                  &c->d15, &c->d15h, &c->d33, &c->d35, &c->d35h, &c->d55, &c->d55h, &c->uip, &c->ujp};
     for (size_t k = 0; k < sizeof all / sizeof all[0]; k++) free(all[k]->p);
     for (int l = 0; l < 9; l++) {
         free(c->d[l].p); free(c->e[l].p); free(c->dip[l].p);
@@ -1450,6 +1454,167 @@ static void correlate(orc_ctx *c, int k)
         }
 }
 
+/* ------------------------------------------------------------------------
+ * SH (PHYSICS 5, src/SH): vz is kept in the vx array, sxz in sxx, syz in
+ * syy; CPML memory psi_sxz_x in psi_sxx_x, psi_syz_y in psi_syy_y, psi_vzx
+ * in psi_vxx, psi_vzy in psi_vyy; 1/rho in rip, the shear moduli in uip, ujp.
+ * ---------------------------------------------------------------------- */
+
+/* src/SH/update_v_PML_SH.c:11-602 (FD8 block :294-378, forces :556-568):
+ * vz += DT rhoi (sxz_x + syz_y) / DH; QUELLTYP 1 adds the signal to vz */
+static void update_v_sh(orc_ctx *c, int nt)
+{
+    const orc_params *P = &c->P;
+    const int NX = P->nx, NY = P->ny, FW = P->fw, fdoh = c->fdoh;
+    const real DT = (real)P->dt, DH = (real)P->dh;
+    const real *hc = c->hc;
+    real *const *pm = c->pml;
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= NY; j++) {
+        for (int i = 1; i <= NX; i++) {
+            real sxz_x = hc[1] * (AT(c->sxx, j, i) - AT(c->sxx, j, i - 1));
+            real syz_y = hc[1] * (AT(c->syy, j, i) - AT(c->syy, j - 1, i));
+            for (int m = 2; m <= fdoh; m++) {
+                sxz_x = sxz_x + hc[m] * (AT(c->sxx, j, i + m - 1) - AT(c->sxx, j, i - m));
+                syz_y = syz_y + hc[m] * (AT(c->syy, j + m - 1, i) - AT(c->syy, j - m, i));
+            }
+            if (!P->boundary && i <= FW) {
+                real *ps = &PSIX(c->psi_sxx_x, j, i);
+                *ps = pm[PML_BX][i] * *ps + pm[PML_AX][i] * sxz_x;
+                sxz_x = sxz_x / pm[PML_KX][i] + *ps;
+            }
+            if (!P->boundary && i >= NX - FW + 1) {
+                int h1 = i - NX + 2 * FW;
+                real *ps = &PSIX(c->psi_sxx_x, j, h1);
+                *ps = pm[PML_BX][h1] * *ps + pm[PML_AX][h1] * sxz_x;
+                sxz_x = sxz_x / pm[PML_KX][h1] + *ps;
+            }
+            if (!P->free_surf && j <= FW) {
+                real *ps = &PSIY(c->psi_syy_y, j, i);
+                *ps = pm[PML_BY][j] * *ps + pm[PML_AY][j] * syz_y;
+                syz_y = syz_y / pm[PML_KY][j] + *ps;
+            }
+            if (j >= NY - FW + 1) {
+                int h1 = j - NY + 2 * FW;
+                real *ps = &PSIY(c->psi_syy_y, h1, i);
+                *ps = pm[PML_BY][h1] * *ps + pm[PML_AY][h1] * syz_y;
+                syz_y = syz_y / pm[PML_KY][h1] + *ps;
+            }
+            AT(c->vx, j, i) += DT * AT(c->rip, j, i) * (sxz_x + syz_y) / DH;
+        }
+    }
+    for (int l = 0; l < c->nsrc; l++) {
+        int i = (int)c->srcpos[0 * c->nsrc + l], j = (int)c->srcpos[1 * c->nsrc + l];
+        int type = (int)c->srcpos[7 * c->nsrc + l];
+        c->qtype = type; /* update_v_PML_SH.c:564 sets the global QUELLTYP */
+        if (type == 1) AT(c->vx, j, i) += c->signals[(size_t)l * P->nt + (nt - 1)];
+    }
+}
+
+/* src/SH/update_s_elastic_PML_SH.c:12-470 (FD8 block :237-306):
+ * sxz += uip vzx, syz += ujp vzy (ADJ_SIGN = 1 in forward runs) */
+static void update_s_sh(orc_ctx *c)
+{
+    const orc_params *P = &c->P;
+    const int NX = P->nx, NY = P->ny, FW = P->fw, fdoh = c->fdoh;
+    const real dhi = (real)P->dt / (real)P->dh;
+    const real *hc = c->hc;
+    real *const *pm = c->pml;
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j <= NY; j++) {
+        for (int i = 1; i <= NX; i++) {
+            real vzx = hc[1] * (AT(c->vx, j, i + 1) - AT(c->vx, j, i));
+            real vzy = hc[1] * (AT(c->vx, j + 1, i) - AT(c->vx, j, i));
+            for (int m = 2; m <= fdoh; m++) {
+                vzx = vzx + hc[m] * (AT(c->vx, j, i + m) - AT(c->vx, j, i - m + 1));
+                vzy = vzy + hc[m] * (AT(c->vx, j + m, i) - AT(c->vx, j - m + 1, i));
+            }
+            vzx = vzx * dhi;
+            vzy = vzy * dhi;
+            if (!P->boundary && i <= FW) {
+                real *ps = &PSIX(c->psi_vxx, j, i);
+                *ps = pm[PML_BXH][i] * *ps + pm[PML_AXH][i] * vzx;
+                vzx = vzx / pm[PML_KXH][i] + *ps;
+            }
+            if (!P->boundary && i >= NX - FW + 1) {
+                int h1 = i - NX + 2 * FW;
+                real *ps = &PSIX(c->psi_vxx, j, h1);
+                *ps = pm[PML_BXH][h1] * *ps + pm[PML_AXH][h1] * vzx;
+                vzx = vzx / pm[PML_KXH][h1] + *ps;
+            }
+            if (!P->free_surf && j <= FW) {
+                real *ps = &PSIY(c->psi_vyy, j, i);
+                *ps = pm[PML_BYH][j] * *ps + pm[PML_AYH][j] * vzy;
+                vzy = vzy / pm[PML_KYH][j] + *ps;
+            }
+            if (j >= NY - FW + 1) {
+                int h1 = j - NY + 2 * FW;
+                real *ps = &PSIY(c->psi_vyy, h1, i);
+                *ps = pm[PML_BYH][h1] * *ps + pm[PML_AYH][h1] * vzy;
+                vzy = vzy / pm[PML_KYH][h1] + *ps;
+            }
+            AT(c->sxx, j, i) += AT(c->uip, j, i) * vzx;
+            AT(c->syy, j, i) += AT(c->ujp, j, i) * vzy;
+        }
+    }
+}
+
+/* src/seismo_ssg.c:8-102 as src/SH/sh.c:133-135 calls it, every section and
+ * field argument vz: SEISMO 1 stores vz, which outseis_SHfor.c writes to
+ * SEIS_FILE_VY (component 1 here); the other modes write no file */
+static void seismo_sh(orc_ctx *c, int nt)
+{
+    if (c->P.seismo != 1) return;
+    for (int itr = 0; itr < c->ntr; itr++) {
+        int i = c->recpos[itr], j = c->recpos[c->ntr + itr];
+        c->sec[1][(size_t)itr * c->P.nt + (nt - 1)] = AT(c->vx, j, i);
+    }
+}
+
+/* Model set-up of src/SH/FD_SH.c:143-199: PML_pro_SH, readmod_elastic_SH
+ * (MFILE.vs and .rho; INVMAT1 3: .mu), matcopy_elastic_SH (one-rank
+ * periodic ghosts), av_mu_SH and inv_rho_SH */
+orc_ctx *orc_create_sh(const orc_params *p, const float *vs, const float *rho)
+{
+    if (!p || !vs || !rho) { seterr("null argument"); return NULL; }
+    if (p->L != 0 || p->mode != 0) { seterr("SH runs are forward modelling (L = 0, MODE 0)"); return NULL; }
+    if (p->seismo > 1) { seterr("SH runs record vz (SEISMO 0 or 1)"); return NULL; }
+    orc_ctx *c = ctx_new(p);
+    if (!c) return NULL;
+    c->physics = 5;
+    for (int k = 0; k < PML_N; k++) memset(c->pml[k], 0, sizeof(real) * ((size_t)c->fw2 + 2));
+    if (p->fw > 0) pml_profiles(c);
+    const int NX = p->nx, NY = p->ny, nd = c->nd;
+    c->uip = arr_new(-nd + 1, NY + nd, -nd + 1, NX + nd);
+    c->ujp = arr_new(-nd + 1, NY + nd, -nd + 1, NX + nd);
+    for (int i = 1; i <= NX; i++)
+        for (int j = 1; j <= NY; j++) {
+            size_t k = (size_t)(i - 1) * NY + (j - 1);
+            AT(c->u, j, i) = vs[k];
+            AT(c->rho, j, i) = rho[k];
+        }
+    matcopy_periodic(c, c->rho);
+    matcopy_periodic(c, c->u);
+    for (int j = 1; j <= NY; j++)
+        for (int i = 1; i <= NX; i++) {
+            real u1, u2, u3;
+            if (p->invmat1 == 3) {
+                u1 = AT(c->u, j, i); u2 = AT(c->u, j, i + 1); u3 = AT(c->u, j + 1, i);
+            } else {
+                u1 = AT(c->rho, j, i) * AT(c->u, j, i) * AT(c->u, j, i);
+                u2 = AT(c->rho, j, i + 1) * AT(c->u, j, i + 1) * AT(c->u, j, i + 1);
+                u3 = AT(c->rho, j + 1, i) * AT(c->u, j + 1, i) * AT(c->u, j + 1, i);
+            }
+            AT(c->uip, j, i) = (real)(2.0 / ((1.0 / u1) + (1.0 / u2)));
+            AT(c->ujp, j, i) = (real)(2.0 / ((1.0 / u1) + (1.0 / u3)));
+            if (p->invmat1 == 1 && (u1 == 0.0 || u2 == 0.0)) AT(c->uip, j, i) = 0.0;
+            if (p->invmat1 == 1 && (u1 == 0.0 || u3 == 0.0)) AT(c->ujp, j, i) = 0.0;
+            AT(c->rip, j, i) = (real)(1.0 / AT(c->rho, j, i));
+            if (AT(c->rho, j, i) < 1e-4) AT(c->rip, j, i) = 0.0;
+        }
+    return c;
+}
+
 /* One time step after the NaN probe: src/PSV/psv.c:104-222 (velocity
  * update, stress update, explosive / adjoint pressure / moment-tensor
  * sources, free surface, receivers).  The VTI and TTI loops
@@ -1459,6 +1624,12 @@ static void correlate(orc_ctx *c, int k)
 static void step_body(orc_ctx *c, int nt, int mode)
 {
     const orc_params *P = &c->P;
+    if (c->physics == 5) { /* src/SH/sh.c:80-135 (no free-surface routine) */
+        update_v_sh(c, nt);
+        update_s_sh(c);
+        if (P->seismo) seismo_sh(c, nt);
+        return;
+    }
     if (c->physics == 2) { /* src/AC/ac.c:96-205 */
         update_v_ac(c, nt);
         update_p_ac(c);
@@ -1498,7 +1669,8 @@ static int run_psv(orc_ctx *c, int mode, int nsteps)
     }
     c->nan_step = 0;
     for (int nt = 1; nt <= nsteps; nt++) {
-        if (isnan((double)AT(c->vy, P->ny / 2, P->nx / 2))) { /* psv.c:99-103 */
+        /* psv.c:99-103 (vy); sh.c:74-77 (vz, kept in vx) */
+        if (isnan((double)AT(c->physics == 5 ? c->vx : c->vy, P->ny / 2, P->nx / 2))) {
             c->nan_step = nt;
             seterr("Simulation is unstable at time step %d", nt);
             return -2;
@@ -1723,6 +1895,8 @@ int orc_get_field(const orc_ctx *c, int id, void *out)
     case ORC_D35: a = &c->d35; break;
     case ORC_D35H: a = &c->d35h; break;
     case ORC_D55H: a = &c->d55h; break;
+    case ORC_UIP: a = &c->uip; break;
+    case ORC_UJP: a = &c->ujp; break;
     default: seterr("bad field id %d", id); return -1;
     }
     memcpy(out, a->p, arr_len(*a) * sizeof(real));

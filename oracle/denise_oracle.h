@@ -54,6 +54,8 @@ enum {
      * d11, d13, d15, d15h, d33, d35, d35h, d55h */
     ORC_C11, ORC_C13, ORC_C33, ORC_C44H,
     ORC_D11, ORC_D13, ORC_D15, ORC_D15H, ORC_D33, ORC_D35, ORC_D35H, ORC_D55H,
+    /* SH shear moduli at the sxz / syz points */
+    ORC_UIP, ORC_UJP,
     ORC_NFIELD
 };
 
@@ -67,6 +69,10 @@ orc_ctx *orc_create_aniso(const orc_params *p, int physics, const float *rho, co
 /* Acoustic forward modelling (PHYSICS 2, L = 0, MODE 0): vp and rho, NX*NY
  * x-major; the pressure is field ORC_SXX. */
 orc_ctx *orc_create_ac(const orc_params *p, const float *vp, const float *rho);
+/* SH forward modelling (PHYSICS 5, L = 0, MODE 0, SEISMO 0 or 1): vs (INVMAT1
+ * 3: mu) and rho, NX*NY x-major; vz is field ORC_VX, sxz ORC_SXX, syz
+ * ORC_SYY, 1/rho ORC_RIP; vz is recorded as seismogram component 1. */
+orc_ctx *orc_create_sh(const orc_params *p, const float *vs, const float *rho);
 void orc_destroy(orc_ctx *c);
 const char *orc_error(void);
 void orc_set_threads(int n);

@@ -16,6 +16,7 @@ ORC_VX, ORC_VY, ORC_SXX, ORC_SYY, ORC_SXY, ORC_VXP1, ORC_VYP1, ORC_UX, ORC_UY, O
 ORC_RIP, ORC_RJP, ORC_UIPJP, ORC_PRHO, ORC_PPI, ORC_PU, ORC_FIPJP, ORC_F, ORC_G = range(10, 19)
 ORC_C11, ORC_C13, ORC_C33, ORC_C44H = range(19, 23)
 ORC_D11, ORC_D13, ORC_D15, ORC_D15H, ORC_D33, ORC_D35, ORC_D35H, ORC_D55H = range(23, 31)
+ORC_UIP, ORC_UJP = range(31, 33)
 
 
 class _OParams(C.Structure):
@@ -43,6 +44,7 @@ def load(prec="f32"):
         "orc_create": ([C.POINTER(_OParams), fp, fp, fp, fp, fp], vp),
         "orc_create_aniso": ([C.POINTER(_OParams), C.c_int, fp, fp, fp, fp, fp, fp], vp),
         "orc_create_ac": ([C.POINTER(_OParams), fp, fp], vp),
+        "orc_create_sh": ([C.POINTER(_OParams), fp, fp], vp),
         "orc_destroy": ([vp], None),
         "orc_error": ([], C.c_char_p),
         "orc_set_threads": ([C.c_int], None),
@@ -112,7 +114,8 @@ class Oracle:
 
     def __init__(self, params, vp, vs, rho, taus=None, taup=None, prec="f32", aniso=None):
         """aniso: None (isotropic vp, vs, rho), "ac" (acoustic: vp and rho,
-        vs ignored; the pressure is field ORC_SXX) or (physics, c11, c13, c33,
+        vs ignored; the pressure is field ORC_SXX), "sh" (SH: vs and rho, vp
+        ignored; vz is field ORC_VX, sxz ORC_SXX, syz ORC_SYY) or (physics, c11, c13, c33,
         c44, theta) for VTI (physics 3, theta None) / TTI (physics 4, theta in
         degrees); then vp and vs are ignored."""
         self.lib = load(prec)
@@ -130,6 +133,9 @@ class Oracle:
         if aniso == "ac":
             arrs = [_f32(a) for a in (vp, rho)]
             self.h = self.lib.orc_create_ac(C.byref(cp), *[_p(a) for a in arrs])
+        elif aniso == "sh":
+            arrs = [_f32(a) for a in (vs, rho)]
+            self.h = self.lib.orc_create_sh(C.byref(cp), *[_p(a) for a in arrs])
         elif aniso is None:
             arrs = [_f32(a) for a in (vp, vs, rho, taus, taup)]
             self.h = self.lib.orc_create(C.byref(cp), *[_p(a) for a in arrs])

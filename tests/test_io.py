@@ -290,10 +290,15 @@ def test_cli_refuses_what_is_not_built(tmp_path):
     r = _run_cli(str(tmp_path / "missing.inp"))
     assert r.returncode == 1 and "parameter file" in r.stderr
     src = os.path.join(GOLD, "DENISE_marm_OBC.inp")
-    text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 5")
-    (tmp_path / "sh.inp").write_text(text)
-    r = _run_cli(str(tmp_path / "sh.inp"))
+    text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 6")
+    (tmp_path / "p6.inp").write_text(text)
+    r = _run_cli(str(tmp_path / "p6.inp"))
     assert r.returncode == 1 and "PHYSICS" in r.stderr
+    # the viscoelastic SH set-up is not built
+    text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 5").replace("_(L) = 0", "_(L) = 2")
+    (tmp_path / "shv.inp").write_text(text)
+    r = _run_cli(str(tmp_path / "shv.inp"))
+    assert r.returncode == 1 and "SH" in r.stderr
     # the acoustic loop is built for forward modelling only
     text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 2").replace("_(MODE) = 0", "_(MODE) = 1")
     (tmp_path / "ac.inp").write_text(text)
