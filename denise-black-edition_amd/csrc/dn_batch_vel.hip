@@ -36,8 +36,9 @@ bool batch_tile_supported(const Tile &t, int sweep)
 #undef X
         return false;
     }
-    // stress-sweep shapes compiled in dn_batch_str.hip
-    return (t.tbx == 64 && t.by == 4 && t.ty == 1 && (t.vw == 1 || t.vw == 4));
+    // stress-sweep shapes compiled in dn_batch_str.hip (the group's L decides)
+    return (t.tbx == 64 && t.by == 4 && t.ty == 1 && (t.vw == 1 || t.vw == 4)) ||
+           (t.tbx == 64 && t.by == 4 && t.ty == 2 && (t.vw == 1 || t.vw == 2));
 }
 
 void launch_batch_vel(const BatchShot *shots, const BatchBlock *blk, int nblocks, int nfull, const int *clock,

@@ -269,8 +269,10 @@ int dn_engine_attach(dn_engine *e, dn_ctx *c, int phase)
     // tile shapes the multi-shot kernels are compiled for
     int rc;
     if (!batch_tile_supported(c->tile_v, 0) && (rc = dn_set_tile(c, 0, 128, 2, 4, 1))) return rc;
-    if ((!batch_tile_supported(c->tile_s, 1) || (c->p.L == 0 && c->tile_s.vw == 1)) &&
-        (rc = dn_set_tile(c, 1, 64, 4, 1, c->p.L == 0 ? 4 : 1)))
+    // (elastic groups default to 64,4,2,2: on the 64-shot mix 50.7 Gcell/s
+    // against 49.2 with 64,4,1,4 and 42.2 with 64,4,2,1)
+    if ((!batch_str_supported(c->tile_s, c->p.L) || (c->p.L == 0 && c->tile_s.ty == 1 && c->tile_s.vw == 1)) &&
+        (rc = c->p.L == 0 ? dn_set_tile(c, 1, 64, 4, 2, 2) : dn_set_tile(c, 1, 64, 4, 1, 1)))
         return rc;
     if ((rc = zero_state(c))) return rc;
     if (phase == 1) {

@@ -230,6 +230,8 @@ struct alignas(16) BatchBlock {
 // k: offset of this step from it (runs of steps are enqueued, or captured
 // in a graph, with k = 0, 1, ... and the clock moved on once at the end)
 bool batch_tile_supported(const Tile &t, int sweep);
+// stress-sweep shapes of the multi-shot kernels for a group with L relaxation mechanisms
+bool batch_str_supported(const Tile &t, int L);
 // the first nfull blocks of blk are full tiles, the rest lean tiles
 void launch_batch_vel(const BatchShot *shots, const BatchBlock *blk, int nblocks, int nfull, const int *clock,
                       int k, const Tile &t, int fdoh, int sw, void *stream);

@@ -209,6 +209,29 @@ def test_engine_lean_tiles_equal_single_shots():
         assert np.array_equal(g.gradient(w), single.gradient(w)), w
 
 
+@pytest.mark.parametrize("stress", [(64, 4, 1, 4), (64, 4, 2, 1), (64, 4, 2, 2)])
+def test_engine_elastic_stress_shapes(stress):
+    """Every stress shape the multi-shot kernels compile for elastic groups
+    (the context's shape is kept): shots equal their single-shot runs."""
+    cases = [make_case(nx=600, ny=200, nt=150, fdorder=8, L=0, seismo=4, seed=31, src_xz=(3000.0, 900.0)),
+             make_case(nx=333, ny=121, nt=170, fdorder=4, L=0, seismo=1, seed=32, stype=2)]
+    tile = ((128, 2, 4, 1), stress)
+    alone = []
+    for case in cases:
+        g = _ctx(case, tile)
+        g.forward()
+        alone.append(g)
+    eng = dnhip.Engine()
+    shots = [_ctx(case, tile) for case in cases]
+    for g in shots:
+        eng.attach(g)
+        assert g.tile(1) == stress
+    print(f"[engine stress {stress}]", flush=True)
+    eng.run(chunk=64)
+    for k, (case, a, b) in enumerate(zip(cases, alone, shots)):
+        _same(a, b, case, f"stress{k}")
+
+
 def test_engine_rejects_moment_tensor_and_double_attach():
     case = make_case(nx=64, ny=48, nt=50, stype=5)
     g = _ctx(case)
