@@ -312,8 +312,13 @@ int validate(const dn_params *p)
     if (!(p->dh > 0.0f) || !(p->dt > 0.0f)) return fail(DN_ERR_ARG, "DH and DT must be positive");
     if (p->physics < 0 || p->physics > 5)
         return fail(DN_ERR_ARG, "PHYSICS must be 1 (P/SV), 2 (acoustic), 3 (VTI), 4 (TTI) or 5 (SH)");
-    if (p->physics >= 2 && (p->L != 0 || p->mode != 0))
-        return fail(DN_ERR_ARG, "acoustic, VTI, TTI and SH runs are forward modelling: L = 0 and MODE 0");
+    if (p->physics >= 2 && p->L != 0) return fail(DN_ERR_ARG, "acoustic, VTI, TTI and SH runs are elastic: L = 0");
+    // the VTI / TTI loops also run the RTM passes (physics_VTI.c:27-29, physics_TTI.c)
+    if ((p->physics == 2 || p->physics == 5) && p->mode != 0)
+        return fail(DN_ERR_ARG, "acoustic and SH runs are forward modelling: MODE 0");
+    // TTI.c stores ux / uy / uxy for GRAD_FORM 2, which its stress update never writes
+    if (p->physics == 4 && p->mode > 0 && p->grad_form == 2)
+        return fail(DN_ERR_ARG, "TTI adjoint runs need GRAD_FORM 1 (the TTI update writes no ux/uy/uxy)");
     // sh.c:133-135 samples vz into every section; outseis_SHfor.c writes SEISMO 1 only
     if (p->physics == 5 && p->seismo > 1) return fail(DN_ERR_ARG, "SH runs record vz (SEISMO 0 or 1)");
     if (p->L > 0) {
@@ -1245,6 +1250,9 @@ int dn_assemble_gradient(dn_ctx *c)
 {
     if (!c) return fail(DN_ERR_ARG, "null context");
     if (c->p.mode <= 0) return fail(DN_ERR_STATE, "gradient assembly needs params.mode > 0");
+    if (aniso_planes(c->p.physics))
+        return fail(DN_ERR_STATE, "VTI/TTI images are the raw correlation sums (ass_gradVTI assembles nothing): "
+                                  "read them with dn_get_gradient 0..2");
     HIP_OK(hipSetDevice(c->device));
     launch_assemble(c->a, c->wc, c->wcu, c->wcr, c->gvp, c->gvs, c->grho, c->p.invmat1, c->stream);
     HIP_OK(hipGetLastError());

@@ -107,10 +107,17 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
 {
     if (p->physics < 1 || p->physics > 5)
         die("PHYSICS", "2D P/SV (1), acoustic (2), VTI (3), TTI (4) and SH (5) are built");
-    /* physics_AC.c / physics_VTI.c / physics_TTI.c / physics_SH.c: of their
-     * modes, forward modelling (MODE 0) is built */
-    if (p->physics != 1 && p->mode != 0)
-        die("MODE", "the acoustic, VTI, TTI and SH loops are built for forward modelling (MODE 0)");
+    /* physics_AC.c / physics_SH.c: of their modes, forward modelling (MODE
+     * 0) is built; physics_VTI.c / physics_TTI.c offer MODE 0 and RTM (MODE
+     * 2), their FWI branch is commented out */
+    if ((p->physics == 2 || p->physics == 5) && p->mode != 0)
+        die("MODE", "the acoustic and SH loops are built for forward modelling (MODE 0)");
+    if ((p->physics == 3 || p->physics == 4) && p->mode == 1)
+        die("MODE", "the VTI/TTI loops run forward modelling (MODE 0) and RTM (MODE 2); the reference has no "
+                    "VTI/TTI FWI");
+    /* TTI.c stores ux/uy/uxy for GRAD_FORM 2, which update_s_elastic_PML_TTI never writes */
+    if (p->physics == 4 && p->mode == 2 && p->grad_form == 2)
+        die("GRAD_FORM", "TTI RTM runs with GRAD_FORM 1 (the TTI update writes no ux/uy/uxy)");
     if (p->physics != 1 && p->snap)
         die("SNAP", "wavefield snapshots of the acoustic, VTI/TTI and SH loops (snap_AC, snap_ssg_VTI, snap) are not "
                     "built");
@@ -697,8 +704,14 @@ static void forward_pass(run *r)
 static void shot_gradient(run *r, int ishot, double l2, int test, float **gs, float *ws, float *wr, float **g_sum)
 {
     dnio_par *P = r->P;
-    dn_ok(dn_assemble_gradient(r->ctx), "gradient");
-    for (int k = 0; k < 3; k++) dn_ok(dn_get_gradient(r->ctx, 3 + k, gs[k]), "gradient");
+    if (P->physics == 3 || P->physics == 4) {
+        /* grad_obj_VTI.c:196-197 / grad_obj_TTI.c: ass_gradVTI is not called
+         * (and assembles nothing): the images are the raw correlation sums */
+        for (int k = 0; k < 3; k++) dn_ok(dn_get_gradient(r->ctx, k, gs[k]), "image");
+    } else {
+        dn_ok(dn_assemble_gradient(r->ctx), "gradient");
+        for (int k = 0; k < 3; k++) dn_ok(dn_get_gradient(r->ctx, 3 + k, gs[k]), "gradient");
+    }
     if (r->eprecond) {
         /* grad_obj_psv.c:219-238: energy weights of this shot */
         dn_ok(dn_get_energy(r->ctx, 0, ws), "energy");
@@ -1090,9 +1103,13 @@ static void rtm(run *r, const dnio_stage *st)
     r->twlength_minus = st->twlength_minus;
     r->twlength_plus = st->twlength_plus;
     r->offsetc = st->offsetc;
-    r->c_vp = dnfwi_model_mean(r->vp, P->nx, P->ny); /* RTM_PSV.c:316-352 */
-    r->c_vs = dnfwi_model_mean(r->vs, P->nx, P->ny);
-    r->c_rho = dnfwi_model_mean(r->rho, P->nx, P->ny);
+    if (P->physics == 3 || P->physics == 4) {
+        r->c_vp = r->c_vs = r->c_rho = 1.0f; /* RTM_VTI.c:309-311, RTM_TTI.c:318-320 */
+    } else {
+        r->c_vp = dnfwi_model_mean(r->vp, P->nx, P->ny); /* RTM_PSV.c:316-352 */
+        r->c_vs = dnfwi_model_mean(r->vs, P->nx, P->ny);
+        r->c_rho = dnfwi_model_mean(r->rho, P->nx, P->ny);
+    }
     r->time_filt = st->time_filt; /* RTM_PSV.c:283: FC = FC_END */
     r->order = st->order;
     r->fc = st->fc_end;

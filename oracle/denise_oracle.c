@@ -625,7 +625,9 @@ orc_ctx *orc_create_aniso(const orc_params *p, int physics, const float *rho, co
     if (!p || !rho || !c11 || !c13 || !c33 || !c44) { seterr("null argument"); return NULL; }
     if (physics != 3 && physics != 4) { seterr("physics must be 3 (VTI) or 4 (TTI)"); return NULL; }
     if (physics == 4 && !theta) { seterr("TTI needs theta"); return NULL; }
-    if (p->L != 0 || p->mode != 0) { seterr("anisotropic runs are elastic forward modelling (L = 0, MODE 0)"); return NULL; }
+    if (p->L != 0) { seterr("anisotropic runs are elastic (L = 0)"); return NULL; }
+    /* TTI.c stores ux/uy/uxy for GRAD_FORM 2, which update_s_elastic_PML_TTI never writes */
+    if (physics == 4 && p->mode > 0 && p->grad_form == 2) { seterr("TTI adjoint runs need GRAD_FORM 1"); return NULL; }
     orc_ctx *c = ctx_new(p);
     if (!c) return NULL;
     c->physics = physics;
@@ -1153,8 +1155,9 @@ orc_ctx *orc_create_ac(const orc_params *p, const float *vp, const float *rho)
 
 /* src/VTI/update_s_elastic_PML_VTI.c:13-637 (FD8 block :304-396): the
  * derivatives and CPML of the isotropic update, then
- * sxy += c44h (vyx + vxy), sxx += c11 vxx + c13 vyy, syy += c13 vxx + c33 vyy */
-static void update_s_vti(orc_ctx *c)
+ * sxy += c44h (vyx + vxy), sxx += c11 vxx + c13 vyy, syy += c13 vxx + c33 vyy;
+ * a forward run with GRAD_FORM 2 saves ux, uy, uxy (:384-389) */
+static void update_s_vti(orc_ctx *c, int mode)
 {
     const orc_params *P = &c->P;
     const real dhi = (real)P->dt / (real)P->dh;
@@ -1163,6 +1166,11 @@ static void update_s_vti(orc_ctx *c)
         for (int i = 1; i <= P->nx; i++) {
             real vxx, vyy, vxy, vyx;
             stress_derivs(c, j, i, dhi, &vxx, &vyy, &vxy, &vyx);
+            if (mode == 0 && P->grad_form == 2) {
+                AT(c->ux, j, i) = (AT(c->c11, j, i) * vxx + AT(c->c13, j, i) * vyy) / (real)P->dt;
+                AT(c->uy, j, i) = (AT(c->c13, j, i) * vxx + AT(c->c33, j, i) * vyy) / (real)P->dt;
+                AT(c->uxy, j, i) = AT(c->c44h, j, i) * (vyx + vxy) / (real)P->dt;
+            }
             AT(c->sxy, j, i) += AT(c->c44h, j, i) * (vyx + vxy);
             AT(c->sxx, j, i) += AT(c->c11, j, i) * vxx + AT(c->c13, j, i) * vyy;
             AT(c->syy, j, i) += AT(c->c13, j, i) * vxx + AT(c->c33, j, i) * vyy;
@@ -1428,6 +1436,8 @@ static void correlate(orc_ctx *c, int k)
             real axx = AT(c->sxx, j, i), ayy = AT(c->syy, j, i), axy = AT(c->sxy, j, i);
             AT(c->wconv_rho, j, i) += (AT(c->vxp1, j, i) * frx) + (AT(c->vyp1, j, i) * fry);
             AT(c->wconv, j, i) += (fx + fy) * (axx + ayy);
+            /* VTI.c:306-331 / TTI.c:304-329: the shear term is commented out */
+            if (c->physics == 3 || c->physics == 4) continue;
             real muss = 0, lamss = 0;
             if (P->invmat1 == 1) {
                 muss = AT(c->rho, j, i) * AT(c->u, j, i) * AT(c->u, j, i);
@@ -1639,7 +1649,7 @@ static void step_body(orc_ctx *c, int nt, int mode)
         return;
     }
     update_v(c, nt, mode == 1 ? 1 : 0);
-    if (c->physics == 3) update_s_vti(c);
+    if (c->physics == 3) update_s_vti(c, mode);
     else if (c->physics == 4) update_s_tti(c);
     else if (P->L) update_s_visc(c, mode);
     else update_s_elastic(c, mode);

@@ -262,9 +262,72 @@ def test_seismo_modes_of_the_anisotropic_loop():
         assert not o3.seismogram(comp).any(), comp
 
 
-def test_oracle_refuses_anisotropic_fwi():
-    p = params(mode=1)
+def test_oracle_refuses_tti_adjoint_with_grad_form_2():
+    """TTI.c stores ux/uy/uxy for GRAD_FORM 2, which the TTI stress update
+    never writes; the oracle refuses that combination (GRAD_FORM 1 runs)."""
+    p = params(mode=1, grad_form=2)
     nx, ny = p.nx, p.ny
-    with pytest.raises(ValueError, match="MODE 0"):
-        orc.Oracle(p, None, None, const(nx, ny, 2000.0), prec="f32",
-                   aniso=(3, const(nx, ny, C11), const(nx, ny, C13), const(nx, ny, C33), const(nx, ny, C44), None))
+    th = np.zeros((nx, ny), np.float32)
+    model = (4, const(nx, ny, C11), const(nx, ny, C13), const(nx, ny, C33), const(nx, ny, C44), th)
+    with pytest.raises(ValueError, match="GRAD_FORM"):
+        orc.Oracle(p, None, None, const(nx, ny, 2000.0), prec="f32", aniso=model)
+    orc.Oracle(params(mode=1, grad_form=1), None, None, const(nx, ny, 2000.0), prec="f32", aniso=model).close()
+
+
+def _adjoint(o, obs):
+    o.forward()
+    o.residuals(obs[0], obs[1], None, 1)
+    o.adjoint()
+    return [o.gradient(k) for k in range(3)]
+
+
+@pytest.mark.parametrize("grad_form", [1, 2])
+def test_vti_rtm_images_in_the_isotropic_limit(grad_form):
+    """The VTI loop's adjoint pass (src/VTI/VTI.c:235-368): with the
+    isotropic tensor its P image (sum of (fx + fy)(sxx + syy)) and density
+    image equal the isotropic P/SV run's raw correlations, and its shear
+    image stays 0 (the reference comments that term out); FP64, GRAD_FORM 1
+    (stored stresses) and 2 (stored ux, uy, uxy)."""
+    nx, ny, nt = 72, 56, 200
+    vp, vs, rho = synth.layered_model(nx, ny, 10.0, 9, sigma=3.0)
+    dt = float(np.float32(synth.stable_dt(float(vp.max()), 10.0, 8, safety=0.6)))
+    p = Params(nx=nx, ny=ny, nt=nt, dh=10.0, dt=dt, fdorder=8, fw=10, free_surf=0, seismo=1, ts=0.05,
+               mode=1, dtinv=2, grad_form=grad_form)
+    r, a, b = rho.astype(np.float64), vp.astype(np.float64), vs.astype(np.float64)
+    mu, lam = r * b * b, r * (a * a - 2 * b * b)
+    src = synth.source_table(10.0 * (nx // 2), 100.0, 10.0, fc=20.0)
+    sig = synth.ricker(nt, dt, 20.0, amp=1e3)[None, :]
+    rec = synth.receiver_line(30.0, 20.0, 30, 150.0, 10.0)
+    iso = orc.Oracle(p, vp, vs, rho, prec="f64")
+    iso.set_sources(src, sig)
+    iso.set_receivers(rec)
+    iso.forward()
+    obs = (0.8 * iso.seismogram(0), 1.1 * iso.seismogram(1))
+    g_iso = _adjoint(iso, obs)
+    ani = orc.Oracle(p, None, None, rho, prec="f64",
+                     aniso=(3, (lam + 2 * mu).astype(np.float32), lam.astype(np.float32),
+                            (lam + 2 * mu).astype(np.float32), mu.astype(np.float32), None))
+    ani.set_sources(src, sig)
+    ani.set_receivers(rec)
+    g_ani = _adjoint(ani, obs)
+    for k in (0, 2):
+        err = rel_l2(g_ani[k], g_iso[k])
+        assert err < 1e-6 and np.abs(g_iso[k]).max() > 0, (k, err)
+    assert not g_ani[1].any() and np.abs(g_iso[1]).max() > 0
+
+
+def test_tti_rtm_with_zero_tilt_is_the_vti_rtm():
+    p, m, src, sig, rec = _aniso_case(3, seismo=1)
+    p = Params(**{**p.__dict__, "mode": 1, "dtinv": 2})
+    nx, ny = p.nx, p.ny
+    v = _run(p, m, src, sig, rec, 3)
+    obs = (0.7 * v.seismogram(0), 0.7 * v.seismogram(1))
+    gv = _adjoint(v, obs)
+    t = orc.Oracle(p, None, None, m[0], prec="f32",
+                   aniso=(4, m[1], m[2], m[3], m[4], np.zeros((nx, ny), np.float32)))
+    t.set_sources(src, sig)
+    t.set_receivers(rec)
+    gt = _adjoint(t, obs)
+    for k in range(3):
+        assert np.array_equal(gv[k], gt[k]), k
+    assert np.abs(gv[0]).max() > 0 and np.abs(gv[2]).max() > 0

@@ -187,8 +187,11 @@ def test_errors_of_the_anisotropic_entry_points():
     with pytest.raises(dnhip.DnError, match="theta"):
         tti.set_model_aniso(rho, c11, c13, c33, c44)
     tti.close()
-    with pytest.raises(dnhip.DnError, match="MODE 0"):
-        dnhip.Context(dnhip.Params(**{**p.__dict__, "mode": 1}))
+    # TTI.c stores ux/uy/uxy for GRAD_FORM 2, which the TTI update never writes
+    with pytest.raises(dnhip.DnError, match="GRAD_FORM"):
+        dnhip.Context(dnhip.Params(**{**p.__dict__, "physics": 4, "mode": 1, "grad_form": 2}))
+    with pytest.raises(dnhip.DnError, match="L = 0"):
+        dnhip.Context(dnhip.Params(**{**p.__dict__, "L": 2}))
 
 
 @pytest.mark.parametrize("physics", [3, 4])
@@ -248,3 +251,109 @@ def test_cli_forward_writes_the_library_seismograms(tmp_path, physics):
             _, data = dnio.parse_su(os.path.join(tmp, f"{name}.shot{ishot}"))
             assert np.array_equal(data, g.seismogram(c)), (ishot, c)
     g.close()
+
+
+def _images(runner, obs):
+    runner.forward()
+    runner.residuals(obs[0], obs[1], obs[2], 1)
+    runner.adjoint()
+    return [runner.gradient(k) for k in range(3)]
+
+
+@pytest.mark.parametrize("physics,grad_form,dtinv,quelltypb",
+                         [(3, 1, 2, 1), (3, 2, 1, 1), (3, 1, 3, 4), (4, 1, 2, 1), (4, 1, 1, 4)])
+def test_rtm_images_equal_oracle(physics, grad_form, dtinv, quelltypb):
+    """The VTI/TTI adjoint pass (src/VTI/VTI.c:235-368, src/TTI/TTI.c):
+    forward with the snapshot store, residual adjoint sources (pressure
+    too for QUELLTYPB 4), adjoint with the density and P correlations; the
+    raw images equal the FP32 oracle's bit for bit and the shear image
+    stays 0."""
+    print(f"[aniso rtm] physics {physics} GRAD_FORM {grad_form} DTINV {dtinv} QUELLTYPB {quelltypb}", flush=True)
+    p, m, src, sig, rec = aniso_case(physics, nt=240, seismo=4)
+    p = dnhip.Params(**{**p.__dict__, "mode": 1, "grad_form": grad_form, "dtinv": dtinv, "quelltypb": quelltypb})
+    g = dnhip.Context(p)
+    g.set_model_aniso(*m)
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    o = orc.Oracle(p, None, None, m[0], prec="f32", aniso=(physics, m[1], m[2], m[3], m[4], m[5]))
+    o.set_sources(src, sig)
+    o.set_receivers(rec)
+    o.forward()
+    obs = (0.8 * o.seismogram(0), 0.9 * o.seismogram(1), 1.1 * o.seismogram(2))
+    gi = _images(g, obs)
+    oi = _images(o, obs)
+    for k in range(3):
+        assert np.array_equal(gi[k], oi[k]), (k, rel_l2(gi[k], oi[k]))
+    assert np.abs(gi[0]).max() > 0 and np.abs(gi[2]).max() > 0
+    assert not gi[1].any()
+    with pytest.raises(dnhip.DnError, match="raw correlation"):
+        g.assemble()
+    g.close()
+
+
+def test_cli_rtm_images_equal_the_library(tmp_path):
+    """MODE 2 with PHYSICS 3 (RTM_VTI.c): the executable's P and S images
+    are the stacked raw correlations of the library's adjoint passes with
+    the time-integrated observed data as adjoint sources (C_vp = C_vs =
+    C_rho = 1, no assembly), bit for bit; the S image is 0."""
+    import fwi_oracle as fo
+    tmp = str(tmp_path)
+    nx, ny, dh, nt = 100, 64, 10.0, 360
+    rho, c11, c13, c33, c44, _ = synth.anisotropic_model(nx, ny, dh, 23)
+    vmax = float(np.sqrt(np.maximum(c11, c33) / rho).max())
+    dt = float(np.float32(synth.stable_dt(vmax, dh, 8, safety=0.6)))
+    for d in ("model", "su", "obs", "jac"):
+        os.makedirs(os.path.join(tmp, d))
+    true = dict(c11=c11.copy(), c13=c13, c33=c33.copy(), c44=c44, rho=rho)
+    true["c11"][30:70, 30:34] *= 1.2
+    true["c33"][30:70, 30:34] *= 1.2
+    mig = dict(c11=c11, c13=c13, c33=c33, c44=c44, rho=rho)
+    for name, mod in (("true", true), ("mig", mig)):
+        for ext, a in mod.items():
+            np.ascontiguousarray(a, np.float32).tofile(os.path.join(tmp, f"model/{name}.{ext}"))
+    with open(os.path.join(tmp, "source.dat"), "w") as f:
+        f.write("2\n300.00 0.0 30.00 0.0 15.00 1.00 0.0 1\n650.00 0.0 30.00 0.0 15.00 1.00 0.0 1\n")
+    with open(os.path.join(tmp, "receiver.dat"), "w") as f:
+        for k in range(40):
+            f.write(f"{40.0 + 20.0 * k:.3f}\t20.000\n")
+    inpfile.write_inp(os.path.join(tmp, "true.inp"), PHYSICS=3, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/true", SEIS_FILE_VX="obs/d_x.su", SEIS_FILE_VY="obs/d_y.su")
+    r = subprocess.run([BIN, "true.inp"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    inpfile.write_inp(os.path.join(tmp, "rtm.inp"), PHYSICS=3, MODE=2, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/mig", DTINV=2, RTM_SHOT=0, TESTSHOT="1, 2, 1")
+    inpfile.write_workflow(os.path.join(tmp, "wf.inp"))
+    print(f"[aniso cli rtm] in {tmp}", flush=True)
+    r = subprocess.run([BIN, "rtm.inp", "wf.inp"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr
+    par = dnio.read_par(os.path.join(tmp, "rtm.inp"))
+    cwd = os.getcwd()
+    os.chdir(tmp)
+    try:
+        src, ts = dnio.read_sources(par.get("source_file"))
+        rec = dnio.read_receivers(par)
+    finally:
+        os.chdir(cwd)
+    p = dnhip.Params(nx=nx, ny=ny, nt=par.nt, dh=dh, dt=par.dt, fdorder=8, maxrelerror=par.maxrelerror, fw=par.fw,
+                     free_surf=par.free_surf, seismo=1, ts=ts, physics=3, mode=1, dtinv=2, grad_form=par.grad_form,
+                     damping=par.damping, fpml=par.fpml, npower=par.npower, k_max_pml=par.k_max_pml)
+    g = dnhip.Context(p)
+    g.set_model_aniso(rho, c11, c13, c33, c44)
+    total = [np.zeros((nx, ny), np.float32) for _ in range(2)]
+    for ishot in (1, 2):
+        loc = dnio.split_sources(par, src, ishot - 1, 1)
+        g.set_sources(loc, dnio.wavelet(par, loc, ishot, ts))
+        g.set_receivers(rec[:2])
+        g.forward()
+        obs = [dnio.read_su(os.path.join(tmp, f"obs/d_{c}.su.shot{ishot}"), rec.shape[1], par.nt) for c in "xy"]
+        diffs = [fo.rtm_adjoint_source(obs[c], g.seismogram(c), par.dt, par.grad_form, 1)[0] for c in (0, 1)]
+        g.set_adjoint_sources(diffs[0], diffs[1], None)
+        g.adjoint()
+        for k in range(2):
+            total[k] += g.gradient(k)
+    g.close()
+    for k, name in enumerate(("P", "S")):
+        got = np.fromfile(os.path.join(tmp, f"jac/j_{name}_image"), np.float32).reshape(nx, ny)
+        assert np.array_equal(got, total[k]), name
+    assert np.abs(total[0]).max() > 0 and not total[1].any()
