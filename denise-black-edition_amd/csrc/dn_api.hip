@@ -313,9 +313,9 @@ int validate(const dn_params *p)
     if (p->physics < 0 || p->physics > 5)
         return fail(DN_ERR_ARG, "PHYSICS must be 1 (P/SV), 2 (acoustic), 3 (VTI), 4 (TTI) or 5 (SH)");
     if (p->physics >= 2 && p->L != 0) return fail(DN_ERR_ARG, "acoustic, VTI, TTI and SH runs are elastic: L = 0");
-    // the VTI / TTI loops also run the RTM passes (physics_VTI.c:27-29, physics_TTI.c)
-    if ((p->physics == 2 || p->physics == 5) && p->mode != 0)
-        return fail(DN_ERR_ARG, "acoustic and SH runs are forward modelling: MODE 0");
+    // the acoustic and VTI / TTI loops also run the RTM passes (physics_AC.c,
+    // physics_VTI.c:27-29, physics_TTI.c); the SH loop forward modelling here
+    if (p->physics == 5 && p->mode != 0) return fail(DN_ERR_ARG, "SH runs are forward modelling: MODE 0");
     // TTI.c stores ux / uy / uxy for GRAD_FORM 2, which its stress update never writes
     if (p->physics == 4 && p->mode > 0 && p->grad_form == 2)
         return fail(DN_ERR_ARG, "TTI adjoint runs need GRAD_FORM 1 (the TTI update writes no ux/uy/uxy)");
@@ -633,8 +633,10 @@ static int set_model_ac(dn_ctx *c, const float *vp, const float *rho)
     if ((rc = upload(c, c->rip, c->mat.rip.data(), pl)) || (rc = upload(c, c->rjp, c->mat.rjp.data(), pl)) ||
         (rc = upload(c, c->ani[0], c->mat.ani[0].data(), pl)))
         return rc;
+    if (c->vp && ((rc = upload(c, c->vp, c->mat.vp.data(), pl)) || (rc = upload(c, c->rho, c->mat.rho.data(), pl))))
+        return rc;
     HIP_OK(hipStreamSynchronize(c->stream));
-    for (auto *v : {&c->mat.rip, &c->mat.rjp, &c->mat.ani[0]}) {
+    for (auto *v : {&c->mat.rip, &c->mat.rjp, &c->mat.ani[0], &c->mat.vp, &c->mat.rho}) {
         v->clear();
         v->shrink_to_fit();
     }
@@ -1250,11 +1252,11 @@ int dn_assemble_gradient(dn_ctx *c)
 {
     if (!c) return fail(DN_ERR_ARG, "null context");
     if (c->p.mode <= 0) return fail(DN_ERR_STATE, "gradient assembly needs params.mode > 0");
-    if (aniso_planes(c->p.physics))
+    if (c->p.physics == 3 || c->p.physics == 4)
         return fail(DN_ERR_STATE, "VTI/TTI images are the raw correlation sums (ass_gradVTI assembles nothing): "
                                   "read them with dn_get_gradient 0..2");
     HIP_OK(hipSetDevice(c->device));
-    launch_assemble(c->a, c->wc, c->wcu, c->wcr, c->gvp, c->gvs, c->grho, c->p.invmat1, c->stream);
+    launch_assemble(c->a, c->wc, c->wcu, c->wcr, c->gvp, c->gvs, c->grho, c->p.invmat1, c->a.physics, c->stream);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(c->stream));
     return DN_OK;

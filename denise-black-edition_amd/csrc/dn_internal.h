@@ -249,7 +249,7 @@ void launch_msource(const KArgs &a, const DevAcq &q, int nt, void *stream);
 void launch_energy(const KArgs &a, float *W, void *stream);
 void launch_store(const KArgs &a, float *const fwd[5], long k, void *stream);
 void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const float *wcr, float *gvp,
-                     float *gvs, float *grho, int invmat1, void *stream);
+                     float *gvs, float *grho, int invmat1, int physics, void *stream);
 void launch_residual(const DevAcq &q, int comp, const float *obs, float dt, float w, void *stream);
 void launch_l2(const DevAcq &q, int comp, void *stream);
 int l2_scratch_doubles();  // doubles behind DevAcq::l2

@@ -76,10 +76,11 @@ __global__ __launch_bounds__(256) void k_energy(KArgs a, float *W)
     W[t] = W[t] + ((vx * vx) + (vy * vy));
 }
 
-// src/PSV/ass_gradPSV.c:10-155 (gradients stored x-major like the files)
+// src/PSV/ass_gradPSV.c:10-155, for acoustic contexts src/AC/ass_gradAC.c:12-93
+// (gradients stored x-major like the files)
 __global__ __launch_bounds__(256) void k_assemble(KArgs a, const float *wc, const float *wcu,
                                                   const float *wcr, float *gvp, float *gvs, float *grho,
-                                                  int invmat1)
+                                                  int invmat1, int physics)
 {
     const long n = (long)a.nx * a.ny;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -93,7 +94,18 @@ __global__ __launch_bounds__(256) void k_assemble(KArgs a, const float *wc, cons
     float vpg, vsg, rhog;
     const float mu = -DT * wcu[g];
     const float rho_s = -DT * wcr[g];
-    if (invmat1 == 3) {
+    if (physics == 2) {
+        const float lamss = invmat1 == 3 ? ppi : prho * ppi * ppi;
+        if (lamss > 0.0f) {
+            if (a.grad_form == 1)
+                lam = (float)((1.0 / (4.0 * (double)lamss * (double)lamss)) * (double)lam);
+            else
+                lam = (float)(-(1.0 / (double)(lamss * lamss)) * (double)lam);
+        }
+        vpg = invmat1 == 3 ? lam : (float)(2.0 * (double)ppi * (double)prho * (double)lam);
+        vsg = 0.0f;
+        rhog = invmat1 == 1 ? ((ppi * ppi) * lam) + rho_s : rho_s;
+    } else if (invmat1 == 3) {
         const float muss = pu, lamss = ppi;
         if (a.grad_form == 1 && (muss > 0.0f || lamss > 0.0f)) {
             const double lm = (double)(lamss + muss);
@@ -295,12 +307,12 @@ void launch_energy(const KArgs &a, float *W, void *stream)
 }
 
 void launch_assemble(const KArgs &a, const float *wc, const float *wcu, const float *wcr, float *gvp,
-                     float *gvs, float *grho, int invmat1, void *stream)
+                     float *gvs, float *grho, int invmat1, int physics, void *stream)
 {
     const long n = (long)a.nx * a.ny;
     dim3 g((unsigned)((n + 255) / 256));
     hipLaunchKernelGGL(k_assemble, g, dim3(256), 0, (hipStream_t)stream, a, wc, wcu, wcr, gvp, gvs, grho,
-                       invmat1);
+                       invmat1, physics);
 }
 
 void launch_residual(const DevAcq &q, int comp, const float *obs, float dt, float w, void *stream)

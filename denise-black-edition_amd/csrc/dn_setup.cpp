@@ -550,9 +550,16 @@ void prepare_material_ac(const dn_params &p, const Layout &lay, const float *vp,
     m.rip.assign(plane, 0.0f);
     m.rjp.assign(plane, 0.0f);
     for (int k = 0; k < 8; ++k) m.ani[k].assign(k < 1 ? plane : 0, 0.0f);
+    // the model itself for the gradient assembly (ass_gradAC.c reads ppi, prho)
+    m.vp.assign(p.mode > 0 ? plane : 0, 0.0f);
+    m.rho.assign(p.mode > 0 ? plane : 0, 0.0f);
     for (int j = 1; j <= NY; ++j)
         for (int i = 1; i <= NX; ++i) {
             const size_t o = dev(j, i);
+            if (p.mode > 0) {
+                m.vp[o] = PI.at(j, i);
+                m.rho[o] = R.at(j, i);
+            }
             float rj = (float)(1.0 / (0.5 * (R.at(j, i) + R.at(j + 1, i))));
             float ri = (float)(1.0 / (0.5 * (R.at(j, i) + R.at(j, i + 1))));
             if (R.at(j, i) < 1e-4 && R.at(j + 1, i) < 1e-4) rj = 0.0f;

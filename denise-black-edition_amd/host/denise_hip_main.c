@@ -107,11 +107,13 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
 {
     if (p->physics < 1 || p->physics > 5)
         die("PHYSICS", "2D P/SV (1), acoustic (2), VTI (3), TTI (4) and SH (5) are built");
-    /* physics_AC.c / physics_SH.c: of their modes, forward modelling (MODE
-     * 0) is built; physics_VTI.c / physics_TTI.c offer MODE 0 and RTM (MODE
-     * 2), their FWI branch is commented out */
-    if ((p->physics == 2 || p->physics == 5) && p->mode != 0)
-        die("MODE", "the acoustic and SH loops are built for forward modelling (MODE 0)");
+    /* physics_SH.c: of its modes, forward modelling (MODE 0) is built;
+     * physics_AC.c: MODE 0 and RTM (MODE 2), its FWI (FWI_AC.c) is not;
+     * physics_VTI.c / physics_TTI.c offer MODE 0 and RTM, their FWI branch
+     * is commented out */
+    if (p->physics == 5 && p->mode != 0) die("MODE", "the SH loop is built for forward modelling (MODE 0)");
+    if (p->physics == 2 && p->mode == 1)
+        die("MODE", "the acoustic loop runs forward modelling (MODE 0) and RTM (MODE 2); acoustic FWI is not built");
     if ((p->physics == 3 || p->physics == 4) && p->mode == 1)
         die("MODE", "the VTI/TTI loops run forward modelling (MODE 0) and RTM (MODE 2); the reference has no "
                     "VTI/TTI FWI");
@@ -732,7 +734,8 @@ static void shot_gradient(run *r, int ishot, double l2, int test, float **gs, fl
     }
     for (int k = 0; k < 3; k++) {
         for (size_t q = 0; q < r->ncell; q++) g_sum[k][q] += gs[k][q];
-        if (P->rtm_shot && k < 2) { /* grad_obj_psv.c:263-266 */
+        /* grad_obj_psv.c:263-266; RTM_AC_out_shot.c writes the P image only */
+        if (P->rtm_shot && k < (P->physics == 2 ? 1 : 2)) {
             snprintf(r->name, sizeof r->name, "%s_%s_image_shot_%i", P->jacobian, k == 0 ? "P" : "S", ishot);
             write_plane(r->name, gs[k], P->nx, P->ny);
         }
@@ -816,6 +819,7 @@ static void precondition(run *r, float **g)
     if (!P->sws_taper_grad_vert && !P->sws_taper_grad_hor && !P->sws_taper_grad_sources && !P->sws_taper_file) return;
     float *coeff = xmalloc(sizeof(float) * r->ncell);
     for (int k = 0; k < 3; k++) {
+        if (k == 1 && P->physics == 2) continue; /* precond_AC.c: no shear-modulus gradient */
         if (P->sws_taper_grad_vert) {
             io_ok(dnfwi_taper_vert(P->nx, P->ny, P->dh, P->gradt, r->nsrc, r->srcpos, r->ntr, r->recpos, coeff, g[k]),
                   "vertical gradient taper");
@@ -1121,7 +1125,7 @@ static void rtm(run *r, const dnio_stage *st)
     printf(" L2 of the data (test shots): %.9e\n", l2);
     if (P->rtm_shot == 0) {
         const char *img[2] = {"P", "S"};
-        for (int k = 0; k < 2; k++) {
+        for (int k = 0; k < (P->physics == 2 ? 1 : 2); k++) { /* RTM_AC_out.c: the P image only */
             snprintf(r->name, sizeof r->name, "%s_%s_image", P->jacobian, img[k]);
             write_plane(r->name, g[k], P->nx, P->ny);
             printf(" wrote %s\n", r->name);

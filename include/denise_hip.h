@@ -73,11 +73,11 @@ typedef struct dn_params {
     float fl[8];             /* FL[1..L]                                       */
     int physics;             /* PHYSICS: 1 (or 0) isotropic P/SV, 2 acoustic,
                                 3 VTI, 4 TTI, 5 SH (src/denise.c:116-141);
-                                2-5 with L = 0; 2 and 5 forward runs (MODE 0;
-                                SH: SEISMO 0 or 1, vz recorded as component
-                                1); 3 and 4 also adjoint runs (mode 1, the
-                                RTM passes of RTM_VTI.c / RTM_TTI.c; TTI with
-                                GRAD_FORM 1)                                */
+                                2-5 with L = 0; 5 forward runs (MODE 0,
+                                SEISMO 0 or 1, vz recorded as component 1);
+                                2, 3 and 4 also adjoint runs (mode 1, the
+                                RTM passes of RTM_AC.c, RTM_VTI.c,
+                                RTM_TTI.c; TTI with GRAD_FORM 1)            */
 } dn_params;
 
 typedef struct dn_ctx dn_ctx;
@@ -192,9 +192,11 @@ int dn_get_adjoint_sources(dn_ctx *ctx, int comp, float *out);
  * params.mode > 0 over the full NT. */
 int dn_run_adjoint(dn_ctx *ctx, int nsteps);
 
-/* ass_gradPSV (src/PSV/ass_gradPSV.c:10-155), iteration 1 with INV_*_ITER=0.
- * VTI/TTI contexts refuse it (DN_ERR_STATE): their images are the raw
- * correlation sums (grad_obj_VTI.c:196-197 does not assemble). */
+/* ass_gradPSV (src/PSV/ass_gradPSV.c:10-155), for acoustic contexts
+ * ass_gradAC (src/AC/ass_gradAC.c:12-93: vp or lambda and rho, no shear
+ * term), iteration 1 with INV_*_ITER=0.  VTI/TTI contexts refuse it
+ * (DN_ERR_STATE): their images are the raw correlation sums
+ * (grad_obj_VTI.c:196-197 does not assemble). */
 int dn_assemble_gradient(dn_ctx *ctx);
 
 /* which: 0..2 raw correlation sums (waveconv_shot, waveconv_u_shot,

@@ -956,8 +956,9 @@ static void update_s_visc(orc_ctx *c, int mode)
 
 static real source_amp(const orc_ctx *c, int l, int nt);
 
-/* src/AC/update_v_PML_AC.c:11-676 (FD8 block :328-420; forces :646-656) */
-static void update_v_ac(orc_ctx *c, int nt)
+/* src/AC/update_v_PML_AC.c:11-676 (FD8 block :328-420; forces :640-656,
+ * adjoint forces :660-672): sw 0 forward, 1 adjoint */
+static void update_v_ac(orc_ctx *c, int nt, int sw)
 {
     const orc_params *P = &c->P;
     const int NX = P->nx, NY = P->ny, FW = P->fw, fdoh = c->fdoh;
@@ -997,11 +998,31 @@ static void update_v_ac(orc_ctx *c, int nt)
                 p_y = p_y / pm[PML_KYH][h1] + *ps;
             }
             real rip = AT(c->rip, j, i), rjp = AT(c->rjp, j, i);
-            AT(c->vxp1, j, i) = rip * p_x / DH;
-            AT(c->vyp1, j, i) = rjp * p_y / DH;
+            if (sw == 0) { /* :390-411, both GRAD_FORMs */
+                AT(c->vxp1, j, i) = rip * p_x / DH;
+                AT(c->vyp1, j, i) = rjp * p_y / DH;
+            } else if (P->grad_form == 1) {
+                AT(c->vxp1, j, i) += DT * AT(c->vx, j, i);
+                AT(c->vyp1, j, i) += DT * AT(c->vy, j, i);
+            } else {
+                AT(c->vxp1, j, i) = AT(c->vx, j, i);
+                AT(c->vyp1, j, i) = AT(c->vy, j, i);
+            }
             AT(c->vx, j, i) += DT * rip * p_x / DH;
             AT(c->vy, j, i) += DT * rjp * p_y / DH;
         }
+    }
+    if (sw == 1) {
+        const int QB = P->quelltypb;
+        for (int l = 0; l < c->ntr; l++) {
+            int i = c->recpos[l], j = c->recpos[c->ntr + l];
+            real sx = c->adj[0][(size_t)l * P->nt + (nt - 1)];
+            real sy = c->adj[1][(size_t)l * P->nt + (nt - 1)];
+            if (QB == 1) { AT(c->vx, j, i) += sx; AT(c->vy, j, i) += sy; }
+            if (QB == 2 || QB == 6 || QB == 7) AT(c->vy, j, i) += sy;
+            if (QB == 3 || QB == 5 || QB == 7) AT(c->vx, j, i) += sx;
+        }
+        return;
     }
     for (int l = 0; l < c->nsrc; l++) {
         int i = (int)c->srcpos[0 * c->nsrc + l], j = (int)c->srcpos[1 * c->nsrc + l];
@@ -1060,8 +1081,9 @@ static void pressure_derivs(orc_ctx *c, int j, int i, real dhi, real *pvxx, real
 }
 
 /* src/AC/update_s_acoustic_PML_AC.c:13-496 (FD8 block :249-316):
- * p += g (vxx + vyy), g = rho vp^2 (INVMAT1 1) or the modulus (3) */
-static void update_p_ac(orc_ctx *c)
+ * p += g (vxx + vyy), g = rho vp^2 (INVMAT1 1) or the modulus (3); a
+ * forward run with GRAD_FORM 2 saves ux = g (vxx + vyy) / DT (:309-311) */
+static void update_p_ac(orc_ctx *c, int mode)
 {
     const orc_params *P = &c->P;
     const real dhi = (real)P->dt / (real)P->dh;
@@ -1071,17 +1093,26 @@ static void update_p_ac(orc_ctx *c)
             real vxx, vyy;
             pressure_derivs(c, j, i, dhi, &vxx, &vyy);
             real g = P->invmat1 == 3 ? AT(c->pi, j, i) : AT(c->rho, j, i) * AT(c->pi, j, i) * AT(c->pi, j, i);
+            if (mode == 0 && P->grad_form == 2) AT(c->ux, j, i) = g * (vxx + vyy) / (real)P->dt;
             AT(c->sxx, j, i) += g * (vxx + vyy);
         }
     }
 }
 
-/* src/AC/psource_AC.c:9-51 (explosive source on p) */
-static void psource_ac(orc_ctx *c, int nt)
+/* src/AC/psource_AC.c:9-51: sw 0 the explosive source on p, sw 1 the
+ * adjoint pressure sources at the receivers (QUELLTYPB >= 4), once each */
+static void psource_ac(orc_ctx *c, int nt, int sw)
 {
-    for (int l = 0; l < c->nsrc; l++) {
-        int i = (int)c->srcpos[0 * c->nsrc + l], j = (int)c->srcpos[1 * c->nsrc + l];
-        AT(c->sxx, j, i) += source_amp(c, l, nt);
+    if (sw == 0) {
+        for (int l = 0; l < c->nsrc; l++) {
+            int i = (int)c->srcpos[0 * c->nsrc + l], j = (int)c->srcpos[1 * c->nsrc + l];
+            AT(c->sxx, j, i) += source_amp(c, l, nt);
+        }
+    } else if (c->P.quelltypb >= 4) {
+        for (int l = 0; l < c->ntr; l++) {
+            int i = c->recpos[l], j = c->recpos[c->ntr + l];
+            AT(c->sxx, j, i) += c->adj[2][(size_t)l * c->P.nt + (nt - 1)];
+        }
     }
 }
 
@@ -1136,7 +1167,7 @@ static void seismo_ac(orc_ctx *c, int nt)
 orc_ctx *orc_create_ac(const orc_params *p, const float *vp, const float *rho)
 {
     if (!p || !vp || !rho) { seterr("null argument"); return NULL; }
-    if (p->L != 0 || p->mode != 0) { seterr("acoustic runs are forward modelling (L = 0, MODE 0)"); return NULL; }
+    if (p->L != 0) { seterr("acoustic runs have no relaxation mechanisms (L = 0)"); return NULL; }
     orc_ctx *c = ctx_new(p);
     if (!c) return NULL;
     c->physics = 2;
@@ -1435,6 +1466,10 @@ static void correlate(orc_ctx *c, int k)
             real frx = c->fwd[0][o], fry = c->fwd[1][o], fx = c->fwd[2][o], fy = c->fwd[3][o], fu = c->fwd[4][o];
             real axx = AT(c->sxx, j, i), ayy = AT(c->syy, j, i), axy = AT(c->sxy, j, i);
             AT(c->wconv_rho, j, i) += (AT(c->vxp1, j, i) * frx) + (AT(c->vyp1, j, i) * fry);
+            if (c->physics == 2) { /* ac.c:260-276: forward p (or ux) times adjoint p */
+                AT(c->wconv, j, i) += fx * axx;
+                continue;
+            }
             AT(c->wconv, j, i) += (fx + fy) * (axx + ayy);
             /* VTI.c:306-331 / TTI.c:304-329: the shear term is commented out */
             if (c->physics == 3 || c->physics == 4) continue;
@@ -1641,11 +1676,12 @@ static void step_body(orc_ctx *c, int nt, int mode)
         return;
     }
     if (c->physics == 2) { /* src/AC/ac.c:96-205 */
-        update_v_ac(c, nt);
-        update_p_ac(c);
-        if (c->qtype == 1) psource_ac(c, nt);
+        update_v_ac(c, nt, mode == 1 ? 1 : 0);
+        update_p_ac(c, mode);
+        if (c->qtype == 1 && mode == 0) psource_ac(c, nt, 0);
+        if (P->quelltypb >= 4 && mode == 1) psource_ac(c, nt, 1);
         if (P->free_surf) surface_ac(c);
-        if (P->seismo) seismo_ac(c, nt);
+        if (P->seismo && mode == 0) seismo_ac(c, nt);
         return;
     }
     update_v(c, nt, mode == 1 ? 1 : 0);
@@ -1796,6 +1832,26 @@ int orc_assemble(orc_ctx *c)
 {
     const orc_params *P = &c->P;
     const real DT = (real)P->dt;
+    if (c->physics == 2) { /* src/AC/ass_gradAC.c:12-93 with INV_*_ITER 0 */
+        for (int i = 1; i <= P->nx; i++)
+            for (int j = 1; j <= P->ny; j++) {
+                AT(c->wlam, j, i) = -DT * AT(c->wconv, j, i);
+                real ppi = AT(c->pi, j, i), prho = AT(c->rho, j, i);
+                real lamss = P->invmat1 == 3 ? ppi : prho * ppi * ppi;
+                if (lamss > 0.0) {
+                    if (P->grad_form == 1)
+                        AT(c->wlam, j, i) = (real)((1.0 / (4.0 * lamss * lamss)) * AT(c->wlam, j, i));
+                    else
+                        AT(c->wlam, j, i) = (real)(-(1.0 / (lamss * lamss)) * AT(c->wlam, j, i));
+                }
+                AT(c->gvp, j, i) = P->invmat1 == 3 ? AT(c->wlam, j, i) : (real)(2.0 * ppi * prho * AT(c->wlam, j, i));
+                AT(c->gvs, j, i) = 0;
+                AT(c->wrho_s, j, i) = -DT * AT(c->wconv_rho, j, i);
+                AT(c->grho, j, i) = P->invmat1 == 1 ? ((ppi * ppi) * AT(c->wlam, j, i)) + AT(c->wrho_s, j, i)
+                                                    : AT(c->wrho_s, j, i);
+            }
+        return 0;
+    }
     for (int i = 1; i <= P->nx; i++)
         for (int j = 1; j <= P->ny; j++) {
             AT(c->wlam, j, i) = -DT * AT(c->wconv, j, i);

@@ -150,3 +150,68 @@ def test_seismo_modes_of_the_acoustic_loop():
     o3.forward()
     assert not o3.seismogram(0).any() and not o3.seismogram(2).any()
     assert np.array_equal(o3.seismogram(3), o.seismogram(3))
+
+
+def _adjoint(o, obs):
+    o.forward()
+    o.residuals(obs[0], obs[1], None, 1)
+    o.adjoint()
+    raw = [o.gradient(k) for k in range(3)]
+    o.assemble()
+    return raw, [o.gradient(k) for k in range(3, 6)]
+
+
+@pytest.mark.parametrize("grad_form", [1, 2])
+def test_acoustic_adjoint_is_the_elastic_adjoint_without_shear(grad_form):
+    """The acoustic adjoint pass (ac.c mode 1: adjoint forces, vxp1 time
+    integral, p correlation) against the P/SV one on the same model with
+    mu = 0 (INVMAT1 3, FP64, no free surface): there sxx = syy = p, so the
+    P/SV P correlation (fx + fy)(sxx + syy) is 4 times the acoustic fx p
+    and the density correlations agree; assembled (GRAD_FORM 1), the
+    acoustic lambda gradient -DT fx p / (4 lambda^2) is a quarter of the
+    P/SV one and the density gradients -DT w_rho agree."""
+    nx, ny, nt = 72, 56, 200
+    vp, _, rho = synth.layered_model(nx, ny, 10.0, 9, sigma=3.0)
+    lam = (rho.astype(np.float64) * vp.astype(np.float64) ** 2).astype(np.float32)
+    dt = float(np.float32(synth.stable_dt(float(vp.max()), 10.0, 8, safety=0.6)))
+    base = dict(nx=nx, ny=ny, nt=nt, dh=10.0, dt=dt, fdorder=8, maxrelerror=1, fw=10, free_surf=0, seismo=1,
+                ts=0.05, mode=1, dtinv=2, grad_form=grad_form, invmat1=3)
+    src = synth.source_table(10.0 * (nx // 2), 100.0, 10.0, fc=20.0)
+    sig = synth.ricker(nt, dt, 20.0, amp=1e3)[None, :]
+    rec = synth.receiver_line(30.0, 20.0, 30, 150.0, 10.0)
+    el = orc.Oracle(Params(**base, physics=1), lam, np.zeros_like(lam), rho, prec="f64")
+    ac = orc.Oracle(Params(**base, physics=2), lam, None, rho, prec="f64", aniso="ac")
+    for o in (el, ac):
+        o.set_sources(src, sig)
+        o.set_receivers(rec)
+    el.forward()
+    obs = (0.8 * el.seismogram(0), 1.1 * el.seismogram(1))
+    raw_el, grad_el = _adjoint(el, obs)
+    raw_ac, grad_ac = _adjoint(ac, obs)
+    assert np.abs(raw_ac[0]).max() > 0 and np.abs(raw_ac[2]).max() > 0
+    assert rel_l2(raw_ac[0], 0.25 * raw_el[0]) < 1e-9
+    assert rel_l2(raw_ac[2], raw_el[2]) < 1e-9
+    assert not raw_ac[1].any() and not grad_ac[1].any()
+    if grad_form == 1:
+        assert rel_l2(grad_ac[0], 0.25 * grad_el[0]) < 1e-9
+    assert rel_l2(grad_ac[2], grad_el[2]) < 1e-9
+
+
+def test_acoustic_adjoint_pressure_source_is_added_once():
+    """QUELLTYPB 4 (psource_AC.c:36-47): the residual of the pressure
+    seismogram enters p once per receiver and step; with the adjoint
+    sources set to one unit impulse at step 1, p at that receiver after one
+    adjoint step is exactly 1 (the velocities are still 0, so the pressure
+    update adds nothing)."""
+    p = params(mode=1, quelltypb=4, seismo=2, nt=20)
+    o = _ac(p, const(p.nx, p.ny, VP), const(p.nx, p.ny, RHO))
+    o.set_sources(synth.source_table(200.0, 200.0, 10.0, fc=20.0), synth.ricker(20, p.dt, 20.0)[None, :])
+    rec = np.array([[20], [15]], np.int32)
+    o.set_receivers(rec)
+    dp = np.zeros((1, 20), np.float32)
+    dp[0, 0] = 1.0
+    o.set_adjoint_sources(None, None, dp)
+    o.zero()
+    o.step(1, mode=1)
+    nd = o.nd
+    assert o.field(orc.ORC_SXX)[15 - 1 + nd, 20 - 1 + nd] == 1.0

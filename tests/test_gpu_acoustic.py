@@ -225,9 +225,7 @@ def test_errors_of_the_acoustic_entry_points():
         e.attach(g)
     e.close()
     g.close()
-    with pytest.raises(dnhip.DnError, match="MODE 0"):
-        dnhip.Context(dnhip.Params(**{**p.__dict__, "mode": 1}))
-    with pytest.raises(dnhip.DnError):
+    with pytest.raises(dnhip.DnError, match="L = 0"):
         dnhip.Context(dnhip.Params(**{**p.__dict__, "L": 2}))
 
 
@@ -285,3 +283,105 @@ def test_cli_forward_writes_the_library_seismograms(tmp_path, invmat1):
             _, data = dnio.parse_su(os.path.join(tmp, f"{name}.shot{ishot}"))
             assert np.array_equal(data, g.seismogram(c)), (ishot, c)
     g.close()
+
+
+def _adjoint(runner, obs):
+    runner.forward()
+    runner.residuals(obs[0], obs[1], obs[2], 1)
+    runner.adjoint()
+    raw = [runner.gradient(k) for k in range(3)]
+    runner.assemble()
+    return raw, [runner.gradient(k) for k in range(3, 6)]
+
+
+@pytest.mark.parametrize("grad_form,invmat1,dtinv,quelltypb",
+                         [(1, 1, 2, 1), (2, 1, 1, 1), (1, 3, 3, 4), (2, 3, 2, 4), (1, 1, 1, 2)])
+def test_rtm_gradients_equal_oracle(grad_form, invmat1, dtinv, quelltypb):
+    """The acoustic adjoint pass (src/AC/ac.c mode 1) and ass_gradAC: the
+    raw correlations and the assembled vp (or lambda) and density gradients
+    equal the FP32 oracle's bit for bit."""
+    print(f"[acoustic rtm] GRAD_FORM {grad_form} INVMAT1 {invmat1} DTINV {dtinv} QUELLTYPB {quelltypb}", flush=True)
+    p, m, src, sig, rec = ac_case(nt=240, seismo=4, invmat1=invmat1)
+    p = dnhip.Params(**{**p.__dict__, "mode": 1, "grad_form": grad_form, "dtinv": dtinv, "quelltypb": quelltypb})
+    g = dnhip.Context(p)
+    g.set_model(m[0], None, m[1])
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    o = orc.Oracle(p, m[0], None, m[1], prec="f32", aniso="ac")
+    o.set_sources(src, sig)
+    o.set_receivers(rec)
+    o.forward()
+    obs = (0.8 * o.seismogram(0), 0.9 * o.seismogram(1), 1.1 * o.seismogram(2))
+    graw, ggrad = _adjoint(g, obs)
+    oraw, ograd = _adjoint(o, obs)
+    for k in range(3):
+        assert np.array_equal(graw[k], oraw[k]), ("raw", k, rel_l2(graw[k], oraw[k]))
+        assert np.array_equal(ggrad[k], ograd[k]), ("grad", k, rel_l2(ggrad[k], ograd[k]))
+    assert np.abs(graw[0]).max() > 0 and np.abs(ggrad[0]).max() > 0 and np.abs(ggrad[2]).max() > 0
+    assert not graw[1].any() and not ggrad[1].any()
+    g.close()
+
+
+def test_cli_rtm_image_equals_the_library(tmp_path):
+    """MODE 2 with PHYSICS 2 (RTM_AC.c): the executable's P image is the
+    stacked ass_gradAC vp gradient of the library's adjoint passes with the
+    time-integrated observed data as adjoint sources, bit for bit; no S
+    image is written (RTM_AC_out.c)."""
+    import fwi_oracle as fo
+    tmp = str(tmp_path)
+    nx, ny, dh, nt = 100, 64, 10.0, 360
+    vp, _, rho = synth.layered_model(nx, ny, dh, 27, sigma=3.0)
+    dt = float(np.float32(synth.stable_dt(float(vp.max()) * 1.2, dh, 8, safety=0.6)))
+    for d in ("model", "su", "obs", "jac"):
+        os.makedirs(os.path.join(tmp, d))
+    vt = vp.copy()
+    vt[30:70, 30:34] *= 1.2
+    for name, v in (("true", vt), ("mig", vp)):
+        np.ascontiguousarray(v, np.float32).tofile(os.path.join(tmp, f"model/{name}.vp"))
+        np.ascontiguousarray(rho, np.float32).tofile(os.path.join(tmp, f"model/{name}.rho"))
+    with open(os.path.join(tmp, "source.dat"), "w") as f:
+        f.write("2\n300.00 0.0 30.00 0.0 15.00 1.00 0.0 1\n650.00 0.0 30.00 0.0 15.00 1.00 0.0 1\n")
+    with open(os.path.join(tmp, "receiver.dat"), "w") as f:
+        for k in range(40):
+            f.write(f"{40.0 + 20.0 * k:.3f}\t20.000\n")
+    inpfile.write_inp(os.path.join(tmp, "true.inp"), PHYSICS=2, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/true", SEIS_FILE_VX="obs/d_x.su", SEIS_FILE_VY="obs/d_y.su")
+    r = subprocess.run([BIN, "true.inp"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    inpfile.write_inp(os.path.join(tmp, "rtm.inp"), PHYSICS=2, MODE=2, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/mig", DTINV=2, RTM_SHOT=0, TESTSHOT="1, 2, 1")
+    inpfile.write_workflow(os.path.join(tmp, "wf.inp"))
+    print(f"[acoustic cli rtm] in {tmp}", flush=True)
+    r = subprocess.run([BIN, "rtm.inp", "wf.inp"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr
+    assert not os.path.exists(os.path.join(tmp, "jac/j_S_image"))
+    par = dnio.read_par(os.path.join(tmp, "rtm.inp"))
+    cwd = os.getcwd()
+    os.chdir(tmp)
+    try:
+        src, ts = dnio.read_sources(par.get("source_file"))
+        rec = dnio.read_receivers(par)
+    finally:
+        os.chdir(cwd)
+    p = dnhip.Params(nx=nx, ny=ny, nt=par.nt, dh=dh, dt=par.dt, fdorder=8, maxrelerror=par.maxrelerror, fw=par.fw,
+                     free_surf=par.free_surf, seismo=1, ts=ts, physics=2, mode=1, dtinv=2, grad_form=par.grad_form,
+                     damping=par.damping, fpml=par.fpml, npower=par.npower, k_max_pml=par.k_max_pml)
+    g = dnhip.Context(p)
+    g.set_model(vp, None, rho)
+    total = np.zeros((nx, ny), np.float32)
+    for ishot in (1, 2):
+        loc = dnio.split_sources(par, src, ishot - 1, 1)
+        g.set_sources(loc, dnio.wavelet(par, loc, ishot, ts))
+        g.set_receivers(rec[:2])
+        g.forward()
+        obs = [dnio.read_su(os.path.join(tmp, f"obs/d_{c}.su.shot{ishot}"), rec.shape[1], par.nt) for c in "xy"]
+        diffs = [fo.rtm_adjoint_source(obs[c], g.seismogram(c), par.dt, par.grad_form, 1)[0] for c in (0, 1)]
+        g.set_adjoint_sources(diffs[0], diffs[1], None)
+        g.adjoint()
+        g.assemble()
+        total += g.gradient(3)
+    g.close()
+    got = np.fromfile(os.path.join(tmp, "jac/j_P_image"), np.float32).reshape(nx, ny)
+    assert np.array_equal(got, total)
+    assert np.abs(total).max() > 0
