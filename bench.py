@@ -745,7 +745,7 @@ def main():
         print(f"# executable: {executable}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
-        # one GPU's share of the box's host: OMP_NUM_THREADS (16 on the GPU boxes)
+        # one GPU's share of the host's cores: OMP_NUM_THREADS when it is set
         host = host_cpu()
         threads = int(host["omp_num_threads"] or 0) or min(16, os.cpu_count() or 1)
         gcps, secs = cpu_baseline(L, args.cpu_steps, threads)
