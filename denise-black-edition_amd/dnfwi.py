@@ -41,6 +41,7 @@ def load():
     fp, vp_ = C.POINTER(C.c_float), C.c_void_p
     sig = {
         "dnfwi_lbfgs_create": ([C.c_int, C.c_int, C.c_int, C.POINTER(vp_)], C.c_int),
+        "dnfwi_lbfgs_create_ac": ([C.c_int, C.c_int, C.c_int, C.POINTER(vp_)], C.c_int),
         "dnfwi_lbfgs_destroy": ([vp_], None),
         "dnfwi_lbfgs_reset": ([vp_], None),
         "dnfwi_lbfgs_pointer": ([vp_], C.c_int),
@@ -87,8 +88,15 @@ def load():
 
 
 def _f(a):
+    if a is None:
+        return None
     a = np.ascontiguousarray(a, dtype=np.float32)
     return a
+
+
+def _c(a):
+    """a float32 copy (None stays None: no vs in the acoustic problem)"""
+    return None if a is None else _f(a).copy()
 
 
 def _p(a):
@@ -102,20 +110,22 @@ def update_par(vpupperlim=1e9, vplowerlim=0.0, vsupperlim=1e9, vslowerlim=0.0, r
 
 
 class LBFGS:
-    """dnfwi_lbfgs: the L-BFGS memory of one FWI run."""
+    """dnfwi_lbfgs: the L-BFGS memory of one FWI run (ac: the acoustic
+    problem's two classes, no vs; pass vs arguments as None)."""
 
-    def __init__(self, nx, ny, nlbfgs):
+    def __init__(self, nx, ny, nlbfgs, ac=False):
         self.nx, self.ny = nx, ny
         h = C.c_void_p()
-        dnio._check(load().dnfwi_lbfgs_create(nx, ny, nlbfgs, C.byref(h)))
+        create = load().dnfwi_lbfgs_create_ac if ac else load().dnfwi_lbfgs_create
+        dnio._check(create(nx, ny, nlbfgs, C.byref(h)))
         self.h = h
 
     def direction(self, it, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho, vp, vs, rho):
         """The search direction (new arrays); the gradients are not modified."""
-        d = [_f(g).copy() for g in (g_vp, g_vs, g_rho)]
+        d = [_c(g) for g in (g_vp, g_vs, g_rho)]
         m = [_f(a) for a in (vp, vs, rho)]
         for a in d + m:
-            assert a.shape == (self.nx, self.ny)
+            assert a is None or a.shape == (self.nx, self.ny)
         dnio._check(load().dnfwi_lbfgs_direction(self.h, it, c_vp, c_vs, c_rho, *[_p(a) for a in d],
                                                  *[_p(a) for a in m]))
         return tuple(d)
@@ -152,7 +162,7 @@ def model_update(par, it, eps_scale, vp, vs, rho, d_vp, d_vs, d_rho):
     """(vp_new, vs_new, rho_new, eps_true)."""
     m = [_f(a) for a in (vp, vs, rho)]
     d = [_f(a) for a in (d_vp, d_vs, d_rho)]
-    out = [np.empty_like(m[0]) for _ in range(3)]
+    out = [None if (k == 1 and vs is None) else np.empty_like(m[0]) for k in range(3)]
     nx, ny = m[0].shape
     e = load().dnfwi_model_update(C.byref(par), it, eps_scale, nx, ny, *[_p(a) for a in m + d + out])
     return out[0], out[1], out[2], float(e)
@@ -174,7 +184,7 @@ def step_length(par, it, eps_scale0, scalefac, stepmax, L2t0, vp, vs, rho, d_vp,
 
     def cb(_user, pv, ps, pr, itest):
         try:
-            arrs = [np.ctypeslib.as_array(p, shape=(nx, ny)).copy() for p in (pv, ps, pr)]
+            arrs = [np.ctypeslib.as_array(p, shape=(nx, ny)).copy() if p else None for p in (pv, ps, pr)]
             return float(misfit(*arrs, itest))
         except Exception as e:  # an exception cannot cross the C frame
             err.append(e)
@@ -205,7 +215,7 @@ def energy_precond(eprecond, dh, Ws, Wr, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho):
     """(We, g_vp, g_vs, g_rho) after dnfwi_energy_precond (new arrays)."""
     ws = _f(Ws)
     wr = _f(Wr) if Wr is not None else np.zeros_like(ws)
-    g = [_f(a).copy() for a in (g_vp, g_vs, g_rho)]
+    g = [_c(a) for a in (g_vp, g_vs, g_rho)]
     We = np.empty_like(ws)
     nx, ny = ws.shape
     dnio._check(load().dnfwi_energy_precond(eprecond, nx, ny, dh, _p(ws), _p(wr), c_vp, c_vs, c_rho,
@@ -278,7 +288,7 @@ class PCG:
 
     def direction(self, pcg_beta, g_vp, g_vs, g_rho):
         """(d_vp, d_vs, d_rho, betas): new arrays."""
-        d = [_f(a).copy() for a in (g_vp, g_vs, g_rho)]
+        d = [_c(a) for a in (g_vp, g_vs, g_rho)]
         b = np.zeros(3, np.float32)
         dnio._check(load().dnfwi_pcg_direction(self.h, pcg_beta, *[_p(a) for a in d], _p(b)))
         return d[0], d[1], d[2], b

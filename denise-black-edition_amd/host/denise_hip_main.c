@@ -108,12 +108,10 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
     if (p->physics < 1 || p->physics > 5)
         die("PHYSICS", "2D P/SV (1), acoustic (2), VTI (3), TTI (4) and SH (5) are built");
     /* physics_SH.c: of its modes, forward modelling (MODE 0) is built;
-     * physics_AC.c: MODE 0 and RTM (MODE 2), its FWI (FWI_AC.c) is not;
+     * physics_AC.c: MODE 0, FWI (FWI_AC.c) and RTM (MODE 2) are built;
      * physics_VTI.c / physics_TTI.c offer MODE 0 and RTM, their FWI branch
      * is commented out */
     if (p->physics == 5 && p->mode != 0) die("MODE", "the SH loop is built for forward modelling (MODE 0)");
-    if (p->physics == 2 && p->mode == 1)
-        die("MODE", "the acoustic loop runs forward modelling (MODE 0) and RTM (MODE 2); acoustic FWI is not built");
     if ((p->physics == 3 || p->physics == 4) && p->mode == 1)
         die("MODE", "the VTI/TTI loops run forward modelling (MODE 0) and RTM (MODE 2); the reference has no "
                     "VTI/TTI FWI");
@@ -714,25 +712,30 @@ static void shot_gradient(run *r, int ishot, double l2, int test, float **gs, fl
         dn_ok(dn_assemble_gradient(r->ctx), "gradient");
         for (int k = 0; k < 3; k++) dn_ok(dn_get_gradient(r->ctx, 3 + k, gs[k]), "gradient");
     }
+    /* grad_obj_ac.c: the acoustic problem has vp and rho gradients only */
+    const int ac = P->physics == 2;
     if (r->eprecond) {
-        /* grad_obj_psv.c:219-238: energy weights of this shot */
+        /* grad_obj_psv.c:219-238, grad_obj_ac.c:223-234: energy weights of this shot */
         dn_ok(dn_get_energy(r->ctx, 0, ws), "energy");
         if (r->eprecond == 1) dn_ok(dn_get_energy(r->ctx, 1, wr), "energy");
         io_ok(dnfwi_energy_precond(r->eprecond, P->nx, P->ny, P->dh, ws, wr, r->c_vp, r->c_vs, r->c_rho, gs[0],
-                                   gs[1], gs[2], NULL),
+                                   ac ? NULL : gs[1], gs[2], NULL),
               "energy preconditioning");
     }
     if (P->sws_taper_circular_per_shot) {
         /* grad_obj_psv.c:241-250, taper_grad_shot.c: the source taper of this shot's source alone */
         float col[8], *coeff = xmalloc(sizeof(float) * r->ncell);
         for (int q = 0; q < 8; q++) col[q] = r->srcpos[(size_t)q * r->nsrc + (ishot - 1)];
-        for (int k = 0; k < 3; k++)
+        for (int k = 0; k < 3; k++) {
+            if (ac && k == 1) continue;
             io_ok(dnfwi_taper_sources(P->nx, P->ny, P->dh, P->srtradius, P->srtshape, P->filtsize, 1, col, coeff,
                                       gs[k]),
                   "per-shot gradient taper");
+        }
         free(coeff);
     }
     for (int k = 0; k < 3; k++) {
+        if (ac && k == 1) continue;
         for (size_t q = 0; q < r->ncell; q++) g_sum[k][q] += gs[k][q];
         /* grad_obj_psv.c:263-266; RTM_AC_out_shot.c writes the P image only */
         if (P->rtm_shot && k < (P->physics == 2 ? 1 : 2)) {
@@ -907,29 +910,39 @@ static double misfit_of(void *user, const float *vp, const float *vs, const floa
     return l2;
 }
 
+/* vs NULL: the acoustic run writes vp and rho (model_freq_out_AC.c,
+ * model_it_out_AC.c, calc_mat_change_test_AC.c:168-180) */
 static void write_models(const char *fmt_vp, const char *fmt_vs, const char *fmt_rho, const float *vp, const float *vs,
                          const float *rho, int nx, int ny)
 {
     write_plane(fmt_vp, vp, nx, ny);
-    write_plane(fmt_vs, vs, nx, ny);
+    if (vs) write_plane(fmt_vs, vs, nx, ny);
     write_plane(fmt_rho, rho, nx, ny);
 }
 
-/* MODE 1: src/PSV/FWI_PSV.c:381-836 with GRAD_METHOD = 2 */
-static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, const float *taus, const float *taup)
+/* MODE 1: src/PSV/FWI_PSV.c:381-836, and for PHYSICS 2 src/AC/FWI_AC.c:386-848,
+ * whose iteration is the same with the parameters vp and rho only (vs NULL
+ * below): two-class L-BFGS / PCG memories, calc_mat_change_test_AC, the
+ * models and gradients of vp and rho, no MODEL_FILTER (commented out at
+ * FWI_AC.c:784-787) and smooth_grad with the vp model (FWI_AC.c:676) */
+static void fwi(run *r, const char *workflow, float *vp, float *vs_in, float *rho, const float *taus,
+                const float *taup)
 {
     dnio_par *P = r->P;
     const int nx = P->nx, ny = P->ny;
+    const int ac = P->physics == 2;
+    float *vs = ac ? NULL : vs_in;
     int stagemax = 0;
     io_ok(dnio_count_stages(workflow, &stagemax), "workflow file");
     float *g[3];
     for (int k = 0; k < 3; k++) g[k] = xmalloc(sizeof(float) * r->ncell);
-    float *nv = xmalloc(sizeof(float) * r->ncell), *ns = xmalloc(sizeof(float) * r->ncell);
+    float *nv = xmalloc(sizeof(float) * r->ncell), *ns = ac ? NULL : xmalloc(sizeof(float) * r->ncell);
     float *nr = xmalloc(sizeof(float) * r->ncell);
     dnfwi_lbfgs *lb = NULL;
     dnfwi_pcg *pc = NULL;
     if (P->grad_method == 2)
-        io_ok(dnfwi_lbfgs_create(nx, ny, P->nlbfgs, &lb), "L-BFGS memory");
+        io_ok(ac ? dnfwi_lbfgs_create_ac(nx, ny, P->nlbfgs, &lb) : dnfwi_lbfgs_create(nx, ny, P->nlbfgs, &lb),
+              "L-BFGS memory");
     else
         io_ok(dnfwi_pcg_create(nx, ny, &pc), "PCG memory");
     double *L2_hist = xmalloc(sizeof(double) * (size_t)(P->itermax + 2));
@@ -968,11 +981,15 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
             printf("\n TDFWI ITERATION %d of %d (stage %d of %d)\n", iter, P->itermax, nstage, stagemax);
             fflush(stdout);
             if (iter_true == 1) {
-                /* FWI_PSV.c:540-580: the covariances are the initial model's averages */
+                /* FWI_PSV.c:540-580, FWI_AC.c:478-513: the covariances are the
+                 * initial model's averages (C_vs stays 0 in the acoustic run) */
                 c_vp = dnfwi_model_mean(vp, nx, ny);
-                c_vs = dnfwi_model_mean(vs, nx, ny);
+                c_vs = ac ? 0.0f : dnfwi_model_mean(vs, nx, ny);
                 c_rho = dnfwi_model_mean(rho, nx, ny);
-                printf(" Vp_avg = %.0f \t Vs_avg = %.0f \t rho_avg = %.0f\n", c_vp, c_vs, c_rho);
+                if (ac)
+                    printf(" Vp_avg = %.0f \t rho_avg = %.0f\n", c_vp, c_rho);
+                else
+                    printf(" Vp_avg = %.0f \t Vs_avg = %.0f \t rho_avg = %.0f\n", c_vp, c_vs, c_rho);
                 r->c_vp = c_vp;
                 r->c_vs = c_vs;
                 r->c_rho = c_rho;
@@ -984,36 +1001,47 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
             r->stf_now = r->inv_stf && iter == 1; /* grad_obj_psv.c:127 */
             const double L2sum = gradient_pass(r, g);
             r->stf_now = 0;
-            if (st.spatfilter == 3) { /* FWI_PSV.c:628-631: smooth_grad before precond_PSV */
+            if (st.spatfilter == 3) {
+                /* FWI_PSV.c:628-631: smooth_grad before precond_PSV; its width
+                 * follows C_vs, which is 0 in the acoustic run (an empty filter) */
                 int w = 0;
-                for (int k = 0; k < 3; k++)
+                for (int k = 0; k < 3; k++) {
+                    if (ac && k == 1) continue;
                     io_ok(dnfwi_gauss_filter(nx, ny, P->dh, st.wd_damp, c_vs, st.fc_end, g[k], &w), "SPATFILTER 3");
+                }
                 printf(" gradients smoothed with a %d-point Gaussian (%.2f m)\n", w, w * P->dh);
             }
-            if (st.spatfilter == 4) { /* gauss_filt_var.c with the current Vs model (FWI_PSV.c:629-631) */
-                for (int k = 0; k < 3; k++)
-                    io_ok(dnfwi_gauss_filter_var(nx, ny, P->dh, st.wd_damp, st.wd_damp1, st.fc_end, vs, g[k]),
+            if (st.spatfilter == 4) {
+                /* gauss_filt_var.c with the current Vs model (FWI_PSV.c:629-631),
+                 * with the Vp model in the acoustic run (FWI_AC.c:676-677) */
+                for (int k = 0; k < 3; k++) {
+                    if (ac && k == 1) continue;
+                    io_ok(dnfwi_gauss_filter_var(nx, ny, P->dh, st.wd_damp, st.wd_damp1, st.fc_end, ac ? vp : vs,
+                                                 g[k]),
                           "SPATFILTER 4");
+                }
                 printf(" gradients smoothed with the velocity-dependent Gaussian\n");
             }
-            precondition(r, g); /* FWI_PSV.c:634: precond_PSV */
+            precondition(r, g); /* FWI_PSV.c:634: precond_PSV; FWI_AC.c:680: precond_AC */
             const char *suffix[3] = {"vp", "vs", "rho"};
             for (int k = 0; k < 3; k++) {
+                if (ac && k == 1) continue;
                 snprintf(r->name, sizeof r->name, "%s_grad_%s", P->jacobian, suffix[k]);
                 write_plane(r->name, g[k], nx, ny);
             }
+            float *gvs = ac ? NULL : g[1];
             printf(" misfit (L2, test shots): %.9e\n", L2sum);
             /* FWI_PSV.c:636-676: PCG or L-BFGS search direction */
             if (lb) {
-                io_ok(dnfwi_lbfgs_direction(lb, iter, c_vp, c_vs, c_rho, g[0], g[1], g[2], vp, vs, rho), "L-BFGS");
-                /* extract_LBFGS_PSV.c:80-249: JACOBIAN_p*.old, _c*.old */
-                io_ok(dnfwi_lbfgs_save(lb, P->jacobian, g[0], g[1], g[2]), "L-BFGS files");
+                io_ok(dnfwi_lbfgs_direction(lb, iter, c_vp, c_vs, c_rho, g[0], gvs, g[2], vp, vs, rho), "L-BFGS");
+                /* extract_LBFGS_PSV.c:80-249 / extract_LBFGS_AC.c:60-178: JACOBIAN_p*.old, _c*.old */
+                io_ok(dnfwi_lbfgs_save(lb, P->jacobian, g[0], gvs, g[2]), "L-BFGS files");
             } else
-                io_ok(dnfwi_pcg_direction(pc, P->pcg_beta, g[0], g[1], g[2], NULL), "PCG");
+                io_ok(dnfwi_pcg_direction(pc, P->pcg_beta, g[0], gvs, g[2], NULL), "PCG");
             if (iter == 1) min_iter_help = P->min_iter; /* FWI_PSV.c:687-690 */
             obj_env oe = {r, taus, taup};
             io_ok(dnfwi_step_length(&up, iter, P->eps_scale, P->scalefac, P->stepmax, L2sum, nx, ny, vp, vs, rho, g[0],
-                                    g[1], g[2], misfit_of, &oe, &ls),
+                                    gvs, g[2], misfit_of, &oe, &ls),
                   "step length");
             /* FWI_PSV.c:700-708 */
             float eps_scale = ls.eps_scale, opteps_vp;
@@ -1037,10 +1065,11 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
             printf(" step length %e (trials %e %e %e, misfits %e %e %e)\n", opteps_vp, ls.epst[0], ls.epst[1],
                    ls.epst[2], ls.L2t[0], ls.L2t[1], ls.L2t[2]);
             L2_hist[iter] = L2sum;
-            /* FWI_PSV.c:750: the update, written as INV_MODELFILE_{vp,vs,rho}.bin */
-            dnfwi_model_update(&up, iter, eps_scale, nx, ny, vp, vs, rho, g[0], g[1], g[2], nv, ns, nr);
+            /* FWI_PSV.c:750 / FWI_AC.c:782: the update, written as
+             * INV_MODELFILE_{vp,vs,rho}.bin (vp and rho for the acoustic run) */
+            dnfwi_model_update(&up, iter, eps_scale, nx, ny, vp, vs, rho, g[0], gvs, g[2], nv, ns, nr);
             memcpy(vp, nv, sizeof(float) * r->ncell);
-            memcpy(vs, ns, sizeof(float) * r->ncell);
+            if (vs) memcpy(vs, ns, sizeof(float) * r->ncell);
             memcpy(rho, nr, sizeof(float) * r->ncell);
             snprintf(name[0], sizeof name[0], "%s_vp.bin", P->inv_modelfile);
             snprintf(name[1], sizeof name[1], "%s_vs.bin", P->inv_modelfile);
@@ -1048,7 +1077,7 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs, float *rho, 
             write_models(name[0], name[1], name[2], vp, vs, rho, nx, ny);
             /* FWI_PSV.c:752, smooth_model.c:37: MODEL_FILTER 1 replaces the
              * updated models (not the files just written) by their medians */
-            if (P->model_filter == 1 && P->filt_size != 0) {
+            if (!ac && P->model_filter == 1 && P->filt_size != 0) {
                 float *mods[3] = {vp, vs, rho};
                 for (int k = 0; k < 3; k++)
                     if (dnfwi_median_filter(nx, ny, P->filt_size, mods[k])) die("MODEL_FILTER", dnio_last_error());

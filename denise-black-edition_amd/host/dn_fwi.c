@@ -24,7 +24,8 @@ int dnio_fail_msg(const char *msg); /* dn_io.c */
 
 struct dnfwi_lbfgs {
     int nx, ny, nlbfgs, pointer;
-    size_t ncell, nvec;          /* nvec = 3 * ncell: [rho | vs | vp] */
+    int nclass;                  /* 3: [rho | vs | vp]; 2 (acoustic): [rho | vp] */
+    size_t ncell, nvec;          /* nvec = nclass * ncell */
     float *y, *s;                /* nlbfgs * nvec */
     float *q, *r;                /* nvec */
     float *rho_l, *alpha_l;      /* nlbfgs */
@@ -32,7 +33,7 @@ struct dnfwi_lbfgs {
     int have_old;
 };
 
-int dnfwi_lbfgs_create(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out)
+static int lbfgs_create(int nx, int ny, int nlbfgs, int nclass, dnfwi_lbfgs **out)
 {
     if (!out || nx < 1 || ny < 1 || nlbfgs < 1) return dnio_fail_msg("dnfwi_lbfgs_create: bad size");
     dnfwi_lbfgs *lb = calloc(1, sizeof *lb);
@@ -40,9 +41,10 @@ int dnfwi_lbfgs_create(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out)
     lb->nx = nx;
     lb->ny = ny;
     lb->nlbfgs = nlbfgs;
+    lb->nclass = nclass;
     lb->ncell = (size_t)nx * ny;
-    lb->nvec = 3 * lb->ncell;
-    /* FWI_PSV.c:289-298: NR vectors, zero-initialised */
+    lb->nvec = (size_t)nclass * lb->ncell;
+    /* FWI_PSV.c:289-298 / FWI_AC.c:269-283: NR vectors, zero-initialised */
     lb->y = calloc((size_t)nlbfgs * lb->nvec, sizeof(float));
     lb->s = calloc((size_t)nlbfgs * lb->nvec, sizeof(float));
     lb->q = calloc(lb->nvec, sizeof(float));
@@ -58,6 +60,16 @@ int dnfwi_lbfgs_create(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out)
     lb->pointer = 1; /* FWI_PSV.c:290 */
     *out = lb;
     return 0;
+}
+
+int dnfwi_lbfgs_create(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out)
+{
+    return lbfgs_create(nx, ny, nlbfgs, 3, out);
+}
+
+int dnfwi_lbfgs_create_ac(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out)
+{
+    return lbfgs_create(nx, ny, nlbfgs, 2, out); /* FWI_AC.c:271: NLBFGS_class = 2 */
 }
 
 void dnfwi_lbfgs_destroy(dnfwi_lbfgs *lb)
@@ -100,33 +112,36 @@ static float dotp(const float *a, const float *b, size_t n)
 int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, float c_rho, float *g_vp,
                           float *g_vs, float *g_rho, const float *vp, const float *vs, const float *rho)
 {
-    if (!lb || !g_vp || !g_vs || !g_rho || !vp || !vs || !rho || iter < 1)
+    const int ac = lb && lb->nclass == 2;
+    if (!lb || !g_vp || !g_rho || !vp || !rho || (!ac && (!g_vs || !vs)) || iter < 1)
         return dnio_fail_msg("dnfwi_lbfgs_direction: null argument or iter < 1");
     if (iter > 1 && !lb->have_old)
         return dnio_fail_msg("dnfwi_lbfgs_direction: iter > 1 needs the previous iteration's gradients");
     const size_t nc = lb->ncell, nv = lb->nvec;
-    const int nl = lb->nlbfgs;
-    float *g[3] = {g_rho, g_vs, g_vp};           /* vector order of the L-BFGS vectors */
-    const float *m[3] = {rho, vs, vp};
-    const float cs[3] = {c_rho, c_vs, c_vp};
+    const int nl = lb->nlbfgs, ncl = lb->nclass;
+    /* vector order of the L-BFGS vectors: [rho | vs | vp] (store_LBFGS_PSV.c)
+     * or [rho | vp] (store_LBFGS_AC.c:63-140) */
+    float *g[3] = {g_rho, ac ? g_vp : g_vs, g_vp};
+    const float *m[3] = {rho, ac ? vp : vs, vp};
+    const float cs[3] = {c_rho, ac ? c_vp : c_vs, c_vp};
 
     /* FWI_PSV.c:455-464 */
     if (iter > 2) lb->pointer++;
     if (lb->pointer > nl) lb->pointer = 1;
 
     /* store_LBFGS_PSV.c:37-89: normalisation; gradp = the normalised gradient */
-    for (int c = 0; c < 3; c++)
+    for (int c = 0; c < ncl; c++)
         for (size_t k = 0; k < nc; k++) g[c][k] = cs[c] * g[c][k];
 
     if (iter > 1) {
         /* store_LBFGS_PSV.c:91-205: y, s into the current slot; q = gradient */
         float *ys = lb->y + (size_t)(lb->pointer - 1) * nv, *ss = lb->s + (size_t)(lb->pointer - 1) * nv;
-        for (int c = 0; c < 3; c++)
+        for (int c = 0; c < ncl; c++)
             for (size_t k = 0; k < nc; k++) {
                 ys[c * nc + k] = g[c][k] - lb->gold[c * nc + k];
                 ss[c * nc + k] = m[c][k] - lb->mold[c * nc + k];
             }
-        for (int c = 0; c < 3; c++) memcpy(lb->q + c * nc, g[c], sizeof(float) * nc);
+        for (int c = 0; c < ncl; c++) memcpy(lb->q + c * nc, g[c], sizeof(float) * nc);
 
         /* LBFGS.c:26-95 */
         const float *yp = lb->y + (size_t)(lb->pointer - 1) * nv, *sp = lb->s + (size_t)(lb->pointer - 1) * nv;
@@ -156,7 +171,7 @@ int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, flo
 
     /* extract_LBFGS_PSV.c:80-249: keep the model and the normalised
      * gradients (before they are replaced by the direction) */
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < ncl; c++) {
         memcpy(lb->mold + c * nc, m[c], sizeof(float) * nc);
         memcpy(lb->gold + c * nc, g[c], sizeof(float) * nc);
     }
@@ -164,10 +179,10 @@ int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, flo
 
     if (iter > 1) {
         /* extract_LBFGS_PSV.c:31-78: direction = r, scaled back */
-        for (int c = 0; c < 3; c++) memcpy(g[c], lb->r + c * nc, sizeof(float) * nc);
+        for (int c = 0; c < ncl; c++) memcpy(g[c], lb->r + c * nc, sizeof(float) * nc);
         for (size_t k = 0; k < nc; k++) {
             g_vp[k] = g_vp[k] * c_vp;
-            g_vs[k] = g_vs[k] * c_vs;
+            if (!ac) g_vs[k] = g_vs[k] * c_vs;
             g_rho[k] = g_rho[k] * c_rho;
         }
     }
@@ -177,18 +192,23 @@ int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, flo
 int dnfwi_lbfgs_save(const dnfwi_lbfgs *lb, const char *jacobian, const float *d_vp, const float *d_vs,
                      const float *d_rho)
 {
-    if (!lb || !jacobian || !d_vp || !d_vs || !d_rho) return dnio_fail_msg("dnfwi_lbfgs_save: null argument");
+    const int ac = lb && lb->nclass == 2;
+    if (!lb || !jacobian || !d_vp || (!ac && !d_vs) || !d_rho)
+        return dnio_fail_msg("dnfwi_lbfgs_save: null argument");
     if (!lb->have_old) return dnio_fail_msg("dnfwi_lbfgs_save: no direction taken yet");
     const size_t nc = lb->ncell;
     /* extract_LBFGS_PSV.c:80-249: per parameter the model, the normalised
-     * gradient, then the direction; vp, vs, rho ([rho | vs | vp] slots 2, 1, 0) */
+     * gradient, then the direction; vp, vs, rho ([rho | vs | vp] slots 2, 1, 0).
+     * extract_LBFGS_AC.c:60-178: vp, rho ([rho | vp] slots 1, 0) */
     static const char *const names[3][3] = {
         {"p_vp", "p", "c"}, {"p_vs", "p_u", "c_u"}, {"p_mrho", "p_rho", "c_rho"}};
     static const int slot[3] = {2, 1, 0};
     const float *dir[3] = {d_vp, d_vs, d_rho};
     char name[1024];
     for (int k = 0; k < 3; k++) {
-        const float *arr[3] = {lb->mold + (size_t)slot[k] * nc, lb->gold + (size_t)slot[k] * nc, dir[k]};
+        if (ac && k == 1) continue;
+        const int sl = ac ? (k == 0 ? 1 : 0) : slot[k];
+        const float *arr[3] = {lb->mold + (size_t)sl * nc, lb->gold + (size_t)sl * nc, dir[k]};
         for (int f = 0; f < 3; f++) {
             if ((size_t)snprintf(name, sizeof name, "%s_%s.old", jacobian, names[k][f]) >= sizeof name)
                 return dnio_fail_msg("dnfwi_lbfgs_save: JACOBIAN path too long");
@@ -227,45 +247,52 @@ float dnfwi_model_update(const dnfwi_update_par *u, int iter, float eps_scale, i
                          float *vp_new, float *vs_new, float *rho_new)
 {
     const size_t n = (size_t)nx * ny;
-    float pimax, gradmax, umax, gradmax_u, rhomax, gradmax_rho;
+    /* no vs (acoustic): calc_mat_change_test_AC.c, vp and rho only */
+    const int ac = !vs;
+    float pimax, gradmax, umax = 0.0f, gradmax_u = 0.0f, rhomax, gradmax_rho;
     maxima(vp, d_vp, n, &pimax, &gradmax);
-    maxima(vs, d_vs, n, &umax, &gradmax_u);
+    if (!ac) maxima(vs, d_vs, n, &umax, &gradmax_u);
     maxima(rho, d_rho, n, &rhomax, &gradmax_rho);
-    /* calc_mat_change_test_PSV.c:111-147 */
+    /* calc_mat_change_test_PSV.c:111-147, calc_mat_change_test_AC.c:70-98 */
     float eps = eps_scale * (pimax / gradmax);
     if (iter < u->inv_vp_iter) eps = 0.0f;
-    float eps_u = eps_scale * (umax / gradmax_u);
-    if (iter < u->inv_vs_iter) eps_u = 0.0f;
+    float eps_u = 0.0f;
+    if (!ac) {
+        eps_u = eps_scale * (umax / gradmax_u);
+        if (iter < u->inv_vs_iter) eps_u = 0.0f;
+    }
     float eps_rho = eps_scale * (rhomax / gradmax_rho) * u->scalerho;
     if (iter < u->inv_rho_iter) eps_rho = 0.0f;
-    /* calc_mat_change_test_PSV.c:157-215 */
+    /* calc_mat_change_test_PSV.c:157-215, calc_mat_change_test_AC.c:111-160 */
     if (u->invmat1 == 1 || u->invmat1 == 3) {
         for (size_t k = 0; k < n; k++) {
             float p1 = vp[k] - eps * d_vp[k];
-            float u1 = vs[k] - eps_u * d_vs[k];
+            float u1 = ac ? 0.0f : vs[k] - eps_u * d_vs[k];
             float r1 = rho[k] - eps_rho * d_rho[k];
             if (u->invmat1 == 1) {
                 if (p1 < u->vplowerlim) p1 = vp[k];
                 if (p1 > u->vpupperlim) p1 = vp[k];
-                if ((u1 < u->vslowerlim) && (u1 > 1e-6)) u1 = vs[k];
-                if (u1 > u->vsupperlim) u1 = vs[k];
+                if (!ac && (u1 < u->vslowerlim) && (u1 > 1e-6)) u1 = vs[k];
+                if (!ac && u1 > u->vsupperlim) u1 = vs[k];
                 if (r1 < u->rholowerlim) r1 = rho[k];
                 if (r1 > u->rhoupperlim) r1 = rho[k];
             }
             if (p1 < 0.0) p1 = vp[k];
-            if (u1 < 0.0) u1 = vs[k];
+            if (!ac && u1 < 0.0) u1 = vs[k];
             if (r1 < 0.0) r1 = rho[k];
             vp_new[k] = p1;
-            vs_new[k] = u1;
+            if (!ac) vs_new[k] = u1;
             rho_new[k] = r1;
         }
     } else {
         /* INVMAT1 = 2: the reference leaves the model unchanged */
         memcpy(vp_new, vp, sizeof(float) * n);
-        memcpy(vs_new, vs, sizeof(float) * n);
+        if (!ac) memcpy(vs_new, vs, sizeof(float) * n);
         memcpy(rho_new, rho, sizeof(float) * n);
     }
-    return eps_u;
+    /* the reference's eps_true: EPSILON_u (P/SV), EPSILON (acoustic,
+     * calc_mat_change_test_AC.c:103) */
+    return ac ? eps : eps_u;
 }
 
 float dnfwi_opt_step(const double *L2t, const float *epst)
@@ -303,11 +330,13 @@ int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, flo
                       const float *d_vp, const float *d_vs, const float *d_rho, dnfwi_misfit_fn misfit,
                       void *user, dnfwi_linesearch *out)
 {
-    if (!u || !misfit || !out || !vp || !vs || !rho || !d_vp || !d_vs || !d_rho || nx < 1 || ny < 1)
+    if (!u || !misfit || !out || !vp || !rho || !d_vp || !d_rho || (vs && !d_vs) || nx < 1 || ny < 1)
         return dnio_fail_msg("dnfwi_step_length: null argument");
     const size_t n = (size_t)nx * ny;
-    float *tv = malloc(sizeof(float) * n), *ts = malloc(sizeof(float) * n), *tr = malloc(sizeof(float) * n);
-    if (!tv || !ts || !tr) {
+    /* no vs: the acoustic search (step_length_est_ac.c), test models of vp and rho */
+    float *tv = malloc(sizeof(float) * n), *ts = vs ? malloc(sizeof(float) * n) : NULL;
+    float *tr = malloc(sizeof(float) * n);
+    if (!tv || (vs && !ts) || !tr) {
         free(tv);
         free(ts);
         free(tr);
@@ -406,8 +435,7 @@ int dnfwi_rtm_adjoint_source(const float *data, const float *syn, int ntr, int n
 int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws, const float *Wr, float c_vp,
                          float c_vs, float c_rho, float *g_vp, float *g_vs, float *g_rho, float *We)
 {
-    if ((eprecond != 1 && eprecond != 3) || !Ws || (eprecond == 1 && !Wr) || !g_vp || !g_vs || !g_rho || nx < 1 ||
-        ny < 1)
+    if ((eprecond != 1 && eprecond != 3) || !Ws || (eprecond == 1 && !Wr) || !g_vp || !g_rho || nx < 1 || ny < 1)
         return dnio_fail_msg("dnfwi_energy_precond: bad argument");
     const size_t n = (size_t)nx * ny;
     float *w = We ? We : malloc(sizeof(float) * n);
@@ -428,11 +456,13 @@ int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws
         }
     /* eprecond1.c:50-58 */
     for (size_t k = 0; k < n; k++) w[k] = w[k] + (eps_we * maxwe);
-    /* grad_obj_psv.c:224-237 */
+    /* grad_obj_psv.c:224-237; grad_obj_ac.c:225-233 has no vs gradient */
     for (size_t k = 0; k < n; k++) {
         g_vp[k] = g_vp[k] / (w[k] * c_vp * c_vp);
-        g_vs[k] = g_vs[k] / (w[k] * c_vs * c_vs);
-        if (c_vs == 0.0) g_vs[k] = 0.0f;
+        if (g_vs) {
+            g_vs[k] = g_vs[k] / (w[k] * c_vs * c_vs);
+            if (c_vs == 0.0) g_vs[k] = 0.0f;
+        }
         g_rho[k] = g_rho[k] / (w[k] * c_rho * c_rho);
     }
     if (!We) free(w);
@@ -732,13 +762,16 @@ void dnfwi_pcg_reset(dnfwi_pcg *pc)
 
 int dnfwi_pcg_direction(dnfwi_pcg *pc, int pcg_beta, float *g_vp, float *g_vs, float *g_rho, float *beta_out)
 {
-    if (!pc || !g_vp || !g_vs || !g_rho || pcg_beta < 1 || pcg_beta > 4)
+    if (!pc || !g_vp || !g_rho || pcg_beta < 1 || pcg_beta > 4)
         return dnio_fail_msg("dnfwi_pcg_direction: bad argument (PCG_BETA 1..4)");
     const size_t nc = pc->ncell;
     float *g[3] = {g_vp, g_vs, g_rho};
-    /* descent + store_PCG_PSV: PCG_new = -g */
+    /* descent + store_PCG_PSV: PCG_new = -g.  Without g_vs (the acoustic
+     * store_PCG_AC.c, classes vp and rho) the vs class stays 0: every class
+     * has its own beta (PCG.c:21-78), so the others are those of a two-class
+     * run. */
     for (int c = 0; c < 3; c++)
-        for (size_t k = 0; k < nc; k++) pc->nw[c * nc + k] = -g[c][k];
+        for (size_t k = 0; k < nc; k++) pc->nw[c * nc + k] = g[c] ? -g[c][k] : 0.0f;
     /* PCG.c:21-78 */
     float beta[3];
     for (int c = 0; c < 3; c++) {
@@ -769,7 +802,7 @@ int dnfwi_pcg_direction(dnfwi_pcg *pc, int pcg_beta, float *g_vp, float *g_vs, f
         const float *n = pc->nw + c * nc;
         for (size_t k = 0; k < nc; k++) {
             d[k] = n[k] + beta[c] * d[k];
-            g[c][k] = -d[k];
+            if (g[c]) g[c][k] = -d[k];
         }
         memcpy(pc->old + c * nc, n, sizeof(float) * nc);
         if (beta_out) beta_out[c] = beta[c];

@@ -29,6 +29,13 @@ extern "C" {
 typedef struct dnfwi_lbfgs dnfwi_lbfgs;
 
 int dnfwi_lbfgs_create(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out);
+/* The acoustic problem's memory (src/AC/FWI_AC.c:269-283, NLBFGS_class = 2):
+ * vectors of 2*nx*ny floats [rho | vp] (src/AC/store_LBFGS_AC.c:63-150,
+ * src/AC/extract_LBFGS_AC.c:28-57).  dnfwi_lbfgs_direction and
+ * dnfwi_lbfgs_save then ignore the vs arguments (they may be NULL) and
+ * dnfwi_lbfgs_save writes the vp and rho files only
+ * (src/AC/extract_LBFGS_AC.c:60-178). */
+int dnfwi_lbfgs_create_ac(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out);
 void dnfwi_lbfgs_destroy(dnfwi_lbfgs *lb);
 /* zero_LBFGS (src/zero_LBFGS.c) and LBFGS_pointer = 1 (src/PSV/FWI_PSV.c:807-811):
  * the memory is cleared at the start of each workflow stage after the first. */
@@ -77,7 +84,10 @@ void dnfwi_pcg_reset(dnfwi_pcg *pc);
  * 3 Hestenes-Stiefel, 4 Dai-Yuan; beta = 0 when its denominator is 0;
  * dir = -g + beta dir), extract_PCG_PSV, and descent again: the gradients
  * g_vp, g_vs, g_rho are replaced by -dir, the direction the model update
- * subtracts.  beta_out (may be NULL) gets the three betas. */
+ * subtracts.  beta_out (may be NULL) gets the three betas.  g_vs may be
+ * NULL (the acoustic problem, src/AC/FWI_AC.c:683-706 with store_PCG_AC and
+ * extract_PCG_AC): the vs class then stays 0 and its beta is 0; the vp and
+ * rho classes, each with its own beta, are those of the two-class run. */
 int dnfwi_pcg_direction(dnfwi_pcg *pc, int pcg_beta, float *g_vp, float *g_vs, float *g_rho, float *beta_out);
 
 /* Source-time-function inversion of one shot (INV_STF 1: src/PSV/stf_psv.c:13-66
@@ -115,7 +125,10 @@ typedef struct dnfwi_update_par {
  * new = old - eps * direction with eps = eps_scale * max(model) /
  * max|direction| per parameter (times SCALERHO for rho; 0 before the
  * parameter's INV_*_ITER), cells leaving the limits or turning negative
- * keep the old value.  Returns EPSILON_u, the reference's "eps_true". */
+ * keep the old value.  Returns EPSILON_u, the reference's "eps_true".
+ * With vs = NULL (d_vs and vs_new are then not used) it is
+ * calc_mat_change_test_AC (src/AC/calc_mat_change_test_AC.c:9-189): vp and
+ * rho only, and the value returned is EPSILON, that routine's eps_true. */
 float dnfwi_model_update(const dnfwi_update_par *u, int iter, float eps_scale, int nx, int ny, const float *vp,
                          const float *vs, const float *rho, const float *d_vp, const float *d_vs, const float *d_rho,
                          float *vp_new, float *vs_new, float *rho_new);
@@ -144,7 +157,10 @@ typedef struct dnfwi_linesearch {
  * Test models are the model update of (vp, vs, rho) along d at the trial step.
  * `out` also carries epst and L2t from one iteration to the next, as the
  * reference's arrays do (a failed search leaves slot 3 as it was): zero it
- * before the first iteration and pass the same record every iteration. */
+ * before the first iteration and pass the same record every iteration.
+ * vs = NULL runs the acoustic search (src/AC/step_length_est_ac.c:37-181,
+ * the same steps with calc_mat_change_test_AC): the misfit callback then
+ * gets vs = NULL. */
 int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, float scalefac, int stepmax,
                       double L2t0, int nx, int ny, const float *vp, const float *vs, const float *rho,
                       const float *d_vp, const float *d_vs, const float *d_rho, dnfwi_misfit_fn misfit,
@@ -158,7 +174,8 @@ int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, flo
  * regularised by 0.005 * max(We); then g_vp /= We*C_vp*C_vp, g_vs /=
  * We*C_vs*C_vs (0 when C_vs = 0), g_rho /= We*C_rho*C_rho.  Ws, Wr, g_*:
  * x-major NX x NY (Wr only for EPRECOND 1); We (may be NULL) gets the
- * weights. */
+ * weights.  g_vs may be NULL (src/AC/grad_obj_ac.c:225-233: vp and rho
+ * gradients only). */
 int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws, const float *Wr, float c_vp,
                          float c_vs, float c_rho, float *g_vp, float *g_vs, float *g_rho, float *We);
 

@@ -9,9 +9,12 @@ Written from the reference's functions, one per function, keeping its
   - store_LBFGS / LBFGS / extract_LBFGS: src/PSV/store_LBFGS_PSV.c:37-205,
     src/LBFGS.c:26-95, src/PSV/extract_LBFGS_PSV.c:31-249, with the slot
     pointer of src/PSV/FWI_PSV.c:452-466 and dotp of src/dotp.c:12-40
-    (a float accumulator, i.e. a sequential single-precision sum);
+    (a float accumulator, i.e. a sequential single-precision sum); with
+    nclass 2 the acoustic src/AC/store_LBFGS_AC.c:37-152 and
+    src/AC/extract_LBFGS_AC.c:28-57 ([rho | vp] vectors);
   - model_mean: src/PSV/FWI_PSV.c:540-580;
-  - mat_change_test: src/PSV/calc_mat_change_test_PSV.c:10-251;
+  - mat_change_test: src/PSV/calc_mat_change_test_PSV.c:10-251; without vs
+    src/AC/calc_mat_change_test_AC.c:9-189;
   - opt_step: src/calc_opt_step.c:14-84 with src/solvelin.c:19-48;
   - step_length: src/PSV/step_length_est_psv.c:37-182;
   - rtm_adjoint_source: the MODE 2 branch of src/calc_res.c:416-446 and
@@ -54,11 +57,13 @@ def dotp(a, b):
 
 class LBFGS:
     """The reference's L-BFGS state: y, s of NLBFGS slots of 3*NX*NY floats
-    ([rho | vs | vp]), q, r, rho, alpha, the cyclic pointer, and the
-    previous model / normalised gradients (the JACOBIAN_*.old files)."""
+    ([rho | vs | vp]; nclass 2: 2*NX*NY floats [rho | vp], the acoustic
+    problem), q, r, rho, alpha, the cyclic pointer, and the previous model /
+    normalised gradients (the JACOBIAN_*.old files)."""
 
-    def __init__(self, nx, ny, nlbfgs):
-        self.n = 3 * nx * ny
+    def __init__(self, nx, ny, nlbfgs, nclass=3):
+        self.nclass = nclass
+        self.n = nclass * nx * ny
         self.nl = nlbfgs
         self.y = np.zeros((nlbfgs + 1, self.n), F32)   # row 0 unused: slots 1..NLBFGS
         self.s = np.zeros((nlbfgs + 1, self.n), F32)
@@ -84,11 +89,16 @@ class LBFGS:
         if self.pointer > self.nl:
             self.pointer = 1
         # store_LBFGS_PSV.c:37-89: waveconv = C * waveconv; gradp = waveconv
+        ac = self.nclass == 2
         w_vp = (c_vp * g_vp.astype(F32)).astype(F32)
-        w_vs = (c_vs * g_vs.astype(F32)).astype(F32)
+        w_vs = None if ac else (c_vs * g_vs.astype(F32)).astype(F32)
         w_rho = (c_rho * g_rho.astype(F32)).astype(F32)
-        gradp = np.concatenate([w_rho.ravel(), w_vs.ravel(), w_vp.ravel()])
-        model = np.concatenate([rho.ravel(), vs.ravel(), vp.ravel()]).astype(F32)
+        if ac:   # store_LBFGS_AC.c:63-150
+            gradp = np.concatenate([w_rho.ravel(), w_vp.ravel()])
+            model = np.concatenate([rho.ravel(), vp.ravel()]).astype(F32)
+        else:
+            gradp = np.concatenate([w_rho.ravel(), w_vs.ravel(), w_vp.ravel()])
+            model = np.concatenate([rho.ravel(), vs.ravel(), vp.ravel()]).astype(F32)
         if it > 1:
             # store_LBFGS_PSV.c:91-205
             p = self.pointer
@@ -114,9 +124,9 @@ class LBFGS:
             # extract_LBFGS_PSV.c:31-78
             nc = g_vp.size
             d_rho = self.r[:nc].reshape(g_rho.shape)
-            d_vs = self.r[nc:2 * nc].reshape(g_vs.shape)
-            d_vp = self.r[2 * nc:].reshape(g_vp.shape)
-            return ((d_vp * c_vp).astype(F32), (d_vs * c_vs).astype(F32), (d_rho * c_rho).astype(F32))
+            d_vp = self.r[(self.nclass - 1) * nc:].reshape(g_vp.shape)
+            d_vs = None if ac else (self.r[nc:2 * nc].reshape(g_vs.shape) * c_vs).astype(F32)
+            return ((d_vp * c_vp).astype(F32), d_vs, (d_rho * c_rho).astype(F32))
         return w_vp, w_vs, w_rho
 
 
@@ -136,7 +146,10 @@ def mat_change_test(par, it, eps_scale, vp, vs, rho, d_vp, d_vs, d_rho):
     """calc_mat_change_test_PSV for one test or final step.  par: dict with
     vpupperlim, vplowerlim, vsupperlim, vslowerlim, rhoupperlim,
     rholowerlim, scalerho, inv_vp_iter, inv_vs_iter, inv_rho_iter, invmat1.
-    Returns (vp_new, vs_new, rho_new, eps_true)."""
+    Returns (vp_new, vs_new, rho_new, eps_true).  vs None:
+    calc_mat_change_test_AC (vp and rho; vs_new None, eps_true = EPSILON)."""
+    if vs is None:
+        return _mat_change_test_ac(par, it, eps_scale, vp, rho, d_vp, d_rho)
     eps_scale = F32(eps_scale)
     pimax, gmax = _maxima(vp, d_vp)
     umax, gmax_u = _maxima(vs, d_vs)
@@ -167,6 +180,34 @@ def mat_change_test(par, it, eps_scale, vp, vs, rho, d_vp, d_vs, d_rho):
     u1 = np.where(u1 < 0, vs, u1)
     r1 = np.where(r1 < 0, rho, r1)
     return p1.astype(F32), u1.astype(F32), r1.astype(F32), eps_u
+
+
+def _mat_change_test_ac(par, it, eps_scale, vp, rho, d_vp, d_rho):
+    """calc_mat_change_test_AC.c:33-160: the steps of vp (EPSILON) and rho
+    (EPSILON_rho, times SCALERHO), the limits for INVMAT1 1, no negative
+    values; returns EPSILON as eps_true (:103)."""
+    eps_scale = F32(eps_scale)
+    pimax, gmax = _maxima(vp, d_vp)
+    rhomax, gmax_rho = _maxima(rho, d_rho)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        eps = F32(eps_scale * F32(pimax / gmax))
+        eps_rho = F32(F32(eps_scale * F32(rhomax / gmax_rho)) * F32(par["scalerho"]))
+    if it < par["inv_vp_iter"]:
+        eps = F32(0.0)
+    if it < par["inv_rho_iter"]:
+        eps_rho = F32(0.0)
+    if par["invmat1"] not in (1, 3):
+        return vp.copy(), None, rho.copy(), eps
+    p1 = (vp - (eps * d_vp).astype(F32)).astype(F32)
+    r1 = (rho - (eps_rho * d_rho).astype(F32)).astype(F32)
+    if par["invmat1"] == 1:
+        p1 = np.where(p1 < F32(par["vplowerlim"]), vp, p1)
+        p1 = np.where(p1 > F32(par["vpupperlim"]), vp, p1)
+        r1 = np.where(r1 < F32(par["rholowerlim"]), rho, r1)
+        r1 = np.where(r1 > F32(par["rhoupperlim"]), rho, r1)
+    p1 = np.where(p1 < 0, vp, p1)
+    r1 = np.where(r1 < 0, rho, r1)
+    return p1.astype(F32), None, r1.astype(F32), eps
 
 
 def opt_step(L2t, epst):
@@ -204,7 +245,8 @@ def opt_step(L2t, epst):
 
 def step_length(par, it, eps_scale0, scalefac, stepmax, L2t0, vp, vs, rho, d_vp, d_vs, d_rho, misfit,
                 carry=None):
-    """step_length_est_psv.  misfit(vp, vs, rho, itest) -> float.  carry:
+    """step_length_est_psv (vs None: step_length_est_ac, the misfit gets vs
+    None).  misfit(vp, vs, rho, itest) -> float.  carry:
     (epst[3], L2t[3]) of the previous iteration (the reference's arrays
     persist), zeros at the first.  Returns a dict like dnfwi_linesearch."""
     epst1 = [0.0, 0.0, 0.0, 0.0]
@@ -279,7 +321,8 @@ def rtm_adjoint_source(data, syn, dt, grad_form=1, testshot=1):
 
 
 def energy_precond(eprecond, dh, Ws, Wr, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho):
-    """Returns (We, g_vp, g_vs, g_rho) as new float32 arrays; (nx, ny) inputs."""
+    """Returns (We, g_vp, g_vs, g_rho) as new float32 arrays; (nx, ny) inputs.
+    g_vs None: the acoustic grad_obj_ac.c:225-233 (g_vs stays None)."""
     import math
     nx, ny = Ws.shape
     dh = F32(dh)
@@ -300,9 +343,11 @@ def energy_precond(eprecond, dh, Ws, Wr, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho):
     We = (We + F32(F32(0.005) * maxwe)).astype(F32)
     c_vp, c_vs, c_rho = F32(c_vp), F32(c_vs), F32(c_rho)
     gv = (g_vp / ((We * c_vp).astype(F32) * c_vp).astype(F32)).astype(F32)
-    gs = (g_vs / ((We * c_vs).astype(F32) * c_vs).astype(F32)).astype(F32)
-    if c_vs == 0.0:
-        gs[...] = 0.0
+    gs = None
+    if g_vs is not None:
+        gs = (g_vs / ((We * c_vs).astype(F32) * c_vs).astype(F32)).astype(F32)
+        if c_vs == 0.0:
+            gs[...] = 0.0
     gr = (g_rho / ((We * c_rho).astype(F32) * c_rho).astype(F32)).astype(F32)
     return We, gv, gs, gr
 
@@ -501,7 +546,12 @@ class PCG:
         self.dir[...] = 0
 
     def direction(self, pcg_beta, g_vp, g_vs, g_rho):
+        """g_vs None: the acoustic store_PCG_AC / extract_PCG_AC (vp and rho
+        classes, the vs class left 0 and None on return)."""
         shape = g_vp.shape
+        ac = g_vs is None
+        if ac:
+            return self._direction_ac(pcg_beta, g_vp, g_rho)
         new = np.concatenate([-g_vp.ravel(), -g_vs.ravel(), -g_rho.ravel()]).astype(F32)
         betas = []
         for k in range(3):
@@ -528,6 +578,37 @@ class PCG:
         d = -self.dir
         return (d[:self.n].reshape(shape), d[self.n:2 * self.n].reshape(shape), d[2 * self.n:].reshape(shape),
                 np.array(betas, F32))
+
+    def _direction_ac(self, pcg_beta, g_vp, g_rho):
+        """PCG.c with PCG_class 2 over [vp | rho] (store_PCG_AC.c:11-44),
+        kept in the vp and rho thirds of the state."""
+        shape = g_vp.shape
+        n = self.n
+        sl = [slice(0, n), slice(2 * n, 3 * n)]
+        new = [(-g_vp.ravel()).astype(F32), (-g_rho.ravel()).astype(F32)]
+        betas = []
+        for k in range(2):
+            nw, od, dr = new[k], self.old[sl[k]], self.dir[sl[k]]
+            if pcg_beta == 1:
+                tz, tn = (nw * nw).astype(F32), (od * od).astype(F32)
+            elif pcg_beta == 2:
+                tz, tn = (nw * (nw - od).astype(F32)).astype(F32), (od * od).astype(F32)
+            elif pcg_beta == 3:
+                tz, tn = ((-nw) * (nw - od).astype(F32)).astype(F32), (dr * (nw - od).astype(F32)).astype(F32)
+            else:
+                tz, tn = ((-nw) * nw).astype(F32), (dr * (nw - od).astype(F32)).astype(F32)
+            bz = np.add.accumulate(tz, dtype=F32)[-1]
+            bn = np.add.accumulate(tn, dtype=F32)[-1]
+            b = F32(0.0)
+            if bn != 0:
+                b = F32(bz / bn)
+            if b < 0.0 and pcg_beta == 2:
+                b = F32(0.0)
+            betas.append(b)
+            self.dir[sl[k]] = (nw + (b * dr).astype(F32)).astype(F32)
+            self.old[sl[k]] = nw
+        return ((-self.dir[sl[0]]).reshape(shape), None, (-self.dir[sl[1]]).reshape(shape),
+                np.array([betas[0], F32(0.0), betas[1]], F32))
 
 
 def taper_vert(nx, ny, dh, gradt, srcpos, recpos, g):

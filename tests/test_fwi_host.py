@@ -691,3 +691,143 @@ def test_gauss_filter_var_pins():
     np.testing.assert_allclose(dnfwi.gauss_filter_var(c, vel, 10.0, 0.5, 0.2, 10.0), c, rtol=2e-6)
     r = np.random.default_rng(2).standard_normal((nx, ny)).astype(np.float32)
     assert np.array_equal(dnfwi.gauss_filter_var(r, vel, 10.0, 0.5, 0.0, 10.0), r)
+
+
+# ---- the acoustic problem: vp and rho only (src/AC/FWI_AC.c) ----
+
+def run_iterations_ac(nl, iters, seed):
+    """One gradient/model sequence through the two-class product memory, the
+    two-class oracle, and the three-class oracle with vs = 0 everywhere."""
+    rng = np.random.default_rng(seed)
+    lb, ob, o3 = dnfwi.LBFGS(NX, NY, nl, ac=True), fo.LBFGS(NX, NY, nl, nclass=2), fo.LBFGS(NX, NY, nl)
+    vp, _, rho = models(rng)
+    zero = np.zeros((NX, NY), F32)
+    c = (fo.model_mean(vp), F32(0.0), fo.model_mean(rho))
+    out = []
+    for it in range(1, iters + 1):
+        g = [rnd(rng, -1e-3, 1e-3), None, rnd(rng, -1e-3, 1e-3)]
+        d_prod = lb.direction(it, *c, *g, vp, None, rho)
+        d_orc = ob.direction(it, *c, *g, vp, None, rho)
+        d_3 = o3.direction(it, *c, g[0], zero, g[2], vp, zero, rho)
+        out.append((it, g, d_prod, d_orc, d_3, lb.pointer(), ob.pointer))
+        vp = (vp - F32(0.5) * d_prod[0] / np.abs(d_prod[0]).max()).astype(F32)
+        rho = (rho - F32(0.2) * d_prod[2] / np.abs(d_prod[2]).max()).astype(F32)
+    lb.close()
+    return out
+
+
+def test_lbfgs_ac_matches_oracle_and_three_class_run():
+    """store_LBFGS_AC / extract_LBFGS_AC ([rho | vp] vectors): the product
+    equals the two-class oracle bit for bit, and the two-class oracle equals
+    the three-class one (pinned against the BFGS matrix recursion above)
+    with a zero vs class, whose products add nothing to the dot products."""
+    for it, g, dp, do, d3, pp, po in run_iterations_ac(nl=3, iters=7, seed=21):
+        assert pp == po
+        assert dp[1] is None and do[1] is None
+        for k in (0, 2):
+            assert np.array_equal(dp[k], do[k]), (it, k)
+            assert np.array_equal(do[k], d3[k]), (it, k)
+        assert not d3[1].any()
+
+
+def test_lbfgs_ac_files(tmp_path):
+    """extract_LBFGS_AC.c:60-178: _p_vp, _p, _c, _p_mrho, _p_rho, _c_rho
+    (model, normalised gradient, direction of vp and rho); no vs files."""
+    rng = np.random.default_rng(23)
+    lb, ob = dnfwi.LBFGS(NX, NY, 2, ac=True), fo.LBFGS(NX, NY, 2, nclass=2)
+    vp, _, rho = models(rng)
+    c = (fo.model_mean(vp), F32(0.0), fo.model_mean(rho))
+    jac = str(tmp_path / "jac")
+    nc = NX * NY
+    for it in (1, 2, 3):
+        g = [rnd(rng, -1e-3, 1e-3), None, rnd(rng, -1e-3, 1e-3)]
+        d = lb.direction(it, *c, *g, vp, None, rho)
+        do = ob.direction(it, *c, *g, vp, None, rho)
+        lb.save(jac, d[0], None, d[2])
+        for names, slot, dd in ((("p_vp", "p", "c"), 1, do[0]), (("p_mrho", "p_rho", "c_rho"), 0, do[2])):
+            want = (ob.mold[slot * nc:(slot + 1) * nc], ob.gold[slot * nc:(slot + 1) * nc], dd.ravel())
+            for n, w in zip(names, want):
+                for sfx in (".old", ".old.0.0"):
+                    assert np.array_equal(np.fromfile(f"{jac}_{n}{sfx}", np.float32), w), (it, n, sfx)
+        assert np.array_equal(np.fromfile(f"{jac}_p_vp.old", np.float32), vp.ravel())
+        assert np.array_equal(np.fromfile(f"{jac}_p.old", np.float32), (F32(c[0]) * g[0]).ravel())
+        for n in ("p_vs", "p_u", "c_u"):
+            assert not (tmp_path / f"jac_{n}.old").exists()
+        vp = (vp - F32(0.5) * d[0] / np.abs(d[0]).max()).astype(F32)
+        rho = (rho - F32(0.2) * d[2] / np.abs(d[2]).max()).astype(F32)
+    lb.close()
+
+
+def test_model_update_ac():
+    """calc_mat_change_test_AC: the product equals the oracle, and the vp and
+    rho of the three-class update (whose vs part is independent of them);
+    eps_true is the vp step EPSILON (calc_mat_change_test_AC.c:103)."""
+    rng = np.random.default_rng(25)
+    for trial in range(5):
+        vp, vs, rho = models(rng)
+        d = [rnd(rng) for _ in range(3)]
+        par = dict(PAR, inv_rho_iter=2 if trial == 2 else 0, invmat1=3 if trial == 3 else 1)
+        eps = [0.01, 0.5, 0.05, 0.3, 0.0][trial]
+        got = dnfwi.model_update(dnfwi.update_par(**par), 1, eps, vp, None, rho, d[0], None, d[2])
+        ref = fo.mat_change_test(par, 1, eps, vp, None, rho, d[0], None, d[2])
+        full = fo.mat_change_test(par, 1, eps, vp, vs, rho, *d)
+        assert got[1] is None and ref[1] is None
+        for k in (0, 2):
+            assert np.array_equal(got[k], ref[k]), trial
+            assert np.array_equal(ref[k], full[k]), trial
+        assert got[3] == float(ref[3])
+    one = np.ones((NX, NY), F32)
+    nv, _, nr, eps_true = dnfwi.model_update(dnfwi.update_par(**PAR), 1, 0.01, vp, None, rho, one, None, one)
+    assert np.array_equal(nv, (vp - F32(F32(0.01) * vp.max())).astype(F32))
+    assert eps_true == float(F32(F32(0.01) * vp.max()))
+
+
+def test_step_length_ac_on_parabolic_misfit():
+    """step_length_est_ac: the misfit gets vs None; on a misfit parabolic in
+    the vp step the search returns the parabola's minimiser, and the product
+    equals the oracle's search."""
+    rng = np.random.default_rng(27)
+    vp, _, rho = models(rng)
+    d_vp, d_rho = np.ones((NX, NY), F32), np.ones((NX, NY), F32)
+    vmax = float(vp.max())
+    best = 0.013
+
+    def misfit(tv, ts, tr, itest):
+        assert ts is None
+        e = (float(vp[0, 0]) - float(tv[0, 0])) / vmax
+        return 1.0 + (e - best) ** 2
+
+    par = dnfwi.update_par(**dict(PAR, vplowerlim=0.0))
+    st = dnfwi.step_length(par, 1, 0.01, 4.0, 6, 1.0 + best ** 2, vp, None, rho, d_vp, None, d_rho, misfit)
+    ref = fo.step_length(dict(PAR, vplowerlim=0.0), 1, 0.01, 4.0, 6, 1.0 + best ** 2, vp, None, rho, d_vp, None,
+                         d_rho, misfit)
+    assert st.eps_scale == float(ref["eps_scale"]) and st.nevals == ref["nevals"]
+    assert abs(st.eps_scale - best) < 1e-3 * best
+
+
+def test_energy_precond_and_pcg_ac():
+    """grad_obj_ac.c:225-233 and PCG with PCG_class 2: without a vs gradient
+    the vp and rho results equal the three-class ones."""
+    rng = np.random.default_rng(29)
+    Ws = rnd(rng, 0.1, 2.0)
+    g = [rnd(rng) for _ in range(3)]
+    c = (F32(2500.0), F32(1400.0), F32(2100.0))
+    We, gv, gs, gr = dnfwi.energy_precond(3, 10.0, Ws, None, *c, g[0], None, g[2])
+    We3, gv3, _, gr3 = dnfwi.energy_precond(3, 10.0, Ws, None, *c, *g)
+    oW, ov, os_, orr = fo.energy_precond(3, 10.0, Ws, None, *c, g[0], None, g[2])
+    assert gs is None and os_ is None
+    assert np.array_equal(gv, gv3) and np.array_equal(gr, gr3) and np.array_equal(We, We3)
+    assert np.array_equal(gv, ov) and np.array_equal(gr, orr)
+    for pcg_beta in (1, 2, 3, 4):
+        p2, p3, po = dnfwi.PCG(NX, NY), dnfwi.PCG(NX, NY), fo.PCG(NX, NY)
+        for it in range(4):
+            gg = [rnd(rng) for _ in range(3)]
+            a = p2.direction(pcg_beta, gg[0], None, gg[2])
+            b = p3.direction(pcg_beta, gg[0], np.zeros((NX, NY), F32), gg[2])
+            o = po.direction(pcg_beta, gg[0], None, gg[2])
+            assert a[1] is None and o[1] is None
+            for k in (0, 2):
+                assert np.array_equal(a[k], b[k]) and np.array_equal(a[k], o[k]), (pcg_beta, it, k)
+            assert a[3][0] == b[3][0] == o[3][0] and a[3][2] == b[3][2] == o[3][2] and a[3][1] == 0.0
+        p2.close()
+        p3.close()

@@ -385,3 +385,168 @@ def test_cli_rtm_image_equals_the_library(tmp_path):
     got = np.fromfile(os.path.join(tmp, "jac/j_P_image"), np.float32).reshape(nx, ny)
     assert np.array_equal(got, total)
     assert np.abs(total).max() > 0
+
+
+@pytest.mark.parametrize("variant", ["plain", "precond_pcg", "spatfilter4", "pressure"])
+def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
+    """MODE 1 with PHYSICS 2 (FWI_AC.c, grad_obj_ac.c, step_length_est_ac.c,
+    calc_mat_change_test_AC.c): three iterations of the executable against
+    the same iterations in Python, with the library's acoustic gradients
+    (ass_gradAC, their parity with the FP32 oracle tested above) and the
+    numpy restatement's two-class L-BFGS / PCG, step search and model update
+    of vp and rho.  The vp and rho models after every iteration, the final
+    INV_MODELFILE_vp/_rho.bin, the misfit log and the L-BFGS files equal the
+    Python run's bit for bit; no vs model or vs file is written; the misfit
+    falls.  plain: L-BFGS, vx and vy residuals.  precond_pcg: EPRECOND 3
+    (energy of vx and vy, ac.c:251-253) and the horizontal taper with PCG
+    (PCG_BETA 2).  spatfilter4: the velocity-dependent Gaussian with the vp
+    model (FWI_AC.c:676-677).  pressure: QUELLTYPB 4 with SEISMO 2 (the
+    pressure residual as the adjoint source)."""
+    import fwi_oracle as fo
+    from test_gpu_cli import _files, _run
+    itermax = 3
+    pcg = variant == "precond_pcg"
+    precond = pcg
+    spat4 = variant == "spatfilter4"
+    qb, seismo = (4, 2) if variant == "pressure" else (1, 1)
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
+    nx, ny, dh, dt, nt = 96, 64, 10.0, 1e-3, 400
+    vp0, vs0, rho0 = synth.layered_model(nx, ny, dh, seed=5, sigma=3.0)
+    vt, rt = vp0.copy(), rho0.copy()
+    vt[40:56, 10:20] *= 1.08
+    rt[40:56, 10:20] *= 1.05
+    sources = [(300.0, 30.0, 15.0, 1.0, 1), (650.0, 30.0, 15.0, 1.0, 1)]
+    receivers = [(40.0 + 20.0 * k, 20.0) for k in range(40)]
+    _files(tmp, nx, ny, dh, (vt, vs0, rt), sources, receivers, mname="model/true")
+    inpfile.write_inp(os.path.join(tmp, "true.inp"), PHYSICS=2, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/true", SEIS_FILE_VX="obs/d_x.su", SEIS_FILE_VY="obs/d_y.su",
+                      SEIS_FILE_P="obs/d_p.su", SEISMO=seismo)
+    _run(tmp, "true.inp")
+    _files(tmp, nx, ny, dh, (vp0, vs0, rho0), sources, receivers, mname="model/start")
+    inpfile.write_inp(os.path.join(tmp, "fwi.inp"), PHYSICS=2, MODE=1, QUELLTYPB=qb, SEISMO=seismo, NX=nx, NY=ny,
+                      DH=dh, DT=dt, TIME=nt * dt, MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1",
+                      ITERMAX=itermax, INV_MOD_OUT=1, MISFIT_LOG_FILE="misfit.dat",
+                      SWS_TAPER_GRAD_HOR=int(precond), GRADT="3, 12, 60, 62", EXP_TAPER_GRAD_HOR=2.0,
+                      GRAD_METHOD=1 if pcg else 2, PCG_BETA=2)
+    fce = 10.0 if spat4 else 0.0
+    inpfile.write_workflow(os.path.join(tmp, "wf.inp"), fc_end=fce, eprecond=3 if precond else 0,
+                           spatfilter=4 if spat4 else 0, wd_damp=0.5, wd_damp1=0.3)
+    out = _run(tmp, "fwi.inp", "wf.inp")
+    assert "rho_avg" in out and "Vs_avg" not in out
+
+    par = dnio.read_par(os.path.join(tmp, "fwi.inp"))
+    cwd = os.getcwd()
+    os.chdir(tmp)
+    try:
+        src, ts = dnio.read_sources(par.get("source_file"))
+        rec = dnio.read_receivers(par)
+    finally:
+        os.chdir(cwd)
+    shots = []
+    for ishot in (1, 2):
+        loc = dnio.split_sources(par, src, ishot - 1, 1)
+        shots.append((loc, dnio.wavelet(par, loc, ishot, ts)))
+    p = dnhip.Params(nx=nx, ny=ny, nt=par.nt, dh=dh, dt=par.dt, fdorder=par.fdorder, maxrelerror=par.maxrelerror,
+                     fw=par.fw, free_surf=par.free_surf, seismo=seismo, ts=ts, physics=2, mode=1, dtinv=2,
+                     grad_form=par.grad_form, quelltypb=qb, damping=par.damping, fpml=par.fpml,
+                     npower=par.npower, k_max_pml=par.k_max_pml)
+    st = dnio.read_stage(os.path.join(tmp, "wf.inp"), 1)
+    upd = dict(vpupperlim=par.vpupperlim, vplowerlim=par.vplowerlim, vsupperlim=par.vsupperlim,
+               vslowerlim=par.vslowerlim, rhoupperlim=par.rhoupperlim, rholowerlim=par.rholowerlim,
+               scalerho=st.scalerho, inv_vp_iter=st.inv_vp_iter, inv_vs_iter=st.inv_vs_iter,
+               inv_rho_iter=st.inv_rho_iter, invmat1=par.invmat1)
+    comps = "p" if qb == 4 else "xy"
+    obs = {i: [dnio.read_su(os.path.join(tmp, f"obs/d_{c}.su.shot{i}"), rec.shape[1], nt) for c in comps]
+           for i in (1, 2)}
+    g = dnhip.Context(p)
+    if precond:
+        g.set_energy_weights(3)
+
+    def shot(i, vp, rho):
+        g.set_model(vp, None, rho)
+        g.set_sources(*shots[i - 1])
+        g.set_receivers(rec[:2])
+        g.forward()
+
+    def residuals(i):
+        if qb == 4:
+            return g.residuals(None, None, obs[i][0], 1)
+        return g.residuals(obs[i][0], obs[i][1], None, 1)
+
+    def misfit(tv, ts_, tr, itest):
+        assert ts_ is None
+        l2 = 0.0
+        for i in (1, 2):
+            shot(i, tv, tr)
+            l2 += residuals(i)
+        return l2
+
+    vp, rho = vp0.copy(), rho0.copy()
+    lb = fo.PCG(nx, ny) if pcg else fo.LBFGS(nx, ny, par.nlbfgs, nclass=2)
+    c = (fo.model_mean(vp), np.float32(0.0), fo.model_mean(rho))
+    carry = ([0.0, 0.0, 0.0], [0.0, 0.0, 0.0])
+    lines, l2s = [], []
+    for it in range(1, itermax + 1):
+        grads = [np.zeros((nx, ny), np.float32), None, np.zeros((nx, ny), np.float32)]
+        l2sum = 0.0
+        for i in (1, 2):
+            shot(i, vp, rho)
+            l2sum += residuals(i)
+            g.adjoint()
+            g.assemble()
+            gs = [g.gradient(3), None, g.gradient(5)]
+            if precond:
+                _, gv, _, gr = fo.energy_precond(3, par.dh, g.energy(0), None, *c, gs[0], None, gs[2])
+                gs = [gv, None, gr]
+            for k in (0, 2):
+                grads[k] += gs[k]
+        if spat4:
+            grads = [fo.gauss_filter_var(a, vp, par.dh, 0.5, 0.3, fce) if a is not None else None for a in grads]
+        if precond:
+            coeff = fo.taper_hor(nx, ny, par.dh, 3, 12, 2.0)
+            grads = [(a * coeff).astype(np.float32) if a is not None else None for a in grads]
+        l2s.append(l2sum)
+        d = lb.direction(2, *grads)[:3] if pcg else lb.direction(it, *c, *grads, vp, None, rho)
+        assert d[1] is None
+        if not pcg:
+            saved = (lb.mold.copy(), lb.gold.copy(), [None if a is None else a.copy() for a in d])
+        ls = fo.step_length(upd, it, par.eps_scale, par.scalefac, par.stepmax, l2sum, vp, None, rho, *d, misfit,
+                            carry=carry)
+        carry = (ls["epst"], ls["L2t"])
+        eps = ls["eps_scale"]
+        if it > par.min_iter and ls["step1"] == 0:
+            eps = np.float32(0.0)
+        lines.append("%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %d " % (eps, *ls["epst"], *ls["L2t"], l2sum, 1))
+        vp, _, rho, _ = fo.mat_change_test(upd, it, eps, vp, None, rho, *d)
+        print(f"[acoustic fwi] iteration {it}: misfit {l2sum:.6e}, step {float(eps):.4e}, "
+              f"{ls['nevals']} test forward runs", flush=True)
+        if it > par.min_iter and ((it > 2 and abs((l2s[it - 3] - l2s[it - 1]) / l2s[it - 3]) <= st.pro)
+                                  or ls["step3"] == 1):
+            break
+        for name, a in (("vp", vp), ("rho", rho)):
+            got = np.fromfile(os.path.join(tmp, f"model/inv_{name}_stage_1_it_{it}.bin"), np.float32)
+            assert np.array_equal(got.reshape(nx, ny), a), (it, name)
+        assert not os.path.exists(os.path.join(tmp, f"model/inv_vs_stage_1_it_{it}.bin"))
+    g.close()
+    for name, a in (("vp", vp), ("rho", rho)):
+        got = np.fromfile(os.path.join(tmp, f"model/inv_{name}.bin"), np.float32).reshape(nx, ny)
+        assert np.array_equal(got, a), name
+    assert not os.path.exists(os.path.join(tmp, "model/inv_vs.bin"))
+    assert not os.path.exists(os.path.join(tmp, "jac/j_grad_vs"))
+    log = open(os.path.join(tmp, "misfit.dat")).read().splitlines()
+    assert log == lines
+    jac = os.path.join(tmp, "jac/j")
+    if pcg:
+        assert not os.path.exists(f"{jac}_p.old")
+    else:
+        mold, gold, dd = saved
+        nc = nx * ny
+        for names, slot, dk in ((("p_vp", "p", "c"), 1, dd[0]), (("p_mrho", "p_rho", "c_rho"), 0, dd[2])):
+            want = (mold[slot * nc:(slot + 1) * nc], gold[slot * nc:(slot + 1) * nc], dk.ravel())
+            for n, w in zip(names, want):
+                assert np.array_equal(np.fromfile(f"{jac}_{n}.old", np.float32), w), n
+        for n in ("p_vs", "p_u", "c_u"):
+            assert not os.path.exists(f"{jac}_{n}.old"), n
+    assert l2s[-1] < l2s[0]
+    assert not np.array_equal(vp, vp0)

@@ -299,11 +299,11 @@ def test_cli_refuses_what_is_not_built(tmp_path):
     (tmp_path / "shv.inp").write_text(text)
     r = _run_cli(str(tmp_path / "shv.inp"))
     assert r.returncode == 1 and "SH" in r.stderr
-    # the acoustic loop is built for forward modelling only
-    text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 2").replace("_(MODE) = 0", "_(MODE) = 1")
-    (tmp_path / "ac.inp").write_text(text)
-    r = _run_cli(str(tmp_path / "ac.inp"))
-    assert r.returncode == 1 and "MODE" in r.stderr
+    # the reference has no VTI FWI (physics_VTI.c comments it out)
+    text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 3").replace("_(MODE) = 0", "_(MODE) = 1")
+    (tmp_path / "vti.inp").write_text(text)
+    r = _run_cli(str(tmp_path / "vti.inp"), os.path.join(GOLD, "FWI_workflow_marmousi.inp"))
+    assert r.returncode == 1 and "VTI/TTI" in r.stderr
     # the Marmousi example (filtered spike source, QUELLART 6) passes the
     # checks and stops at its source file, which the fixtures do not hold
     r = _run_cli(src)
