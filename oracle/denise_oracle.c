@@ -1441,6 +1441,11 @@ static void store_snapshot(orc_ctx *c, int k)
             size_t o = base + (size_t)(i - 1) * NY + (j - 1);
             c->fwd[0][o] = AT(c->vxp1, j, i);
             c->fwd[1][o] = AT(c->vyp1, j, i);
+            if (c->physics == 5) { /* sh.c:150-172: vzp1, and sxz / syz or uzx / uz */
+                c->fwd[2][o] = AT(c->ux, j, i);
+                c->fwd[3][o] = AT(c->uy, j, i);
+                continue;
+            }
             if (c->P.grad_form == 1) {
                 c->fwd[2][o] = AT(c->sxx, j, i);
                 c->fwd[3][o] = AT(c->syy, j, i);
@@ -1465,6 +1470,12 @@ static void correlate(orc_ctx *c, int k)
             size_t o = base + (size_t)(i - 1) * NY + (j - 1);
             real frx = c->fwd[0][o], fry = c->fwd[1][o], fx = c->fwd[2][o], fy = c->fwd[3][o], fu = c->fwd[4][o];
             real axx = AT(c->sxx, j, i), ayy = AT(c->syy, j, i), axy = AT(c->sxy, j, i);
+            if (c->physics == 5) { /* sh.c:186-223: density and shear-modulus terms */
+                AT(c->wconv_rho, j, i) += AT(c->vxp1, j, i) * frx;
+                real muss = P->invmat1 == 3 ? AT(c->u, j, i) : AT(c->rho, j, i) * AT(c->u, j, i) * AT(c->u, j, i);
+                if (muss > 0.0) AT(c->wconv_u, j, i) += (fy * ayy) + (fx * axx);
+                continue;
+            }
             AT(c->wconv_rho, j, i) += (AT(c->vxp1, j, i) * frx) + (AT(c->vyp1, j, i) * fry);
             if (c->physics == 2) { /* ac.c:260-276: forward p (or ux) times adjoint p */
                 AT(c->wconv, j, i) += fx * axx;
@@ -1506,8 +1517,12 @@ static void correlate(orc_ctx *c, int k)
  * ---------------------------------------------------------------------- */
 
 /* src/SH/update_v_PML_SH.c:11-602 (FD8 block :294-378, forces :556-568):
- * vz += DT rhoi (sxz_x + syz_y) / DH; QUELLTYP 1 adds the signal to vz */
-static void update_v_sh(orc_ctx *c, int nt)
+ * vz += DT rhoi (sxz_x + syz_y) / DH; QUELLTYP 1 adds the signal to vz.
+ * Before the update the gradient field vzp1 (kept in vxp1, :341-372):
+ * forward (sw 0) rhoi (sxz_x + syz_y) / DH; adjoint (sw 1) vzp1 + vz DT
+ * (GRAD_FORM 1) or vz (GRAD_FORM 2).  The adjoint run adds the vz residual
+ * at the receivers for QUELLTYPB 1 (:571-583). */
+static void update_v_sh(orc_ctx *c, int nt, int sw)
 {
     const orc_params *P = &c->P;
     const int NX = P->nx, NY = P->ny, FW = P->fw, fdoh = c->fdoh;
@@ -1545,19 +1560,35 @@ static void update_v_sh(orc_ctx *c, int nt)
                 *ps = pm[PML_BY][h1] * *ps + pm[PML_AY][h1] * syz_y;
                 syz_y = syz_y / pm[PML_KY][h1] + *ps;
             }
+            if (sw == 0)
+                AT(c->vxp1, j, i) = AT(c->rip, j, i) * (sxz_x + syz_y) / DH;
+            else if (P->grad_form == 1)
+                AT(c->vxp1, j, i) += AT(c->vx, j, i) * DT;
+            else
+                AT(c->vxp1, j, i) = AT(c->vx, j, i);
             AT(c->vx, j, i) += DT * AT(c->rip, j, i) * (sxz_x + syz_y) / DH;
         }
     }
-    for (int l = 0; l < c->nsrc; l++) {
-        int i = (int)c->srcpos[0 * c->nsrc + l], j = (int)c->srcpos[1 * c->nsrc + l];
-        int type = (int)c->srcpos[7 * c->nsrc + l];
-        c->qtype = type; /* update_v_PML_SH.c:564 sets the global QUELLTYP */
-        if (type == 1) AT(c->vx, j, i) += c->signals[(size_t)l * P->nt + (nt - 1)];
+    if (sw == 0) {
+        for (int l = 0; l < c->nsrc; l++) {
+            int i = (int)c->srcpos[0 * c->nsrc + l], j = (int)c->srcpos[1 * c->nsrc + l];
+            int type = (int)c->srcpos[7 * c->nsrc + l];
+            c->qtype = type; /* update_v_PML_SH.c:564 sets the global QUELLTYP */
+            if (type == 1) AT(c->vx, j, i) += c->signals[(size_t)l * P->nt + (nt - 1)];
+        }
+    } else if (P->quelltypb == 1) {
+        for (int l = 0; l < c->ntr; l++) {
+            int i = c->recpos[l], j = c->recpos[c->ntr + l];
+            AT(c->vx, j, i) += c->adj[1][(size_t)l * P->nt + (nt - 1)];
+        }
     }
 }
 
 /* src/SH/update_s_elastic_PML_SH.c:12-470 (FD8 block :237-306):
- * sxz += uip vzx, syz += ujp vzy (ADJ_SIGN = 1 in forward runs) */
+ * sxz += uip vzx, syz += ujp vzy.  ADJ_SIGN is 1 in every run: sh.c:92 sets
+ * it to -1 for the adjoint run, but update_s_elastic_PML_SH.c:38 sets it back
+ * to 1 before each use.  The gradient fields uzx, uz (kept in ux, uy) are the
+ * new sxz, syz (GRAD_FORM 1) or vzx, vzy (GRAD_FORM 2), :89-101. */
 static void update_s_sh(orc_ctx *c)
 {
     const orc_params *P = &c->P;
@@ -1600,6 +1631,13 @@ static void update_s_sh(orc_ctx *c)
             }
             AT(c->sxx, j, i) += AT(c->uip, j, i) * vzx;
             AT(c->syy, j, i) += AT(c->ujp, j, i) * vzy;
+            if (P->grad_form == 1) {
+                AT(c->uy, j, i) = AT(c->syy, j, i);
+                AT(c->ux, j, i) = AT(c->sxx, j, i);
+            } else {
+                AT(c->uy, j, i) = vzy;
+                AT(c->ux, j, i) = vzx;
+            }
         }
     }
 }
@@ -1622,7 +1660,7 @@ static void seismo_sh(orc_ctx *c, int nt)
 orc_ctx *orc_create_sh(const orc_params *p, const float *vs, const float *rho)
 {
     if (!p || !vs || !rho) { seterr("null argument"); return NULL; }
-    if (p->L != 0 || p->mode != 0) { seterr("SH runs are forward modelling (L = 0, MODE 0)"); return NULL; }
+    if (p->L != 0) { seterr("SH runs are elastic (L = 0)"); return NULL; }
     if (p->seismo > 1) { seterr("SH runs record vz (SEISMO 0 or 1)"); return NULL; }
     orc_ctx *c = ctx_new(p);
     if (!c) return NULL;
@@ -1670,9 +1708,9 @@ static void step_body(orc_ctx *c, int nt, int mode)
 {
     const orc_params *P = &c->P;
     if (c->physics == 5) { /* src/SH/sh.c:80-135 (no free-surface routine) */
-        update_v_sh(c, nt);
+        update_v_sh(c, nt, mode == 1 ? 1 : 0);
         update_s_sh(c);
-        if (P->seismo) seismo_sh(c, nt);
+        if (P->seismo && mode != 1) seismo_sh(c, nt);
         return;
     }
     if (c->physics == 2) { /* src/AC/ac.c:96-205 */
@@ -1810,6 +1848,10 @@ double orc_calc_res(orc_ctx *c, const float *obs_vx, const float *obs_vy, const 
     const int QB = c->P.quelltypb;
     double L2 = 0.0;
     const float wx = c->resw[0] != 0.0f ? c->resw[0] : 1.0f, wy = c->resw[1] != 0.0f ? c->resw[1] : 1.0f;
+    if (c->physics == 5) { /* src/SH/calc_res_SH.c:20-44: vz (component 1) for any QUELLTYPB, no weights */
+        if (QB && obs_vy) L2 = calc_res_comp(c, c->sec[1], obs_vy, c->adj[1], L2, testshot, 1.0f);
+        return L2;
+    }
     if ((QB == 1 || QB == 3 || QB == 5 || QB == 7) && obs_vx)
         L2 = calc_res_comp(c, c->sec[0], obs_vx, c->adj[0], L2, testshot, wx);
     if ((QB == 1 || QB == 2 || QB == 6 || QB == 7) && obs_vy)
@@ -1832,6 +1874,48 @@ int orc_assemble(orc_ctx *c)
 {
     const orc_params *P = &c->P;
     const real DT = (real)P->dt;
+    if (c->physics == 5) { /* src/SH/ass_gradSH.c:10-98 with INV_*_ITER 0: gvs, grho */
+        for (int i = 1; i <= P->nx; i++)
+            for (int j = 1; j <= P->ny; j++) {
+                real pu = AT(c->u, j, i), prho = AT(c->rho, j, i);
+                AT(c->wmu, j, i) = -DT * AT(c->wconv_u, j, i);
+                real gu = AT(c->wconv_u, j, i); /* kept where ass_gradSH.c writes nothing */
+                if (P->invmat1 == 1) {
+                    if (P->grad_form == 1) {
+                        if (pu > 0.0) {
+                            real vs3 = pu * pu * pu;
+                            gu = (real)((2.0 / prho * vs3) * AT(c->wmu, j, i));
+                        }
+                    } else {
+                        gu = (real)(2.0 * prho * pu * AT(c->wmu, j, i));
+                    }
+                } else {
+                    gu = AT(c->wmu, j, i);
+                }
+                AT(c->gvs, j, i) = gu;
+                AT(c->gvp, j, i) = 0;
+            }
+        for (int i = 1; i <= P->nx; i++)
+            for (int j = 1; j <= P->ny; j++) {
+                real pu = AT(c->u, j, i), prho = AT(c->rho, j, i);
+                AT(c->wrho_s, j, i) = -DT * AT(c->wconv_rho, j, i);
+                real gr = AT(c->wconv_rho, j, i);
+                if (P->invmat1 == 1) {
+                    if (P->grad_form == 1) {
+                        if (pu > 0.0) {
+                            real vs2rho2 = (real)(1.0 / (pu * pu * prho * prho));
+                            gr = (vs2rho2 * AT(c->wmu, j, i)) + AT(c->wrho_s, j, i);
+                        }
+                    } else {
+                        gr = (pu * pu * AT(c->wmu, j, i)) + AT(c->wrho_s, j, i);
+                    }
+                } else {
+                    gr = AT(c->wrho_s, j, i);
+                }
+                AT(c->grho, j, i) = gr;
+            }
+        return 0;
+    }
     if (c->physics == 2) { /* src/AC/ass_gradAC.c:12-93 with INV_*_ITER 0 */
         for (int i = 1; i <= P->nx; i++)
             for (int j = 1; j <= P->ny; j++) {

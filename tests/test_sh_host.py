@@ -25,6 +25,7 @@ import pytest
 
 import orc
 import synth
+from cases import rel_l2
 from dnhip import Params
 
 orc.set_threads(2 if os.environ.get("PYTEST_XDIST_WORKER") else 8)
@@ -220,3 +221,144 @@ def test_sh_refuses_what_the_loop_does_not_record():
     p = params(seismo=2)
     with pytest.raises(ValueError, match="SEISMO"):
         _sh(p, const(p.nx, p.ny, VS), const(p.nx, p.ny, RHO))
+
+
+# --------------------------------------------------------------------------
+# SH gradient (src/SH/sh.c:150-223, ass_gradSH.c)
+# --------------------------------------------------------------------------
+def _sh_gradient_setup(dtinv=1, invmat1=3, grad_form=1, nx=64, ny=56, nt=700, fc=8.0, recrow=14, zero_data=False):
+    """A force on vz in a model homogeneous around the source and receiver
+    row and smoothly stiffer below; observed vz from the same model with
+    +5% Gaussian anomalies of vs and rho.  misfit(model, grad) runs the
+    FP64 oracle: forward, the vz residual (calc_res_SH.c), and with grad the
+    adjoint run and ass_gradSH; the model is (vs or mu, rho)."""
+    from test_oracle_pins import _bump
+    p = params(nx=nx, ny=ny, nt=nt, dt=1e-3, fw=10, seismo=1, mode=1, dtinv=dtinv, fpml=fc, damping=2500.0,
+               ts=1.0 / fc, invmat1=invmat1, grad_form=grad_form, quelltypb=1)
+    jj = np.arange(ny)[None, :] * np.ones((nx, 1))
+    f = np.clip((jj - recrow - 4) / 20.0, 0.0, 1.0)
+    vs, rho = 1800.0 * (1 + 0.15 * f), 2000.0 * (1 + 0.1 * f)
+
+    def model(vs, rho):
+        first = vs if invmat1 == 1 else rho * vs * vs
+        return [first.astype(np.float32), rho.astype(np.float32)]
+
+    m0 = model(vs, rho)
+    mt = model(vs * (1 + 0.05 * _bump(nx, ny, nx // 2 - 6, ny - 18, 5)),
+               rho * (1 + 0.05 * _bump(nx, ny, nx // 2 + 6, ny - 20, 5)))
+    src = np.zeros((8, 1), np.float32)
+    src[0, 0], src[1, 0], src[4, 0], src[5, 0], src[7, 0] = nx // 2, recrow, fc, 1.0, 1
+    sig = synth.ricker(nt, p.dt, fc, amp=1e3)[None, :]
+    xs = list(range(14, nx - 13, 2))
+    rec = np.array([xs, [recrow] * len(xs)], np.int32)
+
+    def runner(m):
+        o = _sh(p, *m)
+        o.set_sources(src, sig)
+        o.set_receivers(rec)
+        return o
+
+    t = runner(mt)
+    t.forward()
+    obs = t.seismogram(1).astype(np.float32)
+    if zero_data:
+        obs = np.zeros_like(obs)
+
+    def misfit(m, grad=False):
+        o = runner([np.asarray(a, np.float32) for a in m])
+        o.forward()
+        l2 = o.residuals(None, obs, None, 1)
+        if not grad:
+            return l2, None
+        o.adjoint()
+        o.assemble()
+        return l2, [o.gradient(k).astype(np.float64) for k in (4, 5, 1, 2)]
+
+    return p, m0, misfit, float(rho[0, recrow])
+
+
+def _fd(misfit, m0, w, d):
+    mp = [a.astype(np.float64) for a in m0]
+    mm = [a.astype(np.float64) for a in m0]
+    mp[w] = mp[w] + d
+    mm[w] = mm[w] - d
+    return (misfit(mp)[0] - misfit(mm)[0]) / 2.0
+
+
+# The scale of the P/SV gradient pins (test_oracle_pins.py) holds here for
+# the same reasons: the vz residual R is added to vz (update_v_PML_SH.c:
+# 571-583, where the transposed update carries 1/rho), the adjoint vz is
+# integrated into vzp1 (DT vz) and every DTINV-th step is correlated, so a
+# gradient equals (rho_rec / DTINV) dJ/dm.  The shear term correlates
+# forward and adjoint stresses with no 1/mu^2 (sh.c:195-205) and
+# ass_gradSH.c:28 takes mu' = -DT times that sum, so mu' / mu^2 is the
+# scaled derivative of the misfit with respect to mu.
+@pytest.mark.parametrize("dtinv", [1, 2])
+def test_sh_mu_and_rho_gradients_equal_scaled_misfit_derivative(dtinv):
+    """INVMAT1 3, GRAD_FORM 1: along smooth directions (mu and rho enter
+    through averages at staggered points) the misfit's central differences
+    over mu'/mu^2 . dmu and over rho' . drho are DTINV / rho_rec within 8%."""
+    from synth import gaussian_smooth as gaussian_filter
+    p, m0, misfit, rho_rec = _sh_gradient_setup(dtinv=dtinv)
+    nx, ny = p.nx, p.ny
+    _, g = misfit(m0, grad=True)
+    mu = m0[0].astype(np.float64)
+    gm = [g[0] / (mu * mu), g[1]]
+    for w, name in ((0, "mu"), (1, "rho")):
+        ratios = []
+        for s in (2.0, 3.0):
+            for region in ((slice(12, 52), slice(24, 44)), (slice(14, 34), slice(22, 46)),
+                           (slice(30, 52), slice(22, 46))):
+                d = np.zeros((nx, ny))
+                d[region] = gaussian_filter(gm[w], s)[region]
+                d = gaussian_filter(d, s)
+                d *= 1e-3 * float(np.abs(m0[w]).max()) / np.abs(d).max()
+                ratios.append(_fd(misfit, m0, w, d) / float(np.sum(gm[w] * d)) * rho_rec / dtinv)
+        ratios = np.array(ratios)
+        print(f"SH {name}: FD / gradient x rho_rec / DTINV =", np.round(ratios, 4))
+        assert np.all(np.abs(ratios - 1.0) < 0.08), name
+
+
+@pytest.mark.parametrize("grad_form", [1, 2])
+def test_sh_vs_parameterisation(grad_form):
+    """INVMAT1 1 against INVMAT1 3 on the same physical model (zero data,
+    FP64): the raw correlations are equal, the density gradient is the
+    chain rule d/drho|vs = vs^2 d/dmu + d/drho|mu with d/dmu = mu'/mu^2
+    (GRAD_FORM 1, whose mu' lacks the 1/mu^2) or mu' (GRAD_FORM 2), and the
+    vs gradient is 2 rho vs mu' for GRAD_FORM 2 (the chain rule) and, for
+    GRAD_FORM 1, (2/rho) vs^3 mu' as ass_gradSH.c:37 writes it -- not the
+    chain rule's 2/(rho vs^3) mu', a precedence slip of the reference that
+    the drop-in keeps."""
+    _, m1, mis1, _ = _sh_gradient_setup(invmat1=1, grad_form=grad_form, zero_data=True, nt=500)
+    _, m3, mis3, _ = _sh_gradient_setup(invmat1=3, grad_form=grad_form, zero_data=True, nt=500)
+    _, g1 = mis1(m1, grad=True)
+    _, g3 = mis3(m3, grad=True)
+    vs, rho = (a.astype(np.float64) for a in m1)
+    mu = m3[0].astype(np.float64)
+    for k in (2, 3):                     # raw mu and rho correlations
+        assert rel_l2(g1[k], g3[k]) < 1e-5, k
+    dmu = g3[0] / (mu * mu) if grad_form == 1 else g3[0]
+    want_rho = vs * vs * dmu + g3[1]
+    want_vs = (2.0 / rho) * vs ** 3 * g3[0] if grad_form == 1 else 2.0 * rho * vs * g3[0]
+    for got, want, name in ((g1[1], want_rho, "rho"), (g1[0], want_vs, "vs")):
+        err = rel_l2(got, want)
+        print(f"SH GRAD_FORM {grad_form} {name}: relative L2 {err:.2e}")
+        assert np.abs(want).max() > 0
+        assert err < 1e-5, name
+
+
+def test_sh_adjoint_adds_the_residual_to_vz_only_for_quelltypb_1():
+    """update_v_PML_SH.c:571-583: QUELLTYPB 1 adds the residual to vz at the
+    receivers; other values inject nothing (the gradient stays 0)."""
+    for qb, nonzero in ((1, True), (2, False)):
+        p = params(nx=40, ny=36, nt=120, dt=1e-3, fw=8, seismo=1, mode=1, quelltypb=qb)
+        o = _sh(p, const(p.nx, p.ny, VS), const(p.nx, p.ny, RHO))
+        rec = np.array([[10, 20, 30], [5, 5, 5]], np.int32)
+        o.set_receivers(rec)
+        o.set_sources(np.array([[20], [10], [0], [0], [8.0], [1.0], [0], [1]], np.float32),
+                      synth.ricker(p.nt, p.dt, 8.0, amp=1e3)[None, :])
+        o.forward()
+        l2 = o.residuals(None, np.zeros((3, p.nt), np.float32), None, 1)
+        assert l2 > 0
+        o.adjoint()
+        assert bool(np.abs(o.gradient(2)).max() > 0) == nonzero
