@@ -314,8 +314,8 @@ int validate(const dn_params *p)
         return fail(DN_ERR_ARG, "PHYSICS must be 1 (P/SV), 2 (acoustic), 3 (VTI), 4 (TTI) or 5 (SH)");
     if (p->physics >= 2 && p->L != 0) return fail(DN_ERR_ARG, "acoustic, VTI, TTI and SH runs are elastic: L = 0");
     // the acoustic and VTI / TTI loops also run the RTM passes (physics_AC.c,
-    // physics_VTI.c:27-29, physics_TTI.c); the SH loop forward modelling here
-    if (p->physics == 5 && p->mode != 0) return fail(DN_ERR_ARG, "SH runs are forward modelling: MODE 0");
+    // physics_VTI.c:27-29, physics_TTI.c); the SH loop its FWI gradient
+    // (sh.c mode 1, FWI_SH.c)
     // TTI.c stores ux / uy / uxy for GRAD_FORM 2, which its stress update never writes
     if (p->physics == 4 && p->mode > 0 && p->grad_form == 2)
         return fail(DN_ERR_ARG, "TTI adjoint runs need GRAD_FORM 1 (the TTI update writes no ux/uy/uxy)");
@@ -441,7 +441,7 @@ std::vector<size_t> plan_sizes(const dn_params &p, int nsrc, int ntr)
     add(sizeof(float) * (2 * p.fw + 1), P_NVEC);
     if (p.mode > 0) {
         const size_t ntd = (size_t)std::ceil((float)p.nt / (float)p.dtinv);
-        add((size_t)p.nx * p.ny * ntd * sizeof(float), 5);
+        add((size_t)p.nx * p.ny * ntd * sizeof(float), (size_t)snap_planes(p.physics));
         add((size_t)p.nx * p.ny * sizeof(float), 6);
     }
     add(sizeof(int), 1);
@@ -596,7 +596,7 @@ static int ctx_create(const dn_params *p, int device, dn_arena *arena, dn_ctx **
     if (p->mode > 0) {
         const size_t snap = (size_t)p->nx * p->ny * c->ntdtinv * sizeof(float);
         for (int k = 0; k < 5; ++k)
-            if ((rc = c->dalloc((void **)&c->fwd[k], snap))) return bail(rc);
+            if (snap_plane_used(p->physics, k) && (rc = c->dalloc((void **)&c->fwd[k], snap))) return bail(rc);
         const size_t g = (size_t)p->nx * p->ny * sizeof(float);
         float **grads[] = {&c->wc, &c->wcu, &c->wcr, &c->gvp, &c->gvs, &c->grho};
         for (auto m : grads)
@@ -667,8 +667,10 @@ static int set_model_sh(dn_ctx *c, const float *vs, const float *rho)
     if ((rc = upload(c, c->rip, c->mat.rip.data(), pl)) || (rc = upload(c, c->ani[0], c->mat.ani[0].data(), pl)) ||
         (rc = upload(c, c->ani[1], c->mat.ani[1].data(), pl)))
         return rc;
+    if (c->vs && ((rc = upload(c, c->vs, c->mat.vs.data(), pl)) || (rc = upload(c, c->rho, c->mat.rho.data(), pl))))
+        return rc;
     HIP_OK(hipStreamSynchronize(c->stream));
-    for (auto *v : {&c->mat.rip, &c->mat.ani[0], &c->mat.ani[1]}) {
+    for (auto *v : {&c->mat.rip, &c->mat.ani[0], &c->mat.ani[1], &c->mat.vs, &c->mat.rho}) {
         v->clear();
         v->shrink_to_fit();
     }
@@ -1150,11 +1152,13 @@ int dn_calc_residuals(dn_ctx *c, const float *obs_vx, const float *obs_vy, const
     for (int k = 0; k < 3; ++k)
         if (q.adj[k]) HIP_OK(hipMemsetAsync(q.adj[k], 0, n * sizeof(float), c->stream));
     const float *obs[3] = {obs_vx, obs_vy, obs_p};
-    const bool use[3] = {QB == 1 || QB == 3 || QB == 5 || QB == 7, QB == 1 || QB == 2 || QB == 6 || QB == 7, QB >= 4};
+    bool use[3];
+    float w[3];
+    residual_components(c->p.physics, QB, c->res_w, use, w);
     for (int comp = 0; comp < 3; ++comp) {
         if (!use[comp] || !obs[comp] || q.ntr == 0) continue;
         HIP_OK(hipMemcpyAsync(q.obs, obs[comp], n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        launch_residual(q, comp, q.obs, c->p.dt, comp < 2 ? c->res_w[comp] : 1.0f, c->stream);
+        launch_residual(q, comp, q.obs, c->p.dt, w[comp], c->stream);
         if (testshot) launch_l2(q, comp, c->stream);
         HIP_OK(hipStreamSynchronize(c->stream));
     }

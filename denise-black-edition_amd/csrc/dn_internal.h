@@ -75,6 +75,41 @@ inline int aniso_planes(int physics)
 // the physics whose sweeps have scalar columns only in both sweeps
 inline bool scalar_only(int physics) { return physics == 2 || physics == 5; }
 
+// forward snapshot planes a physics stores: vxp1, vyp1 and three stress (or
+// ux, uy, uxy) planes for P/SV, VTI and TTI; vxp1, vyp1 and p (or ux) for
+// the acoustic loop (ac.c:222-247); vzp1 and sxz, syz (or uzx, uz) for SH
+// (sh.c:150-172)
+inline bool snap_plane_used(int physics, int k)
+{
+    if (physics == 2) return k <= 2;
+    if (physics == 5) return k == 0 || k == 2 || k == 3;
+    return true;
+}
+inline int snap_planes(int physics)
+{
+    int n = 0;
+    for (int k = 0; k < 5; ++k) n += snap_plane_used(physics, k);
+    return n;
+}
+
+// the residual components of calc_res_PSV.c (vx, vy, p by QUELLTYPB) with
+// their COMP_WEIGHT factors; calc_res_SH.c:20-44 takes the vz residual
+// (component 1) for any QUELLTYPB and no weights
+inline void residual_components(int physics, int qb, const float res_w[2], bool use[3], float w[3])
+{
+    use[0] = qb == 1 || qb == 3 || qb == 5 || qb == 7;
+    use[1] = qb == 1 || qb == 2 || qb == 6 || qb == 7;
+    use[2] = qb >= 4;
+    w[0] = res_w[0];
+    w[1] = res_w[1];
+    w[2] = 1.0f;
+    if (physics == 5) {
+        use[0] = use[2] = false;
+        use[1] = qb != 0;
+        w[1] = 1.0f;
+    }
+}
+
 // set-up math (dn_setup.cpp)
 int holberg(int fdorder, int maxrelerror, float hc[7], std::string &err);
 void pml_profiles(const dn_params &p, HostSetup &s);

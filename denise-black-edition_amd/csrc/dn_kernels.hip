@@ -94,6 +94,31 @@ __global__ __launch_bounds__(256) void k_assemble(KArgs a, const float *wc, cons
     float vpg, vsg, rhog;
     const float mu = -DT * wcu[g];
     const float rho_s = -DT * wcr[g];
+    if (physics == 5) {
+        // ass_gradSH.c:19-98 (INV_*_ITER 0): where the INVMAT1 1 / GRAD_FORM 1
+        // branches write nothing (vs = 0) the raw sums stay
+        float gu = wcu[g], gr = wcr[g];
+        if (invmat1 == 1) {
+            if (a.grad_form == 1) {
+                if (pu > 0.0f) {
+                    const float vs3 = pu * pu * pu;
+                    gu = (float)((2.0 / (double)prho * (double)vs3) * (double)mu);
+                    const float vs2rho2 = (float)(1.0 / (double)(pu * pu * prho * prho));
+                    gr = (vs2rho2 * mu) + rho_s;
+                }
+            } else {
+                gu = (float)(2.0 * (double)prho * (double)pu * (double)mu);
+                gr = (pu * pu * mu) + rho_s;
+            }
+        } else {
+            gu = mu;
+            gr = rho_s;
+        }
+        gvp[t] = 0.0f;
+        gvs[t] = gu;
+        grho[t] = gr;
+        return;
+    }
     if (physics == 2) {
         const float lamss = invmat1 == 3 ? ppi : prho * ppi * ppi;
         if (lamss > 0.0f) {

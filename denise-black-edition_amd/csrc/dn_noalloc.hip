@@ -87,11 +87,13 @@ extern "C" int dn_calc_residuals_noalloc(dn_ctx *c, const float *obs_vx, const f
     for (int k = 0; k < 3; ++k)
         if (q.adj[k]) zero_floats(q.adj[k], n, c->stream);
     const float *obs[3] = {obs_vx, obs_vy, obs_p};
-    const bool use[3] = {QB == 1 || QB == 3 || QB == 5 || QB == 7, QB == 1 || QB == 2 || QB == 6 || QB == 7, QB >= 4};
+    bool use[3];
+    float w[3];
+    residual_components(c->p.physics, QB, c->res_w, use, w);
     for (int comp = 0; comp < 3; ++comp) {
         if (!use[comp] || !obs[comp] || q.ntr == 0) continue;
         HIP_OK(hipMemcpyAsync(d_obs, obs[comp], n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-        launch_residual(q, comp, d_obs, c->p.dt, comp < 2 ? c->res_w[comp] : 1.0f, c->stream);
+        launch_residual(q, comp, d_obs, c->p.dt, w[comp], c->stream);
         if (testshot) launch_l2(q, comp, c->stream);
         // the staging area is reused by the next component
         HIP_OK(hipStreamSynchronize(c->stream));

@@ -586,6 +586,10 @@ void prepare_material_sh(const dn_params &p, const Layout &lay, const float *vs,
     m.rip.assign(plane, 0.0f);
     m.rjp.clear();
     for (int k = 0; k < 8; ++k) m.ani[k].assign(k < 2 ? plane : 0, 0.0f);
+    // the model itself for the adjoint run (sh.c:193-199 tests mu > 0) and
+    // the assembly (ass_gradSH.c reads pu, prho)
+    m.vs.assign(p.mode > 0 ? plane : 0, 0.0f);
+    m.rho.assign(p.mode > 0 ? plane : 0, 0.0f);
     const bool mod = p.invmat1 == 3;
     auto modulus = [&](int j, int i) { return mod ? U.at(j, i) : R.at(j, i) * U.at(j, i) * U.at(j, i); };
     for (int j = 1; j <= NY; ++j)
@@ -601,6 +605,10 @@ void prepare_material_sh(const dn_params &p, const Layout &lay, const float *vs,
             float ri = (float)(1.0 / R.at(j, i));
             if (R.at(j, i) < 1e-4) ri = 0.0f;
             m.rip[o] = ri;
+            if (p.mode > 0) {
+                m.vs[o] = U.at(j, i);
+                m.rho[o] = R.at(j, i);
+            }
         }
 }
 }  // namespace dn

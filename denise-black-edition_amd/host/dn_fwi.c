@@ -24,7 +24,8 @@ int dnio_fail_msg(const char *msg); /* dn_io.c */
 
 struct dnfwi_lbfgs {
     int nx, ny, nlbfgs, pointer;
-    int nclass;                  /* 3: [rho | vs | vp]; 2 (acoustic): [rho | vp] */
+    int nclass;                  /* 3: [rho | vs | vp]; 2 (acoustic): [rho | vp]; 2 (SH): [vs | rho] */
+    int sh;                      /* SH: models normalised by C, the direction left normalised */
     size_t ncell, nvec;          /* nvec = nclass * ncell */
     float *y, *s;                /* nlbfgs * nvec */
     float *q, *r;                /* nvec */
@@ -72,6 +73,13 @@ int dnfwi_lbfgs_create_ac(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out)
     return lbfgs_create(nx, ny, nlbfgs, 2, out); /* FWI_AC.c:271: NLBFGS_class = 2 */
 }
 
+int dnfwi_lbfgs_create_sh(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out)
+{
+    int rc = lbfgs_create(nx, ny, nlbfgs, 2, out); /* FWI_SH.c: NLBFGS_class = 2 */
+    if (!rc) (*out)->sh = 1;
+    return rc;
+}
+
 void dnfwi_lbfgs_destroy(dnfwi_lbfgs *lb)
 {
     if (!lb) return;
@@ -112,8 +120,8 @@ static float dotp(const float *a, const float *b, size_t n)
 int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, float c_rho, float *g_vp,
                           float *g_vs, float *g_rho, const float *vp, const float *vs, const float *rho)
 {
-    const int ac = lb && lb->nclass == 2;
-    if (!lb || !g_vp || !g_rho || !vp || !rho || (!ac && (!g_vs || !vs)) || iter < 1)
+    const int sh = lb && lb->sh, ac = lb && lb->nclass == 2 && !sh;
+    if (!lb || !g_rho || !rho || (!sh && (!g_vp || !vp)) || (!ac && (!g_vs || !vs)) || iter < 1)
         return dnio_fail_msg("dnfwi_lbfgs_direction: null argument or iter < 1");
     if (iter > 1 && !lb->have_old)
         return dnio_fail_msg("dnfwi_lbfgs_direction: iter > 1 needs the previous iteration's gradients");
@@ -123,7 +131,15 @@ int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, flo
      * or [rho | vp] (store_LBFGS_AC.c:63-140) */
     float *g[3] = {g_rho, ac ? g_vp : g_vs, g_vp};
     const float *m[3] = {rho, ac ? vp : vs, vp};
-    const float cs[3] = {c_rho, ac ? c_vp : c_vs, c_vp};
+    float cs[3] = {c_rho, ac ? c_vp : c_vs, c_vp};
+    if (sh) { /* store_LBFGS_SH.c:36-134: [vs | rho] */
+        g[0] = g_vs;
+        g[1] = g_rho;
+        m[0] = vs;
+        m[1] = rho;
+        cs[0] = c_vs;
+        cs[1] = c_rho;
+    }
 
     /* FWI_PSV.c:455-464 */
     if (iter > 2) lb->pointer++;
@@ -139,7 +155,9 @@ int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, flo
         for (int c = 0; c < ncl; c++)
             for (size_t k = 0; k < nc; k++) {
                 ys[c * nc + k] = g[c][k] - lb->gold[c * nc + k];
-                ss[c * nc + k] = m[c][k] - lb->mold[c * nc + k];
+                /* store_LBFGS_SH.c:81, :104: the model normalised, (m - C_min) / C with C_min = 0 */
+                const float mk = sh ? (m[c][k] - 0.0f) / cs[c] : m[c][k];
+                ss[c * nc + k] = mk - lb->mold[c * nc + k];
             }
         for (int c = 0; c < ncl; c++) memcpy(lb->q + c * nc, g[c], sizeof(float) * nc);
 
@@ -172,12 +190,18 @@ int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, flo
     /* extract_LBFGS_PSV.c:80-249: keep the model and the normalised
      * gradients (before they are replaced by the direction) */
     for (int c = 0; c < ncl; c++) {
-        memcpy(lb->mold + c * nc, m[c], sizeof(float) * nc);
+        if (sh) /* extract_LBFGS_SH.c:66-70: the normalised model */
+            for (size_t k = 0; k < nc; k++) lb->mold[c * nc + k] = (m[c][k] - 0.0f) / cs[c];
+        else
+            memcpy(lb->mold + c * nc, m[c], sizeof(float) * nc);
         memcpy(lb->gold + c * nc, g[c], sizeof(float) * nc);
     }
     lb->have_old = 1;
 
-    if (iter > 1) {
+    if (iter > 1 && sh) {
+        /* extract_LBFGS_SH.c:27-48: the direction r, not scaled back */
+        for (int c = 0; c < ncl; c++) memcpy(g[c], lb->r + c * nc, sizeof(float) * nc);
+    } else if (iter > 1) {
         /* extract_LBFGS_PSV.c:31-78: direction = r, scaled back */
         for (int c = 0; c < ncl; c++) memcpy(g[c], lb->r + c * nc, sizeof(float) * nc);
         for (size_t k = 0; k < nc; k++) {
@@ -192,8 +216,8 @@ int dnfwi_lbfgs_direction(dnfwi_lbfgs *lb, int iter, float c_vp, float c_vs, flo
 int dnfwi_lbfgs_save(const dnfwi_lbfgs *lb, const char *jacobian, const float *d_vp, const float *d_vs,
                      const float *d_rho)
 {
-    const int ac = lb && lb->nclass == 2;
-    if (!lb || !jacobian || !d_vp || (!ac && !d_vs) || !d_rho)
+    const int sh = lb && lb->sh, ac = lb && lb->nclass == 2 && !sh;
+    if (!lb || !jacobian || (!sh && !d_vp) || (!ac && !d_vs) || !d_rho)
         return dnio_fail_msg("dnfwi_lbfgs_save: null argument");
     if (!lb->have_old) return dnio_fail_msg("dnfwi_lbfgs_save: no direction taken yet");
     const size_t nc = lb->ncell;
@@ -206,8 +230,9 @@ int dnfwi_lbfgs_save(const dnfwi_lbfgs *lb, const char *jacobian, const float *d
     const float *dir[3] = {d_vp, d_vs, d_rho};
     char name[1024];
     for (int k = 0; k < 3; k++) {
-        if (ac && k == 1) continue;
-        const int sl = ac ? (k == 0 ? 1 : 0) : slot[k];
+        if ((ac && k == 1) || (sh && k == 0)) continue;
+        /* extract_LBFGS_SH.c:50-178: vs, rho ([vs | rho] slots 0, 1) */
+        const int sl = sh ? (k == 1 ? 0 : 1) : ac ? (k == 0 ? 1 : 0) : slot[k];
         const float *arr[3] = {lb->mold + (size_t)sl * nc, lb->gold + (size_t)sl * nc, dir[k]};
         for (int f = 0; f < 3; f++) {
             if ((size_t)snprintf(name, sizeof name, "%s_%s.old", jacobian, names[k][f]) >= sizeof name)
@@ -242,11 +267,45 @@ static void maxima(const float *m, const float *d, size_t n, float *mmax, float 
     *dmax = b;
 }
 
+/* calc_mat_change_test_SH.c:10-189: vs and rho, the rho step without SCALERHO */
+static float model_update_sh(const dnfwi_update_par *u, int iter, float eps_scale, size_t n, const float *vs,
+                             const float *rho, const float *d_vs, const float *d_rho, float *vs_new, float *rho_new)
+{
+    float umax, gradmax_u, rhomax, gradmax_rho;
+    maxima(vs, d_vs, n, &umax, &gradmax_u);
+    maxima(rho, d_rho, n, &rhomax, &gradmax_rho);
+    float eps_u = eps_scale * (umax / gradmax_u);
+    if (iter < u->inv_vs_iter) eps_u = 0.0f;
+    float eps_rho = eps_scale * (rhomax / gradmax_rho);
+    if (iter < u->inv_rho_iter) eps_rho = 0.0f;
+    if (u->invmat1 != 1 && u->invmat1 != 3) {
+        memcpy(vs_new, vs, sizeof(float) * n);
+        memcpy(rho_new, rho, sizeof(float) * n);
+        return eps_u;
+    }
+    for (size_t k = 0; k < n; k++) {
+        float u1 = vs[k] - eps_u * d_vs[k];
+        float r1 = rho[k] - eps_rho * d_rho[k];
+        if (u->invmat1 == 1) {
+            if ((u1 < u->vslowerlim) && (u1 > 1e-6)) u1 = vs[k];
+            if (u1 > u->vsupperlim) u1 = vs[k];
+            if (r1 < u->rholowerlim) r1 = rho[k];
+            if (r1 > u->rhoupperlim) r1 = rho[k];
+        }
+        if (u1 < 0.0) u1 = vs[k];
+        if (r1 < 0.0) r1 = rho[k];
+        vs_new[k] = u1;
+        rho_new[k] = r1;
+    }
+    return eps_u; /* eps_true = EPSILON_u (:104) */
+}
+
 float dnfwi_model_update(const dnfwi_update_par *u, int iter, float eps_scale, int nx, int ny, const float *vp,
                          const float *vs, const float *rho, const float *d_vp, const float *d_vs, const float *d_rho,
                          float *vp_new, float *vs_new, float *rho_new)
 {
     const size_t n = (size_t)nx * ny;
+    if (!vp) return model_update_sh(u, iter, eps_scale, n, vs, rho, d_vs, d_rho, vs_new, rho_new);
     /* no vs (acoustic): calc_mat_change_test_AC.c, vp and rho only */
     const int ac = !vs;
     float pimax, gradmax, umax = 0.0f, gradmax_u = 0.0f, rhomax, gradmax_rho;
@@ -330,13 +389,15 @@ int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, flo
                       const float *d_vp, const float *d_vs, const float *d_rho, dnfwi_misfit_fn misfit,
                       void *user, dnfwi_linesearch *out)
 {
-    if (!u || !misfit || !out || !vp || !rho || !d_vp || !d_rho || (vs && !d_vs) || nx < 1 || ny < 1)
+    if (!u || !misfit || !out || !rho || !d_rho || (!vp && !vs) || (vp && !d_vp) || (vs && !d_vs) || nx < 1 ||
+        ny < 1)
         return dnio_fail_msg("dnfwi_step_length: null argument");
     const size_t n = (size_t)nx * ny;
-    /* no vs: the acoustic search (step_length_est_ac.c), test models of vp and rho */
-    float *tv = malloc(sizeof(float) * n), *ts = vs ? malloc(sizeof(float) * n) : NULL;
+    /* no vs: the acoustic search (step_length_est_ac.c), test models of vp
+     * and rho; no vp: the SH search (step_length_est_sh.c), vs and rho */
+    float *tv = vp ? malloc(sizeof(float) * n) : NULL, *ts = vs ? malloc(sizeof(float) * n) : NULL;
     float *tr = malloc(sizeof(float) * n);
-    if (!tv || (vs && !ts) || !tr) {
+    if ((vp && !tv) || (vs && !ts) || !tr) {
         free(tv);
         free(ts);
         free(tr);
@@ -435,7 +496,8 @@ int dnfwi_rtm_adjoint_source(const float *data, const float *syn, int ntr, int n
 int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws, const float *Wr, float c_vp,
                          float c_vs, float c_rho, float *g_vp, float *g_vs, float *g_rho, float *We)
 {
-    if ((eprecond != 1 && eprecond != 3) || !Ws || (eprecond == 1 && !Wr) || !g_vp || !g_rho || nx < 1 || ny < 1)
+    if ((eprecond != 1 && eprecond != 3) || !Ws || (eprecond == 1 && !Wr) || (!g_vp && !g_vs) || !g_rho || nx < 1 ||
+        ny < 1)
         return dnio_fail_msg("dnfwi_energy_precond: bad argument");
     const size_t n = (size_t)nx * ny;
     float *w = We ? We : malloc(sizeof(float) * n);
@@ -458,7 +520,7 @@ int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws
     for (size_t k = 0; k < n; k++) w[k] = w[k] + (eps_we * maxwe);
     /* grad_obj_psv.c:224-237; grad_obj_ac.c:225-233 has no vs gradient */
     for (size_t k = 0; k < n; k++) {
-        g_vp[k] = g_vp[k] / (w[k] * c_vp * c_vp);
+        if (g_vp) g_vp[k] = g_vp[k] / (w[k] * c_vp * c_vp);
         if (g_vs) {
             g_vs[k] = g_vs[k] / (w[k] * c_vs * c_vs);
             if (c_vs == 0.0) g_vs[k] = 0.0f;
@@ -932,5 +994,52 @@ int dnfwi_taper_sources(int nx, int ny, float dh, float srtradius, int srtshape,
     }
     free(m);
     free(msum);
+    return 0;
+}
+
+int dnfwi_assemble_sh(int nx, int ny, float dt, int invmat1, int grad_form, int iter, int inv_vs_iter,
+                      int inv_rho_iter, const float *vs, const float *rho, float *g_u, float *g_rho)
+{
+    if (!vs || !rho || !g_u || !g_rho || nx < 1 || ny < 1 || (grad_form != 1 && grad_form != 2))
+        return dnio_fail_msg("dnfwi_assemble_sh: bad argument");
+    const size_t n = (size_t)nx * ny;
+    float *wmu = malloc(sizeof(float) * n);
+    if (!wmu) return dnio_fail_msg("dnfwi_assemble_sh: out of host memory");
+    /* ass_gradSH.c:19-61: mu' = -DT w_u, then the vs (INVMAT1 1), Zs (2) or
+     * mu (3) gradient; where the INVMAT1 1 / GRAD_FORM 1 branch writes
+     * nothing (vs = 0) the sum stays */
+    for (size_t k = 0; k < n; k++) {
+        wmu[k] = -dt * g_u[k];
+        if (invmat1 == 1) {
+            if (grad_form == 1) {
+                if (vs[k] > 0.0) {
+                    const float vs3 = vs[k] * vs[k] * vs[k];
+                    g_u[k] = (2.0 / rho[k] * vs3) * wmu[k];
+                }
+            } else {
+                g_u[k] = 2.0 * rho[k] * vs[k] * wmu[k];
+            }
+        }
+        if (invmat1 == 2) g_u[k] = 2.0 * vs[k] * wmu[k];
+        if (invmat1 == 3) g_u[k] = wmu[k];
+        if (iter < inv_vs_iter) g_u[k] = 0.0f;
+    }
+    /* ass_gradSH.c:63-96: rho' = -DT w_rho plus the mu term */
+    for (size_t k = 0; k < n; k++) {
+        const float rho_s = -dt * g_rho[k];
+        if (invmat1 == 1) {
+            if (grad_form == 1) {
+                if (vs[k] > 0.0) {
+                    const float vs2rho2 = 1.0 / (vs[k] * vs[k] * rho[k] * rho[k]);
+                    g_rho[k] = (vs2rho2 * wmu[k]) + rho_s;
+                }
+            } else {
+                g_rho[k] = (vs[k] * vs[k] * wmu[k]) + rho_s;
+            }
+        }
+        if (invmat1 == 3) g_rho[k] = rho_s;
+        if (iter < inv_rho_iter) g_rho[k] = 0.0f;
+    }
+    free(wmu);
     return 0;
 }

@@ -36,6 +36,16 @@ int dnfwi_lbfgs_create(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out);
  * dnfwi_lbfgs_save writes the vp and rho files only
  * (src/AC/extract_LBFGS_AC.c:60-178). */
 int dnfwi_lbfgs_create_ac(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out);
+/* The SH problem's memory (src/SH/FWI_SH.c, NLBFGS_class = 2): vectors
+ * [vs | rho] (src/SH/store_LBFGS_SH.c:36-134) whose model differences are of
+ * the normalised models vs / C_vs and rho / C_rho (C_vs_min = C_rho_min = 0,
+ * FWI_SH.c:572-573), and a direction left in normalised units
+ * (src/SH/extract_LBFGS_SH.c:27-48 scales nothing back).
+ * dnfwi_lbfgs_direction and dnfwi_lbfgs_save ignore the vp arguments (they
+ * may be NULL); the files are the vs ones (_p_vs.old holding the normalised
+ * model, _p_u.old, _c_u.old) and the rho ones (_p_mrho.old normalised too,
+ * _p_rho.old, _c_rho.old). */
+int dnfwi_lbfgs_create_sh(int nx, int ny, int nlbfgs, dnfwi_lbfgs **out);
 void dnfwi_lbfgs_destroy(dnfwi_lbfgs *lb);
 /* zero_LBFGS (src/zero_LBFGS.c) and LBFGS_pointer = 1 (src/PSV/FWI_PSV.c:807-811):
  * the memory is cleared at the start of each workflow stage after the first. */
@@ -128,7 +138,10 @@ typedef struct dnfwi_update_par {
  * keep the old value.  Returns EPSILON_u, the reference's "eps_true".
  * With vs = NULL (d_vs and vs_new are then not used) it is
  * calc_mat_change_test_AC (src/AC/calc_mat_change_test_AC.c:9-189): vp and
- * rho only, and the value returned is EPSILON, that routine's eps_true. */
+ * rho only, and the value returned is EPSILON, that routine's eps_true.
+ * With vp = NULL (d_vp and vp_new not used) it is calc_mat_change_test_SH
+ * (src/SH/calc_mat_change_test_SH.c:10-189): vs and rho, the rho step
+ * WITHOUT SCALERHO, and the value returned is EPSILON_u. */
 float dnfwi_model_update(const dnfwi_update_par *u, int iter, float eps_scale, int nx, int ny, const float *vp,
                          const float *vs, const float *rho, const float *d_vp, const float *d_vs, const float *d_rho,
                          float *vp_new, float *vs_new, float *rho_new);
@@ -160,7 +173,8 @@ typedef struct dnfwi_linesearch {
  * before the first iteration and pass the same record every iteration.
  * vs = NULL runs the acoustic search (src/AC/step_length_est_ac.c:37-181,
  * the same steps with calc_mat_change_test_AC): the misfit callback then
- * gets vs = NULL. */
+ * gets vs = NULL.  vp = NULL runs the SH search (src/SH/step_length_est_sh.c,
+ * with calc_mat_change_test_SH): the callback gets vp = NULL. */
 int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, float scalefac, int stepmax,
                       double L2t0, int nx, int ny, const float *vp, const float *vs, const float *rho,
                       const float *d_vp, const float *d_vs, const float *d_rho, dnfwi_misfit_fn misfit,
@@ -175,7 +189,8 @@ int dnfwi_step_length(const dnfwi_update_par *u, int iter, float eps_scale0, flo
  * We*C_vs*C_vs (0 when C_vs = 0), g_rho /= We*C_rho*C_rho.  Ws, Wr, g_*:
  * x-major NX x NY (Wr only for EPRECOND 1); We (may be NULL) gets the
  * weights.  g_vs may be NULL (src/AC/grad_obj_ac.c:225-233: vp and rho
- * gradients only). */
+ * gradients only), and g_vp (src/SH/grad_obj_sh.c:211-219: the shear and
+ * density sums of the SH run, divided by We C_vs^2 and We C_rho^2). */
 int dnfwi_energy_precond(int eprecond, int nx, int ny, float dh, const float *Ws, const float *Wr, float c_vp,
                          float c_vs, float c_rho, float *g_vp, float *g_vs, float *g_rho, float *We);
 
@@ -268,6 +283,17 @@ int dnfwi_taper_vert(int nx, int ny, float dh, const int *gradt, int nsrc, const
  * be NULL) is multiplied by the taper before the boxes are set. */
 int dnfwi_taper_sources(int nx, int ny, float dh, float srtradius, int srtshape, int filtsize, int nsrc,
                         const float *srcpos, float *coeff, float *g);
+
+/* ass_gradSH (src/SH/ass_gradSH.c:10-98) on the shot-stacked SH sums (FWI_SH.c
+ * assembles after the stacking): with mu' = -DT g_u, g_u becomes the vs
+ * gradient (2 / rho * vs^3) mu' (INVMAT1 1, GRAD_FORM 1, where vs > 0, as the
+ * reference writes it), 2 rho vs mu' (INVMAT1 1, GRAD_FORM 2) or mu'
+ * (INVMAT1 3); g_rho becomes 1/(vs^2 rho^2) mu' - DT g_rho (INVMAT1 1,
+ * GRAD_FORM 1, vs > 0), vs^2 mu' - DT g_rho (GRAD_FORM 2) or -DT g_rho
+ * (INVMAT1 3); 0 before INV_VS_ITER / INV_RHO_ITER.  vs: the model's vs (or
+ * mu for INVMAT1 3), rho; x-major NX x NY. */
+int dnfwi_assemble_sh(int nx, int ny, float dt, int invmat1, int grad_form, int iter, int inv_vs_iter,
+                      int inv_rho_iter, const float *vs, const float *rho, float *g_u, float *g_rho);
 
 /* Adjoint source of reverse-time migration (MODE 2), the MODE 2 branch of
  * calc_res with LNORM = 2 (src/calc_res.c:416-446): per trace the first data
