@@ -42,6 +42,9 @@ def load():
     sig = {
         "dnfwi_lbfgs_create": ([C.c_int, C.c_int, C.c_int, C.POINTER(vp_)], C.c_int),
         "dnfwi_lbfgs_create_ac": ([C.c_int, C.c_int, C.c_int, C.POINTER(vp_)], C.c_int),
+        "dnfwi_lbfgs_create_sh": ([C.c_int, C.c_int, C.c_int, C.POINTER(vp_)], C.c_int),
+        "dnfwi_assemble_sh": ([C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fp, fp, fp,
+                               fp], C.c_int),
         "dnfwi_lbfgs_destroy": ([vp_], None),
         "dnfwi_lbfgs_reset": ([vp_], None),
         "dnfwi_lbfgs_pointer": ([vp_], C.c_int),
@@ -111,12 +114,14 @@ def update_par(vpupperlim=1e9, vplowerlim=0.0, vsupperlim=1e9, vslowerlim=0.0, r
 
 class LBFGS:
     """dnfwi_lbfgs: the L-BFGS memory of one FWI run (ac: the acoustic
-    problem's two classes, no vs; pass vs arguments as None)."""
+    problem's two classes, no vs; sh: the SH problem's, no vp; pass the
+    missing parameter's arguments as None)."""
 
-    def __init__(self, nx, ny, nlbfgs, ac=False):
+    def __init__(self, nx, ny, nlbfgs, ac=False, sh=False):
         self.nx, self.ny = nx, ny
         h = C.c_void_p()
-        create = load().dnfwi_lbfgs_create_ac if ac else load().dnfwi_lbfgs_create
+        create = (load().dnfwi_lbfgs_create_sh if sh else
+                  load().dnfwi_lbfgs_create_ac if ac else load().dnfwi_lbfgs_create)
         dnio._check(create(nx, ny, nlbfgs, C.byref(h)))
         self.h = h
 
@@ -162,10 +167,20 @@ def model_update(par, it, eps_scale, vp, vs, rho, d_vp, d_vs, d_rho):
     """(vp_new, vs_new, rho_new, eps_true)."""
     m = [_f(a) for a in (vp, vs, rho)]
     d = [_f(a) for a in (d_vp, d_vs, d_rho)]
-    out = [None if (k == 1 and vs is None) else np.empty_like(m[0]) for k in range(3)]
-    nx, ny = m[0].shape
+    out = [None if m[k] is None else np.empty_like(m[2]) for k in range(3)]
+    nx, ny = m[2].shape
     e = load().dnfwi_model_update(C.byref(par), it, eps_scale, nx, ny, *[_p(a) for a in m + d + out])
     return out[0], out[1], out[2], float(e)
+
+
+def assemble_sh(dt, invmat1, grad_form, it, inv_vs_iter, inv_rho_iter, vs, rho, g_u, g_rho):
+    """dnfwi_assemble_sh on copies of the stacked sums: (g_vs, g_rho)."""
+    v, r = _f(vs), _f(rho)
+    gu, gr = _c(g_u), _c(g_rho)
+    nx, ny = v.shape
+    dnio._check(load().dnfwi_assemble_sh(nx, ny, dt, invmat1, grad_form, it, inv_vs_iter, inv_rho_iter, _p(v), _p(r),
+                                         _p(gu), _p(gr)))
+    return gu, gr
 
 
 def opt_step(L2t, epst):
@@ -179,7 +194,7 @@ def step_length(par, it, eps_scale0, scalefac, stepmax, L2t0, vp, vs, rho, d_vp,
     `state` (a LineSearch) carries epst/L2t between iterations."""
     m = [_f(a) for a in (vp, vs, rho)]
     d = [_f(a) for a in (d_vp, d_vs, d_rho)]
-    nx, ny = m[0].shape
+    nx, ny = m[2].shape
     err = []
 
     def cb(_user, pv, ps, pr, itest):

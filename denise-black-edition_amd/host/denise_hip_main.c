@@ -107,11 +107,15 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
 {
     if (p->physics < 1 || p->physics > 5)
         die("PHYSICS", "2D P/SV (1), acoustic (2), VTI (3), TTI (4) and SH (5) are built");
-    /* physics_SH.c: of its modes, forward modelling (MODE 0) is built;
-     * physics_AC.c: MODE 0, FWI (FWI_AC.c) and RTM (MODE 2) are built;
-     * physics_VTI.c / physics_TTI.c offer MODE 0 and RTM, their FWI branch
-     * is commented out */
-    if (p->physics == 5 && p->mode != 0) die("MODE", "the SH loop is built for forward modelling (MODE 0)");
+    /* physics_SH.c: forward modelling (MODE 0) and FWI (FWI_SH.c), no RTM
+     * (its RTM call is commented out); physics_AC.c: MODE 0, FWI (FWI_AC.c)
+     * and RTM (MODE 2); physics_VTI.c / physics_TTI.c offer MODE 0 and RTM,
+     * their FWI branch is commented out */
+    if (p->physics == 5 && p->mode == 2) die("MODE", "the SH loop has no RTM (physics_SH.c comments it out)");
+    if (p->physics == 5 && p->mode == 1 && p->grad_method != 2)
+        die("GRAD_METHOD", "SH FWI is built with L-BFGS (GRAD_METHOD 2); its PCG variant (store_PCG_SH) is not");
+    if (p->physics == 5 && st && st->inv_stf)
+        die("INV_STF", "the SH source-wavelet inversion (stf_sh.c) is not built");
     if ((p->physics == 3 || p->physics == 4) && p->mode == 1)
         die("MODE", "the VTI/TTI loops run forward modelling (MODE 0) and RTM (MODE 2); the reference has no "
                     "VTI/TTI FWI");
@@ -356,6 +360,7 @@ static int *read_kill(const char *path, int nsrc, int ntr)
 static float comp_weight(const run *r, int c)
 {
     const int qb = r->P->quelltypb;
+    if (r->P->physics == 5) return 1.0f; /* calc_res_SH.c:39: COMP_WEIGHT = 0 */
     const int weighted = (c == 0 && (qb == 5 || qb == 7)) || (c == 1 && (qb == 6 || qb == 7));
     if (!weighted) return 1.0f;
     const float tmp = r->c_vp * r->c_rho;
@@ -375,7 +380,11 @@ static double shot_residuals(run *r, int ishot, int test)
     dnio_par *P = r->P;
     float *obs[3] = {NULL, NULL, NULL};
     const int qb = P->quelltypb;
-    const int use[3] = {qb == 1 || qb == 3 || qb == 5 || qb == 7, qb == 1 || qb == 2 || qb == 6 || qb == 7, qb >= 4};
+    int use[3] = {qb == 1 || qb == 3 || qb == 5 || qb == 7, qb == 1 || qb == 2 || qb == 6 || qb == 7, qb >= 4};
+    if (P->physics == 5) { /* calc_res_SH.c:20-44: the vz data (SEIS_FILE_VY) for any QUELLTYPB */
+        use[0] = use[2] = 0;
+        use[1] = qb != 0;
+    }
     for (int c = 0; c < 3; c++)
         if (use[c]) {
             obs[c] = xmalloc(sizeof(float) * (size_t)r->ntr * P->nt);
@@ -708,18 +717,23 @@ static void shot_gradient(run *r, int ishot, double l2, int test, float **gs, fl
         /* grad_obj_VTI.c:196-197 / grad_obj_TTI.c: ass_gradVTI is not called
          * (and assembles nothing): the images are the raw correlation sums */
         for (int k = 0; k < 3; k++) dn_ok(dn_get_gradient(r->ctx, k, gs[k]), "image");
+    } else if (P->physics == 5) {
+        /* grad_obj_sh.c:211-238: the raw shear and density sums of the shot;
+         * FWI_SH.c assembles (ass_gradSH) after the stacking */
+        for (int k = 1; k < 3; k++) dn_ok(dn_get_gradient(r->ctx, k, gs[k]), "gradient sums");
     } else {
         dn_ok(dn_assemble_gradient(r->ctx), "gradient");
         for (int k = 0; k < 3; k++) dn_ok(dn_get_gradient(r->ctx, 3 + k, gs[k]), "gradient");
     }
-    /* grad_obj_ac.c: the acoustic problem has vp and rho gradients only */
-    const int ac = P->physics == 2;
+    /* grad_obj_ac.c: the acoustic problem has vp and rho gradients only,
+     * grad_obj_sh.c vs and rho */
+    const int ac = P->physics == 2, sh = P->physics == 5;
     if (r->eprecond) {
         /* grad_obj_psv.c:219-238, grad_obj_ac.c:223-234: energy weights of this shot */
         dn_ok(dn_get_energy(r->ctx, 0, ws), "energy");
         if (r->eprecond == 1) dn_ok(dn_get_energy(r->ctx, 1, wr), "energy");
-        io_ok(dnfwi_energy_precond(r->eprecond, P->nx, P->ny, P->dh, ws, wr, r->c_vp, r->c_vs, r->c_rho, gs[0],
-                                   ac ? NULL : gs[1], gs[2], NULL),
+        io_ok(dnfwi_energy_precond(r->eprecond, P->nx, P->ny, P->dh, ws, wr, r->c_vp, r->c_vs, r->c_rho,
+                                   sh ? NULL : gs[0], ac ? NULL : gs[1], gs[2], NULL),
               "energy preconditioning");
     }
     if (P->sws_taper_circular_per_shot) {
@@ -727,7 +741,7 @@ static void shot_gradient(run *r, int ishot, double l2, int test, float **gs, fl
         float col[8], *coeff = xmalloc(sizeof(float) * r->ncell);
         for (int q = 0; q < 8; q++) col[q] = r->srcpos[(size_t)q * r->nsrc + (ishot - 1)];
         for (int k = 0; k < 3; k++) {
-            if (ac && k == 1) continue;
+            if ((ac && k == 1) || (sh && k == 0)) continue;
             io_ok(dnfwi_taper_sources(P->nx, P->ny, P->dh, P->srtradius, P->srtshape, P->filtsize, 1, col, coeff,
                                       gs[k]),
                   "per-shot gradient taper");
@@ -735,7 +749,7 @@ static void shot_gradient(run *r, int ishot, double l2, int test, float **gs, fl
         free(coeff);
     }
     for (int k = 0; k < 3; k++) {
-        if (ac && k == 1) continue;
+        if ((ac && k == 1) || (sh && k == 0)) continue;
         for (size_t q = 0; q < r->ncell; q++) g_sum[k][q] += gs[k][q];
         /* grad_obj_psv.c:263-266; RTM_AC_out_shot.c writes the P image only */
         if (P->rtm_shot && k < (P->physics == 2 ? 1 : 2)) {
@@ -823,6 +837,7 @@ static void precondition(run *r, float **g)
     float *coeff = xmalloc(sizeof(float) * r->ncell);
     for (int k = 0; k < 3; k++) {
         if (k == 1 && P->physics == 2) continue; /* precond_AC.c: no shear-modulus gradient */
+        if (k == 0 && P->physics == 5) continue; /* precond_SH.c: vs and rho (taper files _u, _rho) */
         if (P->sws_taper_grad_vert) {
             io_ok(dnfwi_taper_vert(P->nx, P->ny, P->dh, P->gradt, r->nsrc, r->srcpos, r->ntr, r->recpos, coeff, g[k]),
                   "vertical gradient taper");
@@ -915,7 +930,7 @@ static double misfit_of(void *user, const float *vp, const float *vs, const floa
 static void write_models(const char *fmt_vp, const char *fmt_vs, const char *fmt_rho, const float *vp, const float *vs,
                          const float *rho, int nx, int ny)
 {
-    write_plane(fmt_vp, vp, nx, ny);
+    if (vp) write_plane(fmt_vp, vp, nx, ny); /* vp NULL: the SH run (model_it_out_SH.c) */
     if (vs) write_plane(fmt_vs, vs, nx, ny);
     write_plane(fmt_rho, rho, nx, ny);
 }
@@ -925,23 +940,29 @@ static void write_models(const char *fmt_vp, const char *fmt_vs, const char *fmt
  * below): two-class L-BFGS / PCG memories, calc_mat_change_test_AC, the
  * models and gradients of vp and rho, no MODEL_FILTER (commented out at
  * FWI_AC.c:784-787) and smooth_grad with the vp model (FWI_AC.c:676) */
-static void fwi(run *r, const char *workflow, float *vp, float *vs_in, float *rho, const float *taus,
+static void fwi(run *r, const char *workflow, float *vp_in, float *vs_in, float *rho, const float *taus,
                 const float *taup)
 {
     dnio_par *P = r->P;
     const int nx = P->nx, ny = P->ny;
-    const int ac = P->physics == 2;
-    float *vs = ac ? NULL : vs_in;
+    /* PHYSICS 5: src/SH/FWI_SH.c:384-872 with vs and rho (vp NULL below):
+     * per-shot raw sums, ass_gradSH on the stack, precond_SH, GRAD_FILTER
+     * and smooth_grad in that order, the normalised SH L-BFGS and the SH
+     * model update (no SCALERHO) */
+    const int ac = P->physics == 2, sh = P->physics == 5;
+    float *vs = ac ? NULL : vs_in, *vp = sh ? NULL : vp_in;
     int stagemax = 0;
     io_ok(dnio_count_stages(workflow, &stagemax), "workflow file");
     float *g[3];
     for (int k = 0; k < 3; k++) g[k] = xmalloc(sizeof(float) * r->ncell);
-    float *nv = xmalloc(sizeof(float) * r->ncell), *ns = ac ? NULL : xmalloc(sizeof(float) * r->ncell);
+    float *nv = sh ? NULL : xmalloc(sizeof(float) * r->ncell), *ns = ac ? NULL : xmalloc(sizeof(float) * r->ncell);
     float *nr = xmalloc(sizeof(float) * r->ncell);
     dnfwi_lbfgs *lb = NULL;
     dnfwi_pcg *pc = NULL;
     if (P->grad_method == 2)
-        io_ok(ac ? dnfwi_lbfgs_create_ac(nx, ny, P->nlbfgs, &lb) : dnfwi_lbfgs_create(nx, ny, P->nlbfgs, &lb),
+        io_ok(sh   ? dnfwi_lbfgs_create_sh(nx, ny, P->nlbfgs, &lb)
+              : ac ? dnfwi_lbfgs_create_ac(nx, ny, P->nlbfgs, &lb)
+                   : dnfwi_lbfgs_create(nx, ny, P->nlbfgs, &lb),
               "L-BFGS memory");
     else
         io_ok(dnfwi_pcg_create(nx, ny, &pc), "PCG memory");
@@ -983,10 +1004,12 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs_in, float *rh
             if (iter_true == 1) {
                 /* FWI_PSV.c:540-580, FWI_AC.c:478-513: the covariances are the
                  * initial model's averages (C_vs stays 0 in the acoustic run) */
-                c_vp = dnfwi_model_mean(vp, nx, ny);
+                c_vp = sh ? 0.0f : dnfwi_model_mean(vp, nx, ny);
                 c_vs = ac ? 0.0f : dnfwi_model_mean(vs, nx, ny);
                 c_rho = dnfwi_model_mean(rho, nx, ny);
-                if (ac)
+                if (sh) /* FWI_SH.c:526-569: C_vs = Vs_avg, C_rho = rho_avg (printed as rho_max) */
+                    printf(" Vs_avg = %e \t rho_max = %e\n", c_vs, c_rho);
+                else if (ac)
                     printf(" Vp_avg = %.0f \t rho_avg = %.0f\n", c_vp, c_rho);
                 else
                     printf(" Vp_avg = %.0f \t Vs_avg = %.0f \t rho_avg = %.0f\n", c_vp, c_vs, c_rho);
@@ -1001,12 +1024,23 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs_in, float *rh
             r->stf_now = r->inv_stf && iter == 1; /* grad_obj_psv.c:127 */
             const double L2sum = gradient_pass(r, g);
             r->stf_now = 0;
+            if (sh) {
+                /* FWI_SH.c:645-665: ass_gradSH on the stack, precond_SH,
+                 * GRAD_FILTER 1 (median of the gradients), then smooth_grad */
+                io_ok(dnfwi_assemble_sh(nx, ny, P->dt, P->invmat1, P->grad_form, iter, st.inv_vs_iter, st.inv_rho_iter,
+                                        vs, rho, g[1], g[2]),
+                      "ass_gradSH");
+                precondition(r, g);
+                if (P->grad_filter == 1)
+                    for (int k = 1; k < 3; k++)
+                        if (dnfwi_median_filter(nx, ny, P->filt_size_grad, g[k])) die("GRAD_FILTER", dnio_last_error());
+            }
             if (st.spatfilter == 3) {
                 /* FWI_PSV.c:628-631: smooth_grad before precond_PSV; its width
                  * follows C_vs, which is 0 in the acoustic run (an empty filter) */
                 int w = 0;
                 for (int k = 0; k < 3; k++) {
-                    if (ac && k == 1) continue;
+                    if ((ac && k == 1) || (sh && k == 0)) continue;
                     io_ok(dnfwi_gauss_filter(nx, ny, P->dh, st.wd_damp, c_vs, st.fc_end, g[k], &w), "SPATFILTER 3");
                 }
                 printf(" gradients smoothed with a %d-point Gaussian (%.2f m)\n", w, w * P->dh);
@@ -1015,32 +1049,33 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs_in, float *rh
                 /* gauss_filt_var.c with the current Vs model (FWI_PSV.c:629-631),
                  * with the Vp model in the acoustic run (FWI_AC.c:676-677) */
                 for (int k = 0; k < 3; k++) {
-                    if (ac && k == 1) continue;
+                    if ((ac && k == 1) || (sh && k == 0)) continue;
                     io_ok(dnfwi_gauss_filter_var(nx, ny, P->dh, st.wd_damp, st.wd_damp1, st.fc_end, ac ? vp : vs,
                                                  g[k]),
                           "SPATFILTER 4");
                 }
                 printf(" gradients smoothed with the velocity-dependent Gaussian\n");
             }
-            precondition(r, g); /* FWI_PSV.c:634: precond_PSV; FWI_AC.c:680: precond_AC */
+            if (!sh) precondition(r, g); /* FWI_PSV.c:634: precond_PSV; FWI_AC.c:680: precond_AC */
             const char *suffix[3] = {"vp", "vs", "rho"};
             for (int k = 0; k < 3; k++) {
-                if (ac && k == 1) continue;
+                if ((ac && k == 1) || (sh && k == 0)) continue;
                 snprintf(r->name, sizeof r->name, "%s_grad_%s", P->jacobian, suffix[k]);
                 write_plane(r->name, g[k], nx, ny);
             }
-            float *gvs = ac ? NULL : g[1];
+            float *gvs = ac ? NULL : g[1], *gvp = sh ? NULL : g[0];
             printf(" misfit (L2, test shots): %.9e\n", L2sum);
             /* FWI_PSV.c:636-676: PCG or L-BFGS search direction */
             if (lb) {
-                io_ok(dnfwi_lbfgs_direction(lb, iter, c_vp, c_vs, c_rho, g[0], gvs, g[2], vp, vs, rho), "L-BFGS");
-                /* extract_LBFGS_PSV.c:80-249 / extract_LBFGS_AC.c:60-178: JACOBIAN_p*.old, _c*.old */
-                io_ok(dnfwi_lbfgs_save(lb, P->jacobian, g[0], gvs, g[2]), "L-BFGS files");
+                io_ok(dnfwi_lbfgs_direction(lb, iter, c_vp, c_vs, c_rho, gvp, gvs, g[2], vp, vs, rho), "L-BFGS");
+                /* extract_LBFGS_PSV.c:80-249 / extract_LBFGS_AC.c:60-178 /
+                 * extract_LBFGS_SH.c:50-178: JACOBIAN_p*.old, _c*.old */
+                io_ok(dnfwi_lbfgs_save(lb, P->jacobian, gvp, gvs, g[2]), "L-BFGS files");
             } else
-                io_ok(dnfwi_pcg_direction(pc, P->pcg_beta, g[0], gvs, g[2], NULL), "PCG");
+                io_ok(dnfwi_pcg_direction(pc, P->pcg_beta, gvp, gvs, g[2], NULL), "PCG");
             if (iter == 1) min_iter_help = P->min_iter; /* FWI_PSV.c:687-690 */
             obj_env oe = {r, taus, taup};
-            io_ok(dnfwi_step_length(&up, iter, P->eps_scale, P->scalefac, P->stepmax, L2sum, nx, ny, vp, vs, rho, g[0],
+            io_ok(dnfwi_step_length(&up, iter, P->eps_scale, P->scalefac, P->stepmax, L2sum, nx, ny, vp, vs, rho, gvp,
                                     gvs, g[2], misfit_of, &oe, &ls),
                   "step length");
             /* FWI_PSV.c:700-708 */
@@ -1067,8 +1102,8 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs_in, float *rh
             L2_hist[iter] = L2sum;
             /* FWI_PSV.c:750 / FWI_AC.c:782: the update, written as
              * INV_MODELFILE_{vp,vs,rho}.bin (vp and rho for the acoustic run) */
-            dnfwi_model_update(&up, iter, eps_scale, nx, ny, vp, vs, rho, g[0], gvs, g[2], nv, ns, nr);
-            memcpy(vp, nv, sizeof(float) * r->ncell);
+            dnfwi_model_update(&up, iter, eps_scale, nx, ny, vp, vs, rho, gvp, gvs, g[2], nv, ns, nr);
+            if (vp) memcpy(vp, nv, sizeof(float) * r->ncell);
             if (vs) memcpy(vs, ns, sizeof(float) * r->ncell);
             memcpy(rho, nr, sizeof(float) * r->ncell);
             snprintf(name[0], sizeof name[0], "%s_vp.bin", P->inv_modelfile);
@@ -1080,7 +1115,8 @@ static void fwi(run *r, const char *workflow, float *vp, float *vs_in, float *rh
             if (!ac && P->model_filter == 1 && P->filt_size != 0) {
                 float *mods[3] = {vp, vs, rho};
                 for (int k = 0; k < 3; k++)
-                    if (dnfwi_median_filter(nx, ny, P->filt_size, mods[k])) die("MODEL_FILTER", dnio_last_error());
+                    if (mods[k] && dnfwi_median_filter(nx, ny, P->filt_size, mods[k]))
+                        die("MODEL_FILTER", dnio_last_error());
                 printf(" models smoothed with a %d-point median filter\n", abs(P->filt_size) | 1);
             }
             /* FWI_PSV.c:781-827: stage end on the misfit change over two
@@ -1231,7 +1267,7 @@ int main(int argc, char **argv)
     dp.physics = P.physics;
     dp.seismo = P.mode >= 1 && P.seismo == 0 ? 1 : P.seismo;
     /* outseis_SHfor.c writes SH seismograms for SEISMO 1 only */
-    if (P.physics == 5) dp.seismo = P.seismo == 1;
+    if (P.physics == 5) dp.seismo = P.seismo == 1 || P.mode >= 1;
     dp.ndt = P.ndt;
     dp.invmat1 = P.invmat1;
     dp.grad_form = P.grad_form;

@@ -61,8 +61,10 @@ class LBFGS:
     problem), q, r, rho, alpha, the cyclic pointer, and the previous model /
     normalised gradients (the JACOBIAN_*.old files)."""
 
-    def __init__(self, nx, ny, nlbfgs, nclass=3):
-        self.nclass = nclass
+    def __init__(self, nx, ny, nlbfgs, nclass=3, sh=False):
+        self.nclass = 2 if sh else nclass
+        self.sh = sh
+        nclass = self.nclass
         self.n = nclass * nx * ny
         self.nl = nlbfgs
         self.y = np.zeros((nlbfgs + 1, self.n), F32)   # row 0 unused: slots 1..NLBFGS
@@ -89,11 +91,16 @@ class LBFGS:
         if self.pointer > self.nl:
             self.pointer = 1
         # store_LBFGS_PSV.c:37-89: waveconv = C * waveconv; gradp = waveconv
-        ac = self.nclass == 2
-        w_vp = (c_vp * g_vp.astype(F32)).astype(F32)
+        sh = self.sh
+        ac = self.nclass == 2 and not sh
+        w_vp = None if sh else (c_vp * g_vp.astype(F32)).astype(F32)
         w_vs = None if ac else (c_vs * g_vs.astype(F32)).astype(F32)
         w_rho = (c_rho * g_rho.astype(F32)).astype(F32)
-        if ac:   # store_LBFGS_AC.c:63-150
+        if sh:   # store_LBFGS_SH.c:36-134: [vs | rho], models (m - 0) / C
+            gradp = np.concatenate([w_vs.ravel(), w_rho.ravel()])
+            model = np.concatenate([((vs.astype(F32) - F32(0.0)) / c_vs).astype(F32).ravel(),
+                                    ((rho.astype(F32) - F32(0.0)) / c_rho).astype(F32).ravel()]).astype(F32)
+        elif ac:   # store_LBFGS_AC.c:63-150
             gradp = np.concatenate([w_rho.ravel(), w_vp.ravel()])
             model = np.concatenate([rho.ravel(), vp.ravel()]).astype(F32)
         else:
@@ -120,6 +127,10 @@ class LBFGS:
         # extract_LBFGS_PSV.c:80-249: old model and old normalised gradient
         self.mold = model.copy()
         self.gold = gradp.copy()
+        if it > 1 and sh:
+            # extract_LBFGS_SH.c:27-48: r itself, not scaled back
+            nc = g_vs.size
+            return None, self.r[:nc].reshape(g_vs.shape).copy(), self.r[nc:].reshape(g_rho.shape).copy()
         if it > 1:
             # extract_LBFGS_PSV.c:31-78
             nc = g_vp.size
@@ -150,6 +161,8 @@ def mat_change_test(par, it, eps_scale, vp, vs, rho, d_vp, d_vs, d_rho):
     calc_mat_change_test_AC (vp and rho; vs_new None, eps_true = EPSILON)."""
     if vs is None:
         return _mat_change_test_ac(par, it, eps_scale, vp, rho, d_vp, d_rho)
+    if vp is None:
+        return _mat_change_test_sh(par, it, eps_scale, vs, rho, d_vs, d_rho)
     eps_scale = F32(eps_scale)
     pimax, gmax = _maxima(vp, d_vp)
     umax, gmax_u = _maxima(vs, d_vs)
@@ -208,6 +221,65 @@ def _mat_change_test_ac(par, it, eps_scale, vp, rho, d_vp, d_rho):
     p1 = np.where(p1 < 0, vp, p1)
     r1 = np.where(r1 < 0, rho, r1)
     return p1.astype(F32), None, r1.astype(F32), eps
+
+
+def _mat_change_test_sh(par, it, eps_scale, vs, rho, d_vs, d_rho):
+    """calc_mat_change_test_SH.c:29-170: the steps of vs (EPSILON_u) and rho
+    (EPSILON_rho, no SCALERHO), the limits for INVMAT1 1 (the vs lower limit
+    above 1e-6 only), no negative values; eps_true = EPSILON_u."""
+    eps_scale = F32(eps_scale)
+    umax, gmax_u = _maxima(vs, d_vs)
+    rhomax, gmax_rho = _maxima(rho, d_rho)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        eps_u = F32(eps_scale * F32(umax / gmax_u))
+        eps_rho = F32(eps_scale * F32(rhomax / gmax_rho))
+    if it < par["inv_vs_iter"]:
+        eps_u = F32(0.0)
+    if it < par["inv_rho_iter"]:
+        eps_rho = F32(0.0)
+    if par["invmat1"] not in (1, 3):
+        return None, vs.copy(), rho.copy(), eps_u
+    u1 = (vs - (eps_u * d_vs).astype(F32)).astype(F32)
+    r1 = (rho - (eps_rho * d_rho).astype(F32)).astype(F32)
+    if par["invmat1"] == 1:
+        u1 = np.where((u1 < F32(par["vslowerlim"])) & (u1.astype(np.float64) > 1e-6), vs, u1)
+        u1 = np.where(u1 > F32(par["vsupperlim"]), vs, u1)
+        r1 = np.where(r1 < F32(par["rholowerlim"]), rho, r1)
+        r1 = np.where(r1 > F32(par["rhoupperlim"]), rho, r1)
+    u1 = np.where(u1 < 0, vs, u1)
+    r1 = np.where(r1 < 0, rho, r1)
+    return None, u1.astype(F32), r1.astype(F32), eps_u
+
+
+def assemble_sh(dt, invmat1, grad_form, it, inv_vs_iter, inv_rho_iter, vs, rho, g_u, g_rho):
+    """ass_gradSH.c:19-96 on the stacked sums (nx, ny): (g_vs, g_rho)."""
+    dt = F32(dt)
+    vs = vs.astype(F32)
+    rho = rho.astype(F32)
+    wmu = ((-dt) * g_u.astype(F32)).astype(F32)
+    gu = g_u.astype(F32).copy()
+    gr = g_rho.astype(F32).copy()
+    rho_s = ((-dt) * g_rho.astype(F32)).astype(F32)
+    v64, r64, w64 = vs.astype(np.float64), rho.astype(np.float64), wmu.astype(np.float64)
+    pos = vs > 0
+    if invmat1 == 1 and grad_form == 1:
+        vs3 = ((vs * vs).astype(F32) * vs).astype(F32).astype(np.float64)
+        gu = np.where(pos, ((2.0 / r64 * vs3) * w64).astype(F32), gu)
+        with np.errstate(divide="ignore"):
+            den = ((((vs * vs).astype(F32) * rho).astype(F32)) * rho).astype(F32).astype(np.float64)
+            vs2rho2 = (1.0 / den).astype(F32)
+        gr = np.where(pos, ((vs2rho2 * wmu).astype(F32) + rho_s).astype(F32), gr)
+    elif invmat1 == 1:
+        gu = (((2.0 * r64) * v64) * w64).astype(F32)
+        gr = ((((vs * vs).astype(F32) * wmu).astype(F32)) + rho_s).astype(F32)
+    elif invmat1 == 3:
+        gu = wmu
+        gr = rho_s
+    if it < inv_vs_iter:
+        gu = np.zeros_like(gu)
+    if it < inv_rho_iter:
+        gr = np.zeros_like(gr)
+    return gu.astype(F32), gr.astype(F32)
 
 
 def opt_step(L2t, epst):
@@ -342,7 +414,7 @@ def energy_precond(eprecond, dh, Ws, Wr, c_vp, c_vs, c_rho, g_vp, g_vs, g_rho):
                 maxwe = w
     We = (We + F32(F32(0.005) * maxwe)).astype(F32)
     c_vp, c_vs, c_rho = F32(c_vp), F32(c_vs), F32(c_rho)
-    gv = (g_vp / ((We * c_vp).astype(F32) * c_vp).astype(F32)).astype(F32)
+    gv = None if g_vp is None else (g_vp / ((We * c_vp).astype(F32) * c_vp).astype(F32)).astype(F32)
     gs = None
     if g_vs is not None:
         gs = (g_vs / ((We * c_vs).astype(F32) * c_vs).astype(F32)).astype(F32)

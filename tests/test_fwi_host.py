@@ -14,6 +14,7 @@ import pytest
 
 import dnfwi
 import fwi_oracle as fo
+from cases import rel_l2
 
 F32 = np.float32
 NX, NY = 9, 7
@@ -831,3 +832,118 @@ def test_energy_precond_and_pcg_ac():
             assert a[3][0] == b[3][0] == o[3][0] and a[3][2] == b[3][2] == o[3][2] and a[3][1] == 0.0
         p2.close()
         p3.close()
+
+
+# ---- the SH problem: vs and rho (src/SH/FWI_SH.c) ----
+
+def test_lbfgs_sh_matches_oracle_and_normalised_three_class_run():
+    """store_LBFGS_SH / extract_LBFGS_SH ([vs | rho] vectors of normalised
+    models vs / C_vs, rho / C_rho and gradients C g; the direction left
+    normalised): the product equals the SH oracle bit for bit; and the
+    oracle's direction equals, up to float rounding of the normalisation,
+    the three-class recursion run on the normalised models with C = 1 and
+    a zero vp class (the same two-loop recursion, pinned against the BFGS
+    matrix recursion above)."""
+    rng = np.random.default_rng(31)
+    lb, ob, o3 = dnfwi.LBFGS(NX, NY, 3, sh=True), fo.LBFGS(NX, NY, 3, sh=True), fo.LBFGS(NX, NY, 3)
+    _, vs, rho = models(rng)
+    zero = np.zeros((NX, NY), F32)
+    c = (F32(0.0), fo.model_mean(vs), fo.model_mean(rho))
+    for it in range(1, 7):
+        g = [None, rnd(rng, -1e-3, 1e-3), rnd(rng, -1e-3, 1e-3)]
+        dp = lb.direction(it, *c, *g, None, vs, rho)
+        do = ob.direction(it, *c, *g, None, vs, rho)
+        # three-class run on normalised quantities (C = 1): vs class <- vs / C_vs, gradient C_vs g
+        gn = [zero, (c[1] * g[1]).astype(F32), (c[2] * g[2]).astype(F32)]
+        mn = [zero, (vs / c[1]).astype(F32), (rho / c[2]).astype(F32)]
+        d3 = o3.direction(it, 1.0, 1.0, 1.0, *gn, *mn)
+        assert dp[0] is None and do[0] is None and lb.pointer() == ob.pointer
+        for k in (1, 2):
+            assert np.array_equal(dp[k], do[k]), (it, k)
+            assert rel_l2(do[k], d3[k]) < 1e-5, (it, k)
+        vs = (vs - F32(0.3) * dp[1] / np.abs(dp[1]).max()).astype(F32)
+        rho = (rho - F32(0.2) * dp[2] / np.abs(dp[2]).max()).astype(F32)
+    lb.close()
+
+
+def test_lbfgs_sh_files(tmp_path):
+    """extract_LBFGS_SH.c:50-178: _p_vs.old holds the normalised model
+    vs / C_vs, _p_u.old the normalised gradient, _c_u.old the direction;
+    likewise _p_mrho, _p_rho, _c_rho; no vp files."""
+    rng = np.random.default_rng(33)
+    lb = dnfwi.LBFGS(NX, NY, 2, sh=True)
+    _, vs, rho = models(rng)
+    c = (F32(0.0), fo.model_mean(vs), fo.model_mean(rho))
+    jac = str(tmp_path / "jac")
+    g = [None, rnd(rng, -1e-3, 1e-3), rnd(rng, -1e-3, 1e-3)]
+    d = lb.direction(1, *c, *g, None, vs, rho)
+    lb.save(jac, None, d[1], d[2])
+    assert np.array_equal(np.fromfile(f"{jac}_p_vs.old", np.float32), ((vs - F32(0.0)) / c[1]).astype(F32).ravel())
+    assert np.array_equal(np.fromfile(f"{jac}_p_mrho.old", np.float32), ((rho - F32(0.0)) / c[2]).astype(F32).ravel())
+    assert np.array_equal(np.fromfile(f"{jac}_p_u.old", np.float32), (c[1] * g[1]).astype(F32).ravel())
+    assert np.array_equal(np.fromfile(f"{jac}_c_rho.old", np.float32), d[2].ravel())
+    for n in ("p_vp", "p", "c"):
+        assert not (tmp_path / f"jac_{n}.old").exists()
+    lb.close()
+
+
+def test_model_update_sh():
+    """calc_mat_change_test_SH: product = oracle; the vs and rho of the P/SV
+    update with SCALERHO 1 (the SH rho step has no SCALERHO); eps_true is
+    EPSILON_u."""
+    rng = np.random.default_rng(35)
+    for trial in range(5):
+        vp, vs, rho = models(rng)
+        vs[0, :2] = 0.0
+        d = [rnd(rng) for _ in range(3)]
+        par = dict(PAR, inv_vs_iter=2 if trial == 2 else 0, invmat1=3 if trial == 3 else 1)
+        eps = [0.01, 0.5, 0.05, 0.3, 0.0][trial]
+        got = dnfwi.model_update(dnfwi.update_par(**par), 1, eps, None, vs, rho, None, d[1], d[2])
+        ref = fo.mat_change_test(par, 1, eps, None, vs, rho, None, d[1], d[2])
+        full = fo.mat_change_test(dict(par, scalerho=1.0), 1, eps, vp, vs, rho, *d)
+        assert got[0] is None and ref[0] is None
+        for k in (1, 2):
+            assert np.array_equal(got[k], ref[k]), trial
+            assert np.array_equal(ref[k], full[k]), trial
+        assert got[3] == float(ref[3]) == float(full[3])
+
+
+def test_assemble_sh():
+    """ass_gradSH on stacked sums: the product equals the oracle; INVMAT1 3
+    gives mu' = -DT g_u and rho' = -DT g_rho; INVMAT1 1 GRAD_FORM 2 is the
+    chain rule of those (2 rho vs mu', vs^2 mu' + rho'); INV_*_ITER zero
+    the gradients; cells with vs = 0 keep the GRAD_FORM 1 sums."""
+    rng = np.random.default_rng(37)
+    _, vs, rho = models(rng)
+    vs[1, :3] = 0.0
+    gu, gr = rnd(rng, -1.0, 1.0), rnd(rng, -1.0, 1.0)
+    dt = 1e-3
+    for invmat1 in (1, 3):
+        for gf in (1, 2):
+            a = dnfwi.assemble_sh(dt, invmat1, gf, 1, 0, 0, vs, rho, gu, gr)
+            b = fo.assemble_sh(dt, invmat1, gf, 1, 0, 0, vs, rho, gu, gr)
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y), (invmat1, gf)
+    mu3, rho3 = dnfwi.assemble_sh(dt, 3, 1, 1, 0, 0, vs, rho, gu, gr)
+    assert np.array_equal(mu3, (F32(-dt) * gu).astype(F32)) and np.array_equal(rho3, (F32(-dt) * gr).astype(F32))
+    v1, r1 = dnfwi.assemble_sh(dt, 1, 2, 1, 0, 0, vs, rho, gu, gr)
+    v64, r64, m64 = vs.astype(np.float64), rho.astype(np.float64), mu3.astype(np.float64)
+    assert rel_l2(v1, 2 * r64 * v64 * m64) < 1e-6
+    assert rel_l2(r1, v64 * v64 * m64 + rho3) < 1e-6
+    g1u, g1r = dnfwi.assemble_sh(dt, 1, 1, 1, 0, 0, vs, rho, gu, gr)
+    assert np.array_equal(g1u[1, :3], gu[1, :3]) and np.array_equal(g1r[1, :3], gr[1, :3])
+    z = dnfwi.assemble_sh(dt, 1, 1, 1, 2, 2, vs, rho, gu, gr)
+    assert not z[0].any() and not z[1].any()
+
+
+def test_energy_precond_sh():
+    """grad_obj_sh.c:211-219: the vs and rho sums divided by We C^2, vp absent."""
+    rng = np.random.default_rng(39)
+    Ws = rnd(rng, 0.1, 2.0)
+    g = [rnd(rng) for _ in range(3)]
+    c = (F32(2500.0), F32(1400.0), F32(2100.0))
+    We, gv, gs, gr = dnfwi.energy_precond(3, 10.0, Ws, None, *c, None, g[1], g[2])
+    We3, _, gs3, gr3 = dnfwi.energy_precond(3, 10.0, Ws, None, *c, *g)
+    assert gv is None and np.array_equal(gs, gs3) and np.array_equal(gr, gr3)
+    oW, ov, os_, orr = fo.energy_precond(3, 10.0, Ws, None, *c, None, g[1], g[2])
+    assert ov is None and np.array_equal(gs, os_) and np.array_equal(gr, orr)

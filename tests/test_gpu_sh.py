@@ -6,7 +6,9 @@ without the FREE_SURF switch, every source type, both INVMAT1
 parametrisations and every scalar tile shape; the error against the FP64
 oracle stays within the seismogram tolerance (rel-L2 1e-5).  The drop-in
 executable writes the library's vz seismograms (SEIS_FILE_VY) and the
-reference's model copies for PHYSICS 5 parameter files."""
+reference's model copies for PHYSICS 5 parameter files.  The SH gradient
+(sh.c mode 1, ass_gradSH) equals the oracle bit for bit, and the
+executable's SH FWI (MODE 1) equals a Python re-run of its iterations."""
 import os
 import subprocess
 
@@ -321,3 +323,153 @@ def test_gradient_error_against_fp64_oracle():
         print(f"[sh grad] gradient {k} rel-L2 vs FP64 oracle {err:.2e}", flush=True)
         assert err <= 1e-4
     g.close()
+
+
+@pytest.mark.parametrize("variant", ["plain", "precond", "spatfilter3"])
+def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
+    """MODE 1 with PHYSICS 5 (FWI_SH.c, grad_obj_sh.c, ass_gradSH.c,
+    step_length_est_sh.c, calc_mat_change_test_SH.c): three iterations of
+    the executable against the same iterations in Python, with the
+    library's raw SH sums and the numpy restatement's stacking, ass_gradSH,
+    SH L-BFGS ([vs | rho], normalised models), step search and update of vs
+    and rho.  The vs and rho models after every iteration, INV_MODELFILE
+    _vs/_rho.bin, the misfit log and the L-BFGS files equal the Python
+    run's bit for bit; no vp file is written; the misfit falls.  plain:
+    L-BFGS.  precond: EPRECOND 3 (energy of vz, eprecond_SH.c) divided out
+    per shot, the horizontal taper (precond_SH) and GRAD_FILTER 1 (a 3 x 3
+    median of the gradients) after it.  spatfilter3: the Gaussian gradient
+    filter with the start model's mean vs."""
+    import fwi_oracle as fo
+    from test_gpu_cli import _files, _run
+    itermax = 3
+    precond = variant == "precond"
+    spat3 = variant == "spatfilter3"
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
+    nx, ny, dh, nt = 96, 64, 10.0, 400
+    _, vs0, rho0 = synth.layered_model(nx, ny, dh, seed=6, sigma=3.0)
+    dt = float(np.float32(synth.stable_dt(float(vs0.max()) * 1.2, dh, 8, safety=0.6)))
+    vt, rt = vs0.copy(), rho0.copy()
+    vt[40:56, 10:20] *= 1.08
+    rt[40:56, 10:20] *= 1.05
+    sources = [(300.0, 30.0, 15.0, 1.0, 1), (650.0, 30.0, 15.0, 1.0, 1)]
+    receivers = [(40.0 + 20.0 * k, 20.0) for k in range(40)]
+    _files(tmp, nx, ny, dh, (vt, vt, rt), sources, receivers, mname="model/true")
+    inpfile.write_inp(os.path.join(tmp, "true.inp"), PHYSICS=5, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/true", SEIS_FILE_VX="obs/d_x.su", SEIS_FILE_VY="obs/d_y.su", SEISMO=1)
+    _run(tmp, "true.inp")
+    _files(tmp, nx, ny, dh, (vs0, vs0, rho0), sources, receivers, mname="model/start")
+    inpfile.write_inp(os.path.join(tmp, "fwi.inp"), PHYSICS=5, MODE=1, QUELLTYPB=1, SEISMO=1, NX=nx, NY=ny, DH=dh,
+                      DT=dt, TIME=nt * dt, MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1", ITERMAX=itermax,
+                      INV_MOD_OUT=1, MISFIT_LOG_FILE="misfit.dat", SWS_TAPER_GRAD_HOR=int(precond),
+                      GRADT="3, 12, 60, 62", EXP_TAPER_GRAD_HOR=2.0, GRAD_METHOD=2, GRAD_FILTER=int(precond),
+                      FILT_SIZE_GRAD=3)
+    fce = 10.0 if spat3 else 0.0
+    inpfile.write_workflow(os.path.join(tmp, "wf.inp"), fc_end=fce, eprecond=3 if precond else 0,
+                           spatfilter=3 if spat3 else 0, wd_damp=0.5)
+    out = _run(tmp, "fwi.inp", "wf.inp")
+    assert "Vs_avg" in out
+
+    par = dnio.read_par(os.path.join(tmp, "fwi.inp"))
+    cwd = os.getcwd()
+    os.chdir(tmp)
+    try:
+        src, ts = dnio.read_sources(par.get("source_file"))
+        rec = dnio.read_receivers(par)
+    finally:
+        os.chdir(cwd)
+    shots = []
+    for ishot in (1, 2):
+        loc = dnio.split_sources(par, src, ishot - 1, 1)
+        shots.append((loc, dnio.wavelet(par, loc, ishot, ts)))
+    p = dnhip.Params(nx=nx, ny=ny, nt=par.nt, dh=dh, dt=par.dt, fdorder=par.fdorder, maxrelerror=par.maxrelerror,
+                     fw=par.fw, free_surf=par.free_surf, seismo=1, ts=ts, physics=5, mode=1, dtinv=2,
+                     grad_form=par.grad_form, quelltypb=1, damping=par.damping, fpml=par.fpml, npower=par.npower,
+                     k_max_pml=par.k_max_pml)
+    st = dnio.read_stage(os.path.join(tmp, "wf.inp"), 1)
+    upd = dict(vpupperlim=par.vpupperlim, vplowerlim=par.vplowerlim, vsupperlim=par.vsupperlim,
+               vslowerlim=par.vslowerlim, rhoupperlim=par.rhoupperlim, rholowerlim=par.rholowerlim,
+               scalerho=st.scalerho, inv_vp_iter=st.inv_vp_iter, inv_vs_iter=st.inv_vs_iter,
+               inv_rho_iter=st.inv_rho_iter, invmat1=par.invmat1)
+    obs = {i: dnio.read_su(os.path.join(tmp, f"obs/d_y.su.shot{i}"), rec.shape[1], nt) for i in (1, 2)}
+    g = dnhip.Context(p)
+    if precond:
+        g.set_energy_weights(3)
+
+    def shot(i, vs, rho):
+        g.set_model(None, vs, rho)
+        g.set_sources(*shots[i - 1])
+        g.set_receivers(rec[:2])
+        g.forward()
+
+    def misfit(tv, ts_, tr, itest):
+        assert tv is None
+        l2 = 0.0
+        for i in (1, 2):
+            shot(i, ts_, tr)
+            l2 += g.residuals(None, obs[i], None, 1)
+        return l2
+
+    vs, rho = vs0.copy(), rho0.copy()
+    lb = fo.LBFGS(nx, ny, par.nlbfgs, sh=True)
+    c = (np.float32(0.0), fo.model_mean(vs), fo.model_mean(rho))
+    carry = ([0.0, 0.0, 0.0], [0.0, 0.0, 0.0])
+    lines, l2s = [], []
+    for it in range(1, itermax + 1):
+        gu, gr = np.zeros((nx, ny), np.float32), np.zeros((nx, ny), np.float32)
+        l2sum = 0.0
+        for i in (1, 2):
+            shot(i, vs, rho)
+            l2sum += g.residuals(None, obs[i], None, 1)
+            g.adjoint()
+            su, sr = g.gradient(1), g.gradient(2)
+            if precond:
+                _, _, su, sr = fo.energy_precond(3, par.dh, g.energy(0), None, *c, None, su, sr)
+            gu += su
+            gr += sr
+        gu, gr = fo.assemble_sh(par.dt, par.invmat1, par.grad_form, it, st.inv_vs_iter, st.inv_rho_iter, vs, rho,
+                                gu, gr)
+        if precond:
+            coeff = fo.taper_hor(nx, ny, par.dh, 3, 12, 2.0)
+            gu, gr = (gu * coeff).astype(np.float32), (gr * coeff).astype(np.float32)
+            gu, gr = fo.median_filter(gu, 3), fo.median_filter(gr, 3)
+        if spat3:
+            (gu, wu), (gr, _) = fo.gauss_filter(gu, par.dh, 0.5, float(c[1]), fce), fo.gauss_filter(gr, par.dh, 0.5,
+                                                                                               float(c[1]), fce)
+            assert wu >= 3
+        l2s.append(l2sum)
+        d = lb.direction(it, *c, None, gu, gr, None, vs, rho)
+        saved = (lb.mold.copy(), lb.gold.copy(), [None if a is None else a.copy() for a in d])
+        ls = fo.step_length(upd, it, par.eps_scale, par.scalefac, par.stepmax, l2sum, None, vs, rho, *d, misfit,
+                            carry=carry)
+        carry = (ls["epst"], ls["L2t"])
+        eps = ls["eps_scale"]
+        if it > par.min_iter and ls["step1"] == 0:
+            eps = np.float32(0.0)
+        lines.append("%e \t %e \t %e \t %e \t %e \t %e \t %e \t %e \t %d " % (eps, *ls["epst"], *ls["L2t"], l2sum, 1))
+        _, vs, rho, _ = fo.mat_change_test(upd, it, eps, None, vs, rho, *d)
+        print(f"[sh fwi] iteration {it}: misfit {l2sum:.6e}, step {float(eps):.4e}, {ls['nevals']} test runs",
+              flush=True)
+        if it > par.min_iter and ((it > 2 and abs((l2s[it - 3] - l2s[it - 1]) / l2s[it - 3]) <= st.pro)
+                                  or ls["step3"] == 1):
+            break
+        for name, a in (("vs", vs), ("rho", rho)):
+            got = np.fromfile(os.path.join(tmp, f"model/inv_{name}_stage_1_it_{it}.bin"), np.float32)
+            assert np.array_equal(got.reshape(nx, ny), a), (it, name)
+        assert not os.path.exists(os.path.join(tmp, f"model/inv_vp_stage_1_it_{it}.bin"))
+    g.close()
+    for name, a in (("vs", vs), ("rho", rho)):
+        got = np.fromfile(os.path.join(tmp, f"model/inv_{name}.bin"), np.float32).reshape(nx, ny)
+        assert np.array_equal(got, a), name
+    assert not os.path.exists(os.path.join(tmp, "model/inv_vp.bin"))
+    assert open(os.path.join(tmp, "misfit.dat")).read().splitlines() == lines
+    jac = os.path.join(tmp, "jac/j")
+    mold, gold, dd = saved
+    nc = nx * ny
+    for names, slot, dk in ((("p_vs", "p_u", "c_u"), 0, dd[1]), (("p_mrho", "p_rho", "c_rho"), 1, dd[2])):
+        want = (mold[slot * nc:(slot + 1) * nc], gold[slot * nc:(slot + 1) * nc], dk.ravel())
+        for n, w in zip(names, want):
+            assert np.array_equal(np.fromfile(f"{jac}_{n}.old", np.float32), w), n
+    assert not os.path.exists(f"{jac}_p_vp.old")
+    assert l2s[-1] < l2s[0]
+    assert not np.array_equal(vs, vs0)
