@@ -312,7 +312,13 @@ int validate(const dn_params *p)
     if (!(p->dh > 0.0f) || !(p->dt > 0.0f)) return fail(DN_ERR_ARG, "DH and DT must be positive");
     if (p->physics < 0 || p->physics > 5)
         return fail(DN_ERR_ARG, "PHYSICS must be 1 (P/SV), 2 (acoustic), 3 (VTI), 4 (TTI) or 5 (SH)");
-    if (p->physics >= 2 && p->L != 0) return fail(DN_ERR_ARG, "acoustic, VTI, TTI and SH runs are elastic: L = 0");
+    if (p->physics >= 2 && p->physics <= 4 && p->L != 0)
+        return fail(DN_ERR_ARG, "acoustic, VTI and TTI runs are elastic: L = 0");
+    // FD_SH.c:171-206: viscoelastic SH forward modelling; its coefficients
+    // share the eight material planes (2 + 2 L)
+    if (p->physics == 5 && p->L > 3) return fail(DN_ERR_ARG, "viscoelastic SH runs take L <= 3");
+    if (p->physics == 5 && p->L > 0 && p->mode != 0)
+        return fail(DN_ERR_ARG, "viscoelastic SH runs are forward modelling (MODE 0; FWI_SH_visc is not built)");
     // the acoustic and VTI / TTI loops also run the RTM passes (physics_AC.c,
     // physics_VTI.c:27-29, physics_TTI.c); the SH loop its FWI gradient
     // (sh.c mode 1, FWI_SH.c)
@@ -427,7 +433,7 @@ std::vector<size_t> plan_sizes(const dn_params &p, int nsrc, int ntr)
     };
     add(pb, F_N - (gf2 ? 0 : 3));
     add(pb, 2);                                   // rip, rjp
-    const int nani = aniso_planes(p.physics);
+    const int nani = aniso_planes(p.physics, p.L);
     if (nani) add(pb, nani);                      // stiffness planes
     else if (p.L == 0) add(pb, 3);                // muipjp, f, g
     if (p.mode > 0) add(pb, 3);                   // vp, vs, rho
@@ -560,8 +566,8 @@ static int ctx_create(const dn_params *p, int device, dn_arena *arena, dn_ctx **
         if ((rc = c->dalloc((void **)&c->fld[k], pb))) return bail(rc);
     }
     if ((rc = c->dalloc((void **)&c->rip, pb)) || (rc = c->dalloc((void **)&c->rjp, pb))) return bail(rc);
-    if (aniso_planes(p->physics)) {
-        for (int k = 0; k < aniso_planes(p->physics); ++k)
+    if (aniso_planes(p->physics, p->L)) {
+        for (int k = 0; k < aniso_planes(p->physics, p->L); ++k)
             if ((rc = c->dalloc((void **)&c->ani[k], pb))) return bail(rc);
     } else if (p->L == 0) {
         float **mats[] = {&c->muipjp, &c->f, &c->g};
@@ -655,21 +661,27 @@ static int set_model_ac(dn_ctx *c, const float *vp, const float *rho)
 // (INVMAT1 3) and rho; checkfd_elast_SH's DT limit is the acoustic one with
 // the shear velocity (checkfd_elast_SH.c:30-75), then matcopy_elastic_SH,
 // av_mu_SH and inv_rho_SH (src/SH/FD_SH.c:175-199).
-static int set_model_sh(dn_ctx *c, const float *vs, const float *rho)
+static int set_model_sh(dn_ctx *c, const float *vs, const float *rho, const float *taus)
 {
     if (!vs || !rho) return fail(DN_ERR_ARG, "null argument");
+    if (c->p.L > 0 && !taus) return fail(DN_ERR_ARG, "a viscoelastic SH model needs taus");
     HIP_OK(hipSetDevice(c->device));
     std::string err;
-    int rc = check_stability_ac(c->p, c->setup.hc, vs, rho, err);
+    int rc = c->p.L > 0 ? check_stability_sh_visc(c->p, c->setup.hc, vs, rho, taus, err)
+                        : check_stability_ac(c->p, c->setup.hc, vs, rho, err);
     if (rc) return fail(rc, err);
-    prepare_material_sh(c->p, c->lay, vs, rho, c->mat);
+    prepare_material_sh(c->p, c->lay, vs, rho, taus, c->mat);
     const size_t pl = c->lay.plane();
-    if ((rc = upload(c, c->rip, c->mat.rip.data(), pl)) || (rc = upload(c, c->ani[0], c->mat.ani[0].data(), pl)) ||
-        (rc = upload(c, c->ani[1], c->mat.ani[1].data(), pl)))
-        return rc;
+    if ((rc = upload(c, c->rip, c->mat.rip.data(), pl))) return rc;
+    for (int k = 0; k < aniso_planes(5, c->p.L); ++k)
+        if ((rc = upload(c, c->ani[k], c->mat.ani[k].data(), pl))) return rc;
     if (c->vs && ((rc = upload(c, c->vs, c->mat.vs.data(), pl)) || (rc = upload(c, c->rho, c->mat.rho.data(), pl))))
         return rc;
     HIP_OK(hipStreamSynchronize(c->stream));
+    for (int k = 2; k < 8; ++k) {
+        c->mat.ani[k].clear();
+        c->mat.ani[k].shrink_to_fit();
+    }
     for (auto *v : {&c->mat.rip, &c->mat.ani[0], &c->mat.ani[1], &c->mat.vs, &c->mat.rho}) {
         v->clear();
         v->shrink_to_fit();
@@ -684,7 +696,7 @@ int dn_set_model(dn_ctx *c, const float *vp, const float *vs, const float *rho, 
                  const float *taup)
 {
     if (c && c->p.physics == 2) return set_model_ac(c, vp, rho);
-    if (c && c->p.physics == 5) return set_model_sh(c, vs, rho);
+    if (c && c->p.physics == 5) return set_model_sh(c, vs, rho, taus);
     if (!c || !vp || !vs || !rho) return fail(DN_ERR_ARG, "null argument");
     if (aniso_planes(c->p.physics)) return fail(DN_ERR_ARG, "VTI/TTI models are set with dn_set_model_aniso");
     if (c->p.L > 0 && (!taus || !taup)) return fail(DN_ERR_ARG, "viscoelastic model needs taus and taup");

@@ -67,9 +67,10 @@ struct HostMaterial {
 // material planes of the acoustic (physics 2: the modulus g), the
 // anisotropic (3 VTI, 4 TTI: the stiffness planes) and the SH runs (5: the
 // shear moduli uip, ujp); 0 for isotropic P/SV
-inline int aniso_planes(int physics)
+inline int aniso_planes(int physics, int L = 0)
 {
-    return physics == 2 ? 1 : (physics == 3 ? 4 : (physics == 4 ? 8 : (physics == 5 ? 2 : 0)));
+    // SH: uip, ujp, or with L > 0 fipjp, f, dip[1..L], d[1..L]
+    return physics == 2 ? 1 : (physics == 3 ? 4 : (physics == 4 ? 8 : (physics == 5 ? 2 + 2 * L : 0)));
 }
 
 // the physics whose sweeps have scalar columns only in both sweeps
@@ -119,7 +120,10 @@ void prepare_material(const dn_params &p, const Layout &lay, const float *vp, co
                       const float *rho, const float *taus, const float *taup, HostMaterial &m);
 int check_stability_ac(const dn_params &p, const float hc[7], const float *vp, const float *rho, std::string &err);
 void prepare_material_ac(const dn_params &p, const Layout &lay, const float *vp, const float *rho, HostMaterial &m);
-void prepare_material_sh(const dn_params &p, const Layout &lay, const float *vs, const float *rho, HostMaterial &m);
+void prepare_material_sh(const dn_params &p, const Layout &lay, const float *vs, const float *rho, const float *taus,
+                         HostMaterial &m);
+int check_stability_sh_visc(const dn_params &p, const float hc[7], const float *vs, const float *rho,
+                            const float *taus, std::string &err);
 int check_stability_vti(const dn_params &p, const float hc[7], const float *rho, const float *c11,
                         const float *c13, const float *c33, const float *c44, std::string &err);
 void prepare_material_aniso(const dn_params &p, const Layout &lay, const float *rho, const float *c11,

@@ -535,6 +535,38 @@ int check_stability_ac(const dn_params &p, const float hc[7], const float *vp, c
     return DN_OK;
 }
 
+// checkfd_visc_SH.c:21-95: the unrelaxed shear velocity
+// sqrt(mu (1 + L taus) / (rho (1 + sumu))), sumu at the source's centre
+// frequency 2 pi / TS, against the Courant limit
+int check_stability_sh_visc(const dn_params &p, const float hc[7], const float *vs, const float *rho,
+                            const float *taus, std::string &err)
+{
+    const size_t n = (size_t)p.nx * p.ny;
+    const float ws = (float)(2.0 * kPi / p.ts);
+    float cmax = 0.0f;
+    for (size_t k = 0; k < n; ++k) {
+        float sumu = 0.0f;
+        for (int l = 1; l <= p.L; ++l) {
+            const float ts = (float)(1.0 / (2.0 * kPi * p.fl[l - 1]));
+            sumu = (float)(sumu + ((ws * ws * ts * ts * taus[k]) / (1.0 + ws * ws * ts * ts)));
+        }
+        const float pu_ref = p.invmat1 == 3 ? vs[k] : rho[k] * vs[k] * vs[k];
+        const float c = (float)std::sqrt(pu_ref * (1.0 + p.L * taus[k]) / (rho[k] * (1.0 + sumu)));
+        if (cmax < c) cmax = c;
+    }
+    const float gamma = std::fabs(hc[1]) + std::fabs(hc[2]) + std::fabs(hc[3]) + std::fabs(hc[4]) +
+                        std::fabs(hc[5]) + std::fabs(hc[6]);
+    const float dtstab = (float)(p.dh / (std::sqrt(2.0) * gamma * cmax));
+    if (p.dt > dtstab) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf, " The simulation will get unstable, choose smaller DT. (DT=%e > %e)",
+                      (double)p.dt, (double)dtstab);
+        err = buf;
+        return DN_ERR_CFL;
+    }
+    return DN_OK;
+}
+
 // Acoustic material (src/AC/FD_AC.c:184-207): the one-rank periodic ghosts
 // of matcopy_acoustic_AC, av_rho (src/av_rho.c:5-28), and the modulus of
 // the pressure update, g = rho vp^2 (INVMAT1 1) or the given modulus (3)
@@ -575,17 +607,32 @@ void prepare_material_ac(const dn_params &p, const Layout &lay, const float *vp,
 // the shear modulus at the sxz and syz points, in double; for INVMAT1 1 the
 // modulus rho vs^2 in float and 0 next to a fluid cell) and inv_rho_SH
 // (src/SH/inv_rho_SH.c:12-24: 1/rho at the vz points, 0 below 1e-4).
-void prepare_material_sh(const dn_params &p, const Layout &lay, const float *vs, const float *rho, HostMaterial &m)
+void prepare_material_sh(const dn_params &p, const Layout &lay, const float *vs, const float *rho, const float *taus,
+                         HostMaterial &m)
 {
-    const int NX = p.nx, NY = p.ny;
+    const int NX = p.nx, NY = p.ny, L = p.L;
     const size_t plane = lay.plane();
     auto dev = [&](int j, int i) { return lay.origin() + (size_t)lay.off(j, i); };
-    Ring R(NX, NY), U(NX, NY);
+    Ring R(NX, NY), U(NX, NY), T(NX, NY);
     R.load(rho);
     U.load(vs);
+    if (L > 0) T.load(taus);  // matcopy_SH.c: the periodic ghosts of taus too
     m.rip.assign(plane, 0.0f);
     m.rjp.clear();
-    for (int k = 0; k < 8; ++k) m.ani[k].assign(k < 2 ? plane : 0, 0.0f);
+    // elastic: uip, ujp; viscoelastic: fipjp, f, dip[1..L], d[1..L]
+    for (int k = 0; k < 8; ++k) m.ani[k].assign(k < 2 + 2 * L ? plane : 0, 0.0f);
+    // prepare_update_s_visc_SH.c:17-42 with readmod_visc_SH.c:24-27 (eta)
+    float pts[MAXL + 1] = {0}, sumu = 0.0f;
+    const float ws = L > 0 ? (float)(2.0 * kPi * p.fl[0]) : 0.0f;
+    for (int l = 1; l <= L; ++l) {
+        pts[l] = (float)(1.0 / (2.0 * kPi * p.fl[l - 1]));
+        m.peta[l] = p.dt / pts[l];
+        sumu = (float)(sumu + ((ws * ws * pts[l] * pts[l]) / (1.0 + ws * ws * pts[l] * pts[l])));
+        m.bip[l] = (float)(1.0 / (1.0 + (m.peta[l] * 0.5)));
+        m.bjm[l] = (float)(1.0 / (1.0 + (m.peta[l] * 0.5)));
+        m.cip[l] = (float)(1.0 - (m.peta[l] * 0.5));
+        m.cjm[l] = (float)(1.0 - (m.peta[l] * 0.5));
+    }
     // the model itself for the adjoint run (sh.c:193-199 tests mu > 0) and
     // the assembly (ass_gradSH.c reads pu, prho)
     m.vs.assign(p.mode > 0 ? plane : 0, 0.0f);
@@ -602,6 +649,20 @@ void prepare_material_sh(const dn_params &p, const Layout &lay, const float *vs,
             if (!mod && (u1 == 0.0f || u3 == 0.0f)) ujp = 0.0f;
             m.ani[0][o] = uip;
             m.ani[1][o] = ujp;
+            if (L > 0) {
+                // av_tau.c:5-15 and prepare_update_s_visc_SH.c:48-58
+                const float ts = T.at(j, i);
+                const float tsij =
+                    (float)(0.25 * (T.at(j, i) + T.at(j, i + 1) + T.at(j + 1, i) + T.at(j + 1, i + 1)));
+                const float mujp = (float)(ujp / (1.0 + sumu * ts));
+                const float muip = (float)(uip / (1.0 + sumu * tsij));
+                m.ani[0][o] = (float)(muip * p.dt * (1.0 + L * tsij));
+                m.ani[1][o] = (float)(mujp * p.dt * (1.0 + L * ts));
+                for (int l = 1; l <= L; ++l) {
+                    m.ani[1 + l][o] = muip * m.peta[l] * tsij;
+                    m.ani[1 + L + l][o] = mujp * m.peta[l] * ts;
+                }
+            }
             float ri = (float)(1.0 / R.at(j, i));
             if (R.at(j, i) < 1e-4) ri = 0.0f;
             m.rip[o] = ri;

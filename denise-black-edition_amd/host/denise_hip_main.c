@@ -111,6 +111,12 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
      * (its RTM call is commented out); physics_AC.c: MODE 0, FWI (FWI_AC.c)
      * and RTM (MODE 2); physics_VTI.c / physics_TTI.c offer MODE 0 and RTM,
      * their FWI branch is commented out */
+    /* FD_SH.c:171-206: with L > 0 the SH forward run reads the viscoelastic
+     * model (readmod_visc_SH) and updates the stresses with memory
+     * variables; the viscoelastic SH FWI (FWI_SH_visc.c) is not built */
+    if (p->physics == 5 && p->L > 0 && p->mode != 0)
+        die("L", "viscoelastic SH runs forward modelling (MODE 0); FWI_SH_visc is not built");
+    if (p->physics == 5 && p->L > 3) die("L", "viscoelastic SH runs take L <= 3");
     if (p->physics == 5 && p->mode == 2) die("MODE", "the SH loop has no RTM (physics_SH.c comments it out)");
     if (p->physics == 5 && p->mode == 1 && p->grad_method != 2)
         die("GRAD_METHOD", "SH FWI is built with L-BFGS (GRAD_METHOD 2); its PCG variant (store_PCG_SH) is not");
@@ -125,9 +131,6 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
     if (p->physics != 1 && p->snap)
         die("SNAP", "wavefield snapshots of the acoustic, VTI/TTI and SH loops (snap_AC, snap_ssg_VTI, snap) are not "
                     "built");
-    /* FD_SH.c:171-203: with L > 0 the SH set-up reads the viscoelastic model
-     * and prepares its coefficients (readmod_visc_SH, prepare_update_s_visc_SH) */
-    if (p->physics == 5 && p->L > 0) die("L", "the viscoelastic SH set-up is not built (SH runs with L = 0)");
     if (p->mode < 0 || p->mode > 2) die("MODE", "MODE 0 (forward), 1 (FWI) and 2 (RTM) are built");
     if (p->boundary != 0) die("BOUNDARY", "periodic boundaries are not supported");
     if (!p->srcrec) die("SRCREC", "sources must come from SOURCE_FILE (SRCREC = 1)");
@@ -1230,7 +1233,7 @@ int main(int argc, char **argv)
     const int aniso = P.physics == 3 || P.physics == 4;
     float *vp = xmalloc(sizeof(float) * r.ncell), *vs = xmalloc(sizeof(float) * r.ncell);
     float *rho = xmalloc(sizeof(float) * r.ncell);
-    const int elastic_only = aniso || P.physics == 2 || P.physics == 5;
+    const int elastic_only = aniso || P.physics == 2 || (P.physics == 5 && P.L == 0);
     float *taus = P.L && !elastic_only ? xmalloc(sizeof(float) * r.ncell) : NULL;
     float *taup = P.L && !elastic_only ? xmalloc(sizeof(float) * r.ncell) : NULL;
     float *cani[4] = {NULL, NULL, NULL, NULL}, *theta = NULL;
@@ -1245,7 +1248,10 @@ int main(int argc, char **argv)
          * no relaxation mechanisms (ac.c) */
         io_ok(dnio_read_model_ac(&P, vp, rho), "model");
     } else if (P.physics == 5) {
-        io_ok(dnio_read_model_sh(&P, vs, rho), "model"); /* vs (or mu) and rho */
+        if (P.L > 0) /* vs (or mu), rho and taus from MFILE.qs */
+            io_ok(dnio_read_model_sh_visc(&P, vs, rho, taus), "model");
+        else
+            io_ok(dnio_read_model_sh(&P, vs, rho), "model"); /* vs (or mu) and rho */
     } else {
         io_ok(dnio_read_model(&P, vp, vs, rho, taus, taup), "model");
     }

@@ -623,6 +623,17 @@ int dnio_read_model_sh(const dnio_par *p, float *vs, float *rho)
     return 0;
 }
 
+int dnio_read_model_sh_visc(const dnio_par *p, float *vs, float *rho, float *taus)
+{
+    if (!p || !taus) return fail("null argument");
+    if (dnio_read_model_sh(p, vs, rho)) return -1;
+    /* readmod_visc_SH.c:45-84: MFILE.qs, taus = 2 / Qs */
+    const size_t n = (size_t)p->nx * p->ny;
+    if (read_model_file(p->mfile, "qs", taus, n)) return -1;
+    for (size_t k = 0; k < n; k++) taus[k] = 2.0 / taus[k];
+    return 0;
+}
+
 /* ------------------------------------------------------------------------
  * seismograms (src/outseis_glob.c:8-303, src/inseis.c:8-110,
  * include/segy.h:429-710)

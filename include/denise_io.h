@@ -159,6 +159,9 @@ int dnio_read_model_ac(const dnio_par *p, float *vp, float *rho);
  * MFILE.rho, NX*NY x-major; with WRITEMOD writes MFILE.denise.mu and
  * .denise.rho as src/SH/readmod_elastic_SH.c:21-98 does. */
 int dnio_read_model_sh(const dnio_par *p, float *vs, float *rho);
+/* readmod_visc_SH (src/SH/readmod_visc_SH.c:9-123), PHYSICS 5 with L > 0:
+ * as dnio_read_model_sh plus MFILE.qs, returned as taus = 2 / Qs. */
+int dnio_read_model_sh_visc(const dnio_par *p, float *vs, float *rho, float *taus);
 
 /* writemod + mergemod on one rank (src/writemod.c, src/mergemod.c): a
  * float32 x-major NX x NY file `name` and its per-rank copy `name.0.0`. */

@@ -1592,8 +1592,11 @@ static void update_v_sh(orc_ctx *c, int nt, int sw)
 static void update_s_sh(orc_ctx *c)
 {
     const orc_params *P = &c->P;
-    const int NX = P->nx, NY = P->ny, FW = P->fw, fdoh = c->fdoh;
-    const real dhi = (real)P->dt / (real)P->dh;
+    const int NX = P->nx, NY = P->ny, FW = P->fw, fdoh = c->fdoh, L = P->L;
+    /* update_s_visc_PML_SH.c:34-35: the viscoelastic update takes the
+     * derivatives over DH and multiplies by DT inside fipjp / f */
+    const real dhi = L ? (real)(1.0 / P->dh) : (real)P->dt / (real)P->dh;
+    const real dthalbe = (real)(P->dt / 2.0);
     const real *hc = c->hc;
     real *const *pm = c->pml;
 #pragma omp parallel for schedule(static)
@@ -1629,6 +1632,25 @@ static void update_s_sh(orc_ctx *c)
                 *ps = pm[PML_BYH][h1] * *ps + pm[PML_AYH][h1] * vzy;
                 vzy = vzy / pm[PML_KYH][h1] + *ps;
             }
+            if (L) { /* update_s_visc_PML_SH.c:345-378 (FD8 block) */
+                real sumr = 0.0, sumq = 0.0;
+                for (int l = 1; l <= L; l++) {
+                    sumr += AT(c->r[l], j, i);
+                    sumq += AT(c->q[l], j, i);
+                }
+                AT(c->sxx, j, i) += (AT(c->fipjp, j, i) * vzx) + (dthalbe * sumr);
+                AT(c->syy, j, i) += (AT(c->f, j, i) * vzy) + (dthalbe * sumq);
+                sumr = sumq = 0.0;
+                for (int l = 1; l <= L; l++) {
+                    AT(c->r[l], j, i) = c->bip[l] * (AT(c->r[l], j, i) * c->cip[l] - (AT(c->dip[l], j, i) * vzx));
+                    AT(c->q[l], j, i) = c->bjm[l] * (AT(c->q[l], j, i) * c->cjm[l] - (AT(c->d[l], j, i) * vzy));
+                    sumr += AT(c->r[l], j, i);
+                    sumq += AT(c->q[l], j, i);
+                }
+                AT(c->sxx, j, i) += (dthalbe * sumr);
+                AT(c->syy, j, i) += (dthalbe * sumq);
+                continue;
+            }
             AT(c->sxx, j, i) += AT(c->uip, j, i) * vzx;
             AT(c->syy, j, i) += AT(c->ujp, j, i) * vzy;
             if (P->grad_form == 1) {
@@ -1659,8 +1681,17 @@ static void seismo_sh(orc_ctx *c, int nt)
  * periodic ghosts), av_mu_SH and inv_rho_SH */
 orc_ctx *orc_create_sh(const orc_params *p, const float *vs, const float *rho)
 {
-    if (!p || !vs || !rho) { seterr("null argument"); return NULL; }
-    if (p->L != 0) { seterr("SH runs are elastic (L = 0)"); return NULL; }
+    if (p && p->L != 0) { seterr("viscoelastic SH models need taus (orc_create_sh_visc)"); return NULL; }
+    return orc_create_sh_visc(p, vs, rho, NULL);
+}
+
+/* ... and with L > 0 src/SH/FD_SH.c:171-206: readmod_visc_SH (taus = 2/Qs,
+ * eta = DT / (1/(2 pi FL))), matcopy_SH (periodic ghosts of rho, vs, taus),
+ * av_mu_SH, av_tau and prepare_update_s_visc_SH.c:8-86 */
+orc_ctx *orc_create_sh_visc(const orc_params *p, const float *vs, const float *rho, const float *taus)
+{
+    if (!p || !vs || !rho || (p->L > 0 && !taus)) { seterr("null argument"); return NULL; }
+    if (p->L > 0 && p->mode != 0) { seterr("viscoelastic SH runs are forward modelling (MODE 0)"); return NULL; }
     if (p->seismo > 1) { seterr("SH runs record vz (SEISMO 0 or 1)"); return NULL; }
     orc_ctx *c = ctx_new(p);
     if (!c) return NULL;
@@ -1678,6 +1709,11 @@ orc_ctx *orc_create_sh(const orc_params *p, const float *vs, const float *rho)
         }
     matcopy_periodic(c, c->rho);
     matcopy_periodic(c, c->u);
+    if (p->L > 0) {
+        for (int i = 1; i <= NX; i++)
+            for (int j = 1; j <= NY; j++) AT(c->taus, j, i) = taus[(size_t)(i - 1) * NY + (j - 1)];
+        matcopy_periodic(c, c->taus);
+    }
     for (int j = 1; j <= NY; j++)
         for (int i = 1; i <= NX; i++) {
             real u1, u2, u3;
@@ -1695,6 +1731,36 @@ orc_ctx *orc_create_sh(const orc_params *p, const float *vs, const float *rho)
             AT(c->rip, j, i) = (real)(1.0 / AT(c->rho, j, i));
             if (AT(c->rho, j, i) < 1e-4) AT(c->rip, j, i) = 0.0;
         }
+    if (p->L > 0) {
+        const int L = p->L;
+        const real DT = (real)p->dt;
+        av_tau(c);
+        real pts[9];
+        for (int l = 1; l <= L; l++) {
+            pts[l] = (real)(1.0 / (2.0 * PI_D * p->fl[l - 1]));
+            c->peta[l] = DT / pts[l];
+        }
+        real ws = (real)(2.0 * PI_D * p->fl[0]), sumu = 0.0;
+        for (int l = 1; l <= L; l++) sumu = (real)(sumu + ((ws * ws * pts[l] * pts[l]) / (1.0 + ws * ws * pts[l] * pts[l])));
+        for (int l = 1; l <= L; l++) {
+            c->bip[l] = (real)(1.0 / (1.0 + (c->peta[l] * 0.5)));
+            c->bjm[l] = (real)(1.0 / (1.0 + (c->peta[l] * 0.5)));
+            c->cip[l] = (real)(1.0 - (c->peta[l] * 0.5));
+            c->cjm[l] = (real)(1.0 - (c->peta[l] * 0.5));
+        }
+        for (int j = 1; j <= NY; j++)
+            for (int i = 1; i <= NX; i++) {
+                real ts = AT(c->taus, j, i), tsij = AT(c->tausipjp, j, i);
+                real mujp = (real)(AT(c->ujp, j, i) / (1.0 + sumu * ts));
+                real muip = (real)(AT(c->uip, j, i) / (1.0 + sumu * tsij));
+                AT(c->fipjp, j, i) = (real)(muip * DT * (1.0 + L * tsij));
+                AT(c->f, j, i) = (real)(mujp * DT * (1.0 + L * ts));
+                for (int l = 1; l <= L; l++) {
+                    AT(c->dip[l], j, i) = muip * c->peta[l] * tsij;
+                    AT(c->d[l], j, i) = mujp * c->peta[l] * ts;
+                }
+            }
+    }
     return c;
 }
 

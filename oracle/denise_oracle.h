@@ -73,6 +73,7 @@ orc_ctx *orc_create_ac(const orc_params *p, const float *vp, const float *rho);
  * 3: mu) and rho, NX*NY x-major; vz is field ORC_VX, sxz ORC_SXX, syz
  * ORC_SYY, 1/rho ORC_RIP; vz is recorded as seismogram component 1. */
 orc_ctx *orc_create_sh(const orc_params *p, const float *vs, const float *rho);
+orc_ctx *orc_create_sh_visc(const orc_params *p, const float *vs, const float *rho, const float *taus);
 void orc_destroy(orc_ctx *c);
 const char *orc_error(void);
 void orc_set_threads(int n);

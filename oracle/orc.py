@@ -45,6 +45,7 @@ def load(prec="f32"):
         "orc_create_aniso": ([C.POINTER(_OParams), C.c_int, fp, fp, fp, fp, fp, fp], vp),
         "orc_create_ac": ([C.POINTER(_OParams), fp, fp], vp),
         "orc_create_sh": ([C.POINTER(_OParams), fp, fp], vp),
+        "orc_create_sh_visc": ([C.POINTER(_OParams), fp, fp, fp], vp),
         "orc_destroy": ([vp], None),
         "orc_error": ([], C.c_char_p),
         "orc_set_threads": ([C.c_int], None),
@@ -134,8 +135,8 @@ class Oracle:
             arrs = [_f32(a) for a in (vp, rho)]
             self.h = self.lib.orc_create_ac(C.byref(cp), *[_p(a) for a in arrs])
         elif aniso == "sh":
-            arrs = [_f32(a) for a in (vs, rho)]
-            self.h = self.lib.orc_create_sh(C.byref(cp), *[_p(a) for a in arrs])
+            arrs = [_f32(a) for a in (vs, rho, taus)]
+            self.h = self.lib.orc_create_sh_visc(C.byref(cp), *[_p(a) for a in arrs])
         elif aniso is None:
             arrs = [_f32(a) for a in (vp, vs, rho, taus, taup)]
             self.h = self.lib.orc_create(C.byref(cp), *[_p(a) for a in arrs])

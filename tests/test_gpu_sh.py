@@ -473,3 +473,105 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     assert not os.path.exists(f"{jac}_p_vp.old")
     assert l2s[-1] < l2s[0]
     assert not np.array_equal(vs, vs0)
+
+
+@pytest.mark.parametrize("L,fdorder,invmat1", [(1, 8, 1), (2, 4, 3), (3, 12, 1)])
+def test_viscoelastic_forward_equals_oracle(L, fdorder, invmat1):
+    """FD_SH.c with L > 0 (readmod_visc_SH, av_tau, prepare_update_s_visc_SH,
+    update_s_visc_PML_SH): vz seismograms, final fields, CPML memory and the
+    memory variables r (sxz) and q (syz) equal the FP32 oracle bit for bit;
+    every scalar tile shape for L = 2."""
+    print(f"[sh visco] L {L} FD{fdorder} INVMAT1 {invmat1}", flush=True)
+    p, m, src, sig, rec = sh_case(fdorder=fdorder, invmat1=invmat1, nt=260)
+    p = dnhip.Params(**{**p.__dict__, "L": L, "fl": [3.0, 25.0, 60.0][:L]})
+    rng = np.random.default_rng(L)
+    taus = rng.uniform(0.02, 0.12, m[0].shape).astype(np.float32)
+    o = orc.Oracle(p, None, m[0], m[1], taus, prec="f32", aniso="sh")
+    o.set_sources(src, sig)
+    o.set_receivers(rec)
+    o.forward()
+    shapes = SCALAR if L == 2 else [None]
+    for k, ts in enumerate(shapes):
+        g = dnhip.Context(p)
+        g.set_model(None, m[0], m[1], taus)
+        g.set_sources(src, sig)
+        g.set_receivers(rec)
+        if ts:
+            g.set_tile(*SCALAR[(k + 1) % len(SCALAR)], sweep=0)
+            g.set_tile(*ts, sweep=1)
+        g.forward()
+        assert_equal_runs(g, o)
+        for which in (0, 2):       # r at sxz, q at syz
+            for l in range(1, L + 1):
+                assert np.array_equal(g.memvar(which, l), o.memvar(which, l)), (which, l)
+        assert np.abs(o.memvar(0, 1)).max() > 0
+        g.close()
+
+
+def test_viscoelastic_error_against_fp64_oracle_and_refusals():
+    p, m, src, sig, rec = sh_case(nt=400)
+    p = dnhip.Params(**{**p.__dict__, "L": 2, "fl": [3.0, 25.0]})
+    taus = np.full(m[0].shape, 0.08, np.float32)
+    g = dnhip.Context(p)
+    with pytest.raises(dnhip.DnError):
+        g.set_model(None, m[0], m[1])              # taus missing
+    g.set_model(None, m[0], m[1], taus)
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    g.forward()
+    o = orc.Oracle(p, None, m[0], m[1], taus, prec="f64", aniso="sh")
+    o.set_sources(src, sig)
+    o.set_receivers(rec)
+    o.forward()
+    err = rel_l2(g.seismogram(1), o.seismogram(1))
+    print(f"[sh visco] vz rel-L2 vs FP64 oracle {err:.2e}", flush=True)
+    assert err <= SEIS_TOL
+    g.close()
+    for bad in ({"L": 4, "fl": [3.0, 25.0, 60.0, 90.0]}, {"L": 2, "fl": [3.0, 25.0], "mode": 1}):
+        with pytest.raises(dnhip.DnError):
+            dnhip.Context(dnhip.Params(**{**p.__dict__, **bad}))
+
+
+def test_cli_viscoelastic_forward_writes_the_library_seismograms(tmp_path):
+    """PHYSICS 5 with L = 2 through the executable (readmod_visc_SH: MFILE.vs,
+    .rho and .qs, taus = 2 / Qs): its vz seismograms equal the library's."""
+    tmp = str(tmp_path)
+    nx, ny, dh, nt = 110, 70, 10.0, 280
+    _, vs, rho = synth.layered_model(nx, ny, dh, 23, sigma=3.0)
+    dt = float(np.float32(synth.stable_dt(float(vs.max()) * 1.3, dh, 8, safety=0.6)))
+    qs = np.random.default_rng(4).uniform(20.0, 60.0, (nx, ny)).astype(np.float32)
+    for d in ("model", "su"):
+        os.makedirs(os.path.join(tmp, d))
+    for ext, a in (("vs", vs), ("rho", rho), ("qs", qs)):
+        np.ascontiguousarray(a, np.float32).tofile(os.path.join(tmp, f"model/m.{ext}"))
+    with open(os.path.join(tmp, "source.dat"), "w") as f:
+        f.write("1\n400.00 0.0 30.00 0.0 15.00 1.00 0.0 1\n")
+    with open(os.path.join(tmp, "receiver.dat"), "w") as f:
+        for k in range(50):
+            f.write(f"{50.0 + 20.0 * k:.3f}\t20.000\n")
+    inpfile.write_inp(os.path.join(tmp, "m.inp"), PHYSICS=5, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt, SEISMO=1,
+                      L=2, FL="3.0 30.0", INVMAT1=1)
+    r = subprocess.run([BIN, "m.inp"], cwd=tmp, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-800:], r.stderr[-800:], flush=True)
+    assert r.returncode == 0, r.stderr
+    par = dnio.read_par(os.path.join(tmp, "m.inp"))
+    cwd = os.getcwd()
+    os.chdir(tmp)
+    try:
+        src, ts = dnio.read_sources(par.get("source_file"))
+        rec = dnio.read_receivers(par)
+    finally:
+        os.chdir(cwd)
+    p = dnhip.Params(nx=nx, ny=ny, nt=par.nt, dh=dh, dt=par.dt, fdorder=8, maxrelerror=par.maxrelerror, fw=par.fw,
+                     free_surf=par.free_surf, seismo=1, ts=ts, physics=5, invmat1=1, L=2, fl=[3.0, 30.0],
+                     damping=par.damping, fpml=par.fpml, npower=par.npower, k_max_pml=par.k_max_pml)
+    g = dnhip.Context(p)
+    g.set_model(None, vs, rho, (2.0 / qs.astype(np.float64)).astype(np.float32))
+    loc = dnio.split_sources(par, src, 0, 1)
+    g.set_sources(loc, dnio.wavelet(par, loc, 1, ts))
+    g.set_receivers(rec[:2])
+    g.forward()
+    _, data = dnio.parse_su(os.path.join(tmp, "su/out_y.su.shot1"))
+    assert np.array_equal(data, g.seismogram(1))
+    assert np.abs(data).max() > 0
+    g.close()

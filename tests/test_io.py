@@ -294,11 +294,12 @@ def test_cli_refuses_what_is_not_built(tmp_path):
     (tmp_path / "p6.inp").write_text(text)
     r = _run_cli(str(tmp_path / "p6.inp"))
     assert r.returncode == 1 and "PHYSICS" in r.stderr
-    # the viscoelastic SH set-up is not built
-    text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 5").replace("_(L) = 0", "_(L) = 2")
+    # the viscoelastic SH FWI (FWI_SH_visc.c) is not built
+    text = (open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 5").replace("_(L) = 0", "_(L) = 2")
+            .replace("_(MODE) = 0", "_(MODE) = 1"))
     (tmp_path / "shv.inp").write_text(text)
-    r = _run_cli(str(tmp_path / "shv.inp"))
-    assert r.returncode == 1 and "SH" in r.stderr
+    r = _run_cli(str(tmp_path / "shv.inp"), os.path.join(GOLD, "FWI_workflow_marmousi.inp"))
+    assert r.returncode == 1 and "FWI_SH_visc" in r.stderr
     # the reference has no VTI FWI (physics_VTI.c comments it out)
     text = open(src).read().replace("_(PHYSICS) = 1", "_(PHYSICS) = 3").replace("_(MODE) = 0", "_(MODE) = 1")
     (tmp_path / "vti.inp").write_text(text)

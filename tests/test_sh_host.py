@@ -362,3 +362,67 @@ def test_sh_adjoint_adds_the_residual_to_vz_only_for_quelltypb_1():
         assert l2 > 0
         o.adjoint()
         assert bool(np.abs(o.gradient(2)).max() > 0) == nonzero
+
+
+# --------------------------------------------------------------------------
+# viscoelastic SH (src/SH/update_s_visc_PML_SH.c, prepare_update_s_visc_SH.c)
+# --------------------------------------------------------------------------
+def _sh_visc(p, vs, rho, taus, prec="f64"):
+    return orc.Oracle(p, None, vs, rho, taus, prec=prec, aniso="sh")
+
+
+def test_sh_visco_constant_strain_rate_matches_gsls():
+    """Under uniform vz gradients held fixed (vz reset every step) each shear
+    stress integrates the GSLS relaxation ODE: with mu_R = mu / (1 + tau
+    sum_l w0^2 tau_l^2 / (1 + w0^2 tau_l^2)), w0 = 2 pi FL[1],
+    sxz(t) = mu_R a t + mu_R tau a sum_l (1 - exp(-w_l t)) / w_l and syz
+    likewise with b; trapezoidal steps leave O((w DT)^2)."""
+    import math
+    nx = ny = 40
+    dh, dt, nsteps = 10.0, 1e-4, 400
+    fl = [5.0, 40.0]
+    tau = 0.08
+    p = params(nx=nx, ny=ny, nt=nsteps, dh=dh, dt=dt, L=2, fl=fl)
+    o = _sh_visc(p, const(nx, ny, VS), const(nx, ny, RHO), const(nx, ny, tau))
+    a, b = 2.0e-3, -0.7e-3
+    j, i = _coords(o)
+    vz = a * (i - 1) * dh + b * (j - 1) * dh
+    for nt_ in range(1, nsteps + 1):
+        o.set_field(orc.ORC_VX, vz)
+        o.step(nt_)
+    t = nsteps * float(np.float32(dt))
+    w = [2 * math.pi * f for f in fl]
+    S = sum((w[0] / wl) ** 2 / (1 + (w[0] / wl) ** 2) for wl in w)
+    tau32 = float(np.float32(tau))
+    mu = RHO * float(np.float32(VS)) ** 2 / (1 + tau32 * S)
+    relax = sum((1 - math.exp(-wl * t)) / wl for wl in w)
+    nd = o.nd
+    jc, ic = ny // 2 + nd - 1, nx // 2 + nd - 1
+    for fid, rate in ((orc.ORC_SXX, a), (orc.ORC_SYY, b)):
+        got = o.field(fid)[jc, ic]
+        want = mu * rate * t + mu * tau32 * rate * relax
+        assert got == pytest.approx(want, rel=2e-4), (fid, got, want)
+        assert abs(mu * tau32 * rate * relax) > 50 * 2e-4 * abs(want)
+
+
+def test_sh_visco_with_zero_tau_is_the_elastic_update():
+    """taus = 0: the memory variables stay 0 and fipjp = uip DT, f = ujp DT,
+    so the run equals the elastic SH run (FP64, to rounding)."""
+    nx, ny = 60, 50
+    _, vs, rho = synth.layered_model(nx, ny, 10.0, 3, sigma=3.0)
+    base = dict(nx=nx, ny=ny, nt=300, dt=1e-3, fw=10, seismo=1)
+    pv = params(L=2, fl=[4.0, 30.0], **base)
+    pe = params(**base)
+    src = np.array([[30], [10], [0], [0], [10.0], [1.0], [0], [1]], np.float32)
+    sig = synth.ricker(pv.nt, pv.dt, 10.0, amp=1e3)[None, :]
+    rec = np.array([[10, 30, 50], [5, 5, 5]], np.int32)
+    runs = []
+    for o in (_sh_visc(pv, vs, rho, np.zeros((nx, ny), np.float32)), _sh(pe, vs, rho)):
+        o.set_sources(src, sig)
+        o.set_receivers(rec)
+        o.forward()
+        runs.append(o)
+    assert rel_l2(runs[0].seismogram(1), runs[1].seismogram(1)) < 1e-12
+    assert np.abs(runs[1].seismogram(1)).max() > 0
+    for fid in (orc.ORC_VX, orc.ORC_SXX, orc.ORC_SYY):
+        assert rel_l2(runs[0].field(fid), runs[1].field(fid)) < 1e-12
