@@ -73,11 +73,14 @@ typedef struct dn_params {
     float fl[8];             /* FL[1..L]                                       */
     int physics;             /* PHYSICS: 1 (or 0) isotropic P/SV, 2 acoustic,
                                 3 VTI, 4 TTI, 5 SH (src/denise.c:116-141);
-                                2-5 with L = 0; 5 forward runs (MODE 0,
-                                SEISMO 0 or 1, vz recorded as component 1);
-                                2, 3 and 4 also adjoint runs (mode 1, the
-                                RTM passes of RTM_AC.c, RTM_VTI.c,
-                                RTM_TTI.c; TTI with GRAD_FORM 1)            */
+                                2-4 with L = 0, 5 with L = 0..3 (L > 0:
+                                forward runs, FD_SH.c:171-206); 5 records
+                                vz as component 1 (SEISMO 0 or 1);
+                                2, 3, 4 and elastic 5 also adjoint runs
+                                (mode 1: the RTM passes of RTM_AC.c,
+                                RTM_VTI.c, RTM_TTI.c, TTI with GRAD_FORM 1;
+                                the gradients of FWI_AC.c and FWI_SH.c; SH
+                                runs the vz residual, calc_res_SH.c)        */
 } dn_params;
 
 typedef struct dn_ctx dn_ctx;
@@ -111,9 +114,10 @@ int dn_ctx_destroy(dn_ctx *ctx);
  * (INVMAT1=3: the modulus) and rho only, vs/taus/taup ignored (may be
  * NULL): readmod_AC, matcopy_acoustic_AC, av_rho and checkfd_acoustic
  * (src/AC/FD_AC.c:184-207).  SH contexts (params.physics 5) take vs
- * (INVMAT1=3: mu) and rho only, vp/taus/taup ignored: readmod_elastic_SH,
- * matcopy_elastic_SH, av_mu_SH, inv_rho_SH and checkfd_elast_SH
- * (src/SH/FD_SH.c:171-199). */
+ * (INVMAT1=3: mu) and rho, and for L > 0 taus (2/Qs), vp/taup ignored:
+ * readmod_elastic_SH / readmod_visc_SH, matcopy_SH, av_mu_SH, inv_rho_SH,
+ * av_tau, prepare_update_s_visc_SH and checkfd_elast_SH / checkfd_visc_SH
+ * (src/SH/FD_SH.c:171-206). */
 int dn_set_model(dn_ctx *ctx, const float *vp, const float *vs, const float *rho,
                  const float *taus, const float *taup);
 
@@ -194,14 +198,18 @@ int dn_run_adjoint(dn_ctx *ctx, int nsteps);
 
 /* ass_gradPSV (src/PSV/ass_gradPSV.c:10-155), for acoustic contexts
  * ass_gradAC (src/AC/ass_gradAC.c:12-93: vp or lambda and rho, no shear
- * term), iteration 1 with INV_*_ITER=0.  VTI/TTI contexts refuse it
+ * term), for SH contexts ass_gradSH (src/SH/ass_gradSH.c:10-98: vs or mu
+ * and rho, vp gradient 0) of this shot, iteration 1 with INV_*_ITER=0
+ * (the SH inversion assembles the shot stack instead: dnfwi_assemble_sh in
+ * denise_fwi.h).  VTI/TTI contexts refuse it
  * (DN_ERR_STATE): their images are the raw correlation sums
  * (grad_obj_VTI.c:196-197 does not assemble). */
 int dn_assemble_gradient(dn_ctx *ctx);
 
 /* which: 0..2 raw correlation sums (waveconv_shot, waveconv_u_shot,
  * waveconv_rho_shot; VTI/TTI: the P image, 0, the density image,
- * src/VTI/VTI.c:297-368), 3..5 per-shot vp / vs / rho gradients.  NX*NY
+ * src/VTI/VTI.c:297-368; SH: 0, the shear sum, the density sum,
+ * src/SH/sh.c:186-223), 3..5 per-shot vp / vs / rho gradients.  NX*NY
  * x-major. */
 int dn_get_gradient(dn_ctx *ctx, int which, float *out);
 
