@@ -442,7 +442,7 @@ def run_config1(threads, shots=5):
     return out
 
 
-def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20, tiles=None):
+def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20, tiles=None, L=0):
     """One acoustic (physics 2), VTI (3), TTI (4) or SH (5) shot on the
     4000 x 1200 grid of the headline workload (DH=5 m, FD8, DT=0.4 ms, CPML
     20 + free surface / top switch, Ricker 15 Hz, 800 receivers); the
@@ -452,21 +452,28 @@ def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20, tiles=None):
     acoustic velocity 28 B (p, rip, rjp, vx, vy read; vx, vy written),
     pressure 20 B (vx, vy, p, modulus read; p written); SH velocity 20 B
     (sxz, syz, 1/rho, vz read; vz written), stress 28 B (vz, sxz, syz, uip,
-    ujp read; sxz, syz written); elastic velocity 36 B, stress VTI 48 B (5
-    wavefields + 4 stiffness planes read, 3 stresses written), TTI 64 B (8
+    ujp read; sxz, syz written), viscoelastic SH stress (L > 0) 4 (7 + 6 L) B
+    (vz, sxz, syz, fipjp, f, dip[L], d[L], r[L], q[L] read; sxz, syz, r[L],
+    q[L] written; Qs 20-60 seeded); elastic velocity 36 B, stress VTI 48 B
+    (5 wavefields + 4 stiffness planes read, 3 stresses written), TTI 64 B (8
     stiffness planes)."""
     import dnhip
     import synth
     nx, ny, dh, dt, fc = 4000, 1200, 5.0, 4.0e-4, 15.0
     p = dnhip.Params(nx=nx, ny=ny, nt=nt, dh=dh, dt=dt, fdorder=8, maxrelerror=1, fw=20, free_surf=1, seismo=1,
-                     ts=float(np.float32(1.0 / fc)), damping=1500.0, fpml=fc, physics=physics)
+                     ts=float(np.float32(1.0 / fc)), damping=1500.0, fpml=fc, physics=physics, L=L,
+                     fl=[3.0, 30.0, 60.0][:L])
     ctx = dnhip.Context(p)
     if physics == 2:
         vp, _, rho = synth.layered_model(nx, ny, dh, 11)
         ctx.set_model(vp, None, rho)
     elif physics == 5:
         _, vs, rho = synth.layered_model(nx, ny, dh, 11)
-        ctx.set_model(None, vs, rho)
+        taus = None
+        if L:
+            qs = np.random.default_rng(11).uniform(20.0, 60.0, (nx, ny))
+            taus = (2.0 / qs).astype(np.float32)
+        ctx.set_model(None, vs, rho, taus)
     else:
         rho, c11, c13, c33, c44, theta = synth.anisotropic_model(nx, ny, dh, 11, tilt=30.0 if physics == 4 else None)
         ctx.set_model_aniso(rho, c11, c13, c33, c44, theta)
@@ -485,9 +492,9 @@ def run_aniso(physics, nt=6000, tune_steps=40, sample_every=20, tiles=None):
     tm = ctx.timing()
     assert np.isfinite(ctx.seismogram(0)).all()
     cells = nx * ny
-    sb = {2: 20, 3: 48, 4: 64, 5: 28}[physics]
+    sb = {2: 20, 3: 48, 4: 64, 5: 28 if L == 0 else 4 * (7 + 6 * L)}[physics]
     vb = {2: 28, 5: 20}.get(physics, 36)
-    name = {2: "acoustic", 3: "VTI elastic", 4: "TTI elastic", 5: "SH elastic"}[physics]
+    name = {2: "acoustic", 3: "VTI elastic", 4: "TTI elastic", 5: "SH elastic" if L == 0 else f"SH visco L={L}"}[physics]
     out = {"workload": f"{name} 4000x1200, DH=5 m, FD8, NT={nt}",
            "gcells_per_s": cells * nt / wall / 1e9, "ms_per_shot": wall * 1e3,
            "tile": {"velocity": list(ctx.tile(0)), "stress": list(ctx.tile(1))},
@@ -668,6 +675,7 @@ def main():
     ap.add_argument("--no-aniso", action="store_true", help="skip the acoustic, VTI, TTI and SH shots")
     ap.add_argument("--aniso-only", type=int, default=0, metavar="PHYSICS",
                     help="profiling: only the acoustic (2), VTI (3), TTI (4) or SH (5) shot")
+    ap.add_argument("--sh-L", type=int, default=0, help="with --aniso-only 5: the viscoelastic SH shot with L")
     ap.add_argument("--no-executable", action="store_true",
                     help="skip the drop-in executable on the shipped Marmousi example (100 shots, MODE 0)")
     ap.add_argument("--fwi-iters", type=int, default=2,
@@ -700,7 +708,8 @@ def main():
         print(json.dumps(out))
         return
     if args.aniso_only:
-        print(json.dumps({"aniso": run_aniso(args.aniso_only, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles)}))
+        print(json.dumps({"aniso": run_aniso(args.aniso_only, nt=args.nt, tune_steps=args.tune_steps, tiles=tiles,
+                                             L=args.sh_L if args.aniso_only == 5 else 0)}))
         return
     if args.executable_only:
         print(json.dumps({"executable": run_executable_marmousi()}))
@@ -737,7 +746,8 @@ def main():
         aniso = {"acoustic": run_aniso(2, nt=args.nt, tune_steps=args.tune_steps),
                  "vti": run_aniso(3, nt=args.nt, tune_steps=args.tune_steps),
                  "tti": run_aniso(4, nt=args.nt, tune_steps=args.tune_steps),
-                 "sh": run_aniso(5, nt=args.nt, tune_steps=args.tune_steps)}
+                 "sh": run_aniso(5, nt=args.nt, tune_steps=args.tune_steps),
+                 "sh_visco": run_aniso(5, nt=args.nt, tune_steps=args.tune_steps, L=2)}
         print(f"# aniso: {aniso}", file=sys.stderr, flush=True)
     executable = None
     if not args.no_executable:
