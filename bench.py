@@ -238,6 +238,51 @@ def gradient_workload(nt=4000):
     return common, bg, true, src, sig, rec
 
 
+def run_gradient_other(physics, nt=4000, jobs=2, tune_steps=40):
+    """The gradient job of the acoustic (PHYSICS 2, FWI_AC.c) or SH (5,
+    FWI_SH.c) inversion on the elastic job's grid and shot (2000 x 600,
+    NT 4000, snapshots every 2nd step: 3 planes per stored step), vp/rho or
+    vs/rho of the same background and true models: forward with the store,
+    residuals (vx/vy, or vz), adjoint with the correlations, assembly;
+    whole-job Gcell-updates/s (2 NX NY NT per job) over `jobs` timed jobs."""
+    import dnhip
+    common, bg, true, src, sig, rec = gradient_workload(nt)
+    pick = (lambda m: (m[0], None, m[2])) if physics == 2 else (lambda m: (None, m[1], m[2]))
+    fwd = dnhip.Context(dnhip.Params(mode=0, physics=physics, **common))
+    fwd.set_model(*pick(true))
+    fwd.set_sources(src, sig)
+    fwd.set_receivers(rec)
+    fwd.forward()
+    obs = (fwd.seismogram(0), fwd.seismogram(1))
+    fwd.close()
+    g = dnhip.Context(dnhip.Params(mode=1, physics=physics, quelltypb=1, **common))
+    g.set_model(*pick(bg))
+    g.set_sources(src, sig)
+    g.set_receivers(rec)
+    if tune_steps > 0:
+        g.autotune(tune_steps)
+
+    def job():
+        g.forward()
+        l2 = g.residuals(obs[0] if physics == 2 else None, obs[1], None, 1)
+        g.adjoint()
+        g.assemble()
+        return l2
+
+    job()
+    t0 = time.perf_counter()
+    for _ in range(jobs):
+        l2 = job()
+    wall = (time.perf_counter() - t0) / jobs
+    k = 4 if physics == 5 else 3
+    assert np.isfinite(l2) and np.abs(g.gradient(k)).max() > 0
+    out = {"workload": f"{'acoustic' if physics == 2 else 'SH'} gradient job 2000x600, FD8, NT={nt}, DTINV 2",
+           "s_per_job": wall, "gcells_per_s": 2.0 * common["nx"] * common["ny"] * nt / wall / 1e9,
+           "tile": {"velocity": list(g.tile(0)), "stress": list(g.tile(1))}, "misfit": l2}
+    g.close()
+    return out
+
+
 def run_fwi_iterations(g, model, obs, iters, nlbfgs=20):
     """FWI iterations of the gradient job's shot (the only, and test, shot)
     through the product path: gradient on the device with energy
@@ -747,7 +792,9 @@ def main():
                  "vti": run_aniso(3, nt=args.nt, tune_steps=args.tune_steps),
                  "tti": run_aniso(4, nt=args.nt, tune_steps=args.tune_steps),
                  "sh": run_aniso(5, nt=args.nt, tune_steps=args.tune_steps),
-                 "sh_visco": run_aniso(5, nt=args.nt, tune_steps=args.tune_steps, L=2)}
+                 "sh_visco": run_aniso(5, nt=args.nt, tune_steps=args.tune_steps, L=2),
+                 "acoustic_gradient": run_gradient_other(2, tune_steps=args.tune_steps),
+                 "sh_gradient": run_gradient_other(5, tune_steps=args.tune_steps)}
         print(f"# aniso: {aniso}", file=sys.stderr, flush=True)
     executable = None
     if not args.no_executable:
