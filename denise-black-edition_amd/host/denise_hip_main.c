@@ -118,8 +118,6 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         die("L", "viscoelastic SH runs forward modelling (MODE 0); FWI_SH_visc is not built");
     if (p->physics == 5 && p->L > 3) die("L", "viscoelastic SH runs take L <= 3");
     if (p->physics == 5 && p->mode == 2) die("MODE", "the SH loop has no RTM (physics_SH.c comments it out)");
-    if (p->physics == 5 && p->mode == 1 && p->grad_method != 2)
-        die("GRAD_METHOD", "SH FWI is built with L-BFGS (GRAD_METHOD 2); its PCG variant (store_PCG_SH) is not");
     if (p->physics == 5 && st && st->inv_stf)
         die("INV_STF", "the SH source-wavelet inversion (stf_sh.c) is not built");
     if ((p->physics == 3 || p->physics == 4) && p->mode == 1)
@@ -1074,8 +1072,21 @@ static void fwi(run *r, const char *workflow, float *vp_in, float *vs_in, float 
                 /* extract_LBFGS_PSV.c:80-249 / extract_LBFGS_AC.c:60-178 /
                  * extract_LBFGS_SH.c:50-178: JACOBIAN_p*.old, _c*.old */
                 io_ok(dnfwi_lbfgs_save(lb, P->jacobian, gvp, gvs, g[2]), "L-BFGS files");
-            } else
+            } else {
+                if (sh) {
+                    /* store_PCG_SH.c:33-64 writes the descent directions -g of
+                     * vs and rho as JACOBIAN_p_u.old and _p_rho.old */
+                    float *neg = xmalloc(sizeof(float) * r->ncell);
+                    const char *nm[2] = {"p_u", "p_rho"};
+                    for (int k = 1; k < 3; k++) {
+                        for (size_t q = 0; q < r->ncell; q++) neg[q] = -g[k][q];
+                        snprintf(r->name, sizeof r->name, "%s_%s.old", P->jacobian, nm[k - 1]);
+                        io_ok(dnio_write_model(r->name, neg, nx, ny), "PCG files");
+                    }
+                    free(neg);
+                }
                 io_ok(dnfwi_pcg_direction(pc, P->pcg_beta, gvp, gvs, g[2], NULL), "PCG");
+            }
             if (iter == 1) min_iter_help = P->min_iter; /* FWI_PSV.c:687-690 */
             obj_env oe = {r, taus, taup};
             io_ok(dnfwi_step_length(&up, iter, P->eps_scale, P->scalefac, P->stepmax, L2sum, nx, ny, vp, vs, rho, gvp,

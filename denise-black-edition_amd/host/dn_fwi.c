@@ -824,7 +824,7 @@ void dnfwi_pcg_reset(dnfwi_pcg *pc)
 
 int dnfwi_pcg_direction(dnfwi_pcg *pc, int pcg_beta, float *g_vp, float *g_vs, float *g_rho, float *beta_out)
 {
-    if (!pc || !g_vp || !g_rho || pcg_beta < 1 || pcg_beta > 4)
+    if (!pc || (!g_vp && !g_vs) || !g_rho || pcg_beta < 1 || pcg_beta > 4)
         return dnio_fail_msg("dnfwi_pcg_direction: bad argument (PCG_BETA 1..4)");
     const size_t nc = pc->ncell;
     float *g[3] = {g_vp, g_vs, g_rho};

@@ -97,7 +97,9 @@ void dnfwi_pcg_reset(dnfwi_pcg *pc);
  * subtracts.  beta_out (may be NULL) gets the three betas.  g_vs may be
  * NULL (the acoustic problem, src/AC/FWI_AC.c:683-706 with store_PCG_AC and
  * extract_PCG_AC): the vs class then stays 0 and its beta is 0; the vp and
- * rho classes, each with its own beta, are those of the two-class run. */
+ * rho classes, each with its own beta, are those of the two-class run.
+ * Likewise g_vp may be NULL (the SH problem, src/SH/FWI_SH.c:667-677 with
+ * store_PCG_SH and extract_PCG_SH: classes vs and rho). */
 int dnfwi_pcg_direction(dnfwi_pcg *pc, int pcg_beta, float *g_vp, float *g_vs, float *g_rho, float *beta_out);
 
 /* Source-time-function inversion of one shot (INV_STF 1: src/PSV/stf_psv.c:13-66

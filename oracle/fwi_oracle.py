@@ -620,10 +620,13 @@ class PCG:
     def direction(self, pcg_beta, g_vp, g_vs, g_rho):
         """g_vs None: the acoustic store_PCG_AC / extract_PCG_AC (vp and rho
         classes, the vs class left 0 and None on return)."""
-        shape = g_vp.shape
         ac = g_vs is None
         if ac:
             return self._direction_ac(pcg_beta, g_vp, g_rho)
+        if g_vp is None:   # SH (store_PCG_SH: vs and rho): an all-zero vp class keeps dir 0, beta 0
+            d = self.direction(pcg_beta, np.zeros_like(g_vs), g_vs, g_rho)
+            return (None, d[1], d[2], d[3])
+        shape = g_vp.shape
         new = np.concatenate([-g_vp.ravel(), -g_vs.ravel(), -g_rho.ravel()]).astype(F32)
         betas = []
         for k in range(3):

@@ -947,3 +947,23 @@ def test_energy_precond_sh():
     assert gv is None and np.array_equal(gs, gs3) and np.array_equal(gr, gr3)
     oW, ov, os_, orr = fo.energy_precond(3, 10.0, Ws, None, *c, None, g[1], g[2])
     assert ov is None and np.array_equal(gs, os_) and np.array_equal(gr, orr)
+
+
+def test_pcg_sh_without_vp():
+    """store_PCG_SH / extract_PCG_SH (classes vs and rho): without g_vp the vs
+    and rho directions and betas equal the three-class run with a zero vp
+    class, in the product and in the oracle."""
+    rng = np.random.default_rng(41)
+    for pcg_beta in (1, 2, 3, 4):
+        p2, p3, po = dnfwi.PCG(NX, NY), dnfwi.PCG(NX, NY), fo.PCG(NX, NY)
+        for it in range(4):
+            gg = [rnd(rng) for _ in range(3)]
+            a = p2.direction(pcg_beta, None, gg[1], gg[2])
+            b = p3.direction(pcg_beta, np.zeros((NX, NY), F32), gg[1], gg[2])
+            o = po.direction(pcg_beta, None, gg[1], gg[2])
+            assert a[0] is None and o[0] is None
+            for k in (1, 2):
+                assert np.array_equal(a[k], b[k]) and np.array_equal(a[k], o[k]), (pcg_beta, it, k)
+            assert a[3][0] == 0.0 and a[3][1] == b[3][1] == o[3][1] and a[3][2] == b[3][2] == o[3][2]
+        p2.close()
+        p3.close()

@@ -325,7 +325,7 @@ def test_gradient_error_against_fp64_oracle():
     g.close()
 
 
-@pytest.mark.parametrize("variant", ["plain", "precond", "spatfilter3"])
+@pytest.mark.parametrize("variant", ["plain", "precond", "spatfilter3", "pcg"])
 def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     """MODE 1 with PHYSICS 5 (FWI_SH.c, grad_obj_sh.c, ass_gradSH.c,
     step_length_est_sh.c, calc_mat_change_test_SH.c): three iterations of
@@ -338,12 +338,15 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     L-BFGS.  precond: EPRECOND 3 (energy of vz, eprecond_SH.c) divided out
     per shot, the horizontal taper (precond_SH) and GRAD_FILTER 1 (a 3 x 3
     median of the gradients) after it.  spatfilter3: the Gaussian gradient
-    filter with the start model's mean vs."""
+    filter with the start model's mean vs.  pcg: conjugate gradients (PCG_BETA
+    2) over the vs and rho classes, whose store_PCG_SH writes the descent
+    directions -g as JACOBIAN_p_u.old and _p_rho.old."""
     import fwi_oracle as fo
     from test_gpu_cli import _files, _run
     itermax = 3
     precond = variant == "precond"
     spat3 = variant == "spatfilter3"
+    pcg = variant == "pcg"
     tmp = str(tmp_path)
     os.makedirs(os.path.join(tmp, "taper"), exist_ok=True)
     nx, ny, dh, nt = 96, 64, 10.0, 400
@@ -362,7 +365,8 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     inpfile.write_inp(os.path.join(tmp, "fwi.inp"), PHYSICS=5, MODE=1, QUELLTYPB=1, SEISMO=1, NX=nx, NY=ny, DH=dh,
                       DT=dt, TIME=nt * dt, MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1", ITERMAX=itermax,
                       INV_MOD_OUT=1, MISFIT_LOG_FILE="misfit.dat", SWS_TAPER_GRAD_HOR=int(precond),
-                      GRADT="3, 12, 60, 62", EXP_TAPER_GRAD_HOR=2.0, GRAD_METHOD=2, GRAD_FILTER=int(precond),
+                      GRADT="3, 12, 60, 62", EXP_TAPER_GRAD_HOR=2.0, GRAD_METHOD=1 if pcg else 2, PCG_BETA=2,
+                      GRAD_FILTER=int(precond),
                       FILT_SIZE_GRAD=3)
     fce = 10.0 if spat3 else 0.0
     inpfile.write_workflow(os.path.join(tmp, "wf.inp"), fc_end=fce, eprecond=3 if precond else 0,
@@ -411,7 +415,7 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
         return l2
 
     vs, rho = vs0.copy(), rho0.copy()
-    lb = fo.LBFGS(nx, ny, par.nlbfgs, sh=True)
+    lb = fo.PCG(nx, ny) if pcg else fo.LBFGS(nx, ny, par.nlbfgs, sh=True)
     c = (np.float32(0.0), fo.model_mean(vs), fo.model_mean(rho))
     carry = ([0.0, 0.0, 0.0], [0.0, 0.0, 0.0])
     lines, l2s = [], []
@@ -438,8 +442,12 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
                                                                                                float(c[1]), fce)
             assert wu >= 3
         l2s.append(l2sum)
-        d = lb.direction(it, *c, None, gu, gr, None, vs, rho)
-        saved = (lb.mold.copy(), lb.gold.copy(), [None if a is None else a.copy() for a in d])
+        if pcg:
+            descent = ((-gu).astype(np.float32), (-gr).astype(np.float32))
+            d = lb.direction(2, None, gu, gr)[:3]
+        else:
+            d = lb.direction(it, *c, None, gu, gr, None, vs, rho)
+            saved = (lb.mold.copy(), lb.gold.copy(), [None if a is None else a.copy() for a in d])
         ls = fo.step_length(upd, it, par.eps_scale, par.scalefac, par.stepmax, l2sum, None, vs, rho, *d, misfit,
                             carry=carry)
         carry = (ls["epst"], ls["L2t"])
@@ -464,12 +472,18 @@ def test_cli_fwi_iterations_equal_python_iterations(tmp_path, variant):
     assert not os.path.exists(os.path.join(tmp, "model/inv_vp.bin"))
     assert open(os.path.join(tmp, "misfit.dat")).read().splitlines() == lines
     jac = os.path.join(tmp, "jac/j")
-    mold, gold, dd = saved
-    nc = nx * ny
-    for names, slot, dk in ((("p_vs", "p_u", "c_u"), 0, dd[1]), (("p_mrho", "p_rho", "c_rho"), 1, dd[2])):
-        want = (mold[slot * nc:(slot + 1) * nc], gold[slot * nc:(slot + 1) * nc], dk.ravel())
-        for n, w in zip(names, want):
-            assert np.array_equal(np.fromfile(f"{jac}_{n}.old", np.float32), w), n
+    if pcg:
+        for n, w in (("p_u", descent[0]), ("p_rho", descent[1])):
+            for sfx in (".old", ".old.0.0"):
+                assert np.array_equal(np.fromfile(f"{jac}_{n}{sfx}", np.float32), w.ravel()), n + sfx
+        assert not os.path.exists(f"{jac}_c_u.old")
+    else:
+        mold, gold, dd = saved
+        nc = nx * ny
+        for names, slot, dk in ((("p_vs", "p_u", "c_u"), 0, dd[1]), (("p_mrho", "p_rho", "c_rho"), 1, dd[2])):
+            want = (mold[slot * nc:(slot + 1) * nc], gold[slot * nc:(slot + 1) * nc], dk.ravel())
+            for n, w in zip(names, want):
+                assert np.array_equal(np.fromfile(f"{jac}_{n}.old", np.float32), w), n
     assert not os.path.exists(f"{jac}_p_vp.old")
     assert l2s[-1] < l2s[0]
     assert not np.array_equal(vs, vs0)
