@@ -118,8 +118,6 @@ static void check_supported(const dnio_par *p, const dnio_stage *st)
         die("L", "viscoelastic SH runs forward modelling (MODE 0); FWI_SH_visc is not built");
     if (p->physics == 5 && p->L > 3) die("L", "viscoelastic SH runs take L <= 3");
     if (p->physics == 5 && p->mode == 2) die("MODE", "the SH loop has no RTM (physics_SH.c comments it out)");
-    if (p->physics == 5 && st && st->inv_stf)
-        die("INV_STF", "the SH source-wavelet inversion (stf_sh.c) is not built");
     if ((p->physics == 3 || p->physics == 4) && p->mode == 1)
         die("MODE", "the VTI/TTI loops run forward modelling (MODE 0) and RTM (MODE 2); the reference has no "
                     "VTI/TTI FWI");
@@ -198,6 +196,7 @@ typedef struct run {
     float *picks, *offs;  /* this shot's picked times (TIMEWIN 1, 4) and offsets (ntr) */
     int lnorm;            /* LNORM of the current stage */
     int inv_stf, normalize; /* INV_STF, NORMALIZE, OFFSETC_STF, EPS_STF of the current stage */
+    int stf_stage_filter;   /* the SH STF run filters its spike with the stage filter too */
     float offsetc_stf, eps_stf;
     int stf_now;          /* invert the source wavelets in this gradient pass (iter 1) */
     float hc[7];
@@ -237,7 +236,7 @@ static void shot_setup(run *r, int ishot, float *first_signal)
         io_ok(dnio_tdfilt(sig, nloc, P->nt, P->dt, P->order_spike, P->fc_spike_2, P->fc_spike_1), "spike filter");
     /* grad_obj_psv.c:151, obj_psv.c:116: no TIME_FILT of the source with
      * INV_STF (the inverted wavelet was fitted to the filtered data) */
-    if (r->time_filt && !r->inv_stf)
+    if (r->time_filt && (!r->inv_stf || r->stf_stage_filter))
         io_ok(dnio_tdfilt(sig, nloc, P->nt, P->dt, r->order, r->fc, r->fc_start), "TIME_FILT");
     dn_ok(dn_set_sources(r->ctx, nloc, sp, sig), "sources");
     if (first_signal) {
@@ -443,7 +442,8 @@ static double shot_residuals(run *r, int ishot, int test)
 
 /* INV_STF 1 (grad_obj_psv.c:127-129 -> src/PSV/stf_psv.c:13-66): a forward
  * run of this shot with the band-limited spike (QUELLART 6, its
- * ORDER_SPIKE filter only), the observed vy data with the stage's filter,
+ * ORDER_SPIKE filter only; SH, src/SH/stf_sh.c:30-50: then the stage's
+ * filter too), the observed vy (SH: vz) data with the stage's filter,
  * and the Wiener deconvolution of src/stf.c; the wavelet is written as
  * SIGNAL_FILE_shot_<ishot>.dat (%e per line), and from here on every
  * wavelet of the run is read from those files (QUELLART 3, as the
@@ -455,9 +455,14 @@ static void stf_shot(run *r, int ishot)
     float *spike = xmalloc(sizeof(float) * P->nt), *stf = xmalloc(sizeof(float) * P->nt);
     P->quellart = 6;
     const int tf = r->time_filt;
-    r->time_filt = 0;
+    /* stf_sh.c:30-50 filters the spike with the stage's filter as well */
+    if (P->physics == 5)
+        r->stf_stage_filter = 1;
+    else
+        r->time_filt = 0;
     shot_setup(r, ishot, spike);
     r->time_filt = tf;
+    r->stf_stage_filter = 0;
     run_forward(r->ctx, "STF forward run");
     float *syn = xmalloc(sizeof(float) * n), *obs = xmalloc(sizeof(float) * n);
     dn_ok(dn_get_seismogram(r->ctx, 1, syn), "seismograms");

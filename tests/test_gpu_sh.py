@@ -589,3 +589,68 @@ def test_cli_viscoelastic_forward_writes_the_library_seismograms(tmp_path):
     assert np.array_equal(data, g.seismogram(1))
     assert np.abs(data).max() > 0
     g.close()
+
+
+def test_cli_fwi_source_wavelet_inversion(tmp_path):
+    """INV_STF 1 in SH FWI (grad_obj_sh.c:118-120 -> stf_sh.c) with a
+    band-pass stage (TIME_FILT, FC_START 4 Hz, FC_END 12 Hz, ORDER 4): the
+    spike run's signal is the band-limited spike (ORDER_SPIKE 5, FC_SPIKE_2
+    15 Hz) filtered again with the stage's filter (stf_sh.c:30-50, unlike
+    stf_psv.c), the observed vz data get the stage's filter, and the wavelet
+    file SIGNAL_FILE_shot_<N>.dat equals the Wiener deconvolution of the
+    restatement (oracle/stf_oracle.py) within its printed precision."""
+    import stf_oracle as so
+    import tdfilt_oracle as tf
+    from test_gpu_cli import _files, _run
+    tmp = str(tmp_path)
+    os.makedirs(os.path.join(tmp, "wavelet"), exist_ok=True)
+    nx, ny, dh, nt = 96, 64, 10.0, 400
+    _, vs0, rho0 = synth.layered_model(nx, ny, dh, seed=6, sigma=3.0)
+    dt = float(np.float32(synth.stable_dt(float(vs0.max()) * 1.2, dh, 8, safety=0.6)))
+    vt = vs0.copy()
+    vt[40:56, 10:20] *= 1.08
+    sources = [(300.0, 30.0, 15.0, 1.0, 1), (650.0, 30.0, 15.0, 1.0, 1)]
+    receivers = [(40.0 + 20.0 * k, 20.0) for k in range(40)]
+    _files(tmp, nx, ny, dh, (vt, vt, rho0), sources, receivers, mname="model/true")
+    inpfile.write_inp(os.path.join(tmp, "true.inp"), PHYSICS=5, NX=nx, NY=ny, DH=dh, DT=dt, TIME=nt * dt,
+                      MFILE="model/true", SEIS_FILE_VX="obs/d_x.su", SEIS_FILE_VY="obs/d_y.su", SEISMO=1)
+    _run(tmp, "true.inp")
+    _files(tmp, nx, ny, dh, (vs0, vs0, rho0), sources, receivers, mname="model/start")
+    inpfile.write_inp(os.path.join(tmp, "fwi.inp"), PHYSICS=5, MODE=1, QUELLTYPB=1, SEISMO=1, NX=nx, NY=ny, DH=dh,
+                      DT=dt, TIME=nt * dt, MFILE="model/start", DTINV=2, TESTSHOT="1, 2, 1", ITERMAX=1,
+                      MISFIT_LOG_FILE="misfit.dat", GRAD_METHOD=2)
+    fcs, fce, order = 4.0, 12.0, 4
+    inpfile.write_workflow(os.path.join(tmp, "wf.inp"), time_filt=1, fc_start=fcs, fc_end=fce, order=order,
+                           inv_stf=1, normalize=2, eps_stf=0.1)
+    _run(tmp, "fwi.inp", "wf.inp")
+    par = dnio.read_par(os.path.join(tmp, "fwi.inp"))
+    cwd = os.getcwd()
+    os.chdir(tmp)
+    try:
+        src, ts = dnio.read_sources(par.get("source_file"))
+        rec = dnio.read_receivers(par)
+    finally:
+        os.chdir(cwd)
+    p = dnhip.Params(nx=nx, ny=ny, nt=par.nt, dh=dh, dt=par.dt, fdorder=par.fdorder, maxrelerror=par.maxrelerror,
+                     fw=par.fw, free_surf=par.free_surf, seismo=1, ts=ts, physics=5, damping=par.damping,
+                     fpml=par.fpml, npower=par.npower, k_max_pml=par.k_max_pml)
+    g = dnhip.Context(p)
+    g.set_model(None, vs0, rho0)
+    for i in (1, 2):
+        loc = dnio.split_sources(par, src, i - 1, 1)
+        spike = np.zeros(nt, np.float32)
+        spike[int(np.floor(loc[3, 0] / np.float32(par.dt) + 0.5))] = np.float32(1.0) * loc[5, 0]
+        spike = tf.apply_tdfilt(spike[None, :], par.dt, 5, 15.0, -5.0)
+        spike = tf.apply_tdfilt(spike, par.dt, order, fce, fcs)
+        g.set_sources(loc, spike)
+        g.set_receivers(rec[:2])
+        g.forward()
+        obs = tf.apply_tdfilt(dnio.read_su(os.path.join(tmp, f"obs/d_y.su.shot{i}"), rec.shape[1], nt), par.dt,
+                              order, fce, fcs)
+        want = so.stf(obs, g.seismogram(1), None, 2, 0.1, spike[0])
+        got = np.loadtxt(os.path.join(tmp, f"wavelet/w_shot_{i}.dat"), dtype=np.float64).astype(np.float32)
+        err = rel_l2(got, want)
+        print(f"[sh stf] shot {i}: wavelet file vs restatement rel-L2 {err:.2e}", flush=True)
+        assert got.shape == (nt,) and err < 1e-6
+        assert np.max(np.abs(got - want)) <= 1e-6 * np.abs(want).max()
+    g.close()
